@@ -1,0 +1,167 @@
+// C ABI (include/phd.h) over phd::Body. Exceptions stop here and become
+// return codes plus phd_last_error().
+#include "../../include/phd.h"
+
+#include <map>
+#include <stdexcept>
+#include <string>
+
+#include "phd_solver.h"
+
+namespace {
+struct Ctx {
+    std::unique_ptr<phd::Body> body;
+    std::string err;
+};
+thread_local std::string g_create_error;
+
+template <class F>
+int guard(phd_ctx c, F&& f) {
+    Ctx* x = static_cast<Ctx*>(c);
+    if (!x) return 2;
+    try {
+        f(*x->body);
+        x->err.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        x->err = e.what();
+        return 1;
+    } catch (...) {
+        x->err = "Unknown error.";
+        return 1;
+    }
+}
+std::map<std::string, double> opts(const char* const* keys, const double* vals, int n) {
+    std::map<std::string, double> o;
+    for (int i = 0; i < n; ++i) o[keys[i]] = vals[i];
+    return o;
+}
+}  // namespace
+
+extern "C" {
+
+phd_ctx phd_ctx_create(int device) {
+    try {
+        Ctx* c = new Ctx();
+        c->body.reset(new phd::Body(device));
+        return c;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return nullptr;
+    }
+}
+void phd_ctx_destroy(phd_ctx ctx) { delete static_cast<Ctx*>(ctx); }
+const char* phd_last_error(phd_ctx ctx) {
+    if (!ctx) return g_create_error.c_str();
+    return static_cast<Ctx*>(ctx)->err.c_str();
+}
+
+int phd_init_mesh(phd_ctx ctx, int edim, int nv, const double* X, int ne, const int* elems, double density) {
+    return guard(ctx, [&](phd::Body& b) { b.init_mesh(edim, nv, X, ne, elems, density); });
+}
+int phd_add_pd_energy(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni) {
+    return guard(ctx, [&](phd::Body& b) {
+        b.add_pd_energy(type, std::vector<double>(params, params + np), std::vector<int>(idx, idx + ni));
+    });
+}
+int phd_set_dirichlet(phd_ctx ctx, int dof, double val) {
+    return guard(ctx, [&](phd::Body& b) { b.set_dirichlet(dof, val); });
+}
+int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np) {
+    return guard(ctx, [&](phd::Body& b) {
+        if (std::string(type) != "gravity") throw std::runtime_error(std::string("Unsupported state force type: ") + type);
+        if (np != 3) throw std::runtime_error("Inconsistent params for GravitionalStateForce.");
+        b.add_gravity(params);
+    });
+}
+int phd_set_frictional_boundary(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni) {
+    return guard(ctx, [&](phd::Body& b) {
+        b.set_boundary(type, std::vector<double>(params, params + np), std::vector<int>(idx, idx + ni));
+    });
+}
+int phd_set_press_plane(phd_ctx ctx, double z) {
+    return guard(ctx, [&](phd::Body& b) { b.set_press_plane(z); });
+}
+int phd_factor_build(phd_ctx ctx, double dt, int threads) {
+    return guard(ctx, [&](phd::Body& b) { b.setup(dt, threads); });
+}
+int phd_apply_ainv(phd_ctx ctx, double dt, const double* bvec, double* x, int precision, int refine) {
+    return guard(ctx, [&](phd::Body& b) { b.apply_ainv(dt, bvec, x, precision, refine); });
+}
+int phd_step_forward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                     const char* const* keys, const double* vals, int n, double* q_next, double* v_next, int* active,
+                     int* n_active) {
+    return guard(ctx, [&](phd::Body& b) {
+        std::vector<int> act;
+        b.forward(q, v, f_ext, dt, opts(keys, vals, n), q_next, v_next, act);
+        if ((int)act.size() > *n_active) throw std::runtime_error("active_contact_idx buffer too small");
+        for (size_t i = 0; i < act.size(); ++i) active[i] = act[i];
+        *n_active = (int)act.size();
+    });
+}
+int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                      const double* q_next, const double* v_next, const int* active, int n_active,
+                      const double* dl_dq_next, const double* dl_dv_next, const char* const* keys, const double* vals,
+                      int n, double* dl_dq, double* dl_dv, double* dl_df_ext, double* dl_dmat_w, double* dl_dstate_p,
+                      double* dl_dw_elem) {
+    return guard(ctx, [&](phd::Body& b) {
+        b.backward(q, v, f_ext, dt, q_next, v_next, std::vector<int>(active, active + n_active), dl_dq_next, dl_dv_next,
+                   opts(keys, vals, n), dl_dq, dl_dv, dl_df_ext, dl_dmat_w, dl_dstate_p, dl_dw_elem);
+    });
+}
+int phd_dofs(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->dofs() : -1; }
+int phd_num_elements(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_elements() : -1; }
+int phd_num_pd_element_energies(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_energies() : -1; }
+long long phd_s_memory_bytes(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->factor_bytes() : 0; }
+long long phd_s_nnz(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->factor_nnz() : 0; }
+int phd_cached_n(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->cached_n() : -1; }
+int phd_frame_stats(phd_ctx ctx, int* iters, int* converged, double* seconds, int cap) {
+    if (!ctx) return -1;
+    const auto& st = static_cast<Ctx*>(ctx)->body->frame_stats;
+    const int n = (int)st.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (iters) iters[i] = st[i].iters;
+        if (converged) converged[i] = st[i].converged;
+        if (seconds) seconds[i] = st[i].seconds;
+    }
+    return n;
+}
+void phd_reset_frame_stats(phd_ctx ctx) {
+    if (ctx) static_cast<Ctx*>(ctx)->body->frame_stats.clear();
+}
+int phd_last_backward_iters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->last_backward_iters : -1; }
+int phd_factor_info(phd_ctx ctx, double* out, int cap) {
+    return guard(ctx, [&](phd::Body& b) {
+        const phd::Factor& f = b.factor();
+        const double v[10] = {(double)f.nnz, (double)f.v_len, (double)f.max_depth, f.order_seconds, f.factor_seconds,
+                              b.setup_seconds, 0, 0, (double)f.Lx.size(), 0};
+        for (int i = 0; i < cap && i < 10; ++i) out[i] = v[i];
+    });
+}
+int phd_timing_enable(phd_ctx ctx, int enable) {
+    return guard(ctx, [&](phd::Body& b) { b.timing = enable != 0; });
+}
+int phd_timing_read(phd_ctx ctx, double* out4, int reset) {
+    return guard(ctx, [&](phd::Body& b) {
+        b.collect_timing();
+        out4[0] = b.t_spmv_rows_ms;
+        out4[1] = (double)b.n_spmv_rows;
+        out4[2] = b.t_spmv_cols_ms;
+        out4[3] = (double)b.n_spmv_cols;
+        if (reset) { b.t_spmv_rows_ms = b.t_spmv_cols_ms = 0; b.n_spmv_rows = b.n_spmv_cols = 0; }
+    });
+}
+
+int phd_host_factor_check(int n, const int* Ap, const int* Ai, const double* Ax, const double* X, double* out7) {
+    try {
+        std::vector<int> ap(Ap, Ap + n + 1), ai(Ai, Ai + Ap[n]);
+        std::vector<double> ax(Ax, Ax + Ap[n]), x(X, X + 3 * (size_t)n);
+        phd::host_factor_check(n, ap, ai, ax, x, out7);
+        return 0;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return 1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Launchers for the HIP kernels of the PD time step (phd_kernels.hip).
+// All pointers are device pointers; every launcher enqueues on `stream`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace phd {
+
+// Per-element material description of the merged PD energies.
+struct DevMaterials {
+    int ne = 0, edim = 8, samples = 8;
+    const int* elems = nullptr;   // SoA: elems[k * ne + e]
+    const double* G = nullptr;    // grad_undeformed_sample_weight rows (edim x 3, row-major) per sample
+    int g_stride = 0;             // doubles per element (0: shared by all elements)
+    const double* wc = nullptr;   // corotated weight * vol / samples (nullptr: none)
+    const double* wv = nullptr;   // volume weight * vol / samples
+    int n_nh = 0;
+    const double* wn[4] = {nullptr, nullptr, nullptr, nullptr};
+    double nh_mu[4] = {0, 0, 0, 0}, nh_la[4] = {0, 0, 0, 0}, nh_k[4] = {0, 0, 0, 0};
+};
+
+struct DotArgs {
+    int m = 0;
+    const double* a[16];
+    const double* b[16];
+};
+
+// Boundary (frictional obstacle) description for the contact kernels.
+struct DevBoundary {
+    int type = 0;                 // 1 planar, 2 spherical, 3 heightfield, 4 press
+    double n[3] = {0, 0, 1}, off = 0;
+    double c[3] = {0, 0, 0}, r = 0;
+    double z0 = 0, plane_z = 0;
+    int nwaves = 0;
+    const double* waves = nullptr;  // device (amp, kx, ky, phase) quadruples
+};
+
+// ---- factor expansion and the S / S^T products --------------------------------
+void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t* Lp, const int* Li, const double* Lx,
+                     const int* parent, const int* depth, const int* fd, const int64_t* base, double* ws,
+                     int64_t ws_stride, double* V64, float* V32);
+
+template <class TV>
+void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* ru_c0, const int* ru_c1, const int64_t* base,
+                      const TV* V, const TV* w, int n, TV* part);
+template <class TV>
+void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const TV* part, TV* y);
+template <class TV>
+void launch_spmv_cols(hipStream_t s, int nunits, const int* cu_tile, const int* cu_i0, const int* cu_i1, const int* items,
+                      const int* fd, const int64_t* base, const TV* V, const TV* y, int n, TV* xpart);
+// out[3 perm[j] + d] = (add ? add[...] : 0) + sum of tile partials; vertices with dmask set keep dval.
+template <class TV>
+void launch_spmv_cols_fin(hipStream_t s, int n, const int* tile_unit_ptr, const TV* xpart, const int* perm,
+                          const double* add, const unsigned char* dmask, const double* dval, double* out);
+// w[d * n + j] = r[3 perm[j] + d] (SoA, postorder), zero at dmask vertices.
+template <class TV>
+void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, const unsigned char* dmask, TV* w);
+
+// ---- local step, gather, residual --------------------------------------------
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out);
+// pd_rhs = sum of element contributions - c_dir; b = inv_h2m * q_tilde + pd_rhs
+void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
+                       const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b);
+// r = b - A q (scalar CSR shared by the three axes), zero at dmask vertices
+void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const double* Ax, const double* b,
+                     const double* q, const unsigned char* dmask, double* r);
+void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out);
+
+// ---- reductions and vector updates -------------------------------------------
+void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out);
+int dots_partial_size();
+// out = x + alpha * (y - z) ... generic axpby: out = a*x + b*y (+ c*z)
+void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double* out);
+// out = q_prev + g - sum_i gamma_i (dQ_i + dG_i)
+void launch_aa_combine(hipStream_t s, int n, const double* q_prev, const double* g, int m, const double* const* dQ,
+                       const double* const* dG, const double* gamma, double* out);
+void launch_set_dirichlet(hipStream_t s, int nd, const int* ddof, const double* dval, double* q);
+void launch_zero_dirichlet(hipStream_t s, int nd, const int* ddof, double* q);
+// NaN/Inf -> 0 (returned count ignored), scaling done separately
+void launch_nan_to_zero(hipStream_t s, int n, double* x);
+void launch_scale(hipStream_t s, int n, double a, double* x);
+
+// ---- contact -----------------------------------------------------------------
+// normals / tangents at the frame-start positions (GetLocalFrame)
+void launch_contact_frames(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* nrm,
+                           double* t1, double* t2);
+void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta);
+// u[c] = sum_p val[p] * y[d * n + row[p]] for the three axes: out[3c + d]
+template <class TV>
+void launch_contact_gather(hipStream_t s, int K, const int* cp_ptr, const int* cp_row, const TV* cp_val, const TV* y,
+                           int n, double* out);
+// y[d * n + row] += sum over the row's contacts of val * f[3c + d]
+template <class TV>
+void launch_contact_scatter(hipStream_t s, int nrows, const int* rc_ptr, const int* rc_row, const int* rc_c,
+                            const TV* rc_val, const double* f, int n, TV* y);
+// add f[3c + d] at node c: r[3 node + d] += f
+void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r);
+// M[i][j] = om_i (n_i . n_j) H[i][j] om_j + (i == j) E_i  (row-major, TM = float or double)
+template <class TM>
+void launch_assemble_msys(hipStream_t s, int K, const double* H, const double* nrm, const double* om, const double* E, TM* M);
+// out[3c + d] = sum_j H[c][j] f[3j + d]
+void launch_h_apply(hipStream_t s, int K, const double* H, const double* f, double* out);
+
+void launch_gather_vals(hipStream_t s, int64_t m, const int* rows, const int* cols, const int64_t* base,
+                        const double* V64, double* o64, float* o32);
+void launch_scatter_dense(hipStream_t s, int64_t m, const int* ridx, const int* cidx, const double* val, int K, double* D);
+// mirrors the lower triangle (column-major, as written by rocblas_dsyrk) into the upper one
+void launch_symmetrize_lower(hipStream_t s, int K, double* H);
+
+void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
+void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
+
+// ---- backward ----------------------------------------------------------------
+// per-sample 9x9 proximal-map Jacobians weighted by the K_pd weights
+template <class TW>
+void launch_kpd_build(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* const* kn,
+                      const double* q, TW* W);
+template <class TW>
+void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const double* dq, double* elem_out);
+// g[type][e]: z_e^T (-(vol/s) sum_s A_s^T vec(P_s / stiffness)) per merged energy type
+void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
+                  double* g_vol, double* g_nh);
+
+}  // namespace phd

@@ -1,0 +1,1014 @@
+// HIP kernels (gfx950) of the PD forward/backward time step.
+//
+// Data layout in HBM:
+//  * positions / forces: FP64 AoS, 3 per vertex, original vertex order;
+//  * solver vectors of the factor: SoA per axis (w[d * n + j]) in postorder;
+//  * factor values V (FP32 or FP64): row k of S = L^-1 holds the contiguous
+//    column range [fd(k), k] at V[base[k] + j], base[k] a multiple of 64, so
+//    both products stream V with 256/512-byte coalesced wavefront accesses and
+//    no column indices at all.
+#include <hip/hip_runtime.h>
+
+#include "phd_kernels.h"
+#include "phd_math.h"
+
+namespace phd {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, WAVE);
+    return v;
+}
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// -----------------------------------------------------------------------------
+// S = L^-1 expansion: one wavefront per tile of 64 consecutive postorder
+// columns. The wave walks the tile's rows and then the ancestors of its last
+// column; lane l carries column 64t + l of L^-1 in a private workspace column.
+// -----------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_expand_s(int n, int t_begin, const int64_t* __restrict__ Lp,
+                                                 const int* __restrict__ Li, const double* __restrict__ Lx,
+                                                 const int* __restrict__ parent, const int* __restrict__ depth,
+                                                 const int* __restrict__ fd, const int64_t* __restrict__ base,
+                                                 double* __restrict__ ws, int64_t ws_stride, double* __restrict__ V64,
+                                                 float* __restrict__ V32) {
+    const int t = t_begin + blockIdx.x;
+    const int l = threadIdx.x;
+    const int g0 = 64 * t;
+    const int glast = min(n - 1, g0 + 63);
+    const int ng = glast - g0 + 1;
+    const int dl = depth[glast];
+    const int nrows = ng + dl;
+    double* X = ws + (int64_t)blockIdx.x * ws_stride;
+    for (int r = 0; r < nrows; ++r) X[(int64_t)r * 64 + l] = 0.0;
+    if (l < ng) X[(int64_t)l * 64 + l] = 1.0;
+    const int j = g0 + l;
+    auto process = [&](int i, int li) {
+        const int64_t p0 = Lp[i], p1 = Lp[i + 1];
+        const double x = X[(int64_t)li * 64 + l] / Lx[p0];
+        if (l < ng && j >= fd[i] && j <= i) {
+            const int64_t a = base[i] + j;
+            V64[a] = x;
+            V32[a] = (float)x;
+        }
+        for (int64_t p = p0 + 1; p < p1; ++p) {
+            const int k = Li[p];
+            const int lk = (k <= glast) ? (k - g0) : (ng + dl - depth[k] - 1);
+            double* xk = X + (int64_t)lk * 64 + l;
+            *xk -= Lx[p] * x;
+        }
+    };
+    for (int i = g0; i <= glast; ++i) process(i, i - g0);
+    for (int k = parent[glast]; k != -1; k = parent[k]) process(k, ng + dl - depth[k] - 1);
+}
+
+// -----------------------------------------------------------------------------
+// y = S w: one wavefront per (row, column chunk) unit, 3 right-hand sides.
+// -----------------------------------------------------------------------------
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_rows(int nunits, const int* __restrict__ ru_row,
+                                                   const int* __restrict__ ru_c0, const int* __restrict__ ru_c1,
+                                                   const int64_t* __restrict__ base, const TV* __restrict__ V,
+                                                   const TV* __restrict__ w, int n, TV* __restrict__ part) {
+    const int u = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (u >= nunits) return;
+    const int k = ru_row[u];
+    const int c0 = ru_c0[u], c1 = ru_c1[u];
+    const TV* vr = V + base[k];
+    TV a0 = 0, a1 = 0, a2 = 0;
+    int c = c0 + lane;
+    for (; c + 3 * WAVE <= c1; c += 4 * WAVE) {
+        const TV v0 = vr[c], v1 = vr[c + WAVE], v2 = vr[c + 2 * WAVE], v3 = vr[c + 3 * WAVE];
+        a0 += v0 * w[c] + v1 * w[c + WAVE] + v2 * w[c + 2 * WAVE] + v3 * w[c + 3 * WAVE];
+        a1 += v0 * w[n + c] + v1 * w[n + c + WAVE] + v2 * w[n + c + 2 * WAVE] + v3 * w[n + c + 3 * WAVE];
+        a2 += v0 * w[2 * n + c] + v1 * w[2 * n + c + WAVE] + v2 * w[2 * n + c + 2 * WAVE] + v3 * w[2 * n + c + 3 * WAVE];
+    }
+    for (; c <= c1; c += WAVE) {
+        const TV v = vr[c];
+        a0 += v * w[c];
+        a1 += v * w[n + c];
+        a2 += v * w[2 * n + c];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane == 0) {
+        part[3 * (int64_t)u] = a0;
+        part[3 * (int64_t)u + 1] = a1;
+        part[3 * (int64_t)u + 2] = a2;
+    }
+}
+
+template <class TV>
+__global__ void k_spmv_rows_fin(int n, const int* __restrict__ row_unit_ptr, const TV* __restrict__ part, TV* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    TV a0 = 0, a1 = 0, a2 = 0;
+    for (int u = row_unit_ptr[k]; u < row_unit_ptr[k + 1]; ++u) {
+        a0 += part[3 * (int64_t)u];
+        a1 += part[3 * (int64_t)u + 1];
+        a2 += part[3 * (int64_t)u + 2];
+    }
+    y[k] = a0;
+    y[n + k] = a1;
+    y[2 * n + k] = a2;
+}
+
+// -----------------------------------------------------------------------------
+// x = S^T y: one wavefront per (tile, item range) unit; lane l owns column
+// 64t + l and accumulates V[base[k] + j] * y[k] over the tile's rows.
+// -----------------------------------------------------------------------------
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_cols(int nunits, const int* __restrict__ cu_tile,
+                                                   const int* __restrict__ cu_i0, const int* __restrict__ cu_i1,
+                                                   const int* __restrict__ items, const int* __restrict__ fd,
+                                                   const int64_t* __restrict__ base, const TV* __restrict__ V,
+                                                   const TV* __restrict__ y, int n, TV* __restrict__ xpart) {
+    const int u = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (u >= nunits) return;
+    const int t = cu_tile[u];
+    const int j = 64 * t + lane;
+    const int i0 = cu_i0[u], i1 = cu_i1[u];
+    TV a0 = 0, a1 = 0, a2 = 0;
+    int it = i0;
+    for (; it + 3 < i1; it += 4) {
+        int kk[4];
+        TV v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) kk[q] = items[it + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kk[q];
+            v[q] = (j >= fd[k] && j <= k) ? V[base[k] + j] : TV(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kk[q];
+            a0 += v[q] * y[k];
+            a1 += v[q] * y[n + k];
+            a2 += v[q] * y[2 * n + k];
+        }
+    }
+    for (; it < i1; ++it) {
+        const int k = items[it];
+        const TV v = (j >= fd[k] && j <= k) ? V[base[k] + j] : TV(0);
+        a0 += v * y[k];
+        a1 += v * y[n + k];
+        a2 += v * y[2 * n + k];
+    }
+    TV* xp = xpart + (int64_t)u * 3 * WAVE;
+    xp[lane] = a0;
+    xp[WAVE + lane] = a1;
+    xp[2 * WAVE + lane] = a2;
+}
+
+template <class TV>
+__global__ void k_spmv_cols_fin(int n, const int* __restrict__ tile_unit_ptr, const TV* __restrict__ xpart,
+                                const int* __restrict__ perm, const double* __restrict__ add,
+                                const unsigned char* __restrict__ dmask, const double* __restrict__ dval,
+                                double* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int t = j / WAVE, lane = j % WAVE;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int u = tile_unit_ptr[t]; u < tile_unit_ptr[t + 1]; ++u) {
+        const TV* xp = xpart + (int64_t)u * 3 * WAVE;
+        a0 += (double)xp[lane];
+        a1 += (double)xp[WAVE + lane];
+        a2 += (double)xp[2 * WAVE + lane];
+    }
+    const int v = perm[j];
+    if (dmask && dmask[v]) {
+        out[3 * v] = dval[3 * v];
+        out[3 * v + 1] = dval[3 * v + 1];
+        out[3 * v + 2] = dval[3 * v + 2];
+        return;
+    }
+    if (add) { a0 += add[3 * v]; a1 += add[3 * v + 1]; a2 += add[3 * v + 2]; }
+    out[3 * v] = a0;
+    out[3 * v + 1] = a1;
+    out[3 * v + 2] = a2;
+}
+
+template <class TV>
+__global__ void k_perm_in(int n, const int* __restrict__ perm, const double* __restrict__ r,
+                          const unsigned char* __restrict__ dmask, TV* __restrict__ w) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int v = perm[j];
+    const bool z = dmask && dmask[v];
+    w[j] = z ? TV(0) : (TV)r[3 * v];
+    w[n + j] = z ? TV(0) : (TV)r[3 * v + 1];
+    w[2 * n + j] = z ? TV(0) : (TV)r[3 * v + 2];
+}
+
+// -----------------------------------------------------------------------------
+// Local step (ProjectiveDynamicsLocalStep, deformable_projective_dynamics_forward.cpp:186-292):
+// one thread per element; per sample F = x_e G_s, one SVD shared by every
+// projection, contribution sum_energies w A_s^T vec(p(F)).
+// -----------------------------------------------------------------------------
+template <int EDIM>
+__global__ void __launch_bounds__(128) k_local_step(DevMaterials m, const double* __restrict__ q,
+                                                    double* __restrict__ elem_out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    double x[EDIM][3];
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) {
+        const int v = m.elems[k * m.ne + e];
+        x[k][0] = q[3 * v]; x[k][1] = q[3 * v + 1]; x[k][2] = q[3 * v + 2];
+    }
+    double acc[EDIM][3];
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+    const double wc = m.wc ? m.wc[e] : 0.0;
+    const double wv = m.wv ? m.wv[e] : 0.0;
+    const double* Ge = m.G + (int64_t)e * m.g_stride;
+    for (int s = 0; s < m.samples; ++s) {
+        const double* G = Ge + s * EDIM * 3;
+        double F[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const double g = G[k * 3 + jj];
+                F[0 + 3 * jj] += x[k][0] * g;
+                F[1 + 3 * jj] += x[k][1] * g;
+                F[2 + 3 * jj] += x[k][2] * g;
+            }
+        double U[9], sg[3], V[9];
+        svd3(F, U, sg, V);
+        double P[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = 0.0;
+        if (wc != 0.0) {
+            const double one[3] = {1.0, 1.0, 1.0};
+            double R[9];
+            udv(U, one, V, R);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) P[i] += wc * R[i];
+        }
+        if (wv != 0.0) {
+            double D[3], sd[3], Pv[9];
+            volume_D(sg, D);
+            sd[0] = sg[0] + D[0]; sd[1] = sg[1] + D[1]; sd[2] = sg[2] + D[2];
+            udv(U, sd, V, Pv);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) P[i] += wv * Pv[i];
+        }
+        for (int h = 0; h < m.n_nh; ++h) {
+            const double wn = m.wn[h][e];
+            if (wn == 0.0) continue;
+            double ss[3], Pn[9];
+            nh_prox(m.nh_mu[h], m.nh_la[h], m.nh_k[h], sg, ss);
+            udv(U, ss, V, Pn);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) P[i] += wn * Pn[i];
+        }
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                acc[k][i] += P[i] * G[k * 3] + P[i + 3] * G[k * 3 + 1] + P[i + 6] * G[k * 3 + 2];
+    }
+    double* o = elem_out + (int64_t)e * EDIM * 3;
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) { o[3 * k] = acc[k][0]; o[3 * k + 1] = acc[k][1]; o[3 * k + 2] = acc[k][2]; }
+}
+
+__global__ void k_gather_rhs(int nv, const int* __restrict__ inc_ptr, const int* __restrict__ inc_idx,
+                             const double* __restrict__ elem_out, const double* __restrict__ c_dir,
+                             const double* __restrict__ q_tilde, double inv_h2m, double* __restrict__ pd_rhs,
+                             double* __restrict__ b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int p = inc_ptr[v]; p < inc_ptr[v + 1]; ++p) {
+        const int64_t i = inc_idx[p];
+        s0 += elem_out[3 * i];
+        s1 += elem_out[3 * i + 1];
+        s2 += elem_out[3 * i + 2];
+    }
+    s0 -= c_dir[3 * v]; s1 -= c_dir[3 * v + 1]; s2 -= c_dir[3 * v + 2];
+    pd_rhs[3 * v] = s0; pd_rhs[3 * v + 1] = s1; pd_rhs[3 * v + 2] = s2;
+    b[3 * v] = inv_h2m * q_tilde[3 * v] + s0;
+    b[3 * v + 1] = inv_h2m * q_tilde[3 * v + 1] + s1;
+    b[3 * v + 2] = inv_h2m * q_tilde[3 * v + 2] + s2;
+}
+
+__global__ void k_gather_vertex(int nv, const int* __restrict__ inc_ptr, const int* __restrict__ inc_idx,
+                                const double* __restrict__ elem_out, double* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int p = inc_ptr[v]; p < inc_ptr[v + 1]; ++p) {
+        const int64_t i = inc_idx[p];
+        s0 += elem_out[3 * i];
+        s1 += elem_out[3 * i + 1];
+        s2 += elem_out[3 * i + 2];
+    }
+    out[3 * v] = s0; out[3 * v + 1] = s1; out[3 * v + 2] = s2;
+}
+
+__global__ void k_residual(int nv, const int* __restrict__ Ap, const int* __restrict__ Ai, const double* __restrict__ Ax,
+                           const double* __restrict__ b, const double* __restrict__ q,
+                           const unsigned char* __restrict__ dmask, double* __restrict__ r) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    if (dmask && dmask[v]) { r[3 * v] = r[3 * v + 1] = r[3 * v + 2] = 0.0; return; }
+    double s0 = b[3 * v], s1 = b[3 * v + 1], s2 = b[3 * v + 2];
+    for (int p = Ap[v]; p < Ap[v + 1]; ++p) {
+        const int c = Ai[p];
+        const double a = Ax[p];
+        s0 -= a * q[3 * c];
+        s1 -= a * q[3 * c + 1];
+        s2 -= a * q[3 * c + 2];
+    }
+    r[3 * v] = s0; r[3 * v + 1] = s1; r[3 * v + 2] = s2;
+}
+
+// -----------------------------------------------------------------------------
+// Reductions: block partial sums of up to 16 dot products, then a final pass.
+// -----------------------------------------------------------------------------
+constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_THREADS = 256;
+
+__global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial) {
+    __shared__ double sh[DOT_THREADS / WAVE][16];
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < a.m) acc[i] += a.a[i][idx] * a.b[i][idx];
+    }
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        if (i < a.m) {
+            const double s = wave_sum(acc[i]);
+            if (lane == 0) sh[wid][i] = s;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < a.m) {
+        double s = 0;
+        for (int w = 0; w < DOT_THREADS / WAVE; ++w) s += sh[w][threadIdx.x];
+        partial[blockIdx.x * 16 + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_dots_fin(int m, const double* __restrict__ partial, double* __restrict__ out) {
+    const int i = threadIdx.x;
+    if (i >= m) return;
+    double s = 0;
+    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b * 16 + i];
+    out[i] = s;
+}
+
+__global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double b, const double* __restrict__ y, double c,
+                           const double* __restrict__ z, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r = a * x[i];
+    if (y) r += b * y[i];
+    if (z) r += c * z[i];
+    out[i] = r;
+}
+
+struct AAPtrs {
+    const double* dQ[8];
+    const double* dG[8];
+    double gamma[8];
+};
+
+__global__ void k_aa_combine(int n, const double* __restrict__ q_prev, const double* __restrict__ g, int m, AAPtrs p,
+                             double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c = q_prev[i] + g[i];
+    for (int a = 0; a < m; ++a) c -= (p.dQ[a][i] + p.dG[a][i]) * p.gamma[a];
+    out[i] = c;
+}
+
+__global__ void k_set_dirichlet(int nd, const int* __restrict__ ddof, const double* __restrict__ dval, double* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) q[ddof[i]] = dval[i];
+}
+__global__ void k_zero_dirichlet(int nd, const int* __restrict__ ddof, double* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) q[ddof[i]] = 0.0;
+}
+__global__ void k_nan_to_zero(int n, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !isfinite(x[i])) x[i] = 0.0;
+}
+__global__ void k_scale(int n, double a, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+
+// -----------------------------------------------------------------------------
+// Contact: obstacle geometry (planar/spherical as in friction/*.cpp, plus the
+// heightfield ground and the two-obstacle press).
+// -----------------------------------------------------------------------------
+__device__ void hf_eval(const DevBoundary& b, double x, double y, double& h, double& hx, double& hy) {
+    h = b.z0; hx = 0; hy = 0;
+    for (int i = 0; i < b.nwaves; ++i) {
+        const double a = b.waves[4 * i], kx = b.waves[4 * i + 1], ky = b.waves[4 * i + 2], ph = b.waves[4 * i + 3];
+        const double arg = kx * x + ky * y + ph;
+        double sn, cs;
+        sincos(arg, &sn, &cs);
+        h += a * sn;
+        hx += a * kx * cs;
+        hy += a * ky * cs;
+    }
+}
+
+__device__ void boundary_eval(const DevBoundary& b, const double* q, double& dist, double* nn) {
+    if (b.type == 1) {
+        dist = q[0] * b.n[0] + q[1] * b.n[1] + q[2] * b.n[2] + b.off;
+        nn[0] = b.n[0]; nn[1] = b.n[1]; nn[2] = b.n[2];
+        return;
+    }
+    if (b.type == 2) {
+        const double d0 = q[0] - b.c[0], d1 = q[1] - b.c[1], d2 = q[2] - b.c[2];
+        const double rho = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        dist = rho - b.r;
+        nn[0] = d0 / rho; nn[1] = d1 / rho; nn[2] = d2 / rho;
+        return;
+    }
+    double h, hx, hy;
+    hf_eval(b, q[0], q[1], h, hx, hy);
+    const double s = sqrt(1 + hx * hx + hy * hy);
+    const double dg = (q[2] - h) / s;
+    if (b.type == 4) {
+        const double dp = b.plane_z - q[2];
+        if (dp < dg) { dist = dp; nn[0] = 0; nn[1] = 0; nn[2] = -1; return; }
+    }
+    dist = dg;
+    nn[0] = -hx / s; nn[1] = -hy / s; nn[2] = 1 / s;
+}
+
+__global__ void k_contact_frames(int K, DevBoundary b, const int* __restrict__ nodes, const double* __restrict__ q,
+                                 double* __restrict__ nrm, double* __restrict__ t1, double* __restrict__ t2) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const int v = nodes[c];
+    double d, nn[3];
+    boundary_eval(b, q + 3 * v, d, nn);
+    // GetLocalFrame: t1 = argmax_i |n x e_i| normalized, t2 = n x t1 normalized
+    double ux[3] = {0, 0, 0}, best = 0;
+    for (int i = 0; i < 3; ++i) {
+        double e[3] = {0, 0, 0};
+        e[i] = 1;
+        const double x0 = nn[1] * e[2] - nn[2] * e[1], x1 = nn[2] * e[0] - nn[0] * e[2], x2 = nn[0] * e[1] - nn[1] * e[0];
+        const double sq = x0 * x0 + x1 * x1 + x2 * x2;
+        if (sq > best) { best = sq; ux[0] = x0; ux[1] = x1; ux[2] = x2; }
+    }
+    const double un = sqrt(best);
+    const double a0 = ux[0] / un, a1 = ux[1] / un, a2 = ux[2] / un;
+    double y0 = nn[1] * a2 - nn[2] * a1, y1 = nn[2] * a0 - nn[0] * a2, y2 = nn[0] * a1 - nn[1] * a0;
+    const double yn = sqrt(y0 * y0 + y1 * y1 + y2 * y2);
+    nrm[3 * c] = nn[0]; nrm[3 * c + 1] = nn[1]; nrm[3 * c + 2] = nn[2];
+    t1[3 * c] = a0; t1[3 * c + 1] = a1; t1[3 * c + 2] = a2;
+    t2[3 * c] = y0 / yn; t2[3 * c + 1] = y1 / yn; t2[3 * c + 2] = y2 / yn;
+}
+
+__global__ void k_contact_distance(int K, DevBoundary b, const int* __restrict__ nodes, const double* __restrict__ q,
+                                   double* __restrict__ delta) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    double d, nn[3];
+    boundary_eval(b, q + 3 * nodes[c], d, nn);
+    delta[c] = d;
+}
+
+template <class TV>
+__global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const int* __restrict__ cp_row,
+                                 const TV* __restrict__ cp_val, const TV* __restrict__ y, int n, double* __restrict__ out) {
+    const int c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (c >= K) return;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int p = cp_ptr[c] + lane; p < cp_ptr[c + 1]; p += WAVE) {
+        const int k = cp_row[p];
+        const double v = (double)cp_val[p];
+        a0 += v * (double)y[k];
+        a1 += v * (double)y[n + k];
+        a2 += v * (double)y[2 * n + k];
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+    if (lane == 0) { out[3 * c] = a0; out[3 * c + 1] = a1; out[3 * c + 2] = a2; }
+}
+
+template <class TV>
+__global__ void k_contact_scatter(int nrows, const int* __restrict__ rc_ptr, const int* __restrict__ rc_row,
+                                  const int* __restrict__ rc_c, const TV* __restrict__ rc_val, const double* __restrict__ f,
+                                  int n, TV* __restrict__ y) {
+    const int r = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (r >= nrows) return;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int p = rc_ptr[r] + lane; p < rc_ptr[r + 1]; p += WAVE) {
+        const int c = rc_c[p];
+        const double v = (double)rc_val[p];
+        a0 += v * f[3 * c];
+        a1 += v * f[3 * c + 1];
+        a2 += v * f[3 * c + 2];
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+    if (lane == 0) {
+        const int k = rc_row[r];
+        y[k] += (TV)a0;
+        y[n + k] += (TV)a1;
+        y[2 * n + k] += (TV)a2;
+    }
+}
+
+__global__ void k_add_node_force(int K, const int* __restrict__ nodes, const double* __restrict__ f, double* __restrict__ r) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const int v = nodes[c];
+    r[3 * v] += f[3 * c];
+    r[3 * v + 1] += f[3 * c + 1];
+    r[3 * v + 2] += f[3 * c + 2];
+}
+
+template <class TM>
+__global__ void k_assemble_msys(int K, const double* __restrict__ H, const double* __restrict__ nrm,
+                                const double* __restrict__ om, const double* __restrict__ E, TM* __restrict__ M) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)K * K) return;
+    const int i = (int)(idx / K), j = (int)(idx % K);
+    const double nd = nrm[3 * i] * nrm[3 * j] + nrm[3 * i + 1] * nrm[3 * j + 1] + nrm[3 * i + 2] * nrm[3 * j + 2];
+    double v = om[i] * nd * H[idx] * om[j];
+    if (i == j) v += E[i];
+    M[idx] = (TM)v;
+}
+
+__global__ void k_h_apply(int K, const double* __restrict__ H, const double* __restrict__ f, double* __restrict__ out) {
+    const int c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (c >= K) return;
+    const double* Hr = H + (int64_t)c * K;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int j = lane; j < K; j += WAVE) {
+        const double h = Hr[j];
+        a0 += h * f[3 * j];
+        a1 += h * f[3 * j + 1];
+        a2 += h * f[3 * j + 2];
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+    if (lane == 0) { out[3 * c] = a0; out[3 * c + 1] = a1; out[3 * c + 2] = a2; }
+}
+
+// -----------------------------------------------------------------------------
+// Backward: per-sample 9x9 proximal-map Jacobians (SetupProjectiveDynamicsLocalStepDifferential,
+// deformable_projective_dynamics_backward.cpp:10-80) and their matrix-free application
+// (ApplyProjectiveDynamicsLocalStepDifferential, :82-139).
+// -----------------------------------------------------------------------------
+struct KnPtrs { const double* kn[4]; };
+
+template <int EDIM, class TW>
+__global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* __restrict__ kc, const double* __restrict__ kv,
+                                                  KnPtrs kn, const double* __restrict__ q, TW* __restrict__ W) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)m.ne * m.samples) return;
+    const int e = (int)(gid / m.samples), s = (int)(gid % m.samples);
+    const double* G = m.G + (int64_t)e * m.g_stride + s * EDIM * 3;
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = 0.0;
+    for (int k = 0; k < EDIM; ++k) {
+        const int v = m.elems[k * m.ne + e];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const double g = G[k * 3 + jj];
+            F[0 + 3 * jj] += q[3 * v] * g;
+            F[1 + 3 * jj] += q[3 * v + 1] * g;
+            F[2 + 3 * jj] += q[3 * v + 2] * g;
+        }
+    }
+    double U[9], sg[3], V[9];
+    svd3(F, U, sg, V);
+    const double wc = kc ? kc[e] : 0.0, wv = kv ? kv[e] : 0.0;
+    double R[9], Sp[9], D[3], ssn[4][3];
+    const double one[3] = {1.0, 1.0, 1.0};
+    udv(U, one, V, R);
+    udv(V, sg, V, Sp);
+    if (wv != 0.0) volume_D(sg, D);
+    for (int h = 0; h < m.n_nh; ++h)
+        if (kn.kn[h] && kn.kn[h][e] != 0.0) nh_prox(m.nh_mu[h], m.nh_la[h], m.nh_k[h], sg, ssn[h]);
+    TW* out = W + gid * 81;
+    for (int c = 0; c < 9; ++c) {
+        double dF[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dF[i] = (i == c) ? 1.0 : 0.0;
+        double col[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) col[i] = 0.0;
+        if (wc != 0.0) {
+            double dR[9];
+            dR_from_dF(R, Sp, dF, dR);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) col[i] += wc * dR[i];
+        }
+        if (wv != 0.0) {
+            double dP[9];
+            volume_diff(U, sg, V, D, dF, dP);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) col[i] += wv * dP[i];
+        }
+        for (int h = 0; h < m.n_nh; ++h) {
+            if (!kn.kn[h] || kn.kn[h][e] == 0.0) continue;
+            double dP[9];
+            nh_diff(m.nh_mu[h], m.nh_la[h], m.nh_k[h], U, sg, V, ssn[h], dF, dP);
+            const double w = kn.kn[h][e];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) col[i] += w * dP[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i + 9 * c] = (TW)col[i];
+    }
+}
+
+template <int EDIM, class TW>
+__global__ void __launch_bounds__(128) k_kpd_apply(DevMaterials m, const TW* __restrict__ W, const double* __restrict__ dq,
+                                                   double* __restrict__ elem_out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    double x[EDIM][3], acc[EDIM][3];
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) {
+        const int v = m.elems[k * m.ne + e];
+        x[k][0] = dq[3 * v]; x[k][1] = dq[3 * v + 1]; x[k][2] = dq[3 * v + 2];
+        acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+    }
+    const double* Ge = m.G + (int64_t)e * m.g_stride;
+    for (int s = 0; s < m.samples; ++s) {
+        const double* G = Ge + s * EDIM * 3;
+        double dF[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dF[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const double g = G[k * 3 + jj];
+                dF[0 + 3 * jj] += x[k][0] * g;
+                dF[1 + 3 * jj] += x[k][1] * g;
+                dF[2 + 3 * jj] += x[k][2] * g;
+            }
+        const TW* Ws = W + ((int64_t)e * m.samples + s) * 81;
+        double y[9];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            double a = 0;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) a += (double)Ws[r + 9 * c] * dF[c];
+            y[r] = a;
+        }
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[k][i] += y[i] * G[k * 3] + y[i + 3] * G[k * 3 + 1] + y[i + 6] * G[k * 3 + 2];
+    }
+    double* o = elem_out + (int64_t)e * EDIM * 3;
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) { o[3 * k] = acc[k][0]; o[3 * k + 1] = acc[k][1]; o[3 * k + 2] = acc[k][2]; }
+}
+
+// dl/d(mat_w) per merged type (PdEnergyForceDifferential require_dw,
+// deformable_pd_energy.cpp:368-403 with StressTensor/stiffness = F - p for the
+// PD energies and the true Neo-Hookean P / (2 mu_mean) for pd_neohookean).
+template <int EDIM>
+__global__ void __launch_bounds__(128) k_dmatw(DevMaterials m, const double* __restrict__ q, const double* __restrict__ z,
+                                               double vs, double* __restrict__ g_cor, double* __restrict__ g_vol,
+                                               double* __restrict__ g_nh) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    double x[EDIM][3], zz[EDIM][3];
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) {
+        const int v = m.elems[k * m.ne + e];
+        x[k][0] = q[3 * v]; x[k][1] = q[3 * v + 1]; x[k][2] = q[3 * v + 2];
+        zz[k][0] = z[3 * v]; zz[k][1] = z[3 * v + 1]; zz[k][2] = z[3 * v + 2];
+    }
+    double sc = 0, sv = 0, sn[4] = {0, 0, 0, 0};
+    const double* Ge = m.G + (int64_t)e * m.g_stride;
+    for (int s = 0; s < m.samples; ++s) {
+        const double* G = Ge + s * EDIM * 3;
+        double F[9], ZG[9];  // ZG = sum_k z_k G_k^T: z_e^T A^T vec(P) = <ZG, P>
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { F[i] = 0.0; ZG[i] = 0.0; }
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const double g = G[k * 3 + jj];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { F[i + 3 * jj] += x[k][i] * g; ZG[i + 3 * jj] += zz[k][i] * g; }
+            }
+        double U[9], sg[3], V[9];
+        svd3(F, U, sg, V);
+        if (g_cor) {
+            const double one[3] = {1, 1, 1};
+            double R[9];
+            udv(U, one, V, R);
+            double a = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) a += ZG[i] * (F[i] - R[i]);
+            sc += a;
+        }
+        if (g_vol) {
+            double D[3], sd[3], P[9];
+            volume_D(sg, D);
+            sd[0] = sg[0] + D[0]; sd[1] = sg[1] + D[1]; sd[2] = sg[2] + D[2];
+            udv(U, sd, V, P);
+            double a = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) a += ZG[i] * (F[i] - P[i]);
+            sv += a;
+        }
+        if (g_nh && m.n_nh > 0) {
+            const double J = F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
+                             F[6] * (F[1] * F[5] - F[4] * F[2]);
+            const double lj = log(fmax(J, 1e-12));
+            // F^-T = cof(F) / det
+            double FiT[9];
+            FiT[0] = (F[4] * F[8] - F[5] * F[7]) / J;
+            FiT[1] = (F[5] * F[6] - F[3] * F[8]) / J;
+            FiT[2] = (F[3] * F[7] - F[4] * F[6]) / J;
+            FiT[3] = (F[2] * F[7] - F[1] * F[8]) / J;
+            FiT[4] = (F[0] * F[8] - F[2] * F[6]) / J;
+            FiT[5] = (F[1] * F[6] - F[0] * F[7]) / J;
+            FiT[6] = (F[1] * F[5] - F[2] * F[4]) / J;
+            FiT[7] = (F[2] * F[3] - F[0] * F[5]) / J;
+            FiT[8] = (F[0] * F[4] - F[1] * F[3]) / J;
+            for (int h = 0; h < m.n_nh; ++h) {
+                double a = 0;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) a += ZG[i] * (m.nh_mu[h] * (F[i] - FiT[i]) + m.nh_la[h] * lj * FiT[i]);
+                sn[h] += a / m.nh_k[h];
+            }
+        }
+    }
+    if (g_cor) g_cor[e] = -vs * sc;
+    if (g_vol) g_vol[e] = -vs * sv;
+    if (g_nh)
+        for (int h = 0; h < m.n_nh; ++h) g_nh[(int64_t)h * m.ne + e] = -vs * sn[h];
+}
+
+// Gathers factor values V[base[row] + col] (setup of the contact paths).
+__global__ void k_gather_vals(int64_t m, const int* __restrict__ rows, const int* __restrict__ cols,
+                              const int64_t* __restrict__ base, const double* __restrict__ V64, double* __restrict__ o64,
+                              float* __restrict__ o32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double v = V64[base[rows[i]] + cols[i]];
+    o64[i] = v;
+    o32[i] = (float)v;
+}
+// Dense S_C[ridx][c] = value (row-major, K columns) for the contact Gram H = S_C^T S_C.
+__global__ void k_scatter_dense(int64_t m, const int* __restrict__ ridx, const int* __restrict__ cidx,
+                                const double* __restrict__ val, int K, double* __restrict__ D) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    D[(int64_t)ridx[i] * K + cidx[i]] = val[i];
+}
+__global__ void k_symmetrize_lower(int K, double* __restrict__ H) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)K * K) return;
+    const int i = (int)(idx / K), j = (int)(idx % K);
+    // column-major lower triangle (i >= j in column-major = H[j*K + i]) mirrored to the upper part
+    if (i < j) H[(int64_t)j * K + i] = H[(int64_t)i * K + j];
+}
+
+__global__ void k_gather_nodes(int K, const int* __restrict__ nodes, const double* __restrict__ q, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const int v = nodes[c];
+    out[3 * c] = q[3 * v];
+    out[3 * c + 1] = q[3 * v + 1];
+    out[3 * c + 2] = q[3 * v + 2];
+}
+// G[a][b] = (d_a . d_b) H[c_a][c_b]: Gram of node-local directions under A^-1.
+__global__ void k_assemble_g(int nb, const int* __restrict__ cidx, const double* __restrict__ dirs,
+                             const double* __restrict__ H, int K, double* __restrict__ G) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nb * nb) return;
+    const int a = (int)(idx / nb), b = (int)(idx % nb);
+    const double dd = dirs[3 * a] * dirs[3 * b] + dirs[3 * a + 1] * dirs[3 * b + 1] + dirs[3 * a + 2] * dirs[3 * b + 2];
+    G[idx] = dd * H[(int64_t)cidx[a] * K + cidx[b]];
+}
+
+}  // namespace
+
+// ============================================================================
+// launchers
+// ============================================================================
+void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t* Lp, const int* Li, const double* Lx,
+                     const int* parent, const int* depth, const int* fd, const int64_t* base, double* ws,
+                     int64_t ws_stride, double* V64, float* V32) {
+    if (t_end <= t_begin) return;
+    hipLaunchKernelGGL(k_expand_s, dim3(t_end - t_begin), dim3(64), 0, s, n, t_begin, Lp, Li, Lx, parent, depth, fd, base,
+                       ws, ws_stride, V64, V32);
+}
+
+template <class TV>
+void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* ru_c0, const int* ru_c1, const int64_t* base,
+                      const TV* V, const TV* w, int n, TV* part) {
+    if (nunits == 0) return;
+    hipLaunchKernelGGL(k_spmv_rows<TV>, dim3(blocks_for(nunits, 4)), dim3(256), 0, s, nunits, ru_row, ru_c0, ru_c1, base,
+                       V, w, n, part);
+}
+template <class TV>
+void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const TV* part, TV* y) {
+    hipLaunchKernelGGL(k_spmv_rows_fin<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, row_unit_ptr, part, y);
+}
+template <class TV>
+void launch_spmv_cols(hipStream_t s, int nunits, const int* cu_tile, const int* cu_i0, const int* cu_i1, const int* items,
+                      const int* fd, const int64_t* base, const TV* V, const TV* y, int n, TV* xpart) {
+    if (nunits == 0) return;
+    hipLaunchKernelGGL(k_spmv_cols<TV>, dim3(blocks_for(nunits, 4)), dim3(256), 0, s, nunits, cu_tile, cu_i0, cu_i1,
+                       items, fd, base, V, y, n, xpart);
+}
+template <class TV>
+void launch_spmv_cols_fin(hipStream_t s, int n, const int* tile_unit_ptr, const TV* xpart, const int* perm,
+                          const double* add, const unsigned char* dmask, const double* dval, double* out) {
+    hipLaunchKernelGGL(k_spmv_cols_fin<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, tile_unit_ptr, xpart, perm, add,
+                       dmask, dval, out);
+}
+template <class TV>
+void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, const unsigned char* dmask, TV* w) {
+    hipLaunchKernelGGL(k_perm_in<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, perm, r, dmask, w);
+}
+
+#define PHD_INST_TV(TV)                                                                                                  \
+    template void launch_spmv_rows<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*, const TV*, \
+                                       const TV*, int, TV*);                                                             \
+    template void launch_spmv_rows_fin<TV>(hipStream_t, int, const int*, const TV*, TV*);                               \
+    template void launch_spmv_cols<TV>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,    \
+                                       const int64_t*, const TV*, const TV*, int, TV*);                                  \
+    template void launch_spmv_cols_fin<TV>(hipStream_t, int, const int*, const TV*, const int*, const double*,          \
+                                           const unsigned char*, const double*, double*);                                \
+    template void launch_perm_in<TV>(hipStream_t, int, const int*, const double*, const unsigned char*, TV*);           \
+    template void launch_contact_gather<TV>(hipStream_t, int, const int*, const int*, const TV*, const TV*, int, double*); \
+    template void launch_contact_scatter<TV>(hipStream_t, int, const int*, const int*, const int*, const TV*,           \
+                                             const double*, int, TV*);
+
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out) {
+    if (m.edim == 8)
+        hipLaunchKernelGGL(k_local_step<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, elem_out);
+    else
+        hipLaunchKernelGGL(k_local_step<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, elem_out);
+}
+void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
+                       const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b) {
+    hipLaunchKernelGGL(k_gather_rhs, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, inc_ptr, inc_idx, elem_out, c_dir,
+                       q_tilde, inv_h2m, pd_rhs, b);
+}
+void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out) {
+    hipLaunchKernelGGL(k_gather_vertex, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, inc_ptr, inc_idx, elem_out, out);
+}
+void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const double* Ax, const double* b,
+                     const double* q, const unsigned char* dmask, double* r) {
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, Ap, Ai, Ax, b, q, dmask, r);
+}
+int dots_partial_size() { return DOT_BLOCKS * 16; }
+void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {
+    hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial);
+    hipLaunchKernelGGL(k_dots_fin, dim3(1), dim3(64), 0, s, args.m, partial, out);
+}
+void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
+                     const double* z, double* out) {
+    hipLaunchKernelGGL(k_axpbypcz, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x, b, y, c, z, out);
+}
+void launch_aa_combine(hipStream_t s, int n, const double* q_prev, const double* g, int m, const double* const* dQ,
+                       const double* const* dG, const double* gamma, double* out) {
+    AAPtrs p;
+    for (int a = 0; a < m && a < 8; ++a) { p.dQ[a] = dQ[a]; p.dG[a] = dG[a]; p.gamma[a] = gamma[a]; }
+    hipLaunchKernelGGL(k_aa_combine, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, q_prev, g, m, p, out);
+}
+void launch_set_dirichlet(hipStream_t s, int nd, const int* ddof, const double* dval, double* q) {
+    if (nd == 0) return;
+    hipLaunchKernelGGL(k_set_dirichlet, dim3(blocks_for(nd, 256)), dim3(256), 0, s, nd, ddof, dval, q);
+}
+void launch_zero_dirichlet(hipStream_t s, int nd, const int* ddof, double* q) {
+    if (nd == 0) return;
+    hipLaunchKernelGGL(k_zero_dirichlet, dim3(blocks_for(nd, 256)), dim3(256), 0, s, nd, ddof, q);
+}
+void launch_nan_to_zero(hipStream_t s, int n, double* x) {
+    hipLaunchKernelGGL(k_nan_to_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x);
+}
+void launch_scale(hipStream_t s, int n, double a, double* x) {
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x);
+}
+void launch_contact_frames(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* nrm,
+                           double* t1, double* t2) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_contact_frames, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, b, nodes, q, nrm, t1, t2);
+}
+void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_contact_distance, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, b, nodes, q, delta);
+}
+template <class TV>
+void launch_contact_gather(hipStream_t s, int K, const int* cp_ptr, const int* cp_row, const TV* cp_val, const TV* y,
+                           int n, double* out) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_contact_gather<TV>, dim3(blocks_for(K, 4)), dim3(256), 0, s, K, cp_ptr, cp_row, cp_val, y, n, out);
+}
+template <class TV>
+void launch_contact_scatter(hipStream_t s, int nrows, const int* rc_ptr, const int* rc_row, const int* rc_c,
+                            const TV* rc_val, const double* f, int n, TV* y) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(k_contact_scatter<TV>, dim3(blocks_for(nrows, 4)), dim3(256), 0, s, nrows, rc_ptr, rc_row, rc_c,
+                       rc_val, f, n, y);
+}
+PHD_INST_TV(float)
+PHD_INST_TV(double)
+
+void launch_gather_vals(hipStream_t s, int64_t m, const int* rows, const int* cols, const int64_t* base,
+                        const double* V64, double* o64, float* o32) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_gather_vals, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, rows, cols, base, V64, o64, o32);
+}
+void launch_scatter_dense(hipStream_t s, int64_t m, const int* ridx, const int* cidx, const double* val, int K, double* D) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_scatter_dense, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, ridx, cidx, val, K, D);
+}
+void launch_symmetrize_lower(hipStream_t s, int K, double* H) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_symmetrize_lower, dim3(blocks_for((int64_t)K * K, 256)), dim3(256), 0, s, K, H);
+}
+void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_gather_nodes, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, nodes, q, out);
+}
+void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G) {
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
+}
+void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_add_node_force, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, nodes, f, r);
+}
+template <class TM>
+void launch_assemble_msys(hipStream_t s, int K, const double* H, const double* nrm, const double* om, const double* E, TM* M) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_assemble_msys<TM>, dim3(blocks_for((int64_t)K * K, 256)), dim3(256), 0, s, K, H, nrm, om, E, M);
+}
+template void launch_assemble_msys<float>(hipStream_t, int, const double*, const double*, const double*, const double*, float*);
+template void launch_assemble_msys<double>(hipStream_t, int, const double*, const double*, const double*, const double*, double*);
+void launch_h_apply(hipStream_t s, int K, const double* H, const double* f, double* out) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_h_apply, dim3(blocks_for(K, 4)), dim3(256), 0, s, K, H, f, out);
+}
+
+template <class TW>
+void launch_kpd_build(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* const* kn,
+                      const double* q, TW* W) {
+    KnPtrs p;
+    for (int h = 0; h < 4; ++h) p.kn[h] = (h < m.n_nh) ? kn[h] : nullptr;
+    const int64_t total = (int64_t)m.ne * m.samples;
+    if (m.edim == 8)
+        hipLaunchKernelGGL((k_kpd_build<8, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, W);
+    else
+        hipLaunchKernelGGL((k_kpd_build<4, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, W);
+}
+template <class TW>
+void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const double* dq, double* elem_out) {
+    if (m.edim == 8)
+        hipLaunchKernelGGL((k_kpd_apply<8, TW>), dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, W, dq, elem_out);
+    else
+        hipLaunchKernelGGL((k_kpd_apply<4, TW>), dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, W, dq, elem_out);
+}
+template void launch_kpd_build<float>(hipStream_t, const DevMaterials&, const double*, const double*, const double* const*,
+                                      const double*, float*);
+template void launch_kpd_build<double>(hipStream_t, const DevMaterials&, const double*, const double*, const double* const*,
+                                       const double*, double*);
+template void launch_kpd_apply<float>(hipStream_t, const DevMaterials&, const float*, const double*, double*);
+template void launch_kpd_apply<double>(hipStream_t, const DevMaterials&, const double*, const double*, double*);
+
+void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
+                  double* g_vol, double* g_nh) {
+    if (m.edim == 8)
+        hipLaunchKernelGGL(k_dmatw<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_cor, g_vol, g_nh);
+    else
+        hipLaunchKernelGGL(k_dmatw<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_cor, g_vol, g_nh);
+}
+
+}  // namespace phd

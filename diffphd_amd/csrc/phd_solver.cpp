@@ -1,0 +1,1278 @@
+// Host orchestration of the PD forward/backward time step on one MI355X.
+//
+// The forward restates ForwardProjectiveDynamicsAlgPhd
+// (cpp/core/src/fem/deformable_projective_dynamics_forward_gpu.cpp:387-1594) and
+// the backward BackwardProjectiveDynamicsAlgPhd
+// (cpp/core/src/fem/deformable_projective_dynamics_backward_gpu.cpp:648-1250);
+// every vector of size 3N lives in HBM and only K-sized contact vectors and a
+// handful of reduction results cross PCIe per iteration.
+//
+// Each global solve is written in residual form, x_{k+1} = x_k + S^T S r_k with
+// r_k evaluated in FP64 by the 27-point stencil of A, so the FP32 factor acts
+// as a preconditioner and the fixed point keeps FP64 accuracy; with the FP64
+// factor the form is algebraically identical to x = A^-1 b.
+#include "phd_solver.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace phd {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+static void check(bool ok, const std::string& msg) {
+    if (!ok) throw std::runtime_error(msg);
+}
+static double opt_or(const std::map<std::string, double>& o, const char* k, double d) {
+    auto it = o.find(k);
+    return it == o.end() ? d : it->second;
+}
+static double opt_req(const std::map<std::string, double>& o, const char* k) {
+    auto it = o.find(k);
+    check(it != o.end(), std::string("Missing option ") + k + ".");
+    return it->second;
+}
+static double nrm2(const std::vector<double>& v) {
+    double s = 0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+}
+// Dense symmetric solve by LDL^T (small systems: AA Gram). Row-major.
+static std::vector<double> small_ldlt_solve(std::vector<double> M, int n, std::vector<double> x) {
+    std::vector<double> d(n);
+    for (int j = 0; j < n; ++j) {
+        double djj = M[j * n + j];
+        for (int k = 0; k < j; ++k) djj -= M[j * n + k] * M[j * n + k] * d[k];
+        d[j] = djj;
+        for (int i = j + 1; i < n; ++i) {
+            double s = M[i * n + j];
+            for (int k = 0; k < j; ++k) s -= M[i * n + k] * M[j * n + k] * d[k];
+            M[i * n + j] = djj != 0 ? s / djj : 0;
+        }
+    }
+    for (int i = 0; i < n; ++i) for (int k = 0; k < i; ++k) x[i] -= M[i * n + k] * x[k];
+    for (int i = 0; i < n; ++i) x[i] = d[i] != 0 ? x[i] / d[i] : 0;
+    for (int i = n - 1; i >= 0; --i) for (int k = i + 1; k < n; ++k) x[i] -= M[k * n + i] * x[k];
+    return x;
+}
+static double det3(const double* A) {  // column-major
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) + A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+Body::Body(int device) : device_(device) {
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    rocblas_handle h;
+    check(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
+    rocblas_set_stream(h, stream_);
+    blas_ = h;
+}
+Body::~Body() {
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    if (blas_) rocblas_destroy_handle((rocblas_handle)blas_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ---------------------------------------------------------------------------
+// Scene: Mesh::Initialize (mesh.cpp:4-32), HexDeformable/TetDeformable
+// ::InitializeFiniteElementSamples (hex_deformable.cpp:3-50, tet_deformable.cpp:3-41).
+// ---------------------------------------------------------------------------
+void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems, double density) {
+    check(edim == 8 || edim == 4, "Only hex (8) and tet (4) elements are supported.");
+    edim_ = edim; nv_ = nv; ne_ = ne; dofs_ = 3 * nv; density_ = density;
+    samples_ = edim == 8 ? 8 : 1;
+    X_.assign(X, X + 3 * (size_t)nv);
+    elems_.assign(elems, elems + (size_t)edim * ne);
+    double vsum = 0;
+    for (int e = 0; e < ne; ++e) {
+        const int* v = &elems_[(size_t)e * edim];
+        const int c[3] = {1, 2, edim == 8 ? 4 : 3};
+        double A[9];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) A[i + 3 * j] = X_[3 * v[c[j]] + i] - X_[3 * v[0] + i];
+        vsum += std::fabs(det3(A)) / (edim == 8 ? 1.0 : 6.0);
+    }
+    element_volume_ = vsum / ne;
+    const double dx = std::pow(element_volume_, 1.0 / 3.0);
+    if (edim == 8) {
+        g_stride_ = 0;
+        G_.assign(8 * 24, 0.0);
+        const double r = 1 / std::sqrt(3.0), inv = 1 / dx;
+        for (int s = 0; s < 8; ++s) {
+            const int bx = (s >> 2) & 1, by = (s >> 1) & 1, bz = s & 1;
+            const double x = ((bx - 0.5) * r + 0.5) * dx, y = ((by - 0.5) * r + 0.5) * dx, z = ((bz - 0.5) * r + 0.5) * dx;
+            const double nx = x * inv, ny = y * inv, nz = z * inv, cx = 1 - nx, cy = 1 - ny, cz = 1 - nz;
+            const double rows[8][3] = {
+                {-inv * cy * cz, cx * -inv * cz, cx * cy * -inv}, {-inv * cy * nz, cx * -inv * nz, cx * cy * inv},
+                {-inv * ny * cz, cx * inv * cz, cx * ny * -inv},  {-inv * ny * nz, cx * inv * nz, cx * ny * inv},
+                {inv * cy * cz, nx * -inv * cz, nx * cy * -inv},  {inv * cy * nz, nx * -inv * nz, nx * cy * inv},
+                {inv * ny * cz, nx * inv * cz, nx * ny * -inv},   {inv * ny * nz, nx * inv * nz, nx * ny * inv}};
+            for (int k = 0; k < 8; ++k)
+                for (int j = 0; j < 3; ++j) G_[s * 24 + k * 3 + j] = rows[k][j];
+        }
+    } else {
+        g_stride_ = 12;
+        G_.assign((size_t)ne * 12, 0.0);
+        for (int e = 0; e < ne; ++e) {
+            const int* v = &elems_[(size_t)e * 4];
+            double Bi[9];
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) Bi[i + 3 * j] = X_[3 * v[j + 1] + i] - X_[3 * v[0] + i];
+            const double d = det3(Bi);
+            double B[9];  // inverse, column-major
+            B[0] = (Bi[4] * Bi[8] - Bi[7] * Bi[5]) / d; B[3] = (Bi[6] * Bi[5] - Bi[3] * Bi[8]) / d; B[6] = (Bi[3] * Bi[7] - Bi[6] * Bi[4]) / d;
+            B[1] = (Bi[7] * Bi[2] - Bi[1] * Bi[8]) / d; B[4] = (Bi[0] * Bi[8] - Bi[6] * Bi[2]) / d; B[7] = (Bi[6] * Bi[1] - Bi[0] * Bi[7]) / d;
+            B[2] = (Bi[1] * Bi[5] - Bi[4] * Bi[2]) / d; B[5] = (Bi[3] * Bi[2] - Bi[0] * Bi[5]) / d; B[8] = (Bi[0] * Bi[4] - Bi[3] * Bi[1]) / d;
+            double* g = &G_[(size_t)e * 12];
+            for (int j = 0; j < 3; ++j) {
+                g[0 * 3 + j] = -B[0 + 3 * j] - B[1 + 3 * j] - B[2 + 3 * j];
+                g[1 * 3 + j] = B[0 + 3 * j];
+                g[2 * 3 + j] = B[1 + 3 * j];
+                g[3 * 3 + j] = B[2 + 3 * j];
+            }
+        }
+    }
+    energies_.clear();
+    dirichlet_.clear();
+    gravity_.clear();
+    boundary_ = HostBoundary();
+    fb_vertices_.clear();
+    tape_.clear();
+    ready_ = false;
+}
+
+// AddPdEnergy, deformable_pd_energy.cpp:42-153 (element energies).
+void Body::add_pd_energy(const std::string& type, const std::vector<double>& params, const std::vector<int>& idx) {
+    ready_ = false;
+    Energy en;
+    const int np = (int)params.size();
+    if (type == "corotated" || type == "volume") {
+        check(np == 1 || np == ne_, "Inconsistent param size: expected 1 (homogeneous) or element_num=" +
+                                       std::to_string(ne_) + " (per-element), got " + std::to_string(np));
+        en.type = type == "corotated" ? kCorotated : kVolume;
+        if (np == 1) en.stiffness = params[0];
+        else { en.elem_stiff = params; en.stiffness = std::accumulate(params.begin(), params.end(), 0.0) / np; }
+    } else if (type == "pd_neohookean") {
+        check(np == 2 || np == 2 * ne_, "pd_neohookean expects params = [mu, lambda] or per-element [mu_1, la_1, ..., mu_n, la_n] of length 2*element_num=" +
+                                            std::to_string(2 * ne_) + ", got " + std::to_string(np));
+        en.type = kNeoHookean;
+        if (np == 2) { en.mu = params[0]; en.la = params[1]; en.stiffness = 2 * params[0]; }
+        else {
+            double ms = 0, ls = 0;
+            en.elem_stiff.resize(ne_);
+            for (int i = 0; i < ne_; ++i) { ms += params[2 * i]; ls += params[2 * i + 1]; en.elem_stiff[i] = 2 * params[2 * i]; }
+            en.mu = ms / ne_; en.la = ls / ne_;
+            en.stiffness = std::accumulate(en.elem_stiff.begin(), en.elem_stiff.end(), 0.0) / ne_;
+        }
+        check((int)std::count_if(energies_.begin(), energies_.end(), [](const Energy& x) { return x.type == kNeoHookean; }) < 4,
+              "At most 4 pd_neohookean energies are supported.");
+    } else {
+        throw std::runtime_error("Unsupported PD energy: " + type);
+    }
+    en.member.assign(ne_, 0);
+    if (idx.empty()) std::fill(en.member.begin(), en.member.end(), 1);
+    for (int i : idx) { check(0 <= i && i < ne_, "Element index out of bound."); en.member[i] = 1; }
+    energies_.push_back(en);
+}
+
+void Body::set_dirichlet(int dof, double val) {
+    check(0 <= dof && dof < dofs_, "Dirichlet dof out of bound.");
+    dirichlet_[dof] = val;
+    ready_ = false;
+}
+
+void Body::add_gravity(const double* g) { gravity_.assign(g, g + 3); }
+
+// SetFrictionalBoundary, deformable_friction.cpp:7-89, plus heightfield/press.
+void Body::set_boundary(const std::string& type, const std::vector<double>& p, const std::vector<int>& idx) {
+    ready_ = false;
+    HostBoundary b;
+    if (type == "planar") {
+        check(p.size() == 4, "Incompatible parameter number.");
+        const double nn = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        check(nn > 2.220446049250313e-16, "Singular normal.");
+        b.type = 1;
+        for (int i = 0; i < 3; ++i) b.n[i] = p[i] / nn;
+        b.off = p[3] / nn;
+    } else if (type == "spherical") {
+        check(p.size() == 4, "Incompatible parameter number.");
+        b.type = 2;
+        for (int i = 0; i < 3; ++i) b.c[i] = p[i];
+        b.r = p[3];
+    } else if (type == "heightfield" || type == "press") {
+        check(p.size() >= 2, "Incompatible parameter number.");
+        const int nw = (int)p[1];
+        const size_t need = 2 + 4 * (size_t)nw + (type == "press" ? 1 : 0);
+        check(p.size() == need, "Incompatible parameter number.");
+        b.type = type == "press" ? 4 : 3;
+        b.z0 = p[0];
+        b.waves.assign(p.begin() + 2, p.begin() + 2 + 4 * nw);
+        if (type == "press") b.plane_z = p[2 + 4 * nw];
+    } else {
+        throw std::runtime_error("Unsupported frictional boundary type: " + type);
+    }
+    std::set<int> uniq(idx.begin(), idx.end());
+    check(uniq.size() == idx.size(), "Duplicated vertex elements.");
+    for (int i : uniq) check(0 <= i && i < nv_, "Frictional boundary vertex out of bound.");
+    boundary_ = b;
+    fb_vertices_.clear();
+    int cnt = 0;
+    for (int i : uniq) fb_vertices_[i] = cnt++;
+}
+
+DevBoundary Body::dev_boundary() {
+    DevBoundary b;
+    b.type = boundary_.type;
+    for (int i = 0; i < 3; ++i) { b.n[i] = boundary_.n[i]; b.c[i] = boundary_.c[i]; }
+    b.off = boundary_.off;
+    b.r = boundary_.r;
+    b.z0 = boundary_.z0;
+    b.plane_z = boundary_.plane_z;
+    b.nwaves = (int)boundary_.waves.size() / 4;
+    b.waves = d_waves_.p;
+    return b;
+}
+
+long long Body::factor_bytes() const {
+    return (long long)(V64_.n * sizeof(double) + V32_.n * sizeof(float));
+}
+
+// ---------------------------------------------------------------------------
+// Per-element weights of the merged energies (local step and K_pd).
+// ---------------------------------------------------------------------------
+void Body::build_materials() {
+    const double vs = element_volume_ / samples_;
+    std::vector<double> wc(ne_, 0.0), wv(ne_, 0.0), kc(ne_, 0.0), kv(ne_, 0.0);
+    has_cor_ = has_vol_ = false;
+    n_nh_ = 0;
+    nh_energy_.clear();
+    std::vector<std::vector<double>> wn, kn;
+    for (int ei = 0; ei < (int)energies_.size(); ++ei) {
+        const Energy& en = energies_[ei];
+        if (en.type == kNeoHookean) {
+            std::vector<double> w(ne_, 0.0), k(ne_, 0.0);
+            for (int e = 0; e < ne_; ++e)
+                if (en.member[e]) { w[e] = en.elem_w(e) * vs; k[e] = (kpd_exact_ ? en.elem_w(e) : en.stiffness) * vs; }
+            wn.push_back(w);
+            kn.push_back(k);
+            nh_energy_.push_back(ei);
+            continue;
+        }
+        auto& w = en.type == kCorotated ? wc : wv;
+        auto& k = en.type == kCorotated ? kc : kv;
+        (en.type == kCorotated ? has_cor_ : has_vol_) = true;
+        for (int e = 0; e < ne_; ++e)
+            if (en.member[e]) { w[e] += en.elem_w(e) * vs; k[e] += (kpd_exact_ ? en.elem_w(e) : en.stiffness) * vs; }
+    }
+    n_nh_ = (int)wn.size();
+    if (has_cor_) { d_wc_.upload(wc); d_kc_.upload(kc); } else { d_wc_.release(); d_kc_.release(); }
+    if (has_vol_) { d_wv_.upload(wv); d_kv_.upload(kv); } else { d_wv_.release(); d_kv_.release(); }
+    for (int h = 0; h < 4; ++h) {
+        if (h < n_nh_) { d_wn_[h].upload(wn[h]); d_kn_[h].upload(kn[h]); }
+        else { d_wn_[h].release(); d_kn_[h].release(); }
+    }
+    mat_ = DevMaterials();
+    mat_.ne = ne_;
+    mat_.edim = edim_;
+    mat_.samples = samples_;
+    mat_.elems = d_elems_.p;
+    mat_.G = d_G_.p;
+    mat_.g_stride = g_stride_;
+    mat_.wc = has_cor_ ? d_wc_.p : nullptr;
+    mat_.wv = has_vol_ ? d_wv_.p : nullptr;
+    mat_.n_nh = n_nh_;
+    for (int h = 0; h < n_nh_; ++h) {
+        const Energy& en = energies_[nh_energy_[h]];
+        mat_.wn[h] = d_wn_[h].p;
+        mat_.nh_mu[h] = en.mu;
+        mat_.nh_la[h] = en.la;
+        mat_.nh_k[h] = en.stiffness;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SetupProjectiveDynamicsSolver (deformable_projective_dynamics_forward.cpp:8-183)
+// and the sparse-inverse factor build (forward_gpu.cpp:584-764).
+// ---------------------------------------------------------------------------
+void Body::setup(double dt, int threads) {
+    if (ready_) return;  // like pd_solver_ready_, a dt change does not rebuild (reference quirk)
+    auto t0 = std::chrono::steady_clock::now();
+    check(nv_ > 0, "Mesh is not initialized.");
+    check(!energies_.empty(), "No PD energy was added.");
+    threads = std::max(1, threads);
+    // Dirichlet conditions must fix whole vertices (one factor shared by the three axes).
+    dmask_.assign(nv_, 0);
+    dval_full_.assign(dofs_, 0.0);
+    ddof_.clear();
+    dval_.clear();
+    for (auto& kv : dirichlet_) {
+        dmask_[kv.first / 3] = 1;
+        dval_full_[kv.first] = kv.second;
+        ddof_.push_back(kv.first);
+        dval_.push_back(kv.second);
+    }
+    for (int v = 0; v < nv_; ++v)
+        if (dmask_[v])
+            for (int d = 0; d < 3; ++d)
+                check(dirichlet_.count(3 * v + d), "Dirichlet conditions must fix all three DOFs of a vertex "
+                                                   "(per-axis Dirichlet sets are not supported by this build).");
+    const double mass = density_ * element_volume_;
+    const double inv_h2m = mass / (dt * dt);
+    const double vs = element_volume_ / samples_;
+    // scalar A (identical for the three axes)
+    std::vector<double> w_e(ne_, 0.0);
+    for (int e = 0; e < ne_; ++e)
+        for (auto& en : energies_) if (en.member[e]) w_e[e] += en.elem_w(e);
+    std::vector<std::vector<std::pair<int, double>>> rows(nv_);
+    for (int v = 0; v < nv_; ++v) rows[v].push_back({v, dmask_[v] ? 1.0 : inv_h2m});
+    std::vector<double> Ke((size_t)edim_ * edim_);
+    for (int e = 0; e < ne_; ++e) {
+        if (g_stride_ == 0 && e > 0) { /* shared G: Ke reused */ }
+        else {
+            std::fill(Ke.begin(), Ke.end(), 0.0);
+            for (int s = 0; s < samples_; ++s) {
+                const double* g = &G_[(size_t)e * g_stride_ + (size_t)s * edim_ * 3];
+                for (int k = 0; k < edim_; ++k)
+                    for (int l = 0; l < edim_; ++l)
+                        Ke[k * edim_ + l] += vs * (g[k * 3] * g[l * 3] + g[k * 3 + 1] * g[l * 3 + 1] + g[k * 3 + 2] * g[l * 3 + 2]);
+            }
+        }
+        const int* v = &elems_[(size_t)e * edim_];
+        for (int k = 0; k < edim_; ++k) {
+            if (dmask_[v[k]]) continue;
+            for (int l = 0; l < edim_; ++l) {
+                if (dmask_[v[l]]) continue;
+                rows[v[k]].push_back({v[l], w_e[e] * Ke[k * edim_ + l]});
+            }
+        }
+    }
+    Ap_.assign(nv_ + 1, 0);
+    Ai_.clear();
+    Ax_.clear();
+    for (int v = 0; v < nv_; ++v) {
+        auto& r = rows[v];
+        std::sort(r.begin(), r.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t i = 0; i < r.size(); ++i) {
+            if (!Ai_.empty() && (int)Ai_.size() > Ap_[v] && Ai_.back() == r[i].first) Ax_.back() += r[i].second;
+            else { Ai_.push_back(r[i].first); Ax_.push_back(r[i].second); }
+        }
+        Ap_[v + 1] = (int)Ai_.size();
+        std::vector<std::pair<int, double>>().swap(r);
+    }
+    // factor (host FP64) and expansion of S = L^-1 (device FP64, FP32 copy)
+    factor_.build(nv_, Ap_, Ai_, Ax_, X_, threads);
+    plan_.build(factor_, 2048, 512);
+    {
+        DevBuf<int64_t> dLp;
+        DevBuf<int> dLi, dpar, ddep;
+        DevBuf<double> dLx;
+        dLp.upload(factor_.Lp);
+        dLi.upload(factor_.Li);
+        dLx.upload(factor_.Lx);
+        dpar.upload(factor_.parent);
+        ddep.upload(factor_.depth);
+        d_fd_.upload(factor_.fd);
+        d_base_.upload(factor_.base);
+        V64_.alloc(factor_.v_len);
+        V32_.alloc(factor_.v_len);
+        hip_check(hipMemsetAsync(V64_.p, 0, V64_.n * sizeof(double), stream_), "memset V64");
+        hip_check(hipMemsetAsync(V32_.p, 0, V32_.n * sizeof(float), stream_), "memset V32");
+        const int ntiles = (nv_ + 63) / 64;
+        const int64_t ws_stride = (int64_t)(64 + factor_.max_depth + 1) * 64;
+        const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)(1LL << 31) / (ws_stride * 8)));
+        DevBuf<double> ws;
+        ws.alloc((size_t)(batch * ws_stride));
+        for (int t0b = 0; t0b < ntiles; t0b += (int)batch) {
+            const int t1b = std::min<int>(ntiles, t0b + (int)batch);
+            launch_expand_s(stream_, nv_, t0b, t1b, dLp.p, dLi.p, dLx.p, dpar.p, ddep.p, d_fd_.p, d_base_.p, ws.p, ws_stride,
+                            V64_.p, V32_.p);
+        }
+        hip_check(hipStreamSynchronize(stream_), "expand S");
+        hip_check(hipGetLastError(), "expand S launch");
+    }
+    d_perm_.upload(factor_.perm);
+    d_ru_row_.upload(plan_.ru_row);
+    d_ru_c0_.upload(plan_.ru_c0);
+    d_ru_c1_.upload(plan_.ru_c1);
+    d_row_unit_ptr_.upload(plan_.row_unit_ptr);
+    d_items_.upload(plan_.items);
+    d_cu_tile_.upload(plan_.cu_tile);
+    d_cu_i0_.upload(plan_.cu_i0);
+    d_cu_i1_.upload(plan_.cu_i1);
+    d_tile_unit_ptr_.upload(plan_.tile_unit_ptr);
+    d_Ap_.upload(Ap_);
+    d_Ai_.upload(Ai_);
+    d_Ax_.upload(Ax_);
+    d_dmask_.upload(dmask_);
+    d_ddof_.upload(ddof_);
+    d_dval_.upload(dval_);
+    d_dval_full_.upload(dval_full_);
+    // constant Dirichlet part of the local step: sum_e w_e (vol/s) sum_s A_s^T F_bound,s
+    c_dir_.assign(dofs_, 0.0);
+    if (!dirichlet_.empty()) {
+        for (int e = 0; e < ne_; ++e) {
+            const int* v = &elems_[(size_t)e * edim_];
+            bool any = false;
+            for (int k = 0; k < edim_; ++k) any |= dmask_[v[k]] != 0;
+            if (!any || w_e[e] == 0) continue;
+            for (int s = 0; s < samples_; ++s) {
+                const double* g = &G_[(size_t)e * g_stride_ + (size_t)s * edim_ * 3];
+                double Fb[9] = {0};
+                for (int k = 0; k < edim_; ++k)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) Fb[i + 3 * j] += dval_full_[3 * v[k] + i] * g[k * 3 + j];
+                for (int k = 0; k < edim_; ++k)
+                    for (int i = 0; i < 3; ++i) {
+                        double a = 0;
+                        for (int j = 0; j < 3; ++j) a += Fb[i + 3 * j] * g[k * 3 + j];
+                        c_dir_[3 * v[k] + i] += w_e[e] * vs * a;
+                    }
+            }
+        }
+    }
+    d_c_dir_.upload(c_dir_);
+    // mesh data on device
+    std::vector<int> soa((size_t)edim_ * ne_);
+    for (int e = 0; e < ne_; ++e)
+        for (int k = 0; k < edim_; ++k) soa[(size_t)k * ne_ + e] = elems_[(size_t)e * edim_ + k];
+    d_elems_.upload(soa);
+    d_G_.upload(G_);
+    std::vector<int> cnt(nv_ + 1, 0), inc((size_t)ne_ * edim_);
+    for (size_t i = 0; i < elems_.size(); ++i) cnt[elems_[i] + 1]++;
+    for (int v = 0; v < nv_; ++v) cnt[v + 1] += cnt[v];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < ne_; ++e)
+        for (int k = 0; k < edim_; ++k) inc[fill[elems_[(size_t)e * edim_ + k]]++] = e * edim_ + k;
+    d_inc_ptr_.upload(cnt);
+    d_inc_idx_.upload(inc);
+    build_materials();
+    // work vectors
+    for (auto* b : {&d_q_, &d_qprev_, &d_qtilde_, &d_b_, &d_r_, &d_pdrhs_, &d_pdrhs_prev_, &d_g_, &d_tmp_, &d_kz_})
+        b->alloc(dofs_);
+    d_elem_.alloc((size_t)ne_ * edim_ * 3);
+    d_dot_partial_.alloc(dots_partial_size());
+    d_dot_out_.alloc(16);
+    w32_.alloc((size_t)3 * nv_); y32_.alloc((size_t)3 * nv_);
+    w64_.alloc((size_t)3 * nv_); y64_.alloc((size_t)3 * nv_);
+    part32_.alloc(3 * plan_.ru_row.size() + 3); part64_.alloc(3 * plan_.ru_row.size() + 3);
+    xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
+    setup_contact(threads);
+    ready_ = true;
+    ready_dt_ = dt;
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Contact paths of S restricted to the candidate columns and the candidate
+// block H = A^-1_CC = S_C^T S_C (the Delassus operator's building block,
+// forward_gpu.cpp:840-1066).
+void Body::setup_contact(int threads) {
+    (void)threads;
+    cand_.clear();
+    for (auto& kv : fb_vertices_) cand_.push_back(kv.first);
+    K_ = (int)cand_.size();
+    if (!boundary_.waves.empty()) d_waves_.upload(boundary_.waves);
+    if (K_ == 0) return;
+    d_cand_.upload(cand_);
+    // paths: contact c in column iperm[v] walks its ancestors
+    std::vector<int> cp_ptr(K_ + 1, 0), cp_row, cp_col;
+    for (int c = 0; c < K_; ++c) {
+        const int j = factor_.iperm[cand_[c]];
+        for (int k = j; k != -1; k = factor_.parent[k]) { cp_row.push_back(k); cp_col.push_back(j); }
+        cp_ptr[c + 1] = (int)cp_row.size();
+    }
+    const int64_t m = (int64_t)cp_row.size();
+    DevBuf<int> drow, dcol;
+    drow.upload(cp_row);
+    dcol.upload(cp_col);
+    d_cp_val64_.alloc(m);
+    d_cp_val32_.alloc(m);
+    launch_gather_vals(stream_, m, drow.p, dcol.p, d_base_.p, V64_.p, d_cp_val64_.p, d_cp_val32_.p);
+    d_cp_ptr_.upload(cp_ptr);
+    d_cp_row_.upload(cp_row);
+    // row-compact transpose
+    std::vector<int> rowid(nv_, -1), urows;
+    for (int64_t p = 0; p < m; ++p)
+        if (rowid[cp_row[p]] < 0) { rowid[cp_row[p]] = (int)urows.size(); urows.push_back(cp_row[p]); }
+    const int R = (int)urows.size();
+    std::vector<int> rc_ptr(R + 1, 0);
+    for (int64_t p = 0; p < m; ++p) rc_ptr[rowid[cp_row[p]] + 1]++;
+    for (int r = 0; r < R; ++r) rc_ptr[r + 1] += rc_ptr[r];
+    std::vector<int> rc_c(m), rc_src(m), fillp(rc_ptr.begin(), rc_ptr.end() - 1);
+    for (int c = 0; c < K_; ++c)
+        for (int p = cp_ptr[c]; p < cp_ptr[c + 1]; ++p) {
+            const int slot = fillp[rowid[cp_row[p]]]++;
+            rc_c[slot] = c;
+            rc_src[slot] = p;
+        }
+    std::vector<int> rc_rowk(R), rc_rows_full(m), rc_cols_full(m);
+    for (int r = 0; r < R; ++r) rc_rowk[r] = urows[r];
+    for (int64_t s = 0; s < m; ++s) { rc_rows_full[s] = cp_row[rc_src[s]]; rc_cols_full[s] = cp_col[rc_src[s]]; }
+    DevBuf<int> dr2, dc2;
+    dr2.upload(rc_rows_full);
+    dc2.upload(rc_cols_full);
+    d_rc_val64_.alloc(m);
+    d_rc_val32_.alloc(m);
+    launch_gather_vals(stream_, m, dr2.p, dc2.p, d_base_.p, V64_.p, d_rc_val64_.p, d_rc_val32_.p);
+    d_rc_ptr_.upload(rc_ptr);
+    d_rc_row_.upload(rc_rowk);
+    d_rc_c_.upload(rc_c);
+    rc_nrows_ = R;
+    // H = S_C^T S_C via a dense S_C (R x K) and rocBLAS dsyrk
+    {
+        DevBuf<double> D;
+        D.alloc((size_t)R * K_);
+        hip_check(hipMemsetAsync(D.p, 0, D.n * sizeof(double), stream_), "memset S_C");
+        std::vector<int> ridx(m), cidx(m);
+        for (int c = 0; c < K_; ++c)
+            for (int p = cp_ptr[c]; p < cp_ptr[c + 1]; ++p) { ridx[p] = rowid[cp_row[p]]; cidx[p] = c; }
+        DevBuf<int> dri, dci;
+        dri.upload(ridx);
+        dci.upload(cidx);
+        launch_scatter_dense(stream_, m, dri.p, dci.p, d_cp_val64_.p, K_, D.p);
+        d_H_.alloc((size_t)K_ * K_);
+        const double one = 1.0, zero = 0.0;
+        // D is row-major R x K == column-major K x R (= S_C^T); H = D D^T.
+        check(rocblas_dsyrk((rocblas_handle)blas_, rocblas_fill_lower, rocblas_operation_none, K_, R, &one, D.p, K_, &zero,
+                            d_H_.p, K_) == rocblas_status_success, "rocblas_dsyrk failed");
+        launch_symmetrize_lower(stream_, K_, d_H_.p);
+        hip_check(hipStreamSynchronize(stream_), "contact H");
+    }
+    std::vector<double> Hdiag(K_);
+    for (int c = 0; c < K_; ++c)
+        hip_check(hipMemcpy(&Hdiag[c], d_H_.p + (size_t)c * K_ + c, sizeof(double), hipMemcpyDeviceToHost), "H diag");
+    Hdiag_ = Hdiag;
+    for (auto* b : {&d_nrm_, &d_t1_, &d_t2_, &d_F_, &d_dF_, &d_uC_, &d_tmpK3_}) b->alloc((size_t)3 * K_);
+    for (auto* b : {&d_delta_, &d_om_, &d_E_, &d_rhsK_}) b->alloc(K_);
+    d_rhsK32_.alloc(K_);
+    d_info_.alloc(1);
+}
+
+// ---------------------------------------------------------------------------
+// Building blocks.
+// ---------------------------------------------------------------------------
+hipEvent_t Body::next_event() {
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        ev_pool_.push_back(e);
+    }
+    return ev_pool_[ev_used_++];
+}
+
+// Sums the recorded kernel durations (milliseconds) and recycles the events.
+void Body::collect_timing() {
+    hip_check(hipStreamSynchronize(stream_), "timing sync");
+    for (auto& p : ev_rows_) { float ms = 0; (void)hipEventElapsedTime(&ms, p.first, p.second); t_spmv_rows_ms += ms; ++n_spmv_rows; }
+    for (auto& p : ev_cols_) { float ms = 0; (void)hipEventElapsedTime(&ms, p.first, p.second); t_spmv_cols_ms += ms; ++n_spmv_cols; }
+    ev_rows_.clear();
+    ev_cols_.clear();
+    ev_used_ = 0;
+}
+
+template <class TV>
+void Body::spmv_S(const double* r, DevBuf<TV>& w, DevBuf<TV>& y, DevBuf<TV>& part, const TV* V) {
+    launch_perm_in<TV>(stream_, nv_, d_perm_.p, r, nullptr, w.p);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timing) { a = next_event(); b = next_event(); (void)hipEventRecord(a, stream_); }
+    launch_spmv_rows<TV>(stream_, (int)plan_.ru_row.size(), d_ru_row_.p, d_ru_c0_.p, d_ru_c1_.p, d_base_.p, V, w.p, nv_, part.p);
+    if (timing) { (void)hipEventRecord(b, stream_); ev_rows_.push_back({a, b}); }
+    launch_spmv_rows_fin<TV>(stream_, nv_, d_row_unit_ptr_.p, part.p, y.p);
+}
+
+template <class TV>
+void Body::spmv_ST(DevBuf<TV>& y, DevBuf<TV>& xpart, const TV* V, const double* add, const double* dval, double* out) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timing) { a = next_event(); b = next_event(); (void)hipEventRecord(a, stream_); }
+    launch_spmv_cols<TV>(stream_, (int)plan_.cu_tile.size(), d_cu_tile_.p, d_cu_i0_.p, d_cu_i1_.p, d_items_.p, d_fd_.p,
+                         d_base_.p, V, y.p, nv_, xpart.p);
+    if (timing) { (void)hipEventRecord(b, stream_); ev_cols_.push_back({a, b}); }
+    launch_spmv_cols_fin<TV>(stream_, nv_, d_tile_unit_ptr_.p, xpart.p, d_perm_.p, add, dval ? d_dmask_.p : nullptr, dval, out);
+}
+
+template <class TV>
+void Body::contact_gather(DevBuf<TV>& y, double* out) {
+    const TV* val = std::is_same<TV, float>::value ? (const TV*)d_cp_val32_.p : (const TV*)d_cp_val64_.p;
+    launch_contact_gather<TV>(stream_, K_, d_cp_ptr_.p, d_cp_row_.p, val, y.p, nv_, out);
+}
+template <class TV>
+void Body::contact_scatter(const double* f, DevBuf<TV>& y) {
+    const TV* val = std::is_same<TV, float>::value ? (const TV*)d_rc_val32_.p : (const TV*)d_rc_val64_.p;
+    launch_contact_scatter<TV>(stream_, rc_nrows_, d_rc_ptr_.p, d_rc_row_.p, d_rc_c_.p, val, f, nv_, y.p);
+}
+
+void Body::local_rhs(const double* q, double inv_h2m) {
+    launch_local_step(stream_, mat_, q, d_elem_.p);
+    launch_gather_rhs(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, d_c_dir_.p, d_qtilde_.p, inv_h2m, d_pdrhs_.p, d_b_.p);
+}
+
+std::vector<double> Body::dots(const std::vector<std::pair<const double*, const double*>>& pairs) {
+    DotArgs a;
+    a.m = (int)pairs.size();
+    check(a.m <= 16, "too many dot products");
+    for (int i = 0; i < a.m; ++i) { a.a[i] = pairs[i].first; a.b[i] = pairs[i].second; }
+    launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_out_.p);
+    std::vector<double> out(a.m);
+    hip_check(hipMemcpyAsync(out.data(), d_dot_out_.p, sizeof(double) * a.m, hipMemcpyDeviceToHost, stream_), "dots D2H");
+    hip_check(hipStreamSynchronize(stream_), "dots sync");
+    return out;
+}
+
+void Body::kpd_apply(const double* x, double* out, int precision) {
+    if (precision == 1) launch_kpd_apply<float>(stream_, mat_, d_W32_.p, x, d_elem_.p);
+    else launch_kpd_apply<double>(stream_, mat_, d_W64_.p, x, d_elem_.p);
+    launch_gather_vertex(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, out);
+}
+
+// Dense SPD solve on the device (rocSOLVER Cholesky). rhs (host, n) is
+// overwritten with the solution. The matrix is already in d_M64_/d_M32_.
+void Body::solve_dense_spd(int n, int precision, const double* /*unused*/, double* rhs) {
+    rocblas_handle h = (rocblas_handle)blas_;
+    int info = 0;
+    if (precision == 1) {
+        std::vector<float> r32(rhs, rhs + n);
+        hip_check(hipMemcpyAsync(d_rhsK32_.p, r32.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
+        check(rocsolver_spotrf(h, rocblas_fill_lower, n, d_M32_.p, n, d_info_.p) == rocblas_status_success, "spotrf failed");
+        check(rocsolver_spotrs(h, rocblas_fill_lower, n, 1, d_M32_.p, n, d_rhsK32_.p, n) == rocblas_status_success, "spotrs failed");
+        hip_check(hipMemcpyAsync(r32.data(), d_rhsK32_.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
+        hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info D2H");
+        hip_check(hipStreamSynchronize(stream_), "potrs sync");
+        for (int i = 0; i < n; ++i) rhs[i] = r32[i];
+    } else {
+        hip_check(hipMemcpyAsync(d_rhsK_.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
+        check(rocsolver_dpotrf(h, rocblas_fill_lower, n, d_M64_.p, n, d_info_.p) == rocblas_status_success, "dpotrf failed");
+        check(rocsolver_dpotrs(h, rocblas_fill_lower, n, 1, d_M64_.p, n, d_rhsK_.p, n) == rocblas_status_success, "dpotrs failed");
+        hip_check(hipMemcpyAsync(rhs, d_rhsK_.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
+        hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info D2H");
+        hip_check(hipStreamSynchronize(stream_), "potrs sync");
+    }
+    check(info == 0, "Dense Cholesky of the contact system failed (matrix not positive definite, info=" +
+                         std::to_string(info) + ").");
+}
+
+// x = A^-1 b with optional FP64 refinement (test entry point; Dirichlet rows are identity).
+void Body::apply_ainv(double dt, const double* b, double* x, int precision, int refine) {
+    setup(dt, 8);
+    d_b_.upload(b, dofs_);
+    hip_check(hipMemsetAsync(d_q_.p, 0, sizeof(double) * dofs_, stream_), "zero x");
+    hip_check(hipMemcpyAsync(d_r_.p, d_b_.p, sizeof(double) * dofs_, hipMemcpyDeviceToDevice, stream_), "r=b");
+    for (int it = 0; it <= refine; ++it) {
+        if (it > 0) launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, nullptr, d_r_.p);
+        if (precision == 1) {
+            spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
+            spmv_ST<float>(y32_, xpart32_, V32_.p, d_q_.p, nullptr, d_q_.p);
+        } else {
+            spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+            spmv_ST<double>(y64_, xpart64_, V64_.p, d_q_.p, nullptr, d_q_.p);
+        }
+    }
+    hip_check(hipStreamSynchronize(stream_), "apply_ainv");
+    d_q_.download(x, dofs_);
+}
+
+// ---------------------------------------------------------------------------
+// Type-II Anderson acceleration on device vectors (forward_gpu.cpp:1346-1379,
+// backward_gpu.cpp:822-851): raw (x_i, g_i) history in ring slots.
+// ---------------------------------------------------------------------------
+namespace {
+struct DevAnderson {
+    int window = 5;
+    std::deque<int> slots;      // indices into the slot arrays, oldest first
+    std::vector<int> free_slots;
+    DevBuf<double>* X = nullptr;  // 8 slots of x
+    DevBuf<double>* Gs = nullptr; // 8 slots of g
+    DevBuf<double> dQ[7], dG[7];
+    int n = 0;
+    void init(int w, int n_, DevBuf<double>* xs, DevBuf<double>* gs) {
+        window = std::min(w, 7);
+        n = n_;
+        X = xs;
+        Gs = gs;
+        slots.clear();
+        free_slots.clear();
+        for (int i = 7; i >= 0; --i) free_slots.push_back(i);
+        for (int i = 0; i < 8; ++i) { X[i].alloc(n); Gs[i].alloc(n); }
+    }
+    void clear() {
+        for (int s : slots) free_slots.push_back(s);
+        slots.clear();
+    }
+};
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Forward step.
+// ---------------------------------------------------------------------------
+void Body::forward(const double* q, const double* v, const double* f_ext, double dt,
+                   const std::map<std::string, double>& opt, double* q_next, double* v_next, std::vector<int>& active) {
+    auto tstart = std::chrono::steady_clock::now();
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    const int verbose = (int)opt_req(opt, "verbose");
+    const int threads = (int)opt_req(opt, "thread_ct");
+    const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    const double fmu = opt_or(opt, "friction_mu", 0.0);
+    const double restitution = opt_or(opt, "restitution", 0.0);
+    const int precision = (int)opt_or(opt, "phd_precision", 0.0);
+    check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
+    setup(dt, threads);
+    const int n3 = dofs_;
+    const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    std::vector<double> qt(n3);
+    const double mg[3] = {gravity_.size() == 3 ? mass * gravity_[0] : 0.0, gravity_.size() == 3 ? mass * gravity_[1] : 0.0,
+                          gravity_.size() == 3 ? mass * gravity_[2] : 0.0};
+    for (int i = 0; i < n3; ++i) qt[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * mg[i % 3];
+    // new-rollout detection for the tape (:437-449)
+    if (!tape_.empty()) {
+        const auto& last = tape_.back();
+        double dn = 0, a = 0, b = 0;
+        for (int i = 0; i < n3; ++i) { dn += (last.q_sol[i] - q[i]) * (last.q_sol[i] - q[i]); a += last.q_sol[i] * last.q_sol[i]; b += q[i] * q[i]; }
+        if (!(std::sqrt(dn) <= 1e-8 * std::max(1.0, std::max(std::sqrt(a), std::sqrt(b))))) tape_.clear();
+    }
+    d_qtilde_.upload(qt);
+    d_q_.upload(q, n3);
+    const int K = K_;
+    const DevBoundary bnd = dev_boundary();
+    std::vector<double> nrm(3 * K), t1(3 * K), t2(3 * K), rprec(K), lambda(K, 0.0), lt1(K, 0.0), lt2(K, 0.0), omega_last(K, 0.0);
+    std::vector<double> F(3 * K, 0.0);
+    const bool do_fric = K > 0 && fmu > 0;
+    if (K > 0) {
+        launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
+        hip_check(hipStreamSynchronize(stream_), "frames");
+        d_nrm_.download(nrm.data(), 3 * K);
+        d_t1_.download(t1.data(), 3 * K);
+        d_t2_.download(t2.data(), 3 * K);
+        for (int c = 0; c < K; ++c) rprec[c] = std::max(h * h * Hdiag_[c], 1e-10);
+        if (!prev_candidates_.empty() && prev_lambda_.size() == prev_candidates_.size()) {
+            std::unordered_map<int, int> pidx;
+            for (size_t i = 0; i < prev_candidates_.size(); ++i) pidx[prev_candidates_[i]] = (int)i;
+            for (int c = 0; c < K; ++c) {
+                auto it = pidx.find(cand_[c]);
+                if (it != pidx.end()) lambda[c] = prev_lambda_[it->second];
+            }
+        }
+        hip_check(hipMemsetAsync(d_F_.p, 0, sizeof(double) * 3 * K, stream_), "F=0");
+        if (precision == 1) d_M32_.alloc((size_t)K * K); else d_M64_.alloc((size_t)K * K);
+    }
+    DevAnderson aa;
+    aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
+    hip_check(hipMemsetAsync(d_pdrhs_prev_.p, 0, sizeof(double) * n3, stream_), "prev pd_rhs");
+    bool converged = false;
+    int iters = 0;
+    std::vector<double> delta(K), uC(3 * K), hv(3 * K), om(K), E(K), hvec(K), rhs(K), Fg(3 * K), Fn(3 * K), qC(3 * K);
+    std::vector<double> q_frame_C(3 * K);
+    for (int c = 0; c < K; ++c)
+        for (int d = 0; d < 3; ++d) q_frame_C[3 * c + d] = q[3 * cand_[c] + d];
+    for (int iter = 0; iter < max_pd_iter; ++iter) {
+        local_rhs(d_q_.p, inv_h2m);
+        hip_check(hipMemcpyAsync(d_qprev_.p, d_q_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "q_prev");
+        // residual of the global system with the previous contact impulses
+        launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
+        if (K > 0) launch_add_node_force(stream_, K, d_cand_.p, d_F_.p, d_r_.p);
+        if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
+        else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+        if (K > 0) {
+            // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev
+            if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
+            launch_h_apply(stream_, K, d_H_.p, d_F_.p, d_tmpK3_.p);
+            launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
+            launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
+            launch_axpbypcz(stream_, 3 * K, 1.0, d_uC_.p, 1.0, d_dF_.p, -1.0, d_tmpK3_.p, d_uC_.p);
+            hip_check(hipStreamSynchronize(stream_), "contact stage");
+            d_uC_.download(uC.data(), 3 * K);
+            d_delta_.download(delta.data(), K);
+            d_dF_.download(qC.data(), 3 * K);
+            // omega, E, h (forward_gpu.cpp:1160-1197)
+            for (int c = 0; c < K; ++c) {
+                const double dj = delta[c], lj = lambda[c], rj = rprec[c];
+                double den = std::sqrt(dj * dj + rj * rj * lj * lj);
+                if (den < 1e-14) den = 1e-14;
+                om[c] = 1.0 - dj / den;
+                E[c] = (1.0 - rj * lj / den) * rj;
+                const double phi = dj + rj * lj - den;
+                const double Hq = nrm[3 * c] * qC[3 * c] + nrm[3 * c + 1] * qC[3 * c + 1] + nrm[3 * c + 2] * qC[3 * c + 2];
+                hvec[c] = -phi + om[c] * Hq;
+            }
+            omega_last = om;
+            // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2
+            for (int c = 0; c < K; ++c)
+                for (int d = 0; d < 3; ++d)
+                    Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
+            d_tmpK3_.upload(Fg);
+            launch_h_apply(stream_, K, d_H_.p, d_tmpK3_.p, d_dF_.p);
+            hip_check(hipStreamSynchronize(stream_), "HFg");
+            d_dF_.download(hv.data(), 3 * K);
+            for (int c = 0; c < K; ++c) {
+                double val = 0;
+                for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (uC[3 * c + d] + hv[3 * c + d]);
+                rhs[c] = hvec[c] - om[c] * val;
+            }
+            d_om_.upload(om);
+            d_E_.upload(E);
+            if (precision == 1) launch_assemble_msys<float>(stream_, K, d_H_.p, d_nrm_.p, d_om_.p, d_E_.p, d_M32_.p);
+            else launch_assemble_msys<double>(stream_, K, d_H_.p, d_nrm_.p, d_om_.p, d_E_.p, d_M64_.p);
+            solve_dense_spd(K, precision, nullptr, rhs.data());
+            for (int c = 0; c < K; ++c) lambda[c] = std::min(std::max(lambda[c] + rhs[c], 0.0), 1e8);
+            // NCP-integrated Coulomb friction (:1255-1302)
+            if (do_fric) {
+                for (int c = 0; c < K; ++c)
+                    for (int d = 0; d < 3; ++d) Fn[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d];
+                d_tmpK3_.upload(Fn);
+                launch_h_apply(stream_, K, d_H_.p, d_tmpK3_.p, d_dF_.p);
+                hip_check(hipStreamSynchronize(stream_), "HFn");
+                d_dF_.download(hv.data(), 3 * K);
+                for (int c = 0; c < K; ++c) {
+                    if (lambda[c] <= 0.0) { lt1[c] = lt2[c] = 0.0; continue; }
+                    double step[3];
+                    for (int d = 0; d < 3; ++d) step[d] = uC[3 * c + d] + hv[3 * c + d] - q_frame_C[3 * c + d];
+                    const double* n = &nrm[3 * c];
+                    const double sn = step[0] * n[0] + step[1] * n[1] + step[2] * n[2];
+                    double ut[3];
+                    for (int d = 0; d < 3; ++d) ut[d] = step[d] - sn * n[d];
+                    const double u1 = ut[0] * t1[3 * c] + ut[1] * t1[3 * c + 1] + ut[2] * t1[3 * c + 2];
+                    const double u2 = ut[0] * t2[3 * c] + ut[1] * t2[3 * c + 1] + ut[2] * t2[3 * c + 2];
+                    // W_tt = (t_a . t_b) H_cc: diagonal for orthonormal tangents
+                    const double w11 = Hdiag_[c] * (t1[3 * c] * t1[3 * c] + t1[3 * c + 1] * t1[3 * c + 1] + t1[3 * c + 2] * t1[3 * c + 2]);
+                    const double w12 = Hdiag_[c] * (t1[3 * c] * t2[3 * c] + t1[3 * c + 1] * t2[3 * c + 1] + t1[3 * c + 2] * t2[3 * c + 2]);
+                    const double w22 = Hdiag_[c] * (t2[3 * c] * t2[3 * c] + t2[3 * c + 1] * t2[3 * c + 1] + t2[3 * c + 2] * t2[3 * c + 2]);
+                    const double det = w11 * w22 - w12 * w12;
+                    double s1 = 0, s2 = 0;
+                    if (std::fabs(det) > 1e-20) { s1 = (w22 * -u1 - w12 * -u2) / det; s2 = (-w12 * -u1 + w11 * -u2) / det; }
+                    const double cone = fmu * lambda[c];
+                    if (std::sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1[c] = s1; lt2[c] = s2; }
+                    else {
+                        const double um = std::sqrt(u1 * u1 + u2 * u2);
+                        if (um > 1e-14) { lt1[c] = -cone * u1 / um; lt2[c] = -cone * u2 / um; }
+                        else lt1[c] = lt2[c] = 0.0;
+                    }
+                }
+            }
+            // new impulses; the S pass carries only the change
+            std::vector<double> Fnew(3 * K), dF(3 * K);
+            for (int c = 0; c < K; ++c)
+                for (int d = 0; d < 3; ++d) {
+                    Fnew[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
+                    dF[3 * c + d] = Fnew[3 * c + d] - F[3 * c + d];
+                }
+            F = Fnew;
+            d_dF_.upload(dF);
+            d_F_.upload(F);
+            if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
+        }
+        if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+        else spmv_ST<double>(y64_, xpart64_, V64_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+        ++iters;
+        // Anderson acceleration
+        if (aa_window > 0) {
+            launch_axpbypcz(stream_, n3, 1.0, d_q_.p, -1.0, d_qprev_.p, 0.0, nullptr, d_g_.p);
+            int slot = aa.free_slots.back();
+            aa.free_slots.pop_back();
+            hip_check(hipMemcpyAsync(aa.X[slot].p, d_qprev_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
+            hip_check(hipMemcpyAsync(aa.Gs[slot].p, d_g_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
+            aa.slots.push_back(slot);
+            const int m = (int)aa.slots.size() - 1;
+            if (m >= 1) {
+                for (int i = 0; i < m; ++i) {
+                    aa.dQ[i].alloc(n3);
+                    aa.dG[i].alloc(n3);
+                    launch_axpbypcz(stream_, n3, 1.0, aa.X[aa.slots[i + 1]].p, -1.0, aa.X[aa.slots[i]].p, 0.0, nullptr, aa.dQ[i].p);
+                    launch_axpbypcz(stream_, n3, 1.0, aa.Gs[aa.slots[i + 1]].p, -1.0, aa.Gs[aa.slots[i]].p, 0.0, nullptr, aa.dG[i].p);
+                }
+                std::vector<std::pair<const double*, const double*>> pr;
+                for (int a = 0; a < m; ++a)
+                    for (int b = 0; b <= a; ++b) pr.push_back({aa.dG[a].p, aa.dG[b].p});
+                for (int a = 0; a < m; ++a) pr.push_back({aa.dG[a].p, d_g_.p});
+                const std::vector<double> dd = dots(pr);
+                std::vector<double> GtG(m * m), rr(m);
+                double sq = 0;
+                int idx = 0;
+                for (int a = 0; a < m; ++a)
+                    for (int b = 0; b <= a; ++b) { GtG[a * m + b] = GtG[b * m + a] = dd[idx++]; }
+                for (int a = 0; a < m; ++a) { rr[a] = dd[idx++]; sq += GtG[a * m + a]; }
+                const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+                for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
+                const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
+                if (nrm2(gamma) >= 10.0) aa.clear();
+                else {
+                    std::vector<const double*> pq(m), pg(m);
+                    for (int a = 0; a < m; ++a) { pq[a] = aa.dQ[a].p; pg[a] = aa.dG[a].p; }
+                    launch_aa_combine(stream_, n3, d_qprev_.p, d_g_.p, m, pq.data(), pg.data(), gamma.data(), d_q_.p);
+                    launch_set_dirichlet(stream_, (int)ddof_.size(), d_ddof_.p, d_dval_.p, d_q_.p);
+                }
+            }
+            while ((int)aa.slots.size() > aa.window) { aa.free_slots.push_back(aa.slots.front()); aa.slots.pop_front(); }
+        }
+        // dual gate (:1382-1397)
+        launch_axpbypcz(stream_, n3, 1.0, d_q_.p, -1.0, d_qprev_.p, 0.0, nullptr, d_g_.p);
+        launch_axpbypcz(stream_, n3, 1.0, d_pdrhs_.p, -1.0, d_pdrhs_prev_.p, 0.0, nullptr, d_tmp_.p);
+        const std::vector<double> gd = dots({{d_g_.p, d_g_.p}, {d_qprev_.p, d_qprev_.p}, {d_tmp_.p, d_tmp_.p},
+                                             {d_pdrhs_prev_.p, d_pdrhs_prev_.p}});
+        hip_check(hipMemcpyAsync(d_pdrhs_prev_.p, d_pdrhs_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "prev");
+        const double abs_err = std::sqrt(gd[0]), rhs_norm = std::sqrt(gd[1]), rc = std::sqrt(gd[2]), rr = std::sqrt(gd[3]);
+        if (verbose > 0) printf("alg_phd PD iter %d  dq=%g  dpd_rhs=%g\n", iter, abs_err, rc);
+        if (iter > 0 && abs_err <= rel_tol * rhs_norm + abs_tol && rc <= rel_tol * rr + abs_tol) { converged = true; break; }
+    }
+    hip_check(hipStreamSynchronize(stream_), "forward");
+    std::vector<double> qs(n3);
+    d_q_.download(qs.data(), n3);
+    if (!converged) fprintf(stderr, "[Warning] alg_phd: PD loop did not converge.\n");
+    // restitution post-pass (:1483-1531)
+    if (restitution > 0 && K > 0) {
+        const double slop = opt_or(opt, "friction_contact_slop", 1e-8);
+        for (int c = 0; c < K; ++c) {
+            if (lambda[c] <= 0) continue;
+            const int node = cand_[c];
+            double* x = &qs[3 * node];
+            double sd, nh[3];
+            // host evaluation of the obstacle at the converged position
+            if (boundary_.type == 1) { sd = x[0] * boundary_.n[0] + x[1] * boundary_.n[1] + x[2] * boundary_.n[2] + boundary_.off; for (int d = 0; d < 3; ++d) nh[d] = boundary_.n[d]; }
+            else if (boundary_.type == 2) {
+                double dd[3] = {x[0] - boundary_.c[0], x[1] - boundary_.c[1], x[2] - boundary_.c[2]};
+                const double rho = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+                sd = rho - boundary_.r;
+                for (int d = 0; d < 3; ++d) nh[d] = dd[d] / rho;
+            } else continue;
+            const double nn = std::sqrt(nh[0] * nh[0] + nh[1] * nh[1] + nh[2] * nh[2]);
+            if (nn <= 1e-12) continue;
+            for (int d = 0; d < 3; ++d) nh[d] /= nn;
+            if (sd > slop) continue;
+            const double vin = v[3 * node] * nh[0] + v[3 * node + 1] * nh[1] + v[3 * node + 2] * nh[2];
+            if (vin < 0) {
+                double vout = 0;
+                for (int d = 0; d < 3; ++d) vout += (x[d] - q[3 * node + d]) / h * nh[d];
+                const double dv = -restitution * vin - vout;
+                if (dv > 0) for (int d = 0; d < 3; ++d) x[d] += h * dv * nh[d];
+            }
+        }
+    }
+    for (int i = 0; i < n3; ++i) { q_next[i] = qs[i]; v_next[i] = (qs[i] - q[i]) / h; }
+    active.clear();
+    for (int c = 0; c < K; ++c) if (lambda[c] > 0) active.push_back(cand_[c]);
+    prev_candidates_ = cand_;
+    prev_lambda_ = lambda;
+    ReplayFrame fr;
+    fr.active = active;
+    fr.q_sol = qs;
+    std::unordered_map<int, int> cidx;
+    for (int c = 0; c < K; ++c) cidx[cand_[c]] = c;
+    for (int node : active) {
+        const int c = cidx[node];
+        fr.lambda.push_back(lambda[c]);
+        fr.omega.push_back(omega_last[c]);
+        fr.normals.push_back({nrm[3 * c], nrm[3 * c + 1], nrm[3 * c + 2]});
+    }
+    tape_.push_back(std::move(fr));
+    if (tape_.size() > 8192) tape_.erase(tape_.begin(), tape_.begin() + (tape_.size() - 8192));
+    StepStats st;
+    st.iters = iters;
+    st.converged = converged ? 1 : 0;
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
+    frame_stats.push_back(st);
+}
+
+// ---------------------------------------------------------------------------
+// Backward step.
+// ---------------------------------------------------------------------------
+void Body::backward(const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                    const double* v_next, const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
+                    const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                    double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem) {
+    (void)q; (void)v; (void)f_ext; (void)v_next;
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    const int threads = (int)opt_req(opt, "thread_ct");
+    const int precision = (int)opt_or(opt, "phd_precision", 0.0);
+    check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
+    setup(dt, threads);
+    const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
+    if (exact != kpd_exact_) { kpd_exact_ = exact; build_materials(); }
+    const int n3 = dofs_;
+    const double h = dt, inv_h = 1 / h, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    for (auto& kv : dirichlet_) check(q_next[kv.first] == kv.second, "Boundary conditions violated.");
+    std::vector<double> rhs_qbar(n3);
+    for (int i = 0; i < n3; ++i) {
+        dl_dq[i] = -dl_dv_next[i] * inv_h;
+        dl_dv[i] = 0;
+        dl_df_ext[i] = 0;
+        rhs_qbar[i] = dl_dq_next[i] + dl_dv_next[i] * inv_h;
+    }
+    const int nE = (int)energies_.size();
+    for (int i = 0; i < nE; ++i) dl_dmat_w[i] = 0;
+    if (gravity_.size() == 3) for (int d = 0; d < 3; ++d) dl_dstate_p[d] = 0;
+    if (dl_dw_elem) std::fill(dl_dw_elem, dl_dw_elem + (size_t)nE * ne_, 0.0);
+    // proximal-map Jacobians at q_next
+    d_q_.upload(q_next, n3);
+    if (precision == 1) { d_W32_.alloc((size_t)ne_ * samples_ * 81); d_W64_.release(); }
+    else { d_W64_.alloc((size_t)ne_ * samples_ * 81); d_W32_.release(); }
+    const double* kn[4] = {d_kn_[0].p, d_kn_[1].p, d_kn_[2].p, d_kn_[3].p};
+    if (precision == 1) launch_kpd_build<float>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W32_.p);
+    else launch_kpd_build<double>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W64_.p);
+    // tape
+    ReplayFrame replay;
+    bool from_tape = false;
+    if (!tape_.empty()) {
+        const auto& tail = tape_.back();
+        bool match = tail.active == active && (int)tail.q_sol.size() == n3;
+        if (match) {
+            double dn = 0, a = 0, b = 0;
+            for (int i = 0; i < n3; ++i) { dn += (tail.q_sol[i] - q_next[i]) * (tail.q_sol[i] - q_next[i]); a += tail.q_sol[i] * tail.q_sol[i]; b += q_next[i] * q_next[i]; }
+            match = std::sqrt(dn) <= 1e-8 * std::max(1.0, std::max(std::sqrt(a), std::sqrt(b)));
+        }
+        if (match) { replay = tail; tape_.pop_back(); from_tape = true; }
+    }
+    check(from_tape || active.empty(), "Backward needs the forward tape of this frame; the fixed-active-set replay is not supported.");
+    const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    const double clip = opt_or(opt, "alg_phd_bwd_clip_norm", 1e6);
+    DevBuf<double> d_rhs, d_s, d_sprev, d_zero;
+    d_rhs.upload(rhs_qbar);
+    d_s.alloc(n3);
+    d_sprev.alloc(n3);
+    d_zero.alloc(n3);
+    hip_check(hipMemsetAsync(d_zero.p, 0, sizeof(double) * n3, stream_), "zero");
+
+    // safe_clip (backward_gpu.cpp:809-816) on a device vector
+    auto safe_clip = [&](double* x) {
+        launch_nan_to_zero(stream_, n3, x);
+        const double nn = std::sqrt(dots({{x, x}})[0]);
+        if (clip > 0 && std::isfinite(nn) && nn > clip) launch_scale(stream_, n3, clip / nn, x);
+    };
+    // Constrained solve data (Woodbury basis); nb == 0 means the plain backbone.
+    int nb = 0;
+    DevBuf<double> d_G, d_dirs;
+    DevBuf<int> d_bcidx;
+    std::vector<int> b_c;            // candidate index of each basis row
+    std::vector<double> b_dir;       // 3 per basis row
+    // one fixed point  s <- A_c^-1 (rhs + K s)  in residual form with AA
+    auto fixed_point = [&](int max_it, int* used) {
+        hip_check(hipMemsetAsync(d_s.p, 0, sizeof(double) * n3, stream_), "s=0");
+        DevAnderson aa;
+        aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
+        int it_used = 0;
+        std::vector<double> uC(3 * K_), ub(nb), Fc(3 * K_);
+        for (int i = 0; i < max_it; ++i) {
+            hip_check(hipMemcpyAsync(d_sprev.p, d_s.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "sprev");
+            kpd_apply(d_s.p, d_kz_.p, precision);
+            launch_axpbypcz(stream_, n3, 1.0, d_rhs.p, 1.0, d_kz_.p, 0.0, nullptr, d_tmp_.p);
+            launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_tmp_.p, d_s.p, d_dmask_.p, d_r_.p);
+            if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
+            else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+            if (nb > 0) {
+                if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
+                hip_check(hipStreamSynchronize(stream_), "basis gather");
+                d_uC_.download(uC.data(), 3 * K_);
+                for (int a = 0; a < nb; ++a) {
+                    const int c = b_c[a];
+                    ub[a] = b_dir[3 * a] * uC[3 * c] + b_dir[3 * a + 1] * uC[3 * c + 1] + b_dir[3 * a + 2] * uC[3 * c + 2];
+                }
+                hip_check(hipMemcpy(d_rhsK_.p, ub.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "ub");
+                check(rocsolver_dpotrs((rocblas_handle)blas_, rocblas_fill_lower, nb, 1, d_G.p, nb, d_rhsK_.p, nb) ==
+                          rocblas_status_success, "dpotrs (basis) failed");
+                hip_check(hipStreamSynchronize(stream_), "basis solve");
+                hip_check(hipMemcpy(ub.data(), d_rhsK_.p, sizeof(double) * nb, hipMemcpyDeviceToHost), "ub D2H");
+                std::fill(Fc.begin(), Fc.end(), 0.0);
+                for (int a = 0; a < nb; ++a)
+                    for (int d = 0; d < 3; ++d) Fc[3 * b_c[a] + d] -= ub[a] * b_dir[3 * a + d];
+                d_dF_.upload(Fc);
+                if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
+            }
+            if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_sprev.p, d_zero.p, d_s.p);
+            else spmv_ST<double>(y64_, xpart64_, V64_.p, d_sprev.p, d_zero.p, d_s.p);
+            safe_clip(d_s.p);
+            launch_axpbypcz(stream_, n3, 1.0, d_s.p, -1.0, d_sprev.p, 0.0, nullptr, d_g_.p);
+            const double err = std::sqrt(dots({{d_g_.p, d_g_.p}})[0]);
+            if (aa_window > 0) {
+                int slot = aa.free_slots.back();
+                aa.free_slots.pop_back();
+                hip_check(hipMemcpyAsync(aa.X[slot].p, d_sprev.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
+                hip_check(hipMemcpyAsync(aa.Gs[slot].p, d_g_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
+                aa.slots.push_back(slot);
+                const int m = (int)aa.slots.size() - 1;
+                if (m >= 1) {
+                    for (int a = 0; a < m; ++a) {
+                        aa.dQ[a].alloc(n3);
+                        aa.dG[a].alloc(n3);
+                        launch_axpbypcz(stream_, n3, 1.0, aa.X[aa.slots[a + 1]].p, -1.0, aa.X[aa.slots[a]].p, 0.0, nullptr, aa.dQ[a].p);
+                        launch_axpbypcz(stream_, n3, 1.0, aa.Gs[aa.slots[a + 1]].p, -1.0, aa.Gs[aa.slots[a]].p, 0.0, nullptr, aa.dG[a].p);
+                    }
+                    std::vector<std::pair<const double*, const double*>> pr;
+                    for (int a = 0; a < m; ++a)
+                        for (int b = 0; b <= a; ++b) pr.push_back({aa.dG[a].p, aa.dG[b].p});
+                    for (int a = 0; a < m; ++a) pr.push_back({aa.dG[a].p, d_g_.p});
+                    const std::vector<double> dd = dots(pr);
+                    std::vector<double> GtG(m * m), rr(m);
+                    double sq = 0;
+                    int idx = 0;
+                    for (int a = 0; a < m; ++a)
+                        for (int b = 0; b <= a; ++b) GtG[a * m + b] = GtG[b * m + a] = dd[idx++];
+                    for (int a = 0; a < m; ++a) { rr[a] = dd[idx++]; sq += GtG[a * m + a]; }
+                    const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+                    for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
+                    const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
+                    if (nrm2(gamma) < 10.0) {
+                        std::vector<const double*> pq(m), pg(m);
+                        for (int a = 0; a < m; ++a) { pq[a] = aa.dQ[a].p; pg[a] = aa.dG[a].p; }
+                        launch_aa_combine(stream_, n3, d_sprev.p, d_g_.p, m, pq.data(), pg.data(), gamma.data(), d_s.p);
+                        safe_clip(d_s.p);
+                    } else {
+                        aa.clear();
+                    }
+                }
+                while ((int)aa.slots.size() > aa.window) { aa.free_slots.push_back(aa.slots.front()); aa.slots.pop_front(); }
+            }
+            it_used = i + 1;
+            const double sn = std::sqrt(dots({{d_s.p, d_s.p}})[0]);
+            if (i > 0 && err < rel_tol * sn + abs_tol) break;
+        }
+        if (used) *used = it_used;
+    };
+    int bb = 0;
+    fixed_point(max_pd_iter, &bb);
+    last_backward_iters = bb;
+    std::vector<double> db(n3);
+    hip_check(hipStreamSynchronize(stream_), "backbone");
+    d_s.download(db.data(), n3);
+    const bool diverged = clip > 0 && nrm2(db) >= clip * 0.999;
+    if (!diverged) {
+        std::vector<double> inj(n3);
+        for (int i = 0; i < n3; ++i) inj[i] = inv_h2m * db[i];
+        const double cap = nrm2(rhs_qbar) + 1.0, in = nrm2(inj);
+        if (in > cap) for (double& t : inj) t *= cap / in;
+        for (int i = 0; i < n3; ++i) { dl_dq[i] += inj[i]; dl_df_ext[i] += h2m * inj[i]; dl_dv[i] += h * inj[i]; }
+        if (gravity_.size() == 3) for (int i = 0; i < n3; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
+    }
+    std::vector<double> z = db;
+    const bool skip_wb = opt_or(opt, "skip_contact_woodbury", 0.0) > 0.5;
+    const int nl = (int)replay.active.size();
+    if (!diverged && !skip_wb && nl > 0) {
+        const double fmu = opt_or(opt, "friction_mu", 0.0);
+        const bool tb = fmu > 0;
+        std::unordered_map<int, int> cidx;
+        for (int c = 0; c < K_; ++c) cidx[cand_[c]] = c;
+        struct BC { int ci, basis; };
+        std::vector<BC> full;
+        const double tlmin = opt_or(opt, "alg_phd_bwd_tangent_lambda_min", 1e-8);
+        std::vector<std::array<double, 3>> T1(nl), T2(nl);
+        for (int c = 0; c < nl; ++c) {
+            if (!tb) break;
+            const auto& n = replay.normals[c];
+            int axis = 0;
+            for (int d = 1; d < 3; ++d) if (std::fabs(n[d]) < std::fabs(n[axis])) axis = d;
+            double e[3] = {0, 0, 0};
+            e[axis] = 1;
+            const double ne_ = n[0] * e[0] + n[1] * e[1] + n[2] * e[2];
+            double a1[3] = {e[0] - ne_ * n[0], e[1] - ne_ * n[1], e[2] - ne_ * n[2]};
+            const double an = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+            if (an > 1e-12) for (double& t : a1) t /= an; else { a1[0] = 1; a1[1] = 0; a1[2] = 0; }
+            double a2[3] = {n[1] * a1[2] - n[2] * a1[1], n[2] * a1[0] - n[0] * a1[2], n[0] * a1[1] - n[1] * a1[0]};
+            const double bn = std::sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+            if (bn > 1e-12) for (double& t : a2) t /= bn; else { a2[0] = 0; a2[1] = 1; a2[2] = 0; }
+            for (int d = 0; d < 3; ++d) { T1[c][d] = a1[d]; T2[c][d] = a2[d]; }
+        }
+        for (int c = 0; c < nl; ++c) {
+            if (replay.omega[c] < 0.5) continue;
+            full.push_back({c, 0});
+            if (tb && replay.lambda[c] > tlmin) { full.push_back({c, 1}); full.push_back({c, 2}); }
+        }
+        auto dir = [&](const BC& b, double* o) {
+            for (int d = 0; d < 3; ++d)
+                o[d] = b.basis == 0 ? replay.omega[b.ci] * replay.normals[b.ci][d] : (b.basis == 1 ? T1[b.ci][d] : T2[b.ci][d]);
+        };
+        const int nbf = (int)full.size();
+        if (nbf > 0) {
+            std::vector<double> rf(nbf);
+            double rmax = 0;
+            for (int i = 0; i < nbf; ++i) {
+                double dd[3];
+                dir(full[i], dd);
+                const int node = replay.active[full[i].ci];
+                rf[i] = dd[0] * z[3 * node] + dd[1] * z[3 * node + 1] + dd[2] * z[3 * node + 2];
+                rmax = std::max(rmax, std::fabs(rf[i]));
+            }
+            const double gate = std::max(opt_or(opt, "alg_phd_bwd_rhs_prune_abs", 1e-12), opt_or(opt, "alg_phd_bwd_rhs_prune_rel", 1e-3) * rmax);
+            std::vector<int> sel;
+            for (int i = 0; i < nbf; ++i) if (std::fabs(rf[i]) >= gate) sel.push_back(i);
+            const int topk = (int)opt_or(opt, "alg_phd_bwd_basis_topk", 0);
+            if (topk > 0 && (int)sel.size() > topk) {
+                std::nth_element(sel.begin(), sel.begin() + topk, sel.end(), [&](int a, int b) { return std::fabs(rf[a]) > std::fabs(rf[b]); });
+                sel.resize(topk);
+            }
+            nb = (int)sel.size();
+            if (nb > 0) {
+                b_c.resize(nb);
+                b_dir.resize(3 * nb);
+                for (int a = 0; a < nb; ++a) {
+                    const BC& b = full[sel[a]];
+                    b_c[a] = cidx.at(replay.active[b.ci]);
+                    dir(b, &b_dir[3 * a]);
+                }
+                d_bcidx.upload(b_c);
+                d_dirs.upload(b_dir);
+                d_G.alloc((size_t)nb * nb);
+                launch_assemble_g(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, d_G.p);
+                if ((int)d_rhsK_.n < nb) d_rhsK_.alloc(nb);
+                check(rocsolver_dpotrf((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) == rocblas_status_success,
+                      "dpotrf (basis) failed");
+                int info = 0;
+                hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                hip_check(hipStreamSynchronize(stream_), "basis factor");
+                check(info == 0, "Contact basis Gram matrix is not positive definite.");
+                int used = 0;
+                const int wt_cap = (int)opt_or(opt, "alg_phd_bwd_wt_max_iter", 120);
+                (void)wt_cap;
+                fixed_point(max_pd_iter, &used);
+                hip_check(hipStreamSynchronize(stream_), "woodbury");
+                d_s.download(z.data(), n3);
+            }
+        }
+    }
+    // dl/d(mat_w)
+    const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
+    if (!skip_mat && !diverged) {
+        d_s.upload(z);
+        d_gcor_.alloc(ne_); d_gvol_.alloc(ne_); d_gnh_.alloc((size_t)std::max(1, n_nh_) * ne_);
+        launch_dmatw(stream_, mat_, d_q_.p, d_s.p, element_volume_ / samples_, has_cor_ ? d_gcor_.p : nullptr,
+                     has_vol_ ? d_gvol_.p : nullptr, n_nh_ > 0 ? d_gnh_.p : nullptr);
+        hip_check(hipStreamSynchronize(stream_), "dmatw");
+        std::vector<double> gc(ne_, 0.0), gv(ne_, 0.0), gn((size_t)std::max(1, n_nh_) * ne_, 0.0);
+        if (has_cor_) d_gcor_.download(gc.data(), ne_);
+        if (has_vol_) d_gvol_.download(gv.data(), ne_);
+        if (n_nh_ > 0) d_gnh_.download(gn.data(), (size_t)n_nh_ * ne_);
+        std::vector<double> contrib(nE, 0.0);
+        bool ok = true;
+        int hslot = 0;
+        for (int ei = 0; ei < nE; ++ei) {
+            const Energy& en = energies_[ei];
+            const double* src = en.type == kCorotated ? gc.data() : en.type == kVolume ? gv.data() : &gn[(size_t)(hslot++) * ne_];
+            double s = 0;
+            for (int e = 0; e < ne_; ++e) if (en.member[e]) s += src[e];
+            if (!std::isfinite(s)) ok = false;
+            contrib[ei] = s;
+        }
+        if (ok) {
+            for (int ei = 0; ei < nE; ++ei) dl_dmat_w[ei] += contrib[ei];
+            if (dl_dw_elem) {
+                hslot = 0;
+                for (int ei = 0; ei < nE; ++ei) {
+                    const Energy& en = energies_[ei];
+                    const double* src = en.type == kCorotated ? gc.data() : en.type == kVolume ? gv.data() : &gn[(size_t)(hslot++) * ne_];
+                    for (int e = 0; e < ne_; ++e) if (en.member[e]) dl_dw_elem[(size_t)ei * ne_ + e] = src[e];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace phd

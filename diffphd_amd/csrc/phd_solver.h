@@ -1,0 +1,234 @@
+// Host orchestration of the MI355X PD time step: the product counterpart of
+// the reference's Deformable<3, 8/4> state (cpp/core/include/fem/deformable.h)
+// restricted to the `pd_eigen_alg_phd` path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "phd_factor.h"
+#include "phd_kernels.h"
+
+namespace phd {
+
+void hip_check(hipError_t e, const char* what);
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const std::vector<T>& h) {
+        alloc(h.size());
+        if (!h.empty()) hip_check(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+    void upload(const T* h, size_t count) {
+        alloc(count);
+        if (count) hip_check(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+    void download(T* h, size_t count) const {
+        if (count) hip_check(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost), "download");
+    }
+    std::vector<T> to_host() const {
+        std::vector<T> h(n);
+        download(h.data(), n);
+        return h;
+    }
+};
+
+enum EnergyKind { kCorotated = 0, kVolume = 1, kNeoHookean = 2 };
+
+struct Energy {
+    int type = kCorotated;
+    double stiffness = 0;             // mean stiffness (stiffness_())
+    std::vector<double> elem_stiff;   // per-element stiffness (empty: homogeneous)
+    double mu = 0, la = 0;            // Neo-Hookean mean Lame parameters
+    std::vector<char> member;
+    double elem_w(int e) const { return elem_stiff.empty() ? stiffness : elem_stiff[e]; }
+};
+
+struct HostBoundary {
+    int type = 0;  // 0 none, 1 planar, 2 spherical, 3 heightfield, 4 press
+    double n[3] = {0, 0, 1}, off = 0, c[3] = {0, 0, 0}, r = 0, z0 = 0, plane_z = 0;
+    std::vector<double> waves;
+};
+
+struct ReplayFrame {  // AlgPhdReplayCache fields used by the backward (deformable.h:336-355)
+    std::vector<int> active;
+    std::vector<double> q_sol;
+    std::vector<double> lambda, omega;
+    std::vector<std::array<double, 3>> normals;
+};
+
+struct StepStats {
+    int iters = 0;
+    int converged = 0;
+    double seconds = 0;
+};
+
+class Body {
+public:
+    explicit Body(int device);
+    ~Body();
+
+    // ---- scene (Deformable mutators) ----
+    void init_mesh(int edim, int nv, const double* X, int ne, const int* elems, double density);
+    void add_pd_energy(const std::string& type, const std::vector<double>& params, const std::vector<int>& idx);
+    void set_dirichlet(int dof, double val);
+    void add_gravity(const double* g);
+    void set_boundary(const std::string& type, const std::vector<double>& p, const std::vector<int>& idx);
+    void set_press_plane(double z) { boundary_.plane_z = z; }
+
+    int dofs() const { return dofs_; }
+    int num_elements() const { return ne_; }
+    int num_energies() const { return (int)energies_.size(); }
+
+    // ---- hot path ----
+    void forward(const double* q, const double* v, const double* f_ext, double dt,
+                 const std::map<std::string, double>& opt, double* q_next, double* v_next, std::vector<int>& active);
+    void backward(const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                  const double* v_next, const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
+                  const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                  double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem);
+    // x = A^-1 b (AoS 3N) with the FP32 (precision=1) or FP64 (0) factor and `refine` FP64 refinement steps.
+    void apply_ainv(double dt, const double* b, double* x, int precision, int refine);
+    void setup(double dt, int threads);
+
+    // ---- statistics ----
+    std::vector<StepStats> frame_stats;
+    int last_backward_iters = 0;
+    long long factor_bytes() const;
+    long long factor_nnz() const { return factor_.nnz; }
+    int cached_n() const { return ready_ ? nv_ : -1; }
+    double setup_seconds = 0;
+    // device-event timing of the S and S^T kernels (milliseconds, accumulated)
+    double t_spmv_rows_ms = 0, t_spmv_cols_ms = 0;
+    long long n_spmv_rows = 0, n_spmv_cols = 0;
+    bool timing = false;
+    hipStream_t stream() const { return stream_; }
+    void collect_timing();
+    const Factor& factor() const { return factor_; }
+
+private:
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    // scene
+    int edim_ = 8, nv_ = 0, ne_ = 0, dofs_ = 0, samples_ = 8;
+    std::vector<double> X_;
+    std::vector<int> elems_;
+    double density_ = 0, element_volume_ = 0;
+    std::vector<double> G_;
+    int g_stride_ = 0;
+    std::vector<Energy> energies_;
+    std::map<int, double> dirichlet_;
+    std::vector<double> gravity_;
+    HostBoundary boundary_;
+    std::map<int, int> fb_vertices_;
+
+    // solver cache
+    bool ready_ = false;
+    double ready_dt_ = 0;
+    Factor factor_;
+    SpmvPlan plan_;
+    std::vector<int> Ap_, Ai_;
+    std::vector<double> Ax_;
+    std::vector<unsigned char> dmask_;
+    std::vector<int> ddof_;
+    std::vector<double> dval_, dval_full_;
+    std::vector<double> c_dir_;
+    // device
+    DevBuf<double> V64_;
+    DevBuf<float> V32_;
+    DevBuf<int64_t> d_base_;
+    DevBuf<int> d_fd_, d_perm_, d_ru_row_, d_ru_c0_, d_ru_c1_, d_row_unit_ptr_, d_items_, d_cu_tile_, d_cu_i0_, d_cu_i1_,
+        d_tile_unit_ptr_;
+    DevBuf<int> d_Ap_, d_Ai_;
+    DevBuf<double> d_Ax_;
+    DevBuf<unsigned char> d_dmask_;
+    DevBuf<int> d_ddof_;
+    DevBuf<double> d_dval_, d_dval_full_;
+    DevBuf<double> d_c_dir_;
+    DevBuf<int> d_elems_, d_inc_ptr_, d_inc_idx_;
+    DevBuf<double> d_G_, d_wc_, d_wv_, d_wn_[4];
+    DevBuf<double> d_kc_, d_kv_, d_kn_[4];
+    DevMaterials mat_;
+    int n_nh_ = 0;
+    std::vector<int> nh_energy_;  // energy index of each NH slot
+    bool has_cor_ = false, has_vol_ = false;
+    bool kpd_exact_ = false;
+    // work vectors (3N FP64)
+    DevBuf<double> d_q_, d_qprev_, d_qtilde_, d_b_, d_r_, d_pdrhs_, d_pdrhs_prev_, d_elem_, d_g_, d_tmp_, d_kz_;
+    DevBuf<double> d_hist_q_[8], d_hist_g_[8];
+    DevBuf<double> d_dot_partial_, d_dot_out_;
+    DevBuf<float> w32_, y32_, part32_, xpart32_;
+    DevBuf<double> w64_, y64_, part64_, xpart64_;
+    // contact
+    int K_ = 0;
+    std::vector<int> cand_;
+    DevBuf<int> d_cand_;
+    DevBuf<double> d_H_;        // A^-1 restricted to the candidates (K x K, FP64)
+    std::vector<double> Hdiag_;
+    DevBuf<int> d_cp_ptr_, d_cp_row_, d_rc_ptr_, d_rc_row_, d_rc_c_;
+    DevBuf<double> d_cp_val64_, d_rc_val64_;
+    DevBuf<float> d_cp_val32_, d_rc_val32_;
+    int rc_nrows_ = 0;
+    DevBuf<double> d_waves_;
+    DevBuf<double> d_nrm_, d_t1_, d_t2_, d_delta_, d_F_, d_dF_, d_uC_, d_tmpK3_, d_om_, d_E_;
+    DevBuf<double> d_M64_;
+    DevBuf<float> d_M32_;
+    DevBuf<double> d_rhsK_;
+    DevBuf<float> d_rhsK32_;
+    DevBuf<int> d_info_;
+    void* blas_ = nullptr;
+    std::vector<hipEvent_t> ev_pool_;
+    size_t ev_used_ = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_rows_, ev_cols_;
+    hipEvent_t next_event();
+    // backward
+    DevBuf<double> d_W64_;
+    DevBuf<float> d_W32_;
+    DevBuf<double> d_gcor_, d_gvol_, d_gnh_;
+    // state kept between calls (mutable members of the reference Deformable)
+    std::vector<ReplayFrame> tape_;
+    std::vector<int> prev_candidates_;
+    std::vector<double> prev_lambda_;
+
+    void build_materials();
+    void setup_contact(int threads);
+    DevBoundary dev_boundary();
+    // one global solve pass: r (AoS 3N FP64) -> y (S r, factor precision); returns nothing
+    template <class TV>
+    void spmv_S(const double* r, DevBuf<TV>& w, DevBuf<TV>& y, DevBuf<TV>& part, const TV* V);
+    template <class TV>
+    void spmv_ST(DevBuf<TV>& y, DevBuf<TV>& xpart, const TV* V, const double* add, const double* dval, double* out);
+    template <class TV>
+    void contact_gather(DevBuf<TV>& y, double* out);
+    template <class TV>
+    void contact_scatter(const double* f, DevBuf<TV>& y);
+    void local_rhs(const double* q, double inv_h2m);
+    void kpd_apply(const double* x, double* out, int precision);
+    std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
+    void solve_dense_spd(int n, int precision, const double* M64_host_or_null, double* rhs_inout);
+};
+
+}  // namespace phd

@@ -1,0 +1,308 @@
+"""Python module with the names of the reference's SWIG module `py_diff_pd_core`
+(cpp/core/src/py_diff_pd_core.i:1-83) for the `pd_eigen_alg_phd` path.
+
+`HexDeformable` / `TetDeformable` keep the reference's method names and
+argument meaning (cpp/core/include/fem/deformable.h:23-141): outputs are
+caller-allocated `StdRealVector`s that the call resizes and fills
+(deformable.cpp:157-187), errors are raised as RuntimeError. The work is done
+by libphd.so through the C ABI of include/phd.h.
+"""
+import numpy as np
+
+from . import _lib
+from .hex_mesh import write_bin
+
+METHODS = ("pd_eigen_alg_phd", "pd_eigen_pcg_proj_gpu")
+
+
+class StdRealVector(list):
+    """std::vector<real> stand-in: a list the callee fills in place."""
+
+    def __init__(self, n=0):
+        super().__init__([0.0] * int(n))
+
+
+class StdIntVector(list):
+    def __init__(self, n=0):
+        super().__init__([0] * int(n))
+
+
+StdMap = dict
+
+
+def _read_bin(path):
+    import struct
+    with open(path, "rb") as f:
+        vdim, edim = struct.unpack("ii", f.read(8))
+        r, c = struct.unpack("ii", f.read(8))
+        V = np.frombuffer(f.read(8 * r * c), dtype=np.float64).reshape(c, r).T
+        r2, c2 = struct.unpack("ii", f.read(8))
+        E = np.frombuffer(f.read(4 * r2 * c2), dtype=np.int32).reshape(c2, r2).T
+    return vdim, edim, np.array(V), np.array(E)
+
+
+class _Mesh:
+    """Mesh<3, edim> (cpp/core/src/mesh/mesh.cpp)."""
+    _edim = 8
+
+    def __init__(self):
+        self._V = np.zeros((3, 0))
+        self._E = np.zeros((self._edim, 0), dtype=np.int32)
+
+    def Initialize(self, bin_file_name):
+        vdim, edim, V, E = _read_bin(bin_file_name)
+        if vdim != 3 or edim != self._edim:
+            raise RuntimeError("Corrupted mesh file: " + str(bin_file_name))
+        self._V, self._E = V, E
+
+    def InitializeFromArrays(self, V, E):
+        self._V, self._E = np.asarray(V, dtype=np.float64), np.asarray(E, dtype=np.int32)
+
+    def NumOfVertices(self):
+        return self._V.shape[1]
+
+    def NumOfElements(self):
+        return self._E.shape[1]
+
+    def py_vertex(self, i):
+        return list(self._V[:, i])
+
+    def py_vertices(self):
+        return list(self._V.T.ravel())
+
+    def py_element(self, i):
+        return list(int(x) for x in self._E[:, i])
+
+    def SaveToFile(self, path):
+        write_bin(path, self._V, self._E)
+
+
+class HexMesh3d(_Mesh):
+    _edim = 8
+
+
+class TetMesh3d(_Mesh):
+    _edim = 4
+
+
+_last_ctx = [None]
+
+
+class _Deformable:
+    _edim = 8
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        self._ctx = L.phd_ctx_create(int(device))
+        if not self._ctx:
+            raise RuntimeError("phd_ctx_create failed: " + L.phd_last_error(None).decode())
+        self._dofs = 0
+        self._ne = 0
+        self._gravity = False
+        _last_ctx[0] = self
+
+    def __del__(self):
+        try:
+            if self._ctx:
+                _lib.lib().phd_ctx_destroy(self._ctx)
+                self._ctx = None
+        except Exception:
+            pass
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise RuntimeError(_lib.lib().phd_last_error(self._ctx).decode())
+
+    # --- Deformable::Initialize (deformable.cpp:17-43) -----------------------
+    def Initialize(self, binary_file_name, density, material_type, youngs_modulus, poissons_ratio):
+        _, edim, V, E = _read_bin(binary_file_name)
+        if edim != self._edim:
+            raise RuntimeError("Corrupted mesh file: " + str(binary_file_name))
+        self._init_arrays(V, E, density, material_type)
+
+    def InitializeFromArrays(self, vertices, elements, density, material_type="none", youngs_modulus=0.0,
+                             poissons_ratio=0.0):
+        self._init_arrays(np.asarray(vertices), np.asarray(elements), density, material_type)
+
+    def _init_arrays(self, V, E, density, material_type):
+        if material_type != "none":
+            raise RuntimeError("Only material type 'none' is supported on the pd_eigen_alg_phd path.")
+        V = np.asarray(V, dtype=np.float64)
+        E = np.asarray(E, dtype=np.int32)
+        Vf, vp = _lib.dptr(V.T.ravel())
+        Ef, ep = _lib.iptr(E.T.ravel())
+        self._ok(_lib.lib().phd_init_mesh(self._ctx, self._edim, V.shape[1], vp, E.shape[1], ep, float(density)))
+        self._dofs = 3 * V.shape[1]
+        self._ne = E.shape[1]
+        self._V, self._E = V, E
+
+    def AddPdEnergy(self, energy_type, params, indices):
+        p, pp = _lib.dptr(list(params) or [0.0])
+        ix, ip = _lib.iptr(list(indices) or [0])
+        self._ok(_lib.lib().phd_add_pd_energy(self._ctx, energy_type.encode(), pp, len(params), ip, len(indices)))
+
+    def SetDirichletBoundaryCondition(self, dof, val):
+        self._ok(_lib.lib().phd_set_dirichlet(self._ctx, int(dof), float(val)))
+
+    def AddStateForce(self, force_type, params):
+        p, pp = _lib.dptr(list(params))
+        self._ok(_lib.lib().phd_add_state_force(self._ctx, force_type.encode(), pp, len(params)))
+        self._gravity = True
+
+    def SetFrictionalBoundary(self, boundary_type, params, indices):
+        p, pp = _lib.dptr(list(params))
+        ix, ip = _lib.iptr(list(indices) or [0])
+        self._ok(_lib.lib().phd_set_frictional_boundary(self._ctx, boundary_type.encode(), pp, len(params), ip,
+                                                        len(indices)))
+
+    def SetPressPlaneHeight(self, z):
+        self._ok(_lib.lib().phd_set_press_plane(self._ctx, float(z)))
+
+    def dofs(self):
+        return self._dofs
+
+    def act_dofs(self):
+        return 0
+
+    def NumOfPdElementEnergies(self):
+        return _lib.lib().phd_num_pd_element_energies(self._ctx)
+
+    def NumOfPdMuscleEnergies(self):
+        return 0
+
+    def NumOfStateForceParameters(self):
+        return 3 if self._gravity else 0
+
+    def mesh(self):
+        m = HexMesh3d() if self._edim == 8 else TetMesh3d()
+        m.InitializeFromArrays(self._V, self._E)
+        return m
+
+    # --- PyForward / PyBackward (deformable.cpp:157-187) ---------------------
+    def PyForward(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
+        if method not in METHODS:
+            raise RuntimeError("Unsupported forward method: " + str(method))
+        n = self._dofs
+        qa, qp = _lib.dptr(q)
+        va, vp = _lib.dptr(v)
+        fa, fp = _lib.dptr(f_ext)
+        if qa.size != n or va.size != n or fa.size != n:
+            raise RuntimeError("Inconsistent state size.")
+        qn, qnp = _lib.dptr(np.zeros(n))
+        vn, vnp = _lib.dptr(np.zeros(n))
+        cap = n // 3 + 1
+        act, ap = _lib.iptr(np.zeros(cap))
+        import ctypes
+        na = ctypes.c_int(cap)
+        karr, vals, valp, nk = _lib.options(options)
+        self._ok(_lib.lib().phd_step_forward(self._ctx, qp, vp, fp, float(dt), karr, valp, nk, qnp, vnp, ap,
+                                             ctypes.byref(na)))
+        q_next[:] = qn.tolist()
+        v_next[:] = vn.tolist()
+        active_contact_idx[:] = [int(x) for x in act[: na.value]]
+        _last_ctx[0] = self
+
+    def PyBackward(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
+                   options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p, dl_dw_elem=None):
+        if method not in METHODS:
+            raise RuntimeError("Unsupported backward method: " + str(method))
+        n = self._dofs
+        ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next)]
+        act, ap = _lib.iptr(list(active_contact_idx) or [0])
+        nE = self.NumOfPdElementEnergies()
+        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), 3, max(nE * self._ne, 1))]
+        karr, vals, valp, nk = _lib.options(options)
+        self._ok(_lib.lib().phd_step_backward(self._ctx, ins[0][1], ins[1][1], ins[2][1], float(dt), ins[3][1],
+                                              ins[4][1], ap, len(active_contact_idx), ins[5][1], ins[6][1], karr,
+                                              valp, nk, *[o[1] for o in outs]))
+        dl_dq[:] = outs[0][0].tolist()
+        dl_dv[:] = outs[1][0].tolist()
+        dl_da[:] = []
+        dl_df_ext[:] = outs[2][0].tolist()
+        dl_dmat_w[:] = outs[3][0][:nE].tolist()
+        dl_dact_w[:] = []
+        dl_dstate_p[:] = outs[4][0][: self.NumOfStateForceParameters()].tolist()
+        if dl_dw_elem is not None:
+            dl_dw_elem[:] = outs[5][0][: nE * self._ne].tolist()
+        _last_ctx[0] = self
+
+    # --- extensions used by tests and the benchmark --------------------------
+    def factor_build(self, dt, threads=8):
+        self._ok(_lib.lib().phd_factor_build(self._ctx, float(dt), int(threads)))
+
+    def apply_ainv(self, dt, b, precision=0, refine=0):
+        ba, bp = _lib.dptr(b)
+        x, xp = _lib.dptr(np.zeros_like(ba))
+        self._ok(_lib.lib().phd_apply_ainv(self._ctx, float(dt), bp, xp, int(precision), int(refine)))
+        return x
+
+    def factor_info(self):
+        out, op = _lib.dptr(np.zeros(10))
+        self._ok(_lib.lib().phd_factor_info(self._ctx, op, 10))
+        keys = ["nnz_S", "v_len", "max_depth", "order_s", "factor_s", "setup_s", "row_units", "col_units", "nnz_L", "K"]
+        return dict(zip(keys, out.tolist()))
+
+    def frame_stats(self):
+        L = _lib.lib()
+        n = L.phd_frame_stats(self._ctx, None, None, None, 0)
+        it, ip = _lib.iptr(np.zeros(max(n, 1)))
+        cv, cp = _lib.iptr(np.zeros(max(n, 1)))
+        sc, sp = _lib.dptr(np.zeros(max(n, 1)))
+        L.phd_frame_stats(self._ctx, ip, cp, sp, n)
+        return it[:n].tolist(), cv[:n].tolist(), sc[:n].tolist()
+
+    def reset_frame_stats(self):
+        _lib.lib().phd_reset_frame_stats(self._ctx)
+
+    def last_backward_iters(self):
+        return _lib.lib().phd_last_backward_iters(self._ctx)
+
+    def timing_enable(self, on=True):
+        self._ok(_lib.lib().phd_timing_enable(self._ctx, int(on)))
+
+    def timing_read(self, reset=True):
+        out, op = _lib.dptr(np.zeros(4))
+        self._ok(_lib.lib().phd_timing_read(self._ctx, op, int(reset)))
+        return {"rows_ms": out[0], "rows_launches": int(out[1]), "cols_ms": out[2], "cols_launches": int(out[3])}
+
+
+class HexDeformable(_Deformable):
+    """Deformable<3, 8>."""
+    _edim = 8
+
+
+class TetDeformable(_Deformable):
+    """Deformable<3, 4>."""
+    _edim = 4
+
+
+# --- alg_phd_gpu_memory.h:9-27 (answers for the most recently used body) -------
+def AlgPhdSMemoryBytes():
+    d = _last_ctx[0]
+    return int(_lib.lib().phd_s_memory_bytes(d._ctx)) if d is not None else 0
+
+
+def AlgPhdSNnz():
+    d = _last_ctx[0]
+    return int(_lib.lib().phd_s_nnz(d._ctx)) if d is not None else 0
+
+
+def AlgPhdCachedN():
+    d = _last_ctx[0]
+    return int(_lib.lib().phd_cached_n(d._ctx)) if d is not None else -1
+
+
+def AlgPhdResetFrameStats():
+    d = _last_ctx[0]
+    if d is not None:
+        d.reset_frame_stats()
+
+
+def AlgPhdFrameIters():
+    d = _last_ctx[0]
+    return d.frame_stats()[0] if d is not None else []
+
+
+def AlgPhdFrameConverged():
+    d = _last_ctx[0]
+    return d.frame_stats()[1] if d is not None else []

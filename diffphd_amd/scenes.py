@@ -1,0 +1,187 @@
+"""Synthetic scenes of the PD benchmark configurations.
+
+Every builder returns a plain dict `spec` (mesh arrays, energies, boundary
+conditions, initial state, options); `apply(spec, body)` replays it on any
+object with the reference Deformable's method names (the product's
+HexDeformable/TetDeformable or a test oracle). Seeds are fixed per scene.
+"""
+import numpy as np
+
+from .hex_mesh import hex_grid, tet_grid
+
+
+def lame(E, nu):
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def default_options(**kw):
+    opt = {"max_pd_iter": 1000, "abs_tol": 1e-12, "rel_tol": 1e-9, "verbose": 0, "thread_ct": 8}
+    opt.update(kw)
+    return opt
+
+
+def apply(spec, body):
+    body.InitializeFromArrays(spec["V"], spec["E"], spec["density"])
+    if spec.get("gravity") is not None:
+        body.AddStateForce("gravity", list(spec["gravity"]))
+    for kind, params, idx in spec["energies"]:
+        body.AddPdEnergy(kind, list(params), list(idx))
+    for dof, val in spec.get("dirichlet", []):
+        body.SetDirichletBoundaryCondition(int(dof), float(val))
+    b = spec.get("boundary")
+    if b is not None:
+        body.SetFrictionalBoundary(b[0], list(b[1]), list(b[2]))
+    return body
+
+
+def cantilever(cells=(16, 4, 4), dx=0.01, E=1e5, nu=0.45, rho=1000.0, per_element_energies=False, seed=0):
+    """Cantilever beam (16x4x4 hex, x=0 face fixed, gravity): plain FP64 PD, no
+    contact. Loss = dot(q_T, g), g ~ N(0, 1) with the given seed.
+
+    per_element_energies: one 'corotated' energy per element (the reference's
+    layout for per-element mu gradients) instead of one per-element-parameter energy."""
+    V, Eidx, _, _ = hex_grid(np.ones(cells), dx, (0.0, 0.0, 0.0))
+    ne = Eidx.shape[1]
+    mu, la = lame(E, nu)
+    if per_element_energies:
+        energies = [("corotated", [2 * mu], [e]) for e in range(ne)] + [("volume", [la], [])]
+    else:
+        energies = [("corotated", [2 * mu], []), ("volume", [la], [])]
+    dirichlet = []
+    for i in range(V.shape[1]):
+        if V[0, i] < 1e-12:
+            dirichlet += [(3 * i + k, V[k, i]) for k in range(3)]
+    q0 = V.T.ravel().copy()
+    rng = np.random.default_rng(seed)
+    return {
+        "name": "cantilever", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "energies": energies, "dirichlet": dirichlet, "boundary": None, "q0": q0, "v0": np.zeros_like(q0),
+        "dt": 1e-2, "frames": 100, "loss_g": rng.standard_normal(q0.size), "mu": mu, "la": la,
+        "options": default_options(max_pd_iter=5000),
+    }
+
+
+def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):
+    """Heterogeneous napkin dropped on a sphere (python/example/napkin_3d_test.py:32-45,
+    python/py_diff_pd/env/napkin_env_3d.py:19-79): stiff half / 0.1x soft half
+    by element index, spherical frictional boundary under the bottom nodes
+    within `contact_ratio * 0.2` of the axis, q0 jittered with the legacy
+    numpy RandomState seeded with `seed`."""
+    node_nums = (cells[0] + 1, cells[1] + 1, cells[2] + 1)
+    dx = 0.4 / cells[0]
+    origin = (-0.2, -0.2, 0.52)
+    np.random.seed(seed)
+    V, Eidx, _, _ = hex_grid(np.ones(cells), dx, origin)
+    mu, la = lame(E, nu)
+    n = cells[0] * cells[1] * cells[2]
+    stiff, soft = list(range(n // 2)), list(range(n // 2, n))
+    energies = [("corotated", [2 * mu], stiff), ("volume", [la], stiff),
+                ("corotated", [2 * mu * 0.1], soft), ("volume", [la * 0.1], soft)]
+    radius = 0.2 * contact_ratio
+    fidx = []
+    for i in range(node_nums[0]):
+        for j in range(node_nums[1]):
+            k = i * node_nums[1] * node_nums[2] + j * node_nums[2]
+            px, py, _ = V[:, k]
+            if px ** 2 + py ** 2 <= radius ** 2:
+                fidx.append(k)
+    dofs = 3 * V.shape[1]
+    q0 = V.T.ravel() + np.random.uniform(low=-2 * dx * 0.01, high=2 * dx * 0.01, size=dofs)
+    return {
+        "name": "napkin", "edim": 8, "V": V, "E": Eidx, "density": 1e3, "gravity": (0.0, 0.0, -9.81),
+        "energies": energies, "dirichlet": [], "boundary": ("spherical", [0.0, 0.0, 0.0, 0.5], fidx),
+        "q0": q0, "v0": np.zeros(dofs), "dt": 2e-3, "frames": 125,
+        "options": {"max_pd_iter": 5000, "max_ls_iter": 10, "abs_tol": 1e-9, "rel_tol": 1e-4, "verbose": 0,
+                    "thread_ct": 8, "use_bfgs": 1, "use_acc": 1, "use_sparse": 0, "project_newton": 0, "use_abs": 0},
+    }
+
+
+def _smoothed_field(shape, rng, sigma_cells):
+    """Gaussian random field smoothed by a separable Gaussian (FFT), zero mean."""
+    white = rng.standard_normal(shape)
+    f = np.fft.fftn(white)
+    grids = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
+    k2 = sum(g ** 2 for g in grids)
+    field = np.real(np.fft.ifftn(f * np.exp(-2 * (np.pi ** 2) * (sigma_cells ** 2) * k2)))
+    return field - field.mean()
+
+
+def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0, fmu=0.3, frames=100, seed=0):
+    """Contact-rich press: hex block, corotated + volume with per-element E from
+    a seeded smoothed Gaussian random field in log space spanning `decades`
+    decades, pressed by a rigid plane lowered by 20% of the block height over
+    the rollout into a bumpy heightfield ground (sum of 16 seeded sinusoids,
+    total amplitude 0.05 of the block size). Loss = mean squared distance of
+    the final vertex positions to a seeded target."""
+    rng = np.random.default_rng(seed)
+    V, Eidx, _, _ = hex_grid(np.ones(cells), dx, (0.0, 0.0, 0.0))
+    ne = Eidx.shape[1]
+    size = max(cells) * dx
+    height = cells[2] * dx
+    # ground heightfield
+    nw = 16
+    amp = 0.05 * size
+    u = rng.uniform(0.5, 1.0, nw)
+    a = amp * u / u.sum()
+    kx = 2 * np.pi * rng.integers(1, 5, nw) / (cells[0] * dx) * rng.choice([-1, 1], nw)
+    ky = 2 * np.pi * rng.integers(1, 5, nw) / (cells[1] * dx) * rng.choice([-1, 1], nw)
+    ph = rng.uniform(0, 2 * np.pi, nw)
+    waves = np.stack([a, kx, ky, ph], 1).ravel()
+    # place the block on the highest bump of its footprint
+    xs = np.unique(V[0]); ys = np.unique(V[1])
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    H = np.zeros_like(X)
+    for i in range(nw):
+        H += a[i] * np.sin(kx[i] * X + ky[i] * Y + ph[i])
+    z0 = -H.max()
+    plane0 = height
+    # per-element E: log-space smoothed GRF spanning `decades`
+    field = _smoothed_field(cells, rng, sigma_cells=3.0).ravel()
+    field = (field - field.min()) / max(field.max() - field.min(), 1e-30)
+    Ee = E0 * 10.0 ** (decades * field)
+    mu_e, la_e = lame(Ee, nu)
+    energies = [("corotated", list(2 * mu_e), []), ("volume", list(la_e), [])]
+    bottom = np.flatnonzero(np.abs(V[2]) < 1e-12)
+    top = np.flatnonzero(np.abs(V[2] - height) < 1e-12)
+    fidx = sorted(list(bottom) + list(top))
+    params = [z0, nw] + list(waves) + [plane0]
+    q0 = V.T.ravel().copy()
+    target = q0 + 0.01 * rng.standard_normal(q0.size)
+    return {
+        "name": "press", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "energies": energies, "dirichlet": [], "boundary": ("press", params, fidx),
+        "q0": q0, "v0": np.zeros_like(q0), "dt": 1e-2, "frames": frames, "target": target,
+        "plane_schedule": [plane0 - 0.2 * height * (f + 1) / frames for f in range(frames)],
+        "options": default_options(friction_mu=fmu, max_pd_iter=500, rel_tol=1e-9),
+        "E_elem": Ee,
+    }
+
+
+def tet_block(n=40, dx=None, seed=0, nu=0.4, inclusions=8, fmu=0.5):
+    """Heterogeneous Neo-Hookean tet block on a ground plane: n^3 cubes split
+    into 5 tets, two-phase per-element mu (seeded spherical inclusions at 100x
+    contrast, log-uniform within each phase over [1e4, 1e6] Pa), lambda = 4 mu."""
+    rng = np.random.default_rng(seed)
+    dx = dx if dx is not None else 0.4 / n
+    V, Eidx = tet_grid(n, dx, (0.0, 0.0, 0.02))
+    ne = Eidx.shape[1]
+    cent = V[:, Eidx].mean(axis=1)
+    L = n * dx
+    inc_c = rng.uniform(0.2 * L, 0.8 * L, (inclusions, 3)) + np.array([0, 0, 0.02])
+    inc_r = rng.uniform(0.08 * L, 0.15 * L, inclusions)
+    inside = np.zeros(ne, dtype=bool)
+    for c, r in zip(inc_c, inc_r):
+        inside |= np.linalg.norm(cent.T - c, axis=1) < r
+    base = 10 ** rng.uniform(4, 5, ne)          # soft phase
+    mu = np.where(inside, base * 100, base)      # inclusions 100x stiffer, within [1e4, 1e7]
+    mu = np.clip(mu, 1e4, 1e6 * 100)
+    la = 4 * mu
+    params = np.stack([mu, la], 1).ravel()
+    bottom = np.flatnonzero(np.abs(V[2] - 0.02) < 1e-12)
+    q0 = V.T.ravel().copy()
+    return {
+        "name": "tet_block", "edim": 4, "V": V, "E": Eidx, "density": 1000.0, "gravity": (0.0, 0.0, -9.81),
+        "energies": [("pd_neohookean", list(params), [])], "dirichlet": [],
+        "boundary": ("planar", [0.0, 0.0, 1.0, 0.0], list(bottom)), "q0": q0, "v0": np.zeros_like(q0),
+        "dt": 5e-3, "frames": 200, "options": default_options(friction_mu=fmu, max_pd_iter=1000),
+    }

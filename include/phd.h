@@ -1,0 +1,115 @@
+/*
+ * phd.h -- C ABI of the MI355X-native projective-dynamics (PD) time step.
+ *
+ * Drop-in boundary for DiffPhD's `pd_eigen_alg_phd` path. Plain pointers and
+ * sizes only; every function returns 0 on success and a non-zero code on
+ * failure, with the message available from phd_last_error(ctx) (the binding
+ * raises it as a Python RuntimeError, as the reference's SWIG %exception does
+ * for std::runtime_error, cpp/core/src/py_diff_pd_core.i:17-27). No exception
+ * crosses the ABI. Host buffers belong to the caller; device memory belongs
+ * to the context. One context per body (and per thread): unlike the
+ * reference's file-scope GPU cache (forward_gpu.cpp:280-298) contexts are
+ * independent. All positions/velocities/forces are FP64 AoS vectors
+ * [x0, y0, z0, x1, ...] of length 3N in the mesh's vertex order.
+ */
+#ifndef PHD_H
+#define PHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* phd_ctx;
+
+/* Context on HIP device `device`. Replaces the file-scope cuSPARSE handle and
+ * factor cache of forward_gpu.cpp:55-70, 280-298. NULL on failure. */
+phd_ctx phd_ctx_create(int device);
+void phd_ctx_destroy(phd_ctx ctx);
+/* Last error message of the context (valid until the next call). */
+const char* phd_last_error(phd_ctx ctx);
+
+/* Mesh + density: Deformable::Initialize(vertices, elements, density, 'none', E, nu),
+ * deformable.cpp:24-43 with mesh.cpp:36-57. X: 3*nv (xyz per vertex), elems:
+ * edim*ne (edim = 8 hex in the 000..111 node order, or 4 tet). */
+int phd_init_mesh(phd_ctx ctx, int edim, int nv, const double* X, int ne, const int* elems, double density);
+/* Deformable::AddPdEnergy (deformable_pd_energy.cpp:42-153): type "corotated",
+ * "volume" (params 1 or ne) or "pd_neohookean" (params 2 or 2*ne); idx = element
+ * indices (ni = 0: all elements). */
+int phd_add_pd_energy(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni);
+/* Deformable::SetDirichletBoundaryCondition (deformable.h:70-73). */
+int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
+/* Deformable::AddStateForce("gravity", g) (deformable_state_force.cpp:13-20). */
+int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
+/* Deformable::SetFrictionalBoundary (deformable_friction.cpp:7-89): "planar"
+ * [nx, ny, nz, offset], "spherical" [cx, cy, cz, r], and the two boundaries with
+ * no reference counterpart used by the press scene: "heightfield"
+ * [z0, nw, (amp, kx, ky, phase) * nw] and "press" (heightfield params + plane
+ * height; the plane's normal is -z). */
+int phd_set_frictional_boundary(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni);
+/* Moves the press plane (prescribed motion between frames). No reference counterpart. */
+int phd_set_press_plane(phd_ctx ctx, double z);
+
+/* SetupProjectiveDynamicsSolver (deformable_projective_dynamics_forward.cpp:8-183)
+ * plus the sparse-inverse factor build of forward_gpu.cpp:584-764: assembles A,
+ * orders it by nested dissection, factors it in FP64 on the host and expands
+ * S = L^-1 on the device (FP64 and FP32 copies). Cached until the scene changes. */
+int phd_factor_build(phd_ctx ctx, double dt, int threads);
+
+/* x = A^-1 b = S^T (S b) on the three axes; supersedes AlgPhd_GpuApplyAinv
+ * (forward_gpu.cpp:339-361). precision: 0 FP64 factor, 1 FP32 factor;
+ * refine: number of FP64 iterative-refinement steps. */
+int phd_apply_ainv(phd_ctx ctx, double dt, const double* b, double* x, int precision, int refine);
+
+/* One PD step, ForwardProjectiveDynamicsAlgPhd (forward_gpu.cpp:387-1594),
+ * options as the reference's std::map<string, real> (keys/vals, n entries;
+ * required: max_pd_iter, abs_tol, rel_tol, verbose, thread_ct). active: caller
+ * buffer of capacity *n_active on entry; on exit *n_active = number of contact
+ * nodes with lambda > 0 written to active. */
+int phd_step_forward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                     const char* const* keys, const double* vals, int n, double* q_next, double* v_next, int* active,
+                     int* n_active);
+
+/* Adjoint of one PD step, BackwardProjectiveDynamicsAlgPhd
+ * (deformable_projective_dynamics_backward_gpu.cpp:648-1250). Outputs (3N each):
+ * dl_dq, dl_dv, dl_df_ext; dl_dmat_w (one per PD element energy); dl_dstate_p
+ * (3 when gravity is set); dl_dw_elem (energies x elements, may be NULL): the
+ * per-element stiffness gradient whose sum over an energy's elements is dl_dmat_w. */
+int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                      const double* q_next, const double* v_next, const int* active, int n_active,
+                      const double* dl_dq_next, const double* dl_dv_next, const char* const* keys, const double* vals,
+                      int n, double* dl_dq, double* dl_dv, double* dl_df_ext, double* dl_dmat_w, double* dl_dstate_p,
+                      double* dl_dw_elem);
+
+/* Sizes. */
+int phd_dofs(phd_ctx ctx);
+int phd_num_elements(phd_ctx ctx);
+int phd_num_pd_element_energies(phd_ctx ctx);
+
+/* Statistics (alg_phd_gpu_memory.h:9-27, per context): factor bytes on the
+ * device, nnz of S, cached N, per-frame PD iterations and convergence flags. */
+long long phd_s_memory_bytes(phd_ctx ctx);
+long long phd_s_nnz(phd_ctx ctx);
+int phd_cached_n(phd_ctx ctx);
+int phd_frame_stats(phd_ctx ctx, int* iters, int* converged, double* seconds, int cap);
+void phd_reset_frame_stats(phd_ctx ctx);
+int phd_last_backward_iters(phd_ctx ctx);
+/* Factor description: out[0] nnz(S), [1] padded value count, [2] max etree depth,
+ * [3] ordering seconds, [4] factor seconds, [5] setup seconds, [6] row units,
+ * [7] column units, [8] nnz(L), [9] contact candidates K. */
+int phd_factor_info(phd_ctx ctx, double* out, int cap);
+/* Device-event timing of the S (rows) and S^T (columns) kernels: enable = 1 starts
+ * recording; phd_timing_read writes [rows_ms, rows_launches, cols_ms, cols_launches]
+ * accumulated since the last reset (reset = 1 clears after reading). */
+int phd_timing_enable(phd_ctx ctx, int enable);
+int phd_timing_read(phd_ctx ctx, double* out4, int reset);
+
+/* Host-only self-check of the factor construction (no device needed; small n):
+ * builds the factor of the symmetric CSR matrix (full pattern) with vertex
+ * coordinates X and writes [max|S^T S A - I|, max S-product plan error,
+ * max S^T-product plan error, nnz(S), max depth, nnz(L), structure violations]. */
+int phd_host_factor_check(int n, const int* Ap, const int* Ai, const double* Ax, const double* X, double* out7);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHD_H */

@@ -1,0 +1,1755 @@
+// =============================================================================
+// oracle.cpp -- FP64 CPU restatement of DiffPhD's `pd_eigen_alg_phd` projective
+// dynamics (PD) forward/backward time step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the correctness oracle: it is compiled
+// into oracle/_build/liboracle.so and loaded solely by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+// checker and never as the product path.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the reference repository root). The arithmetic is plain FP64 with no external
+// libraries: own 3x3 one-sided Jacobi SVD, own dense LDL^T/Cholesky and an own
+// up-looking sparse LDL^T under a recursive-coordinate-bisection nested
+// dissection ordering (the reference uses METIS NodeND; the ordering changes
+// only round-off because S^T S = A^-1 for any symmetric permutation).
+// =============================================================================
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace orc {
+
+using Vec = std::vector<double>;
+
+static void check(bool ok, const std::string& msg) {
+    if (!ok) throw std::runtime_error(msg);
+}
+
+// ---------------------------------------------------------------------------
+// Small dense helpers (column-major 3x3 like Eigen: M[i + 3*j] = M(i, j)).
+// ---------------------------------------------------------------------------
+struct M3 {
+    double a[9];
+    double& operator()(int i, int j) { return a[i + 3 * j]; }
+    double operator()(int i, int j) const { return a[i + 3 * j]; }
+    static M3 zero() { M3 m; for (double& x : m.a) x = 0; return m; }
+    static M3 eye() { M3 m = zero(); m(0, 0) = m(1, 1) = m(2, 2) = 1; return m; }
+};
+static M3 mul(const M3& A, const M3& B) {
+    M3 C = M3::zero();
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += A(i, k) * B(k, j);
+            C(i, j) = s;
+        }
+    return C;
+}
+static M3 tr(const M3& A) {
+    M3 B;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B(i, j) = A(j, i);
+    return B;
+}
+static M3 add(const M3& A, const M3& B) { M3 C; for (int i = 0; i < 9; ++i) C.a[i] = A.a[i] + B.a[i]; return C; }
+static M3 scale(const M3& A, double s) { M3 C; for (int i = 0; i < 9; ++i) C.a[i] = A.a[i] * s; return C; }
+static M3 diag_scale(const M3& U, const double* d, const M3& V) {  // U diag(d) V^T
+    M3 C = M3::zero();
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += U(i, k) * d[k] * V(j, k);
+            C(i, j) = s;
+        }
+    return C;
+}
+static double det3(const M3& A) {
+    return A(0, 0) * (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1))
+         - A(0, 1) * (A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0))
+         + A(0, 2) * (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0));
+}
+static M3 inv3(const M3& A) {
+    const double d = det3(A);
+    M3 B;
+    B(0, 0) = (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) / d;
+    B(0, 1) = (A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2)) / d;
+    B(0, 2) = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) / d;
+    B(1, 0) = (A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2)) / d;
+    B(1, 1) = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) / d;
+    B(1, 2) = (A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2)) / d;
+    B(2, 0) = (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0)) / d;
+    B(2, 1) = (A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1)) / d;
+    B(2, 2) = (A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0)) / d;
+    return B;
+}
+
+// Dense symmetric positive (semi)definite solve by LDL^T without pivoting on a
+// row-major n x n matrix. Stands in for Eigen's `.ldlt().solve()` on the small
+// SPD systems of the path (M_sys, AA Gram, G_T, NH prox Hessian); for an SPD
+// matrix every stable LDL^T gives the same solution up to round-off.
+static Vec dense_ldlt_solve(std::vector<double> M, int n, const Vec& b) {
+    Vec d(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        double djj = M[j * n + j];
+        for (int k = 0; k < j; ++k) djj -= M[j * n + k] * M[j * n + k] * d[k];
+        d[j] = djj;
+        for (int i = j + 1; i < n; ++i) {
+            double s = M[i * n + j];
+            for (int k = 0; k < j; ++k) s -= M[i * n + k] * M[j * n + k] * d[k];
+            M[i * n + j] = (djj != 0.0) ? s / djj : 0.0;
+        }
+    }
+    Vec x = b;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < i; ++k) x[i] -= M[i * n + k] * x[k];
+    for (int i = 0; i < n; ++i) x[i] = (d[i] != 0.0) ? x[i] / d[i] : 0.0;
+    for (int i = n - 1; i >= 0; --i)
+        for (int k = i + 1; k < n; ++k) x[i] -= M[k * n + i] * x[k];
+    return x;
+}
+
+static double norm(const Vec& v) { double s = 0; for (double x : v) s += x * x; return std::sqrt(s); }
+static double dot(const Vec& a, const Vec& b) { double s = 0; for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i]; return s; }
+
+// ---------------------------------------------------------------------------
+// 3x3 SVD by one-sided Jacobi: F V = U diag(sig), sig sorted descending and
+// non-negative (the conventions of Eigen::JacobiSVD used at
+// cpp/core/src/pd_energy/deformation_gradient_auxiliary_data.cpp:6 and
+// cpp/core/src/common/geometry.cpp:25,123).
+// ---------------------------------------------------------------------------
+void svd3(const M3& F, M3& U, double sig[3], M3& V) {
+    M3 B = F;
+    V = M3::eye();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, scale_sq = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                double alpha = 0, beta = 0, gamma = 0;
+                for (int i = 0; i < 3; ++i) {
+                    alpha += B(i, p) * B(i, p);
+                    beta += B(i, q) * B(i, q);
+                    gamma += B(i, p) * B(i, q);
+                }
+                scale_sq = std::max(scale_sq, alpha * beta);
+                if (std::fabs(gamma) <= 1e-300) continue;
+                off = std::max(off, gamma * gamma / std::max(alpha * beta, 1e-300));
+                const double zeta = (beta - alpha) / (2 * gamma);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+                const double c = 1 / std::sqrt(1 + t * t), s = c * t;
+                for (int i = 0; i < 3; ++i) {
+                    const double bp = B(i, p), bq = B(i, q);
+                    B(i, p) = c * bp - s * bq;
+                    B(i, q) = s * bp + c * bq;
+                    const double vp = V(i, p), vq = V(i, q);
+                    V(i, p) = c * vp - s * vq;
+                    V(i, q) = s * vp + c * vq;
+                }
+            }
+        (void)scale_sq;
+        if (off < 1e-32) break;
+    }
+    double nrm[3];
+    for (int j = 0; j < 3; ++j) nrm[j] = std::sqrt(B(0, j) * B(0, j) + B(1, j) * B(1, j) + B(2, j) * B(2, j));
+    int ord[3] = {0, 1, 2};
+    std::sort(ord, ord + 3, [&](int a, int b) { return nrm[a] > nrm[b]; });
+    M3 Bs, Vs;
+    for (int j = 0; j < 3; ++j) {
+        sig[j] = nrm[ord[j]];
+        for (int i = 0; i < 3; ++i) { Bs(i, j) = B(i, ord[j]); Vs(i, j) = V(i, ord[j]); }
+    }
+    V = Vs;
+    U = M3::zero();
+    // Columns with a vanishing singular value are completed to an orthonormal
+    // basis (sig is sorted, so only trailing columns can vanish).
+    const double tiny = 1e-300;
+    int valid = 0;
+    for (int j = 0; j < 3; ++j) {
+        if (!(sig[j] > tiny)) break;
+        for (int i = 0; i < 3; ++i) U(i, j) = Bs(i, j) / sig[j];
+        ++valid;
+    }
+    if (valid == 0) { U = M3::eye(); return; }
+    if (valid == 1) {
+        // any unit vector orthogonal to U0: cross with the least-aligned axis
+        int ax = 0;
+        for (int d = 1; d < 3; ++d) if (std::fabs(U(d, 0)) < std::fabs(U(ax, 0))) ax = d;
+        double e[3] = {0, 0, 0};
+        e[ax] = 1;
+        double c[3] = {U(1, 0) * e[2] - U(2, 0) * e[1], U(2, 0) * e[0] - U(0, 0) * e[2], U(0, 0) * e[1] - U(1, 0) * e[0]};
+        const double cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int i = 0; i < 3; ++i) U(i, 1) = c[i] / cn;
+        valid = 2;
+    }
+    if (valid == 2) {
+        U(0, 2) = U(1, 0) * U(2, 1) - U(2, 0) * U(1, 1);
+        U(1, 2) = U(2, 0) * U(0, 1) - U(0, 0) * U(2, 1);
+        U(2, 2) = U(0, 0) * U(1, 1) - U(1, 0) * U(0, 1);
+    }
+}
+
+// PolarDecomposition, cpp/core/src/common/geometry.cpp:24-31: R = U V^T, S = V Sig V^T.
+void polar3(const M3& F, M3& R, M3& S) {
+    M3 U, V;
+    double s[3];
+    svd3(F, U, s, V);
+    R = mul(U, tr(V));
+    S = diag_scale(V, s, V);
+}
+
+// dRFromdF, cpp/core/src/common/geometry.cpp:85-113.
+M3 dR_from_dF(const M3& R, const M3& S, const M3& dF) {
+    const M3 lhs = add(mul(tr(R), dF), scale(mul(tr(dF), R), -1.0));
+    M3 A = M3::zero();
+    A(0, 0) = S(0, 0) + S(1, 1);
+    A(1, 1) = S(0, 0) + S(2, 2);
+    A(2, 2) = S(1, 1) + S(2, 2);
+    A(0, 1) = A(1, 0) = S(1, 2);
+    A(0, 2) = A(2, 0) = -S(0, 2);
+    A(1, 2) = A(2, 1) = S(0, 1);
+    const M3 Ai = inv3(A);
+    const double b[3] = {lhs(0, 1), lhs(0, 2), lhs(1, 2)};
+    double xyz[3];
+    for (int i = 0; i < 3; ++i) xyz[i] = Ai(i, 0) * b[0] + Ai(i, 1) * b[1] + Ai(i, 2) * b[2];
+    M3 W = M3::zero();
+    W(0, 1) = xyz[0]; W(0, 2) = xyz[1];
+    W(1, 0) = -xyz[0]; W(1, 2) = xyz[2];
+    W(2, 0) = -xyz[1]; W(2, 1) = -xyz[2];
+    return mul(R, W);
+}
+
+// dSvd, cpp/core/src/common/geometry.cpp:159-186 (rotation terms dropped when
+// sig_i - sig_j <= 10 * DBL_EPSILON).
+void dsvd3(const M3& U, const double sig[3], const M3& V, const M3& dF, M3& dU, double dsig[3], M3& dV) {
+    const M3 dP = mul(mul(tr(U), dF), V);
+    for (int i = 0; i < 3; ++i) dsig[i] = dP(i, i);
+    const double eps = 10 * std::numeric_limits<double>::epsilon();
+    M3 Sij = M3::zero();
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (sig[i] - sig[j] > eps) {
+                Sij(i, j) = 1.0 / (sig[j] * sig[j] - sig[i] * sig[i]);
+                Sij(j, i) = -Sij(i, j);
+            }
+    M3 dwU, dwV;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            dwU(i, j) = Sij(i, j) * (dP(i, j) * sig[j] + sig[i] * dP(j, i));
+            dwV(i, j) = Sij(i, j) * (sig[i] * dP(i, j) + dP(j, i) * sig[j]);
+        }
+    dU = mul(U, dwU);
+    dV = mul(V, dwV);
+}
+
+// ---------------------------------------------------------------------------
+// PD element energies (cpp/core/src/pd_energy/*.cpp).
+// ---------------------------------------------------------------------------
+enum EnergyType { kCorotated = 0, kVolume = 1, kNeoHookean = 2 };
+
+struct Energy {
+    int type = kCorotated;
+    double stiffness = 0;              // stiffness_(), mean when per-element
+    std::vector<double> elem_stiff;    // element_stiffnesses_ (empty: homogeneous)
+    double mu = 0, la = 0;             // NH mean Lame parameters
+    std::vector<char> member;          // element in this energy's index set
+    double elem_w(int e) const { return elem_stiff.empty() ? stiffness : elem_stiff[e]; }
+};
+
+// VolumePdElementEnergy::ProjectToManifold, volume_pd_element_energy.cpp:21-59.
+static void volume_project_D(const double sig[3], double D[3]) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    check(std::min(sig[0], std::min(sig[1], sig[2])) >= eps, "Singular F.");
+    const double p = std::pow(sig[0] * sig[1] * sig[2], 1.0 / 3.0);
+    for (int i = 0; i < 3; ++i) D[i] = sig[i] / p - sig[i];
+    for (int it = 0; it < 50; ++it) {
+        const double C = (sig[0] + D[0]) * (sig[1] + D[1]) * (sig[2] + D[2]) - 1;
+        double g[3];
+        for (int j = 0; j < 3; ++j) {
+            g[j] = 1;
+            for (int k = 0; k < 3; ++k) if (k != j) g[j] *= (sig[k] + D[k]);
+        }
+        const double gD = g[0] * D[0] + g[1] * D[1] + g[2] * D[2];
+        const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        double Dn[3], diff = 0;
+        for (int j = 0; j < 3; ++j) { Dn[j] = (gD - C) / gg * g[j]; diff = std::max(diff, std::fabs(Dn[j] - D[j])); }
+        if (diff <= eps) break;
+        for (int j = 0; j < 3; ++j) D[j] = Dn[j];
+    }
+}
+
+// NeoHookeanPdElementEnergy::HessianProximal, neohookean_pd_element_energy.cpp:37-56.
+static void nh_hessian(const Energy& en, const double s[3], double H[9]) {
+    const double k = en.stiffness;
+    const double L = std::log(std::max(s[0] * s[1] * s[2], 1e-12));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            H[i * 3 + j] = (i == j) ? en.mu + (en.mu + en.la - en.la * L) / (s[i] * s[i]) + k
+                                    : en.la / (s[i] * s[j]);
+}
+
+// NeoHookeanPdElementEnergy::SolveProximal, neohookean_pd_element_energy.cpp:58-103.
+void nh_solve_prox(const Energy& en, const double sF[3], double s[3]) {
+    const double k = en.stiffness, smin = 1e-4, tol = 1e-10;
+    for (int i = 0; i < 3; ++i) s[i] = std::max(sF[i], smin);
+    for (int it = 0; it < 32; ++it) {
+        const double L = std::log(std::max(s[0] * s[1] * s[2], 1e-12));
+        Vec g(3);
+        for (int i = 0; i < 3; ++i)
+            g[i] = en.mu * s[i] - en.mu / s[i] + en.la * L / s[i] + k * (s[i] - sF[i]);
+        if (norm(g) < tol) break;
+        double H[9];
+        nh_hessian(en, s, H);
+        for (int i = 0; i < 3; ++i) H[i * 4] += 1e-9;
+        const Vec delta = dense_ldlt_solve(std::vector<double>(H, H + 9), 3, g);
+        double alpha = 1, sn[3];
+        for (int i = 0; i < 3; ++i) sn[i] = s[i] - alpha * delta[i];
+        for (int ls = 0; ls < 20; ++ls) {
+            const double mn = std::min(sn[0], std::min(sn[1], sn[2]));
+            if (mn > smin * 0.5) break;
+            alpha *= 0.5;
+            for (int i = 0; i < 3; ++i) sn[i] = s[i] - alpha * delta[i];
+        }
+        double step = 0;
+        for (int i = 0; i < 3; ++i) step = std::max(step, std::fabs(sn[i] - s[i]));
+        for (int i = 0; i < 3; ++i) s[i] = std::max(sn[i], smin);
+        if (step < tol) break;
+    }
+}
+
+// F auxiliary data (deformation_gradient_auxiliary_data.cpp:3-12).
+struct FAux {
+    M3 F, U, V, R, S;
+    double sig[3];
+    void init(const M3& F_) {
+        F = F_;
+        svd3(F, U, sig, V);
+        R = mul(U, tr(V));
+        S = diag_scale(V, sig, V);
+    }
+};
+
+// ProjectToManifold(F_auxiliary): corotated_pd_element_energy.cpp:21-24,
+// volume_pd_element_energy.cpp:21-59, neohookean_pd_element_energy.cpp:112-119.
+M3 project(const Energy& en, const FAux& a) {
+    if (en.type == kCorotated) return a.R;
+    if (en.type == kVolume) {
+        double D[3], sd[3];
+        volume_project_D(a.sig, D);
+        for (int i = 0; i < 3; ++i) sd[i] = a.sig[i] + D[i];
+        return diag_scale(a.U, sd, a.V);
+    }
+    double ss[3];
+    nh_solve_prox(en, a.sig, ss);
+    return diag_scale(a.U, ss, a.V);
+}
+
+// ProjectToManifoldDifferential(F_auxiliary, projection, dF):
+// corotated_pd_element_energy.cpp:27-31, volume_pd_element_energy.cpp:61-155,
+// neohookean_pd_element_energy.cpp:128-162.
+M3 project_diff(const Energy& en, const FAux& a, const M3& dF) {
+    if (en.type == kCorotated) return dR_from_dF(a.R, a.S, dF);
+    const double* sig = a.sig;
+    if (en.type == kVolume) {
+        double D[3];
+        volume_project_D(sig, D);
+        double g[3];
+        for (int j = 0; j < 3; ++j) { g[j] = 1; for (int k = 0; k < 3; ++k) if (k != j) g[j] *= (sig[k] + D[k]); }
+        double H[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                if (i == j) { H[i * 3 + j] = 0; continue; }
+                double h = 1;
+                for (int k = 0; k < 3; ++k) if (k != i && k != j) h *= (sig[k] + D[k]);
+                H[i * 3 + j] = h;
+            }
+        M3 dU, dV;
+        double dsig[3];
+        dsvd3(a.U, sig, a.V, dF, dU, dsig, dV);
+        const double Dn = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+        const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        const double la_abs = Dn / gn;
+        double mneg = 0, mpos = 0;
+        for (int i = 0; i < 3; ++i) {
+            mneg += (D[i] - la_abs * g[i]) * (D[i] - la_abs * g[i]);
+            mpos += (D[i] + la_abs * g[i]) * (D[i] + la_abs * g[i]);
+        }
+        const double la = (mneg < mpos) ? -la_abs : la_abs;
+        // [I + la H, g; g^T, 0] [dD; dla] = [-la H dsig; -g.dsig]
+        double A[16] = {0};
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) A[i * 4 + j] = (i == j ? 1.0 : 0.0) + la * H[i * 3 + j];
+            A[i * 4 + 3] = g[i];
+            A[3 * 4 + i] = g[i];
+        }
+        double rhs[4];
+        for (int i = 0; i < 3; ++i) {
+            double s = 0;
+            for (int j = 0; j < 3; ++j) s += H[i * 3 + j] * dsig[j];
+            rhs[i] = -la * s;
+        }
+        rhs[3] = -(g[0] * dsig[0] + g[1] * dsig[1] + g[2] * dsig[2]);
+        // Gaussian elimination with partial pivoting (stands in for A.inverse() * b).
+        double Mx[4][5];
+        for (int i = 0; i < 4; ++i) { for (int j = 0; j < 4; ++j) Mx[i][j] = A[i * 4 + j]; Mx[i][4] = rhs[i]; }
+        for (int c = 0; c < 4; ++c) {
+            int p = c;
+            for (int r = c + 1; r < 4; ++r) if (std::fabs(Mx[r][c]) > std::fabs(Mx[p][c])) p = r;
+            for (int j = 0; j < 5; ++j) std::swap(Mx[c][j], Mx[p][j]);
+            for (int r = 0; r < 4; ++r) {
+                if (r == c) continue;
+                const double f = Mx[r][c] / Mx[c][c];
+                for (int j = c; j < 5; ++j) Mx[r][j] -= f * Mx[c][j];
+            }
+        }
+        double dD[3];
+        for (int i = 0; i < 3; ++i) dD[i] = Mx[i][4] / Mx[i][i];
+        M3 out = dF;
+        out = add(out, diag_scale(dU, D, a.V));
+        out = add(out, diag_scale(a.U, dD, a.V));
+        out = add(out, diag_scale(a.U, D, dV));
+        return out;
+    }
+    // Neo-Hookean proximal map.
+    double ss[3];
+    nh_solve_prox(en, sig, ss);
+    M3 dU, dV;
+    double dsF[3];
+    dsvd3(a.U, sig, a.V, dF, dU, dsF, dV);
+    double H[9];
+    nh_hessian(en, ss, H);
+    for (int i = 0; i < 3; ++i) H[i * 4] += 1e-9;
+    Vec rhs(3);
+    for (int i = 0; i < 3; ++i) rhs[i] = en.stiffness * dsF[i];
+    const Vec dss = dense_ldlt_solve(std::vector<double>(H, H + 9), 3, rhs);
+    const double dsd[3] = {dss[0], dss[1], dss[2]};
+    M3 out = diag_scale(dU, ss, a.V);
+    out = add(out, diag_scale(a.U, dsd, a.V));
+    out = add(out, diag_scale(a.U, ss, dV));
+    return out;
+}
+
+// StressTensor(F_auxiliary, projection): pd_element_energy.cpp:46-50 for the
+// PD energies (stiffness_ * (F - p)); for pd_neohookean the true Neo-Hookean
+// PK1 with the mean Lame parameters, neohookean_pd_element_energy.cpp:188-197.
+M3 stress(const Energy& en, const FAux& a, const M3& p) {
+    if (en.type != kNeoHookean) return scale(add(a.F, scale(p, -1.0)), en.stiffness);
+    const M3& F = a.F;
+    const double J = det3(F);
+    const double lj = std::log(std::max(J, 1e-12));
+    const M3 FinvT = tr(inv3(F));
+    return add(scale(add(F, scale(FinvT, -1.0)), en.mu), scale(FinvT, en.la * lj));
+}
+
+// ---------------------------------------------------------------------------
+// Frictional boundaries (friction/*.cpp, deformable_friction.cpp:7-89) plus
+// the heightfield ground and the two-obstacle press used by the contact-rich
+// press scene (no reference counterpart: the reference has one planar or
+// spherical boundary per body).
+// ---------------------------------------------------------------------------
+enum BoundaryType { kNone = 0, kPlanar = 1, kSpherical = 2, kHeightfield = 3, kPress = 4 };
+
+struct Boundary {
+    int type = kNone;
+    double n[3] = {0, 0, 1}, off = 0;     // planar
+    double c[3] = {0, 0, 0}, r = 0;       // spherical
+    double z0 = 0;                         // heightfield base
+    std::vector<double> waves;             // (amp, kx, ky, phase) quadruples
+    double plane_z = 0;                    // press plane height (normal -z)
+
+    void hf(double x, double y, double& h, double& hx, double& hy) const {
+        h = z0; hx = 0; hy = 0;
+        for (size_t i = 0; i + 3 < waves.size(); i += 4) {
+            const double a = waves[i], kx = waves[i + 1], ky = waves[i + 2], ph = waves[i + 3];
+            const double arg = kx * x + ky * y + ph;
+            h += a * std::sin(arg);
+            hx += a * kx * std::cos(arg);
+            hy += a * ky * std::cos(arg);
+        }
+    }
+    // Which obstacle governs a point (press only): 0 ground, 1 plane.
+    int press_side(const double* q, double& d_ground, double& d_plane) const {
+        double h, hx, hy;
+        hf(q[0], q[1], h, hx, hy);
+        d_ground = (q[2] - h) / std::sqrt(1 + hx * hx + hy * hy);
+        d_plane = plane_z - q[2];
+        return (d_plane < d_ground) ? 1 : 0;
+    }
+    double distance(const double* q) const {
+        if (type == kPlanar) return q[0] * n[0] + q[1] * n[1] + q[2] * n[2] + off;
+        if (type == kSpherical) {
+            const double d0 = q[0] - c[0], d1 = q[1] - c[1], d2 = q[2] - c[2];
+            return std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) - r;
+        }
+        if (type == kHeightfield) {
+            double h, hx, hy;
+            hf(q[0], q[1], h, hx, hy);
+            return (q[2] - h) / std::sqrt(1 + hx * hx + hy * hy);
+        }
+        double dg, dp;
+        return press_side(q, dg, dp) ? dp : dg;
+    }
+    void normal(const double* q, double out[3]) const {
+        if (type == kPlanar) { for (int i = 0; i < 3; ++i) out[i] = n[i]; return; }
+        if (type == kSpherical) {
+            double d[3] = {q[0] - c[0], q[1] - c[1], q[2] - c[2]};
+            const double rho = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            check(rho > std::numeric_limits<double>::epsilon(), "SphericalFrictionalBoundary::GetNormal singular at sphere center.");
+            for (int i = 0; i < 3; ++i) out[i] = d[i] / rho;
+            return;
+        }
+        double dg = 0, dp = 0;
+        if (type == kPress && press_side(q, dg, dp)) { out[0] = 0; out[1] = 0; out[2] = -1; return; }
+        double h, hx, hy;
+        hf(q[0], q[1], h, hx, hy);
+        const double s = std::sqrt(1 + hx * hx + hy * hy);
+        out[0] = -hx / s; out[1] = -hy / s; out[2] = 1 / s;
+    }
+    // GetLocalFrame (planar_frictional_boundary.cpp:25-40,
+    // spherical_frictional_boundary.cpp:25-42): columns t1, t2, n.
+    void frame(const double* q, double t1[3], double t2[3], double nn[3]) const {
+        normal(q, nn);
+        double ux[3] = {0, 0, 0}, best = 0;
+        for (int i = 0; i < 3; ++i) {
+            double e[3] = {0, 0, 0};
+            e[i] = 1;
+            const double x[3] = {nn[1] * e[2] - nn[2] * e[1], nn[2] * e[0] - nn[0] * e[2], nn[0] * e[1] - nn[1] * e[0]};
+            const double sq = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+            if (sq > best) { best = sq; for (int k = 0; k < 3; ++k) ux[k] = x[k]; }
+        }
+        const double un = std::sqrt(best);
+        for (int k = 0; k < 3; ++k) t1[k] = ux[k] / un;
+        double uy[3] = {nn[1] * t1[2] - nn[2] * t1[1], nn[2] * t1[0] - nn[0] * t1[2], nn[0] * t1[1] - nn[1] * t1[0]};
+        const double yn = std::sqrt(uy[0] * uy[0] + uy[1] * uy[1] + uy[2] * uy[2]);
+        for (int k = 0; k < 3; ++k) t2[k] = uy[k] / yn;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Sparse LDL^T (restates the CSparse-style up-looking factorization of
+// RealSim/src/Scomponent/tools/math/SparseLDLT.cpp:433-518) on a symmetric
+// matrix given in full CSR, after a symmetric permutation `perm` (new -> old).
+// ---------------------------------------------------------------------------
+struct SparseLDLT {
+    int n = 0;
+    std::vector<int> perm, iperm;      // perm[new] = old
+    std::vector<int> Lp, Li;           // CSC of unit lower L (strict)
+    std::vector<double> Lx, D;
+
+    void factor(int n_, const std::vector<int>& Ap, const std::vector<int>& Ai, const std::vector<double>& Ax,
+                const std::vector<int>& perm_) {
+        n = n_;
+        perm = perm_;
+        iperm.assign(n, 0);
+        for (int i = 0; i < n; ++i) iperm[perm[i]] = i;
+        // Permuted upper-triangular pattern by columns: C = P A P^T, entries (i,k), i<=k, stored by column k.
+        std::vector<std::vector<std::pair<int, double>>> col(n);
+        for (int r = 0; r < n; ++r)
+            for (int p = Ap[r]; p < Ap[r + 1]; ++p) {
+                const int c = Ai[p];
+                const int i = iperm[r], k = iperm[c];
+                if (i <= k) col[k].push_back({i, Ax[p]});
+            }
+        // Elimination tree and column counts (LDL symbolic, T. Davis 2005).
+        std::vector<int> parent(n, -1), flag(n, -1), Lnz(n, 0);
+        for (int k = 0; k < n; ++k) {
+            flag[k] = k;
+            for (auto& e : col[k]) {
+                int i = e.first;
+                for (; flag[i] != k; i = parent[i]) {
+                    if (parent[i] == -1) parent[i] = k;
+                    Lnz[i]++;
+                    flag[i] = k;
+                }
+            }
+        }
+        Lp.assign(n + 1, 0);
+        for (int k = 0; k < n; ++k) Lp[k + 1] = Lp[k] + Lnz[k];
+        Li.assign(Lp[n], 0);
+        Lx.assign(Lp[n], 0.0);
+        D.assign(n, 0.0);
+        // Numeric up-looking factorization (row k of L from a sparse triangular solve).
+        std::vector<double> Y(n, 0.0);
+        std::vector<int> pattern(n), fill(n, 0);
+        for (int k = 0; k < n; ++k) {
+            Y[k] = 0;
+            int top = n;
+            flag[k] = k;
+            for (auto& e : col[k]) {
+                int i = e.first;
+                Y[i] += e.second;
+                int len = 0;
+                for (; flag[i] != k; i = parent[i]) { pattern[len++] = i; flag[i] = k; }
+                while (len > 0) pattern[--top] = pattern[--len];
+            }
+            D[k] = Y[k];
+            Y[k] = 0;
+            for (; top < n; ++top) {
+                const int i = pattern[top];
+                const double yi = Y[i];
+                Y[i] = 0;
+                int p;
+                for (p = Lp[i]; p < Lp[i] + fill[i]; ++p) Y[Li[p]] -= Lx[p] * yi;
+                const double lki = yi / D[i];
+                D[k] -= lki * yi;
+                Li[p] = k;
+                Lx[p] = lki;
+                fill[i]++;
+            }
+            check(D[k] != 0.0, "LDLT zero pivot.");
+        }
+    }
+    // x = A^-1 b.
+    void solve(const double* b, double* x) const {
+        Vec y(n);
+        for (int i = 0; i < n; ++i) y[i] = b[perm[i]];
+        for (int j = 0; j < n; ++j)
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) y[Li[p]] -= Lx[p] * y[j];
+        for (int j = 0; j < n; ++j) y[j] /= D[j];
+        for (int j = n - 1; j >= 0; --j)
+            for (int p = Lp[j]; p < Lp[j + 1]; ++p) y[j] -= Lx[p] * y[Li[p]];
+        for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
+    }
+};
+
+// Nested dissection by recursive coordinate bisection: split at the median of
+// the widest axis, separator = nodes of the lower half adjacent to the upper
+// half; order (lower, upper, separator).
+static void rcb_order(const std::vector<int>& nodes, const Vec& X, const std::vector<std::vector<int>>& adj,
+                      std::vector<int>& out, std::vector<int>& side) {
+    if (nodes.size() <= 32) { for (int v : nodes) out.push_back(v); return; }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v : nodes)
+        for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], X[3 * v + d]); hi[d] = std::max(hi[d], X[3 * v + d]); }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    std::vector<double> key;
+    for (int v : nodes) key.push_back(X[3 * v + ax]);
+    std::vector<double> sk = key;
+    std::nth_element(sk.begin(), sk.begin() + sk.size() / 2, sk.end());
+    const double med = sk[sk.size() / 2];
+    // side: 1 lower, 2 upper (marks valid only for this call's nodes)
+    for (size_t i = 0; i < nodes.size(); ++i) side[nodes[i]] = (key[i] < med) ? 1 : 2;
+    std::vector<int> L, U, Sep;
+    for (int v : nodes) {
+        if (side[v] == 2) { U.push_back(v); continue; }
+        bool touches = false;
+        for (int w : adj[v]) if (side[w] == 2) { touches = true; break; }
+        (touches ? Sep : L).push_back(v);
+    }
+    for (int v : nodes) side[v] = 0;
+    if (L.empty() || U.empty()) { for (int v : nodes) out.push_back(v); return; }
+    rcb_order(L, X, adj, out, side);
+    rcb_order(U, X, adj, out, side);
+    for (int v : Sep) out.push_back(v);
+}
+
+// ---------------------------------------------------------------------------
+// The deformable body: mesh, FE samples, energies, boundary conditions, state.
+// ---------------------------------------------------------------------------
+struct AlgPhdReplayCache {  // cpp/core/include/fem/deformable.h:336-355 (fields used by the backward)
+    std::vector<int> contact_candidates;
+    Vec q_tilde, q_sol;
+    Vec lambda, omega;
+    std::vector<std::array<double, 3>> normals;
+};
+
+struct Body {
+    int edim = 8;                 // 8 hex, 4 tet
+    int nv = 0, ne = 0, dofs = 0;
+    Vec X;                        // 3*nv rest positions
+    std::vector<int> elems;       // edim*ne
+    double density = 0, element_volume = 0;
+    int samples = 8;
+    std::vector<double> grad;     // per (element, sample): edim x 3 row-major grad_undeformed_sample_weight
+    std::vector<Energy> energies;
+    std::map<int, double> dirichlet;
+    std::vector<double> gravity;  // empty or 3
+    Boundary boundary;
+    std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
+
+    // cached solver state (mutable members of the reference Deformable)
+    bool pd_ready = false;
+    double pd_dt = 0;
+    SparseLDLT ldlt[3];
+    std::vector<AlgPhdReplayCache> tape;
+    std::vector<int> prev_candidates;
+    Vec prev_lambda;
+    std::vector<int> frame_iters, frame_converged;
+    int last_bwd_iters = 0;
+
+    const double* G(int e, int s) const { return &grad[((size_t)e * samples + s) * edim * 3]; }
+
+    // Mesh<3,8>/<3,4>::ComputeElementVolume (mesh.cpp:111-129) and
+    // InitializeAfterMesh (deformable.cpp:33-43).
+    void init(int edim_, int nv_, const double* Xv, int ne_, const int* el, double density_) {
+        edim = edim_; nv = nv_; ne = ne_; dofs = 3 * nv;
+        X.assign(Xv, Xv + 3 * nv);
+        elems.assign(el, el + (size_t)edim * ne);
+        density = density_;
+        samples = (edim == 8) ? 8 : 1;
+        double vol_sum = 0;
+        for (int e = 0; e < ne; ++e) {
+            const int* v = &elems[(size_t)e * edim];
+            M3 A;
+            if (edim == 8) {
+                for (int i = 0; i < 3; ++i) {
+                    A(i, 0) = X[3 * v[1] + i] - X[3 * v[0] + i];
+                    A(i, 1) = X[3 * v[2] + i] - X[3 * v[0] + i];
+                    A(i, 2) = X[3 * v[4] + i] - X[3 * v[0] + i];
+                }
+                vol_sum += std::fabs(det3(A));
+            } else {
+                for (int i = 0; i < 3; ++i) {
+                    A(i, 0) = X[3 * v[1] + i] - X[3 * v[0] + i];
+                    A(i, 1) = X[3 * v[2] + i] - X[3 * v[0] + i];
+                    A(i, 2) = X[3 * v[3] + i] - X[3 * v[0] + i];
+                }
+                vol_sum += std::fabs(det3(A)) / 6;
+            }
+        }
+        element_volume = vol_sum / ne;
+        const double dx = std::pow(element_volume, 1.0 / 3.0);
+        grad.assign((size_t)ne * samples * edim * 3, 0.0);
+        if (edim == 8) {
+            // HexDeformable::InitializeFiniteElementSamples, hex_deformable.cpp:3-50.
+            const double r = 1 / std::sqrt(3.0), inv = 1 / dx;
+            double g8[8][8][3];
+            for (int s = 0; s < 8; ++s) {
+                const int bx = (s >> 2) & 1, by = (s >> 1) & 1, bz = s & 1;
+                const double x = ((bx - 0.5) * r + 0.5) * dx, y = ((by - 0.5) * r + 0.5) * dx, z = ((bz - 0.5) * r + 0.5) * dx;
+                const double nx = x * inv, ny = y * inv, nz = z * inv, cx = 1 - nx, cy = 1 - ny, cz = 1 - nz;
+                const double rows[8][3] = {
+                    {-inv * cy * cz, cx * -inv * cz, cx * cy * -inv}, {-inv * cy * nz, cx * -inv * nz, cx * cy * inv},
+                    {-inv * ny * cz, cx * inv * cz, cx * ny * -inv},  {-inv * ny * nz, cx * inv * nz, cx * ny * inv},
+                    {inv * cy * cz, nx * -inv * cz, nx * cy * -inv},  {inv * cy * nz, nx * -inv * nz, nx * cy * inv},
+                    {inv * ny * cz, nx * inv * cz, nx * ny * -inv},   {inv * ny * nz, nx * inv * nz, nx * ny * inv}};
+                for (int k = 0; k < 8; ++k) for (int j = 0; j < 3; ++j) g8[s][k][j] = rows[k][j];
+            }
+            for (int e = 0; e < ne; ++e)
+                for (int s = 0; s < 8; ++s)
+                    std::memcpy(&grad[((size_t)e * 8 + s) * 24], &g8[s][0][0], sizeof(double) * 24);
+        } else {
+            // TetDeformable::InitializeFiniteElementSamples, tet_deformable.cpp:3-41.
+            for (int e = 0; e < ne; ++e) {
+                const int* v = &elems[(size_t)e * 4];
+                M3 Bi;
+                for (int i = 0; i < 3; ++i) {
+                    Bi(i, 0) = X[3 * v[1] + i] - X[3 * v[0] + i];
+                    Bi(i, 1) = X[3 * v[2] + i] - X[3 * v[0] + i];
+                    Bi(i, 2) = X[3 * v[3] + i] - X[3 * v[0] + i];
+                }
+                const M3 B = inv3(Bi);
+                double* g = &grad[(size_t)e * 12];
+                for (int j = 0; j < 3; ++j) {
+                    g[0 * 3 + j] = -B(0, j) - B(1, j) - B(2, j);
+                    g[1 * 3 + j] = B(0, j);
+                    g[2 * 3 + j] = B(1, j);
+                    g[3 * 3 + j] = B(2, j);
+                }
+            }
+        }
+        pd_ready = false;
+    }
+
+    // F = x_e G (DeformationGradient / pd_A * x_e, finite_element_sample.cpp:11-24).
+    M3 deformation_gradient(const double* q, int e, int s) const {
+        const int* v = &elems[(size_t)e * edim];
+        const double* g = G(e, s);
+        M3 F = M3::zero();
+        for (int k = 0; k < edim; ++k)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) F(i, j) += q[3 * v[k] + i] * g[k * 3 + j];
+        return F;
+    }
+    // out_e[3k+i] += w * (A^T vec(P))_{k,i} = w * sum_j P(i,j) G(k,j).
+    void add_At(const M3& P, int e, int s, double w, double* out) const {
+        const double* g = G(e, s);
+        for (int k = 0; k < edim; ++k)
+            for (int i = 0; i < 3; ++i) {
+                double acc = 0;
+                for (int j = 0; j < 3; ++j) acc += P(i, j) * g[k * 3 + j];
+                out[3 * k + i] += w * acc;
+            }
+    }
+
+    // AddPdEnergy, deformable_pd_energy.cpp:42-153.
+    void add_pd_energy(const std::string& type, const Vec& params, const std::vector<int>& idx) {
+        pd_ready = false;
+        Energy en;
+        const int np = (int)params.size();
+        if (type == "corotated" || type == "volume") {
+            check(np == 1 || np == ne, "Inconsistent param size.");
+            en.type = (type == "corotated") ? kCorotated : kVolume;
+            if (np == 1) en.stiffness = params[0];
+            else { en.elem_stiff = params; en.stiffness = std::accumulate(params.begin(), params.end(), 0.0) / np; }
+        } else if (type == "pd_neohookean") {
+            check(np == 2 || np == 2 * ne, "pd_neohookean expects params = [mu, lambda] or per-element.");
+            en.type = kNeoHookean;
+            if (np == 2) { en.mu = params[0]; en.la = params[1]; en.stiffness = 2 * params[0]; }
+            else {
+                double ms = 0, ls = 0;
+                en.elem_stiff.resize(ne);
+                for (int i = 0; i < ne; ++i) { ms += params[2 * i]; ls += params[2 * i + 1]; en.elem_stiff[i] = 2 * params[2 * i]; }
+                en.mu = ms / ne; en.la = ls / ne;
+                en.stiffness = std::accumulate(en.elem_stiff.begin(), en.elem_stiff.end(), 0.0) / ne;
+            }
+        } else {
+            check(false, "Unsupported PD energy: " + type);
+        }
+        en.member.assign(ne, 0);
+        if (idx.empty()) std::fill(en.member.begin(), en.member.end(), 1);
+        for (int i : idx) { check(0 <= i && i < ne, "Element index out of bound."); en.member[i] = 1; }
+        energies.push_back(en);
+    }
+
+    // SetFrictionalBoundary, deformable_friction.cpp:7-89 (planar/spherical) +
+    // heightfield / press.
+    void set_boundary(const std::string& type, const Vec& p, const std::vector<int>& idx) {
+        pd_ready = false;
+        Boundary b;
+        if (type == "planar") {
+            check(p.size() == 4, "Incompatible parameter number.");
+            const double nn = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            check(nn > std::numeric_limits<double>::epsilon(), "Singular normal.");
+            b.type = kPlanar;
+            for (int i = 0; i < 3; ++i) b.n[i] = p[i] / nn;
+            b.off = p[3] / nn;
+        } else if (type == "spherical") {
+            check(p.size() == 4, "Incompatible parameter number.");
+            b.type = kSpherical;
+            for (int i = 0; i < 3; ++i) b.c[i] = p[i];
+            b.r = p[3];
+        } else if (type == "heightfield" || type == "press") {
+            // params: [z0, nwaves, (amp, kx, ky, phase) * nwaves] (+ [plane_z] for press)
+            check(p.size() >= 2, "Incompatible parameter number.");
+            const int nw = (int)p[1];
+            const size_t need = 2 + 4 * (size_t)nw + (type == "press" ? 1 : 0);
+            check(p.size() == need, "Incompatible parameter number.");
+            b.type = (type == "press") ? kPress : kHeightfield;
+            b.z0 = p[0];
+            b.waves.assign(p.begin() + 2, p.begin() + 2 + 4 * nw);
+            if (type == "press") b.plane_z = p[2 + 4 * nw];
+        } else {
+            check(false, "Unsupported frictional boundary type: " + type);
+        }
+        std::set<int> uniq(idx.begin(), idx.end());
+        check(uniq.size() == idx.size(), "Duplicated vertex elements.");
+        boundary = b;
+        fb_vertices.clear();
+        int cnt = 0;
+        for (int i : uniq) fb_vertices[i] = cnt++;
+    }
+
+    // ForwardStateForce with GravitationalStateForce (gravitational_state_force.cpp:14-22):
+    // f_i = mass * g, mass = density * element_volume.
+    Vec state_force() const {
+        Vec f(dofs, 0.0);
+        if (gravity.size() == 3) {
+            const double m = density * element_volume;
+            for (int i = 0; i < dofs; ++i) f[i] = m * gravity[i % 3];
+        }
+        return f;
+    }
+
+    // SetupProjectiveDynamicsSolver, deformable_projective_dynamics_forward.cpp:8-183
+    // (material 'none' -> base_w = 0; the unused Acc_/AinvIc_ are omitted).
+    void setup(double dt) {
+        if (pd_ready) return;
+        const double mass = density * element_volume;
+        const double inv_h2m = mass / (dt * dt);
+        std::vector<double> w_e(ne, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (auto& en : energies) if (en.member[e]) w_e[e] += en.elem_w(e);
+        std::vector<std::map<int, double>> rows[3];
+        for (int d = 0; d < 3; ++d) rows[d].assign(nv, {});
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < nv; ++i) rows[d][i][i] += dirichlet.count(3 * i + d) ? 1.0 : inv_h2m;
+        for (int e = 0; e < ne; ++e) {
+            const int* v = &elems[(size_t)e * edim];
+            for (int s = 0; s < samples; ++s) {
+                const double* g = G(e, s);
+                const double w = w_e[e] * element_volume / samples;
+                for (int k = 0; k < edim; ++k)
+                    for (int l = 0; l < edim; ++l) {
+                        const double val = w * (g[k * 3] * g[l * 3] + g[k * 3 + 1] * g[l * 3 + 1] + g[k * 3 + 2] * g[l * 3 + 2]);
+                        for (int d = 0; d < 3; ++d) {
+                            if (dirichlet.count(3 * v[k] + d) || dirichlet.count(3 * v[l] + d)) continue;
+                            rows[d][v[k]][v[l]] += val;
+                        }
+                    }
+            }
+        }
+        // Ordering from the pattern of axis 0 (identical Dirichlet sets give identical patterns;
+        // a per-axis ordering is computed otherwise).
+        for (int d = 0; d < 3; ++d) {
+            std::vector<int> Ap(nv + 1, 0), Ai;
+            std::vector<double> Ax;
+            std::vector<std::vector<int>> adj(nv);
+            for (int i = 0; i < nv; ++i) {
+                for (auto& kv : rows[d][i]) {
+                    Ai.push_back(kv.first);
+                    Ax.push_back(kv.second);
+                    if (kv.first != i) adj[i].push_back(kv.first);
+                }
+                Ap[i + 1] = (int)Ai.size();
+            }
+            std::vector<int> all(nv), perm, side(nv, 0);
+            std::iota(all.begin(), all.end(), 0);
+            rcb_order(all, X, adj, perm, side);
+            ldlt[d].factor(nv, Ap, Ai, Ax, perm);
+        }
+        pd_ready = true;
+        pd_dt = dt;
+    }
+
+    Vec solve_axis(int d, const Vec& b) const {
+        Vec x(nv);
+        ldlt[d].solve(b.data(), x.data());
+        return x;
+    }
+
+    // ProjectiveDynamicsLocalStep, deformable_projective_dynamics_forward.cpp:186-292.
+    Vec local_step(const Vec& q) const {
+        Vec qb(dofs, 0.0);
+        for (auto& kv : dirichlet) qb[kv.first] = kv.second;
+        std::vector<double> acc((size_t)ne * edim * 3, 0.0);
+        for (auto& en : energies) {
+#pragma omp parallel for schedule(static)
+            for (int e = 0; e < ne; ++e) {
+                if (!en.member[e]) continue;
+                const double w = en.elem_w(e) * element_volume / samples;
+                for (int s = 0; s < samples; ++s) {
+                    const M3 F = deformation_gradient(q.data(), e, s);
+                    const M3 Fb = deformation_gradient(qb.data(), e, s);
+                    FAux a;
+                    a.init(F);
+                    const M3 Bp = project(en, a);
+                    add_At(add(Bp, scale(Fb, -1.0)), e, s, w, &acc[(size_t)e * edim * 3]);
+                }
+            }
+        }
+        Vec rhs(dofs, 0.0);
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < edim; ++k)
+                for (int i = 0; i < 3; ++i) rhs[3 * elems[(size_t)e * edim + k] + i] += acc[((size_t)e * edim + k) * 3 + i];
+        return rhs;
+    }
+
+    // SetupProjectiveDynamicsLocalStepDifferential, deformable_projective_dynamics_backward.cpp:10-80:
+    // per element the 3*edim square matrix sum_s A_s^T (sum_energies w dBp) A_s with
+    // w = stiffness() (mean) * vol / s, or the element's own stiffness when
+    // `exact` (exact-adjoint mode).
+    std::vector<double> local_differential(const Vec& q, bool exact) const {
+        for (auto& kv : dirichlet) check(q[kv.first] == kv.second, "Boundary conditions violated.");
+        const int nd = 3 * edim;
+        std::vector<double> K((size_t)ne * nd * nd, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            double* Ke = &K[(size_t)e * nd * nd];
+            for (int s = 0; s < samples; ++s) {
+                const M3 F = deformation_gradient(q.data(), e, s);
+                FAux a;
+                a.init(F);
+                double wdBp[81] = {0};  // 9x9, column c = d(vec p)/d(vec F)_c
+                bool any = false;
+                for (auto& en : energies) {
+                    if (!en.member[e]) continue;
+                    any = true;
+                    const double w = (exact ? en.elem_w(e) : en.stiffness) * element_volume / samples;
+                    for (int c = 0; c < 9; ++c) {
+                        M3 dF = M3::zero();
+                        dF.a[c] = 1;
+                        const M3 dp = project_diff(en, a, dF);
+                        for (int r = 0; r < 9; ++r) wdBp[r + 9 * c] += w * dp.a[r];
+                    }
+                }
+                if (!any) continue;
+                // A: 9 x nd with A[i+3j, i+3k] = G(k,j).
+                const double* g = G(e, s);
+                for (int col = 0; col < nd; ++col) {
+                    const int kc = col / 3, ic = col % 3;
+                    double t[9];  // wdBp * A[:, col]
+                    for (int r = 0; r < 9; ++r) {
+                        double acc = 0;
+                        for (int j = 0; j < 3; ++j) acc += wdBp[r + 9 * (ic + 3 * j)] * g[kc * 3 + j];
+                        t[r] = acc;
+                    }
+                    for (int row = 0; row < nd; ++row) {
+                        const int kr = row / 3, ir = row % 3;
+                        double acc = 0;
+                        for (int j = 0; j < 3; ++j) acc += g[kr * 3 + j] * t[ir + 3 * j];
+                        Ke[(size_t)row * nd + col] += acc;
+                    }
+                }
+            }
+        }
+        return K;
+    }
+
+    // ApplyProjectiveDynamicsLocalStepDifferential, deformable_projective_dynamics_backward.cpp:82-139.
+    Vec apply_local_differential(const std::vector<double>& K, const Vec& dq) const {
+        const int nd = 3 * edim;
+        Vec out(dofs, 0.0);
+        std::vector<double> tmp((size_t)ne * nd, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            const int* v = &elems[(size_t)e * edim];
+            double x[24];
+            for (int k = 0; k < edim; ++k) for (int i = 0; i < 3; ++i) x[3 * k + i] = dq[3 * v[k] + i];
+            const double* Ke = &K[(size_t)e * nd * nd];
+            for (int r = 0; r < nd; ++r) {
+                double acc = 0;
+                for (int c = 0; c < nd; ++c) acc += Ke[(size_t)r * nd + c] * x[c];
+                tmp[(size_t)e * nd + r] = acc;
+            }
+        }
+        for (int e = 0; e < ne; ++e)
+            for (int r = 0; r < nd; ++r) out[3 * elems[(size_t)e * edim + r / 3] + r % 3] += tmp[(size_t)e * nd + r];
+        return out;
+    }
+
+    void forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const std::map<std::string, double>& opt,
+                 Vec& q_next, Vec& v_next, std::vector<int>& active);
+    void backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
+                  const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
+                  const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext, Vec& dl_dmat_w,
+                  Vec& dl_dstate_p, Vec& dl_dw_elem);
+};
+
+static double opt_or(const std::map<std::string, double>& o, const char* k, double dflt) {
+    auto it = o.find(k);
+    return it == o.end() ? dflt : it->second;
+}
+static double opt_req(const std::map<std::string, double>& o, const char* k) {
+    auto it = o.find(k);
+    check(it != o.end(), std::string("Missing option ") + k + ".");
+    return it->second;
+}
+
+// Type-II Anderson acceleration (Walker & Ni 2011) with the reference's
+// regularization and restart rule, forward_gpu.cpp:1346-1379 and
+// backward_gpu.cpp:822-851. Returns true if the extrapolated iterate is used.
+struct Anderson {
+    int window;
+    std::deque<Vec> xh, gh;
+    explicit Anderson(int w) : window(w) {}
+    // x_prev: previous iterate, g_k = x_new - x_prev. On acceptance writes candidate.
+    bool step(const Vec& x_prev, const Vec& g_k, Vec& candidate) {
+        xh.push_back(x_prev);
+        gh.push_back(g_k);
+        const int m = (int)gh.size() - 1;
+        bool used = false;
+        if (m >= 1) {
+            const size_t n = g_k.size();
+            std::vector<Vec> dG(m, Vec(n)), dQ(m, Vec(n));
+            double sq = 0;
+            for (int i = 0; i < m; ++i)
+                for (size_t j = 0; j < n; ++j) {
+                    dG[i][j] = gh[i + 1][j] - gh[i][j];
+                    dQ[i][j] = xh[i + 1][j] - xh[i][j];
+                    sq += dG[i][j] * dG[i][j];
+                }
+            std::vector<double> GtG(m * m);
+            Vec rhs(m);
+            for (int a = 0; a < m; ++a) {
+                for (int b = 0; b < m; ++b) GtG[a * m + b] = dot(dG[a], dG[b]);
+                rhs[a] = dot(dG[a], g_k);
+            }
+            const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+            for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
+            const Vec gamma = dense_ldlt_solve(GtG, m, rhs);
+            if (norm(gamma) >= 10.0) {
+                xh.clear();
+                gh.clear();
+            } else {
+                candidate.assign(n, 0.0);
+                for (size_t j = 0; j < n; ++j) {
+                    double c = x_prev[j] + g_k[j];
+                    for (int i = 0; i < m; ++i) c -= (dQ[i][j] + dG[i][j]) * gamma[i];
+                    candidate[j] = c;
+                }
+                used = true;
+            }
+        }
+        while ((int)xh.size() > window) { xh.pop_front(); gh.pop_front(); }
+        return used;
+    }
+};
+
+// ForwardProjectiveDynamicsAlgPhd, deformable_projective_dynamics_forward_gpu.cpp:387-1594.
+void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const std::map<std::string, double>& opt,
+                   Vec& q_next, Vec& v_next, std::vector<int>& active) {
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    opt_req(opt, "thread_ct");
+    const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    check((int)q.size() == dofs && (int)v.size() == dofs && (int)f_ext.size() == dofs, "Inconsistent state size.");
+    setup(dt);
+    const double h = dt, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    const Vec fs = state_force();
+    Vec q_tilde(dofs);
+    for (int i = 0; i < dofs; ++i) q_tilde[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
+
+    // New-rollout detection for the forward tape (:437-449).
+    if (!tape.empty()) {
+        const auto& last = tape.back();
+        bool seq = last.q_sol.size() == q.size();
+        if (seq) {
+            Vec d(dofs);
+            for (int i = 0; i < dofs; ++i) d[i] = last.q_sol[i] - q[i];
+            seq = norm(d) <= 1e-8 * std::max(1.0, std::max(norm(last.q_sol), norm(q)));
+        }
+        if (!seq) tape.clear();
+    }
+
+    // Contact candidates and frames (:456-528).
+    std::vector<int> cand;
+    for (auto& kv : fb_vertices) cand.push_back(kv.first);
+    const int K = (int)cand.size();
+    std::vector<std::array<double, 3>> nrm(K), t1(K), t2(K);
+    for (int c = 0; c < K; ++c) boundary.frame(&q[3 * cand[c]], t1[c].data(), t2[c].data(), nrm[c].data());
+
+    // Delassus W via per-column A^-1 (:840-1013).
+    std::vector<std::array<Vec, 3>> Acol(K), At1(K), At2(K);
+    for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < K; ++c) {
+            Vec rhs(nv, 0.0);
+            rhs[cand[c]] = nrm[c][d];
+            Acol[c][d] = solve_axis(d, rhs);
+        }
+    std::vector<double> W((size_t)K * K, 0.0);
+    for (int a = 0; a < K; ++a)
+        for (int b = a; b < K; ++b) {
+            double w = 0;
+            for (int d = 0; d < 3; ++d) w += nrm[b][d] * Acol[a][d][cand[b]];
+            W[(size_t)a * K + b] = W[(size_t)b * K + a] = w;
+        }
+    const double fmu = opt_or(opt, "friction_mu", 0.0);
+    const double restitution = opt_or(opt, "restitution", 0.0);
+    const bool do_fric = (K > 0) && (fmu > 0);
+    std::vector<std::array<double, 4>> Wtt(K, {0, 0, 0, 0});
+    if (do_fric) {
+        for (int d = 0; d < 3; ++d)
+            for (int c = 0; c < K; ++c) {
+                Vec r1(nv, 0.0), r2(nv, 0.0);
+                r1[cand[c]] = t1[c][d];
+                r2[cand[c]] = t2[c][d];
+                At1[c][d] = solve_axis(d, r1);
+                At2[c][d] = solve_axis(d, r2);
+            }
+        for (int c = 0; c < K; ++c) {
+            double w11 = 0, w12 = 0, w22 = 0;
+            const int ni = cand[c];
+            for (int d = 0; d < 3; ++d) {
+                w11 += t1[c][d] * At1[c][d][ni];
+                w12 += t1[c][d] * At2[c][d][ni];
+                w22 += t2[c][d] * At2[c][d][ni];
+            }
+            Wtt[c] = {w11, w12, w12, w22};
+        }
+    }
+    Vec rprec(K);
+    for (int c = 0; c < K; ++c) rprec[c] = std::max(h * h * W[(size_t)c * K + c], 1e-10);
+
+    Vec q_sol = q, q_prev = q;
+    Vec lambda(K, 0.0), lt1(K, 0.0), lt2(K, 0.0), omega_last(K, 0.0);
+    if (!prev_candidates.empty() && prev_lambda.size() == prev_candidates.size()) {
+        std::unordered_map<int, int> pidx;
+        for (size_t i = 0; i < prev_candidates.size(); ++i) pidx[prev_candidates[i]] = (int)i;
+        for (int c = 0; c < K; ++c) {
+            auto it = pidx.find(cand[c]);
+            if (it != pidx.end()) lambda[c] = prev_lambda[it->second];
+        }
+    }
+    Anderson aa(aa_window);
+    Vec prev_pd_rhs(dofs, 0.0);
+    bool converged = false;
+    int iters = 0;
+    for (int iter = 0; iter < max_pd_iter; ++iter) {
+        const Vec pd_rhs = local_step(q_sol);
+        Vec b_full(dofs);
+        for (int i = 0; i < dofs; ++i) b_full[i] = inv_h2m * q_tilde[i] + pd_rhs[i];
+        Vec b_bc = b_full;
+        for (auto& kv : dirichlet) b_bc[kv.first] = 0;
+        q_prev = q_sol;
+        if (K > 0) {
+            Vec omega(K), E(K), hvec(K);
+            for (int c = 0; c < K; ++c) {
+                const double dj = boundary.distance(&q_sol[3 * cand[c]]);
+                const double lj = lambda[c], rj = rprec[c];
+                double den = std::sqrt(dj * dj + rj * rj * lj * lj);
+                if (den < 1e-14) den = 1e-14;
+                omega[c] = 1.0 - dj / den;
+                E[c] = (1.0 - rj * lj / den) * rj;
+                const double phi = dj + rj * lj - den;
+                double Hq = 0;
+                for (int d = 0; d < 3; ++d) Hq += nrm[c][d] * q_sol[3 * cand[c] + d];
+                hvec[c] = -phi + omega[c] * Hq;
+            }
+            omega_last = omega;
+            std::vector<double> M((size_t)K * K);
+            for (int a = 0; a < K; ++a)
+                for (int b = 0; b < K; ++b) M[(size_t)a * K + b] = omega[a] * W[(size_t)a * K + b] * omega[b];
+            for (int a = 0; a < K; ++a) M[(size_t)a * K + a] += E[a];
+            // A^-1 b_full per axis (:1199-1210; note b_full, not b_bc).
+            std::array<Vec, 3> Ab;
+            for (int d = 0; d < 3; ++d) {
+                Vec bd(nv);
+                for (int i = 0; i < nv; ++i) bd[i] = b_full[3 * i + d];
+                Ab[d] = solve_axis(d, bd);
+            }
+            Vec Ag(dofs);
+            for (int d = 0; d < 3; ++d) {
+                Vec gd = Ab[d];
+                for (int c = 0; c < K; ++c) {
+                    const double s = omega[c] * lambda[c];
+                    for (int i = 0; i < nv; ++i) gd[i] += s * Acol[c][d][i];
+                    if (do_fric)
+                        for (int i = 0; i < nv; ++i) gd[i] += lt1[c] * At1[c][d][i] + lt2[c] * At2[c][d][i];
+                }
+                for (int i = 0; i < nv; ++i) Ag[3 * i + d] = gd[i];
+            }
+            Vec rhs(K);
+            for (int c = 0; c < K; ++c) {
+                double val = 0;
+                for (int d = 0; d < 3; ++d) val += nrm[c][d] * Ag[3 * cand[c] + d];
+                rhs[c] = hvec[c] - omega[c] * val;
+            }
+            const Vec dl = dense_ldlt_solve(M, K, rhs);
+            for (int c = 0; c < K; ++c) lambda[c] = std::min(std::max(lambda[c] + dl[c], 0.0), 1e8);
+            Vec wl(K);
+            for (int c = 0; c < K; ++c) wl[c] = omega[c] * lambda[c];
+            if (do_fric) {
+                for (int c = 0; c < K; ++c) {
+                    if (lambda[c] <= 0.0) { lt1[c] = lt2[c] = 0.0; continue; }
+                    const int node = cand[c];
+                    double qn[3], step[3];
+                    for (int d = 0; d < 3; ++d) {
+                        qn[d] = Ab[d][node];
+                        for (int b = 0; b < K; ++b) qn[d] += wl[b] * Acol[b][d][node];
+                        step[d] = qn[d] - q[3 * node + d];
+                    }
+                    const double sn = step[0] * nrm[c][0] + step[1] * nrm[c][1] + step[2] * nrm[c][2];
+                    double ut[3];
+                    for (int d = 0; d < 3; ++d) ut[d] = step[d] - sn * nrm[c][d];
+                    const double u1 = ut[0] * t1[c][0] + ut[1] * t1[c][1] + ut[2] * t1[c][2];
+                    const double u2 = ut[0] * t2[c][0] + ut[1] * t2[c][1] + ut[2] * t2[c][2];
+                    const auto& Wt = Wtt[c];
+                    const double det = Wt[0] * Wt[3] - Wt[1] * Wt[2];
+                    double s1 = 0, s2 = 0;
+                    if (std::fabs(det) > 1e-20) {
+                        s1 = (Wt[3] * -u1 - Wt[1] * -u2) / det;
+                        s2 = (-Wt[2] * -u1 + Wt[0] * -u2) / det;
+                    }
+                    const double cone = fmu * lambda[c];
+                    if (std::sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1[c] = s1; lt2[c] = s2; }
+                    else {
+                        const double um = std::sqrt(u1 * u1 + u2 * u2);
+                        if (um > 1e-14) { lt1[c] = -cone * u1 / um; lt2[c] = -cone * u2 / um; }
+                        else lt1[c] = lt2[c] = 0.0;
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) {
+                Vec qd = Ab[d];
+                for (int c = 0; c < K; ++c) {
+                    for (int i = 0; i < nv; ++i) qd[i] += wl[c] * Acol[c][d][i];
+                    if (do_fric)
+                        for (int i = 0; i < nv; ++i) qd[i] += lt1[c] * At1[c][d][i] + lt2[c] * At2[c][d][i];
+                }
+                for (int i = 0; i < nv; ++i) q_sol[3 * i + d] = qd[i];
+            }
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                Vec bd(nv);
+                for (int i = 0; i < nv; ++i) bd[i] = b_bc[3 * i + d];
+                const Vec qd = solve_axis(d, bd);
+                for (int i = 0; i < nv; ++i) q_sol[3 * i + d] = qd[i];
+            }
+        }
+        ++iters;
+        for (auto& kv : dirichlet) q_sol[kv.first] = kv.second;
+        if (aa_window > 0) {
+            Vec gk(dofs), cand_q;
+            for (int i = 0; i < dofs; ++i) gk[i] = q_sol[i] - q_prev[i];
+            if (aa.step(q_prev, gk, cand_q)) {
+                q_sol = cand_q;
+                for (auto& kv : dirichlet) q_sol[kv.first] = kv.second;
+            }
+        }
+        Vec dq(dofs), dr(dofs);
+        for (int i = 0; i < dofs; ++i) { dq[i] = q_sol[i] - q_prev[i]; dr[i] = pd_rhs[i] - prev_pd_rhs[i]; }
+        const double abs_err = norm(dq), rhs_norm = norm(q_prev), rc = norm(dr), rr = norm(prev_pd_rhs);
+        prev_pd_rhs = pd_rhs;
+        if (iter > 0 && abs_err <= rel_tol * rhs_norm + abs_tol && rc <= rel_tol * rr + abs_tol) { converged = true; break; }
+    }
+    frame_iters.push_back(iters);
+    frame_converged.push_back(converged ? 1 : 0);
+
+    // Restitution post-pass (:1483-1531).
+    if (restitution > 0) {
+        const double slop = opt_or(opt, "friction_contact_slop", 1e-8);
+        for (int c = 0; c < K; ++c) {
+            if (lambda[c] <= 0) continue;
+            const int node = cand[c];
+            const double sd = boundary.distance(&q_sol[3 * node]);
+            double nh[3];
+            boundary.normal(&q_sol[3 * node], nh);
+            const double nn = std::sqrt(nh[0] * nh[0] + nh[1] * nh[1] + nh[2] * nh[2]);
+            if (nn <= 1e-12) continue;
+            for (int d = 0; d < 3; ++d) nh[d] /= nn;
+            if (sd > slop) continue;
+            const double vin = v[3 * node] * nh[0] + v[3 * node + 1] * nh[1] + v[3 * node + 2] * nh[2];
+            if (vin < 0) {
+                double vout = 0;
+                for (int d = 0; d < 3; ++d) vout += (q_sol[3 * node + d] - q[3 * node + d]) / h * nh[d];
+                const double dv = -restitution * vin - vout;
+                if (dv > 0) for (int d = 0; d < 3; ++d) q_sol[3 * node + d] += h * dv * nh[d];
+            }
+        }
+    }
+    q_next = q_sol;
+    v_next.assign(dofs, 0.0);
+    for (int i = 0; i < dofs; ++i) v_next[i] = (q_next[i] - q[i]) / h;
+    active.clear();
+    for (int c = 0; c < K; ++c) if (lambda[c] > 0) active.push_back(cand[c]);
+    prev_candidates = cand;
+    prev_lambda = lambda;
+    AlgPhdReplayCache fc;
+    fc.contact_candidates = active;
+    fc.q_tilde = q_tilde;
+    fc.q_sol = q_next;
+    const int na = (int)active.size();
+    fc.lambda.assign(na, 0.0);
+    fc.omega.assign(na, 0.0);
+    fc.normals.assign(na, {0, 0, 0});
+    std::unordered_map<int, int> cidx;
+    for (int c = 0; c < K; ++c) cidx[cand[c]] = c;
+    for (int a = 0; a < na; ++a) {
+        const int c = cidx[active[a]];
+        fc.normals[a] = nrm[c];
+        fc.omega[a] = omega_last[c];
+        fc.lambda[a] = lambda[c];
+    }
+    tape.push_back(std::move(fc));
+    if (tape.size() > 8192) tape.erase(tape.begin(), tape.begin() + (tape.size() - 8192));
+}
+
+// BackwardProjectiveDynamicsAlgPhd, deformable_projective_dynamics_backward_gpu.cpp:648-1250.
+// The tape must match (the fixed-active-set replay at :113-418 is not restated).
+void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
+                    const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
+                    const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext, Vec& dl_dmat_w,
+                    Vec& dl_dstate_p, Vec& dl_dw_elem) {
+    (void)v; (void)f_ext; (void)v_next;
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    opt_req(opt, "thread_ct");
+    const double h = dt, inv_h = 1 / h, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    setup(dt);
+    const int nE = (int)energies.size();
+    dl_dq.assign(dofs, 0.0);
+    dl_dv.assign(dofs, 0.0);
+    dl_df_ext.assign(dofs, 0.0);
+    dl_dmat_w.assign(nE, 0.0);
+    dl_dstate_p.assign(gravity.size() == 3 ? 3 : 0, 0.0);
+    dl_dw_elem.assign((size_t)nE * ne, 0.0);
+    Vec rhs_qbar(dofs);
+    for (int i = 0; i < dofs; ++i) {
+        dl_dq[i] += -dl_dv_next[i] * inv_h;
+        rhs_qbar[i] = dl_dq_next[i] + dl_dv_next[i] * inv_h;
+    }
+    const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
+    const std::vector<double> Kpd = local_differential(q_next, exact);
+
+    AlgPhdReplayCache replay;
+    bool from_tape = false;
+    if (!tape.empty()) {
+        const auto& tail = tape.back();
+        bool match = tail.contact_candidates == active && tail.q_sol.size() == q_next.size();
+        if (match) {
+            Vec d(dofs);
+            for (int i = 0; i < dofs; ++i) d[i] = tail.q_sol[i] - q_next[i];
+            match = norm(d) <= 1e-8 * std::max(1.0, std::max(norm(tail.q_sol), norm(q_next)));
+        }
+        if (match) { replay = tail; tape.pop_back(); from_tape = true; }
+    }
+    check(from_tape || active.empty(), "Backward needs the forward tape of this frame (replay is not restated).");
+    if (!from_tape) { replay.contact_candidates = active; }
+
+    auto apply_ainv = [&](const Vec& rhs) {
+        Vec out(dofs);
+        for (int d = 0; d < 3; ++d) {
+            Vec bd(nv);
+            for (int i = 0; i < nv; ++i) bd[i] = rhs[3 * i + d];
+            const Vec x = solve_axis(d, bd);
+            for (int i = 0; i < nv; ++i) out[3 * i + d] = x[i];
+        }
+        for (auto& kv : dirichlet) out[kv.first] = 0;
+        return out;
+    };
+    const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    const double clip = opt_or(opt, "alg_phd_bwd_clip_norm", 1e6);
+    auto safe_clip = [&](Vec x) {
+        for (double& t : x) if (!std::isfinite(t)) t = 0;
+        const double n = norm(x);
+        if (clip > 0 && std::isfinite(n) && n > clip) for (double& t : x) t *= clip / n;
+        return x;
+    };
+    // Fixed point s <- safe_clip(A^-1 (rhs + K_pd s)) with AA (backward_gpu.cpp:818-861).
+    auto fixed_point = [&](const Vec& rhs, int max_it, int* used_iters) {
+        Vec s(dofs, 0.0);
+        Anderson aa(aa_window);
+        int it_used = 0;
+        for (int i = 0; i < max_it; ++i) {
+            const Vec kpd = apply_local_differential(Kpd, s);
+            Vec t(dofs);
+            for (int j = 0; j < dofs; ++j) t[j] = rhs[j] + kpd[j];
+            Vec s_new = safe_clip(apply_ainv(t));
+            Vec diff(dofs);
+            for (int j = 0; j < dofs; ++j) diff[j] = s_new[j] - s[j];
+            const double err = norm(diff);
+            if (aa_window > 0) {
+                Vec cand_x;
+                if (aa.step(s, diff, cand_x)) s_new = safe_clip(cand_x);
+            }
+            s = s_new;
+            it_used = i + 1;
+            if (i > 0 && err < rel_tol * norm(s) + abs_tol) break;
+        }
+        if (used_iters) *used_iters = it_used;
+        return s;
+    };
+    int bb_iters = 0;
+    const Vec db = fixed_point(rhs_qbar, max_pd_iter, &bb_iters);
+    last_bwd_iters = bb_iters;
+    const bool diverged = clip > 0 && norm(db) >= clip * 0.999;
+    if (!diverged) {
+        Vec inj(dofs);
+        for (int i = 0; i < dofs; ++i) inj[i] = inv_h2m * db[i];
+        const double cap = norm(rhs_qbar) + 1.0, in = norm(inj);
+        if (in > cap) for (double& t : inj) t *= cap / in;
+        for (int i = 0; i < dofs; ++i) {
+            dl_dq[i] += inj[i];
+            dl_df_ext[i] += h2m * inj[i];
+            dl_dv[i] += h * inj[i];
+        }
+        // GravitationalStateForce::BackwardForce (gravitational_state_force.cpp:24-33).
+        if (gravity.size() == 3)
+            for (int i = 0; i < dofs; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
+    }
+    Vec z = db;
+    const bool skip_wb = opt_or(opt, "skip_contact_woodbury", 0.0) > 0.5;
+    const int nl = (int)replay.contact_candidates.size();
+    if (!diverged && !skip_wb && nl > 0) {
+        const double fmu = opt_or(opt, "friction_mu", 0.0);
+        const bool tb = fmu > 0;
+        std::vector<int> act_ci;
+        for (int c = 0; c < nl; ++c) if (replay.omega[c] >= 0.5) act_ci.push_back(c);
+        std::vector<std::array<double, 3>> T1(nl), T2(nl);
+        if (tb)
+            for (int c = 0; c < nl; ++c) {
+                const auto& n = replay.normals[c];
+                int axis = 0;
+                for (int d = 1; d < 3; ++d) if (std::fabs(n[d]) < std::fabs(n[axis])) axis = d;
+                double e[3] = {0, 0, 0};
+                e[axis] = 1;
+                const double ne_ = n[0] * e[0] + n[1] * e[1] + n[2] * e[2];
+                double a1[3] = {e[0] - ne_ * n[0], e[1] - ne_ * n[1], e[2] - ne_ * n[2]};
+                double an = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+                if (an > 1e-12) for (double& t : a1) t /= an;
+                else { a1[0] = 1; a1[1] = 0; a1[2] = 0; }
+                double a2[3] = {n[1] * a1[2] - n[2] * a1[1], n[2] * a1[0] - n[0] * a1[2], n[0] * a1[1] - n[1] * a1[0]};
+                double bn = std::sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+                if (bn > 1e-12) for (double& t : a2) t /= bn;
+                else { a2[0] = 0; a2[1] = 1; a2[2] = 0; }
+                for (int d = 0; d < 3; ++d) { T1[c][d] = a1[d]; T2[c][d] = a2[d]; }
+            }
+        struct BC { int ci, basis; };
+        std::vector<BC> full;
+        const double tlmin = opt_or(opt, "alg_phd_bwd_tangent_lambda_min", 1e-8);
+        for (int c : act_ci) {
+            full.push_back({c, 0});
+            if (tb && replay.lambda[c] > tlmin) { full.push_back({c, 1}); full.push_back({c, 2}); }
+        }
+        auto dir = [&](const BC& b, double out[3]) {
+            for (int d = 0; d < 3; ++d)
+                out[d] = b.basis == 0 ? replay.omega[b.ci] * replay.normals[b.ci][d]
+                                      : (b.basis == 1 ? T1[b.ci][d] : T2[b.ci][d]);
+        };
+        auto project_val = [&](const BC& b, const Vec& x) {
+            const int node = replay.contact_candidates[b.ci];
+            double dd[3];
+            dir(b, dd);
+            return dd[0] * x[3 * node] + dd[1] * x[3 * node + 1] + dd[2] * x[3 * node + 2];
+        };
+        const int nbf = (int)full.size();
+        if (nbf > 0) {
+            Vec rf(nbf);
+            double rmax = 0;
+            for (int i = 0; i < nbf; ++i) { rf[i] = project_val(full[i], z); rmax = std::max(rmax, std::fabs(rf[i])); }
+            const double gate = std::max(opt_or(opt, "alg_phd_bwd_rhs_prune_abs", 1e-12),
+                                         opt_or(opt, "alg_phd_bwd_rhs_prune_rel", 1e-3) * rmax);
+            std::vector<int> sel;
+            for (int i = 0; i < nbf; ++i) if (std::fabs(rf[i]) >= gate) sel.push_back(i);
+            const int topk = (int)opt_or(opt, "alg_phd_bwd_basis_topk", 0);
+            if (topk > 0 && (int)sel.size() > topk) {
+                std::nth_element(sel.begin(), sel.begin() + topk, sel.end(),
+                                 [&](int a, int b) { return std::fabs(rf[a]) > std::fabs(rf[b]); });
+                sel.resize(topk);
+            }
+            const int nb = (int)sel.size();
+            if (nb > 0) {
+                const int wt_cap = (int)opt_or(opt, "alg_phd_bwd_wt_max_iter", 120);
+                const int wt_max = std::min(max_pd_iter, std::max(1, wt_cap));
+                std::vector<Vec> WT(nb);
+                for (int k = 0; k < nb; ++k) {
+                    const BC b = full[sel[k]];
+                    Vec j(dofs, 0.0);
+                    double dd[3];
+                    dir(b, dd);
+                    const int node = replay.contact_candidates[b.ci];
+                    for (int d = 0; d < 3; ++d) j[3 * node + d] = dd[d];
+                    WT[k] = fixed_point(j, wt_max, nullptr);
+                }
+                std::vector<double> GT((size_t)nb * nb);
+                Vec rb(nb);
+                for (int k = 0; k < nb; ++k) {
+                    rb[k] = rf[sel[k]];
+                    for (int l = 0; l < nb; ++l) GT[(size_t)l * nb + k] = project_val(full[sel[l]], WT[k]);
+                }
+                // G_T.ldlt() (:1186): G_T is symmetric up to the fixed-point tolerance.
+                const Vec alpha = dense_ldlt_solve(GT, nb, rb);
+                for (int k = 0; k < nb; ++k)
+                    for (int i = 0; i < dofs; ++i) z[i] -= alpha[k] * WT[k][i];
+                z = safe_clip(z);
+            }
+        }
+    }
+    // dl/d(mat_w) (:1231-1249) with PdEnergyForceDifferential(require_dw),
+    // deformable_pd_energy.cpp:368-403: df/dw = -(vol/s) A^T vec(P / stiffness_).
+    const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
+    if (!skip_mat && !diverged) {
+        Vec contrib(nE, 0.0);
+        std::vector<double> per((size_t)nE * ne, 0.0);
+        bool ok = true;
+        for (int ei = 0; ei < nE; ++ei) {
+            const Energy& en = energies[ei];
+            const double inv_w = 1 / en.stiffness;
+            for (int e = 0; e < ne; ++e) {
+                if (!en.member[e]) continue;
+                const int* vv = &elems[(size_t)e * edim];
+                double acc[24] = {0};
+                for (int s = 0; s < samples; ++s) {
+                    const M3 F = deformation_gradient(q_next.data(), e, s);
+                    FAux a;
+                    a.init(F);
+                    const M3 p = project(en, a);
+                    const M3 P = stress(en, a, p);
+                    add_At(P, e, s, -inv_w * element_volume / samples, acc);
+                }
+                double sum = 0;
+                for (int k = 0; k < edim; ++k)
+                    for (int i = 0; i < 3; ++i) sum += z[3 * vv[k] + i] * acc[3 * k + i];
+                per[(size_t)ei * ne + e] = sum;
+                contrib[ei] += sum;
+            }
+        }
+        for (double c : contrib) if (!std::isfinite(c)) ok = false;
+        if (ok) {
+            for (int ei = 0; ei < nE; ++ei) dl_dmat_w[ei] += contrib[ei];
+            dl_dw_elem = per;
+        }
+    }
+}
+
+}  // namespace orc
+
+// =============================================================================
+// C ABI for the test harness (ctypes).
+// =============================================================================
+namespace {
+struct Handle {
+    orc::Body body;
+    std::string err;
+};
+std::map<std::string, double> make_opts(const char* const* keys, const double* vals, int n) {
+    std::map<std::string, double> o;
+    for (int i = 0; i < n; ++i) o[keys[i]] = vals[i];
+    return o;
+}
+template <class F>
+int guard(Handle* h, F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" {
+void* orc_create() { return new Handle(); }
+void orc_destroy(void* h) { delete static_cast<Handle*>(h); }
+const char* orc_last_error(void* h) { return static_cast<Handle*>(h)->err.c_str(); }
+
+int orc_init_mesh(void* hp, int edim, int nv, const double* X, int ne, const int* elems, double density) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.init(edim, nv, X, ne, elems, density); });
+}
+int orc_add_pd_energy(void* hp, const char* type, const double* p, int np, const int* idx, int ni) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_pd_energy(type, orc::Vec(p, p + np), std::vector<int>(idx, idx + ni)); });
+}
+int orc_set_dirichlet(void* hp, int dof, double val) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.dirichlet[dof] = val; h->body.pd_ready = false; });
+}
+int orc_add_gravity(void* hp, const double* g) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.gravity.assign(g, g + 3); });
+}
+int orc_set_frictional_boundary(void* hp, const char* type, const double* p, int np, const int* idx, int ni) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.set_boundary(type, orc::Vec(p, p + np), std::vector<int>(idx, idx + ni)); });
+}
+int orc_set_press_plane(void* hp, double z) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.boundary.plane_z = z; });
+}
+int orc_dofs(void* hp) { return static_cast<Handle*>(hp)->body.dofs; }
+int orc_num_energies(void* hp) { return (int)static_cast<Handle*>(hp)->body.energies.size(); }
+int orc_num_elements(void* hp) { return static_cast<Handle*>(hp)->body.ne; }
+
+int orc_forward(void* hp, const double* q, const double* v, const double* f_ext, double dt, const char* const* keys,
+                const double* vals, int nopt, double* q_next, double* v_next, int* active, int* n_active, int cap) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        orc::Vec qn, vn;
+        std::vector<int> act;
+        h->body.forward(orc::Vec(q, q + n), orc::Vec(v, v + n), orc::Vec(f_ext, f_ext + n), dt, make_opts(keys, vals, nopt),
+                        qn, vn, act);
+        std::copy(qn.begin(), qn.end(), q_next);
+        std::copy(vn.begin(), vn.end(), v_next);
+        orc::check((int)act.size() <= cap, "active buffer too small");
+        std::copy(act.begin(), act.end(), active);
+        *n_active = (int)act.size();
+    });
+}
+int orc_backward(void* hp, const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                 const double* v_next, const int* active, int n_active, const double* dl_dq_next, const double* dl_dv_next,
+                 const char* const* keys, const double* vals, int nopt, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                 double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        orc::Vec a, b, c, d, e, f;
+        h->body.backward(orc::Vec(q, q + n), orc::Vec(v, v + n), orc::Vec(f_ext, f_ext + n), dt, orc::Vec(q_next, q_next + n),
+                         orc::Vec(v_next, v_next + n), std::vector<int>(active, active + n_active),
+                         orc::Vec(dl_dq_next, dl_dq_next + n), orc::Vec(dl_dv_next, dl_dv_next + n),
+                         make_opts(keys, vals, nopt), a, b, c, d, e, f);
+        std::copy(a.begin(), a.end(), dl_dq);
+        std::copy(b.begin(), b.end(), dl_dv);
+        std::copy(c.begin(), c.end(), dl_df_ext);
+        std::copy(d.begin(), d.end(), dl_dmat_w);
+        std::copy(e.begin(), e.end(), dl_dstate_p);
+        std::copy(f.begin(), f.end(), dl_dw_elem);
+    });
+}
+// x = A_d^-1 b per axis on an AoS 3N vector (solver set up for dt).
+int orc_apply_ainv(void* hp, double dt, const double* b, double* x) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        h->body.setup(dt);
+        const int nv = h->body.nv;
+        for (int d = 0; d < 3; ++d) {
+            orc::Vec bd(nv);
+            for (int i = 0; i < nv; ++i) bd[i] = b[3 * i + d];
+            const orc::Vec xd = h->body.solve_axis(d, bd);
+            for (int i = 0; i < nv; ++i) x[3 * i + d] = xd[i];
+        }
+    });
+}
+int orc_local_step(void* hp, const double* q, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const orc::Vec r = h->body.local_step(orc::Vec(q, q + h->body.dofs));
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+// Per-element K_pd blocks (ne x 3edim x 3edim, row-major).
+int orc_local_differential(void* hp, const double* q, int exact, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const std::vector<double> K = h->body.local_differential(orc::Vec(q, q + h->body.dofs), exact != 0);
+        std::copy(K.begin(), K.end(), out);
+    });
+}
+int orc_frame_stats(void* hp, int* iters, int* conv, int cap) {
+    Handle* h = static_cast<Handle*>(hp);
+    const int n = (int)h->body.frame_iters.size();
+    for (int i = 0; i < n && i < cap; ++i) { iters[i] = h->body.frame_iters[i]; conv[i] = h->body.frame_converged[i]; }
+    return n;
+}
+void orc_reset_frame_stats(void* hp) {
+    Handle* h = static_cast<Handle*>(hp);
+    h->body.frame_iters.clear();
+    h->body.frame_converged.clear();
+}
+int orc_last_backward_iters(void* hp) { return static_cast<Handle*>(hp)->body.last_bwd_iters; }
+
+// --- element-level primitives for unit tests -------------------------------
+void orc_svd3(const double* F, double* U, double* sig, double* V) {
+    orc::M3 f, u, v;
+    std::memcpy(f.a, F, sizeof(double) * 9);
+    orc::svd3(f, u, sig, v);
+    std::memcpy(U, u.a, sizeof(double) * 9);
+    std::memcpy(V, v.a, sizeof(double) * 9);
+}
+// type: 0 corotated, 1 volume, 2 neohookean (stiffness k, mu, la).
+void orc_project(int type, double k, double mu, double la, const double* F, double* out) {
+    orc::Energy en;
+    en.type = type; en.stiffness = k; en.mu = mu; en.la = la;
+    orc::M3 f;
+    std::memcpy(f.a, F, sizeof(double) * 9);
+    orc::FAux a;
+    a.init(f);
+    const orc::M3 p = orc::project(en, a);
+    std::memcpy(out, p.a, sizeof(double) * 9);
+}
+void orc_project_diff(int type, double k, double mu, double la, const double* F, const double* dF, double* out) {
+    orc::Energy en;
+    en.type = type; en.stiffness = k; en.mu = mu; en.la = la;
+    orc::M3 f, df;
+    std::memcpy(f.a, F, sizeof(double) * 9);
+    std::memcpy(df.a, dF, sizeof(double) * 9);
+    orc::FAux a;
+    a.init(f);
+    const orc::M3 p = orc::project_diff(en, a, df);
+    std::memcpy(out, p.a, sizeof(double) * 9);
+}
+void orc_nh_prox(double k, double mu, double la, const double* sF, double* s) {
+    orc::Energy en;
+    en.type = orc::kNeoHookean; en.stiffness = k; en.mu = mu; en.la = la;
+    orc::nh_solve_prox(en, sF, s);
+}
+}
