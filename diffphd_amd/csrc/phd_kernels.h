@@ -21,8 +21,8 @@ struct DevMaterials {
 
 struct DotArgs {
     int m = 0;
-    const double* a[16];
-    const double* b[16];
+    const double* a[32];
+    const double* b[32];
 };
 
 // Boundary (frictional obstacle) description for the contact kernels.

@@ -344,18 +344,18 @@ constexpr int DOT_BLOCKS = 512;
 constexpr int DOT_THREADS = 256;
 
 __global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial) {
-    __shared__ double sh[DOT_THREADS / WAVE][16];
-    double acc[16];
+    __shared__ double sh[DOT_THREADS / WAVE][32];
+    double acc[32];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
+        for (int i = 0; i < 32; ++i)
             if (i < a.m) acc[i] += a.a[i][idx] * a.b[i][idx];
     }
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < 32; ++i) {
         if (i < a.m) {
             const double s = wave_sum(acc[i]);
             if (lane == 0) sh[wid][i] = s;
@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* 
     if (threadIdx.x < a.m) {
         double s = 0;
         for (int w = 0; w < DOT_THREADS / WAVE; ++w) s += sh[w][threadIdx.x];
-        partial[blockIdx.x * 16 + threadIdx.x] = s;
+        partial[blockIdx.x * 32 + threadIdx.x] = s;
     }
 }
 
@@ -373,7 +373,7 @@ __global__ void k_dots_fin(int m, const double* __restrict__ partial, double* __
     const int i = threadIdx.x;
     if (i >= m) return;
     double s = 0;
-    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b * 16 + i];
+    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b * 32 + i];
     out[i] = s;
 }
 
@@ -887,7 +887,7 @@ void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const 
                      const double* q, const unsigned char* dmask, double* r) {
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, Ap, Ai, Ax, b, q, dmask, r);
 }
-int dots_partial_size() { return DOT_BLOCKS * 16; }
+int dots_partial_size() { return DOT_BLOCKS * 32; }
 void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {
     hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial);
     hipLaunchKernelGGL(k_dots_fin, dim3(1), dim3(64), 0, s, args.m, partial, out);

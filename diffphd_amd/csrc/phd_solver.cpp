@@ -462,7 +462,7 @@ void Body::setup(double dt, int threads) {
         b->alloc(dofs_);
     d_elem_.alloc((size_t)ne_ * edim_ * 3);
     d_dot_partial_.alloc(dots_partial_size());
-    d_dot_out_.alloc(16);
+    d_dot_out_.alloc(32);
     w32_.alloc((size_t)3 * nv_); y32_.alloc((size_t)3 * nv_);
     w64_.alloc((size_t)3 * nv_); y64_.alloc((size_t)3 * nv_);
     part32_.alloc(3 * plan_.ru_row.size() + 3); part64_.alloc(3 * plan_.ru_row.size() + 3);
@@ -619,7 +619,7 @@ void Body::local_rhs(const double* q, double inv_h2m) {
 std::vector<double> Body::dots(const std::vector<std::pair<const double*, const double*>>& pairs) {
     DotArgs a;
     a.m = (int)pairs.size();
-    check(a.m <= 16, "too many dot products");
+    check(a.m <= 32, "too many dot products");
     for (int i = 0; i < a.m; ++i) { a.a[i] = pairs[i].first; a.b[i] = pairs[i].second; }
     launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_out_.p);
     std::vector<double> out(a.m);
@@ -694,7 +694,7 @@ struct DevAnderson {
     DevBuf<double> dQ[7], dG[7];
     int n = 0;
     void init(int w, int n_, DevBuf<double>* xs, DevBuf<double>* gs) {
-        window = std::min(w, 7);
+        window = std::min(w, 6);
         n = n_;
         X = xs;
         Gs = gs;
@@ -722,6 +722,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     const int verbose = (int)opt_req(opt, "verbose");
     const int threads = (int)opt_req(opt, "thread_ct");
     const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    check(aa_window >= 0 && aa_window <= 6, "aa_window must be in [0, 6].");
     const double fmu = opt_or(opt, "friction_mu", 0.0);
     const double restitution = opt_or(opt, "restitution", 0.0);
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
@@ -1035,6 +1036,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     }
     check(from_tape || active.empty(), "Backward needs the forward tape of this frame; the fixed-active-set replay is not supported.");
     const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    check(aa_window >= 0 && aa_window <= 6, "aa_window must be in [0, 6].");
     const double clip = opt_or(opt, "alg_phd_bwd_clip_norm", 1e6);
     DevBuf<double> d_rhs, d_s, d_sprev, d_zero;
     d_rhs.upload(rhs_qbar);

@@ -1,0 +1,46 @@
+"""Shared helpers: run one scene on a body (product or oracle)."""
+import numpy as np
+
+from diffphd_amd import scenes
+from diffphd_amd.py_diff_pd_core import StdRealVector
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def product_body(spec):
+    from diffphd_amd.py_diff_pd_core import HexDeformable, TetDeformable
+    body = HexDeformable() if spec["edim"] == 8 else TetDeformable()
+    return scenes.apply(spec, body)
+
+
+def oracle_body(spec):
+    import oracle
+    return scenes.apply(spec, oracle.OracleDeformable(spec["edim"]))
+
+
+def rollout(body, spec, frames, opt=None):
+    opt = dict(spec["options"] if opt is None else opt)
+    dofs = body.dofs()
+    q, v, acts = [np.asarray(spec["q0"], float)], [np.asarray(spec["v0"], float)], [[]]
+    f = np.zeros(dofs)
+    for i in range(frames):
+        if spec.get("plane_schedule") is not None:
+            body.SetPressPlaneHeight(spec["plane_schedule"][i])
+        qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
+        body.PyForward("pd_eigen_alg_phd", q[-1], v[-1], [], f, spec["dt"], opt, qn, vn, act)
+        q.append(np.asarray(qn))
+        v.append(np.asarray(vn))
+        acts.append(list(act))
+    return q, v, acts
+
+
+def backward_frame(body, spec, q, v, acts, i, dq, dv, opt=None):
+    opt = dict(spec["options"] if opt is None else opt)
+    o = [StdRealVector(0) for _ in range(8)]
+    body.PyBackward("pd_eigen_alg_phd", q[i], v[i], [], np.zeros(body.dofs()), spec["dt"], q[i + 1], v[i + 1],
+                    acts[i + 1], dq, dv, opt, *o)
+    return [np.asarray(x) for x in o]

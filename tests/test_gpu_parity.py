@@ -1,0 +1,71 @@
+"""HIP path (libphd.so through the C ABI) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from diffphd_amd import scenes
+from helpers import backward_frame, oracle_body, product_body, rel_err, rollout
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cantilever_pair():
+    spec = scenes.cantilever()
+    return spec, product_body(spec), oracle_body(spec)
+
+
+def test_apply_ainv_fp64_fp32(cantilever_pair):
+    spec, prod, orc = cantilever_pair
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(prod.dofs())
+    ref = orc.apply_ainv(spec["dt"], b)
+    x64 = prod.apply_ainv(spec["dt"], b, precision=0)
+    x32 = prod.apply_ainv(spec["dt"], b, precision=1)
+    x32r = prod.apply_ainv(spec["dt"], b, precision=1, refine=2)
+    print("fp64", rel_err(x64, ref), "fp32", rel_err(x32, ref), "fp32+IR", rel_err(x32r, ref))
+    assert rel_err(x64, ref) < 1e-12
+    assert rel_err(x32, ref) < 1e-5
+    assert rel_err(x32r, ref) < 1e-12
+
+
+def test_forward_cantilever_fp64(cantilever_pair):
+    spec, prod, orc = cantilever_pair
+    qp, vp, _ = rollout(prod, spec, 3)
+    qo, vo, _ = rollout(orc, spec, 3)
+    for i in range(1, 4):
+        print(i, rel_err(qp[i], qo[i]), rel_err(vp[i], vo[i]))
+        assert rel_err(qp[i], qo[i]) < 1e-8
+        assert rel_err(vp[i], vo[i]) < 1e-8
+
+
+def test_forward_cantilever_mixed(cantilever_pair):
+    spec, prod, orc = cantilever_pair
+    opt = dict(spec["options"], phd_precision=1)
+    qp, vp, _ = rollout(prod, spec, 3, opt)
+    qo, vo, _ = rollout(orc, spec, 3)
+    for i in range(1, 4):
+        print(i, rel_err(qp[i], qo[i]), rel_err(vp[i], vo[i]))
+        assert rel_err(qp[i], qo[i]) < 1e-5
+        assert rel_err(vp[i], vo[i]) < 1e-5
+
+
+def test_backward_cantilever_fp64(cantilever_pair):
+    spec, prod, orc = cantilever_pair
+    qp, vp, ap = rollout(prod, spec, 2)
+    qo, vo, ao = rollout(orc, spec, 2)
+    g = spec["loss_g"]
+    outp = backward_frame(prod, spec, qp, vp, ap, 1, g, np.zeros_like(g))
+    outo = backward_frame(orc, spec, qo, vo, ao, 1, g, np.zeros_like(g))
+    for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_df_ext", 3), ("dl_dmat_w", 4), ("dl_dstate_p", 6)):
+        print(name, rel_err(outp[k], outo[k]))
+        assert rel_err(outp[k], outo[k]) < 1e-8, name
+
+
+def test_napkin_contact_forward_fp64():
+    spec = scenes.napkin()
+    prod, orc = product_body(spec), oracle_body(spec)
+    qp, vp, ap = rollout(prod, spec, 40)
+    qo, vo, ao = rollout(orc, spec, 40)
+    for i in (10, 20, 30, 40):
+        print(i, rel_err(qp[i], qo[i]), len(ap[i]), len(ao[i]))
+    assert rel_err(qp[40], qo[40]) < 1e-6

@@ -1,0 +1,256 @@
+"""Pins the FP64 oracle with closed forms, invariants, finite differences and
+the reference's own known answer (the napkin scene of README.md:214-219)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from diffphd_amd import scenes
+from diffphd_amd.env import simulate
+from diffphd_amd.hex_mesh import hex_grid
+from helpers import oracle_body, rel_err, rollout
+
+
+def random_F(rng, kind):
+    if kind == "general":
+        return np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    if kind == "inverted":
+        F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+        F[:, 0] *= -1
+        return F
+    if kind == "repeated":
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        return Q @ np.diag([1.3, 1.3, 0.7]) @ Q.T
+    if kind == "nearly_singular":
+        U, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        V, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        return U @ np.diag([1.5, 0.8, 1e-6]) @ V.T
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["general", "inverted", "repeated", "nearly_singular"])
+def test_svd3_reconstruction_and_conventions(oracle_mod, kind):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        F = random_F(rng, kind)
+        U, s, V = oracle_mod.svd3(F)
+        assert np.allclose(U @ np.diag(s) @ V.T, F, atol=1e-13)
+        assert np.allclose(U.T @ U, np.eye(3), atol=1e-13)
+        assert np.allclose(V.T @ V, np.eye(3), atol=1e-13)
+        assert np.all(np.diff(s) <= 0) and s[-1] >= 0
+        assert np.allclose(s, np.linalg.svd(F, compute_uv=False), rtol=1e-12, atol=1e-14)
+
+
+def test_corotated_projection_is_polar_factor(oracle_mod):
+    rng = np.random.default_rng(4)
+    for kind in ("general", "inverted"):
+        for _ in range(10):
+            F = random_F(rng, kind)
+            R = oracle_mod.project("corotated", F)
+            u, p = scipy.linalg.polar(F)  # F = u p, p symmetric positive semidefinite
+            assert np.allclose(R, u, atol=1e-12)
+
+
+def test_volume_projection_kkt(oracle_mod):
+    """min ||sig* - sig|| s.t. prod sig* = 1: feasibility and D parallel to grad C."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        F = random_F(rng, "general")
+        P = oracle_mod.project("volume", F)
+        U, s, V = oracle_mod.svd3(F)
+        sp = np.diag(U.T @ P @ V)
+        assert abs(np.prod(sp) - 1) < 1e-12
+        D = sp - s
+        g = np.array([sp[1] * sp[2], sp[0] * sp[2], sp[0] * sp[1]])
+        assert np.linalg.norm(np.cross(D, g)) <= 1e-10 * max(np.linalg.norm(D), 1e-12) * np.linalg.norm(g) + 1e-14
+
+
+def test_nh_prox_stationarity(oracle_mod):
+    rng = np.random.default_rng(6)
+    mu, la = 3.4e4, 1.4e5
+    k = 2 * mu
+    for _ in range(30):
+        sF = rng.uniform(0.6, 1.5, 3)
+        s = oracle_mod.nh_prox(sF, k, mu, la)
+        L = np.log(np.prod(s))
+        g = mu * s - mu / s + la * L / s + k * (s - sF)
+        assert np.linalg.norm(g) < 1e-9 * (mu + la)
+
+
+@pytest.mark.parametrize("kind", ["corotated", "volume", "pd_neohookean"])
+def test_projection_differential_matches_central_fd(oracle_mod, kind):
+    rng = np.random.default_rng(7)
+    mu, la = 3.4e4, 1.4e5
+    for _ in range(8):
+        F = random_F(rng, "general")
+        dF = rng.standard_normal((3, 3))
+        eps = 1e-6
+        fd = (oracle_mod.project(kind, F + eps * dF, 2 * mu, mu, la)
+              - oracle_mod.project(kind, F - eps * dF, 2 * mu, mu, la)) / (2 * eps)
+        an = oracle_mod.project_diff(kind, F, dF, 2 * mu, mu, la)
+        assert rel_err(an, fd) < 1e-6, kind
+
+
+def _dense_A(spec, dt):
+    """Independent numpy assembly of the scalar PD matrix (forward.cpp:8-110 semantics)."""
+    V, E = spec["V"], spec["E"]
+    n, ne = V.shape[1], E.shape[1]
+    vol = 0.01 ** 3
+    dx = vol ** (1 / 3)
+    r = 1 / np.sqrt(3)
+    mass = spec["density"] * vol
+    w = 2 * spec["mu"] + spec["la"]
+    dmask = np.zeros(n, bool)
+    for dof, _ in spec["dirichlet"]:
+        dmask[dof // 3] = True
+    # trilinear shape-function gradients at the 8 Gauss points
+    Ke = np.zeros((8, 8))
+    for s in range(8):
+        bits = [(s >> 2) & 1, (s >> 1) & 1, s & 1]
+        xi = [((b - 0.5) * r + 0.5) for b in bits]
+        G = np.zeros((8, 3))
+        for k in range(8):
+            kb = [(k >> 2) & 1, (k >> 1) & 1, k & 1]
+            for j in range(3):
+                val = 1.0
+                for a in range(3):
+                    if a == j:
+                        val *= (1 if kb[a] else -1) / dx
+                    else:
+                        val *= xi[a] if kb[a] else 1 - xi[a]
+                G[k, j] = val
+        Ke += (vol / 8) * G @ G.T
+    A = np.zeros((n, n))
+    for e in range(ne):
+        idx = E[:, e]
+        for a in range(8):
+            for b in range(8):
+                if not dmask[idx[a]] and not dmask[idx[b]]:
+                    A[idx[a], idx[b]] += w * Ke[a, b]
+    A[np.diag_indices(n)] += np.where(dmask, 1.0, mass / dt ** 2)
+    return A
+
+
+def test_sparse_ldlt_solve_matches_dense_assembly(oracle_mod):
+    spec = scenes.cantilever(cells=(4, 2, 2))
+    orc = oracle_body(spec)
+    A = _dense_A(spec, spec["dt"])
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal(orc.dofs())
+    x = orc.apply_ainv(spec["dt"], b)
+    X = x.reshape(-1, 3)
+    for d in range(3):
+        assert rel_err(A @ X[:, d], b.reshape(-1, 3)[:, d]) < 1e-12
+
+
+def test_local_step_is_pd_energy_gradient(oracle_mod):
+    """sum_e w A^T vec(F) - pd_rhs = grad of sum_e w/2 ||F - p(F)||^2 (projections are closest points)."""
+    spec = scenes.cantilever(cells=(2, 1, 1))
+    spec["dirichlet"] = []
+    orc = oracle_body(spec)
+    A = _dense_A(spec, 1.0)
+    n = orc.dofs() // 3
+    mass = spec["density"] * 1e-6
+    K = A - mass * np.eye(n)  # sum w_e (vol/s) A^T A per axis
+    rng = np.random.default_rng(9)
+    q = spec["q0"] + 2e-3 * rng.standard_normal(3 * n)
+    mu, la = spec["mu"], spec["la"]
+    V, E = spec["V"], spec["E"]
+    dx = 0.01
+    r = 1 / np.sqrt(3)
+
+    def energy(qv):
+        tot = 0.0
+        X = qv.reshape(-1, 3)
+        for e in range(E.shape[1]):
+            xe = X[E[:, e]]
+            for s in range(8):
+                bits = [(s >> 2) & 1, (s >> 1) & 1, s & 1]
+                xi = [((b - 0.5) * r + 0.5) for b in bits]
+                G = np.zeros((8, 3))
+                for k in range(8):
+                    kb = [(k >> 2) & 1, (k >> 1) & 1, k & 1]
+                    for j in range(3):
+                        val = 1.0
+                        for a in range(3):
+                            val *= ((1 if kb[a] else -1) / dx) if a == j else (xi[a] if kb[a] else 1 - xi[a])
+                        G[k, j] = val
+                F = xe.T @ G
+                for kind, w in (("corotated", 2 * mu), ("volume", la)):
+                    P = oracle_mod.project(kind, F)
+                    tot += 0.5 * w * (1e-6 / 8) * np.sum((F - P) ** 2)
+        return tot
+
+    pd_rhs = orc.local_step(q)
+    Qm = q.reshape(-1, 3)
+    grad_from_rhs = (K @ Qm).ravel() - pd_rhs
+    eps = 1e-7
+    dirn = rng.standard_normal(3 * n)
+    fd = (energy(q + eps * dirn) - energy(q - eps * dirn)) / (2 * eps)
+    assert abs(fd - grad_from_rhs @ dirn) < 1e-5 * abs(fd) + 1e-12
+
+
+def test_anderson_window_zero_reaches_same_fixed_point(oracle_mod):
+    spec = scenes.cantilever(cells=(6, 2, 2))
+    o1, o2 = oracle_body(spec), oracle_body(spec)
+    opt = dict(spec["options"], rel_tol=1e-11, max_pd_iter=20000)
+    q1, _, _ = rollout(o1, spec, 2, dict(opt, aa_window=5))
+    q2, _, _ = rollout(o2, spec, 2, dict(opt, aa_window=0))
+    assert rel_err(q1[2], q2[2]) < 1e-8
+
+
+def test_napkin_known_answer(oracle_mod):
+    """README.md:214-216: AlgPhd lands and bends with max z_std after landing = 0.06671."""
+    spec = scenes.napkin()
+    orc = oracle_body(spec)
+    _, _, info = simulate(orc, spec["dt"], spec["frames"], "pd_eigen_alg_phd", spec["options"], spec["q0"],
+                          spec["v0"], lambda q, v: (0.0, 0 * q, 0 * v), require_grad=False)
+    z = np.array(info["q"]).reshape(len(info["q"]), -1, 3)[:, :, 2]
+    landed = z[spec["frames"] - 35:].std(axis=1).max()
+    print("max z_std after landing", landed)
+    assert abs(landed - 0.06671) < 5e-6
+    # classifier of napkin_3d_test.py:61-81
+    assert np.isfinite(z).all() and np.abs(z).max() < 5.0 and landed >= 0.025
+
+
+def test_cantilever_gradients_match_central_fd(oracle_mod):
+    """Adjoint gradients of loss = g . q_T w.r.t. v0 and per-element mu (one
+    corotated energy per element: reference layout) against central FD."""
+    spec = scenes.cantilever(cells=(4, 2, 2), per_element_energies=True)
+    frames = 4
+    opt = dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000)
+    g = spec["loss_g"]
+
+    def run(v0=None, mu_scale=None):
+        s = dict(spec)
+        if v0 is not None:
+            s["v0"] = v0
+        if mu_scale is not None:
+            e, f = mu_scale
+            en = list(s["energies"])
+            kind, params, idx = en[e]
+            en[e] = (kind, [params[0] * f], idx)
+            s["energies"] = en
+        body = oracle_body(s)
+        loss, grad, info = simulate(body, s["dt"], frames, "pd_eigen_alg_phd", opt, s["q0"], s["v0"],
+                                    lambda q, v: (float(g @ q), g.copy(), 0 * v), grad_clip_norm=None)
+        return loss, grad, info
+
+    loss, grad, info = run()
+    rng = np.random.default_rng(10)
+    dv = rng.standard_normal(g.size)
+    eps = 1e-3  # forward termination noise dominates below ~1e-4
+    lp = run(v0=spec["v0"] + eps * dv)[0]
+    lm = run(v0=spec["v0"] - eps * dv)[0]
+    fd = (lp - lm) / (2 * eps)
+    an = grad[1] @ dv
+    print("v0 directional", an, fd)
+    assert abs(an - fd) < 1e-5 * abs(fd)
+    for e in (0, 5, 11):
+        w = spec["energies"][e][1][0]
+        h = 1e-3
+        lp = run(mu_scale=(e, 1 + h))[0]
+        lm = run(mu_scale=(e, 1 - h))[0]
+        fd = (lp - lm) / (2 * h * w)
+        an = info["dl_dmat_w"][e]
+        print("w_e", e, an, fd)
+        assert abs(an - fd) < 1e-5 * abs(fd) + 1e-12
