@@ -38,6 +38,10 @@ def parse():
     return p.parse_args()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_steps(body, spec, opt, start_q, start_v, frame0, K):
     """K forward frames from (start_q, start_v) then the K backward frames."""
     from diffphd_amd.py_diff_pd_core import StdRealVector
@@ -51,6 +55,7 @@ def run_steps(body, spec, opt, start_q, start_v, frame0, K):
         q.append(np.asarray(qn))
         v.append(np.asarray(vn))
         acts.append(list(act))
+        log(f"forward frame {frame0 + i}: active contacts {len(act)}")
     n = q[-1].size
     dq = 2.0 * (q[-1] - spec["target"]) / n  # loss = mean squared distance to the target
     dv = np.zeros(n)
@@ -60,6 +65,7 @@ def run_steps(body, spec, opt, start_q, start_v, frame0, K):
         body.PyBackward("pd_eigen_alg_phd", q[i], v[i], [], f, spec["dt"], q[i + 1], v[i + 1], acts[i + 1], dq, dv,
                         opt, *o)
         dq, dv = np.asarray(o[0]), np.asarray(o[1])
+        log(f"backward frame {frame0 + i}: |dl_dq| {np.linalg.norm(dq):.3e}")
     return q, v, acts
 
 
@@ -68,9 +74,11 @@ def time_path(spec, precision, K, W, threads):
     from diffphd_amd.py_diff_pd_core import HexDeformable
     body = scenes.apply(spec, HexDeformable(0))
     opt = dict(spec["options"], phd_precision=1 if precision == "mixed" else 0, thread_ct=threads)
+    log(f"{precision}: building the factor")
     t0 = time.perf_counter()
     body.factor_build(spec["dt"], threads)
     t_setup = time.perf_counter() - t0
+    log(f"{precision}: setup {t_setup:.1f}s {body.factor_info()}")
     q, v, _ = run_steps(body, spec, opt, spec["q0"], spec["v0"], 0, W) if W > 0 else ([spec["q0"]], [spec["v0"]], None)
     body.reset_frame_stats()
     body.timing_enable(True)

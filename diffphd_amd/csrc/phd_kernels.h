@@ -111,6 +111,14 @@ void launch_symmetrize_lower(hipStream_t s, int K, double* H);
 void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
 
+// out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]
+void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
+                        const double* f, double* out);
+// contact system restricted to the index set J (row-major nJ x nJ)
+template <class TM>
+void launch_assemble_msys_idx(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
+                              const double* om, const double* E, TM* M);
+
 // ---- backward ----------------------------------------------------------------
 // per-sample 9x9 proximal-map Jacobians weighted by the K_pd weights
 template <class TW>

@@ -84,11 +84,17 @@ __global__ void __launch_bounds__(256) k_spmv_rows(int nunits, const int* __rest
     const TV* vr = V + base[k];
     TV a0 = 0, a1 = 0, a2 = 0;
     int c = c0 + lane;
-    for (; c + 3 * WAVE <= c1; c += 4 * WAVE) {
-        const TV v0 = vr[c], v1 = vr[c + WAVE], v2 = vr[c + 2 * WAVE], v3 = vr[c + 3 * WAVE];
-        a0 += v0 * w[c] + v1 * w[c + WAVE] + v2 * w[c + 2 * WAVE] + v3 * w[c + 3 * WAVE];
-        a1 += v0 * w[n + c] + v1 * w[n + c + WAVE] + v2 * w[n + c + 2 * WAVE] + v3 * w[n + c + 3 * WAVE];
-        a2 += v0 * w[2 * n + c] + v1 * w[2 * n + c + WAVE] + v2 * w[2 * n + c + 2 * WAVE] + v3 * w[2 * n + c + 3 * WAVE];
+    for (; c + 7 * WAVE <= c1; c += 8 * WAVE) {
+        TV v[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) v[s] = vr[c + s * WAVE];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int cc = c + s * WAVE;
+            a0 += v[s] * w[cc];
+            a1 += v[s] * w[n + cc];
+            a2 += v[s] * w[2 * n + cc];
+        }
     }
     for (; c <= c1; c += WAVE) {
         const TV v = vr[c];
@@ -138,31 +144,43 @@ __global__ void __launch_bounds__(256) k_spmv_cols(int nunits, const int* __rest
     const int j = 64 * t + lane;
     const int i0 = cu_i0[u], i1 = cu_i1[u];
     TV a0 = 0, a1 = 0, a2 = 0;
-    int it = i0;
-    for (; it + 3 < i1; it += 4) {
-        int kk[4];
-        TV v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) kk[q] = items[it + q];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = kk[q];
-            v[q] = (j >= fd[k] && j <= k) ? V[base[k] + j] : TV(0);
+    // The item metadata of 64 rows is loaded cooperatively (one row per lane)
+    // and broadcast with cross-lane reads, so the value loads of eight rows
+    // are in flight together without dependent scalar loads in between.
+    for (int it0 = i0; it0 < i1; it0 += WAVE) {
+        const int my = it0 + lane;
+        const int cnt = min(WAVE, i1 - it0);
+        int mk = -1, mf = 0x7fffffff;
+        int mblo = 0, mbhi = 0;
+        TV my0 = 0, my1 = 0, my2 = 0;
+        if (my < i1) {
+            mk = items[my];
+            mf = fd[mk];
+            const int64_t b = base[mk];
+            mblo = (int)(b & 0xffffffff);
+            mbhi = (int)(b >> 32);
+            my0 = y[mk];
+            my1 = y[n + mk];
+            my2 = y[2 * n + mk];
         }
+        for (int q = 0; q < cnt; q += 8) {
+            TV v[8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = kk[q];
-            a0 += v[q] * y[k];
-            a1 += v[q] * y[n + k];
-            a2 += v[q] * y[2 * n + k];
+            for (int s = 0; s < 8; ++s) {
+                const int src = q + s;
+                const int kk = __shfl(mk, src, WAVE);
+                const int ff = __shfl(mf, src, WAVE);
+                const int64_t bb = ((int64_t)__shfl(mbhi, src, WAVE) << 32) | (uint32_t)__shfl(mblo, src, WAVE);
+                v[s] = (src < cnt && j >= ff && j <= kk) ? V[bb + j] : TV(0);
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int src = q + s;
+                a0 += v[s] * __shfl(my0, src, WAVE);
+                a1 += v[s] * __shfl(my1, src, WAVE);
+                a2 += v[s] * __shfl(my2, src, WAVE);
+            }
         }
-    }
-    for (; it < i1; ++it) {
-        const int k = items[it];
-        const TV v = (j >= fd[k] && j <= k) ? V[base[k] + j] : TV(0);
-        a0 += v * y[k];
-        a1 += v * y[n + k];
-        a2 += v * y[2 * n + k];
     }
     TV* xp = xpart + (int64_t)u * 3 * WAVE;
     xp[lane] = a0;
@@ -814,6 +832,40 @@ __global__ void k_assemble_g(int nb, const int* __restrict__ cidx, const double*
     G[idx] = dd * H[(int64_t)cidx[a] * K + cidx[b]];
 }
 
+// out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]: the candidate block
+// of A^-1 applied to impulses supported on `cols` (one wavefront per row).
+__global__ void k_h_apply_idx(int nr, const int* __restrict__ rows, int nc, const int* __restrict__ cols,
+                              const double* __restrict__ H, int K, const double* __restrict__ f, double* __restrict__ out) {
+    const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (a >= nr) return;
+    const double* Hr = H + (int64_t)rows[a] * K;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int b = lane; b < nc; b += WAVE) {
+        const int c = cols[b];
+        const double h = Hr[c];
+        a0 += h * f[3 * c];
+        a1 += h * f[3 * c + 1];
+        a2 += h * f[3 * c + 2];
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+    if (lane == 0) { out[3 * a] = a0; out[3 * a + 1] = a1; out[3 * a + 2] = a2; }
+}
+// Compressed contact system on the index set J: M[a][b] = om_i (n_i . n_j) H[i][j] om_j + (a == b) E_i.
+template <class TM>
+__global__ void k_assemble_msys_idx(int nJ, const int* __restrict__ J, const double* __restrict__ H, int K,
+                                    const double* __restrict__ nrm, const double* __restrict__ om,
+                                    const double* __restrict__ E, TM* __restrict__ M) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nJ * nJ) return;
+    const int a = (int)(idx / nJ), b = (int)(idx % nJ);
+    const int i = J[a], j = J[b];
+    const double nd = nrm[3 * i] * nrm[3 * j] + nrm[3 * i + 1] * nrm[3 * j + 1] + nrm[3 * i + 2] * nrm[3 * j + 2];
+    double v = om[i] * nd * H[(int64_t)i * K + j] * om[j];
+    if (a == b) v += E[i];
+    M[idx] = (TM)v;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -962,6 +1014,22 @@ void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dir
     if (nb == 0) return;
     hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
 }
+void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
+                        const double* f, double* out) {
+    if (nr == 0) return;
+    hipLaunchKernelGGL(k_h_apply_idx, dim3(blocks_for(nr, 4)), dim3(256), 0, s, nr, rows, nc, cols, H, K, f, out);
+}
+template <class TM>
+void launch_assemble_msys_idx(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
+                              const double* om, const double* E, TM* M) {
+    if (nJ == 0) return;
+    hipLaunchKernelGGL(k_assemble_msys_idx<TM>, dim3(blocks_for((int64_t)nJ * nJ, 256)), dim3(256), 0, s, nJ, J, H, K,
+                       nrm, om, E, M);
+}
+template void launch_assemble_msys_idx<float>(hipStream_t, int, const int*, const double*, int, const double*,
+                                              const double*, const double*, float*);
+template void launch_assemble_msys_idx<double>(hipStream_t, int, const int*, const double*, int, const double*,
+                                               const double*, const double*, double*);
 void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r) {
     if (K == 0) return;
     hipLaunchKernelGGL(k_add_node_force, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, nodes, f, r);

@@ -372,7 +372,7 @@ void Body::setup(double dt, int threads) {
     }
     // factor (host FP64) and expansion of S = L^-1 (device FP64, FP32 copy)
     factor_.build(nv_, Ap_, Ai_, Ax_, X_, threads);
-    plan_.build(factor_, 2048, 512);
+    plan_.build(factor_, 4096, 512);
     {
         DevBuf<int64_t> dLp;
         DevBuf<int> dLi, dpar, ddep;
@@ -554,6 +554,7 @@ void Body::setup_contact(int threads) {
     Hdiag_ = Hdiag;
     for (auto* b : {&d_nrm_, &d_t1_, &d_t2_, &d_F_, &d_dF_, &d_uC_, &d_tmpK3_}) b->alloc((size_t)3 * K_);
     for (auto* b : {&d_delta_, &d_om_, &d_E_, &d_rhsK_}) b->alloc(K_);
+    for (auto* b : {&d_idxJ_, &d_idxP_, &d_idxF_}) b->alloc(K_);
     d_rhsK32_.alloc(K_);
     d_info_.alloc(1);
 }
@@ -636,7 +637,7 @@ void Body::kpd_apply(const double* x, double* out, int precision) {
 
 // Dense SPD solve on the device (rocSOLVER Cholesky). rhs (host, n) is
 // overwritten with the solution. The matrix is already in d_M64_/d_M32_.
-void Body::solve_dense_spd(int n, int precision, const double* /*unused*/, double* rhs) {
+bool Body::solve_dense_spd(int n, int precision, const double* /*unused*/, double* rhs) {
     rocblas_handle h = (rocblas_handle)blas_;
     int info = 0;
     if (precision == 1) {
@@ -656,8 +657,7 @@ void Body::solve_dense_spd(int n, int precision, const double* /*unused*/, doubl
         hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info D2H");
         hip_check(hipStreamSynchronize(stream_), "potrs sync");
     }
-    check(info == 0, "Dense Cholesky of the contact system failed (matrix not positive definite, info=" +
-                         std::to_string(info) + ").");
+    return info == 0;
 }
 
 // x = A^-1 b with optional FP64 refinement (test entry point; Dirichlet rows are identity).
@@ -765,6 +765,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         }
         hip_check(hipMemsetAsync(d_F_.p, 0, sizeof(double) * 3 * K, stream_), "F=0");
         if (precision == 1) d_M32_.alloc((size_t)K * K); else d_M64_.alloc((size_t)K * K);
+        if ((int)d_rhsK_.n < K) d_rhsK_.alloc(K);
     }
     DevAnderson aa;
     aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
@@ -786,14 +787,13 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         if (K > 0) {
             // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
-            launch_h_apply(stream_, K, d_H_.p, d_F_.p, d_tmpK3_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
             launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
-            launch_axpbypcz(stream_, 3 * K, 1.0, d_uC_.p, 1.0, d_dF_.p, -1.0, d_tmpK3_.p, d_uC_.p);
             hip_check(hipStreamSynchronize(stream_), "contact stage");
             d_uC_.download(uC.data(), 3 * K);
             d_delta_.download(delta.data(), K);
             d_dF_.download(qC.data(), 3 * K);
+            for (int i = 0; i < 3 * K; ++i) uC[i] += qC[i];
             // omega, E, h (forward_gpu.cpp:1160-1197)
             for (int c = 0; c < K; ++c) {
                 const double dj = delta[c], lj = lambda[c], rj = rprec[c];
@@ -806,37 +806,93 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 hvec[c] = -phi + om[c] * Hq;
             }
             omega_last = om;
-            // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2
-            for (int c = 0; c < K; ++c)
-                for (int d = 0; d < 3; ++d)
-                    Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
-            d_tmpK3_.upload(Fg);
-            launch_h_apply(stream_, K, d_H_.p, d_tmpK3_.p, d_dF_.p);
-            hip_check(hipStreamSynchronize(stream_), "HFg");
-            d_dF_.download(hv.data(), 3 * K);
+            // Candidates with omega == 0 (separated, lambda == 0) decouple exactly: their row of
+            // Omega W Omega + E is E_c e_c. The dense work runs on J = {omega != 0}; P adds the
+            // support of the previous impulses.
+            std::vector<int> Jv, Pv, Fs;
             for (int c = 0; c < K; ++c) {
-                double val = 0;
-                for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (uC[3 * c + d] + hv[3 * c + d]);
-                rhs[c] = hvec[c] - om[c] * val;
+                const bool inJ = om[c] != 0.0;
+                const bool inF = F[3 * c] != 0.0 || F[3 * c + 1] != 0.0 || F[3 * c + 2] != 0.0;
+                if (inJ) Jv.push_back(c);
+                if (inF) Fs.push_back(c);
+                if (inJ || inF) Pv.push_back(c);
             }
-            d_om_.upload(om);
-            d_E_.upload(E);
-            if (precision == 1) launch_assemble_msys<float>(stream_, K, d_H_.p, d_nrm_.p, d_om_.p, d_E_.p, d_M32_.p);
-            else launch_assemble_msys<double>(stream_, K, d_H_.p, d_nrm_.p, d_om_.p, d_E_.p, d_M64_.p);
-            solve_dense_spd(K, precision, nullptr, rhs.data());
-            for (int c = 0; c < K; ++c) lambda[c] = std::min(std::max(lambda[c] + rhs[c], 0.0), 1e8);
+            const int nJ = (int)Jv.size(), nP = (int)Pv.size(), nF = (int)Fs.size();
+            if (nF > 0) {
+                hip_check(hipMemcpyAsync(d_idxP_.p, Pv.data(), sizeof(int) * nP, hipMemcpyHostToDevice, stream_), "P");
+                hip_check(hipMemcpyAsync(d_idxF_.p, Fs.data(), sizeof(int) * nF, hipMemcpyHostToDevice, stream_), "Fs");
+                launch_h_apply_idx(stream_, nP, d_idxP_.p, nF, d_idxF_.p, d_H_.p, K, d_F_.p, d_tmpK3_.p);
+                hip_check(hipStreamSynchronize(stream_), "HF_prev");
+                d_tmpK3_.download(hv.data(), 3 * nP);
+                for (int a = 0; a < nP; ++a)
+                    for (int d = 0; d < 3; ++d) uC[3 * Pv[a] + d] -= hv[3 * a + d];
+            }
+            for (int c = 0; c < K; ++c) rhs[c] = hvec[c];
+            if (nJ > 0) {
+                // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2 (supported on J)
+                std::fill(Fg.begin(), Fg.end(), 0.0);
+                for (int c : Jv)
+                    for (int d = 0; d < 3; ++d)
+                        Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
+                d_dF_.upload(Fg);
+                hip_check(hipMemcpyAsync(d_idxJ_.p, Jv.data(), sizeof(int) * nJ, hipMemcpyHostToDevice, stream_), "J");
+                launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nJ, d_idxJ_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
+                hip_check(hipStreamSynchronize(stream_), "HFg");
+                d_tmpK3_.download(hv.data(), 3 * nJ);
+                std::vector<double> rJ(nJ);
+                for (int a = 0; a < nJ; ++a) {
+                    const int c = Jv[a];
+                    double val = 0;
+                    for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (uC[3 * c + d] + hv[3 * a + d]);
+                    rJ[a] = hvec[c] - om[c] * val;
+                }
+                d_om_.upload(om);
+                d_E_.upload(E);
+                std::vector<double> rJ_keep = rJ;
+                bool solved = false;
+                if (precision == 1) {
+                    launch_assemble_msys_idx<float>(stream_, nJ, d_idxJ_.p, d_H_.p, K, d_nrm_.p, d_om_.p, d_E_.p, d_M32_.p);
+                    solved = solve_dense_spd(nJ, 1, nullptr, rJ.data());
+                }
+                if (!solved) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
+                    if (d_M64_.n < (size_t)nJ * nJ) d_M64_.alloc((size_t)K * K);
+                    rJ = rJ_keep;
+                    launch_assemble_msys_idx<double>(stream_, nJ, d_idxJ_.p, d_H_.p, K, d_nrm_.p, d_om_.p, d_E_.p, d_M64_.p);
+                    check(solve_dense_spd(nJ, 0, nullptr, rJ.data()),
+                          "Dense Cholesky of the contact system failed (matrix not positive definite).");
+                }
+                for (int a = 0; a < nJ; ++a) rhs[Jv[a]] = rJ[a];
+            }
+            for (int c = 0; c < K; ++c) {
+                const double dl = (om[c] != 0.0) ? rhs[c] : rhs[c] / E[c];
+                lambda[c] = std::min(std::max(lambda[c] + dl, 0.0), 1e8);
+            }
             // NCP-integrated Coulomb friction (:1255-1302)
             if (do_fric) {
-                for (int c = 0; c < K; ++c)
+                std::vector<int> Lp_, Jn;
+                for (int c = 0; c < K; ++c) {
+                    if (lambda[c] > 0.0) Lp_.push_back(c);
+                    if (om[c] * lambda[c] != 0.0) Jn.push_back(c);
+                }
+                std::vector<int> slot(K, -1);
+                std::fill(Fn.begin(), Fn.end(), 0.0);
+                for (int c : Jn)
                     for (int d = 0; d < 3; ++d) Fn[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d];
-                d_tmpK3_.upload(Fn);
-                launch_h_apply(stream_, K, d_H_.p, d_tmpK3_.p, d_dF_.p);
-                hip_check(hipStreamSynchronize(stream_), "HFn");
-                d_dF_.download(hv.data(), 3 * K);
+                if (!Lp_.empty() && !Jn.empty()) {
+                    d_dF_.upload(Fn);
+                    hip_check(hipMemcpyAsync(d_idxP_.p, Lp_.data(), sizeof(int) * Lp_.size(), hipMemcpyHostToDevice, stream_), "L+");
+                    hip_check(hipMemcpyAsync(d_idxF_.p, Jn.data(), sizeof(int) * Jn.size(), hipMemcpyHostToDevice, stream_), "Jn");
+                    launch_h_apply_idx(stream_, (int)Lp_.size(), d_idxP_.p, (int)Jn.size(), d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
+                    hip_check(hipStreamSynchronize(stream_), "HFn");
+                    d_tmpK3_.download(hv.data(), 3 * Lp_.size());
+                } else {
+                    std::fill(hv.begin(), hv.end(), 0.0);
+                }
+                for (size_t a = 0; a < Lp_.size(); ++a) slot[Lp_[a]] = (int)a;
                 for (int c = 0; c < K; ++c) {
                     if (lambda[c] <= 0.0) { lt1[c] = lt2[c] = 0.0; continue; }
                     double step[3];
-                    for (int d = 0; d < 3; ++d) step[d] = uC[3 * c + d] + hv[3 * c + d] - q_frame_C[3 * c + d];
+                    for (int d = 0; d < 3; ++d) step[d] = uC[3 * c + d] + hv[3 * slot[c] + d] - q_frame_C[3 * c + d];
                     const double* n = &nrm[3 * c];
                     const double sn = step[0] * n[0] + step[1] * n[1] + step[2] * n[2];
                     double ut[3];

@@ -199,6 +199,7 @@ private:
     DevBuf<double> d_rhsK_;
     DevBuf<float> d_rhsK32_;
     DevBuf<int> d_info_;
+    DevBuf<int> d_idxJ_, d_idxP_, d_idxF_;
     void* blas_ = nullptr;
     std::vector<hipEvent_t> ev_pool_;
     size_t ev_used_ = 0;
@@ -228,7 +229,7 @@ private:
     void local_rhs(const double* q, double inv_h2m);
     void kpd_apply(const double* x, double* out, int precision);
     std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
-    void solve_dense_spd(int n, int precision, const double* M64_host_or_null, double* rhs_inout);
+    bool solve_dense_spd(int n, int precision, const double* M64_host_or_null, double* rhs_inout);
 };
 
 }  // namespace phd

@@ -1094,7 +1094,7 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
     const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
     const double abs_tol = opt_req(opt, "abs_tol");
     const double rel_tol = opt_req(opt, "rel_tol");
-    opt_req(opt, "verbose");
+    const int verbose = (int)opt_req(opt, "verbose");
     opt_req(opt, "thread_ct");
     const int aa_window = (int)opt_or(opt, "aa_window", 5);
     check((int)q.size() == dofs && (int)v.size() == dofs && (int)f_ext.size() == dofs, "Inconsistent state size.");
@@ -1295,6 +1295,7 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
         for (int i = 0; i < dofs; ++i) { dq[i] = q_sol[i] - q_prev[i]; dr[i] = pd_rhs[i] - prev_pd_rhs[i]; }
         const double abs_err = norm(dq), rhs_norm = norm(q_prev), rc = norm(dr), rr = norm(prev_pd_rhs);
         prev_pd_rhs = pd_rhs;
+        if (verbose > 0) fprintf(stderr, "alg_phd PD iter %d  dq=%g  dpd_rhs=%g\n", iter, abs_err, rc);
         if (iter > 0 && abs_err <= rel_tol * rhs_norm + abs_tol && rc <= rel_tol * rr + abs_tol) { converged = true; break; }
     }
     frame_iters.push_back(iters);

@@ -69,3 +69,42 @@ def test_napkin_contact_forward_fp64():
     for i in (10, 20, 30, 40):
         print(i, rel_err(qp[i], qo[i]), len(ap[i]), len(ao[i]))
     assert rel_err(qp[40], qo[40]) < 1e-6
+
+
+def test_press_small_forward_fp64_and_mixed():
+    """Heightfield ground + descending press plane with friction (8x8x4 hex)."""
+    spec = scenes.press(cells=(8, 8, 4))
+    opt = dict(spec["options"], rel_tol=1e-5, max_pd_iter=400)
+    prod, orc = product_body(spec), oracle_body(spec)
+    qo, vo, ao = rollout(orc, spec, 3, opt)
+    qp, vp, ap = rollout(prod, spec, 3, opt)
+    for i in range(1, 4):
+        print("fp64", i, rel_err(qp[i], qo[i]), ap[i], ao[i])
+    assert ap[3] == ao[3]
+    assert rel_err(qp[3], qo[3]) < 1e-7
+    prod2 = product_body(spec)
+    qm, vm, am = rollout(prod2, spec, 3, dict(opt, phd_precision=1))
+    print("mixed", rel_err(qm[3], qo[3]))
+    assert rel_err(qm[3], qo[3]) < 1e-5
+
+
+def test_napkin_backward_with_contact_basis_fp64():
+    """Adjoint through NCP contact frames: backbone + contact-basis correction
+    (reference Woodbury, backward_gpu.cpp:922-1208) against the oracle."""
+    spec = scenes.napkin()
+    fwd = dict(spec["options"], rel_tol=1e-10, abs_tol=1e-14)
+    bwd = dict(fwd)
+    prod, orc = product_body(spec), oracle_body(spec)
+    qp, vp, ap = rollout(prod, spec, 45, fwd)
+    qo, vo, ao = rollout(orc, spec, 45, fwd)
+    assert ap[45] == ao[45] and len(ap[45]) > 0
+    rng = np.random.default_rng(2)
+    dq = rng.standard_normal(prod.dofs())
+    dv = rng.standard_normal(prod.dofs())
+    for i in (44, 43):
+        outp = backward_frame(prod, spec, qp, vp, ap, i, dq, dv, bwd)
+        outo = backward_frame(orc, spec, qo, vo, ao, i, dq, dv, bwd)
+        for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_dmat_w", 4)):
+            print(i, name, rel_err(outp[k], outo[k]))
+            assert rel_err(outp[k], outo[k]) < 1e-6, (i, name)
+        dq, dv = outo[0], outo[1]
