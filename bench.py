@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 path")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
+    p.add_argument("--profile-spmv", type=int, default=0,
+                   help="only time N global solves (FP32 and FP64 factor) and a 1 GiB copy, for profiling")
     return p.parse_args()
 
 
@@ -119,6 +121,36 @@ def cpu_baseline(spec, threads, budget_s=25.0):
     return {"t_factor": t_factor, "t_solve": t_solve, "t_local": t_local}
 
 
+def profile_spmv(spec, args):
+    """N applications of A^-1 = S^T S with each factor precision plus a plain copy."""
+    from diffphd_amd import scenes
+    from diffphd_amd.py_diff_pd_core import HexDeformable
+    body = scenes.apply(spec, HexDeformable(0))
+    body.factor_build(spec["dt"], args.threads)
+    info = body.factor_info()
+    log(f"factor {info}")
+    b = np.random.default_rng(0).standard_normal(body.dofs())
+    res = {"nnz_S": info["nnz_S"]}
+    for prec, name in ((1, "fp32"), (0, "fp64")):
+        body.apply_ainv(spec["dt"], b, precision=prec)
+        body.timing_enable(True)
+        body.timing_read(reset=True)
+        for _ in range(args.profile_spmv):
+            body.apply_ainv(spec["dt"], b, precision=prec)
+        t = body.timing_read(reset=True)
+        body.timing_enable(False)
+        bv = 4 if prec == 1 else 8
+        for k in ("rows", "cols"):
+            ms = t[k + "_ms"] / max(t[k + "_launches"], 1)
+            res[f"{name}_{k}_ms"] = ms
+            res[f"{name}_{k}_GBs"] = info["nnz_S"] * bv / (ms * 1e-3) / 1e9
+        log(f"{name}: {res}")
+    ms = body.copy_probe(1 << 30, 20)
+    res["copy_1GiB_ms"] = ms
+    res["copy_GBs"] = 2 * (1 << 30) / (ms * 1e-3) / 1e9
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,6 +160,9 @@ def main():
     from diffphd_amd import scenes
     cells = tuple(int(c) for c in args.cells.split(","))
     spec = scenes.press(cells=cells, frames=max(100, args.warmup + args.steps))
+    if args.profile_spmv > 0:
+        profile_spmv(spec, args)
+        return
     res = time_path(spec, args.precision, args.steps, args.warmup, args.threads)
     K = args.steps
     steps_per_s = K / res["elapsed"]

@@ -152,6 +152,36 @@ int phd_timing_read(phd_ctx ctx, double* out4, int reset) {
     });
 }
 
+int phd_last_contact_multipliers(phd_ctx ctx, double* lambda, double* lt1, double* lt2, int cap) {
+    if (!ctx) return -1;
+    phd::Body& b = *static_cast<Ctx*>(ctx)->body;
+    const int n = (int)b.last_lambda_.size();
+    for (int i = 0; i < n && i < cap; ++i) { lambda[i] = b.last_lambda_[i]; lt1[i] = b.last_lt1_[i]; lt2[i] = b.last_lt2_[i]; }
+    return n;
+}
+
+int phd_copy_probe(phd_ctx ctx, long long nbytes, int reps, double* ms_out) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::DevBuf<char> x, y;
+        const long long n16 = nbytes / 16;
+        x.alloc((size_t)n16 * 16);
+        y.alloc((size_t)n16 * 16);
+        phd::hip_check(hipMemset(x.p, 0, (size_t)n16 * 16), "memset");
+        hipEvent_t e0, e1;
+        phd::hip_check(hipEventCreate(&e0), "event");
+        phd::hip_check(hipEventCreate(&e1), "event");
+        phd::hip_check(hipEventRecord(e0, b.stream()), "record");
+        for (int r = 0; r < reps; ++r) phd::launch_copy(b.stream(), n16, x.p, y.p);
+        phd::hip_check(hipEventRecord(e1, b.stream()), "record");
+        phd::hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *ms_out = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int phd_host_factor_check(int n, const int* Ap, const int* Ai, const double* Ax, const double* X, double* out7) {
     try {
         std::vector<int> ap(Ap, Ap + n + 1), ai(Ai, Ai + Ap[n]);

@@ -119,6 +119,9 @@ template <class TM>
 void launch_assemble_msys_idx(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
                               const double* om, const double* E, TM* M);
 
+// streaming copy of n16 16-byte words (byte-counter calibration)
+void launch_copy(hipStream_t s, int64_t n16, const void* a, void* b);
+
 // ---- backward ----------------------------------------------------------------
 // per-sample 9x9 proximal-map Jacobians weighted by the K_pd weights
 template <class TW>

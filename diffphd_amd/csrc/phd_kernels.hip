@@ -866,6 +866,11 @@ __global__ void k_assemble_msys_idx(int nJ, const int* __restrict__ J, const dou
     M[idx] = (TM)v;
 }
 
+// Plain streaming copy (calibration of the HBM byte counters).
+__global__ void k_copy(int64_t n, const float4* __restrict__ a, float4* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 }  // namespace
 
 // ============================================================================
@@ -1030,6 +1035,9 @@ template void launch_assemble_msys_idx<float>(hipStream_t, int, const int*, cons
                                               const double*, const double*, float*);
 template void launch_assemble_msys_idx<double>(hipStream_t, int, const int*, const double*, int, const double*,
                                                const double*, const double*, double*);
+void launch_copy(hipStream_t s, int64_t n16, const void* a, void* b) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, n16, (const float4*)a, (float4*)b);
+}
 void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r) {
     if (K == 0) return;
     hipLaunchKernelGGL(k_add_node_force, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, nodes, f, r);

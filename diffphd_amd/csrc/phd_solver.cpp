@@ -1018,6 +1018,9 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     for (int c = 0; c < K; ++c) if (lambda[c] > 0) active.push_back(cand_[c]);
     prev_candidates_ = cand_;
     prev_lambda_ = lambda;
+    last_lambda_ = lambda;
+    last_lt1_ = lt1;
+    last_lt2_ = lt2;
     ReplayFrame fr;
     fr.active = active;
     fr.q_sol = qs;

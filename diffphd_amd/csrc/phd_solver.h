@@ -116,6 +116,7 @@ public:
 
     // ---- statistics ----
     std::vector<StepStats> frame_stats;
+    std::vector<double> last_lambda_, last_lt1_, last_lt2_;
     int last_backward_iters = 0;
     long long factor_bytes() const;
     long long factor_nnz() const { return factor_.nnz; }

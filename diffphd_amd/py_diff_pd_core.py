@@ -257,6 +257,19 @@ class _Deformable:
     def last_backward_iters(self):
         return _lib.lib().phd_last_backward_iters(self._ctx)
 
+    def last_contact_multipliers(self):
+        cap = self._dofs // 3 + 1
+        a, ap = _lib.dptr(np.zeros(cap))
+        b, bp = _lib.dptr(np.zeros(cap))
+        c, cp = _lib.dptr(np.zeros(cap))
+        n = _lib.lib().phd_last_contact_multipliers(self._ctx, ap, bp, cp, cap)
+        return a[:n].copy(), b[:n].copy(), c[:n].copy()
+
+    def copy_probe(self, nbytes, reps=10):
+        out, op = _lib.dptr(np.zeros(1))
+        self._ok(_lib.lib().phd_copy_probe(self._ctx, int(nbytes), int(reps), op))
+        return float(out[0])
+
     def timing_enable(self, on=True):
         self._ok(_lib.lib().phd_timing_enable(self._ctx, int(on)))
 

@@ -103,6 +103,14 @@ int phd_factor_info(phd_ctx ctx, double* out, int cap);
 int phd_timing_enable(phd_ctx ctx, int enable);
 int phd_timing_read(phd_ctx ctx, double* out4, int reset);
 
+/* Contact multipliers of the last forward frame, per candidate (sorted by vertex):
+ * normal lambda and tangential lt1, lt2. Returns the candidate count. */
+int phd_last_contact_multipliers(phd_ctx ctx, double* lambda, double* lt1, double* lt2, int cap);
+
+/* Streams `nbytes` through a plain device copy kernel `reps` times and writes the
+ * average duration in ms (calibrates the HBM byte counters and the bandwidth peak). */
+int phd_copy_probe(phd_ctx ctx, long long nbytes, int reps, double* ms_out);
+
 /* Host-only self-check of the factor construction (no device needed; small n):
  * builds the factor of the symmetric CSR matrix (full pattern) with vertex
  * coordinates X and writes [max|S^T S A - I|, max S-product plan error,

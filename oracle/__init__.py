@@ -63,6 +63,7 @@ def lib():
         L.orc_local_differential.argtypes = [P, dp, ctypes.c_int, dp]
         L.orc_frame_stats.argtypes = [P, ip, ip, ctypes.c_int]
         L.orc_reset_frame_stats.argtypes = [P]
+        L.orc_last_contact_multipliers.argtypes = [P, dp, dp, dp, ctypes.c_int]
         L.orc_svd3.argtypes = [dp, dp, dp, dp]
         L.orc_project.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
         L.orc_project_diff.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp, dp]
@@ -236,6 +237,14 @@ class OracleDeformable:
 
     def reset_frame_stats(self):
         lib().orc_reset_frame_stats(self._h)
+
+    def last_contact_multipliers(self):
+        cap = self._dofs // 3 + 1
+        a, ap = _d(np.zeros(cap))
+        b, bp = _d(np.zeros(cap))
+        c, cp = _d(np.zeros(cap))
+        n = lib().orc_last_contact_multipliers(self._h, ap, bp, cp, cap)
+        return a[:n].copy(), b[:n].copy(), c[:n].copy()
 
     def last_backward_iters(self):
         return lib().orc_last_backward_iters(self._h)

@@ -685,7 +685,7 @@ struct Body {
     SparseLDLT ldlt[3];
     std::vector<AlgPhdReplayCache> tape;
     std::vector<int> prev_candidates;
-    Vec prev_lambda;
+    Vec prev_lambda, last_lambda, last_lt1, last_lt2;
     std::vector<int> frame_iters, frame_converged;
     int last_bwd_iters = 0;
 
@@ -1330,6 +1330,9 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
     for (int c = 0; c < K; ++c) if (lambda[c] > 0) active.push_back(cand[c]);
     prev_candidates = cand;
     prev_lambda = lambda;
+    last_lambda = lambda;
+    last_lt1 = lt1;
+    last_lt2 = lt2;
     AlgPhdReplayCache fc;
     fc.contact_candidates = active;
     fc.q_tilde = q_tilde;
@@ -1715,6 +1718,12 @@ void orc_reset_frame_stats(void* hp) {
     Handle* h = static_cast<Handle*>(hp);
     h->body.frame_iters.clear();
     h->body.frame_converged.clear();
+}
+int orc_last_contact_multipliers(void* hp, double* l, double* a, double* b, int cap) {
+    Handle* h = static_cast<Handle*>(hp);
+    const int n = (int)h->body.last_lambda.size();
+    for (int i = 0; i < n && i < cap; ++i) { l[i] = h->body.last_lambda[i]; a[i] = h->body.last_lt1[i]; b[i] = h->body.last_lt2[i]; }
+    return n;
 }
 int orc_last_backward_iters(void* hp) { return static_cast<Handle*>(hp)->body.last_bwd_iters; }
 
