@@ -829,16 +829,28 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             }
             for (int c = 0; c < K; ++c) rhs[c] = hvec[c];
             if (nJ > 0) {
-                // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2 (supported on J)
+                // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2. The
+                // friction part can sit on candidates outside J (lambda > 0 with omega rounded to 0).
                 std::fill(Fg.begin(), Fg.end(), 0.0);
-                for (int c : Jv)
+                std::vector<int> Gv;
+                for (int c = 0; c < K; ++c) {
+                    const bool fr = do_fric && (lt1[c] != 0.0 || lt2[c] != 0.0);
+                    if (om[c] * lambda[c] == 0.0 && !fr) continue;
+                    Gv.push_back(c);
                     for (int d = 0; d < 3; ++d)
-                        Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
-                d_dF_.upload(Fg);
+                        Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (fr ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
+                }
+                const int nG = (int)Gv.size();
                 hip_check(hipMemcpyAsync(d_idxJ_.p, Jv.data(), sizeof(int) * nJ, hipMemcpyHostToDevice, stream_), "J");
-                launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nJ, d_idxJ_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
-                hip_check(hipStreamSynchronize(stream_), "HFg");
-                d_tmpK3_.download(hv.data(), 3 * nJ);
+                if (nG > 0) {
+                    d_dF_.upload(Fg);
+                    hip_check(hipMemcpyAsync(d_idxF_.p, Gv.data(), sizeof(int) * nG, hipMemcpyHostToDevice, stream_), "G");
+                    launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nG, d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
+                    hip_check(hipStreamSynchronize(stream_), "HFg");
+                    d_tmpK3_.download(hv.data(), 3 * nJ);
+                } else {
+                    std::fill(hv.begin(), hv.end(), 0.0);
+                }
                 std::vector<double> rJ(nJ);
                 for (int a = 0; a < nJ; ++a) {
                     const int c = Jv[a];
@@ -862,6 +874,14 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                           "Dense Cholesky of the contact system failed (matrix not positive definite).");
                 }
                 for (int a = 0; a < nJ; ++a) rhs[Jv[a]] = rJ[a];
+                if (verbose > 1)
+                    for (int a = 0; a < nJ; ++a) {
+                        const int c = Jv[a];
+                        fprintf(stderr, "contact %d c=%d d=%.17g om=%.17g E=%.17g h=%.17g nu=%.17g nhv=%.17g Hcc=%.17g dl=%.17g\n", iter, c,
+                                delta[c], om[c], E[c], hvec[c],
+                                nrm[3 * c] * uC[3 * c] + nrm[3 * c + 1] * uC[3 * c + 1] + nrm[3 * c + 2] * uC[3 * c + 2],
+                                nrm[3 * c] * hv[3 * a] + nrm[3 * c + 1] * hv[3 * a + 1] + nrm[3 * c + 2] * hv[3 * a + 2], Hdiag_[c], rJ[a]);
+                    }
             }
             for (int c = 0; c < K; ++c) {
                 const double dl = (om[c] != 0.0) ? rhs[c] : rhs[c] / E[c];
@@ -992,14 +1012,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             const int node = cand_[c];
             double* x = &qs[3 * node];
             double sd, nh[3];
-            // host evaluation of the obstacle at the converged position
-            if (boundary_.type == 1) { sd = x[0] * boundary_.n[0] + x[1] * boundary_.n[1] + x[2] * boundary_.n[2] + boundary_.off; for (int d = 0; d < 3; ++d) nh[d] = boundary_.n[d]; }
-            else if (boundary_.type == 2) {
-                double dd[3] = {x[0] - boundary_.c[0], x[1] - boundary_.c[1], x[2] - boundary_.c[2]};
-                const double rho = std::sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-                sd = rho - boundary_.r;
-                for (int d = 0; d < 3; ++d) nh[d] = dd[d] / rho;
-            } else continue;
+            boundary_.eval(x, sd, nh);  // obstacle at the converged position
             const double nn = std::sqrt(nh[0] * nh[0] + nh[1] * nh[1] + nh[2] * nh[2]);
             if (nn <= 1e-12) continue;
             for (int d = 0; d < 3; ++d) nh[d] /= nn;

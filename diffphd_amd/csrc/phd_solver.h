@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "phd_factor.h"
@@ -71,6 +72,39 @@ struct HostBoundary {
     int type = 0;  // 0 none, 1 planar, 2 spherical, 3 heightfield, 4 press
     double n[3] = {0, 0, 1}, off = 0, c[3] = {0, 0, 0}, r = 0, z0 = 0, plane_z = 0;
     std::vector<double> waves;
+
+    // Signed distance and outward unit normal of the governing obstacle at q
+    // (the host twin of the contact kernels' boundary evaluation).
+    void eval(const double* q, double& dist, double nn[3]) const {
+        if (type == 1) {
+            dist = q[0] * n[0] + q[1] * n[1] + q[2] * n[2] + off;
+            for (int d = 0; d < 3; ++d) nn[d] = n[d];
+            return;
+        }
+        if (type == 2) {
+            const double d0 = q[0] - c[0], d1 = q[1] - c[1], d2 = q[2] - c[2];
+            const double rho = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            dist = rho - r;
+            nn[0] = d0 / rho; nn[1] = d1 / rho; nn[2] = d2 / rho;
+            return;
+        }
+        double h = z0, hx = 0, hy = 0;
+        for (size_t i = 0; i + 3 < waves.size(); i += 4) {
+            const double arg = waves[i + 1] * q[0] + waves[i + 2] * q[1] + waves[i + 3];
+            h += waves[i] * std::sin(arg);
+            hx += waves[i] * waves[i + 1] * std::cos(arg);
+            hy += waves[i] * waves[i + 2] * std::cos(arg);
+        }
+        const double s = std::sqrt(1 + hx * hx + hy * hy);
+        const double dg = (q[2] - h) / s;
+        if (type == 4 && plane_z - q[2] < dg) {
+            dist = plane_z - q[2];
+            nn[0] = 0; nn[1] = 0; nn[2] = -1;
+            return;
+        }
+        dist = dg;
+        nn[0] = -hx / s; nn[1] = -hy / s; nn[2] = 1 / s;
+    }
 };
 
 struct ReplayFrame {  // AlgPhdReplayCache fields used by the backward (deformable.h:336-355)

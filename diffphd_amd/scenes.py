@@ -127,13 +127,15 @@ def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0,
     ky = 2 * np.pi * rng.integers(1, 5, nw) / (cells[1] * dx) * rng.choice([-1, 1], nw)
     ph = rng.uniform(0, 2 * np.pi, nw)
     waves = np.stack([a, kx, ky, ph], 1).ravel()
-    # place the block on the highest bump of its footprint
+    # place the block just above the highest bump of its footprint: a gap of
+    # dx/50 keeps every vertex off d == 0, where the NCP weight
+    # omega = 1 - d / max(|d|, 1e-14) jumps between 0 and 2 on round-off
     xs = np.unique(V[0]); ys = np.unique(V[1])
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     H = np.zeros_like(X)
     for i in range(nw):
         H += a[i] * np.sin(kx[i] * X + ky[i] * Y + ph[i])
-    z0 = -H.max()
+    z0 = -H.max() - 0.02 * dx
     plane0 = height
     # per-element E: log-space smoothed GRF spanning `decades`
     field = _smoothed_field(cells, rng, sigma_cells=3.0).ravel()

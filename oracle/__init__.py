@@ -63,6 +63,7 @@ def lib():
         L.orc_local_differential.argtypes = [P, dp, ctypes.c_int, dp]
         L.orc_frame_stats.argtypes = [P, ip, ip, ctypes.c_int]
         L.orc_reset_frame_stats.argtypes = [P]
+        L.orc_set_refinement.argtypes = [P, ctypes.c_int]
         L.orc_last_contact_multipliers.argtypes = [P, dp, dp, dp, ctypes.c_int]
         L.orc_svd3.argtypes = [dp, dp, dp, dp]
         L.orc_project.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
@@ -237,6 +238,10 @@ class OracleDeformable:
 
     def reset_frame_stats(self):
         lib().orc_reset_frame_stats(self._h)
+
+    def set_refinement(self, steps):
+        """Iterative refinement steps (extended-precision residual) after each LDL^T solve."""
+        lib().orc_set_refinement(self._h, int(steps))
 
     def last_contact_multipliers(self):
         cap = self._dofs // 3 + 1

@@ -686,6 +686,9 @@ struct Body {
     std::vector<AlgPhdReplayCache> tape;
     std::vector<int> prev_candidates;
     Vec prev_lambda, last_lambda, last_lt1, last_lt2;
+    int refine_steps = 0;
+    std::array<std::vector<int>, 3> csr_p, csr_i;
+    std::array<std::vector<double>, 3> csr_x;
     std::vector<int> frame_iters, frame_converged;
     int last_bwd_iters = 0;
 
@@ -909,6 +912,9 @@ struct Body {
             std::iota(all.begin(), all.end(), 0);
             rcb_order(all, X, adj, perm, side);
             ldlt[d].factor(nv, Ap, Ai, Ax, perm);
+            csr_p[d] = Ap;
+            csr_i[d] = Ai;
+            csr_x[d] = Ax;
         }
         pd_ready = true;
         pd_dt = dt;
@@ -917,6 +923,18 @@ struct Body {
     Vec solve_axis(int d, const Vec& b) const {
         Vec x(nv);
         ldlt[d].solve(b.data(), x.data());
+        // Optional iterative refinement with an extended-precision residual: a check
+        // on how much of a difference comes from the conditioning of A alone.
+        for (int it = 0; it < refine_steps; ++it) {
+            Vec r(nv), dx(nv);
+            for (int i = 0; i < nv; ++i) {
+                long double acc = b[i];
+                for (int p = csr_p[d][i]; p < csr_p[d][i + 1]; ++p) acc -= (long double)csr_x[d][p] * x[csr_i[d][p]];
+                r[i] = (double)acc;
+            }
+            ldlt[d].solve(r.data(), dx.data());
+            for (int i = 0; i < nv; ++i) x[i] += dx[i];
+        }
         return x;
     }
 
@@ -1230,6 +1248,14 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
                 rhs[c] = hvec[c] - omega[c] * val;
             }
             const Vec dl = dense_ldlt_solve(M, K, rhs);
+            if (verbose > 1)
+                for (int c = 0; c < K; ++c) {
+                    if (omega[c] == 0.0) continue;
+                    double nu = 0, nhv = 0;
+                    for (int d = 0; d < 3; ++d) { nu += nrm[c][d] * Ab[d][cand[c]]; nhv += nrm[c][d] * (Ag[3 * cand[c] + d] - Ab[d][cand[c]]); }
+                    fprintf(stderr, "contact %d c=%d d=%.17g om=%.17g E=%.17g h=%.17g nu=%.17g nhv=%.17g Hcc=%.17g dl=%.17g\n", iter, c,
+                            boundary.distance(&q_sol[3 * cand[c]]), omega[c], E[c], hvec[c], nu, nhv, W[(size_t)c * K + c], dl[c]);
+                }
             for (int c = 0; c < K; ++c) lambda[c] = std::min(std::max(lambda[c] + dl[c], 0.0), 1e8);
             Vec wl(K);
             for (int c = 0; c < K; ++c) wl[c] = omega[c] * lambda[c];
@@ -1719,6 +1745,7 @@ void orc_reset_frame_stats(void* hp) {
     h->body.frame_iters.clear();
     h->body.frame_converged.clear();
 }
+void orc_set_refinement(void* hp, int steps) { static_cast<Handle*>(hp)->body.refine_steps = steps; }
 int orc_last_contact_multipliers(void* hp, double* l, double* a, double* b, int cap) {
     Handle* h = static_cast<Handle*>(hp);
     const int n = (int)h->body.last_lambda.size();
