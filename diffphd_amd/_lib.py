@@ -45,7 +45,9 @@ SIGNATURES = [
     ("phd_factor_info", ctypes.c_int, [_P, _dp, ctypes.c_int]),
     ("phd_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("phd_timing_read", ctypes.c_int, [_P, _dp, ctypes.c_int]),
+    ("phd_phase_times", ctypes.c_int, [_P, _dp, ctypes.c_int, ctypes.c_int]),
     ("phd_last_contact_multipliers", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_int]),
+    ("phd_dense_spd_solve", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     ("phd_copy_probe", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_int, _dp]),
     ("phd_host_factor_check", ctypes.c_int, [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]),
 ]

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "phd_solver.h"
+#include "phd_dense.h"
 
 namespace {
 struct Ctx {
@@ -139,7 +140,10 @@ int phd_factor_info(phd_ctx ctx, double* out, int cap) {
     });
 }
 int phd_timing_enable(phd_ctx ctx, int enable) {
-    return guard(ctx, [&](phd::Body& b) { b.timing = enable != 0; });
+    return guard(ctx, [&](phd::Body& b) {
+        b.timing = enable != 0;
+        b.timing_level = enable;
+    });
 }
 int phd_timing_read(phd_ctx ctx, double* out4, int reset) {
     return guard(ctx, [&](phd::Body& b) {
@@ -152,12 +156,55 @@ int phd_timing_read(phd_ctx ctx, double* out4, int reset) {
     });
 }
 
+int phd_phase_times(phd_ctx ctx, double* out, int cap, int reset) {
+    if (!ctx) return -1;
+    phd::Body& b = *static_cast<Ctx*>(ctx)->body;
+    const int n = (int)phd::Body::kNumPhases;
+    for (int i = 0; i < n && i < cap; ++i) out[i] = b.phase[i];
+    if (reset) for (int i = 0; i < n; ++i) b.phase[i] = 0;
+    return n;
+}
+
 int phd_last_contact_multipliers(phd_ctx ctx, double* lambda, double* lt1, double* lt2, int cap) {
     if (!ctx) return -1;
     phd::Body& b = *static_cast<Ctx*>(ctx)->body;
     const int n = (int)b.last_lambda_.size();
     for (int i = 0; i < n && i < cap; ++i) { lambda[i] = b.last_lambda_[i]; lt1[i] = b.last_lt1_[i]; lt2[i] = b.last_lt2_[i]; }
     return n;
+}
+
+int phd_dense_spd_solve(int n, const double* A, const double* b, double* x, int precision) {
+    try {
+        if (n <= 0) throw std::runtime_error("n must be positive");
+        const int np = phd::chol_padded(n);
+        const int64_t lda = phd::chol_lda(n);
+        std::vector<double> L((size_t)lda * np, 0.0);
+        for (int c = 0; c < np; ++c)
+            for (int r = c; r < np; ++r) L[r + (size_t)c * lda] = (r < n && c < n) ? A[r + (size_t)c * n] : (r == c ? 1.0 : 0.0);
+        for (int c = 0; c < n; ++c) L[np + (size_t)c * lda] = b[c];
+        int fail = 0;
+        auto run = [&](auto tag) {
+            using TV = decltype(tag);
+            std::vector<TV> Lt(L.begin(), L.end()), xt(np);
+            phd::DevBuf<TV> dM, dy, dx;
+            phd::DevBuf<int> df;
+            dM.upload(Lt);
+            dy.alloc(np);
+            dx.alloc(np);
+            df.alloc(1);
+            phd::launch_clear_flag(nullptr, df.p);
+            phd::launch_chol_solve<TV>(nullptr, n, dM.p, dy.p, dx.p, df.p);
+            phd::hip_check(hipDeviceSynchronize(), "dense solve");
+            dx.download(xt.data(), np);
+            df.download(&fail, 1);
+            for (int i = 0; i < n; ++i) x[i] = (double)xt[i];
+        };
+        if (precision == 1) run(float{}); else run(double{});
+        return fail ? 1 : 0;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return -1;
+    }
 }
 
 int phd_copy_probe(phd_ctx ctx, long long nbytes, int reps, double* ms_out) {

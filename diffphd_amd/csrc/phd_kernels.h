@@ -36,9 +36,10 @@ struct DevBoundary {
 };
 
 // ---- factor expansion and the S / S^T products --------------------------------
+// chain[i] = 1 when parent(i) == i + 1 and pattern(i) = {i} + pattern(i + 1).
 void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t* Lp, const int* Li, const double* Lx,
-                     const int* parent, const int* depth, const int* fd, const int64_t* base, double* ws,
-                     int64_t ws_stride, double* V64, float* V32);
+                     const int* parent, const int* depth, const int* fd, const int64_t* base, const unsigned char* chain,
+                     double* ws, int64_t ws_stride, double* V64, float* V32);
 
 template <class TV>
 void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* ru_c0, const int* ru_c1, const int64_t* base,

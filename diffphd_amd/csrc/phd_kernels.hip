@@ -28,44 +28,88 @@ __device__ __forceinline__ T wave_sum(T v) {
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // -----------------------------------------------------------------------------
-// S = L^-1 expansion: one wavefront per tile of 64 consecutive postorder
-// columns. The wave walks the tile's rows and then the ancestors of its last
-// column; lane l carries column 64t + l of L^-1 in a private workspace column.
+// S = L^-1 expansion: one workgroup (4 waves) per tile of 64 consecutive
+// postorder columns; lane l carries column 64t + l of L^-1. The path (the
+// tile's rows, then the ancestors of its last column) is walked in chain
+// segments: runs of at most 64 consecutive columns i -> i+1 = parent(i) whose
+// patterns nest (pattern(i) = {i} + pattern(i+1)), so every column of the
+// segment shares the rows R below its last column. Per segment:
+//   1. its 64-column block of X is staged in LDS and wave 0 solves it against
+//      the segment's dense diagonal block (writing the S entries);
+//   2. the four waves update X on R, four rows at a time, with the product of
+//      L(R, segment) and the LDS block accumulated in registers: one global
+//      read-modify-write per (row, column) per segment.
 // -----------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_expand_s(int n, int t_begin, const int64_t* __restrict__ Lp,
-                                                 const int* __restrict__ Li, const double* __restrict__ Lx,
-                                                 const int* __restrict__ parent, const int* __restrict__ depth,
-                                                 const int* __restrict__ fd, const int64_t* __restrict__ base,
-                                                 double* __restrict__ ws, int64_t ws_stride, double* __restrict__ V64,
-                                                 float* __restrict__ V32) {
+constexpr int EXP_SEG = 64;
+
+__global__ void __launch_bounds__(256) k_expand_s(int n, int t_begin, const int64_t* __restrict__ Lp,
+                                                  const int* __restrict__ Li, const double* __restrict__ Lx,
+                                                  const int* __restrict__ parent, const int* __restrict__ depth,
+                                                  const int* __restrict__ fd, const int64_t* __restrict__ base,
+                                                  const unsigned char* __restrict__ chain, double* __restrict__ ws,
+                                                  int64_t ws_stride, double* __restrict__ V64, float* __restrict__ V32) {
+    __shared__ double xs[EXP_SEG][WAVE];
+    __shared__ int64_t cpos[EXP_SEG];
     const int t = t_begin + blockIdx.x;
-    const int l = threadIdx.x;
-    const int g0 = 64 * t;
-    const int glast = min(n - 1, g0 + 63);
+    const int l = threadIdx.x % WAVE;
+    const int wv = threadIdx.x / WAVE;
+    const int g0 = WAVE * t;
+    const int glast = min(n - 1, g0 + WAVE - 1);
     const int ng = glast - g0 + 1;
     const int dl = depth[glast];
     const int nrows = ng + dl;
     double* X = ws + (int64_t)blockIdx.x * ws_stride;
-    for (int r = 0; r < nrows; ++r) X[(int64_t)r * 64 + l] = 0.0;
-    if (l < ng) X[(int64_t)l * 64 + l] = 1.0;
+    for (int r = wv; r < nrows; r += 4) X[(int64_t)r * WAVE + l] = (r == l && l < ng) ? 1.0 : 0.0;
+    __syncthreads();
     const int j = g0 + l;
-    auto process = [&](int i, int li) {
-        const int64_t p0 = Lp[i], p1 = Lp[i + 1];
-        const double x = X[(int64_t)li * 64 + l] / Lx[p0];
-        if (l < ng && j >= fd[i] && j <= i) {
-            const int64_t a = base[i] + j;
-            V64[a] = x;
-            V32[a] = (float)x;
+    // row of node k in X: tile rows first, then the ancestors by increasing height
+    auto lk_of = [&](int k) { return k <= glast ? k - g0 : ng + dl - depth[k] - 1; };
+    int a = g0;
+    while (a != -1) {
+        int b = a;
+        while (b - a + 1 < EXP_SEG && chain[b]) ++b;
+        const int w = b - a + 1;
+        const int la = lk_of(a);  // segment rows are consecutive in X
+        for (int s2 = wv; s2 < w; s2 += 4) xs[s2][l] = X[(int64_t)(la + s2) * WAVE + l];
+        if (threadIdx.x < w) cpos[threadIdx.x] = Lp[a + threadIdx.x] + (w - (int)threadIdx.x);
+        __syncthreads();
+        if (wv == 0) {
+            for (int s2 = 0; s2 < w; ++s2) {
+                const int i = a + s2;
+                const int64_t p0 = Lp[i];
+                const double x = xs[s2][l] / Lx[p0];
+                xs[s2][l] = x;
+                if (l < ng && j >= fd[i] && j <= i) {
+                    const int64_t at = base[i] + j;
+                    V64[at] = x;
+                    V32[at] = (float)x;
+                }
+                for (int u = s2 + 1; u < w; ++u) xs[u][l] -= Lx[p0 + (u - s2)] * x;
+            }
         }
-        for (int64_t p = p0 + 1; p < p1; ++p) {
-            const int k = Li[p];
-            const int lk = (k <= glast) ? (k - g0) : (ng + dl - depth[k] - 1);
-            double* xk = X + (int64_t)lk * 64 + l;
-            *xk -= Lx[p] * x;
+        __syncthreads();
+        const int64_t pb = Lp[b] + 1;
+        const int nr = (int)(Lp[b + 1] - pb);
+        for (int r0 = 4 * wv; r0 < nr; r0 += 16) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            const int m = min(4, nr - r0);
+            for (int s2 = 0; s2 < w; ++s2) {
+                const double xv = xs[s2][l];
+                const int64_t c = cpos[s2] + r0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < m) acc[q] += Lx[c + q] * xv;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q < m) {
+                    double* xk = X + (int64_t)lk_of(Li[pb + r0 + q]) * WAVE + l;
+                    *xk -= acc[q];
+                }
         }
-    };
-    for (int i = g0; i <= glast; ++i) process(i, i - g0);
-    for (int k = parent[glast]; k != -1; k = parent[k]) process(k, ng + dl - depth[k] - 1);
+        __syncthreads();
+        a = (b < glast) ? b + 1 : parent[b];
+    }
 }
 
 // -----------------------------------------------------------------------------
@@ -877,11 +921,11 @@ __global__ void k_copy(int64_t n, const float4* __restrict__ a, float4* __restri
 // launchers
 // ============================================================================
 void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t* Lp, const int* Li, const double* Lx,
-                     const int* parent, const int* depth, const int* fd, const int64_t* base, double* ws,
-                     int64_t ws_stride, double* V64, float* V32) {
+                     const int* parent, const int* depth, const int* fd, const int64_t* base, const unsigned char* chain,
+                     double* ws, int64_t ws_stride, double* V64, float* V32) {
     if (t_end <= t_begin) return;
-    hipLaunchKernelGGL(k_expand_s, dim3(t_end - t_begin), dim3(64), 0, s, n, t_begin, Lp, Li, Lx, parent, depth, fd, base,
-                       ws, ws_stride, V64, V32);
+    hipLaunchKernelGGL(k_expand_s, dim3(t_end - t_begin), dim3(256), 0, s, n, t_begin, Lp, Li, Lx, parent, depth, fd, base,
+                       chain, ws, ws_stride, V64, V32);
 }
 
 template <class TV>

@@ -12,6 +12,7 @@
 // as a preconditioner and the fixed point keeps FP64 accuracy; with the FP64
 // factor the form is algebraically identical to x = A^-1 b.
 #include "phd_solver.h"
+#include "phd_dense.h"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -27,6 +28,20 @@
 #include <unordered_map>
 
 namespace phd {
+
+Body::PhaseTimer::PhaseTimer(Body* b_, int id_) : b(b_), id(id_), t0(0) {
+    if (b->timing_level >= 2) {
+        hip_check(hipStreamSynchronize(b->stream_), "phase sync");
+        t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+}
+Body::PhaseTimer::~PhaseTimer() {
+    if (b->timing_level >= 2) {
+        (void)hipStreamSynchronize(b->stream_);
+        b->phase[id] += std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+    }
+}
+
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
@@ -388,15 +403,23 @@ void Body::setup(double dt, int threads) {
         V32_.alloc(factor_.v_len);
         hip_check(hipMemsetAsync(V64_.p, 0, V64_.n * sizeof(double), stream_), "memset V64");
         hip_check(hipMemsetAsync(V32_.p, 0, V32_.n * sizeof(float), stream_), "memset V32");
+        // chain[i]: column i + 1 is the parent of i and pattern(i) = {i} + pattern(i + 1)
+        std::vector<unsigned char> chain(nv_, 0);
+        for (int i = 0; i + 1 < nv_; ++i)
+            chain[i] = factor_.parent[i] == i + 1 &&
+                       factor_.Lp[i + 1] - factor_.Lp[i] == factor_.Lp[i + 2] - factor_.Lp[i + 1] + 1;
+        DevBuf<unsigned char> dchain;
+        dchain.upload(chain);
         const int ntiles = (nv_ + 63) / 64;
         const int64_t ws_stride = (int64_t)(64 + factor_.max_depth + 1) * 64;
-        const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)(1LL << 31) / (ws_stride * 8)));
+        // workspace of up to 8 GiB: every tile of the 131k-vertex grid in one launch
+        const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)(8LL << 30) / (ws_stride * 8)));
         DevBuf<double> ws;
         ws.alloc((size_t)(batch * ws_stride));
         for (int t0b = 0; t0b < ntiles; t0b += (int)batch) {
             const int t1b = std::min<int>(ntiles, t0b + (int)batch);
-            launch_expand_s(stream_, nv_, t0b, t1b, dLp.p, dLi.p, dLx.p, dpar.p, ddep.p, d_fd_.p, d_base_.p, ws.p, ws_stride,
-                            V64_.p, V32_.p);
+            launch_expand_s(stream_, nv_, t0b, t1b, dLp.p, dLi.p, dLx.p, dpar.p, ddep.p, d_fd_.p, d_base_.p, dchain.p, ws.p,
+                            ws_stride, V64_.p, V32_.p);
         }
         hip_check(hipStreamSynchronize(stream_), "expand S");
         hip_check(hipGetLastError(), "expand S launch");
@@ -635,29 +658,35 @@ void Body::kpd_apply(const double* x, double* out, int precision) {
     launch_gather_vertex(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, out);
 }
 
-// Dense SPD solve on the device (rocSOLVER Cholesky). rhs (host, n) is
-// overwritten with the solution. The matrix is already in d_M64_/d_M32_.
-bool Body::solve_dense_spd(int n, int precision, const double* /*unused*/, double* rhs) {
-    rocblas_handle h = (rocblas_handle)blas_;
-    int info = 0;
+// Contact system M dl = rhs on the index set J (already in d_idxJ_, with
+// d_om_ / d_E_ / d_nrm_ current): assembled and solved by the blocked
+// Cholesky of phd_dense.hip. rhs (host, n) is overwritten with the solution.
+// Returns false when a pivot was not positive.
+bool Body::solve_contact_system(int n, int precision, double* rhs) {
+    const int np = chol_padded(n);
+    hip_check(hipMemcpyAsync(d_rhsK_.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
+    launch_clear_flag(stream_, d_info_.p);
+    int fail = 0;
     if (precision == 1) {
-        std::vector<float> r32(rhs, rhs + n);
-        hip_check(hipMemcpyAsync(d_rhsK32_.p, r32.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
-        check(rocsolver_spotrf(h, rocblas_fill_lower, n, d_M32_.p, n, d_info_.p) == rocblas_status_success, "spotrf failed");
-        check(rocsolver_spotrs(h, rocblas_fill_lower, n, 1, d_M32_.p, n, d_rhsK32_.p, n) == rocblas_status_success, "spotrs failed");
-        hip_check(hipMemcpyAsync(r32.data(), d_rhsK32_.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
-        hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info D2H");
-        hip_check(hipStreamSynchronize(stream_), "potrs sync");
-        for (int i = 0; i < n; ++i) rhs[i] = r32[i];
+        if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
+        if (d_cy32_.n < (size_t)np) { d_cy32_.alloc(np); d_cx32_.alloc(np); }
+        launch_chol_assemble<float>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
+        launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
+        std::vector<float> x(n);
+        hip_check(hipMemcpyAsync(x.data(), d_cx32_.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
+        hip_check(hipMemcpyAsync(&fail, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "flag D2H");
+        hip_check(hipStreamSynchronize(stream_), "contact solve");
+        for (int a = 0; a < n; ++a) rhs[a] = x[a];
     } else {
-        hip_check(hipMemcpyAsync(d_rhsK_.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
-        check(rocsolver_dpotrf(h, rocblas_fill_lower, n, d_M64_.p, n, d_info_.p) == rocblas_status_success, "dpotrf failed");
-        check(rocsolver_dpotrs(h, rocblas_fill_lower, n, 1, d_M64_.p, n, d_rhsK_.p, n) == rocblas_status_success, "dpotrs failed");
-        hip_check(hipMemcpyAsync(rhs, d_rhsK_.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
-        hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info D2H");
-        hip_check(hipStreamSynchronize(stream_), "potrs sync");
+        if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
+        if (d_cy64_.n < (size_t)np) { d_cy64_.alloc(np); d_cx64_.alloc(np); }
+        launch_chol_assemble<double>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
+        launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
+        hip_check(hipMemcpyAsync(rhs, d_cx64_.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
+        hip_check(hipMemcpyAsync(&fail, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "flag D2H");
+        hip_check(hipStreamSynchronize(stream_), "contact solve");
     }
-    return info == 0;
+    return fail == 0;
 }
 
 // x = A^-1 b with optional FP64 refinement (test entry point; Dirichlet rows are identity).
@@ -728,6 +757,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
     check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
     setup(dt, threads);
+    PhaseTimer t_total(this, kFwdTotal);
     const int n3 = dofs_;
     const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
     std::vector<double> qt(n3);
@@ -764,12 +794,12 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             }
         }
         hip_check(hipMemsetAsync(d_F_.p, 0, sizeof(double) * 3 * K, stream_), "F=0");
-        if (precision == 1) d_M32_.alloc((size_t)K * K); else d_M64_.alloc((size_t)K * K);
         if ((int)d_rhsK_.n < K) d_rhsK_.alloc(K);
     }
     DevAnderson aa;
     aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
     hip_check(hipMemsetAsync(d_pdrhs_prev_.p, 0, sizeof(double) * n3, stream_), "prev pd_rhs");
+    bool dense64 = precision == 0;
     bool converged = false;
     int iters = 0;
     std::vector<double> delta(K), uC(3 * K), hv(3 * K), om(K), E(K), hvec(K), rhs(K), Fg(3 * K), Fn(3 * K), qC(3 * K);
@@ -777,14 +807,21 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     for (int c = 0; c < K; ++c)
         for (int d = 0; d < 3; ++d) q_frame_C[3 * c + d] = q[3 * cand_[c] + d];
     for (int iter = 0; iter < max_pd_iter; ++iter) {
-        local_rhs(d_q_.p, inv_h2m);
+        {
+            PhaseTimer tl(this, kFwdLocal);
+            local_rhs(d_q_.p, inv_h2m);
+        }
         hip_check(hipMemcpyAsync(d_qprev_.p, d_q_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "q_prev");
         // residual of the global system with the previous contact impulses
-        launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
-        if (K > 0) launch_add_node_force(stream_, K, d_cand_.p, d_F_.p, d_r_.p);
-        if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
-        else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+        {
+            PhaseTimer ts(this, kFwdSolve);
+            launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
+            if (K > 0) launch_add_node_force(stream_, K, d_cand_.p, d_F_.p, d_r_.p);
+            if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
+            else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+        }
         if (K > 0) {
+            PhaseTimer tc(this, kFwdContact);
             // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
@@ -861,16 +898,19 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 d_om_.upload(om);
                 d_E_.upload(E);
                 std::vector<double> rJ_keep = rJ;
+                PhaseTimer td(this, kFwdDense);
+                if (timing_level >= 2) { phase[kDenseCalls] += 1; phase[kDenseRows] += nJ; }
                 bool solved = false;
-                if (precision == 1) {
-                    launch_assemble_msys_idx<float>(stream_, nJ, d_idxJ_.p, d_H_.p, K, d_nrm_.p, d_om_.p, d_E_.p, d_M32_.p);
-                    solved = solve_dense_spd(nJ, 1, nullptr, rJ.data());
+                if (precision == 1 && !dense64) {
+                    solved = solve_contact_system(nJ, 1, rJ.data());
+                    if (!solved) {
+                        dense64 = true;  // the FP32 Cholesky broke down: FP64 for the rest of the frame
+                        if (timing_level >= 2) phase[kDenseFp32Fail] += 1;
+                    }
                 }
                 if (!solved) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
-                    if (d_M64_.n < (size_t)nJ * nJ) d_M64_.alloc((size_t)K * K);
                     rJ = rJ_keep;
-                    launch_assemble_msys_idx<double>(stream_, nJ, d_idxJ_.p, d_H_.p, K, d_nrm_.p, d_om_.p, d_E_.p, d_M64_.p);
-                    check(solve_dense_spd(nJ, 0, nullptr, rJ.data()),
+                    check(solve_contact_system(nJ, 0, rJ.data()),
                           "Dense Cholesky of the contact system failed (matrix not positive definite).");
                 }
                 for (int a = 0; a < nJ; ++a) rhs[Jv[a]] = rJ[a];
@@ -947,11 +987,15 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             d_F_.upload(F);
             if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
         }
-        if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
-        else spmv_ST<double>(y64_, xpart64_, V64_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+        {
+            PhaseTimer ts(this, kFwdSolve);
+            if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+            else spmv_ST<double>(y64_, xpart64_, V64_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+        }
         ++iters;
         // Anderson acceleration
         if (aa_window > 0) {
+            PhaseTimer ta(this, kFwdAA);
             launch_axpbypcz(stream_, n3, 1.0, d_q_.p, -1.0, d_qprev_.p, 0.0, nullptr, d_g_.p);
             int slot = aa.free_slots.back();
             aa.free_slots.pop_back();
@@ -991,6 +1035,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             while ((int)aa.slots.size() > aa.window) { aa.free_slots.push_back(aa.slots.front()); aa.slots.pop_front(); }
         }
         // dual gate (:1382-1397)
+        PhaseTimer tg(this, kFwdGate);
         launch_axpbypcz(stream_, n3, 1.0, d_q_.p, -1.0, d_qprev_.p, 0.0, nullptr, d_g_.p);
         launch_axpbypcz(stream_, n3, 1.0, d_pdrhs_.p, -1.0, d_pdrhs_prev_.p, 0.0, nullptr, d_tmp_.p);
         const std::vector<double> gd = dots({{d_g_.p, d_g_.p}, {d_qprev_.p, d_qprev_.p}, {d_tmp_.p, d_tmp_.p},
@@ -1072,6 +1117,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     setup(dt, threads);
     const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
     if (exact != kpd_exact_) { kpd_exact_ = exact; build_materials(); }
+    PhaseTimer t_total(this, kBwdTotal);
     const int n3 = dofs_;
     const double h = dt, inv_h = 1 / h, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
     for (auto& kv : dirichlet_) check(q_next[kv.first] == kv.second, "Boundary conditions violated.");
@@ -1091,8 +1137,10 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     if (precision == 1) { d_W32_.alloc((size_t)ne_ * samples_ * 81); d_W64_.release(); }
     else { d_W64_.alloc((size_t)ne_ * samples_ * 81); d_W32_.release(); }
     const double* kn[4] = {d_kn_[0].p, d_kn_[1].p, d_kn_[2].p, d_kn_[3].p};
+    std::unique_ptr<PhaseTimer> t_kpd(new PhaseTimer(this, kBwdKpd));
     if (precision == 1) launch_kpd_build<float>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W32_.p);
     else launch_kpd_build<double>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W64_.p);
+    t_kpd.reset();
     // tape
     ReplayFrame replay;
     bool from_tape = false;
@@ -1125,7 +1173,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     };
     // Constrained solve data (Woodbury basis); nb == 0 means the plain backbone.
     int nb = 0;
-    DevBuf<double> d_G, d_dirs;
+    DevBuf<double> d_G, d_Gx, d_dirs;
     DevBuf<int> d_bcidx;
     std::vector<int> b_c;            // candidate index of each basis row
     std::vector<double> b_dir;       // 3 per basis row
@@ -1151,11 +1199,14 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                     const int c = b_c[a];
                     ub[a] = b_dir[3 * a] * uC[3 * c] + b_dir[3 * a + 1] * uC[3 * c + 1] + b_dir[3 * a + 2] * uC[3 * c + 2];
                 }
-                hip_check(hipMemcpy(d_rhsK_.p, ub.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "ub");
-                check(rocsolver_dpotrs((rocblas_handle)blas_, rocblas_fill_lower, nb, 1, d_G.p, nb, d_rhsK_.p, nb) ==
-                          rocblas_status_success, "dpotrs (basis) failed");
+                PhaseTimer tbs(this, kBwdBasisSolve);
+                // G^-1 ub with the inverse formed once per frame (lower triangle, symmetric product)
+                hip_check(hipMemcpyAsync(d_rhsK_.p, ub.data(), sizeof(double) * nb, hipMemcpyHostToDevice, stream_), "ub");
+                const double one = 1.0, zero = 0.0;
+                check(rocblas_dsymv((rocblas_handle)blas_, rocblas_fill_lower, nb, &one, d_G.p, nb, d_rhsK_.p, 1, &zero,
+                                    d_Gx.p, 1) == rocblas_status_success, "dsymv (basis) failed");
+                hip_check(hipMemcpyAsync(ub.data(), d_Gx.p, sizeof(double) * nb, hipMemcpyDeviceToHost, stream_), "ub D2H");
                 hip_check(hipStreamSynchronize(stream_), "basis solve");
-                hip_check(hipMemcpy(ub.data(), d_rhsK_.p, sizeof(double) * nb, hipMemcpyDeviceToHost), "ub D2H");
                 std::fill(Fc.begin(), Fc.end(), 0.0);
                 for (int a = 0; a < nb; ++a)
                     for (int d = 0; d < 3; ++d) Fc[3 * b_c[a] + d] -= ub[a] * b_dir[3 * a + d];
@@ -1213,7 +1264,10 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
         if (used) *used = it_used;
     };
     int bb = 0;
-    fixed_point(max_pd_iter, &bb);
+    {
+        PhaseTimer tb(this, kBwdBackbone);
+        fixed_point(max_pd_iter, &bb);
+    }
     last_backward_iters = bb;
     std::vector<double> db(n3);
     hip_check(hipStreamSynchronize(stream_), "backbone");
@@ -1294,6 +1348,8 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 }
                 d_bcidx.upload(b_c);
                 d_dirs.upload(b_dir);
+                std::unique_ptr<PhaseTimer> tbf(new PhaseTimer(this, kBwdBasisFactor));
+                if (timing_level >= 2) phase[kBasisRows] += nb;
                 d_G.alloc((size_t)nb * nb);
                 launch_assemble_g(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, d_G.p);
                 if ((int)d_rhsK_.n < nb) d_rhsK_.alloc(nb);
@@ -1303,6 +1359,14 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
                 hip_check(hipStreamSynchronize(stream_), "basis factor");
                 check(info == 0, "Contact basis Gram matrix is not positive definite.");
+                check(rocsolver_dpotri((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) == rocblas_status_success,
+                      "dpotri (basis) failed");
+                hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                hip_check(hipStreamSynchronize(stream_), "basis inverse");
+                check(info == 0, "Contact basis Gram matrix is singular.");
+                d_Gx.alloc(nb);
+                tbf.reset();
+                PhaseTimer twb(this, kBwdWoodbury);
                 int used = 0;
                 const int wt_cap = (int)opt_or(opt, "alg_phd_bwd_wt_max_iter", 120);
                 (void)wt_cap;
@@ -1315,6 +1379,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     // dl/d(mat_w)
     const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
     if (!skip_mat && !diverged) {
+        PhaseTimer tm(this, kBwdMatGrad);
         d_s.upload(z);
         d_gcor_.alloc(ne_); d_gvol_.alloc(ne_); d_gnh_.alloc((size_t)std::max(1, n_nh_) * ne_);
         launch_dmatw(stream_, mat_, d_q_.p, d_s.p, element_volume_ / samples_, has_cor_ ? d_gcor_.p : nullptr,

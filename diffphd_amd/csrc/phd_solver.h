@@ -160,6 +160,17 @@ public:
     double t_spmv_rows_ms = 0, t_spmv_cols_ms = 0;
     long long n_spmv_rows = 0, n_spmv_cols = 0;
     bool timing = false;
+    int timing_level = 0;  // 2: synchronize at phase boundaries for the wall-clock split
+    // host wall-clock per phase (seconds, accumulated while `timing` is on); see PhaseId
+    enum PhaseId { kFwdTotal = 0, kFwdLocal, kFwdSolve, kFwdContact, kFwdDense, kFwdAA, kFwdGate,
+                   kBwdTotal, kBwdKpd, kBwdBackbone, kBwdBasisFactor, kBwdWoodbury, kBwdBasisSolve, kBwdMatGrad,
+                   kDenseCalls, kDenseRows, kDenseFp32Fail, kBasisRows, kNumPhases };
+    double phase[kNumPhases] = {0};
+    struct PhaseTimer {
+        Body* b; int id; double t0;
+        PhaseTimer(Body* b_, int id_);
+        ~PhaseTimer();
+    };
     hipStream_t stream() const { return stream_; }
     void collect_timing();
     const Factor& factor() const { return factor_; }
@@ -229,8 +240,8 @@ private:
     int rc_nrows_ = 0;
     DevBuf<double> d_waves_;
     DevBuf<double> d_nrm_, d_t1_, d_t2_, d_delta_, d_F_, d_dF_, d_uC_, d_tmpK3_, d_om_, d_E_;
-    DevBuf<double> d_M64_;
-    DevBuf<float> d_M32_;
+    DevBuf<double> d_C64_, d_cy64_, d_cx64_;  // dense contact system (blocked Cholesky layout)
+    DevBuf<float> d_C32_, d_cy32_, d_cx32_;
     DevBuf<double> d_rhsK_;
     DevBuf<float> d_rhsK32_;
     DevBuf<int> d_info_;
@@ -264,7 +275,7 @@ private:
     void local_rhs(const double* q, double inv_h2m);
     void kpd_apply(const double* x, double* out, int precision);
     std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
-    bool solve_dense_spd(int n, int precision, const double* M64_host_or_null, double* rhs_inout);
+    bool solve_contact_system(int n, int precision, double* rhs_inout);
 };
 
 }  // namespace phd

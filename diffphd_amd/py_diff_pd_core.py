@@ -273,6 +273,16 @@ class _Deformable:
     def timing_enable(self, on=True):
         self._ok(_lib.lib().phd_timing_enable(self._ctx, int(on)))
 
+    PHASES = ("fwd_total", "fwd_local", "fwd_solve", "fwd_contact", "fwd_dense", "fwd_aa", "fwd_gate",
+              "bwd_total", "bwd_kpd", "bwd_backbone", "bwd_basis_factor", "bwd_woodbury", "bwd_basis_solve",
+              "bwd_matgrad", "dense_calls", "dense_rows", "dense_fp32_fail", "basis_rows")
+
+    def phase_times(self, reset=True):
+        """Wall-clock split (s) recorded with timing_enable(2)."""
+        out, op = _lib.dptr(np.zeros(len(self.PHASES)))
+        n = _lib.lib().phd_phase_times(self._ctx, op, len(self.PHASES), int(reset))
+        return dict(zip(self.PHASES[:n], out[:n].tolist()))
+
     def timing_read(self, reset=True):
         out, op = _lib.dptr(np.zeros(4))
         self._ok(_lib.lib().phd_timing_read(self._ctx, op, int(reset)))

@@ -103,9 +103,24 @@ int phd_factor_info(phd_ctx ctx, double* out, int cap);
 int phd_timing_enable(phd_ctx ctx, int enable);
 int phd_timing_read(phd_ctx ctx, double* out4, int reset);
 
+/* Host wall-clock split of forward/backward (seconds, accumulated while timing
+ * is enabled at level 2, which synchronizes the stream at phase boundaries).
+ * Order: fwd total, fwd local step, fwd global solve, fwd contact block, fwd
+ * dense contact solve, fwd Anderson, fwd gate, bwd total, bwd K_pd build, bwd
+ * backbone, bwd basis factor, bwd Woodbury fixed point, bwd basis solves, bwd
+ * material gradient; counters: dense solves, dense rows summed, FP32 dense
+ * failures, basis rows summed. Returns the number of entries. */
+int phd_phase_times(phd_ctx ctx, double* out, int cap, int reset);
+
 /* Contact multipliers of the last forward frame, per candidate (sorted by vertex):
  * normal lambda and tangential lt1, lt2. Returns the candidate count. */
 int phd_last_contact_multipliers(phd_ctx ctx, double* lambda, double* lt1, double* lt2, int cap);
+
+/* Dense SPD solve A x = b (A: n x n column-major, host) through the blocked
+ * Cholesky used for the contact system; precision 0 = FP64, 1 = FP32.
+ * Returns 0, 1 when a pivot was not positive, -1 on error (phd_last_error(NULL)).
+ * Test entry point; no counterpart in the reference. */
+int phd_dense_spd_solve(int n, const double* A, const double* b, double* x, int precision);
 
 /* Streams `nbytes` through a plain device copy kernel `reps` times and writes the
  * average duration in ms (calibrates the HBM byte counters and the bandwidth peak). */
