@@ -189,7 +189,7 @@ int phd_dense_spd_solve(int n, const double* A, const double* b, double* x, int 
             phd::DevBuf<TV> dM, dy, dx;
             phd::DevBuf<int> df;
             dM.upload(Lt);
-            dy.alloc(np);
+            dy.alloc((size_t)phd::chol_work(n));
             dx.alloc(np);
             df.alloc(1);
             phd::launch_clear_flag(nullptr, df.p);

@@ -21,6 +21,7 @@ constexpr int kCholBlock = 64;
 inline int chol_padded(int n) { return kCholBlock * ((n + kCholBlock - 1) / kCholBlock); }
 inline int64_t chol_lda(int n) { return (int64_t)chol_padded(n) + kCholBlock; }
 inline int64_t chol_elems(int n) { return chol_lda(n) * chol_padded(n); }
+inline int64_t chol_work(int n) { return (int64_t)chol_padded(n) + kCholBlock * (kCholBlock + 1); }
 
 // Writes the contact system of the index set J (nJ rows) into the layout above:
 // M_ab = om_i (n_i . n_j) H_ij om_j + [a == b] E_i with i = J[a], j = J[b];
@@ -29,11 +30,11 @@ template <class TV>
 void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
                           const double* om, const double* E, const double* rhs, TV* M);
 
-// Factorization + both substitutions. y, x: device scratch of np entries; on
-// return x[0, n) holds the solution. fail[0] is set to 1 when a pivot is not
-// positive (the caller clears it beforehand).
+// Factorization + both substitutions. work: device scratch of chol_work(n)
+// entries; x: np entries, on return x[0, n) holds the solution. fail[0] is set
+// to 1 when a pivot is not positive (the caller clears it beforehand).
 template <class TV>
-void launch_chol_solve(hipStream_t s, int n, TV* M, TV* y, TV* x, int* fail);
+void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail);
 
 // Clears the failure flag with a kernel (safe inside captured graphs).
 void launch_clear_flag(hipStream_t s, int* flag);

@@ -32,61 +32,83 @@ __global__ void __launch_bounds__(256) k_chol_assemble(int nJ, int np, int64_t l
     M[r + (int64_t)c * lda] = (TV)v;
 }
 
-// Panel step k. Workgroup w handles row block i = k + w (i == T is the
-// right-hand-side block). Every workgroup factors the diagonal block itself
-// (left-looking, lane = row), then solves its block X L_kk^T = A_ik.
+// Panel step k, part 1: one wave factors the diagonal block, lane r holding
+// row r in registers as a window that shifts by one column per step (x[m] =
+// A(r, j + m) at step j), so the current column is always x[0] and the
+// update reads the published pivot column at static offsets from a moving
+// base. Results are collected in LDS and written once at the end, so the
+// per-step barriers wait on LDS only. Writes L_kk into M and a column-major
+// copy plus the reciprocal pivots into Ld.
 template <class TV>
-__global__ void __launch_bounds__(64) k_chol_panel(int k, int T, int64_t lda, TV* __restrict__ M, int* __restrict__ fail) {
-    __shared__ TV D[CB][CB + 1];
-    __shared__ TV X[CB][CB + 1];
+__global__ void __launch_bounds__(64) k_chol_diag(int k, int64_t lda, TV* __restrict__ M, TV* __restrict__ Ld,
+                                                  int* __restrict__ fail) {
+    __shared__ TV col[2 * CB];          // L(l, j) of the current step, zero-padded
+    __shared__ TV Lo[CB * CB + CB];     // Lo[j * CB + l] = L(l, j); reciprocal pivots at the end
     const int r = threadIdx.x;
-    const int i = k + blockIdx.x;
     const int64_t c0 = (int64_t)k * CB;
     TV* Mk = M + c0 * lda;
-    for (int c = 0; c < CB; ++c) D[r][c] = Mk[c0 + r + (int64_t)c * lda];
-    if (i != k)
-        for (int c = 0; c < CB; ++c) X[r][c] = Mk[(int64_t)i * CB + r + (int64_t)c * lda];
+    TV x[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) x[c] = Mk[c0 + r + (int64_t)c * lda];
+    col[CB + r] = TV(0);
+    bool bad_any = false;
+    for (int j = 0; j < CB; ++j) {
+        const TV p = __shfl(x[0], j);
+        const bool bad = !(p > TV(0));
+        bad_any |= bad;
+        const TV d = bad ? TV(1) : sqrt(p);
+        const TV inv = TV(1) / d;
+        const TV lrj = (r > j) ? x[0] * inv : (r == j ? d : TV(0));
+        col[r] = lrj;
+        Lo[j * CB + r] = lrj;
+        if (r == j) Lo[CB * CB + j] = inv;
+        // one wave: its LDS operations complete in program order, so a
+        // wavefront-scope fence (compiler ordering only) replaces the barrier
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const TV* cj = col + j;
+#pragma unroll
+        for (int m = 1; m < CB; ++m) x[m - 1] = x[m] - lrj * cj[m];
+        x[CB - 1] = TV(0);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (bad_any && r == 0) fail[0] = 1;
+#pragma unroll
+    for (int e = 0; e < CB + 1; ++e) Ld[e * CB + r] = Lo[e * CB + r];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+        if (c <= r) Mk[c0 + r + (int64_t)c * lda] = Lo[c * CB + r];
+}
+
+// Panel step k, part 2: workgroup w solves X L_kk^T = A_ik for row block
+// i = k + 1 + w (i == T is the right-hand-side block), lane r = row r held in
+// a shifting register window as above, L_kk staged in LDS (read-only here).
+template <class TV>
+__global__ void __launch_bounds__(64) k_chol_trsm(int k, int64_t lda, TV* __restrict__ M, const TV* __restrict__ Ld) {
+    __shared__ TV Lc[CB * CB + 2 * CB];  // Lc[j * CB + l] = L(l, j); reciprocal pivots at CB*CB; zero tail
+    const int r = threadIdx.x;
+    const int i = k + 1 + blockIdx.x;
+    const int64_t c0 = (int64_t)k * CB;
+    TV* Mk = M + c0 * lda;
+    // all loads in flight at once (a rolled loop pays the memory latency per load)
+#pragma unroll
+    for (int e = 0; e < CB + 1; ++e) Lc[e * CB + r] = Ld[e * CB + r];
+    TV y[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) y[c] = Mk[(int64_t)i * CB + r + (int64_t)c * lda];
+    __shared__ TV rp[CB];
+    __syncthreads();
+    rp[r] = Lc[CB * CB + r];
+    Lc[CB * CB + r] = TV(0);  // the window below reads past column 63: keep it zero
+    Lc[CB * CB + CB + r] = TV(0);
     __syncthreads();
     for (int j = 0; j < CB; ++j) {
-        if (r >= j) {
-            TV s0 = D[r][j], s1 = 0, s2 = 0, s3 = 0;
-            int l = 0;
-            for (; l + 3 < j; l += 4) {
-                s0 -= D[r][l] * D[j][l];
-                s1 -= D[r][l + 1] * D[j][l + 1];
-                s2 -= D[r][l + 2] * D[j][l + 2];
-                s3 -= D[r][l + 3] * D[j][l + 3];
-            }
-            for (; l < j; ++l) s0 -= D[r][l] * D[j][l];
-            D[r][j] = (s0 + s1) + (s2 + s3);
-        }
-        __syncthreads();
-        const TV p = D[j][j];
-        const bool bad = !(p > TV(0));
-        const TV d = bad ? TV(1) : sqrt(p);
-        if (bad && r == j && blockIdx.x == 0) fail[0] = 1;
-        __syncthreads();
-        if (r > j) D[r][j] = D[r][j] / d;
-        else if (r == j) D[j][j] = d;
-        __syncthreads();
+        const TV yj = y[0] * rp[j];
+        Mk[(int64_t)i * CB + r + (int64_t)j * lda] = yj;
+        const TV* lj = Lc + j * (CB + 1);  // lj[m] = L(j + m, j)
+#pragma unroll
+        for (int m = 1; m < CB; ++m) y[m - 1] = y[m] - yj * lj[m];
+        y[CB - 1] = TV(0);
     }
-    if (i == k) {
-        for (int c = 0; c <= r; ++c) Mk[c0 + r + (int64_t)c * lda] = D[r][c];
-        return;
-    }
-    for (int j = 0; j < CB; ++j) {
-        TV s0 = X[r][j], s1 = 0, s2 = 0, s3 = 0;
-        int l = 0;
-        for (; l + 3 < j; l += 4) {
-            s0 -= X[r][l] * D[j][l];
-            s1 -= X[r][l + 1] * D[j][l + 1];
-            s2 -= X[r][l + 2] * D[j][l + 2];
-            s3 -= X[r][l + 3] * D[j][l + 3];
-        }
-        for (; l < j; ++l) s0 -= X[r][l] * D[j][l];
-        X[r][j] = ((s0 + s1) + (s2 + s3)) / D[j][j];
-    }
-    for (int c = 0; c < CB; ++c) Mk[(int64_t)i * CB + r + (int64_t)c * lda] = X[r][c];
 }
 
 // Trailing update of step k: A_ij -= L_ik L_jk^T for k < j <= i <= T. Thread
@@ -100,10 +122,20 @@ __global__ void __launch_bounds__(256) k_chol_update(int k, int64_t lda, TV* __r
     __shared__ TV Bs[CB][CB];
     const int t = threadIdx.x;
     const int64_t c0 = (int64_t)k * CB;
-    for (int e = t; e < CB * CB; e += 256) {
-        const int row = e % CB, l = e / CB;
-        As[l][row] = M[(int64_t)i * CB + row + (c0 + l) * lda];
-        Bs[l][row] = M[(int64_t)j * CB + row + (c0 + l) * lda];
+    {
+        TV ta[CB * CB / 256], tb[CB * CB / 256];
+#pragma unroll
+        for (int u = 0; u < CB * CB / 256; ++u) {
+            const int e = t + 256 * u, row = e % CB, l = e / CB;
+            ta[u] = M[(int64_t)i * CB + row + (c0 + l) * lda];
+            tb[u] = M[(int64_t)j * CB + row + (c0 + l) * lda];
+        }
+#pragma unroll
+        for (int u = 0; u < CB * CB / 256; ++u) {
+            const int e = t + 256 * u, row = e % CB, l = e / CB;
+            As[l][row] = ta[u];
+            Bs[l][row] = tb[u];
+        }
     }
     __syncthreads();
     const int tx = t % 16, ty = t / 16;
@@ -137,35 +169,41 @@ __global__ void k_chol_gety(int np, int64_t lda, const TV* __restrict__ M, TV* _
     if (c < np) y[c] = M[np + (int64_t)c * lda];
 }
 
-// Backward substitution step b: x_b = L_bb^-T y_b (every workgroup), then
-// workgroup c < b updates y_c -= L_bc^T x_b; workgroup b stores x_b.
+// Backward substitution step b: x_b = L_bb^-T y_b (every workgroup, lane r
+// holding y_r, shuffles instead of barriers), then workgroup c < b updates
+// y_c -= L_bc^T x_b; workgroup b stores x_b.
 template <class TV>
 __global__ void __launch_bounds__(64) k_chol_bsolve(int b, int64_t lda, const TV* __restrict__ M, TV* __restrict__ y,
                                                     TV* __restrict__ x) {
-    __shared__ TV D[CB][CB + 1];
-    __shared__ TV xs[CB];
+    __shared__ TV Lr[CB][CB + 1];  // Lr[row][col] of the loaded block
     const int r = threadIdx.x;
     const int c = blockIdx.x;
     const int64_t o = (int64_t)b * CB;
-    for (int cc = 0; cc < CB; ++cc) D[r][cc] = M[o + r + (o + cc) * lda];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) Lr[r][cc] = M[o + r + (o + cc) * lda];
     __syncthreads();
     TV s = y[o + r];
+    TV xv[CB];
+    TV mine = 0;
+#pragma unroll
     for (int j = CB - 1; j >= 0; --j) {
-        if (r == j) xs[j] = s / D[j][j];
-        __syncthreads();
-        const TV xj = xs[j];
-        if (r < j) s -= D[j][r] * xj;
+        const TV xj = __shfl(s, j) / Lr[j][j];
+        xv[j] = xj;
+        if (r == j) mine = xj;
+        if (r < j) s -= Lr[j][r] * xj;
     }
     if (c == b) {
-        x[o + r] = xs[r];
+        x[o + r] = mine;
         return;
     }
     __syncthreads();
     const int64_t oc = (int64_t)c * CB;
-    for (int cc = 0; cc < CB; ++cc) D[r][cc] = M[o + r + (oc + cc) * lda];  // D[l][cc] = L_bc(l, cc)
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc) Lr[r][cc] = M[o + r + (oc + cc) * lda];  // Lr[l][cc] = L_bc(l, cc)
     __syncthreads();
     TV acc = 0;
-    for (int l = 0; l < CB; ++l) acc += D[l][r] * xs[l];
+#pragma unroll
+    for (int l = 0; l < CB; ++l) acc += Lr[l][r] * xv[l];
     y[oc + r] -= acc;
 }
 
@@ -186,12 +224,15 @@ void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const double* H, 
 }
 
 template <class TV>
-void launch_chol_solve(hipStream_t s, int n, TV* M, TV* y, TV* x, int* fail) {
+void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail) {
     const int np = chol_padded(n);
     const int64_t lda = chol_lda(n);
     const int T = np / CB;
+    TV* y = work;
+    TV* Ld = work + np;
     for (int k = 0; k < T; ++k) {
-        hipLaunchKernelGGL(k_chol_panel<TV>, dim3(T - k + 1), dim3(64), 0, s, k, T, lda, M, fail);
+        hipLaunchKernelGGL(k_chol_diag<TV>, dim3(1), dim3(64), 0, s, k, lda, M, Ld, fail);
+        hipLaunchKernelGGL(k_chol_trsm<TV>, dim3(T - k), dim3(64), 0, s, k, lda, M, (const TV*)Ld);
         if (k + 1 < T) hipLaunchKernelGGL(k_chol_update<TV>, dim3(T - k - 1, T - k), dim3(256), 0, s, k, lda, M);
     }
     hipLaunchKernelGGL(k_chol_gety<TV>, dim3((np + 255) / 256), dim3(256), 0, s, np, lda, (const TV*)M, y);

@@ -669,7 +669,7 @@ bool Body::solve_contact_system(int n, int precision, double* rhs) {
     int fail = 0;
     if (precision == 1) {
         if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
-        if (d_cy32_.n < (size_t)np) { d_cy32_.alloc(np); d_cx32_.alloc(np); }
+        if (d_cy32_.n < (size_t)chol_work(n)) { d_cy32_.alloc((size_t)chol_work(n)); d_cx32_.alloc(np); }
         launch_chol_assemble<float>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
         launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
         std::vector<float> x(n);
@@ -679,7 +679,7 @@ bool Body::solve_contact_system(int n, int precision, double* rhs) {
         for (int a = 0; a < n; ++a) rhs[a] = x[a];
     } else {
         if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
-        if (d_cy64_.n < (size_t)np) { d_cy64_.alloc(np); d_cx64_.alloc(np); }
+        if (d_cy64_.n < (size_t)chol_work(n)) { d_cy64_.alloc((size_t)chol_work(n)); d_cx64_.alloc(np); }
         launch_chol_assemble<double>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
         launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
         hip_check(hipMemcpyAsync(rhs, d_cx64_.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
