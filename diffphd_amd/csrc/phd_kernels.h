@@ -35,6 +35,12 @@ struct DevBoundary {
     const double* waves = nullptr;  // device (amp, kx, ky, phase) quadruples
 };
 
+// Axis stride of the S-product input w (3 axes): 64-aligned with room for the
+// 16-byte vector loads past the last column.
+__host__ __device__ inline int spmv_ldw(int n) { return ((n + 64 + 63) / 64) * 64; }
+// Tail padding (elements) of the factor arrays V64 / V32 for the same loads.
+constexpr int kVPad = 64;
+
 // ---- factor expansion and the S / S^T products --------------------------------
 // chain[i] = 1 when parent(i) == i + 1 and pattern(i) = {i} + pattern(i + 1).
 void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t* Lp, const int* Li, const double* Lx,

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include "phd_kernels.h"
+
+#include <cstring>
 #include "phd_math.h"
 
 namespace phd {
@@ -112,39 +114,72 @@ __global__ void __launch_bounds__(256) k_expand_s(int n, int t_begin, const int6
     }
 }
 
+// 16-byte vector of a value type: 4 floats or 2 doubles per lane load.
+template <class TV> struct Vec16;
+template <> struct Vec16<float> { using type = float4; static constexpr int N = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };
+
 // -----------------------------------------------------------------------------
 // y = S w: one wavefront per (row, column chunk) unit, 3 right-hand sides.
+// Each lane streams 16-byte vectors of the row (aligned: base[k] is a multiple
+// of 64), four in flight; w has axis stride spmv_ldw(n) so its vector loads are
+// aligned too. Elements outside [c0, c1] are masked.
 // -----------------------------------------------------------------------------
 template <class TV>
 __global__ void __launch_bounds__(256) k_spmv_rows(int nunits, const int* __restrict__ ru_row,
                                                    const int* __restrict__ ru_c0, const int* __restrict__ ru_c1,
                                                    const int64_t* __restrict__ base, const TV* __restrict__ V,
                                                    const TV* __restrict__ w, int n, TV* __restrict__ part) {
+    using V16 = typename Vec16<TV>::type;
+    constexpr int VN = Vec16<TV>::N;
+    constexpr int U = 4;
     const int u = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (u >= nunits) return;
+    const int ldw = spmv_ldw(n);
     const int k = ru_row[u];
     const int c0 = ru_c0[u], c1 = ru_c1[u];
     const TV* vr = V + base[k];
     TV a0 = 0, a1 = 0, a2 = 0;
-    int c = c0 + lane;
-    for (; c + 7 * WAVE <= c1; c += 8 * WAVE) {
-        TV v[8];
+    auto accumulate = [&](const TV (&v)[VN], int c) {
+        const V16 t0 = *reinterpret_cast<const V16*>(w + c);
+        const V16 t1 = *reinterpret_cast<const V16*>(w + ldw + c);
+        const V16 t2 = *reinterpret_cast<const V16*>(w + 2 * ldw + c);
+        TV x0[VN], x1[VN], x2[VN];
+        memcpy(x0, &t0, sizeof(t0));
+        memcpy(x1, &t1, sizeof(t1));
+        memcpy(x2, &t2, sizeof(t2));
 #pragma unroll
-        for (int s = 0; s < 8; ++s) v[s] = vr[c + s * WAVE];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int cc = c + s * WAVE;
-            a0 += v[s] * w[cc];
-            a1 += v[s] * w[n + cc];
-            a2 += v[s] * w[2 * n + cc];
+        for (int e = 0; e < VN; ++e) {
+            const TV vv = (c + e >= c0 && c + e <= c1) ? v[e] : TV(0);
+            a0 += vv * x0[e];
+            a1 += vv * x1[e];
+            a2 += vv * x2[e];
         }
+    };
+    constexpr int SPAN = VN * WAVE * U;
+    int cb = c0 & ~(VN - 1);  // uniform block start
+    // full blocks: every lane's vectors lie inside the row's column range
+    for (; cb + SPAN - 1 <= c1; cb += SPAN) {
+        TV v[U][VN];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const V16 t = *reinterpret_cast<const V16*>(vr + cb + q * VN * WAVE + VN * lane);
+            memcpy(v[q], &t, sizeof(t));
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) accumulate(v[q], cb + q * VN * WAVE + VN * lane);
     }
-    for (; c <= c1; c += WAVE) {
-        const TV v = vr[c];
-        a0 += v * w[c];
-        a1 += v * w[n + c];
-        a2 += v * w[2 * n + c];
+    // last partial block, masked per vector
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+        const int c = cb + q * VN * WAVE + VN * lane;
+        if (c <= c1) {
+            TV v[VN];
+            const V16 t = *reinterpret_cast<const V16*>(vr + c);
+            memcpy(v, &t, sizeof(t));
+            accumulate(v, c);
+        }
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
@@ -172,8 +207,12 @@ __global__ void k_spmv_rows_fin(int n, const int* __restrict__ row_unit_ptr, con
 }
 
 // -----------------------------------------------------------------------------
-// x = S^T y: one wavefront per (tile, item range) unit; lane l owns column
-// 64t + l and accumulates V[base[k] + j] * y[k] over the tile's rows.
+// x = S^T y: one wavefront per (tile, item range) unit. A row of the tile is
+// 64 columns = 64 / VN lanes of 16-byte vectors, so one load instruction
+// covers VN rows (lane sub-group `sub`); four such loads are in flight. The
+// item metadata of 64 rows is loaded cooperatively (one row per lane) and
+// broadcast with cross-lane reads. The sub-group partial sums are reduced at
+// the end.
 // -----------------------------------------------------------------------------
 template <class TV>
 __global__ void __launch_bounds__(256) k_spmv_cols(int nunits, const int* __restrict__ cu_tile,
@@ -181,16 +220,20 @@ __global__ void __launch_bounds__(256) k_spmv_cols(int nunits, const int* __rest
                                                    const int* __restrict__ items, const int* __restrict__ fd,
                                                    const int64_t* __restrict__ base, const TV* __restrict__ V,
                                                    const TV* __restrict__ y, int n, TV* __restrict__ xpart) {
+    using V16 = typename Vec16<TV>::type;
+    constexpr int VN = Vec16<TV>::N;
+    constexpr int LPR = WAVE / VN;  // lanes per row
+    constexpr int U = 4;
     const int u = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (u >= nunits) return;
     const int t = cu_tile[u];
-    const int j = 64 * t + lane;
+    const int sub = lane / LPR, g = lane % LPR;
+    const int jb = 64 * t + VN * g;
     const int i0 = cu_i0[u], i1 = cu_i1[u];
-    TV a0 = 0, a1 = 0, a2 = 0;
-    // The item metadata of 64 rows is loaded cooperatively (one row per lane)
-    // and broadcast with cross-lane reads, so the value loads of eight rows
-    // are in flight together without dependent scalar loads in between.
+    TV acc0[VN], acc1[VN], acc2[VN];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) acc0[e] = acc1[e] = acc2[e] = 0;
     for (int it0 = i0; it0 < i1; it0 += WAVE) {
         const int my = it0 + lane;
         const int cnt = min(WAVE, i1 - it0);
@@ -207,29 +250,54 @@ __global__ void __launch_bounds__(256) k_spmv_cols(int nunits, const int* __rest
             my1 = y[n + mk];
             my2 = y[2 * n + mk];
         }
-        for (int q = 0; q < cnt; q += 8) {
-            TV v[8];
+        for (int q = 0; q < cnt; q += VN * U) {
+            TV v[U][VN];
+            TV ys0[U], ys1[U], ys2[U];
+            int kk[U], ff[U];
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int src = q + s;
-                const int kk = __shfl(mk, src, WAVE);
-                const int ff = __shfl(mf, src, WAVE);
+            for (int s = 0; s < U; ++s) {
+                const int src = q + s * VN + sub;
+                kk[s] = __shfl(mk, src, WAVE);
+                ff[s] = __shfl(mf, src, WAVE);
                 const int64_t bb = ((int64_t)__shfl(mbhi, src, WAVE) << 32) | (uint32_t)__shfl(mblo, src, WAVE);
-                v[s] = (src < cnt && j >= ff && j <= kk) ? V[bb + j] : TV(0);
+                ys0[s] = __shfl(my0, src, WAVE);
+                ys1[s] = __shfl(my1, src, WAVE);
+                ys2[s] = __shfl(my2, src, WAVE);
+                V16 tv;
+                if (src < cnt) tv = *reinterpret_cast<const V16*>(V + bb + jb);
+                else memset(&tv, 0, sizeof(tv));
+                memcpy(v[s], &tv, sizeof(tv));
+                if (src >= cnt) kk[s] = -1;
             }
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int src = q + s;
-                a0 += v[s] * __shfl(my0, src, WAVE);
-                a1 += v[s] * __shfl(my1, src, WAVE);
-                a2 += v[s] * __shfl(my2, src, WAVE);
-            }
+            for (int s = 0; s < U; ++s)
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    const int j = jb + e;
+                    const TV vv = (j >= ff[s] && j <= kk[s]) ? v[s][e] : TV(0);
+                    acc0[e] += vv * ys0[s];
+                    acc1[e] += vv * ys1[s];
+                    acc2[e] += vv * ys2[s];
+                }
         }
     }
-    TV* xp = xpart + (int64_t)u * 3 * WAVE;
-    xp[lane] = a0;
-    xp[WAVE + lane] = a1;
-    xp[2 * WAVE + lane] = a2;
+#pragma unroll
+    for (int off = LPR; off < WAVE; off *= 2)
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            acc0[e] += __shfl_xor(acc0[e], off, WAVE);
+            acc1[e] += __shfl_xor(acc1[e], off, WAVE);
+            acc2[e] += __shfl_xor(acc2[e], off, WAVE);
+        }
+    if (sub == 0) {
+        TV* xp = xpart + (int64_t)u * 3 * WAVE;
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            xp[VN * g + e] = acc0[e];
+            xp[WAVE + VN * g + e] = acc1[e];
+            xp[2 * WAVE + VN * g + e] = acc2[e];
+        }
+    }
 }
 
 template <class TV>
@@ -267,9 +335,10 @@ __global__ void k_perm_in(int n, const int* __restrict__ perm, const double* __r
     if (j >= n) return;
     const int v = perm[j];
     const bool z = dmask && dmask[v];
+    const int ldw = spmv_ldw(n);
     w[j] = z ? TV(0) : (TV)r[3 * v];
-    w[n + j] = z ? TV(0) : (TV)r[3 * v + 1];
-    w[2 * n + j] = z ? TV(0) : (TV)r[3 * v + 2];
+    w[ldw + j] = z ? TV(0) : (TV)r[3 * v + 1];
+    w[2 * ldw + j] = z ? TV(0) : (TV)r[3 * v + 2];
 }
 
 // -----------------------------------------------------------------------------
@@ -911,8 +980,20 @@ __global__ void k_assemble_msys_idx(int nJ, const int* __restrict__ J, const dou
 }
 
 // Plain streaming copy (calibration of the HBM byte counters).
-__global__ void k_copy(int64_t n, const float4* __restrict__ a, float4* __restrict__ b) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
+// Four 16-byte loads in flight per lane, one pass over the data.
+__global__ void __launch_bounds__(256) k_copy(int64_t n, const float4* __restrict__ a, float4* __restrict__ b) {
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x * 4 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = i0 + (int64_t)q * blockDim.x;
+        if (i < n) v[q] = a[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t i = i0 + (int64_t)q * blockDim.x;
+        if (i < n) b[i] = v[q];
+    }
 }
 
 }  // namespace
@@ -1080,7 +1161,7 @@ template void launch_assemble_msys_idx<float>(hipStream_t, int, const int*, cons
 template void launch_assemble_msys_idx<double>(hipStream_t, int, const int*, const double*, int, const double*,
                                                const double*, const double*, double*);
 void launch_copy(hipStream_t s, int64_t n16, const void* a, void* b) {
-    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, s, n16, (const float4*)a, (float4*)b);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, s, n16, (const float4*)a, (float4*)b);
 }
 void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r) {
     if (K == 0) return;

@@ -399,8 +399,8 @@ void Body::setup(double dt, int threads) {
         ddep.upload(factor_.depth);
         d_fd_.upload(factor_.fd);
         d_base_.upload(factor_.base);
-        V64_.alloc(factor_.v_len);
-        V32_.alloc(factor_.v_len);
+        V64_.alloc(factor_.v_len + kVPad);
+        V32_.alloc(factor_.v_len + kVPad);
         hip_check(hipMemsetAsync(V64_.p, 0, V64_.n * sizeof(double), stream_), "memset V64");
         hip_check(hipMemsetAsync(V32_.p, 0, V32_.n * sizeof(float), stream_), "memset V32");
         // chain[i]: column i + 1 is the parent of i and pattern(i) = {i} + pattern(i + 1)
@@ -486,8 +486,8 @@ void Body::setup(double dt, int threads) {
     d_elem_.alloc((size_t)ne_ * edim_ * 3);
     d_dot_partial_.alloc(dots_partial_size());
     d_dot_out_.alloc(32);
-    w32_.alloc((size_t)3 * nv_); y32_.alloc((size_t)3 * nv_);
-    w64_.alloc((size_t)3 * nv_); y64_.alloc((size_t)3 * nv_);
+    w32_.alloc((size_t)3 * spmv_ldw(nv_)); y32_.alloc((size_t)3 * nv_);
+    w64_.alloc((size_t)3 * spmv_ldw(nv_)); y64_.alloc((size_t)3 * nv_);
     part32_.alloc(3 * plan_.ru_row.size() + 3); part64_.alloc(3 * plan_.ru_row.size() + 3);
     xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
     setup_contact(threads);
@@ -609,7 +609,8 @@ void Body::spmv_S(const double* r, DevBuf<TV>& w, DevBuf<TV>& y, DevBuf<TV>& par
     launch_perm_in<TV>(stream_, nv_, d_perm_.p, r, nullptr, w.p);
     hipEvent_t a = nullptr, b = nullptr;
     if (timing) { a = next_event(); b = next_event(); (void)hipEventRecord(a, stream_); }
-    launch_spmv_rows<TV>(stream_, (int)plan_.ru_row.size(), d_ru_row_.p, d_ru_c0_.p, d_ru_c1_.p, d_base_.p, V, w.p, nv_, part.p);
+    launch_spmv_rows<TV>(stream_, (int)plan_.ru_row.size(), d_ru_row_.p, d_ru_c0_.p, d_ru_c1_.p, d_base_.p, V, w.p, nv_,
+                         part.p);
     if (timing) { (void)hipEventRecord(b, stream_); ev_rows_.push_back({a, b}); }
     launch_spmv_rows_fin<TV>(stream_, nv_, d_row_unit_ptr_.p, part.p, y.p);
 }

@@ -109,8 +109,16 @@ def test_press_frictionless_converged():
     prod2 = product_body(spec)
     qm, _, am = rollout(prod2, spec, 3, dict(opt, phd_precision=1))
     print("mixed", rel_err(qm[3], qo[3]))
-    assert am[3] == ao[3]
     assert rel_err(qm[3], qo[3]) < 1e-7
+    # Mixed mode solves the contact system in FP32: contacts sitting at the
+    # complementarity boundary (lambda ~ 0) may fall on either side of it.
+    cand = np.asarray(spec["boundary"][2])
+    lm, lo = prod2.last_contact_multipliers()[0], orc.last_contact_multipliers()[0]
+    scale = max(np.abs(lo).max(), 1e-300)
+    for v in set(am[3]) ^ set(ao[3]):
+        c = int(np.flatnonzero(cand == v)[0])
+        print("boundary contact", v, lm[c], lo[c])
+        assert abs(lm[c]) <= 1e-6 * scale and abs(lo[c]) <= 1e-6 * scale
 
 
 def test_press_friction_rollout_at_solve_tolerance():
