@@ -216,6 +216,14 @@ def main():
     achieved = bytes_per_launch / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
     n_dof = spec["q0"].size
     fwd_iters = int(np.sum(res["fwd_iters"]))
+    # HBM traffic per launch of the dominant kernel from the committed FETCH_SIZE /
+    # WRITE_SIZE passes (calibrated on the copy kernel), for the workload they cover
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"spmv_press_{cells[0]}x{cells[1]}x{cells[2]}_pmc_traffic.json")
+    if os.path.exists(pmc):
+        kern = ("k_spmv_rows" if dom == "spmv_rows" else "k_spmv_cols") + ("<float>" if bv == 4 else "<double>")
+        traffic = json.load(open(pmc))["kernels"].get(kern, {}).get("traffic_bytes")
+    copy_ms = res["body"].copy_probe(1 << 30, 10)
     out = {
         "metric": "fwd+bwd steps/s @131k-hex contact scene, mixed/learned vs plain FP64 PD",
         "value": steps_per_s,
@@ -235,7 +243,9 @@ def main():
                    "active_contacts": res["active"], "fwd_pd_iters_timed": fwd_iters,
                    "setup_s": res["setup_s"], "factor_nnz": nnz, "factor_max_depth": res["max_depth"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/spmv_press_*_pmc_traffic.md" if traffic else None,
+                     "copy_kernel_GBs": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": dom_avg,
                      "spmv_rows_avg_ms": rows_avg, "spmv_cols_avg_ms": cols_avg,
                      "launches": {"rows": tim["rows_launches"], "cols": tim["cols_launches"]}},
