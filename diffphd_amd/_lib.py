@@ -13,6 +13,7 @@ _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
 _P = ctypes.c_void_p
 _cp = ctypes.c_char_p
 
@@ -48,6 +49,17 @@ SIGNATURES = [
     ("phd_phase_times", ctypes.c_int, [_P, _dp, ctypes.c_int, ctypes.c_int]),
     ("phd_last_contact_multipliers", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_int]),
     ("phd_dense_spd_solve", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    ("phd_mlp_num_params", ctypes.c_int, []),
+    ("phd_mlp_init", ctypes.c_int, [_P, ctypes.c_int]),
+    ("phd_mlp_set_normalization", ctypes.c_int, [_P, _fp, _fp, ctypes.c_float]),
+    ("phd_mlp_get_params", ctypes.c_int, [_P, _fp]),
+    ("phd_mlp_set_params", ctypes.c_int, [_P, _fp]),
+    ("phd_mlp_forward", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp]),
+    ("phd_mlp_loss_grad", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp, _fp, _dp]),
+    ("phd_mlp_train", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, _fp]),
+    ("phd_mlp_features", ctypes.c_int, [_P, _dp, _dp, ctypes.c_double, _fp]),
+    ("phd_warm_start_stats", ctypes.c_int, [_P, _ip, _ip, ctypes.c_int]),
+    ("phd_mlp_mfma_probe", ctypes.c_int, [_fp, _fp, _fp]),
     ("phd_copy_probe", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_int, _dp]),
     ("phd_host_factor_check", ctypes.c_int, [ctypes.c_int, _ip, _ip, _dp, _dp, _dp]),
 ]
@@ -71,6 +83,11 @@ def lib():
 def dptr(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
 
 
 def iptr(a):

@@ -8,6 +8,7 @@
 
 #include "phd_solver.h"
 #include "phd_dense.h"
+#include "phd_mlp.h"
 
 namespace {
 struct Ctx {
@@ -201,6 +202,83 @@ int phd_dense_spd_solve(int n, const double* A, const double* b, double* x, int 
         };
         if (precision == 1) run(float{}); else run(double{});
         return fail ? 1 : 0;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return -1;
+    }
+}
+
+int phd_mlp_num_params(void) { return phd::MlpLayout::total; }
+
+int phd_mlp_init(phd_ctx ctx, int seed) {
+    return guard(ctx, [&](phd::Body& b) { b.net().init((uint64_t)seed, b.stream()); });
+}
+int phd_mlp_set_normalization(phd_ctx ctx, const float* fmean, const float* fstd, float tscale) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::WarmStartNet& n = b.net();
+        n.fmean.assign(fmean, fmean + phd::kMlpIn);
+        n.fstd.assign(fstd, fstd + phd::kMlpIn);
+        for (float s : n.fstd) if (!(s > 0)) throw std::runtime_error("feature scales must be positive");
+        n.tscale = tscale;
+    });
+}
+int phd_mlp_get_params(phd_ctx ctx, float* out) {
+    return guard(ctx, [&](phd::Body& b) { b.net().get_params(out, b.stream()); });
+}
+int phd_mlp_set_params(phd_ctx ctx, const float* p) {
+    return guard(ctx, [&](phd::Body& b) { b.net().set_params(p, b.stream()); });
+}
+int phd_mlp_forward(phd_ctx ctx, int n, const float* feat, float* out) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::DevBuf<float> f, o;
+        f.upload(feat, (size_t)n * phd::kMlpInPad);
+        o.alloc((size_t)n * phd::kMlpOut);
+        b.net().forward(b.stream(), n, f.p, o.p);
+        phd::hip_check(hipStreamSynchronize(b.stream()), "mlp forward");
+        o.download(out, (size_t)n * phd::kMlpOut);
+    });
+}
+int phd_mlp_loss_grad(phd_ctx ctx, int n, const float* feat, const float* target, float* grad, double* loss) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::DevBuf<float> f, t, g;
+        f.upload(feat, (size_t)n * phd::kMlpInPad);
+        t.upload(target, (size_t)n * phd::kMlpOut);
+        g.alloc(phd::MlpLayout::total);
+        *loss = b.net().loss_grad(b.stream(), n, f.p, t.p, g.p);
+        g.download(grad, phd::MlpLayout::total);
+    });
+}
+int phd_mlp_train(phd_ctx ctx, int n, const float* feat, const float* target, int steps, int batch, float lr, int seed,
+                  float* losses) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::DevBuf<float> f, t;
+        f.upload(feat, (size_t)n * phd::kMlpInPad);
+        t.upload(target, (size_t)n * phd::kMlpOut);
+        b.net().train(b.stream(), n, f.p, t.p, steps, batch, lr, (uint64_t)seed, losses);
+    });
+}
+int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, double dt, float* feat) {
+    return guard(ctx, [&](phd::Body& b) {
+        phd::DevBuf<double> dq, dv;
+        dq.upload(q, (size_t)b.dofs());
+        dv.upload(v, (size_t)b.dofs());
+        b.mlp_features(dq.p, dv.p, dt, false);
+        phd::hip_check(hipMemcpyAsync(feat, b.feat_dev(), sizeof(float) * (size_t)(b.dofs() / 3) * phd::kMlpInPad,
+                                      hipMemcpyDeviceToHost, b.stream()), "features D2H");
+        phd::hip_check(hipStreamSynchronize(b.stream()), "features");
+    });
+}
+int phd_warm_start_stats(phd_ctx ctx, int* used, int* fallback, int reset) {
+    return guard(ctx, [&](phd::Body& b) {
+        *used = b.warm_used;
+        *fallback = b.warm_fallback;
+        if (reset) b.warm_used = b.warm_fallback = 0;
+    });
+}
+int phd_mlp_mfma_probe(const float* A, const float* B, float* D) {
+    try {
+        phd::mlp_mfma_probe(A, B, D);
+        return 0;
     } catch (const std::exception& e) {
         g_create_error = e.what();
         return -1;

@@ -13,6 +13,7 @@
 // factor the form is algebraically identical to x = A^-1 b.
 #include "phd_solver.h"
 #include "phd_dense.h"
+#include "phd_mlp.h"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -659,6 +660,97 @@ void Body::kpd_apply(const double* x, double* out, int precision) {
     launch_gather_vertex(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, out);
 }
 
+WarmStartNet& Body::net() {
+    if (!net_) net_.reset(new WarmStartNet());
+    return *net_;
+}
+
+// Per-vertex inputs of the warm-start network at state (q, v): mean log10
+// element stiffness, contact flag (candidate with lambda > 0 in the previous
+// frame), obstacle distance for candidates.
+void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, bool standardize) {
+    const double L = std::cbrt(element_volume_);
+    if (d_logk_.n != (size_t)nv_) {
+        std::vector<double> acc(nv_, 0.0);
+        std::vector<int> cnt(nv_, 0);
+        for (int e = 0; e < ne_; ++e) {
+            double w = 0;
+            for (const Energy& en : energies_) if (en.member[e]) w += en.elem_w(e);
+            const double lw = std::log10(std::max(w, 1e-300));
+            for (int k = 0; k < edim_; ++k) { acc[elems_[(size_t)e * edim_ + k]] += lw; cnt[elems_[(size_t)e * edim_ + k]]++; }
+        }
+        std::vector<float> lk(nv_);
+        for (int i = 0; i < nv_; ++i) lk[i] = cnt[i] ? (float)(acc[i] / cnt[i]) : 0.f;
+        d_logk_.upload(lk);
+        d_Xrest_.upload(X_);
+    }
+    std::vector<float> cf(nv_, 0.f), cd(nv_, 4.f);
+    std::vector<double> qh;
+    if (!cand_.empty() || !fb_vertices_.empty()) {
+        qh.resize(dofs_);
+        hip_check(hipMemcpyAsync(qh.data(), q_dev, sizeof(double) * dofs_, hipMemcpyDeviceToHost, stream_), "q D2H");
+        hip_check(hipStreamSynchronize(stream_), "features");
+        for (auto& kv : fb_vertices_) {
+            double d, nn[3];
+            boundary_.eval(&qh[3 * kv.first], d, nn);
+            cd[kv.first] = (float)std::max(-4.0, std::min(4.0, d / L));
+        }
+        for (size_t i = 0; i < prev_candidates_.size() && i < prev_lambda_.size(); ++i)
+            if (prev_lambda_[i] > 0) cf[prev_candidates_[i]] = 1.f;
+    }
+    d_cflag_.upload(cf);
+    d_cdist_.upload(cd);
+    if (d_feat_.n < (size_t)nv_ * kMlpInPad) d_feat_.alloc((size_t)nv_ * kMlpInPad);
+    WarmStartNet& nt = net();
+    d_fmean_.upload(nt.fmean);
+    d_fstd_.upload(nt.fstd);
+    double g = 0;
+    if (gravity_.size() == 3) g = std::sqrt(gravity_[0] * gravity_[0] + gravity_[1] * gravity_[1] + gravity_[2] * gravity_[2]);
+    launch_mlp_features(stream_, nv_, q_dev, v_dev, d_Xrest_.p, d_logk_.p, d_cflag_.p, d_cdist_.p, dt, L, g, d_fmean_.p,
+                        d_fstd_.p, standardize ? 1 : 0, d_feat_.p);
+}
+
+// Initial iterate of the PD loop from the network: x + h v + tscale * net,
+// kept only when its first global step is smaller than that of the plain
+// x + h v guess (learned = false: x + h v). d_q_ holds q on entry and the
+// chosen iterate on return.
+void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) {
+    WarmStartNet& nt = net();
+    check(!learned || nt.ready(), "phd_warm_start = 1 needs a trained warm-start network (phd_mlp_init / phd_mlp_train).");
+    const int n3 = dofs_;
+    d_v_.upload(v, n3);
+    if (d_mlpout_.n < (size_t)n3) d_mlpout_.alloc(n3);
+    if (learned) {
+        mlp_features(d_q_.p, d_v_.p, dt, true);
+        nt.forward(stream_, nv_, d_feat_.p, d_mlpout_.p);
+    } else {
+        hip_check(hipMemsetAsync(d_mlpout_.p, 0, sizeof(float) * n3, stream_), "no net");
+    }
+    if (d_guess_.n < (size_t)2 * n3) d_guess_.alloc((size_t)2 * n3);
+    launch_warm_guess(stream_, nv_, d_q_.p, d_v_.p, dt, d_mlpout_.p, nt.tscale, d_dmask_.p, d_dval_full_.p, d_guess_.p,
+                      d_guess_.p + n3);
+    if (!learned) {
+        hip_check(hipMemcpyAsync(d_q_.p, d_guess_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "x + h v");
+        return;
+    }
+    // size of the first global step from each guess, ||A^-1 (b(g) - A g)||
+    // (contact impulses zero): A^-1 damps the high-frequency part of a
+    // per-vertex prediction's error, which the first solve removes anyway
+    double rn[2];
+    for (int g = 0; g < 2; ++g) {
+        const double* x = d_guess_.p + (size_t)g * n3;
+        local_rhs(x, inv_h2m);
+        launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, x, d_dmask_.p, d_r_.p);
+        spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+        spmv_ST<double>(y64_, xpart64_, V64_.p, nullptr, nullptr, d_tmp_.p);
+        rn[g] = std::sqrt(dots({{d_tmp_.p, d_tmp_.p}})[0]);
+    }
+    const int pick = (std::isfinite(rn[1]) && rn[1] < rn[0]) ? 1 : 0;
+    if (pick) ++warm_used; else ++warm_fallback;
+    hip_check(hipMemcpyAsync(d_q_.p, d_guess_.p + (size_t)pick * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_),
+              "warm start");
+}
+
 // Contact system M dl = rhs on the index set J (already in d_idxJ_, with
 // d_om_ / d_E_ / d_nrm_ current): assembled and solved by the blocked
 // Cholesky of phd_dense.hip. rhs (host, n) is overwritten with the solution.
@@ -797,6 +889,9 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         hip_check(hipMemsetAsync(d_F_.p, 0, sizeof(double) * 3 * K, stream_), "F=0");
         if ((int)d_rhsK_.n < K) d_rhsK_.alloc(K);
     }
+    const int warm = (int)opt_or(opt, "phd_warm_start", 0.0);
+    check(warm >= 0 && warm <= 2, "phd_warm_start must be 0 (q_t), 1 (learned, falls back to q + h v) or 2 (q + h v).");
+    if (warm > 0) warm_start(v, h, inv_h2m, warm == 1);
     DevAnderson aa;
     aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
     hip_check(hipMemsetAsync(d_pdrhs_prev_.p, 0, sizeof(double) * n3, stream_), "prev pd_rhs");

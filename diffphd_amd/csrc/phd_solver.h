@@ -120,6 +120,8 @@ struct StepStats {
     double seconds = 0;
 };
 
+class WarmStartNet;
+
 class Body {
 public:
     explicit Body(int device);
@@ -175,6 +177,13 @@ public:
     void collect_timing();
     const Factor& factor() const { return factor_; }
 
+    // ---- learned warm start (option phd_warm_start = 1) ----
+    WarmStartNet& net();
+    // raw (standardize = false) or standardized features of state (q, v) into d_feat_
+    void mlp_features(const double* q_dev, const double* v_dev, double dt, bool standardize);
+    const float* feat_dev() const { return d_feat_.p; }
+    int warm_used = 0, warm_fallback = 0;
+
 private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
@@ -224,6 +233,11 @@ private:
     bool kpd_exact_ = false;
     // work vectors (3N FP64)
     DevBuf<double> d_q_, d_qprev_, d_qtilde_, d_b_, d_r_, d_pdrhs_, d_pdrhs_prev_, d_elem_, d_g_, d_tmp_, d_kz_;
+    // warm start: network, per-vertex feature inputs, guesses
+    std::unique_ptr<WarmStartNet> net_;
+    DevBuf<float> d_logk_, d_cflag_, d_cdist_, d_feat_, d_mlpout_, d_fmean_, d_fstd_;
+    DevBuf<double> d_Xrest_, d_v_, d_guess_;
+    void warm_start(const double* v, double dt, double inv_h2m, bool learned);
     DevBuf<double> d_hist_q_[8], d_hist_g_[8];
     DevBuf<double> d_dot_partial_, d_dot_out_;
     DevBuf<float> w32_, y32_, part32_, xpart32_;

@@ -7,6 +7,8 @@ caller-allocated `StdRealVector`s that the call resizes and fills
 (deformable.cpp:157-187), errors are raised as RuntimeError. The work is done
 by libphd.so through the C ABI of include/phd.h.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -264,6 +266,59 @@ class _Deformable:
         c, cp = _lib.dptr(np.zeros(cap))
         n = _lib.lib().phd_last_contact_multipliers(self._ctx, ap, bp, cp, cap)
         return a[:n].copy(), b[:n].copy(), c[:n].copy()
+
+    # ---- learned warm start (phd_warm_start option) ----
+    def mlp_init(self, seed=0):
+        self._ok(_lib.lib().phd_mlp_init(self._ctx, int(seed)))
+
+    def mlp_set_normalization(self, fmean, fstd, tscale):
+        a, ap = _lib.fptr(fmean)
+        b, bp = _lib.fptr(fstd)
+        self._ok(_lib.lib().phd_mlp_set_normalization(self._ctx, ap, bp, float(tscale)))
+
+    def mlp_params(self):
+        out, op = _lib.fptr(np.zeros(_lib.lib().phd_mlp_num_params()))
+        self._ok(_lib.lib().phd_mlp_get_params(self._ctx, op))
+        return out.copy()
+
+    def mlp_set_params(self, p):
+        a, ap = _lib.fptr(p)
+        self._ok(_lib.lib().phd_mlp_set_params(self._ctx, ap))
+
+    def mlp_features(self, q, v, dt):
+        """Raw per-vertex features (n x 16) of state (q, v) for this body."""
+        qa, qp = _lib.dptr(q)
+        va, vp = _lib.dptr(v)
+        out, op = _lib.fptr(np.zeros((self._dofs // 3, 16)))
+        self._ok(_lib.lib().phd_mlp_features(self._ctx, qp, vp, float(dt), op))
+        return out.copy()
+
+    def mlp_forward(self, feat):
+        f, fp = _lib.fptr(feat)
+        n = f.shape[0]
+        out, op = _lib.fptr(np.zeros((n, 3)))
+        self._ok(_lib.lib().phd_mlp_forward(self._ctx, n, fp, op))
+        return out.copy()
+
+    def mlp_loss_grad(self, feat, target):
+        f, fp = _lib.fptr(feat)
+        t, tp = _lib.fptr(target)
+        g, gp = _lib.fptr(np.zeros(_lib.lib().phd_mlp_num_params()))
+        loss = ctypes.c_double(0.0)
+        self._ok(_lib.lib().phd_mlp_loss_grad(self._ctx, f.shape[0], fp, tp, gp, ctypes.byref(loss)))
+        return loss.value, g.copy()
+
+    def mlp_train(self, feat, target, steps, batch, lr=1e-3, seed=0):
+        f, fp = _lib.fptr(feat)
+        t, tp = _lib.fptr(target)
+        losses, lp = _lib.fptr(np.zeros(max(steps, 1)))
+        self._ok(_lib.lib().phd_mlp_train(self._ctx, f.shape[0], fp, tp, int(steps), int(batch), float(lr), int(seed), lp))
+        return losses[:steps].copy()
+
+    def warm_start_stats(self, reset=True):
+        u, f = ctypes.c_int(0), ctypes.c_int(0)
+        self._ok(_lib.lib().phd_warm_start_stats(self._ctx, ctypes.byref(u), ctypes.byref(f), int(reset)))
+        return u.value, f.value
 
     def copy_probe(self, nbytes, reps=10):
         out, op = _lib.dptr(np.zeros(1))

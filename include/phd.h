@@ -122,6 +122,32 @@ int phd_last_contact_multipliers(phd_ctx ctx, double* lambda, double* lt1, doubl
  * Test entry point; no counterpart in the reference. */
 int phd_dense_spd_solve(int n, const double* A, const double* b, double* x, int precision);
 
+/* ---- Learned warm start of the PD iteration (no counterpart in the reference:
+ * its initial iterate is q_t, deformable_projective_dynamics_forward_gpu.cpp:1069).
+ * A per-vertex MLP 13 -> 128 -> 128 -> 128 -> 3 (GELU, BF16 weights on MFMA,
+ * FP32 accumulate) predicts (q_next - q - h v) / tscale from standardized
+ * features; forward option phd_warm_start = 1 starts the PD loop from the
+ * prediction when its PD residual is below that of q + h v. */
+int phd_mlp_num_params(void);
+int phd_mlp_init(phd_ctx ctx, int seed);
+/* feature means / scales (13 each) and the output scale */
+int phd_mlp_set_normalization(phd_ctx ctx, const float* fmean, const float* fstd, float tscale);
+int phd_mlp_get_params(phd_ctx ctx, float* out);       /* phd_mlp_num_params() floats */
+int phd_mlp_set_params(phd_ctx ctx, const float* params);
+/* feat: n x 16 standardized features (columns 13..15 zero); out: n x 3 */
+int phd_mlp_forward(phd_ctx ctx, int n, const float* feat, float* out);
+/* full-batch MSE mean((out - target)^2) over n x 3 and its parameter gradient */
+int phd_mlp_loss_grad(phd_ctx ctx, int n, const float* feat, const float* target, float* grad, double* loss);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of a seeded permutation; losses: steps floats or NULL */
+int phd_mlp_train(phd_ctx ctx, int n, const float* feat, const float* target, int steps, int batch, float lr, int seed,
+                  float* losses);
+/* raw (unstandardized) features of state (q, v) for this body: n x 16 */
+int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, double dt, float* feat);
+/* forward frames that started from the network / fell back to q + h v */
+int phd_warm_start_stats(phd_ctx ctx, int* used, int* fallback, int reset);
+/* D = A B for 16 x 16 row-major inputs rounded to BF16, through one MFMA (layout probe) */
+int phd_mlp_mfma_probe(const float* A, const float* B, float* D);
+
 /* Streams `nbytes` through a plain device copy kernel `reps` times and writes the
  * average duration in ms (calibrates the HBM byte counters and the bandwidth peak). */
 int phd_copy_probe(phd_ctx ctx, long long nbytes, int reps, double* ms_out);

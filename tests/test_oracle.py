@@ -254,3 +254,35 @@ def test_cantilever_gradients_match_central_fd(oracle_mod):
         an = info["dl_dmat_w"][e]
         print("w_e", e, an, fd)
         assert abs(an - fd) < 1e-5 * abs(fd) + 1e-12
+
+
+def test_mlp_oracle_gradient_matches_fd():
+    """Backpropagation of the warm-start MLP restatement (FP64, unquantized)
+    against central differences along random directions."""
+    import oracle.mlp as M
+    rng = np.random.default_rng(21)
+    p = 0.2 * rng.standard_normal(M.TOTAL)
+    P = M.unpack(p)
+    P["W1"][13:] = 0
+    P["W4"][:, 3:] = 0
+    P["b4"][3:] = 0
+    p = M.pack(P)
+    X = np.zeros((37, 16))
+    X[:, :13] = rng.standard_normal((37, 13))
+    T = rng.standard_normal((37, 3))
+    loss, g = M.loss_grad(p, X, T, quant=False)
+    for _ in range(4):
+        d = rng.standard_normal(M.TOTAL)
+        D = M.unpack(d)
+        D["W1"][13:] = 0
+        D["W4"][:, 3:] = 0
+        D["b4"][3:] = 0
+        d = M.pack(D)
+        eps = 1e-6
+        fd = (M.loss_grad(p + eps * d, X, T, quant=False)[0] - M.loss_grad(p - eps * d, X, T, quant=False)[0]) / (2 * eps)
+        print("mlp directional", g @ d, fd)
+        assert abs(g @ d - fd) < 1e-7 * max(1.0, abs(fd))
+    # BF16 rounding of the restatement: nearest even, exact on representable values
+    assert np.array_equal(M.bf16(np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8])), np.array([1.0, 1.0, 1.0 + 2 ** -6]))
+    lq, gq = M.loss_grad(p, X, T, quant=True)
+    assert abs(lq - loss) < 2e-2 * loss and np.linalg.norm(gq - g) < 5e-2 * np.linalg.norm(g)
