@@ -57,7 +57,7 @@ SIGNATURES = [
     ("phd_mlp_forward", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp]),
     ("phd_mlp_loss_grad", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp, _fp, _dp]),
     ("phd_mlp_train", ctypes.c_int, [_P, ctypes.c_int, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, _fp]),
-    ("phd_mlp_features", ctypes.c_int, [_P, _dp, _dp, ctypes.c_double, _fp]),
+    ("phd_mlp_features", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_double, _fp]),
     ("phd_warm_start_stats", ctypes.c_int, [_P, _ip, _ip, ctypes.c_int]),
     ("phd_mlp_mfma_probe", ctypes.c_int, [_fp, _fp, _fp]),
     ("phd_copy_probe", ctypes.c_int, [_P, ctypes.c_longlong, ctypes.c_int, _dp]),

@@ -257,8 +257,10 @@ int phd_mlp_train(phd_ctx ctx, int n, const float* feat, const float* target, in
         b.net().train(b.stream(), n, f.p, t.p, steps, batch, lr, (uint64_t)seed, losses);
     });
 }
-int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, double dt, float* feat) {
+int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt, float* feat) {
     return guard(ctx, [&](phd::Body& b) {
+        b.setup(dt, 8);
+        b.set_inertial_target(q, v, f_ext, dt);
         phd::DevBuf<double> dq, dv;
         dq.upload(q, (size_t)b.dofs());
         dv.upload(v, (size_t)b.dofs());

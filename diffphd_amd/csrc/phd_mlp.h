@@ -61,12 +61,13 @@ private:
 };
 
 // Raw per-vertex features (n x 16, columns 13..15 zero):
-//  0-2 (q - X) / L, 3-5 (q - h v - X) / L, 6-8 h v / L, 9 mean log10 element
-//  stiffness, 10 contact flag, 11 obstacle distance / L (clamped to [-4, 4]; 4
-//  off the contact set), 12 h^2 |g| / L.
-void launch_mlp_features(hipStream_t s, int n, const double* q, const double* v, const double* X, const float* logk,
-                         const float* contact, const float* dist, double h, double L, double gacc, const float* fmean,
-                         const float* fstd, int standardize, float* feat);
+//  0-2 (q - X) / L; 3-5 Jacobi step r_i / A_ii / L of the PD residual
+//  r = b(q + h v) - A (q + h v) at the x + h v guess; 6-8 h v / L; 9 mean
+//  log10 element stiffness; 10 contact flag; 11 obstacle distance / L
+//  (clamped to [-4, 4]; 4 off the contact set); 12 h^2 |g| / L.
+void launch_mlp_features(hipStream_t s, int n, const double* q, const double* v, const double* X, const double* res,
+                         const float* adiag, const float* logk, const float* contact, const float* dist, double h, double L,
+                         double gacc, const float* fmean, const float* fstd, int standardize, float* feat);
 
 // Initial iterates: g0 = q + h v, g1 = g0 + tscale * net (Dirichlet vertices fixed).
 void launch_warm_guess(hipStream_t s, int nv, const double* q, const double* v, double h, const float* net, float tscale,

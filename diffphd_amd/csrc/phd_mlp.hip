@@ -277,6 +277,7 @@ __global__ void k_mlp_pack(const float* __restrict__ master, uint16_t* __restric
 }
 
 __global__ void k_mlp_features(int n, const double* __restrict__ q, const double* __restrict__ v, const double* __restrict__ X,
+                               const double* __restrict__ res, const float* __restrict__ adiag,
                                const float* __restrict__ logk, const float* __restrict__ contact,
                                const float* __restrict__ dist, double h, double L, double gacc,
                                const float* __restrict__ fmean, const float* __restrict__ fstd, int standardize,
@@ -288,7 +289,7 @@ __global__ void k_mlp_features(int n, const double* __restrict__ q, const double
     for (int d = 0; d < 3; ++d) {
         const double qd = q[3 * i + d], vd = v[3 * i + d], xd = X[3 * i + d];
         f[d] = (float)((qd - xd) * iL);
-        f[3 + d] = (float)((qd - h * vd - xd) * iL);
+        f[3 + d] = (float)(res[3 * i + d] / (double)adiag[i] * iL);
         f[6 + d] = (float)(h * vd * iL);
     }
     f[9] = logk[i];
@@ -312,7 +313,7 @@ __global__ void k_warm_guess(int nv, const double* __restrict__ q, const double*
         const int k = 3 * i + d;
         const double a = fixed ? dval[k] : q[k] + h * v[k];
         g0[k] = a;
-        g1[k] = fixed ? a : a + (double)tscale * (double)net[k];
+        g1[k] = fixed ? a : a + (net ? (double)tscale * (double)net[k] : 0.0);
     }
 }
 
@@ -441,12 +442,12 @@ void WarmStartNet::train(hipStream_t s, int n, const float* feat, const float* t
     hip_check(hipStreamSynchronize(s), "train");
 }
 
-void launch_mlp_features(hipStream_t s, int n, const double* q, const double* v, const double* X, const float* logk,
-                         const float* contact, const float* dist, double h, double L, double gacc, const float* fmean,
-                         const float* fstd, int standardize, float* feat) {
+void launch_mlp_features(hipStream_t s, int n, const double* q, const double* v, const double* X, const double* res,
+                         const float* adiag, const float* logk, const float* contact, const float* dist, double h, double L,
+                         double gacc, const float* fmean, const float* fstd, int standardize, float* feat) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_mlp_features, dim3((n + 127) / 128), dim3(128), 0, s, n, q, v, X, logk, contact, dist, h, L, gacc,
-                       fmean, fstd, standardize, feat);
+    hipLaunchKernelGGL(k_mlp_features, dim3((n + 127) / 128), dim3(128), 0, s, n, q, v, X, res, adiag, logk, contact, dist,
+                       h, L, gacc, fmean, fstd, standardize, feat);
 }
 
 void launch_warm_guess(hipStream_t s, int nv, const double* q, const double* v, double h, const float* net, float tscale,

@@ -660,6 +660,17 @@ void Body::kpd_apply(const double* x, double* out, int precision) {
     launch_gather_vertex(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, out);
 }
 
+// q~ = q + h v + h^2 / m (f_ext + m g) (forward_gpu.cpp:430-436)
+void Body::set_inertial_target(const double* q, const double* v, const double* f_ext, double dt) {
+    const int n3 = dofs_;
+    const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass;
+    const double mg[3] = {gravity_.size() == 3 ? mass * gravity_[0] : 0.0, gravity_.size() == 3 ? mass * gravity_[1] : 0.0,
+                          gravity_.size() == 3 ? mass * gravity_[2] : 0.0};
+    std::vector<double> qt(n3);
+    for (int i = 0; i < n3; ++i) qt[i] = q[i] + h * v[i] + h2m * (f_ext ? f_ext[i] : 0.0) + h2m * mg[i % 3];
+    d_qtilde_.upload(qt);
+}
+
 WarmStartNet& Body::net() {
     if (!net_) net_.reset(new WarmStartNet());
     return *net_;
@@ -670,6 +681,14 @@ WarmStartNet& Body::net() {
 // frame), obstacle distance for candidates.
 void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, bool standardize) {
     const double L = std::cbrt(element_volume_);
+    const double mass = element_volume_ * density_, inv_h2m = mass / (dt * dt);
+    if (d_adiag_.n != (size_t)nv_) {
+        std::vector<float> ad(nv_, 1.f);
+        for (int i = 0; i < nv_; ++i)
+            for (int p = Ap_[i]; p < Ap_[i + 1]; ++p)
+                if (Ai_[p] == i) ad[i] = (float)Ax_[p];
+        d_adiag_.upload(ad);
+    }
     if (d_logk_.n != (size_t)nv_) {
         std::vector<double> acc(nv_, 0.0);
         std::vector<int> cnt(nv_, 0);
@@ -706,8 +725,14 @@ void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, boo
     d_fstd_.upload(nt.fstd);
     double g = 0;
     if (gravity_.size() == 3) g = std::sqrt(gravity_[0] * gravity_[0] + gravity_[1] * gravity_[1] + gravity_[2] * gravity_[2]);
-    launch_mlp_features(stream_, nv_, q_dev, v_dev, d_Xrest_.p, d_logk_.p, d_cflag_.p, d_cdist_.p, dt, L, g, d_fmean_.p,
-                        d_fstd_.p, standardize ? 1 : 0, d_feat_.p);
+    // PD residual at the x + h v guess (d_qtilde_ holds this frame's inertial target)
+    if (d_guess_.n < (size_t)2 * dofs_) d_guess_.alloc((size_t)2 * dofs_);
+    launch_warm_guess(stream_, nv_, q_dev, v_dev, dt, nullptr, 0.f, d_dmask_.p, d_dval_full_.p, d_guess_.p,
+                      d_guess_.p + dofs_);
+    local_rhs(d_guess_.p, inv_h2m);
+    launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_guess_.p, d_dmask_.p, d_r_.p);
+    launch_mlp_features(stream_, nv_, q_dev, v_dev, d_Xrest_.p, d_r_.p, d_adiag_.p, d_logk_.p, d_cflag_.p, d_cdist_.p, dt,
+                        L, g, d_fmean_.p, d_fstd_.p, standardize ? 1 : 0, d_feat_.p);
 }
 
 // Initial iterate of the PD loop from the network: x + h v + tscale * net,
@@ -853,10 +878,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     PhaseTimer t_total(this, kFwdTotal);
     const int n3 = dofs_;
     const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
-    std::vector<double> qt(n3);
-    const double mg[3] = {gravity_.size() == 3 ? mass * gravity_[0] : 0.0, gravity_.size() == 3 ? mass * gravity_[1] : 0.0,
-                          gravity_.size() == 3 ? mass * gravity_[2] : 0.0};
-    for (int i = 0; i < n3; ++i) qt[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * mg[i % 3];
+    (void)h2m;
     // new-rollout detection for the tape (:437-449)
     if (!tape_.empty()) {
         const auto& last = tape_.back();
@@ -864,7 +886,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         for (int i = 0; i < n3; ++i) { dn += (last.q_sol[i] - q[i]) * (last.q_sol[i] - q[i]); a += last.q_sol[i] * last.q_sol[i]; b += q[i] * q[i]; }
         if (!(std::sqrt(dn) <= 1e-8 * std::max(1.0, std::max(std::sqrt(a), std::sqrt(b))))) tape_.clear();
     }
-    d_qtilde_.upload(qt);
+    set_inertial_target(q, v, f_ext, dt);
     d_q_.upload(q, n3);
     const int K = K_;
     const DevBoundary bnd = dev_boundary();

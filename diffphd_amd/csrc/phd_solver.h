@@ -179,8 +179,11 @@ public:
 
     // ---- learned warm start (option phd_warm_start = 1) ----
     WarmStartNet& net();
-    // raw (standardize = false) or standardized features of state (q, v) into d_feat_
+    // raw (standardize = false) or standardized features of state (q, v) into
+    // d_feat_; needs the solver set up for dt and this frame's inertial target
+    // in d_qtilde_ (set_inertial_target)
     void mlp_features(const double* q_dev, const double* v_dev, double dt, bool standardize);
+    void set_inertial_target(const double* q, const double* v, const double* f_ext, double dt);
     const float* feat_dev() const { return d_feat_.p; }
     int warm_used = 0, warm_fallback = 0;
 
@@ -235,7 +238,7 @@ private:
     DevBuf<double> d_q_, d_qprev_, d_qtilde_, d_b_, d_r_, d_pdrhs_, d_pdrhs_prev_, d_elem_, d_g_, d_tmp_, d_kz_;
     // warm start: network, per-vertex feature inputs, guesses
     std::unique_ptr<WarmStartNet> net_;
-    DevBuf<float> d_logk_, d_cflag_, d_cdist_, d_feat_, d_mlpout_, d_fmean_, d_fstd_;
+    DevBuf<float> d_logk_, d_cflag_, d_cdist_, d_feat_, d_mlpout_, d_fmean_, d_fstd_, d_adiag_;
     DevBuf<double> d_Xrest_, d_v_, d_guess_;
     void warm_start(const double* v, double dt, double inv_h2m, bool learned);
     DevBuf<double> d_hist_q_[8], d_hist_g_[8];

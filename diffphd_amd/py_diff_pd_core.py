@@ -285,12 +285,13 @@ class _Deformable:
         a, ap = _lib.fptr(p)
         self._ok(_lib.lib().phd_mlp_set_params(self._ctx, ap))
 
-    def mlp_features(self, q, v, dt):
+    def mlp_features(self, q, v, dt, f_ext=None):
         """Raw per-vertex features (n x 16) of state (q, v) for this body."""
         qa, qp = _lib.dptr(q)
         va, vp = _lib.dptr(v)
+        fa, fp = _lib.dptr(np.zeros(self._dofs) if f_ext is None else f_ext)
         out, op = _lib.fptr(np.zeros((self._dofs // 3, 16)))
-        self._ok(_lib.lib().phd_mlp_features(self._ctx, qp, vp, float(dt), op))
+        self._ok(_lib.lib().phd_mlp_features(self._ctx, qp, vp, fp, float(dt), op))
         return out.copy()
 
     def mlp_forward(self, feat):

@@ -21,7 +21,7 @@ def collect(body, spec, frames, opt, initial_velocities, plane_schedule=None):
         for i in range(frames):
             if plane_schedule is not None:
                 body.SetPressPlaneHeight(plane_schedule[i])
-            feats.append(body.mlp_features(q, v, h))
+            feats.append(body.mlp_features(q, v, h, f_ext))
             qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
             body.PyForward("pd_eigen_alg_phd", q, v, [], f_ext, h, opt, qn, vn, act)
             qn, vn = np.asarray(qn), np.asarray(vn)

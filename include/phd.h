@@ -141,8 +141,8 @@ int phd_mlp_loss_grad(phd_ctx ctx, int n, const float* feat, const float* target
 /* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of a seeded permutation; losses: steps floats or NULL */
 int phd_mlp_train(phd_ctx ctx, int n, const float* feat, const float* target, int steps, int batch, float lr, int seed,
                   float* losses);
-/* raw (unstandardized) features of state (q, v) for this body: n x 16 */
-int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, double dt, float* feat);
+/* raw (unstandardized) features of state (q, v, f_ext may be NULL) for this body: n x 16 */
+int phd_mlp_features(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt, float* feat);
 /* forward frames that started from the network / fell back to q + h v */
 int phd_warm_start_stats(phd_ctx ctx, int* used, int* fallback, int reset);
 /* D = A B for 16 x 16 row-major inputs rounded to BF16, through one MFMA (layout probe) */
