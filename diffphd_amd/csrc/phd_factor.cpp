@@ -293,7 +293,8 @@ void Factor::build(int n_, const std::vector<int>& Ap, const std::vector<int>& A
     factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 }
 
-void SpmvPlan::build(const Factor& f, int chunk, int unit_items) {
+void SpmvPlan::build(const Factor& f, int chunk, int unit_items, int group_chunk_w) {
+    chunk_w = group_chunk_w;
     const int n = f.n;
     ru_row.clear(); ru_c0.clear(); ru_c1.clear();
     row_unit_ptr.assign(n + 1, 0);
@@ -325,6 +326,16 @@ void SpmvPlan::build(const Factor& f, int chunk, int unit_items) {
         }
         tile_unit_ptr[t + 1] = (int)cu_tile.size();
     }
+    // S w by row groups x column chunks; per-(row, chunk) partial slots, contiguous per row
+    gu_group.clear(); gu_chunk.clear();
+    const int ngroups = (n + kRowGroup - 1) / kRowGroup;
+    for (int g = 0; g < ngroups; ++g) {
+        int lo = n, hi = -1;
+        for (int k = g * kRowGroup; k < std::min(n, (g + 1) * kRowGroup); ++k) { lo = std::min(lo, f.fd[k]); hi = std::max(hi, k); }
+        for (int c = lo / chunk_w; c <= hi / chunk_w; ++c) { gu_group.push_back(g); gu_chunk.push_back(c); }
+    }
+    rbase.assign(n + 1, 0);
+    for (int k = 0; k < n; ++k) rbase[k + 1] = rbase[k] + (k / chunk_w - f.fd[k] / chunk_w + 1);
 }
 
 
@@ -338,7 +349,7 @@ void host_factor_check(int n, const std::vector<int>& Ap, const std::vector<int>
     Factor f;
     f.build(n, Ap, Ai, Ax, X, 2);
     SpmvPlan plan;
-    plan.build(f, 8, 5);  // small chunks so that rows split and chains split
+    plan.build(f, 8, 5, 16);  // small chunks so that rows split and chains split
     std::vector<double> V(f.v_len, 0.0);
     std::vector<double> x(n);
     double viol = 0;
@@ -372,10 +383,26 @@ void host_factor_check(int n, const std::vector<int>& Ap, const std::vector<int>
     std::vector<double> w(n), yref(n), ypl(n, 0.0), xref(n), xpl(n, 0.0);
     for (int j = 0; j < n; ++j) w[j] = std::sin(0.37 * j + 1.0);
     for (int k = 0; k < n; ++k) { double s = 0; for (int j = f.fd[k]; j <= k; ++j) s += S(k, j) * w[j]; yref[k] = s; }
-    for (size_t u = 0; u < plan.ru_row.size(); ++u) {
-        const int k = plan.ru_row[u];
-        for (int c = plan.ru_c0[u]; c <= plan.ru_c1[u]; ++c) ypl[k] += V[f.base[k] + c] * w[c];
+    // S w through the row-group units: per-(row, chunk) slots, then per-row sums
+    std::vector<double> pslot(plan.rbase[n], 0.0);
+    std::vector<char> seen(plan.rbase[n], 0);
+    for (size_t u = 0; u < plan.gu_group.size(); ++u) {
+        const int g = plan.gu_group[u], ch = plan.gu_chunk[u];
+        const int c0 = ch * plan.chunk_w, c1 = c0 + plan.chunk_w - 1;
+        for (int k = g * SpmvPlan::kRowGroup; k < std::min(n, (g + 1) * SpmvPlan::kRowGroup); ++k) {
+            const int lo = std::max(f.fd[k], c0), hi = std::min(k, c1);
+            if (lo > hi) continue;
+            const int slot = plan.rbase[k] + ch - f.fd[k] / plan.chunk_w;
+            if (slot < plan.rbase[k] || slot >= plan.rbase[k + 1] || seen[slot]) viol += 1;
+            seen[slot] = 1;
+            for (int c = lo; c <= hi; ++c) pslot[slot] += V[f.base[k] + c] * w[c];
+        }
     }
+    for (int k = 0; k < n; ++k)
+        for (int s = plan.rbase[k]; s < plan.rbase[k + 1]; ++s) {
+            if (!seen[s]) viol += 1;
+            ypl[k] += pslot[s];
+        }
     for (int j = 0; j < n; ++j) { double s = 0; for (int k = j; k < n; ++k) s += S(k, j) * yref[k]; xref[j] = s; }
     for (size_t u = 0; u < plan.cu_tile.size(); ++u) {
         const int t = plan.cu_tile[u];

@@ -53,7 +53,13 @@ struct SpmvPlan {
     std::vector<int> items;         // concatenated per tile
     std::vector<int> cu_tile, cu_i0, cu_i1;  // unit -> tile, item range
     std::vector<int> tile_unit_ptr;          // units of tile t
-    void build(const Factor& f, int chunk, int unit_items);
+    // S w by units of (group of kRowGroup consecutive rows, chunk of kChunkW
+    // columns): each unit stages w[chunk] once for all rows of the group that
+    // cross it. Row k's partial for chunk c sits in slot rbase[k] + c - fd(k) / chunk_w.
+    static constexpr int kRowGroup = 64;
+    int chunk_w = 1024;
+    std::vector<int> gu_group, gu_chunk, rbase;
+    void build(const Factor& f, int chunk, int unit_items, int group_chunk_w = 1024);
 };
 
 void host_factor_check(int n, const std::vector<int>& Ap, const std::vector<int>& Ai, const std::vector<double>& Ax,

@@ -52,6 +52,13 @@ void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* r
                       const TV* V, const TV* w, int n, TV* part);
 template <class TV>
 void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const TV* part, TV* y);
+// S w by (64-row group, 1024-column chunk) units with w staged in LDS; the
+// per-(row, chunk) partials are then summed per row (plan: SpmvPlan::gu_*, rbase).
+template <class TV>
+void launch_spmv_rows_grp(hipStream_t s, int nunits, const int* gu_group, const int* gu_chunk, const int* fd,
+                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot);
+template <class TV>
+void launch_spmv_rows_slots(hipStream_t s, int n, const int* rbase, const TV* pslot, TV* y);
 template <class TV>
 void launch_spmv_cols(hipStream_t s, int nunits, const int* cu_tile, const int* cu_i0, const int* cu_i1, const int* items,
                       const int* fd, const int64_t* base, const TV* V, const TV* y, int n, TV* xpart);

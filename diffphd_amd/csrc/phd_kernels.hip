@@ -207,6 +207,102 @@ __global__ void k_spmv_rows_fin(int n, const int* __restrict__ row_unit_ptr, con
 }
 
 // -----------------------------------------------------------------------------
+// y = S w by (group of 64 consecutive rows, chunk of 1024 columns) units: the
+// workgroup stages w on the chunk in LDS once, then its four waves stream the
+// rows of the group that cross the chunk (16-byte vectors, the whole chunk in
+// flight per row) against it. Row k's dot product on the chunk goes to its
+// slot rbase[k] + chunk - fd(k) / 1024; k_spmv_rows_slots sums the slots.
+// -----------------------------------------------------------------------------
+constexpr int GRP_CW = 1024;
+
+template <class TV>
+__global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __restrict__ gu_group,
+                                                       const int* __restrict__ gu_chunk, const int* __restrict__ fd,
+                                                       const int64_t* __restrict__ base, const int* __restrict__ rbase,
+                                                       const TV* __restrict__ V, const TV* __restrict__ w, int n,
+                                                       TV* __restrict__ pslot) {
+    using V16 = typename Vec16<TV>::type;
+    constexpr int VN = Vec16<TV>::N;
+    constexpr int U = GRP_CW / (VN * WAVE);
+    __shared__ __align__(16) TV ws[3][GRP_CW];
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int g = gu_group[u], ch = gu_chunk[u];
+    const int c0 = ch * GRP_CW, c1 = c0 + GRP_CW - 1;
+    const int ldw = spmv_ldw(n);
+    for (int e = (int)threadIdx.x * VN; e < GRP_CW; e += 256 * VN)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            V16 t;
+            if (c0 + e + VN <= ldw) t = *reinterpret_cast<const V16*>(w + (int64_t)d * ldw + c0 + e);
+            else memset(&t, 0, sizeof(t));
+            *reinterpret_cast<V16*>(&ws[d][e]) = t;
+        }
+    __syncthreads();
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    for (int rr = wv; rr < 64; rr += 4) {
+        const int k = 64 * g + rr;
+        if (k >= n) break;
+        const int f = fd[k];
+        const int lo = max(f, c0), hi = min(k, c1);
+        if (lo > hi) continue;
+        const TV* vr = V + base[k];
+        const int cs = (lo & ~(VN - 1)) + VN * lane;
+        TV v[U][VN];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int c = cs + q * VN * WAVE;
+            V16 t;
+            if (c <= hi) t = *reinterpret_cast<const V16*>(vr + c);
+            else memset(&t, 0, sizeof(t));
+            memcpy(v[q], &t, sizeof(t));
+        }
+        TV a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int c = cs + q * VN * WAVE;
+            if (c <= hi) {
+                TV x0[VN], x1[VN], x2[VN];
+                memcpy(x0, &ws[0][c - c0], sizeof(V16));
+                memcpy(x1, &ws[1][c - c0], sizeof(V16));
+                memcpy(x2, &ws[2][c - c0], sizeof(V16));
+#pragma unroll
+                for (int e = 0; e < VN; ++e) {
+                    const TV vv = (c + e >= lo && c + e <= hi) ? v[q][e] : TV(0);
+                    a0 += vv * x0[e];
+                    a1 += vv * x1[e];
+                    a2 += vv * x2[e];
+                }
+            }
+        }
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        a2 = wave_sum(a2);
+        if (lane == 0) {
+            TV* p = pslot + 3 * (int64_t)(rbase[k] + ch - f / GRP_CW);
+            p[0] = a0;
+            p[1] = a1;
+            p[2] = a2;
+        }
+    }
+}
+
+template <class TV>
+__global__ void k_spmv_rows_slots(int n, const int* __restrict__ rbase, const TV* __restrict__ pslot, TV* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    TV a0 = 0, a1 = 0, a2 = 0;
+    for (int s = rbase[k]; s < rbase[k + 1]; ++s) {
+        a0 += pslot[3 * (int64_t)s];
+        a1 += pslot[3 * (int64_t)s + 1];
+        a2 += pslot[3 * (int64_t)s + 2];
+    }
+    y[k] = a0;
+    y[n + k] = a1;
+    y[2 * n + k] = a2;
+}
+
+// -----------------------------------------------------------------------------
 // x = S^T y: one wavefront per (tile, item range) unit. A row of the tile is
 // 64 columns = 64 / VN lanes of 16-byte vectors, so one load instruction
 // covers VN rows (lane sub-group `sub`); four such loads are in flight. The
@@ -1021,6 +1117,17 @@ void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const T
     hipLaunchKernelGGL(k_spmv_rows_fin<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, row_unit_ptr, part, y);
 }
 template <class TV>
+void launch_spmv_rows_grp(hipStream_t s, int nunits, const int* gu_group, const int* gu_chunk, const int* fd,
+                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot) {
+    if (nunits == 0) return;
+    hipLaunchKernelGGL(k_spmv_rows_grp<TV>, dim3(nunits), dim3(256), 0, s, nunits, gu_group, gu_chunk, fd, base, rbase, V, w,
+                       n, pslot);
+}
+template <class TV>
+void launch_spmv_rows_slots(hipStream_t s, int n, const int* rbase, const TV* pslot, TV* y) {
+    hipLaunchKernelGGL(k_spmv_rows_slots<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rbase, pslot, y);
+}
+template <class TV>
 void launch_spmv_cols(hipStream_t s, int nunits, const int* cu_tile, const int* cu_i0, const int* cu_i1, const int* items,
                       const int* fd, const int64_t* base, const TV* V, const TV* y, int n, TV* xpart) {
     if (nunits == 0) return;
@@ -1039,6 +1146,9 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
 }
 
 #define PHD_INST_TV(TV)                                                                                                  \
+    template void launch_spmv_rows_grp<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*,       \
+                                           const int*, const TV*, const TV*, int, TV*);                                \
+    template void launch_spmv_rows_slots<TV>(hipStream_t, int, const int*, const TV*, TV*);                          \
     template void launch_spmv_rows<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*, const TV*, \
                                        const TV*, int, TV*);                                                             \
     template void launch_spmv_rows_fin<TV>(hipStream_t, int, const int*, const TV*, TV*);                               \

@@ -426,6 +426,9 @@ void Body::setup(double dt, int threads) {
         hip_check(hipGetLastError(), "expand S launch");
     }
     d_perm_.upload(factor_.perm);
+    d_gu_group_.upload(plan_.gu_group);
+    d_gu_chunk_.upload(plan_.gu_chunk);
+    d_rbase_.upload(plan_.rbase);
     d_ru_row_.upload(plan_.ru_row);
     d_ru_c0_.upload(plan_.ru_c0);
     d_ru_c1_.upload(plan_.ru_c1);
@@ -489,7 +492,8 @@ void Body::setup(double dt, int threads) {
     d_dot_out_.alloc(32);
     w32_.alloc((size_t)3 * spmv_ldw(nv_)); y32_.alloc((size_t)3 * nv_);
     w64_.alloc((size_t)3 * spmv_ldw(nv_)); y64_.alloc((size_t)3 * nv_);
-    part32_.alloc(3 * plan_.ru_row.size() + 3); part64_.alloc(3 * plan_.ru_row.size() + 3);
+    check(plan_.chunk_w == 1024, "row-group chunk width must match the kernel");
+    part32_.alloc(3 * (size_t)plan_.rbase[nv_] + 3); part64_.alloc(3 * (size_t)plan_.rbase[nv_] + 3);
     xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
     setup_contact(threads);
     ready_ = true;
@@ -610,10 +614,10 @@ void Body::spmv_S(const double* r, DevBuf<TV>& w, DevBuf<TV>& y, DevBuf<TV>& par
     launch_perm_in<TV>(stream_, nv_, d_perm_.p, r, nullptr, w.p);
     hipEvent_t a = nullptr, b = nullptr;
     if (timing) { a = next_event(); b = next_event(); (void)hipEventRecord(a, stream_); }
-    launch_spmv_rows<TV>(stream_, (int)plan_.ru_row.size(), d_ru_row_.p, d_ru_c0_.p, d_ru_c1_.p, d_base_.p, V, w.p, nv_,
-                         part.p);
+    launch_spmv_rows_grp<TV>(stream_, (int)plan_.gu_group.size(), d_gu_group_.p, d_gu_chunk_.p, d_fd_.p, d_base_.p,
+                             d_rbase_.p, V, w.p, nv_, part.p);
     if (timing) { (void)hipEventRecord(b, stream_); ev_rows_.push_back({a, b}); }
-    launch_spmv_rows_fin<TV>(stream_, nv_, d_row_unit_ptr_.p, part.p, y.p);
+    launch_spmv_rows_slots<TV>(stream_, nv_, d_rbase_.p, part.p, y.p);
 }
 
 template <class TV>
