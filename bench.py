@@ -221,7 +221,7 @@ def main():
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"spmv_press_{cells[0]}x{cells[1]}x{cells[2]}_pmc_traffic.json")
     if os.path.exists(pmc):
-        kern = ("k_spmv_rows" if dom == "spmv_rows" else "k_spmv_cols") + ("<float>" if bv == 4 else "<double>")
+        kern = ("k_spmv_rows_grp" if dom == "spmv_rows" else "k_spmv_cols") + ("<float>" if bv == 4 else "<double>")
         traffic = json.load(open(pmc))["kernels"].get(kern, {}).get("traffic_bytes")
     copy_ms = res["body"].copy_probe(1 << 30, 10)
     out = {

@@ -293,22 +293,9 @@ void Factor::build(int n_, const std::vector<int>& Ap, const std::vector<int>& A
     factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 }
 
-void SpmvPlan::build(const Factor& f, int chunk, int unit_items, int group_chunk_w) {
+void SpmvPlan::build(const Factor& f, int unit_items, int group_chunk_w) {
     chunk_w = group_chunk_w;
     const int n = f.n;
-    ru_row.clear(); ru_c0.clear(); ru_c1.clear();
-    row_unit_ptr.assign(n + 1, 0);
-    for (int k = 0; k < n; ++k) {
-        int c = f.fd[k];
-        while (c <= k) {
-            const int stop = std::min(k, (c / chunk + 1) * chunk - 1);
-            ru_row.push_back(k);
-            ru_c0.push_back(c);
-            ru_c1.push_back(stop);
-            c = stop + 1;
-        }
-        row_unit_ptr[k + 1] = (int)ru_row.size();
-    }
     const int ntiles = (n + 63) / 64;
     items.clear(); cu_tile.clear(); cu_i0.clear(); cu_i1.clear();
     tile_unit_ptr.assign(ntiles + 1, 0);
@@ -349,7 +336,7 @@ void host_factor_check(int n, const std::vector<int>& Ap, const std::vector<int>
     Factor f;
     f.build(n, Ap, Ai, Ax, X, 2);
     SpmvPlan plan;
-    plan.build(f, 8, 5, 16);  // small chunks so that rows split and chains split
+    plan.build(f, 5, 16);  // small chunks so that rows split and chains split
     std::vector<double> V(f.v_len, 0.0);
     std::vector<double> x(n);
     double viol = 0;

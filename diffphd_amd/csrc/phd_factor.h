@@ -39,16 +39,12 @@ struct Factor {
 };
 
 // Work units of the two SpMVs over the S layout.
-//  * rows: y = S w, unit u covers row k, columns [c0, c1] (c0..c1 within one
-//    64-aligned chunk run); partial sums are reduced per row.
-//  * cols: x = S^T y, tile t = columns [64t, 64t+63]; its items are the rows
+//  * rows (y = S w): units of (group of kRowGroup consecutive rows, chunk of
+//    chunk_w columns), see below.
+//  * cols (x = S^T y): tile t = columns [64t, 64t+63]; its items are the rows
 //    intersecting the tile (the tile's own rows then the ancestors of its last
 //    column), split into units of at most `unit_items` rows.
 struct SpmvPlan {
-    // rows
-    std::vector<int> ru_row;        // row of unit
-    std::vector<int> ru_c0, ru_c1;  // inclusive column range
-    std::vector<int> row_unit_ptr;  // units of row k: [row_unit_ptr[k], row_unit_ptr[k+1])
     // cols
     std::vector<int> items;         // concatenated per tile
     std::vector<int> cu_tile, cu_i0, cu_i1;  // unit -> tile, item range
@@ -59,7 +55,7 @@ struct SpmvPlan {
     static constexpr int kRowGroup = 64;
     int chunk_w = 1024;
     std::vector<int> gu_group, gu_chunk, rbase;
-    void build(const Factor& f, int chunk, int unit_items, int group_chunk_w = 1024);
+    void build(const Factor& f, int unit_items, int group_chunk_w = 1024);
 };
 
 void host_factor_check(int n, const std::vector<int>& Ap, const std::vector<int>& Ai, const std::vector<double>& Ax,

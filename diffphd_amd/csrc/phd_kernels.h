@@ -47,11 +47,6 @@ void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t
                      const int* parent, const int* depth, const int* fd, const int64_t* base, const unsigned char* chain,
                      double* ws, int64_t ws_stride, double* V64, float* V32);
 
-template <class TV>
-void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* ru_c0, const int* ru_c1, const int64_t* base,
-                      const TV* V, const TV* w, int n, TV* part);
-template <class TV>
-void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const TV* part, TV* y);
 // S w by (64-row group, 1024-column chunk) units with w staged in LDS; the
 // per-(row, chunk) partials are then summed per row (plan: SpmvPlan::gu_*, rbase).
 template <class TV>

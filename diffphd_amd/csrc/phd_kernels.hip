@@ -120,93 +120,6 @@ template <> struct Vec16<float> { using type = float4; static constexpr int N = 
 template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };
 
 // -----------------------------------------------------------------------------
-// y = S w: one wavefront per (row, column chunk) unit, 3 right-hand sides.
-// Each lane streams 16-byte vectors of the row (aligned: base[k] is a multiple
-// of 64), four in flight; w has axis stride spmv_ldw(n) so its vector loads are
-// aligned too. Elements outside [c0, c1] are masked.
-// -----------------------------------------------------------------------------
-template <class TV>
-__global__ void __launch_bounds__(256) k_spmv_rows(int nunits, const int* __restrict__ ru_row,
-                                                   const int* __restrict__ ru_c0, const int* __restrict__ ru_c1,
-                                                   const int64_t* __restrict__ base, const TV* __restrict__ V,
-                                                   const TV* __restrict__ w, int n, TV* __restrict__ part) {
-    using V16 = typename Vec16<TV>::type;
-    constexpr int VN = Vec16<TV>::N;
-    constexpr int U = 4;
-    const int u = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    if (u >= nunits) return;
-    const int ldw = spmv_ldw(n);
-    const int k = ru_row[u];
-    const int c0 = ru_c0[u], c1 = ru_c1[u];
-    const TV* vr = V + base[k];
-    TV a0 = 0, a1 = 0, a2 = 0;
-    auto accumulate = [&](const TV (&v)[VN], int c) {
-        const V16 t0 = *reinterpret_cast<const V16*>(w + c);
-        const V16 t1 = *reinterpret_cast<const V16*>(w + ldw + c);
-        const V16 t2 = *reinterpret_cast<const V16*>(w + 2 * ldw + c);
-        TV x0[VN], x1[VN], x2[VN];
-        memcpy(x0, &t0, sizeof(t0));
-        memcpy(x1, &t1, sizeof(t1));
-        memcpy(x2, &t2, sizeof(t2));
-#pragma unroll
-        for (int e = 0; e < VN; ++e) {
-            const TV vv = (c + e >= c0 && c + e <= c1) ? v[e] : TV(0);
-            a0 += vv * x0[e];
-            a1 += vv * x1[e];
-            a2 += vv * x2[e];
-        }
-    };
-    constexpr int SPAN = VN * WAVE * U;
-    int cb = c0 & ~(VN - 1);  // uniform block start
-    // full blocks: every lane's vectors lie inside the row's column range
-    for (; cb + SPAN - 1 <= c1; cb += SPAN) {
-        TV v[U][VN];
-#pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const V16 t = *reinterpret_cast<const V16*>(vr + cb + q * VN * WAVE + VN * lane);
-            memcpy(v[q], &t, sizeof(t));
-        }
-#pragma unroll
-        for (int q = 0; q < U; ++q) accumulate(v[q], cb + q * VN * WAVE + VN * lane);
-    }
-    // last partial block, masked per vector
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-        const int c = cb + q * VN * WAVE + VN * lane;
-        if (c <= c1) {
-            TV v[VN];
-            const V16 t = *reinterpret_cast<const V16*>(vr + c);
-            memcpy(v, &t, sizeof(t));
-            accumulate(v, c);
-        }
-    }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    a2 = wave_sum(a2);
-    if (lane == 0) {
-        part[3 * (int64_t)u] = a0;
-        part[3 * (int64_t)u + 1] = a1;
-        part[3 * (int64_t)u + 2] = a2;
-    }
-}
-
-template <class TV>
-__global__ void k_spmv_rows_fin(int n, const int* __restrict__ row_unit_ptr, const TV* __restrict__ part, TV* __restrict__ y) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    TV a0 = 0, a1 = 0, a2 = 0;
-    for (int u = row_unit_ptr[k]; u < row_unit_ptr[k + 1]; ++u) {
-        a0 += part[3 * (int64_t)u];
-        a1 += part[3 * (int64_t)u + 1];
-        a2 += part[3 * (int64_t)u + 2];
-    }
-    y[k] = a0;
-    y[n + k] = a1;
-    y[2 * n + k] = a2;
-}
-
-// -----------------------------------------------------------------------------
 // y = S w by (group of 64 consecutive rows, chunk of 1024 columns) units: the
 // workgroup stages w on the chunk in LDS once, then its four waves stream the
 // rows of the group that cross the chunk (16-byte vectors, the whole chunk in
@@ -1106,17 +1019,6 @@ void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t
 }
 
 template <class TV>
-void launch_spmv_rows(hipStream_t s, int nunits, const int* ru_row, const int* ru_c0, const int* ru_c1, const int64_t* base,
-                      const TV* V, const TV* w, int n, TV* part) {
-    if (nunits == 0) return;
-    hipLaunchKernelGGL(k_spmv_rows<TV>, dim3(blocks_for(nunits, 4)), dim3(256), 0, s, nunits, ru_row, ru_c0, ru_c1, base,
-                       V, w, n, part);
-}
-template <class TV>
-void launch_spmv_rows_fin(hipStream_t s, int n, const int* row_unit_ptr, const TV* part, TV* y) {
-    hipLaunchKernelGGL(k_spmv_rows_fin<TV>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, row_unit_ptr, part, y);
-}
-template <class TV>
 void launch_spmv_rows_grp(hipStream_t s, int nunits, const int* gu_group, const int* gu_chunk, const int* fd,
                           const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot) {
     if (nunits == 0) return;
@@ -1149,9 +1051,6 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
     template void launch_spmv_rows_grp<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*,       \
                                            const int*, const TV*, const TV*, int, TV*);                                \
     template void launch_spmv_rows_slots<TV>(hipStream_t, int, const int*, const TV*, TV*);                          \
-    template void launch_spmv_rows<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*, const TV*, \
-                                       const TV*, int, TV*);                                                             \
-    template void launch_spmv_rows_fin<TV>(hipStream_t, int, const int*, const TV*, TV*);                               \
     template void launch_spmv_cols<TV>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,    \
                                        const int64_t*, const TV*, const TV*, int, TV*);                                  \
     template void launch_spmv_cols_fin<TV>(hipStream_t, int, const int*, const TV*, const int*, const double*,          \

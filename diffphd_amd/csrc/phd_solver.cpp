@@ -388,7 +388,7 @@ void Body::setup(double dt, int threads) {
     }
     // factor (host FP64) and expansion of S = L^-1 (device FP64, FP32 copy)
     factor_.build(nv_, Ap_, Ai_, Ax_, X_, threads);
-    plan_.build(factor_, 4096, 512);
+    plan_.build(factor_, 512);
     {
         DevBuf<int64_t> dLp;
         DevBuf<int> dLi, dpar, ddep;
@@ -429,10 +429,6 @@ void Body::setup(double dt, int threads) {
     d_gu_group_.upload(plan_.gu_group);
     d_gu_chunk_.upload(plan_.gu_chunk);
     d_rbase_.upload(plan_.rbase);
-    d_ru_row_.upload(plan_.ru_row);
-    d_ru_c0_.upload(plan_.ru_c0);
-    d_ru_c1_.upload(plan_.ru_c1);
-    d_row_unit_ptr_.upload(plan_.row_unit_ptr);
     d_items_.upload(plan_.items);
     d_cu_tile_.upload(plan_.cu_tile);
     d_cu_i0_.upload(plan_.cu_i0);

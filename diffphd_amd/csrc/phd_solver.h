@@ -218,8 +218,8 @@ private:
     DevBuf<double> V64_;
     DevBuf<float> V32_;
     DevBuf<int64_t> d_base_;
-    DevBuf<int> d_fd_, d_perm_, d_ru_row_, d_ru_c0_, d_ru_c1_, d_row_unit_ptr_, d_items_, d_cu_tile_, d_cu_i0_, d_cu_i1_,
-        d_tile_unit_ptr_, d_gu_group_, d_gu_chunk_, d_rbase_;
+    DevBuf<int> d_fd_, d_perm_, d_items_, d_cu_tile_, d_cu_i0_, d_cu_i1_, d_tile_unit_ptr_, d_gu_group_, d_gu_chunk_,
+        d_rbase_;
     DevBuf<int> d_Ap_, d_Ai_;
     DevBuf<double> d_Ax_;
     DevBuf<unsigned char> d_dmask_;
