@@ -4,7 +4,8 @@
 //
 // Blocked right-looking Cholesky in 64 x 64 blocks. The right-hand side rides
 // along as an extra block row, so L^-1 rhs comes out of the factorization; a
-// blocked backward substitution then gives x = L^-T L^-1 rhs.
+// blocked backward substitution with the inverted diagonal blocks (formed by
+// the panel kernel next to the block rows) then gives x = L^-T L^-1 rhs.
 //
 // Layout (column-major, lda = np + 64, np = 64 * ceil(n / 64)):
 //   rows [0, np)        lower triangle of M, identity on the padding;
@@ -21,7 +22,8 @@ constexpr int kCholBlock = 64;
 inline int chol_padded(int n) { return kCholBlock * ((n + kCholBlock - 1) / kCholBlock); }
 inline int64_t chol_lda(int n) { return (int64_t)chol_padded(n) + kCholBlock; }
 inline int64_t chol_elems(int n) { return chol_lda(n) * chol_padded(n); }
-inline int64_t chol_work(int n) { return (int64_t)chol_padded(n) + kCholBlock * (kCholBlock + 1); }
+// y (np) + the inverted diagonal blocks L_kk^-T (np / 64 blocks of 64 x 64)
+inline int64_t chol_work(int n) { return (int64_t)chol_padded(n) * (1 + kCholBlock); }
 
 // Writes the contact system of the index set J (nJ rows) into the layout above:
 // M_ab = om_i (n_i . n_j) H_ij om_j + [a == b] E_i with i = J[a], j = J[b];

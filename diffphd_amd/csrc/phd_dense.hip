@@ -2,6 +2,8 @@
 // the right-hand side as an extra block row, then blocked backward substitution.
 #include "phd_dense.h"
 
+#include <cstring>
+
 namespace phd {
 namespace {
 
@@ -32,83 +34,113 @@ __global__ void __launch_bounds__(256) k_chol_assemble(int nJ, int np, int64_t l
     M[r + (int64_t)c * lda] = (TV)v;
 }
 
-// Panel step k, part 1: one wave factors the diagonal block, lane r holding
-// row r in registers as a window that shifts by one column per step (x[m] =
-// A(r, j + m) at step j), so the current column is always x[0] and the
-// update reads the published pivot column at static offsets from a moving
-// base. Results are collected in LDS and written once at the end, so the
-// per-step barriers wait on LDS only. Writes L_kk into M and a column-major
-// copy plus the reciprocal pivots into Ld.
-template <class TV>
-__global__ void __launch_bounds__(64) k_chol_diag(int k, int64_t lda, TV* __restrict__ M, TV* __restrict__ Ld,
-                                                  int* __restrict__ fail) {
-    __shared__ TV col[2 * CB];          // L(l, j) of the current step, zero-padded
-    __shared__ TV Lo[CB * CB + CB];     // Lo[j * CB + l] = L(l, j); reciprocal pivots at the end
-    const int r = threadIdx.x;
-    const int64_t c0 = (int64_t)k * CB;
-    TV* Mk = M + c0 * lda;
-    TV x[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) x[c] = Mk[c0 + r + (int64_t)c * lda];
-    col[CB + r] = TV(0);
-    bool bad_any = false;
-    for (int j = 0; j < CB; ++j) {
-        const TV p = __shfl(x[0], j);
-        const bool bad = !(p > TV(0));
-        bad_any |= bad;
-        const TV d = bad ? TV(1) : sqrt(p);
-        const TV inv = TV(1) / d;
-        const TV lrj = (r > j) ? x[0] * inv : (r == j ? d : TV(0));
-        col[r] = lrj;
-        Lo[j * CB + r] = lrj;
-        if (r == j) Lo[CB * CB + j] = inv;
-        // one wave: its LDS operations complete in program order, so a
-        // wavefront-scope fence (compiler ordering only) replaces the barrier
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const TV* cj = col + j;
-#pragma unroll
-        for (int m = 1; m < CB; ++m) x[m - 1] = x[m] - lrj * cj[m];
-        x[CB - 1] = TV(0);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-    if (bad_any && r == 0) fail[0] = 1;
-#pragma unroll
-    for (int e = 0; e < CB + 1; ++e) Ld[e * CB + r] = Lo[e * CB + r];
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-        if (c <= r) Mk[c0 + r + (int64_t)c * lda] = Lo[c * CB + r];
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Panel step k, part 2: workgroup w solves X L_kk^T = A_ik for row block
-// i = k + 1 + w (i == T is the right-hand-side block), lane r = row r held in
-// a shifting register window as above, L_kk staged in LDS (read-only here).
-template <class TV>
-__global__ void __launch_bounds__(64) k_chol_trsm(int k, int64_t lda, TV* __restrict__ M, const TV* __restrict__ Ld) {
-    __shared__ TV Lc[CB * CB + 2 * CB];  // Lc[j * CB + l] = L(l, j); reciprocal pivots at CB*CB; zero tail
-    const int r = threadIdx.x;
-    const int i = k + 1 + blockIdx.x;
-    const int64_t c0 = (int64_t)k * CB;
-    TV* Mk = M + c0 * lda;
-    // all loads in flight at once (a rolled loop pays the memory latency per load)
+template <class TV> struct V16;
+template <> struct V16<float> { using type = float4; static constexpr int N = 4; };
+template <> struct V16<double> { using type = double2; static constexpr int N = 2; };
+
+constexpr int kColStride = 2 * CB + 4;  // one published pivot column, see k_chol_panel
+
+// Steps [j, j + 16) of the panel chain. Window z[m] = column j + m of the
+// lane's row; columns past 63 never matter, and from step j on only
+// m <= 63 - j can reach one, so this quarter of the steps stops at m = MB.
+template <class TV, int MB>
+__device__ __forceinline__ void panel_quarter(int& j, TV (&z)[CB], TV* colS, bool diag_wave, int r, bool& bad_any,
+                                              TV* out, int64_t out_ld) {
+    using VT = typename V16<TV>::type;
+    constexpr int VN = V16<TV>::N;
+    for (int jj = 0; jj < 16; ++jj, ++j) {
+        TV* cb = colS + (j & 1) * kColStride;
+        TV coef = TV(0);
+        if (diag_wave) {
+            const TV p = lane_bcast(z[0], j);
+            const bool bad = !(p > TV(0));
+            bad_any |= bad;
+            const TV d = bad ? TV(1) : sqrt(p);
+            const TV inv = TV(1) / d;
+            coef = (r > j) ? z[0] * inv : (r == j ? d : TV(0));
+            // L(r, j) at CB + r - j: L(j + m, j) lands at CB + m for every step
+            cb[CB + r - j] = coef;
+            if (r == CB - 1) {  // the two slots past the column, stale from step j - 2
+                cb[2 * CB - j] = TV(0);
+                cb[2 * CB + 1 - j] = TV(0);
+            }
+        }
+        __syncthreads();
+        if (!diag_wave) {
+            coef = z[0] * (TV(1) / cb[CB]);
+            out[r + (int64_t)j * out_ld] = coef;
+        }
 #pragma unroll
-    for (int e = 0; e < CB + 1; ++e) Lc[e * CB + r] = Ld[e * CB + r];
-    TV y[CB];
+        for (int q = 0; q < (MB + 1) / 16; ++q) {
+            TV c[16];
 #pragma unroll
-    for (int c = 0; c < CB; ++c) y[c] = Mk[(int64_t)i * CB + r + (int64_t)c * lda];
-    __shared__ TV rp[CB];
-    __syncthreads();
-    rp[r] = Lc[CB * CB + r];
-    Lc[CB * CB + r] = TV(0);  // the window below reads past column 63: keep it zero
-    Lc[CB * CB + CB + r] = TV(0);
-    __syncthreads();
-    for (int j = 0; j < CB; ++j) {
-        const TV yj = y[0] * rp[j];
-        Mk[(int64_t)i * CB + r + (int64_t)j * lda] = yj;
-        const TV* lj = Lc + j * (CB + 1);  // lj[m] = L(j + m, j)
+            for (int e = 0; e < 16 / VN; ++e) {
+                const VT t = *reinterpret_cast<const VT*>(cb + CB + 16 * q + VN * e);
+                memcpy(c + VN * e, &t, sizeof(VT));
+            }
 #pragma unroll
-        for (int m = 1; m < CB; ++m) y[m - 1] = y[m] - yj * lj[m];
-        y[CB - 1] = TV(0);
+            for (int e = 0; e < 16; ++e) {
+                const int m = 16 * q + e;
+                if (m >= 1) z[m - 1] = z[m] - coef * c[e];
+            }
+        }
     }
+}
+
+// Panel step k, fused: workgroup w of T - k + 1, two waves. Wave 0 factors the
+// diagonal block (every workgroup repeats it): lane r holds row r as a window
+// that shifts one column per step, the pivot comes from lane j by readlane,
+// and the pivot column L(., j) is published through LDS (two alternating
+// buffers, so one barrier per step). Wave 1 solves X L_kk^T = B in lockstep,
+// B = block row k + 1 + w of M for w < T - k (w = T - k - 1 is the
+// right-hand-side block) or the identity for w = T - k, which gives
+// Xinv_k = L_kk^-T (column-major 64 x 64) for the backward substitution.
+template <class TV>
+__global__ void __launch_bounds__(128) k_chol_panel(int k, int T, int64_t lda, TV* __restrict__ M, TV* __restrict__ Xinv,
+                                                    int* __restrict__ fail) {
+    __shared__ __align__(16) TV colS[2 * kColStride];
+    const int wave = threadIdx.x / 64, r = threadIdx.x % 64;
+    const int w = blockIdx.x;
+    const bool diag_wave = wave == 0;
+    const int64_t c0 = (int64_t)k * CB;
+    TV z[CB];
+    TV* out;
+    int64_t out_ld;
+    if (diag_wave) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) z[c] = M[c0 + r + (c0 + c) * lda];
+        out = nullptr;
+        out_ld = 0;
+    } else if (w < T - k) {
+        const int64_t i0 = (int64_t)(k + 1 + w) * CB;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) z[c] = M[i0 + r + (c0 + c) * lda];
+        out = M + i0 + c0 * lda;
+        out_ld = lda;
+    } else {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) z[c] = (c == r) ? TV(1) : TV(0);
+        out = Xinv + (int64_t)k * CB * CB;
+        out_ld = CB;
+    }
+    for (int e = threadIdx.x; e < 2 * kColStride; e += 128) colS[e] = TV(0);
+    __syncthreads();
+    bool bad_any = false;
+    int j = 0;
+    panel_quarter<TV, 63>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
+    panel_quarter<TV, 47>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
+    panel_quarter<TV, 31>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
+    panel_quarter<TV, 15>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
+    if (diag_wave && bad_any && w == 0) fail[0] = 1;
 }
 
 // Trailing update of step k: A_ij -= L_ik L_jk^T for k < j <= i <= T. Thread
@@ -169,42 +201,43 @@ __global__ void k_chol_gety(int np, int64_t lda, const TV* __restrict__ M, TV* _
     if (c < np) y[c] = M[np + (int64_t)c * lda];
 }
 
-// Backward substitution step b: x_b = L_bb^-T y_b (every workgroup, lane r
-// holding y_r, shuffles instead of barriers), then workgroup c < b updates
-// y_c -= L_bc^T x_b; workgroup b stores x_b.
+// Backward substitution step b with Xinv_b = L_bb^-T (column-major): every
+// workgroup forms x_b = Xinv_b y_b; workgroup b stores it, workgroup a < b
+// updates y_a -= L_ba^T x_b.
 template <class TV>
-__global__ void __launch_bounds__(64) k_chol_bsolve(int b, int64_t lda, const TV* __restrict__ M, TV* __restrict__ y,
-                                                    TV* __restrict__ x) {
-    __shared__ TV Lr[CB][CB + 1];  // Lr[row][col] of the loaded block
+__global__ void __launch_bounds__(64) k_chol_bsolve(int b, int64_t lda, const TV* __restrict__ M,
+                                                    const TV* __restrict__ Xinv, TV* __restrict__ y, TV* __restrict__ x) {
+    __shared__ TV Ls[CB][CB + 1];  // Ls[l][cc] = L(64 b + l, 64 a + cc)
+    __shared__ TV ys[CB], xs[CB];
     const int r = threadIdx.x;
-    const int c = blockIdx.x;
-    const int64_t o = (int64_t)b * CB;
+    const int a = blockIdx.x;
+    const int64_t o = (int64_t)b * CB, oa = (int64_t)a * CB;
+    const TV* Xb = Xinv + (int64_t)b * CB * CB;
+    TV xcol[CB];
 #pragma unroll
-    for (int cc = 0; cc < CB; ++cc) Lr[r][cc] = M[o + r + (o + cc) * lda];
-    __syncthreads();
-    TV s = y[o + r];
-    TV xv[CB];
-    TV mine = 0;
+    for (int c = 0; c < CB; ++c) xcol[c] = Xb[c * CB + r];
+    if (a < b) {
+        TV lv[CB];
 #pragma unroll
-    for (int j = CB - 1; j >= 0; --j) {
-        const TV xj = __shfl(s, j) / Lr[j][j];
-        xv[j] = xj;
-        if (r == j) mine = xj;
-        if (r < j) s -= Lr[j][r] * xj;
+        for (int cc = 0; cc < CB; ++cc) lv[cc] = M[o + r + (oa + cc) * lda];
+#pragma unroll
+        for (int cc = 0; cc < CB; ++cc) Ls[r][cc] = lv[cc];
     }
-    if (c == b) {
-        x[o + r] = mine;
+    ys[r] = y[o + r];
+    __syncthreads();
+    TV xb = 0;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) xb += xcol[c] * ys[c];
+    if (a == b) {
+        x[o + r] = xb;
         return;
     }
-    __syncthreads();
-    const int64_t oc = (int64_t)c * CB;
-#pragma unroll
-    for (int cc = 0; cc < CB; ++cc) Lr[r][cc] = M[o + r + (oc + cc) * lda];  // Lr[l][cc] = L_bc(l, cc)
+    xs[r] = xb;
     __syncthreads();
     TV acc = 0;
 #pragma unroll
-    for (int l = 0; l < CB; ++l) acc += Lr[l][r] * xv[l];
-    y[oc + r] -= acc;
+    for (int l = 0; l < CB; ++l) acc += Ls[l][r] * xs[l];
+    y[oa + r] -= acc;
 }
 
 __global__ void k_clear_flag(int* f) {
@@ -229,14 +262,14 @@ void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail) 
     const int64_t lda = chol_lda(n);
     const int T = np / CB;
     TV* y = work;
-    TV* Ld = work + np;
+    TV* Xinv = work + np;
     for (int k = 0; k < T; ++k) {
-        hipLaunchKernelGGL(k_chol_diag<TV>, dim3(1), dim3(64), 0, s, k, lda, M, Ld, fail);
-        hipLaunchKernelGGL(k_chol_trsm<TV>, dim3(T - k), dim3(64), 0, s, k, lda, M, (const TV*)Ld);
+        hipLaunchKernelGGL(k_chol_panel<TV>, dim3(T - k + 1), dim3(128), 0, s, k, T, lda, M, Xinv, fail);
         if (k + 1 < T) hipLaunchKernelGGL(k_chol_update<TV>, dim3(T - k - 1, T - k), dim3(256), 0, s, k, lda, M);
     }
     hipLaunchKernelGGL(k_chol_gety<TV>, dim3((np + 255) / 256), dim3(256), 0, s, np, lda, (const TV*)M, y);
-    for (int b = T - 1; b >= 0; --b) hipLaunchKernelGGL(k_chol_bsolve<TV>, dim3(b + 1), dim3(64), 0, s, b, lda, (const TV*)M, y, x);
+    for (int b = T - 1; b >= 0; --b)
+        hipLaunchKernelGGL(k_chol_bsolve<TV>, dim3(b + 1), dim3(64), 0, s, b, lda, (const TV*)M, (const TV*)Xinv, y, x);
 }
 
 void launch_clear_flag(hipStream_t s, int* flag) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, flag); }
