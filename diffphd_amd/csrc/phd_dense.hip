@@ -43,155 +43,242 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// 1 / sqrt(p) for the pivots: the hardware reciprocal square root (1 ulp in
+// FP32); in FP64 refined by two Newton steps y <- y (3 - p y^2) / 2.
+__device__ __forceinline__ float pivot_rsqrt(float p) { return __builtin_amdgcn_rsqf(p); }
+__device__ __forceinline__ double pivot_rsqrt(double p) {
+    double y = __builtin_amdgcn_rsq(p);
+    const double h = 0.5 * p;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+
 template <class TV> struct V16;
 template <> struct V16<float> { using type = float4; static constexpr int N = 4; };
 template <> struct V16<double> { using type = double2; static constexpr int N = 2; };
 
-constexpr int kColStride = 2 * CB + 4;  // one published pivot column, see k_chol_panel
+constexpr int kColStride = 2 * CB + 4;  // one published pivot column, see k_chol_step
+
+// Pivot of step j from the window's column j (wave 0): publishes the column
+// into buffer j & 1, L(r, j) at CB + r - j for r > j so that L(j + m, j) sits
+// at CB + m for every j, and 1 / L(j, j) at CB (the slot of the diagonal,
+// which no update reads). Returns L(r, j) for r > j, 1 / L(j, j) on lane j and
+// zero above it (lanes r <= j only ever touch their own upper triangle).
+template <class TV>
+__device__ __forceinline__ TV panel_pivot(int j, TV zj, TV* colS, int r, bool& bad_any) {
+    TV* cb = colS + (j & 1) * kColStride;
+    const TV p = lane_bcast(zj, j);
+    const bool bad = !(p > TV(0));
+    bad_any |= bad;
+    const TV inv = bad ? TV(1) : pivot_rsqrt(p);
+    const TV coef = (r > j) ? zj * inv : (r == j ? inv : TV(0));
+    cb[CB + r - j] = coef;
+    if (r == CB - 1) {  // the two slots past the column, stale from step j - 2
+        cb[2 * CB - j] = TV(0);
+        cb[2 * CB + 1 - j] = TV(0);
+    }
+    return coef;
+}
+
+template <class TV>
+__device__ __forceinline__ void read16(const TV* src, TV (&c)[16]) {
+    using VT = typename V16<TV>::type;
+    constexpr int VN = V16<TV>::N;
+#pragma unroll
+    for (int e = 0; e < 16 / VN; ++e) {
+        const VT t = reinterpret_cast<const VT*>(src)[e];
+        memcpy(c + VN * e, &t, sizeof(VT));
+    }
+}
 
 // Steps [j, j + 16) of the panel chain. Window z[m] = column j + m of the
 // lane's row; columns past 63 never matter, and from step j on only
 // m <= 63 - j can reach one, so this quarter of the steps stops at m = MB.
+// Role 0 (the diagonal wave) takes L(j + 1, j) from lane j + 1 by readlane,
+// updates its column j + 1 and publishes the pivot of step j + 1 before the
+// rest of step j's update, so the pivot chain never waits on LDS. Role 1
+// solves its block row against the published columns; role 2 only keeps the
+// barriers. The published column is read in chunks of 16, each chunk's reads
+// issued before the previous chunk's update so the LDS latency overlaps the
+// FMAs.
 template <class TV, int MB>
-__device__ __forceinline__ void panel_quarter(int& j, TV (&z)[CB], TV* colS, bool diag_wave, int r, bool& bad_any,
+__device__ __forceinline__ void panel_quarter(int& j, TV (&z)[CB], TV* colS, int role, int r, bool& bad_any, TV& coef,
                                               TV* out, int64_t out_ld) {
-    using VT = typename V16<TV>::type;
-    constexpr int VN = V16<TV>::N;
+    constexpr int NQ = (MB + 1) / 16;
     for (int jj = 0; jj < 16; ++jj, ++j) {
-        TV* cb = colS + (j & 1) * kColStride;
-        TV coef = TV(0);
-        if (diag_wave) {
-            const TV p = lane_bcast(z[0], j);
-            const bool bad = !(p > TV(0));
-            bad_any |= bad;
-            const TV d = bad ? TV(1) : sqrt(p);
-            const TV inv = TV(1) / d;
-            coef = (r > j) ? z[0] * inv : (r == j ? d : TV(0));
-            // L(r, j) at CB + r - j: L(j + m, j) lands at CB + m for every step
-            cb[CB + r - j] = coef;
-            if (r == CB - 1) {  // the two slots past the column, stale from step j - 2
-                cb[2 * CB - j] = TV(0);
-                cb[2 * CB + 1 - j] = TV(0);
-            }
-        }
         __syncthreads();
+        if (role == 2) continue;
+        const bool diag_wave = role == 0;
+        const TV* cb = colS + (j & 1) * kColStride + CB;
+        TV c[2][16];
+        read16<TV>(cb, c[0]);
+        if (NQ > 1) read16<TV>(cb + 16, c[1]);
+        const bool chain = diag_wave && (j + 1 < CB);
+        TV z0 = TV(0), coef_next = TV(0);
+        if (chain) {
+            z0 = z[1] - coef * lane_bcast(coef, j + 1);
+            coef_next = panel_pivot<TV>(j + 1, z0, colS, r, bad_any);
+        }
         if (!diag_wave) {
-            coef = z[0] * (TV(1) / cb[CB]);
+            coef = z[0] * c[0][0];
             out[r + (int64_t)j * out_ld] = coef;
         }
+        z[0] = chain ? z0 : z[1] - coef * c[0][1];
 #pragma unroll
-        for (int q = 0; q < (MB + 1) / 16; ++q) {
-            TV c[16];
+        for (int m = 2; m < 16; ++m) z[m - 1] = z[m] - coef * c[0][m];
 #pragma unroll
-            for (int e = 0; e < 16 / VN; ++e) {
-                const VT t = *reinterpret_cast<const VT*>(cb + CB + 16 * q + VN * e);
-                memcpy(c + VN * e, &t, sizeof(VT));
-            }
+        for (int q = 1; q < NQ; ++q) {
+            if (q + 1 < NQ) read16<TV>(cb + 16 * (q + 1), c[(q + 1) & 1]);
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int m = 16 * q + e;
-                if (m >= 1) z[m - 1] = z[m] - coef * c[e];
-            }
+            for (int e = 0; e < 16; ++e) z[16 * q + e - 1] = z[16 * q + e] - coef * c[q & 1][e];
         }
+        if (diag_wave) coef = coef_next;
     }
 }
 
-// Panel step k, fused: workgroup w of T - k + 1, two waves. Wave 0 factors the
-// diagonal block (every workgroup repeats it): lane r holds row r as a window
-// that shifts one column per step, the pivot comes from lane j by readlane,
-// and the pivot column L(., j) is published through LDS (two alternating
-// buffers, so one barrier per step). Wave 1 solves X L_kk^T = B in lockstep,
-// B = block row k + 1 + w of M for w < T - k (w = T - k - 1 is the
-// right-hand-side block) or the identity for w = T - k, which gives
-// Xinv_k = L_kk^-T (column-major 64 x 64) for the backward substitution.
+// S[l][row] = M(r0 + row, c0 + l) for the 64 x 64 block at (r0, c0); 256
+// threads, all loads in flight before the LDS stores.
+constexpr int kTS = CB + 4;  // LDS tile row stride (16-byte aligned rows)
+
 template <class TV>
-__global__ void __launch_bounds__(128) k_chol_panel(int k, int T, int64_t lda, TV* __restrict__ M, TV* __restrict__ Xinv,
-                                                    int* __restrict__ fail) {
+__device__ __forceinline__ void stage_block(const TV* __restrict__ M, int64_t lda, int64_t r0, int64_t c0,
+                                            TV (*S)[kTS]) {
+    const int t = threadIdx.x;
+    TV v[CB * CB / 256];
+#pragma unroll
+    for (int u = 0; u < CB * CB / 256; ++u) {
+        const int e = t + 256 * u;
+        v[u] = M[r0 + e % CB + (c0 + e / CB) * lda];
+    }
+#pragma unroll
+    for (int u = 0; u < CB * CB / 256; ++u) {
+        const int e = t + 256 * u;
+        S[e / CB][e % CB] = v[u];
+    }
+}
+
+// acc[p] = sum_l A[l][tx] B[l][16 ty + p]: thread (tx, ty) of 256 owns row tx
+// and columns 16 ty .. 16 ty + 15 of a rank-64 product tile (a wave holds one
+// column group, so B is read as broadcast 16-byte vectors).
+template <class TV>
+__device__ __forceinline__ void tile_product(const TV (*A)[kTS], const TV (*B)[kTS], int tx, int ty, TV (&acc)[16]) {
+#pragma unroll
+    for (int p = 0; p < 16; ++p) acc[p] = 0;
+#pragma unroll 4
+    for (int l = 0; l < CB; ++l) {
+        const TV a = A[l][tx];
+        TV b[16];
+        read16<TV>(&B[l][16 * ty], b);
+#pragma unroll
+        for (int p = 0; p < 16; ++p) acc[p] += a * b[p];
+    }
+}
+
+// Rank-64 update of step kp on block (ib, jb): A_ij -= L_{i,kp} L_{j,kp}^T
+// (256 threads; the staging tiles are S0 and S1).
+template <class TV>
+__device__ __forceinline__ void update_block(TV* __restrict__ M, int64_t lda, int kp, int ib, int jb, TV (*S0)[kTS],
+                                             TV (*S1)[kTS]) {
+    const int t = threadIdx.x, tx = t % 64, ty = t / 64;
+    stage_block<TV>(M, lda, (int64_t)ib * CB, (int64_t)kp * CB, S0);
+    stage_block<TV>(M, lda, (int64_t)jb * CB, (int64_t)kp * CB, S1);
+    __syncthreads();
+    TV acc[16];
+    tile_product<TV>(S0, S1, tx, ty, acc);
+    TV* dst = M + (int64_t)ib * CB + tx + ((int64_t)jb * CB + 16 * ty) * lda;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) dst[p * lda] -= acc[p];
+}
+
+// Step k of the factorization, one launch of 256-thread workgroups in two parts
+// that touch disjoint blocks, so the panel of step k runs next to the trailing
+// update of step k - 1:
+//  * workgroups w < T - k + 1 (panel of block column k): the rank-64 update of
+//    step k - 1 is applied here, in LDS, to the two blocks the workgroup
+//    reads: the diagonal block (k, k) (every workgroup repeats it) and block
+//    (k + 1 + w, k); w = T - k - 1 is the right-hand-side block row and
+//    w = T - k the identity. Then wave 0 factors the diagonal block (lane r
+//    holds row r as a window that shifts one column per step, pivots by
+//    readlane, the pivot column published through two alternating LDS
+//    buffers, one barrier per step) and wave 1 solves X L_kk^T = B in
+//    lockstep, writing L(k + 1 + w, k) into M, or Xinv_k = L_kk^-T
+//    (column-major 64 x 64) for the identity;
+//  * the remaining workgroups apply the update of step k - 1 to the blocks
+//    (i, j) with k < j <= i <= T.
+template <class TV>
+__global__ void __launch_bounds__(256) k_chol_step(int k, int T, int64_t lda, TV* __restrict__ M,
+                                                   TV* __restrict__ Xinv, int* __restrict__ fail) {
+    __shared__ __align__(16) TV S0[CB][kTS];
+    __shared__ __align__(16) TV S1[CB][kTS];
     __shared__ __align__(16) TV colS[2 * kColStride];
-    const int wave = threadIdx.x / 64, r = threadIdx.x % 64;
+    const int t = threadIdx.x, tx = t % 64, ty = t / 64;
+    const int NP = T - k + 1;
+    const int kp = k - 1;
+    if ((int)blockIdx.x >= NP) {
+        int u = (int)blockIdx.x - NP, jb = k + 1, cnt = T - jb + 1;
+        while (u >= cnt) { u -= cnt; ++jb; cnt = T - jb + 1; }
+        update_block<TV>(M, lda, kp, jb + u, jb, S0, S1);
+        return;
+    }
     const int w = blockIdx.x;
-    const bool diag_wave = wave == 0;
-    const int64_t c0 = (int64_t)k * CB;
+    const int wave = t / 64, r = t % 64;
+    const int role = wave < 2 ? wave : 2;
+    const bool has_row = w < T - k;          // block row k + 1 + w of M (else the identity)
+    const int64_t c0 = (int64_t)k * CB, i0 = (int64_t)(k + 1 + w) * CB;
     TV z[CB];
-    TV* out;
-    int64_t out_ld;
-    if (diag_wave) {
+    if (k > 0) {
+        stage_block<TV>(M, lda, c0, (int64_t)kp * CB, S0);
+        if (has_row) stage_block<TV>(M, lda, i0, (int64_t)kp * CB, S1);
+        __syncthreads();
+        TV accD[16], accR[16];
+        tile_product<TV>(S0, S0, tx, ty, accD);
+        if (has_row) tile_product<TV>(S1, S0, tx, ty, accR);
+        TV aD[16], aR[16];
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const int64_t off = tx + (c0 + 16 * ty + p) * lda;
+            aD[p] = M[c0 + off];
+            aR[p] = has_row ? M[i0 + off] : TV(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            S0[tx][16 * ty + p] = aD[p] - accD[p];
+            if (has_row) S1[tx][16 * ty + p] = aR[p] - accR[p];
+        }
+        __syncthreads();
+        if (role == 0) {
+#pragma unroll
+            for (int c = 0; c < CB; ++c) z[c] = S0[r][c];
+        } else if (role == 1) {
+#pragma unroll
+            for (int c = 0; c < CB; ++c) z[c] = has_row ? S1[r][c] : (c == r ? TV(1) : TV(0));
+        }
+    } else if (role == 0) {
 #pragma unroll
         for (int c = 0; c < CB; ++c) z[c] = M[c0 + r + (c0 + c) * lda];
-        out = nullptr;
-        out_ld = 0;
-    } else if (w < T - k) {
-        const int64_t i0 = (int64_t)(k + 1 + w) * CB;
+    } else if (role == 1) {
 #pragma unroll
-        for (int c = 0; c < CB; ++c) z[c] = M[i0 + r + (c0 + c) * lda];
-        out = M + i0 + c0 * lda;
-        out_ld = lda;
-    } else {
-#pragma unroll
-        for (int c = 0; c < CB; ++c) z[c] = (c == r) ? TV(1) : TV(0);
-        out = Xinv + (int64_t)k * CB * CB;
-        out_ld = CB;
+        for (int c = 0; c < CB; ++c) z[c] = has_row ? M[i0 + r + (c0 + c) * lda] : (c == r ? TV(1) : TV(0));
     }
-    for (int e = threadIdx.x; e < 2 * kColStride; e += 128) colS[e] = TV(0);
+    TV* out = nullptr;
+    int64_t out_ld = 0;
+    if (role == 1) {
+        out = has_row ? M + i0 + c0 * lda : Xinv + (int64_t)k * CB * CB;
+        out_ld = has_row ? lda : CB;
+    }
+    for (int e = t; e < 2 * kColStride; e += 256) colS[e] = TV(0);
     __syncthreads();
     bool bad_any = false;
+    TV coef = TV(0);
+    if (role == 0) coef = panel_pivot<TV>(0, z[0], colS, r, bad_any);
     int j = 0;
-    panel_quarter<TV, 63>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
-    panel_quarter<TV, 47>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
-    panel_quarter<TV, 31>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
-    panel_quarter<TV, 15>(j, z, colS, diag_wave, r, bad_any, out, out_ld);
-    if (diag_wave && bad_any && w == 0) fail[0] = 1;
-}
-
-// Trailing update of step k: A_ij -= L_ik L_jk^T for k < j <= i <= T. Thread
-// (tx, ty) owns rows tx + 16q and columns ty + 16p of the 64 x 64 tile.
-template <class TV>
-__global__ void __launch_bounds__(256) k_chol_update(int k, int64_t lda, TV* __restrict__ M) {
-    const int j = k + 1 + blockIdx.x;
-    const int i = k + 1 + blockIdx.y;
-    if (i < j) return;
-    __shared__ TV As[CB][CB];
-    __shared__ TV Bs[CB][CB];
-    const int t = threadIdx.x;
-    const int64_t c0 = (int64_t)k * CB;
-    {
-        TV ta[CB * CB / 256], tb[CB * CB / 256];
-#pragma unroll
-        for (int u = 0; u < CB * CB / 256; ++u) {
-            const int e = t + 256 * u, row = e % CB, l = e / CB;
-            ta[u] = M[(int64_t)i * CB + row + (c0 + l) * lda];
-            tb[u] = M[(int64_t)j * CB + row + (c0 + l) * lda];
-        }
-#pragma unroll
-        for (int u = 0; u < CB * CB / 256; ++u) {
-            const int e = t + 256 * u, row = e % CB, l = e / CB;
-            As[l][row] = ta[u];
-            Bs[l][row] = tb[u];
-        }
-    }
-    __syncthreads();
-    const int tx = t % 16, ty = t / 16;
-    TV acc[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int p = 0; p < 4; ++p) acc[q][p] = 0;
-    for (int l = 0; l < CB; ++l) {
-        TV a[4], b[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { a[q] = As[l][tx + 16 * q]; b[q] = Bs[l][ty + 16 * q]; }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int p = 0; p < 4; ++p) acc[q][p] += a[q] * b[p];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            TV* dst = M + (int64_t)i * CB + tx + 16 * q + ((int64_t)j * CB + ty + 16 * p) * lda;
-            *dst -= acc[q][p];
-        }
+    panel_quarter<TV, 63>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+    panel_quarter<TV, 47>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+    panel_quarter<TV, 31>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+    panel_quarter<TV, 15>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+    if (role == 0 && bad_any && w == 0) fail[0] = 1;
 }
 
 // y = row np of the layout (L^-1 rhs after the factorization).
@@ -264,8 +351,9 @@ void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail) 
     TV* y = work;
     TV* Xinv = work + np;
     for (int k = 0; k < T; ++k) {
-        hipLaunchKernelGGL(k_chol_panel<TV>, dim3(T - k + 1), dim3(128), 0, s, k, T, lda, M, Xinv, fail);
-        if (k + 1 < T) hipLaunchKernelGGL(k_chol_update<TV>, dim3(T - k - 1, T - k), dim3(256), 0, s, k, lda, M);
+        // update blocks of step k - 1 left to the second part: columns k + 1 .. T - 1, rows j .. T
+        const int nu = k == 0 ? 0 : (T - k) * (T - k + 1) / 2 - 1;
+        hipLaunchKernelGGL(k_chol_step<TV>, dim3(T - k + 1 + nu), dim3(256), 0, s, k, T, lda, M, Xinv, fail);
     }
     hipLaunchKernelGGL(k_chol_gety<TV>, dim3((np + 255) / 256), dim3(256), 0, s, np, lda, (const TV*)M, y);
     for (int b = T - 1; b >= 0; --b)

@@ -119,6 +119,13 @@ void launch_symmetrize_lower(hipStream_t s, int K, double* H);
 
 void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
+// Gs (ld x ld, column-major, ld a multiple of 64, zero past n) = full symmetric
+// matrix from the lower triangle of G (n x n, column-major).
+template <class TV>
+void launch_sym_full(hipStream_t s, int n, int64_t ld, const double* G, TV* Gs);
+// y = Gs x for the full symmetric Gs above.
+template <class TV>
+void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y);
 
 // out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]
 void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,

@@ -1149,6 +1149,69 @@ void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q
     if (K == 0) return;
     hipLaunchKernelGGL(k_gather_nodes, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, nodes, q, out);
 }
+namespace {
+// Gs (ld x ld, column-major, zero past n) = the full symmetric matrix whose
+// lower triangle G (n x n, column-major) holds; 32 x 32 tiles through LDS so
+// that the mirrored reads stay coalesced.
+template <class TV>
+__global__ void __launch_bounds__(256) k_sym_full(int n, int64_t ld, const double* __restrict__ G, TV* __restrict__ Gs) {
+    __shared__ double t[32][33];  // t[q][i] = G(32 sr + i, 32 sc + q), lower entries only
+    const int tr = blockIdx.x, tc = blockIdx.y;
+    const int lx = threadIdx.x % 32, ly = threadIdx.x / 32;
+    const int sr = max(tr, tc), sc = min(tr, tc);
+    for (int q = ly; q < 32; q += 8) {
+        const int r = 32 * sr + lx, c = 32 * sc + q;
+        t[q][lx] = (r < n && c < n && r >= c) ? G[r + (int64_t)c * n] : 0.0;
+    }
+    __syncthreads();
+    for (int q = ly; q < 32; q += 8) {
+        const int r = 32 * tr + lx, c = 32 * tc + q;
+        const double v = (tr > tc || (tr == tc && lx >= q)) ? t[q][lx] : t[lx][q];
+        Gs[r + (int64_t)c * ld] = (TV)v;
+    }
+}
+
+// y = Gs x for a symmetric Gs (ld x ld, zero past n): y_i = (column i) . x,
+// one wavefront per column, 16-byte loads, FP64 accumulation.
+template <class TV>
+__global__ void __launch_bounds__(256) k_gemv_sym(int n, int64_t ld, const TV* __restrict__ Gs,
+                                                  const double* __restrict__ x, double* __restrict__ y) {
+    using V16 = typename Vec16<TV>::type;
+    constexpr int VN = Vec16<TV>::N;
+    const int i = blockIdx.x * 4 + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (i >= n) return;
+    const TV* col = Gs + (int64_t)i * ld;
+    double acc = 0;
+    for (int e0 = VN * lane; e0 < n; e0 += VN * WAVE) {
+        const V16 g = *reinterpret_cast<const V16*>(col + e0);
+        TV gv[VN];
+        memcpy(gv, &g, sizeof(g));
+#pragma unroll
+        for (int e = 0; e < VN; ++e)
+            if (e0 + e < n) acc += (double)gv[e] * x[e0 + e];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) y[i] = acc;
+}
+}  // namespace
+
+template <class TV>
+void launch_sym_full(hipStream_t s, int n, int64_t ld, const double* G, TV* Gs) {
+    if (n == 0) return;
+    const unsigned t = (unsigned)(ld / 32);
+    hipLaunchKernelGGL(k_sym_full<TV>, dim3(t, t), dim3(256), 0, s, n, ld, G, Gs);
+}
+template <class TV>
+void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gemv_sym<TV>, dim3((n + 3) / 4), dim3(256), 0, s, n, ld, Gs, x, y);
+}
+template void launch_sym_full<float>(hipStream_t, int, int64_t, const double*, float*);
+template void launch_sym_full<double>(hipStream_t, int, int64_t, const double*, double*);
+template void launch_gemv_sym<float>(hipStream_t, int, int64_t, const float*, const double*, double*);
+template void launch_gemv_sym<double>(hipStream_t, int, int64_t, const double*, const double*, double*);
+
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G) {
     if (nb == 0) return;
     hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);

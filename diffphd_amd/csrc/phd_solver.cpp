@@ -1291,7 +1291,9 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     };
     // Constrained solve data (Woodbury basis); nb == 0 means the plain backbone.
     int nb = 0;
-    DevBuf<double> d_G, d_Gx, d_dirs;
+    DevBuf<double> d_G, d_Gx, d_dirs, d_Gs64;
+    DevBuf<float> d_Gs32;  // G^-1 as a full symmetric matrix (leading dimension gld)
+    int64_t gld = 0;
     DevBuf<int> d_bcidx;
     std::vector<int> b_c;            // candidate index of each basis row
     std::vector<double> b_dir;       // 3 per basis row
@@ -1318,11 +1320,10 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                     ub[a] = b_dir[3 * a] * uC[3 * c] + b_dir[3 * a + 1] * uC[3 * c + 1] + b_dir[3 * a + 2] * uC[3 * c + 2];
                 }
                 PhaseTimer tbs(this, kBwdBasisSolve);
-                // G^-1 ub with the inverse formed once per frame (lower triangle, symmetric product)
+                // G^-1 ub with the inverse formed once per frame (FP32 copy in mixed mode)
                 hip_check(hipMemcpyAsync(d_rhsK_.p, ub.data(), sizeof(double) * nb, hipMemcpyHostToDevice, stream_), "ub");
-                const double one = 1.0, zero = 0.0;
-                check(rocblas_dsymv((rocblas_handle)blas_, rocblas_fill_lower, nb, &one, d_G.p, nb, d_rhsK_.p, 1, &zero,
-                                    d_Gx.p, 1) == rocblas_status_success, "dsymv (basis) failed");
+                if (precision == 1) launch_gemv_sym<float>(stream_, nb, gld, d_Gs32.p, d_rhsK_.p, d_Gx.p);
+                else launch_gemv_sym<double>(stream_, nb, gld, d_Gs64.p, d_rhsK_.p, d_Gx.p);
                 hip_check(hipMemcpyAsync(ub.data(), d_Gx.p, sizeof(double) * nb, hipMemcpyDeviceToHost, stream_), "ub D2H");
                 hip_check(hipStreamSynchronize(stream_), "basis solve");
                 std::fill(Fc.begin(), Fc.end(), 0.0);
@@ -1482,6 +1483,14 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
                 hip_check(hipStreamSynchronize(stream_), "basis inverse");
                 check(info == 0, "Contact basis Gram matrix is singular.");
+                gld = ((int64_t)nb + 63) / 64 * 64;
+                if (precision == 1) {
+                    d_Gs32.alloc((size_t)(gld * gld));
+                    launch_sym_full<float>(stream_, nb, gld, d_G.p, d_Gs32.p);
+                } else {
+                    d_Gs64.alloc((size_t)(gld * gld));
+                    launch_sym_full<double>(stream_, nb, gld, d_G.p, d_Gs64.p);
+                }
                 d_Gx.alloc(nb);
                 tbf.reset();
                 PhaseTimer twb(this, kBwdWoodbury);
