@@ -478,13 +478,17 @@ __global__ void k_residual(int nv, const int* __restrict__ Ap, const int* __rest
 }
 
 // -----------------------------------------------------------------------------
-// Reductions: block partial sums of up to 16 dot products, then a final pass.
+// Reductions: block partial sums of up to 32 dot products; the last block to
+// finish (ticket counter) adds the partials in block order, so the result does
+// not depend on the order in which blocks complete.
 // -----------------------------------------------------------------------------
-constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_BLOCKS = 256;
 constexpr int DOT_THREADS = 256;
 
-__global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial) {
+__global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial,
+                                                      unsigned int* __restrict__ ticket, double* __restrict__ out) {
     __shared__ double sh[DOT_THREADS / WAVE][32];
+    __shared__ bool last;
     double acc[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.0;
@@ -507,14 +511,18 @@ __global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* 
         for (int w = 0; w < DOT_THREADS / WAVE; ++w) s += sh[w][threadIdx.x];
         partial[blockIdx.x * 32 + threadIdx.x] = s;
     }
-}
-
-__global__ void k_dots_fin(int m, const double* __restrict__ partial, double* __restrict__ out) {
-    const int i = threadIdx.x;
-    if (i >= m) return;
-    double s = 0;
-    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b * 32 + i];
-    out[i] = s;
+    __threadfence();
+    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int i = wid; i < a.m; i += DOT_THREADS / WAVE) {
+        double s = 0;
+        for (int b = lane; b < (int)gridDim.x; b += WAVE) s += partial[b * 32 + i];
+        s = wave_sum(s);
+        if (lane == 0) out[i] = s;
+    }
+    if (threadIdx.x == 0) *ticket = 0u;
 }
 
 __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double b, const double* __restrict__ y, double c,
@@ -1079,9 +1087,8 @@ void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const 
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, Ap, Ai, Ax, b, q, dmask, r);
 }
 int dots_partial_size() { return DOT_BLOCKS * 32; }
-void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {
-    hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial);
-    hipLaunchKernelGGL(k_dots_fin, dim3(1), dim3(64), 0, s, args.m, partial, out);
+void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, unsigned int* ticket, double* out) {
+    hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial, ticket, out);
 }
 void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out) {

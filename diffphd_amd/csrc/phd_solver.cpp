@@ -486,6 +486,10 @@ void Body::setup(double dt, int threads) {
     d_elem_.alloc((size_t)ne_ * edim_ * 3);
     d_dot_partial_.alloc(dots_partial_size());
     d_dot_out_.alloc(32);
+    if (!d_dot_ticket_.p) {  // the reduction leaves its ticket at zero; start it there once
+        d_dot_ticket_.alloc(1);
+        hip_check(hipMemset(d_dot_ticket_.p, 0, sizeof(unsigned int)), "dot ticket");
+    }
     w32_.alloc((size_t)3 * spmv_ldw(nv_)); y32_.alloc((size_t)3 * nv_);
     w64_.alloc((size_t)3 * spmv_ldw(nv_)); y64_.alloc((size_t)3 * nv_);
     check(plan_.chunk_w == 1024, "row-group chunk width must match the kernel");
@@ -647,7 +651,7 @@ std::vector<double> Body::dots(const std::vector<std::pair<const double*, const 
     a.m = (int)pairs.size();
     check(a.m <= 32, "too many dot products");
     for (int i = 0; i < a.m; ++i) { a.a[i] = pairs[i].first; a.b[i] = pairs[i].second; }
-    launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_out_.p);
+    launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_ticket_.p, d_dot_out_.p);
     std::vector<double> out(a.m);
     hip_check(hipMemcpyAsync(out.data(), d_dot_out_.p, sizeof(double) * a.m, hipMemcpyDeviceToHost, stream_), "dots D2H");
     hip_check(hipStreamSynchronize(stream_), "dots sync");
@@ -829,33 +833,93 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
 
 // ---------------------------------------------------------------------------
 // Type-II Anderson acceleration on device vectors (forward_gpu.cpp:1346-1379,
-// backward_gpu.cpp:822-851): raw (x_i, g_i) history in ring slots.
+// backward_gpu.cpp:822-851): the last (x, g), a window of their differences,
+// and the Gram matrix of the g differences cached as they arrive.
 // ---------------------------------------------------------------------------
-namespace {
 struct DevAnderson {
     int window = 5;
-    std::deque<int> slots;      // indices into the slot arrays, oldest first
-    std::vector<int> free_slots;
-    DevBuf<double>* X = nullptr;  // 8 slots of x
-    DevBuf<double>* Gs = nullptr; // 8 slots of g
-    DevBuf<double> dQ[7], dG[7];
     int n = 0;
-    void init(int w, int n_, DevBuf<double>* xs, DevBuf<double>* gs) {
+    DevBuf<double>* Q = nullptr;  // 8 buffers: 0..6 differences of x, 7 the last x
+    DevBuf<double>* G = nullptr;  // 8 buffers: 0..6 differences of g, 7 the last g
+    bool have_last = false;
+    std::deque<int> diffs;        // pool indices of the differences, oldest first
+    std::vector<int> free_slots;
+    double gram[7][7] = {};       // dG_a . dG_b by pool index, filled as differences arrive
+    void init(int w, int n_, DevBuf<double>* q, DevBuf<double>* g) {
         window = std::min(w, 6);
         n = n_;
-        X = xs;
-        Gs = gs;
-        slots.clear();
-        free_slots.clear();
-        for (int i = 7; i >= 0; --i) free_slots.push_back(i);
-        for (int i = 0; i < 8; ++i) { X[i].alloc(n); Gs[i].alloc(n); }
+        Q = q;
+        G = g;
+        clear();
+        for (int i = 0; i < 8; ++i) { Q[i].alloc(n); G[i].alloc(n); }
     }
     void clear() {
-        for (int s : slots) free_slots.push_back(s);
-        slots.clear();
+        diffs.clear();
+        free_slots.clear();
+        for (int i = 6; i >= 0; --i) free_slots.push_back(i);
+        have_last = false;
     }
 };
-}  // namespace
+
+// One Anderson step (forward_gpu.cpp:1346-1379, backward_gpu.cpp:822-851)
+// on the map x -> x + g at x = xprev: records the newest difference pair
+// (x - x_last, g - g_last), forms the Gram matrix of the g differences from
+// the entries cached as the differences arrived (one new row per call), solves
+// the regularized least squares and writes xprev + g - sum gamma_a (dQ_a + dG_a)
+// into out. `extra` dot pairs ride along in the same reduction. Returns false
+// when nothing was combined: fewer than two iterates, or a restart because
+// |gamma| >= 10 (the history is then dropped, the current iterate included).
+bool Body::anderson_step(DevAnderson& aa, const double* xprev, const double* g, double* out,
+                         const std::vector<std::pair<const double*, const double*>>& extra,
+                         std::vector<double>* extra_out) {
+    const int n3 = aa.n;
+    if (aa.have_last) {
+        const int pi = aa.free_slots.back();
+        aa.free_slots.pop_back();
+        launch_axpbypcz(stream_, n3, 1.0, xprev, -1.0, aa.Q[7].p, 0.0, nullptr, aa.Q[pi].p);
+        launch_axpbypcz(stream_, n3, 1.0, g, -1.0, aa.G[7].p, 0.0, nullptr, aa.G[pi].p);
+        aa.diffs.push_back(pi);
+    }
+    hip_check(hipMemcpyAsync(aa.Q[7].p, xprev, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
+    hip_check(hipMemcpyAsync(aa.G[7].p, g, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
+    aa.have_last = true;
+    const int m = (int)aa.diffs.size();
+    std::vector<std::pair<const double*, const double*>> pr;
+    if (m >= 1) {
+        const int nw = aa.diffs.back();
+        for (int a = 0; a < m; ++a) pr.push_back({aa.G[aa.diffs[a]].p, aa.G[nw].p});
+        for (int a = 0; a < m; ++a) pr.push_back({aa.G[aa.diffs[a]].p, g});
+    }
+    for (const auto& e : extra) pr.push_back(e);
+    if (pr.empty()) return false;
+    const std::vector<double> dd = dots(pr);
+    if (extra_out) extra_out->assign(dd.begin() + 2 * m, dd.end());
+    if (m == 0) return false;
+    const int nw = aa.diffs.back();
+    for (int a = 0; a < m; ++a) aa.gram[aa.diffs[a]][nw] = aa.gram[nw][aa.diffs[a]] = dd[a];
+    std::vector<double> GtG(m * m), rr(m);
+    double sq = 0;
+    for (int a = 0; a < m; ++a) {
+        for (int b = 0; b < m; ++b) GtG[a * m + b] = aa.gram[aa.diffs[a]][aa.diffs[b]];
+        rr[a] = dd[m + a];
+        sq += GtG[a * m + a];
+    }
+    const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+    for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
+    const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
+    if (nrm2(gamma) >= 10.0) {
+        aa.clear();
+        return false;
+    }
+    std::vector<const double*> pq(m), pg(m);
+    for (int a = 0; a < m; ++a) { pq[a] = aa.Q[aa.diffs[a]].p; pg[a] = aa.G[aa.diffs[a]].p; }
+    launch_aa_combine(stream_, n3, xprev, g, m, pq.data(), pg.data(), gamma.data(), out);
+    while ((int)aa.diffs.size() > aa.window - 1) {
+        aa.free_slots.push_back(aa.diffs.front());
+        aa.diffs.pop_front();
+    }
+    return true;
+}
 
 // ---------------------------------------------------------------------------
 // Forward step.
@@ -1115,42 +1179,8 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         if (aa_window > 0) {
             PhaseTimer ta(this, kFwdAA);
             launch_axpbypcz(stream_, n3, 1.0, d_q_.p, -1.0, d_qprev_.p, 0.0, nullptr, d_g_.p);
-            int slot = aa.free_slots.back();
-            aa.free_slots.pop_back();
-            hip_check(hipMemcpyAsync(aa.X[slot].p, d_qprev_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
-            hip_check(hipMemcpyAsync(aa.Gs[slot].p, d_g_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
-            aa.slots.push_back(slot);
-            const int m = (int)aa.slots.size() - 1;
-            if (m >= 1) {
-                for (int i = 0; i < m; ++i) {
-                    aa.dQ[i].alloc(n3);
-                    aa.dG[i].alloc(n3);
-                    launch_axpbypcz(stream_, n3, 1.0, aa.X[aa.slots[i + 1]].p, -1.0, aa.X[aa.slots[i]].p, 0.0, nullptr, aa.dQ[i].p);
-                    launch_axpbypcz(stream_, n3, 1.0, aa.Gs[aa.slots[i + 1]].p, -1.0, aa.Gs[aa.slots[i]].p, 0.0, nullptr, aa.dG[i].p);
-                }
-                std::vector<std::pair<const double*, const double*>> pr;
-                for (int a = 0; a < m; ++a)
-                    for (int b = 0; b <= a; ++b) pr.push_back({aa.dG[a].p, aa.dG[b].p});
-                for (int a = 0; a < m; ++a) pr.push_back({aa.dG[a].p, d_g_.p});
-                const std::vector<double> dd = dots(pr);
-                std::vector<double> GtG(m * m), rr(m);
-                double sq = 0;
-                int idx = 0;
-                for (int a = 0; a < m; ++a)
-                    for (int b = 0; b <= a; ++b) { GtG[a * m + b] = GtG[b * m + a] = dd[idx++]; }
-                for (int a = 0; a < m; ++a) { rr[a] = dd[idx++]; sq += GtG[a * m + a]; }
-                const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
-                for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
-                const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
-                if (nrm2(gamma) >= 10.0) aa.clear();
-                else {
-                    std::vector<const double*> pq(m), pg(m);
-                    for (int a = 0; a < m; ++a) { pq[a] = aa.dQ[a].p; pg[a] = aa.dG[a].p; }
-                    launch_aa_combine(stream_, n3, d_qprev_.p, d_g_.p, m, pq.data(), pg.data(), gamma.data(), d_q_.p);
-                    launch_set_dirichlet(stream_, (int)ddof_.size(), d_ddof_.p, d_dval_.p, d_q_.p);
-                }
-            }
-            while ((int)aa.slots.size() > aa.window) { aa.free_slots.push_back(aa.slots.front()); aa.slots.pop_front(); }
+            if (anderson_step(aa, d_qprev_.p, d_g_.p, d_q_.p, {}, nullptr))
+                launch_set_dirichlet(stream_, (int)ddof_.size(), d_ddof_.p, d_dval_.p, d_q_.p);
         }
         // dual gate (:1382-1397)
         PhaseTimer tg(this, kFwdGate);
@@ -1336,45 +1366,13 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             else spmv_ST<double>(y64_, xpart64_, V64_.p, d_sprev.p, d_zero.p, d_s.p);
             safe_clip(d_s.p);
             launch_axpbypcz(stream_, n3, 1.0, d_s.p, -1.0, d_sprev.p, 0.0, nullptr, d_g_.p);
-            const double err = std::sqrt(dots({{d_g_.p, d_g_.p}})[0]);
+            double err;
             if (aa_window > 0) {
-                int slot = aa.free_slots.back();
-                aa.free_slots.pop_back();
-                hip_check(hipMemcpyAsync(aa.X[slot].p, d_sprev.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
-                hip_check(hipMemcpyAsync(aa.Gs[slot].p, d_g_.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
-                aa.slots.push_back(slot);
-                const int m = (int)aa.slots.size() - 1;
-                if (m >= 1) {
-                    for (int a = 0; a < m; ++a) {
-                        aa.dQ[a].alloc(n3);
-                        aa.dG[a].alloc(n3);
-                        launch_axpbypcz(stream_, n3, 1.0, aa.X[aa.slots[a + 1]].p, -1.0, aa.X[aa.slots[a]].p, 0.0, nullptr, aa.dQ[a].p);
-                        launch_axpbypcz(stream_, n3, 1.0, aa.Gs[aa.slots[a + 1]].p, -1.0, aa.Gs[aa.slots[a]].p, 0.0, nullptr, aa.dG[a].p);
-                    }
-                    std::vector<std::pair<const double*, const double*>> pr;
-                    for (int a = 0; a < m; ++a)
-                        for (int b = 0; b <= a; ++b) pr.push_back({aa.dG[a].p, aa.dG[b].p});
-                    for (int a = 0; a < m; ++a) pr.push_back({aa.dG[a].p, d_g_.p});
-                    const std::vector<double> dd = dots(pr);
-                    std::vector<double> GtG(m * m), rr(m);
-                    double sq = 0;
-                    int idx = 0;
-                    for (int a = 0; a < m; ++a)
-                        for (int b = 0; b <= a; ++b) GtG[a * m + b] = GtG[b * m + a] = dd[idx++];
-                    for (int a = 0; a < m; ++a) { rr[a] = dd[idx++]; sq += GtG[a * m + a]; }
-                    const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
-                    for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
-                    const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
-                    if (nrm2(gamma) < 10.0) {
-                        std::vector<const double*> pq(m), pg(m);
-                        for (int a = 0; a < m; ++a) { pq[a] = aa.dQ[a].p; pg[a] = aa.dG[a].p; }
-                        launch_aa_combine(stream_, n3, d_sprev.p, d_g_.p, m, pq.data(), pg.data(), gamma.data(), d_s.p);
-                        safe_clip(d_s.p);
-                    } else {
-                        aa.clear();
-                    }
-                }
-                while ((int)aa.slots.size() > aa.window) { aa.free_slots.push_back(aa.slots.front()); aa.slots.pop_front(); }
+                std::vector<double> e2;
+                if (anderson_step(aa, d_sprev.p, d_g_.p, d_s.p, {{d_g_.p, d_g_.p}}, &e2)) safe_clip(d_s.p);
+                err = std::sqrt(e2[0]);
+            } else {
+                err = std::sqrt(dots({{d_g_.p, d_g_.p}})[0]);
             }
             it_used = i + 1;
             const double sn = std::sqrt(dots({{d_s.p, d_s.p}})[0]);

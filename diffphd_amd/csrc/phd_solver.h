@@ -122,6 +122,8 @@ struct StepStats {
 
 class WarmStartNet;
 
+struct DevAnderson;
+
 class Body {
 public:
     explicit Body(int device);
@@ -243,6 +245,7 @@ private:
     void warm_start(const double* v, double dt, double inv_h2m, bool learned);
     DevBuf<double> d_hist_q_[8], d_hist_g_[8];
     DevBuf<double> d_dot_partial_, d_dot_out_;
+    DevBuf<unsigned int> d_dot_ticket_;
     DevBuf<float> w32_, y32_, part32_, xpart32_;
     DevBuf<double> w64_, y64_, part64_, xpart64_;
     // contact
@@ -292,6 +295,8 @@ private:
     void local_rhs(const double* q, double inv_h2m);
     void kpd_apply(const double* x, double* out, int precision);
     std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
+    bool anderson_step(DevAnderson& aa, const double* xprev, const double* g, double* out,
+                       const std::vector<std::pair<const double*, const double*>>& extra, std::vector<double>* extra_out);
     bool solve_contact_system(int n, int precision, double* rhs_inout);
 };
 
