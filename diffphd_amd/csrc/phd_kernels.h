@@ -76,8 +76,7 @@ void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const 
 void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out);
 
 // ---- reductions and vector updates -------------------------------------------
-// ticket: a device counter that starts at zero (the kernel leaves it at zero)
-void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, unsigned int* ticket, double* out);
+void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out);
 int dots_partial_size();
 // out = x + alpha * (y - z) ... generic axpby: out = a*x + b*y (+ c*z)
 void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,

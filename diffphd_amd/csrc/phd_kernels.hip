@@ -478,17 +478,15 @@ __global__ void k_residual(int nv, const int* __restrict__ Ap, const int* __rest
 }
 
 // -----------------------------------------------------------------------------
-// Reductions: block partial sums of up to 32 dot products; the last block to
-// finish (ticket counter) adds the partials in block order, so the result does
-// not depend on the order in which blocks complete.
+// Reductions: block partial sums of up to 32 dot products, then one wavefront
+// per product adds the block partials in a fixed order (lane b sums blocks
+// b, b + 64, ..., then a butterfly), so results do not depend on scheduling.
 // -----------------------------------------------------------------------------
-constexpr int DOT_BLOCKS = 256;
+constexpr int DOT_BLOCKS = 512;
 constexpr int DOT_THREADS = 256;
 
-__global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial,
-                                                      unsigned int* __restrict__ ticket, double* __restrict__ out) {
+__global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* __restrict__ partial) {
     __shared__ double sh[DOT_THREADS / WAVE][32];
-    __shared__ bool last;
     double acc[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.0;
@@ -511,18 +509,15 @@ __global__ void __launch_bounds__(DOT_THREADS) k_dots(int n, DotArgs a, double* 
         for (int w = 0; w < DOT_THREADS / WAVE; ++w) s += sh[w][threadIdx.x];
         partial[blockIdx.x * 32 + threadIdx.x] = s;
     }
-    __threadfence();
-    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int i = wid; i < a.m; i += DOT_THREADS / WAVE) {
-        double s = 0;
-        for (int b = lane; b < (int)gridDim.x; b += WAVE) s += partial[b * 32 + i];
-        s = wave_sum(s);
-        if (lane == 0) out[i] = s;
-    }
-    if (threadIdx.x == 0) *ticket = 0u;
+}
+
+__global__ void __launch_bounds__(WAVE) k_dots_fin(const double* __restrict__ partial, double* __restrict__ out) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    double s = 0;
+#pragma unroll
+    for (int b = lane; b < DOT_BLOCKS; b += WAVE) s += partial[b * 32 + i];
+    s = wave_sum(s);
+    if (lane == 0) out[i] = s;
 }
 
 __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double b, const double* __restrict__ y, double c,
@@ -643,6 +638,9 @@ __global__ void k_contact_distance(int K, DevBoundary b, const int* __restrict__
     delta[c] = d;
 }
 
+// out_c = (S_C^T y)_c over the etree path of candidate c (one wavefront per
+// candidate). The paths are long (up to the etree depth), so each lane keeps
+// four index -> value chains in flight per pass.
 template <class TV>
 __global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const int* __restrict__ cp_row,
                                  const TV* __restrict__ cp_val, const TV* __restrict__ y, int n, double* __restrict__ out) {
@@ -650,7 +648,22 @@ __global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const in
     const int lane = threadIdx.x % WAVE;
     if (c >= K) return;
     double a0 = 0, a1 = 0, a2 = 0;
-    for (int p = cp_ptr[c] + lane; p < cp_ptr[c + 1]; p += WAVE) {
+    const int end = cp_ptr[c + 1];
+    int p = cp_ptr[c] + lane;
+    for (; p + 3 * WAVE < end; p += 4 * WAVE) {
+        int k[4];
+        TV v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { k[u] = cp_row[p + u * WAVE]; v[u] = cp_val[p + u * WAVE]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double vv = (double)v[u];
+            a0 += vv * (double)y[k[u]];
+            a1 += vv * (double)y[n + k[u]];
+            a2 += vv * (double)y[2 * n + k[u]];
+        }
+    }
+    for (; p < end; p += WAVE) {
         const int k = cp_row[p];
         const double v = (double)cp_val[p];
         a0 += v * (double)y[k];
@@ -661,6 +674,8 @@ __global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const in
     if (lane == 0) { out[3 * c] = a0; out[3 * c + 1] = a1; out[3 * c + 2] = a2; }
 }
 
+// y_k += (S_C f)_k for the rows crossed by candidate paths (one wavefront per
+// row, four chains in flight per lane as above).
 template <class TV>
 __global__ void k_contact_scatter(int nrows, const int* __restrict__ rc_ptr, const int* __restrict__ rc_row,
                                   const int* __restrict__ rc_c, const TV* __restrict__ rc_val, const double* __restrict__ f,
@@ -669,7 +684,22 @@ __global__ void k_contact_scatter(int nrows, const int* __restrict__ rc_ptr, con
     const int lane = threadIdx.x % WAVE;
     if (r >= nrows) return;
     double a0 = 0, a1 = 0, a2 = 0;
-    for (int p = rc_ptr[r] + lane; p < rc_ptr[r + 1]; p += WAVE) {
+    const int end = rc_ptr[r + 1];
+    int p = rc_ptr[r] + lane;
+    for (; p + 3 * WAVE < end; p += 4 * WAVE) {
+        int cc[4];
+        TV v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { cc[u] = rc_c[p + u * WAVE]; v[u] = rc_val[p + u * WAVE]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double vv = (double)v[u];
+            a0 += vv * f[3 * cc[u]];
+            a1 += vv * f[3 * cc[u] + 1];
+            a2 += vv * f[3 * cc[u] + 2];
+        }
+    }
+    for (; p < end; p += WAVE) {
         const int c = rc_c[p];
         const double v = (double)rc_val[p];
         a0 += v * f[3 * c];
@@ -1087,8 +1117,9 @@ void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const 
     hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, Ap, Ai, Ax, b, q, dmask, r);
 }
 int dots_partial_size() { return DOT_BLOCKS * 32; }
-void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, unsigned int* ticket, double* out) {
-    hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial, ticket, out);
+void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {
+    hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, args, partial);
+    hipLaunchKernelGGL(k_dots_fin, dim3(args.m), dim3(WAVE), 0, s, (const double*)partial, out);
 }
 void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out) {

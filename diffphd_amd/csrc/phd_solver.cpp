@@ -486,10 +486,6 @@ void Body::setup(double dt, int threads) {
     d_elem_.alloc((size_t)ne_ * edim_ * 3);
     d_dot_partial_.alloc(dots_partial_size());
     d_dot_out_.alloc(32);
-    if (!d_dot_ticket_.p) {  // the reduction leaves its ticket at zero; start it there once
-        d_dot_ticket_.alloc(1);
-        hip_check(hipMemset(d_dot_ticket_.p, 0, sizeof(unsigned int)), "dot ticket");
-    }
     w32_.alloc((size_t)3 * spmv_ldw(nv_)); y32_.alloc((size_t)3 * nv_);
     w64_.alloc((size_t)3 * spmv_ldw(nv_)); y64_.alloc((size_t)3 * nv_);
     check(plan_.chunk_w == 1024, "row-group chunk width must match the kernel");
@@ -651,7 +647,7 @@ std::vector<double> Body::dots(const std::vector<std::pair<const double*, const 
     a.m = (int)pairs.size();
     check(a.m <= 32, "too many dot products");
     for (int i = 0; i < a.m; ++i) { a.a[i] = pairs[i].first; a.b[i] = pairs[i].second; }
-    launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_ticket_.p, d_dot_out_.p);
+    launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_out_.p);
     std::vector<double> out(a.m);
     hip_check(hipMemcpyAsync(out.data(), d_dot_out_.p, sizeof(double) * a.m, hipMemcpyDeviceToHost, stream_), "dots D2H");
     hip_check(hipStreamSynchronize(stream_), "dots sync");
@@ -1004,6 +1000,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         }
         if (K > 0) {
             PhaseTimer tc(this, kFwdContact);
+            std::unique_ptr<PhaseTimer> tcs(new PhaseTimer(this, kFwdCStage));
             // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
@@ -1013,6 +1010,8 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             d_delta_.download(delta.data(), K);
             d_dF_.download(qC.data(), 3 * K);
             for (int i = 0; i < 3 * K; ++i) uC[i] += qC[i];
+            tcs.reset();
+            std::unique_ptr<PhaseTimer> tcset(new PhaseTimer(this, kFwdCSets));
             // omega, E, h (forward_gpu.cpp:1160-1197)
             for (int c = 0; c < K; ++c) {
                 const double dj = delta[c], lj = lambda[c], rj = rprec[c];
@@ -1037,6 +1036,8 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 if (inJ || inF) Pv.push_back(c);
             }
             const int nJ = (int)Jv.size(), nP = (int)Pv.size(), nF = (int)Fs.size();
+            tcset.reset();
+            std::unique_ptr<PhaseTimer> tcap(new PhaseTimer(this, kFwdCApply));
             if (nF > 0) {
                 hip_check(hipMemcpyAsync(d_idxP_.p, Pv.data(), sizeof(int) * nP, hipMemcpyHostToDevice, stream_), "P");
                 hip_check(hipMemcpyAsync(d_idxF_.p, Fs.data(), sizeof(int) * nF, hipMemcpyHostToDevice, stream_), "Fs");
@@ -1080,6 +1081,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 d_om_.upload(om);
                 d_E_.upload(E);
                 std::vector<double> rJ_keep = rJ;
+                tcap.reset();
                 PhaseTimer td(this, kFwdDense);
                 if (timing_level >= 2) { phase[kDenseCalls] += 1; phase[kDenseRows] += nJ; }
                 bool solved = false;
@@ -1109,7 +1111,9 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 const double dl = (om[c] != 0.0) ? rhs[c] : rhs[c] / E[c];
                 lambda[c] = std::min(std::max(lambda[c] + dl, 0.0), 1e8);
             }
+            tcap.reset();
             // NCP-integrated Coulomb friction (:1255-1302)
+            PhaseTimer tfr(this, kFwdFric);
             if (do_fric) {
                 std::vector<int> Lp_, Jn;
                 for (int c = 0; c < K; ++c) {

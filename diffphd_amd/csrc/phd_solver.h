@@ -168,7 +168,7 @@ public:
     // host wall-clock per phase (seconds, accumulated while `timing` is on); see PhaseId
     enum PhaseId { kFwdTotal = 0, kFwdLocal, kFwdSolve, kFwdContact, kFwdDense, kFwdAA, kFwdGate,
                    kBwdTotal, kBwdKpd, kBwdBackbone, kBwdBasisFactor, kBwdWoodbury, kBwdBasisSolve, kBwdMatGrad,
-                   kDenseCalls, kDenseRows, kDenseFp32Fail, kBasisRows, kNumPhases };
+                   kDenseCalls, kDenseRows, kDenseFp32Fail, kBasisRows, kFwdCStage, kFwdCSets, kFwdCApply, kFwdFric, kNumPhases };
     double phase[kNumPhases] = {0};
     struct PhaseTimer {
         Body* b; int id; double t0;
@@ -245,7 +245,6 @@ private:
     void warm_start(const double* v, double dt, double inv_h2m, bool learned);
     DevBuf<double> d_hist_q_[8], d_hist_g_[8];
     DevBuf<double> d_dot_partial_, d_dot_out_;
-    DevBuf<unsigned int> d_dot_ticket_;
     DevBuf<float> w32_, y32_, part32_, xpart32_;
     DevBuf<double> w64_, y64_, part64_, xpart64_;
     // contact

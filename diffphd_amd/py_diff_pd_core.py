@@ -331,7 +331,8 @@ class _Deformable:
 
     PHASES = ("fwd_total", "fwd_local", "fwd_solve", "fwd_contact", "fwd_dense", "fwd_aa", "fwd_gate",
               "bwd_total", "bwd_kpd", "bwd_backbone", "bwd_basis_factor", "bwd_woodbury", "bwd_basis_solve",
-              "bwd_matgrad", "dense_calls", "dense_rows", "dense_fp32_fail", "basis_rows")
+              "bwd_matgrad", "dense_calls", "dense_rows", "dense_fp32_fail", "basis_rows", "fwd_c_stage",
+              "fwd_c_sets", "fwd_c_apply", "fwd_friction")
 
     def phase_times(self, reset=True):
         """Wall-clock split (s) recorded with timing_enable(2)."""
