@@ -70,9 +70,10 @@ void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, do
 // pd_rhs = sum of element contributions - c_dir; b = inv_h2m * q_tilde + pd_rhs
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b);
-// r = b - A q (scalar CSR shared by the three axes), zero at dmask vertices
-void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const double* Ax, const double* b,
-                     const double* q, const unsigned char* dmask, double* r);
+// r = b - A q for the three axes (AoS); A in ELL form (entry j of row v at
+// j * nv + v, width entries per row), Dirichlet rows (dmask) set to zero.
+void launch_residual(hipStream_t s, int nv, int width, const int* Ai, const double* Ax, const double* b, const double* q,
+                     const unsigned char* dmask, double* r);
 void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out);
 
 // ---- reductions and vector updates -------------------------------------------

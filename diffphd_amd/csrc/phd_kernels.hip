@@ -201,18 +201,28 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
 }
 
 template <class TV>
-__global__ void k_spmv_rows_slots(int n, const int* __restrict__ rbase, const TV* __restrict__ pslot, TV* __restrict__ y) {
+__global__ void __launch_bounds__(256) k_spmv_rows_slots(int n, const int* __restrict__ rbase,
+                                                         const TV* __restrict__ pslot, TV* __restrict__ y) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    TV a0 = 0, a1 = 0, a2 = 0;
-    for (int s = rbase[k]; s < rbase[k + 1]; ++s) {
-        a0 += pslot[3 * (int64_t)s];
-        a1 += pslot[3 * (int64_t)s + 1];
-        a2 += pslot[3 * (int64_t)s + 2];
+    // rows near the top of the etree cross up to ~140 chunks: eight independent
+    // partial sums keep eight loads in flight per lane
+    TV a[8][3] = {};
+    const int s0 = rbase[k], s1 = rbase[k + 1];
+    int s = s0;
+    for (; s + 7 < s1; s += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) a[u][d] += pslot[3 * (int64_t)(s + u) + d];
+    for (; s < s1; ++s)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) a[0][d] += pslot[3 * (int64_t)s + d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const TV t = ((a[0][d] + a[1][d]) + (a[2][d] + a[3][d])) + ((a[4][d] + a[5][d]) + (a[6][d] + a[7][d]));
+        y[(int64_t)d * n + k] = t;
     }
-    y[k] = a0;
-    y[n + k] = a1;
-    y[2 * n + k] = a2;
 }
 
 // -----------------------------------------------------------------------------
@@ -460,16 +470,19 @@ __global__ void k_gather_vertex(int nv, const int* __restrict__ inc_ptr, const i
     out[3 * v] = s0; out[3 * v + 1] = s1; out[3 * v + 2] = s2;
 }
 
-__global__ void k_residual(int nv, const int* __restrict__ Ap, const int* __restrict__ Ai, const double* __restrict__ Ax,
-                           const double* __restrict__ b, const double* __restrict__ q,
-                           const unsigned char* __restrict__ dmask, double* __restrict__ r) {
+// r = b - A q (three right-hand sides, AoS), A in ELL form: entry j of row v
+// at j * nv + v (padding: column v, value 0), so consecutive lanes read
+// consecutive addresses; entries are added in row order.
+__global__ void __launch_bounds__(256) k_residual(int nv, int width, const int* __restrict__ Ai, const double* __restrict__ Ax,
+                                                  const double* __restrict__ b, const double* __restrict__ q,
+                                                  const unsigned char* __restrict__ dmask, double* __restrict__ r) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     if (dmask && dmask[v]) { r[3 * v] = r[3 * v + 1] = r[3 * v + 2] = 0.0; return; }
     double s0 = b[3 * v], s1 = b[3 * v + 1], s2 = b[3 * v + 2];
-    for (int p = Ap[v]; p < Ap[v + 1]; ++p) {
-        const int c = Ai[p];
-        const double a = Ax[p];
+    for (int j = 0; j < width; ++j) {
+        const int c = Ai[(int64_t)j * nv + v];
+        const double a = Ax[(int64_t)j * nv + v];
         s0 -= a * q[3 * c];
         s1 -= a * q[3 * c + 1];
         s2 -= a * q[3 * c + 2];
@@ -1112,9 +1125,9 @@ void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc
 void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out) {
     hipLaunchKernelGGL(k_gather_vertex, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, inc_ptr, inc_idx, elem_out, out);
 }
-void launch_residual(hipStream_t s, int nv, const int* Ap, const int* Ai, const double* Ax, const double* b,
-                     const double* q, const unsigned char* dmask, double* r) {
-    hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, Ap, Ai, Ax, b, q, dmask, r);
+void launch_residual(hipStream_t s, int nv, int width, const int* Ai, const double* Ax, const double* b, const double* q,
+                     const unsigned char* dmask, double* r) {
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, width, Ai, Ax, b, q, dmask, r);
 }
 int dots_partial_size() { return DOT_BLOCKS * 32; }
 void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {
@@ -1210,27 +1223,38 @@ __global__ void __launch_bounds__(256) k_sym_full(int n, int64_t ld, const doubl
 }
 
 // y = Gs x for a symmetric Gs (ld x ld, zero past n): y_i = (column i) . x,
-// one wavefront per column, 16-byte loads, FP64 accumulation.
+// four columns per wavefront so that each x load serves four columns; 16-byte
+// loads of Gs, FP64 accumulation.
 template <class TV>
 __global__ void __launch_bounds__(256) k_gemv_sym(int n, int64_t ld, const TV* __restrict__ Gs,
                                                   const double* __restrict__ x, double* __restrict__ y) {
     using V16 = typename Vec16<TV>::type;
     constexpr int VN = Vec16<TV>::N;
-    const int i = blockIdx.x * 4 + threadIdx.x / WAVE;
+    const int i0 = 4 * (blockIdx.x * 4 + threadIdx.x / WAVE);
     const int lane = threadIdx.x % WAVE;
-    if (i >= n) return;
-    const TV* col = Gs + (int64_t)i * ld;
-    double acc = 0;
+    if (i0 >= n) return;
+    const int nc = min(4, n - i0);
+    double acc[4] = {0, 0, 0, 0};
     for (int e0 = VN * lane; e0 < n; e0 += VN * WAVE) {
-        const V16 g = *reinterpret_cast<const V16*>(col + e0);
-        TV gv[VN];
-        memcpy(gv, &g, sizeof(g));
+        double xv[VN];
 #pragma unroll
-        for (int e = 0; e < VN; ++e)
-            if (e0 + e < n) acc += (double)gv[e] * x[e0 + e];
+        for (int e = 0; e < VN; ++e) xv[e] = (e0 + e < n) ? x[e0 + e] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (c < nc) {
+                const V16 g = *reinterpret_cast<const V16*>(Gs + (int64_t)(i0 + c) * ld + e0);
+                TV gv[VN];
+                memcpy(gv, &g, sizeof(g));
+#pragma unroll
+                for (int e = 0; e < VN; ++e) acc[c] += (double)gv[e] * xv[e];
+            }
+        }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) y[i] = acc;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double t = wave_sum(acc[c]);
+        if (lane == 0 && c < nc) y[i0 + c] = t;
+    }
 }
 }  // namespace
 
@@ -1243,7 +1267,7 @@ void launch_sym_full(hipStream_t s, int n, int64_t ld, const double* G, TV* Gs) 
 template <class TV>
 void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_gemv_sym<TV>, dim3((n + 3) / 4), dim3(256), 0, s, n, ld, Gs, x, y);
+    hipLaunchKernelGGL(k_gemv_sym<TV>, dim3((n + 15) / 16), dim3(256), 0, s, n, ld, Gs, x, y);
 }
 template void launch_sym_full<float>(hipStream_t, int, int64_t, const double*, float*);
 template void launch_sym_full<double>(hipStream_t, int, int64_t, const double*, double*);

@@ -434,9 +434,21 @@ void Body::setup(double dt, int threads) {
     d_cu_i0_.upload(plan_.cu_i0);
     d_cu_i1_.upload(plan_.cu_i1);
     d_tile_unit_ptr_.upload(plan_.tile_unit_ptr);
-    d_Ap_.upload(Ap_);
-    d_Ai_.upload(Ai_);
-    d_Ax_.upload(Ax_);
+    {  // A in ELL form for the residual kernel (entries in CSR row order, zero padding)
+        ell_w_ = 0;
+        for (int v = 0; v < nv_; ++v) ell_w_ = std::max(ell_w_, Ap_[v + 1] - Ap_[v]);
+        std::vector<int> ai((size_t)ell_w_ * nv_);
+        std::vector<double> ax((size_t)ell_w_ * nv_, 0.0);
+        for (int v = 0; v < nv_; ++v)
+            for (int j = 0; j < ell_w_; ++j) {
+                const int p = Ap_[v] + j;
+                const bool in = p < Ap_[v + 1];
+                ai[(size_t)j * nv_ + v] = in ? Ai_[p] : v;
+                if (in) ax[(size_t)j * nv_ + v] = Ax_[p];
+            }
+        d_Aie_.upload(ai);
+        d_Axe_.upload(ax);
+    }
     d_dmask_.upload(dmask_);
     d_ddof_.upload(ddof_);
     d_dval_.upload(dval_);
@@ -730,7 +742,7 @@ void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, boo
     launch_warm_guess(stream_, nv_, q_dev, v_dev, dt, nullptr, 0.f, d_dmask_.p, d_dval_full_.p, d_guess_.p,
                       d_guess_.p + dofs_);
     local_rhs(d_guess_.p, inv_h2m);
-    launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_guess_.p, d_dmask_.p, d_r_.p);
+    launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_guess_.p, d_dmask_.p, d_r_.p);
     launch_mlp_features(stream_, nv_, q_dev, v_dev, d_Xrest_.p, d_r_.p, d_adiag_.p, d_logk_.p, d_cflag_.p, d_cdist_.p, dt,
                         L, g, d_fmean_.p, d_fstd_.p, standardize ? 1 : 0, d_feat_.p);
 }
@@ -765,7 +777,7 @@ void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) 
     for (int g = 0; g < 2; ++g) {
         const double* x = d_guess_.p + (size_t)g * n3;
         local_rhs(x, inv_h2m);
-        launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, x, d_dmask_.p, d_r_.p);
+        launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, x, d_dmask_.p, d_r_.p);
         spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
         spmv_ST<double>(y64_, xpart64_, V64_.p, nullptr, nullptr, d_tmp_.p);
         rn[g] = std::sqrt(dots({{d_tmp_.p, d_tmp_.p}})[0]);
@@ -814,7 +826,7 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
     hip_check(hipMemsetAsync(d_q_.p, 0, sizeof(double) * dofs_, stream_), "zero x");
     hip_check(hipMemcpyAsync(d_r_.p, d_b_.p, sizeof(double) * dofs_, hipMemcpyDeviceToDevice, stream_), "r=b");
     for (int it = 0; it <= refine; ++it) {
-        if (it > 0) launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, nullptr, d_r_.p);
+        if (it > 0) launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, nullptr, d_r_.p);
         if (precision == 1) {
             spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             spmv_ST<float>(y32_, xpart32_, V32_.p, d_q_.p, nullptr, d_q_.p);
@@ -993,7 +1005,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         // residual of the global system with the previous contact impulses
         {
             PhaseTimer ts(this, kFwdSolve);
-            launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
+            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
             if (K > 0) launch_add_node_force(stream_, K, d_cand_.p, d_F_.p, d_r_.p);
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
@@ -1342,7 +1354,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             hip_check(hipMemcpyAsync(d_sprev.p, d_s.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "sprev");
             kpd_apply(d_s.p, d_kz_.p, precision);
             launch_axpbypcz(stream_, n3, 1.0, d_rhs.p, 1.0, d_kz_.p, 0.0, nullptr, d_tmp_.p);
-            launch_residual(stream_, nv_, d_Ap_.p, d_Ai_.p, d_Ax_.p, d_tmp_.p, d_s.p, d_dmask_.p, d_r_.p);
+            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_tmp_.p, d_s.p, d_dmask_.p, d_r_.p);
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             if (nb > 0) {

@@ -222,8 +222,9 @@ private:
     DevBuf<int64_t> d_base_;
     DevBuf<int> d_fd_, d_perm_, d_items_, d_cu_tile_, d_cu_i0_, d_cu_i1_, d_tile_unit_ptr_, d_gu_group_, d_gu_chunk_,
         d_rbase_;
-    DevBuf<int> d_Ap_, d_Ai_;
-    DevBuf<double> d_Ax_;
+    int ell_w_ = 0;            // A in ELL form: column / value of entry j of row v at j * nv + v
+    DevBuf<int> d_Aie_;
+    DevBuf<double> d_Axe_;
     DevBuf<unsigned char> d_dmask_;
     DevBuf<int> d_ddof_;
     DevBuf<double> d_dval_, d_dval_full_;
