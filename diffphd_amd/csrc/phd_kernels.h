@@ -90,6 +90,8 @@ void launch_zero_dirichlet(hipStream_t s, int nd, const int* ddof, double* q);
 // NaN/Inf -> 0 (returned count ignored), scaling done separately
 void launch_nan_to_zero(hipStream_t s, int n, double* x);
 void launch_scale(hipStream_t s, int n, double a, double* x);
+// x *= clip / |x| if |x| = sqrt(norm2[0]) is finite and > clip (norm2 on the device)
+void launch_clip_scale(hipStream_t s, int n, double clip, const double* norm2, double* x);
 
 // ---- contact -----------------------------------------------------------------
 // normals / tangents at the frame-start positions (GetLocalFrame)

@@ -570,6 +570,14 @@ __global__ void k_nan_to_zero(int n, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && !isfinite(x[i])) x[i] = 0.0;
 }
+// x *= clip / |x| when |x| = sqrt(*norm2) is finite and above clip (the norm
+// comes from a device reduction, so no host round trip is needed).
+__global__ void k_clip_scale(int n, double clip, const double* __restrict__ norm2, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double nn = sqrt(*norm2);
+    if (isfinite(nn) && nn > clip) x[i] *= clip / nn;
+}
 __global__ void k_scale(int n, double a, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] *= a;
@@ -1154,6 +1162,9 @@ void launch_zero_dirichlet(hipStream_t s, int nd, const int* ddof, double* q) {
 }
 void launch_nan_to_zero(hipStream_t s, int n, double* x) {
     hipLaunchKernelGGL(k_nan_to_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x);
+}
+void launch_clip_scale(hipStream_t s, int n, double clip, const double* norm2, double* x) {
+    hipLaunchKernelGGL(k_clip_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, clip, norm2, x);
 }
 void launch_scale(hipStream_t s, int n, double a, double* x) {
     hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x);

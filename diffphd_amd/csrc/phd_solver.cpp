@@ -1332,8 +1332,13 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     // safe_clip (backward_gpu.cpp:809-816) on a device vector
     auto safe_clip = [&](double* x) {
         launch_nan_to_zero(stream_, n3, x);
-        const double nn = std::sqrt(dots({{x, x}})[0]);
-        if (clip > 0 && std::isfinite(nn) && nn > clip) launch_scale(stream_, n3, clip / nn, x);
+        if (clip > 0) {  // |x| reduced and applied on the device: no host round trip
+            DotArgs a;
+            a.m = 1;
+            a.a[0] = a.b[0] = x;
+            launch_dots(stream_, n3, a, d_dot_partial_.p, d_dot_out_.p);
+            launch_clip_scale(stream_, n3, clip, d_dot_out_.p, x);
+        }
     };
     // Constrained solve data (Woodbury basis); nb == 0 means the plain backbone.
     int nb = 0;
