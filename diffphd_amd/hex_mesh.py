@@ -82,3 +82,21 @@ def tet_grid(n, dx, origin):
     flip = vol < 0
     E[1, flip], E[2, flip] = E[2, flip].copy(), E[1, flip].copy()
     return V, np.ascontiguousarray(E)
+
+
+def read_tetgen_file(node_file, ele_file):
+    """Vertices (n x 3) and tets (m x 4, zero-based) from tetgen's `.node` and
+    `.ele` text files (python/py_diff_pd/common/tet_mesh.py:312-324): the first
+    line of each holds the count; each following line holds an index then the
+    3 coordinates / 4 vertex indices. As in the reference, the `.ele` indices
+    are taken as one-based; extra columns (attributes, markers) are ignored."""
+    with open(node_file, "r") as f:
+        rows = [l.split() for l in f if l.strip() and not l.lstrip().startswith("#")]
+    nv = int(rows[0][0])
+    verts = np.asarray([[float(x) for x in rows[i + 1][1:4]] for i in range(nv)], dtype=np.float64)
+    with open(ele_file, "r") as f:
+        rows = [l.split() for l in f if l.strip() and not l.lstrip().startswith("#")]
+    ne = int(rows[0][0])
+    elements = np.asarray([[int(x) for x in rows[i + 1][1:5]] for i in range(ne)], dtype=np.int64) - 1
+    return verts, elements
+

@@ -18,15 +18,22 @@ METHODS = ("pd_eigen_alg_phd", "pd_eigen_pcg_proj_gpu")
 
 
 class StdRealVector(list):
-    """std::vector<real> stand-in: a list the callee fills in place."""
+    """std::vector<real> stand-in: a list the callee fills in place. Built from
+    a size (zeros) or from a sequence, like the SWIG vector constructors."""
 
     def __init__(self, n=0):
-        super().__init__([0.0] * int(n))
+        if np.ndim(n) == 0:
+            super().__init__([0.0] * int(n))
+        else:
+            super().__init__(float(x) for x in np.asarray(n, dtype=np.float64).ravel())
 
 
 class StdIntVector(list):
     def __init__(self, n=0):
-        super().__init__([0] * int(n))
+        if np.ndim(n) == 0:
+            super().__init__([0] * int(n))
+        else:
+            super().__init__(int(x) for x in np.asarray(n).ravel())
 
 
 StdMap = dict
