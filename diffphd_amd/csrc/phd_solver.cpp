@@ -95,9 +95,45 @@ Body::Body(int device) : device_(device) {
     blas_ = h;
 }
 Body::~Body() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (pin_) (void)hipHostFree(pin_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (blas_) rocblas_destroy_handle((rocblas_handle)blas_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Body::pin_reserve(size_t bytes) {
+    if (bytes <= pin_cap_) return;
+    hip_check(hipStreamSynchronize(stream_), "pin reserve");
+    if (pin_) hip_check(hipHostFree(pin_), "hipHostFree");
+    pin_ = nullptr;
+    pin_cap_ = pin_off_ = 0;
+    hip_check(hipHostMalloc((void**)&pin_, bytes, hipHostMallocDefault), "hipHostMalloc");
+    pin_cap_ = bytes;
+}
+void Body::pin_reset() {
+    hip_check(hipStreamSynchronize(stream_), "pin reset");
+    pin_off_ = 0;
+}
+template <class T>
+T* Body::pin(size_t n) {
+    const size_t off = (pin_off_ + 255) / 256 * 256;
+    check(off + n * sizeof(T) <= pin_cap_, "pinned staging buffer exhausted");
+    pin_off_ = off + n * sizeof(T);
+    return reinterpret_cast<T*>(pin_ + off);
+}
+template <class T>
+void Body::up(T* dev, const T* host, size_t n) {
+    if (n == 0) return;
+    T* s = pin<T>(n);
+    std::memcpy(s, host, n * sizeof(T));
+    hip_check(hipMemcpyAsync(dev, s, n * sizeof(T), hipMemcpyHostToDevice, stream_), "staged upload");
+}
+template <class T>
+const T* Body::down(const T* dev, size_t n) {
+    T* s = pin<T>(n);
+    if (n) hip_check(hipMemcpyAsync(s, dev, n * sizeof(T), hipMemcpyDeviceToHost, stream_), "staged download");
+    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -794,29 +830,29 @@ void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) 
 // Returns false when a pivot was not positive.
 bool Body::solve_contact_system(int n, int precision, double* rhs) {
     const int np = chol_padded(n);
-    hip_check(hipMemcpyAsync(d_rhsK_.p, rhs, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "rhs H2D");
+    up(d_rhsK_.p, rhs, n);
     launch_clear_flag(stream_, d_info_.p);
-    int fail = 0;
+    const int* hf = nullptr;
     if (precision == 1) {
         if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
         if (d_cy32_.n < (size_t)chol_work(n)) { d_cy32_.alloc((size_t)chol_work(n)); d_cx32_.alloc(np); }
         launch_chol_assemble<float>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
         launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
-        std::vector<float> x(n);
-        hip_check(hipMemcpyAsync(x.data(), d_cx32_.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
-        hip_check(hipMemcpyAsync(&fail, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "flag D2H");
+        const float* hx = down(d_cx32_.p, n);
+        hf = down(d_info_.p, 1);
         hip_check(hipStreamSynchronize(stream_), "contact solve");
-        for (int a = 0; a < n; ++a) rhs[a] = x[a];
+        for (int a = 0; a < n; ++a) rhs[a] = hx[a];
     } else {
         if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
         if (d_cy64_.n < (size_t)chol_work(n)) { d_cy64_.alloc((size_t)chol_work(n)); d_cx64_.alloc(np); }
         launch_chol_assemble<double>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
         launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
-        hip_check(hipMemcpyAsync(rhs, d_cx64_.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream_), "sol D2H");
-        hip_check(hipMemcpyAsync(&fail, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "flag D2H");
+        const double* hx = down(d_cx64_.p, n);
+        hf = down(d_info_.p, 1);
         hip_check(hipStreamSynchronize(stream_), "contact solve");
+        std::memcpy(rhs, hx, sizeof(double) * n);
     }
-    return fail == 0;
+    return hf[0] == 0;
 }
 
 // x = A^-1 b with optional FP64 refinement (test entry point; Dirichlet rows are identity).
@@ -996,7 +1032,9 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     std::vector<double> q_frame_C(3 * K);
     for (int c = 0; c < K; ++c)
         for (int d = 0; d < 3; ++d) q_frame_C[3 * c + d] = q[3 * cand_[c] + d];
+    pin_reserve((size_t)K * 64 * sizeof(double) + (1 << 20));
     for (int iter = 0; iter < max_pd_iter; ++iter) {
+        pin_reset();
         {
             PhaseTimer tl(this, kFwdLocal);
             local_rhs(d_q_.p, inv_h2m);
@@ -1017,10 +1055,13 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
             launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
+            const double* huC = down(d_uC_.p, 3 * K);
+            const double* hdl = down(d_delta_.p, K);
+            const double* hqC = down(d_dF_.p, 3 * K);
             hip_check(hipStreamSynchronize(stream_), "contact stage");
-            d_uC_.download(uC.data(), 3 * K);
-            d_delta_.download(delta.data(), K);
-            d_dF_.download(qC.data(), 3 * K);
+            std::memcpy(uC.data(), huC, sizeof(double) * 3 * K);
+            std::memcpy(delta.data(), hdl, sizeof(double) * K);
+            std::memcpy(qC.data(), hqC, sizeof(double) * 3 * K);
             for (int i = 0; i < 3 * K; ++i) uC[i] += qC[i];
             tcs.reset();
             std::unique_ptr<PhaseTimer> tcset(new PhaseTimer(this, kFwdCSets));
@@ -1051,11 +1092,12 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             tcset.reset();
             std::unique_ptr<PhaseTimer> tcap(new PhaseTimer(this, kFwdCApply));
             if (nF > 0) {
-                hip_check(hipMemcpyAsync(d_idxP_.p, Pv.data(), sizeof(int) * nP, hipMemcpyHostToDevice, stream_), "P");
-                hip_check(hipMemcpyAsync(d_idxF_.p, Fs.data(), sizeof(int) * nF, hipMemcpyHostToDevice, stream_), "Fs");
+                up(d_idxP_.p, Pv.data(), nP);
+                up(d_idxF_.p, Fs.data(), nF);
                 launch_h_apply_idx(stream_, nP, d_idxP_.p, nF, d_idxF_.p, d_H_.p, K, d_F_.p, d_tmpK3_.p);
+                const double* h = down(d_tmpK3_.p, 3 * nP);
                 hip_check(hipStreamSynchronize(stream_), "HF_prev");
-                d_tmpK3_.download(hv.data(), 3 * nP);
+                std::memcpy(hv.data(), h, sizeof(double) * 3 * nP);
                 for (int a = 0; a < nP; ++a)
                     for (int d = 0; d < 3; ++d) uC[3 * Pv[a] + d] -= hv[3 * a + d];
             }
@@ -1073,13 +1115,14 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                         Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (fr ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
                 }
                 const int nG = (int)Gv.size();
-                hip_check(hipMemcpyAsync(d_idxJ_.p, Jv.data(), sizeof(int) * nJ, hipMemcpyHostToDevice, stream_), "J");
+                up(d_idxJ_.p, Jv.data(), nJ);
                 if (nG > 0) {
-                    d_dF_.upload(Fg);
-                    hip_check(hipMemcpyAsync(d_idxF_.p, Gv.data(), sizeof(int) * nG, hipMemcpyHostToDevice, stream_), "G");
+                    up(d_dF_.p, Fg.data(), 3 * K);
+                    up(d_idxF_.p, Gv.data(), nG);
                     launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nG, d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
+                    const double* h = down(d_tmpK3_.p, 3 * nJ);
                     hip_check(hipStreamSynchronize(stream_), "HFg");
-                    d_tmpK3_.download(hv.data(), 3 * nJ);
+                    std::memcpy(hv.data(), h, sizeof(double) * 3 * nJ);
                 } else {
                     std::fill(hv.begin(), hv.end(), 0.0);
                 }
@@ -1090,8 +1133,10 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                     for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (uC[3 * c + d] + hv[3 * a + d]);
                     rJ[a] = hvec[c] - om[c] * val;
                 }
-                d_om_.upload(om);
-                d_E_.upload(E);
+                d_om_.alloc(K);
+                d_E_.alloc(K);
+                up(d_om_.p, om.data(), K);
+                up(d_E_.p, E.data(), K);
                 std::vector<double> rJ_keep = rJ;
                 tcap.reset();
                 PhaseTimer td(this, kFwdDense);
@@ -1137,12 +1182,13 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 for (int c : Jn)
                     for (int d = 0; d < 3; ++d) Fn[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d];
                 if (!Lp_.empty() && !Jn.empty()) {
-                    d_dF_.upload(Fn);
-                    hip_check(hipMemcpyAsync(d_idxP_.p, Lp_.data(), sizeof(int) * Lp_.size(), hipMemcpyHostToDevice, stream_), "L+");
-                    hip_check(hipMemcpyAsync(d_idxF_.p, Jn.data(), sizeof(int) * Jn.size(), hipMemcpyHostToDevice, stream_), "Jn");
+                    up(d_dF_.p, Fn.data(), 3 * K);
+                    up(d_idxP_.p, Lp_.data(), Lp_.size());
+                    up(d_idxF_.p, Jn.data(), Jn.size());
                     launch_h_apply_idx(stream_, (int)Lp_.size(), d_idxP_.p, (int)Jn.size(), d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
+                    const double* h = down(d_tmpK3_.p, 3 * Lp_.size());
                     hip_check(hipStreamSynchronize(stream_), "HFn");
-                    d_tmpK3_.download(hv.data(), 3 * Lp_.size());
+                    std::memcpy(hv.data(), h, sizeof(double) * 3 * Lp_.size());
                 } else {
                     std::fill(hv.begin(), hv.end(), 0.0);
                 }
@@ -1181,8 +1227,9 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                     dF[3 * c + d] = Fnew[3 * c + d] - F[3 * c + d];
                 }
             F = Fnew;
-            d_dF_.upload(dF);
-            d_F_.upload(F);
+            d_F_.alloc(3 * K);
+            up(d_dF_.p, dF.data(), 3 * K);
+            up(d_F_.p, F.data(), 3 * K);
             if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
         }
         {
@@ -1355,7 +1402,9 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
         aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
         int it_used = 0;
         std::vector<double> uC(3 * K_), ub(nb), Fc(3 * K_);
+        if (nb > 0) pin_reserve(((size_t)6 * K_ + 2 * nb) * sizeof(double) + (1 << 20));
         for (int i = 0; i < max_it; ++i) {
+            if (nb > 0) pin_reset();
             hip_check(hipMemcpyAsync(d_sprev.p, d_s.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "sprev");
             kpd_apply(d_s.p, d_kz_.p, precision);
             launch_axpbypcz(stream_, n3, 1.0, d_rhs.p, 1.0, d_kz_.p, 0.0, nullptr, d_tmp_.p);
@@ -1364,23 +1413,25 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             if (nb > 0) {
                 if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
+                const double* hu = down(d_uC_.p, 3 * K_);
                 hip_check(hipStreamSynchronize(stream_), "basis gather");
-                d_uC_.download(uC.data(), 3 * K_);
+                std::memcpy(uC.data(), hu, sizeof(double) * 3 * K_);
                 for (int a = 0; a < nb; ++a) {
                     const int c = b_c[a];
                     ub[a] = b_dir[3 * a] * uC[3 * c] + b_dir[3 * a + 1] * uC[3 * c + 1] + b_dir[3 * a + 2] * uC[3 * c + 2];
                 }
                 PhaseTimer tbs(this, kBwdBasisSolve);
                 // G^-1 ub with the inverse formed once per frame (FP32 copy in mixed mode)
-                hip_check(hipMemcpyAsync(d_rhsK_.p, ub.data(), sizeof(double) * nb, hipMemcpyHostToDevice, stream_), "ub");
+                up(d_rhsK_.p, ub.data(), nb);
                 if (precision == 1) launch_gemv_sym<float>(stream_, nb, gld, d_Gs32.p, d_rhsK_.p, d_Gx.p);
                 else launch_gemv_sym<double>(stream_, nb, gld, d_Gs64.p, d_rhsK_.p, d_Gx.p);
-                hip_check(hipMemcpyAsync(ub.data(), d_Gx.p, sizeof(double) * nb, hipMemcpyDeviceToHost, stream_), "ub D2H");
+                const double* hg = down(d_Gx.p, nb);
                 hip_check(hipStreamSynchronize(stream_), "basis solve");
+                std::memcpy(ub.data(), hg, sizeof(double) * nb);
                 std::fill(Fc.begin(), Fc.end(), 0.0);
                 for (int a = 0; a < nb; ++a)
                     for (int d = 0; d < 3; ++d) Fc[3 * b_c[a] + d] -= ub[a] * b_dir[3 * a + d];
-                d_dF_.upload(Fc);
+                up(d_dF_.p, Fc.data(), 3 * K_);
                 if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
             }
             if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_sprev.p, d_zero.p, d_s.p);

@@ -192,6 +192,16 @@ public:
 private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
+    // Pinned staging of the per-iteration host <-> device vectors: bump-allocated
+    // from one hipHostMalloc buffer, reset (after a stream synchronization) at the
+    // top of every fixed-point iteration.
+    char* pin_ = nullptr;
+    size_t pin_cap_ = 0, pin_off_ = 0;
+    void pin_reserve(size_t bytes);
+    void pin_reset();
+    template <class T> T* pin(size_t n);
+    template <class T> void up(T* dev, const T* host, size_t n);
+    template <class T> const T* down(const T* dev, size_t n);
     // scene
     int edim_ = 8, nv_ = 0, ne_ = 0, dofs_ = 0, samples_ = 8;
     std::vector<double> X_;
