@@ -323,6 +323,23 @@ void SpmvPlan::build(const Factor& f, int unit_items, int group_chunk_w) {
     }
     rbase.assign(n + 1, 0);
     for (int k = 0; k < n; ++k) rbase[k + 1] = rbase[k] + (k / chunk_w - f.fd[k] / chunk_w + 1);
+    // Largest units first: the separators near the root cross every chunk and
+    // would otherwise be dispatched last, leaving a tail. Each slot belongs to
+    // one unit, so the order does not change any result.
+    const size_t nu = gu_group.size();
+    std::vector<int64_t> work(nu, 0);
+    for (size_t u = 0; u < nu; ++u) {
+        const int g = gu_group[u], c0 = gu_chunk[u] * chunk_w, c1 = c0 + chunk_w - 1;
+        for (int k = g * kRowGroup; k < std::min(n, (g + 1) * kRowGroup); ++k)
+            work[u] += std::max(0, std::min(k, c1) - std::max(f.fd[k], c0) + 1);
+    }
+    std::vector<size_t> ord(nu);
+    for (size_t u = 0; u < nu; ++u) ord[u] = u;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+    std::vector<int> gg(nu), gc(nu);
+    for (size_t u = 0; u < nu; ++u) { gg[u] = gu_group[ord[u]]; gc[u] = gu_chunk[ord[u]]; }
+    gu_group.swap(gg);
+    gu_chunk.swap(gc);
 }
 
 
