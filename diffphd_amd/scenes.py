@@ -159,6 +159,40 @@ def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0,
     }
 
 
+def beam(cells=(16, 16, 8), dx=0.01, E=1e5, nu=0.45, rho=1000.0, fmu=0.5, v_std=0.1, seed=0):
+    """One beam of the batched workload: a free hex block (corotated + volume)
+    resting a gap of dx/50 above a planar frictional ground z = 0 (bottom-face
+    vertices are the candidates), gravity, initial velocity ~ N(0, v_std) per
+    DOF. Loss = dot(q_T, g), g ~ N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    V, Eidx, _, _ = hex_grid(np.ones(cells), dx, (0.0, 0.0, 0.02 * dx))
+    mu, la = lame(E, nu)
+    energies = [("corotated", [2 * mu], []), ("volume", [la], [])]
+    bottom = sorted(np.flatnonzero(np.abs(V[2] - 0.02 * dx) < 1e-12).tolist())
+    q0 = V.T.ravel().copy()
+    return {
+        "name": "beam", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "energies": energies, "dirichlet": [], "boundary": ("planar", [0.0, 0.0, 1.0, 0.0], bottom),
+        "q0": q0, "v0": v_std * rng.standard_normal(q0.size), "dt": 1e-2, "frames": 50,
+        "loss_g": rng.standard_normal(q0.size),
+        "options": default_options(friction_mu=fmu, max_pd_iter=500, rel_tol=1e-9),
+    }
+
+
+def beam_batch(n=256, seed=0, dims=((8, 32), (8, 32), (8, 16))):
+    """The batched-rollout workload: `n` beams with cell counts uniform in
+    `dims`, E log-uniform in [1e4, 1e6], nu uniform in [0.3, 0.49], initial
+    velocities ~ N(0, 0.1 m/s), ground friction 0.5, dt 1e-2, 50 frames."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for b in range(n):
+        cells = tuple(int(rng.integers(lo, hi + 1)) for lo, hi in dims)
+        E = 10.0 ** rng.uniform(4.0, 6.0)
+        nu = rng.uniform(0.3, 0.49)
+        specs.append(beam(cells=cells, E=E, nu=nu, seed=int(rng.integers(1 << 30))))
+    return specs
+
+
 def tet_block(n=40, dx=None, seed=0, nu=0.4, inclusions=8, fmu=0.5):
     """Heterogeneous Neo-Hookean tet block on a ground plane: n^3 cubes split
     into 5 tets, two-phase per-element mu (seeded spherical inclusions at 100x
