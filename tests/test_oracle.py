@@ -286,3 +286,45 @@ def test_mlp_oracle_gradient_matches_fd():
     assert np.array_equal(M.bf16(np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8])), np.array([1.0, 1.0, 1.0 + 2 ** -6]))
     lq, gq = M.loss_grad(p, X, T, quant=True)
     assert abs(lq - loss) < 2e-2 * loss and np.linalg.norm(gq - g) < 5e-2 * np.linalg.norm(g)
+
+
+def test_contact_woodbury_material_gradient_matches_fd(oracle_mod):
+    """One step of a beam pressed into a frictionless planar ground (all 16
+    bottom contacts active): the oracle's dl/dmat_w, which uses the
+    Woodbury-corrected adjoint z_q (backward_gpu.cpp:923-1249), equals central
+    differences of the loss in each energy's weight; without the correction it
+    does not (factor ~2 and a sign flip on this scene). The state gradients
+    dl/dq, dl/dv come from the uncorrected backbone adjoint in the reference
+    (:878-916) and are not derivatives of the forward map under contact, so
+    they are not checked here."""
+    spec = scenes.beam(cells=(3, 3, 2), E=1e5, nu=0.4, v_std=0.0, seed=1)
+    spec["v0"] = np.zeros_like(spec["q0"])
+    spec["v0"][2::3] = -0.3
+    opt = dict(spec["options"], friction_mu=0.0, rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000, thread_ct=1)
+    g = spec["loss_g"]
+    body = oracle_body(spec)
+    _, _, inf = simulate(body, spec["dt"], 2, "pd_eigen_alg_phd", opt, spec["q0"], spec["v0"],
+                         lambda q, v: (0.0, 0 * q, 0 * v), require_grad=False)
+    q1, v1 = inf["q"][-1], inf["v"][-1]
+    assert len(inf["active_contact_indices"][-1]) == 16
+
+    def run(e=None, h=0.0, skip_wb=False):
+        s = dict(spec)
+        if e is not None:
+            en = list(s["energies"])
+            kind, p, idx = en[e]
+            en[e] = (kind, [p[0] * (1 + h)], idx)
+            s["energies"] = en
+        o = dict(opt, skip_contact_woodbury=1.0) if skip_wb else opt
+        return simulate(oracle_body(s), s["dt"], 1, "pd_eigen_alg_phd", o, q1, v1,
+                        lambda q, v: (float(g @ q), g.copy(), 0 * v), grad_clip_norm=None)
+
+    an = run()[2]["dl_dmat_w"]
+    raw = run(skip_wb=True)[2]["dl_dmat_w"]
+    for e, tol in ((0, 1e-4), (1, 2e-3)):
+        w = spec["energies"][e][1][0]
+        h = 1e-4
+        fd = (run(e, h)[0] - run(e, -h)[0]) / (2 * h * w)
+        print("energy", e, "woodbury", an[e], "fd", fd, "uncorrected", raw[e])
+        assert abs(an[e] - fd) < tol * abs(fd)
+        assert abs(raw[e] - fd) > 0.3 * abs(fd)
