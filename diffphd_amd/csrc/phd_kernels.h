@@ -98,14 +98,25 @@ void launch_clip_scale(hipStream_t s, int n, double clip, const double* norm2, d
 void launch_contact_frames(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* nrm,
                            double* t1, double* t2);
 void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta);
+// Lists (candidate paths, or rows crossed by paths) cut into pieces of at most
+// kContactPiece entries: piece q covers entries [b[q], e[q]) and the pieces of
+// list l are [ptr[l], ptr[l + 1]) (device arrays).
+constexpr int kContactPiece = 1024;
+struct ContactPieces {
+    int nlists = 0, npieces = 0;
+    const int* b = nullptr;
+    const int* e = nullptr;
+    const int* ptr = nullptr;
+};
 // u[c] = sum_p val[p] * y[d * n + row[p]] for the three axes: out[3c + d]
+// (part: 3 doubles per piece of scratch)
 template <class TV>
-void launch_contact_gather(hipStream_t s, int K, const int* cp_ptr, const int* cp_row, const TV* cp_val, const TV* y,
-                           int n, double* out);
+void launch_contact_gather(hipStream_t s, const ContactPieces& pc, const int* cp_row, const TV* cp_val, const TV* y, int n,
+                           double* part, double* out);
 // y[d * n + row] += sum over the row's contacts of val * f[3c + d]
 template <class TV>
-void launch_contact_scatter(hipStream_t s, int nrows, const int* rc_ptr, const int* rc_row, const int* rc_c,
-                            const TV* rc_val, const double* f, int n, TV* y);
+void launch_contact_scatter(hipStream_t s, const ContactPieces& pc, const int* rc_row, const int* rc_c, const TV* rc_val,
+                            const double* f, double* part, int n, TV* y);
 // add f[3c + d] at node c: r[3 node + d] += f
 void launch_add_node_force(hipStream_t s, int K, const int* nodes, const double* f, double* r);
 // M[i][j] = om_i (n_i . n_j) H[i][j] om_j + (i == j) E_i  (row-major, TM = float or double)

@@ -659,18 +659,20 @@ __global__ void k_contact_distance(int K, DevBoundary b, const int* __restrict__
     delta[c] = d;
 }
 
-// out_c = (S_C^T y)_c over the etree path of candidate c (one wavefront per
-// candidate). The paths are long (up to the etree depth), so each lane keeps
-// four index -> value chains in flight per pass.
+// out_c = (S_C^T y)_c over the etree path of candidate c. Paths run up to
+// the etree depth (~6,000 rows), so each path is cut into pieces of at most
+// kContactPiece entries: one wavefront per piece (four index -> value load
+// chains in flight per lane), then one thread per candidate adds its pieces.
 template <class TV>
-__global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const int* __restrict__ cp_row,
-                                 const TV* __restrict__ cp_val, const TV* __restrict__ y, int n, double* __restrict__ out) {
-    const int c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+__global__ void k_contact_gather(int np, const int* __restrict__ pc_b, const int* __restrict__ pc_e,
+                                 const int* __restrict__ cp_row, const TV* __restrict__ cp_val, const TV* __restrict__ y,
+                                 int n, double* __restrict__ part) {
+    const int pc = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
-    if (c >= K) return;
+    if (pc >= np) return;
     double a0 = 0, a1 = 0, a2 = 0;
-    const int end = cp_ptr[c + 1];
-    int p = cp_ptr[c] + lane;
+    const int end = pc_e[pc];
+    int p = pc_b[pc] + lane;
     for (; p + 3 * WAVE < end; p += 4 * WAVE) {
         int k[4];
         TV v[4];
@@ -692,21 +694,30 @@ __global__ void k_contact_gather(int K, const int* __restrict__ cp_ptr, const in
         a2 += v * (double)y[2 * n + k];
     }
     a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
-    if (lane == 0) { out[3 * c] = a0; out[3 * c + 1] = a1; out[3 * c + 2] = a2; }
+    if (lane == 0) { part[3 * pc] = a0; part[3 * pc + 1] = a1; part[3 * pc + 2] = a2; }
 }
 
-// y_k += (S_C f)_k for the rows crossed by candidate paths (one wavefront per
-// row, four chains in flight per lane as above).
-template <class TV>
-__global__ void k_contact_scatter(int nrows, const int* __restrict__ rc_ptr, const int* __restrict__ rc_row,
-                                  const int* __restrict__ rc_c, const TV* __restrict__ rc_val, const double* __restrict__ f,
-                                  int n, TV* __restrict__ y) {
-    const int r = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    if (r >= nrows) return;
+__global__ void k_contact_gather_fin(int K, const int* __restrict__ piece_ptr, const double* __restrict__ part,
+                                     double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
     double a0 = 0, a1 = 0, a2 = 0;
-    const int end = rc_ptr[r + 1];
-    int p = rc_ptr[r] + lane;
+    for (int q = piece_ptr[c]; q < piece_ptr[c + 1]; ++q) { a0 += part[3 * q]; a1 += part[3 * q + 1]; a2 += part[3 * q + 2]; }
+    out[3 * c] = a0; out[3 * c + 1] = a1; out[3 * c + 2] = a2;
+}
+
+// y_k += (S_C f)_k for the rows crossed by candidate paths; rows near the
+// root cross every candidate, so rows are cut into pieces as above.
+template <class TV>
+__global__ void k_contact_scatter(int np, const int* __restrict__ pc_b, const int* __restrict__ pc_e,
+                                  const int* __restrict__ rc_c, const TV* __restrict__ rc_val, const double* __restrict__ f,
+                                  double* __restrict__ part) {
+    const int pc = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (pc >= np) return;
+    double a0 = 0, a1 = 0, a2 = 0;
+    const int end = pc_e[pc];
+    int p = pc_b[pc] + lane;
     for (; p + 3 * WAVE < end; p += 4 * WAVE) {
         int cc[4];
         TV v[4];
@@ -728,12 +739,20 @@ __global__ void k_contact_scatter(int nrows, const int* __restrict__ rc_ptr, con
         a2 += v * f[3 * c + 2];
     }
     a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
-    if (lane == 0) {
-        const int k = rc_row[r];
-        y[k] += (TV)a0;
-        y[n + k] += (TV)a1;
-        y[2 * n + k] += (TV)a2;
-    }
+    if (lane == 0) { part[3 * pc] = a0; part[3 * pc + 1] = a1; part[3 * pc + 2] = a2; }
+}
+
+template <class TV>
+__global__ void k_contact_scatter_fin(int nrows, const int* __restrict__ piece_ptr, const int* __restrict__ rc_row,
+                                      const double* __restrict__ part, int n, TV* __restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int q = piece_ptr[r]; q < piece_ptr[r + 1]; ++q) { a0 += part[3 * q]; a1 += part[3 * q + 1]; a2 += part[3 * q + 2]; }
+    const int k = rc_row[r];
+    y[k] += (TV)a0;
+    y[n + k] += (TV)a1;
+    y[2 * n + k] += (TV)a2;
 }
 
 __global__ void k_add_node_force(int K, const int* __restrict__ nodes, const double* __restrict__ f, double* __restrict__ r) {
@@ -1115,9 +1134,10 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
     template void launch_spmv_cols_fin<TV>(hipStream_t, int, const int*, const TV*, const int*, const double*,          \
                                            const unsigned char*, const double*, double*);                                \
     template void launch_perm_in<TV>(hipStream_t, int, const int*, const double*, const unsigned char*, TV*);           \
-    template void launch_contact_gather<TV>(hipStream_t, int, const int*, const int*, const TV*, const TV*, int, double*); \
-    template void launch_contact_scatter<TV>(hipStream_t, int, const int*, const int*, const int*, const TV*,           \
-                                             const double*, int, TV*);
+    template void launch_contact_gather<TV>(hipStream_t, const ContactPieces&, const int*, const TV*, const TV*, int,      \
+                                            double*, double*);                                                         \
+    template void launch_contact_scatter<TV>(hipStream_t, const ContactPieces&, const int*, const int*, const TV*,          \
+                                             const double*, double*, int, TV*);
 
 void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out) {
     if (m.edim == 8)
@@ -1179,17 +1199,22 @@ void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const i
     hipLaunchKernelGGL(k_contact_distance, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, b, nodes, q, delta);
 }
 template <class TV>
-void launch_contact_gather(hipStream_t s, int K, const int* cp_ptr, const int* cp_row, const TV* cp_val, const TV* y,
-                           int n, double* out) {
-    if (K == 0) return;
-    hipLaunchKernelGGL(k_contact_gather<TV>, dim3(blocks_for(K, 4)), dim3(256), 0, s, K, cp_ptr, cp_row, cp_val, y, n, out);
+void launch_contact_gather(hipStream_t s, const ContactPieces& pc, const int* cp_row, const TV* cp_val, const TV* y, int n,
+                           double* part, double* out) {
+    if (pc.nlists == 0) return;
+    hipLaunchKernelGGL(k_contact_gather<TV>, dim3(blocks_for(pc.npieces, 4)), dim3(256), 0, s, pc.npieces, pc.b, pc.e, cp_row,
+                       cp_val, y, n, part);
+    hipLaunchKernelGGL(k_contact_gather_fin, dim3(blocks_for(pc.nlists, 256)), dim3(256), 0, s, pc.nlists, pc.ptr,
+                       (const double*)part, out);
 }
 template <class TV>
-void launch_contact_scatter(hipStream_t s, int nrows, const int* rc_ptr, const int* rc_row, const int* rc_c,
-                            const TV* rc_val, const double* f, int n, TV* y) {
-    if (nrows == 0) return;
-    hipLaunchKernelGGL(k_contact_scatter<TV>, dim3(blocks_for(nrows, 4)), dim3(256), 0, s, nrows, rc_ptr, rc_row, rc_c,
-                       rc_val, f, n, y);
+void launch_contact_scatter(hipStream_t s, const ContactPieces& pc, const int* rc_row, const int* rc_c, const TV* rc_val,
+                            const double* f, double* part, int n, TV* y) {
+    if (pc.nlists == 0) return;
+    hipLaunchKernelGGL(k_contact_scatter<TV>, dim3(blocks_for(pc.npieces, 4)), dim3(256), 0, s, pc.npieces, pc.b, pc.e, rc_c,
+                       rc_val, f, part);
+    hipLaunchKernelGGL(k_contact_scatter_fin<TV>, dim3(blocks_for(pc.nlists, 256)), dim3(256), 0, s, pc.nlists, pc.ptr,
+                       rc_row, (const double*)part, n, y);
 }
 PHD_INST_TV(float)
 PHD_INST_TV(double)

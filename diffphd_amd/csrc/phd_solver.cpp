@@ -600,6 +600,29 @@ void Body::setup_contact(int threads) {
     d_rc_row_.upload(rc_rowk);
     d_rc_c_.upload(rc_c);
     rc_nrows_ = R;
+    // both list structures cut into pieces of at most kContactPiece entries
+    auto cut = [](const std::vector<int>& ptr, DevBuf<int>& db, DevBuf<int>& de, DevBuf<int>& dp, ContactPieces& pc) {
+        const int L = (int)ptr.size() - 1;
+        std::vector<int> b, e, pp(L + 1, 0);
+        for (int l = 0; l < L; ++l) {
+            for (int s0 = ptr[l]; s0 < ptr[l + 1]; s0 += kContactPiece) {
+                b.push_back(s0);
+                e.push_back(std::min(ptr[l + 1], s0 + kContactPiece));
+            }
+            pp[l + 1] = (int)b.size();
+        }
+        db.upload(b);
+        de.upload(e);
+        dp.upload(pp);
+        pc.nlists = L;
+        pc.npieces = (int)b.size();
+        pc.b = db.p;
+        pc.e = de.p;
+        pc.ptr = dp.p;
+    };
+    cut(cp_ptr, d_gp_b_, d_gp_e_, d_gp_ptr_, gather_pieces_);
+    cut(rc_ptr, d_sp_b_, d_sp_e_, d_sp_ptr_, scatter_pieces_);
+    d_piece_part_.alloc((size_t)3 * std::max(gather_pieces_.npieces, scatter_pieces_.npieces) + 3);
     // H = S_C^T S_C via a dense S_C (R x K) and rocBLAS dsyrk
     {
         DevBuf<double> D;
@@ -677,12 +700,12 @@ void Body::spmv_ST(DevBuf<TV>& y, DevBuf<TV>& xpart, const TV* V, const double* 
 template <class TV>
 void Body::contact_gather(DevBuf<TV>& y, double* out) {
     const TV* val = std::is_same<TV, float>::value ? (const TV*)d_cp_val32_.p : (const TV*)d_cp_val64_.p;
-    launch_contact_gather<TV>(stream_, K_, d_cp_ptr_.p, d_cp_row_.p, val, y.p, nv_, out);
+    launch_contact_gather<TV>(stream_, gather_pieces_, d_cp_row_.p, val, y.p, nv_, d_piece_part_.p, out);
 }
 template <class TV>
 void Body::contact_scatter(const double* f, DevBuf<TV>& y) {
     const TV* val = std::is_same<TV, float>::value ? (const TV*)d_rc_val32_.p : (const TV*)d_rc_val64_.p;
-    launch_contact_scatter<TV>(stream_, rc_nrows_, d_rc_ptr_.p, d_rc_row_.p, d_rc_c_.p, val, f, nv_, y.p);
+    launch_contact_scatter<TV>(stream_, scatter_pieces_, d_rc_row_.p, d_rc_c_.p, val, f, d_piece_part_.p, nv_, y.p);
 }
 
 void Body::local_rhs(const double* q, double inv_h2m) {

@@ -268,6 +268,9 @@ private:
     DevBuf<double> d_cp_val64_, d_rc_val64_;
     DevBuf<float> d_cp_val32_, d_rc_val32_;
     int rc_nrows_ = 0;
+    ContactPieces gather_pieces_, scatter_pieces_;
+    DevBuf<int> d_gp_b_, d_gp_e_, d_gp_ptr_, d_sp_b_, d_sp_e_, d_sp_ptr_;
+    DevBuf<double> d_piece_part_;
     DevBuf<double> d_waves_;
     DevBuf<double> d_nrm_, d_t1_, d_t2_, d_delta_, d_F_, d_dF_, d_uC_, d_tmpK3_, d_om_, d_E_;
     DevBuf<double> d_C64_, d_cy64_, d_cx64_;  // dense contact system (blocked Cholesky layout)
