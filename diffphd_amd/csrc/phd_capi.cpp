@@ -135,9 +135,11 @@ int phd_last_backward_iters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->
 int phd_factor_info(phd_ctx ctx, double* out, int cap) {
     return guard(ctx, [&](phd::Body& b) {
         const phd::Factor& f = b.factor();
-        const double v[10] = {(double)f.nnz, (double)f.v_len, (double)f.max_depth, f.order_seconds, f.factor_seconds,
-                              b.setup_seconds, 0, 0, (double)f.Lx.size(), 0};
-        for (int i = 0; i < cap && i < 10; ++i) out[i] = v[i];
+        const double v[12] = {(double)f.nnz, (double)f.v_len, (double)f.max_depth, f.order_seconds, f.factor_seconds,
+                              b.setup_seconds, (double)b.plan().gu_group.size(), (double)b.plan().cu_tile.size(),
+                              (double)f.Lx.size(), (double)b.contact_candidates(), (double)b.contact_entries(),
+                              (double)b.contact_rows()};
+        for (int i = 0; i < cap && i < 12; ++i) out[i] = v[i];
     });
 }
 int phd_timing_enable(phd_ctx ctx, int enable) {

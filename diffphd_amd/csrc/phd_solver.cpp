@@ -564,6 +564,7 @@ void Body::setup_contact(int threads) {
         cp_ptr[c + 1] = (int)cp_row.size();
     }
     const int64_t m = (int64_t)cp_row.size();
+    contact_entries_ = m;
     DevBuf<int> drow, dcol;
     drow.upload(cp_row);
     dcol.upload(cp_col);

@@ -178,6 +178,10 @@ public:
     hipStream_t stream() const { return stream_; }
     void collect_timing();
     const Factor& factor() const { return factor_; }
+    const SpmvPlan& plan() const { return plan_; }
+    int contact_candidates() const { return K_; }
+    int64_t contact_entries() const { return contact_entries_; }  // entries of the candidate paths of S
+    int contact_rows() const { return rc_nrows_; }
 
     // ---- learned warm start (option phd_warm_start = 1) ----
     WarmStartNet& net();
@@ -268,6 +272,7 @@ private:
     DevBuf<double> d_cp_val64_, d_rc_val64_;
     DevBuf<float> d_cp_val32_, d_rc_val32_;
     int rc_nrows_ = 0;
+    int64_t contact_entries_ = 0;
     ContactPieces gather_pieces_, scatter_pieces_;
     DevBuf<int> d_gp_b_, d_gp_e_, d_gp_ptr_, d_sp_b_, d_sp_e_, d_sp_ptr_;
     DevBuf<double> d_piece_part_;

@@ -246,9 +246,10 @@ class _Deformable:
         return x
 
     def factor_info(self):
-        out, op = _lib.dptr(np.zeros(10))
-        self._ok(_lib.lib().phd_factor_info(self._ctx, op, 10))
-        keys = ["nnz_S", "v_len", "max_depth", "order_s", "factor_s", "setup_s", "row_units", "col_units", "nnz_L", "K"]
+        out, op = _lib.dptr(np.zeros(12))
+        self._ok(_lib.lib().phd_factor_info(self._ctx, op, 12))
+        keys = ["nnz_S", "v_len", "max_depth", "order_s", "factor_s", "setup_s", "row_units", "col_units", "nnz_L", "K",
+                "contact_entries", "contact_rows"]
         return dict(zip(keys, out.tolist()))
 
     def frame_stats(self):

@@ -95,7 +95,8 @@ void phd_reset_frame_stats(phd_ctx ctx);
 int phd_last_backward_iters(phd_ctx ctx);
 /* Factor description: out[0] nnz(S), [1] padded value count, [2] max etree depth,
  * [3] ordering seconds, [4] factor seconds, [5] setup seconds, [6] row units,
- * [7] column units, [8] nnz(L), [9] contact candidates K. */
+ * [7] column units, [8] nnz(L), [9] contact candidates K, [10] entries of the
+ * candidate paths of S, [11] rows crossed by those paths (up to `cap` values). */
 int phd_factor_info(phd_ctx ctx, double* out, int cap);
 /* Device-event timing of the S (rows) and S^T (columns) kernels: enable = 1 starts
  * recording; phd_timing_read writes [rows_ms, rows_launches, cols_ms, cols_launches]
