@@ -802,9 +802,11 @@ struct KnPtrs { const double* kn[4]; };
 template <int EDIM, class TW>
 __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* __restrict__ kc, const double* __restrict__ kv,
                                                   KnPtrs kn, const double* __restrict__ q, TW* __restrict__ W) {
+    // one thread per (sample, element), elements fastest: the writes of W
+    // (element-minor: entry j of sample s at (81 s + j) ne + e) are coalesced
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (int64_t)m.ne * m.samples) return;
-    const int e = (int)(gid / m.samples), s = (int)(gid % m.samples);
+    const int e = (int)(gid % m.ne), s = (int)(gid / m.ne);
     const double* G = m.G + (int64_t)e * m.g_stride + s * EDIM * 3;
     double F[9];
 #pragma unroll
@@ -829,7 +831,7 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
     if (wv != 0.0) volume_D(sg, D);
     for (int h = 0; h < m.n_nh; ++h)
         if (kn.kn[h] && kn.kn[h][e] != 0.0) nh_prox(m.nh_mu[h], m.nh_la[h], m.nh_k[h], sg, ssn[h]);
-    TW* out = W + gid * 81;
+    TW* out = W + (int64_t)s * 81 * m.ne + e;
     for (int c = 0; c < 9; ++c) {
         double dF[9];
 #pragma unroll
@@ -858,7 +860,7 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
             for (int i = 0; i < 9; ++i) col[i] += w * dP[i];
         }
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i + 9 * c] = (TW)col[i];
+        for (int i = 0; i < 9; ++i) out[(int64_t)(i + 9 * c) * m.ne] = (TW)col[i];
     }
 }
 
@@ -889,13 +891,13 @@ __global__ void __launch_bounds__(128) k_kpd_apply(DevMaterials m, const TW* __r
                 dF[1 + 3 * jj] += x[k][1] * g;
                 dF[2 + 3 * jj] += x[k][2] * g;
             }
-        const TW* Ws = W + ((int64_t)e * m.samples + s) * 81;
+        const TW* Ws = W + (int64_t)s * 81 * m.ne + e;  // element-minor: coalesced across the wave
         double y[9];
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             double a = 0;
 #pragma unroll
-            for (int c = 0; c < 9; ++c) a += (double)Ws[r + 9 * c] * dF[c];
+            for (int c = 0; c < 9; ++c) a += (double)Ws[(int64_t)(r + 9 * c) * m.ne] * dF[c];
             y[r] = a;
         }
 #pragma unroll
