@@ -288,43 +288,71 @@ __global__ void k_chol_gety(int np, int64_t lda, const TV* __restrict__ M, TV* _
     if (c < np) y[c] = M[np + (int64_t)c * lda];
 }
 
-// Backward substitution step b with Xinv_b = L_bb^-T (column-major): every
-// workgroup forms x_b = Xinv_b y_b; workgroup b stores it, workgroup a < b
-// updates y_a -= L_ba^T x_b.
+// Backward substitution over the group of blocks b0 .. b0 + 3 (those below T),
+// with Xinv_b = L_bb^-T (column-major). Wave w owns block b0 + w. Every
+// workgroup solves the group: from the top block down, x_i = Xinv_i y_i, then
+// y_w -= L_{i,w}^T x_i for the group's blocks w < i. Workgroup b0 stores the
+// group's x; workgroup a < b0 updates y_a -= sum_i L_{b0+i,a}^T x_i. Four
+// blocks per launch cut the number of dependent launches by four.
+constexpr int kBsGroup = 4;
+
 template <class TV>
-__global__ void __launch_bounds__(64) k_chol_bsolve(int b, int64_t lda, const TV* __restrict__ M,
-                                                    const TV* __restrict__ Xinv, TV* __restrict__ y, TV* __restrict__ x) {
-    __shared__ TV Ls[CB][CB + 1];  // Ls[l][cc] = L(64 b + l, 64 a + cc)
-    __shared__ TV ys[CB], xs[CB];
-    const int r = threadIdx.x;
+__global__ void __launch_bounds__(256) k_chol_bsolve(int b0, int T, int64_t lda, const TV* __restrict__ M,
+                                                     const TV* __restrict__ Xinv, TV* __restrict__ y, TV* __restrict__ x) {
+    __shared__ TV ys[kBsGroup][CB];
+    __shared__ TV xs[kBsGroup][CB];
+    __shared__ TV part[kBsGroup][CB];
+    const int t = threadIdx.x, w = t / CB, r = t % CB;
     const int a = blockIdx.x;
-    const int64_t o = (int64_t)b * CB, oa = (int64_t)a * CB;
-    const TV* Xb = Xinv + (int64_t)b * CB * CB;
-    TV xcol[CB];
+    const int G = min(kBsGroup, T - b0);
+    const int64_t bw = (int64_t)b0 + w;
+    const bool valid = w < G;
+    TV X[CB];  // X[c] = Xinv_bw(r, c)
+    if (valid) {
+        const TV* Xb = Xinv + bw * CB * CB;
 #pragma unroll
-    for (int c = 0; c < CB; ++c) xcol[c] = Xb[c * CB + r];
-    if (a < b) {
-        TV lv[CB];
-#pragma unroll
-        for (int cc = 0; cc < CB; ++cc) lv[cc] = M[o + r + (oa + cc) * lda];
-#pragma unroll
-        for (int cc = 0; cc < CB; ++cc) Ls[r][cc] = lv[cc];
+        for (int c = 0; c < CB; ++c) X[c] = Xb[c * CB + r];
+        ys[w][r] = y[bw * CB + r];
     }
-    ys[r] = y[o + r];
     __syncthreads();
-    TV xb = 0;
+    for (int i = G - 1; i >= 0; --i) {
+        TV Lv[CB];  // Lv[l] = L(64 (b0 + i) + l, 64 (b0 + w) + r)
+        if (w < i) {
+            const TV* Lc = M + ((int64_t)b0 + i) * CB + (bw * CB + r) * lda;
 #pragma unroll
-    for (int c = 0; c < CB; ++c) xb += xcol[c] * ys[c];
-    if (a == b) {
-        x[o + r] = xb;
+            for (int l = 0; l < CB; ++l) Lv[l] = Lc[l];
+        }
+        if (w == i) {
+            TV s = 0;
+#pragma unroll
+            for (int c = 0; c < CB; ++c) s += X[c] * ys[i][c];
+            xs[i][r] = s;
+        }
+        __syncthreads();
+        if (w < i) {
+            TV s = 0;
+#pragma unroll
+            for (int l = 0; l < CB; ++l) s += Lv[l] * xs[i][l];
+            ys[w][r] -= s;
+        }
+        __syncthreads();
+    }
+    if (a == b0) {
+        if (valid) x[bw * CB + r] = xs[w][r];
         return;
     }
-    xs[r] = xb;
-    __syncthreads();
-    TV acc = 0;
+    TV s = 0;
+    if (valid) {
+        const TV* Lc = M + bw * CB + ((int64_t)a * CB + r) * lda;
+        TV U[CB];
 #pragma unroll
-    for (int l = 0; l < CB; ++l) acc += Ls[l][r] * xs[l];
-    y[oa + r] -= acc;
+        for (int l = 0; l < CB; ++l) U[l] = Lc[l];
+#pragma unroll
+        for (int l = 0; l < CB; ++l) s += U[l] * xs[w][l];
+    }
+    part[w][r] = s;
+    __syncthreads();
+    if (w == 0) y[(int64_t)a * CB + r] -= (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
 }
 
 __global__ void k_clear_flag(int* f) {
@@ -356,8 +384,9 @@ void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail) 
         hipLaunchKernelGGL(k_chol_step<TV>, dim3(T - k + 1 + nu), dim3(256), 0, s, k, T, lda, M, Xinv, fail);
     }
     hipLaunchKernelGGL(k_chol_gety<TV>, dim3((np + 255) / 256), dim3(256), 0, s, np, lda, (const TV*)M, y);
-    for (int b = T - 1; b >= 0; --b)
-        hipLaunchKernelGGL(k_chol_bsolve<TV>, dim3(b + 1), dim3(64), 0, s, b, lda, (const TV*)M, (const TV*)Xinv, y, x);
+    for (int b0 = kBsGroup * ((T - 1) / kBsGroup); b0 >= 0; b0 -= kBsGroup)
+        hipLaunchKernelGGL(k_chol_bsolve<TV>, dim3(b0 + 1), dim3(256), 0, s, b0, T, lda, (const TV*)M, (const TV*)Xinv, y,
+                           x);
 }
 
 void launch_clear_flag(hipStream_t s, int* flag) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, flag); }
