@@ -3,6 +3,7 @@
 #include "phd_dense.h"
 
 #include <cstring>
+#include <type_traits>
 
 namespace phd {
 namespace {
@@ -175,6 +176,31 @@ __device__ __forceinline__ void tile_product(const TV (*A)[kTS], const TV (*B)[k
     }
 }
 
+// FP32 rank-64 tile product on the matrix cores: C(row, col) = sum_l
+// A[l][row] B[l][col], wave w owning rows 32 (w / 2) .. and columns
+// 32 (w % 2) .. of the 64 x 64 tile. v_mfma_f32_32x32x2f32 is fed the
+// transposed product D = C^T block (operand A lane i + 32 h = B[l][32 wj + i],
+// operand B lane j + 32 h = A[l][32 wi + j], l = 2 s + h at step s), so that
+// result register v of lane L holds C(32 wi + (L % 32), 32 wj + mfma_col(v, L)):
+// the 32 lanes of a half-wave hold 32 consecutive rows of one column of the
+// column-major matrix.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int mfma_col(int v, int lane) { return (v & 3) + 4 * (lane >> 5) + 8 * (v >> 2); }
+
+__device__ __forceinline__ f32x16 tile_product_mfma(const float (*A)[kTS], const float (*B)[kTS], int wave, int lane) {
+    const int wi = wave >> 1, wj = wave & 1, i = lane & 31, h = lane >> 5;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll 8
+    for (int st = 0; st < CB / 2; ++st) {
+        const int l = 2 * st + h;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(B[l][32 * wj + i], A[l][32 * wi + i], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // Rank-64 update of step kp on block (ib, jb): A_ij -= L_{i,kp} L_{j,kp}^T
 // (256 threads; the staging tiles are S0 and S1).
 template <class TV>
@@ -184,11 +210,19 @@ __device__ __forceinline__ void update_block(TV* __restrict__ M, int64_t lda, in
     stage_block<TV>(M, lda, (int64_t)ib * CB, (int64_t)kp * CB, S0);
     stage_block<TV>(M, lda, (int64_t)jb * CB, (int64_t)kp * CB, S1);
     __syncthreads();
-    TV acc[16];
-    tile_product<TV>(S0, S1, tx, ty, acc);
-    TV* dst = M + (int64_t)ib * CB + tx + ((int64_t)jb * CB + 16 * ty) * lda;
+    if constexpr (std::is_same<TV, float>::value) {
+        const int lane = t % 64, wave = t / 64;
+        const f32x16 acc = tile_product_mfma(S0, S1, wave, lane);
+        float* dst = M + (int64_t)ib * CB + 32 * (wave >> 1) + (lane & 31) + ((int64_t)jb * CB + 32 * (wave & 1)) * lda;
 #pragma unroll
-    for (int p = 0; p < 16; ++p) dst[p * lda] -= acc[p];
+        for (int v = 0; v < 16; ++v) dst[mfma_col(v, lane) * lda] -= acc[v];
+    } else {
+        TV acc[16];
+        tile_product<TV>(S0, S1, tx, ty, acc);
+        TV* dst = M + (int64_t)ib * CB + tx + ((int64_t)jb * CB + 16 * ty) * lda;
+#pragma unroll
+        for (int p = 0; p < 16; ++p) dst[p * lda] -= acc[p];
+    }
 }
 
 // Step k of the factorization, one launch of 256-thread workgroups in two parts
@@ -231,21 +265,42 @@ __global__ void __launch_bounds__(256) k_chol_step(int k, int T, int64_t lda, TV
         stage_block<TV>(M, lda, c0, (int64_t)kp * CB, S0);
         if (has_row) stage_block<TV>(M, lda, i0, (int64_t)kp * CB, S1);
         __syncthreads();
-        TV accD[16], accR[16];
-        tile_product<TV>(S0, S0, tx, ty, accD);
-        if (has_row) tile_product<TV>(S1, S0, tx, ty, accR);
-        TV aD[16], aR[16];
+        if constexpr (std::is_same<TV, float>::value) {
+            const f32x16 accD = tile_product_mfma(S0, S0, wave, r);
+            f32x16 accR;
+            if (has_row) accR = tile_product_mfma(S1, S0, wave, r);
+            const int row = 32 * (wave >> 1) + (r & 31);
+            float aD[16], aR[16];
 #pragma unroll
-        for (int p = 0; p < 16; ++p) {
-            const int64_t off = tx + (c0 + 16 * ty + p) * lda;
-            aD[p] = M[c0 + off];
-            aR[p] = has_row ? M[i0 + off] : TV(0);
-        }
-        __syncthreads();
+            for (int v = 0; v < 16; ++v) {
+                const int64_t off = row + (c0 + 32 * (wave & 1) + mfma_col(v, r)) * lda;
+                aD[v] = M[c0 + off];
+                aR[v] = has_row ? M[i0 + off] : 0.f;
+            }
+            __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 16; ++p) {
-            S0[tx][16 * ty + p] = aD[p] - accD[p];
-            if (has_row) S1[tx][16 * ty + p] = aR[p] - accR[p];
+            for (int v = 0; v < 16; ++v) {
+                const int col = 32 * (wave & 1) + mfma_col(v, r);
+                S0[row][col] = aD[v] - accD[v];
+                if (has_row) S1[row][col] = aR[v] - accR[v];
+            }
+        } else {
+            TV accD[16], accR[16];
+            tile_product<TV>(S0, S0, tx, ty, accD);
+            if (has_row) tile_product<TV>(S1, S0, tx, ty, accR);
+            TV aD[16], aR[16];
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const int64_t off = tx + (c0 + 16 * ty + p) * lda;
+                aD[p] = M[c0 + off];
+                aR[p] = has_row ? M[i0 + off] : TV(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                S0[tx][16 * ty + p] = aD[p] - accD[p];
+                if (has_row) S1[tx][16 * ty + p] = aR[p] - accR[p];
+            }
         }
         __syncthreads();
         if (role == 0) {
