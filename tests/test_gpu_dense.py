@@ -16,7 +16,10 @@ def _solve(A, b, precision):
     return rc, x_arr.copy()
 
 
-@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 700])
+# The back substitution takes four 64-row blocks per launch from the top:
+# 65, 130 and 700 leave a partial top group of 2, 3 and 3 blocks, 260 and 1030
+# a top group of one block, 256 and 1024 only full groups.
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 256, 260, 700, 1024, 1030])
 def test_dense_spd_solve_matches_numpy(n):
     rng = np.random.default_rng(n)
     B = rng.standard_normal((n, n))
