@@ -153,13 +153,23 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
         }
     __syncthreads();
     const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
-    for (int rr = wv; rr < 64; rr += 4) {
-        const int k = 64 * g + rr;
+    // the metadata of this wave's 16 rows (64 g + wv + 4 l), one row per lane,
+    // loaded together so that each row waits on one memory round trip
+    int mf = 0, mrb = 0;
+    int64_t mb = 0;
+    {
+        const int kk = 64 * g + wv + 4 * lane;
+        if (lane < 16 && kk < n) { mf = fd[kk]; mb = base[kk]; mrb = rbase[kk]; }
+    }
+    for (int l = 0; l < 16; ++l) {
+        const int k = 64 * g + wv + 4 * l;
         if (k >= n) break;
-        const int f = fd[k];
+        const int f = __builtin_amdgcn_readlane(mf, l);
         const int lo = max(f, c0), hi = min(k, c1);
         if (lo > hi) continue;
-        const TV* vr = V + base[k];
+        const int64_t bk = ((int64_t)__builtin_amdgcn_readlane((int)(mb >> 32), l) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)mb, l);
+        const TV* vr = V + bk;
         const int cs = (lo & ~(VN - 1)) + VN * lane;
         TV v[U][VN];
 #pragma unroll
@@ -192,7 +202,7 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
         a1 = wave_sum(a1);
         a2 = wave_sum(a2);
         if (lane == 0) {
-            TV* p = pslot + 3 * (int64_t)(rbase[k] + ch - f / GRP_CW);
+            TV* p = pslot + 3 * (int64_t)(__builtin_amdgcn_readlane(mrb, l) + ch - f / GRP_CW);
             p[0] = a0;
             p[1] = a1;
             p[2] = a2;
