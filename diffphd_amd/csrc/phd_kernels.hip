@@ -161,17 +161,22 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
         const int kk = 64 * g + wv + 4 * lane;
         if (lane < 16 && kk < n) { mf = fd[kk]; mb = base[kk]; mrb = rbase[kk]; }
     }
-    for (int l = 0; l < 16; ++l) {
-        const int k = 64 * g + wv + 4 * l;
-        if (k >= n) break;
-        const int f = __builtin_amdgcn_readlane(mf, l);
-        const int lo = max(f, c0), hi = min(k, c1);
-        if (lo > hi) continue;
+    // row l's values are loaded one row ahead of its FMAs and reduction
+    const int k0 = 64 * g + wv;
+    const int nr = k0 >= n ? 0 : min(16, (n - k0 + 3) / 4);
+    auto bounds = [&](int l, int& f, int& lo, int& hi) {
+        f = __builtin_amdgcn_readlane(mf, l);
+        lo = max(f, c0);
+        hi = min(k0 + 4 * l, c1);
+    };
+    auto load = [&](int l, TV (&v)[U][VN]) {
+        int f, lo, hi;
+        bounds(l, f, lo, hi);
+        if (lo > hi) return;
         const int64_t bk = ((int64_t)__builtin_amdgcn_readlane((int)(mb >> 32), l) << 32) |
                            (uint32_t)__builtin_amdgcn_readlane((int)mb, l);
         const TV* vr = V + bk;
         const int cs = (lo & ~(VN - 1)) + VN * lane;
-        TV v[U][VN];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int c = cs + q * VN * WAVE;
@@ -180,6 +185,12 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
             else memset(&t, 0, sizeof(t));
             memcpy(v[q], &t, sizeof(t));
         }
+    };
+    auto finish = [&](int l, const TV (&v)[U][VN]) {
+        int f, lo, hi;
+        bounds(l, f, lo, hi);
+        if (lo > hi) return;
+        const int cs = (lo & ~(VN - 1)) + VN * lane;
         TV a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
         for (int q = 0; q < U; ++q) {
@@ -207,6 +218,16 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
             p[1] = a1;
             p[2] = a2;
         }
+    };
+    TV va[U][VN], vb[U][VN];
+    if (nr > 0) load(0, va);
+#pragma unroll 1
+    for (int l = 0; l < nr; l += 2) {
+        if (l + 1 < nr) load(l + 1, vb);
+        finish(l, va);
+        if (l + 1 >= nr) break;
+        if (l + 2 < nr) load(l + 2, va);
+        finish(l + 1, vb);
     }
 }
 
