@@ -133,6 +133,11 @@ void launch_symmetrize_lower(hipStream_t s, int K, double* H);
 
 void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
+// ub[a] = dirs_a . u[cidx[a]] (3-vectors per candidate)
+void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, double* ub);
+// F[3c + d] = -sum over the basis rows a of candidate c (clist[cptr[c] .. cptr[c+1]), increasing a) of g[a] dirs_a[d]
+void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist, const double* dirs, const double* g,
+                         double* F);
 // Gs (ld x ld, column-major, ld a multiple of 64, zero past n) = full symmetric
 // matrix from the lower triangle of G (n x n, column-major).
 template <class TV>

@@ -1065,6 +1065,35 @@ __global__ void k_assemble_g(int nb, const int* __restrict__ cidx, const double*
     G[idx] = dd * H[(int64_t)cidx[a] * K + cidx[b]];
 }
 
+// ub[a] = d_a . u[c_a]: the candidate vectors projected on the basis rows,
+// J_basis u (backward_gpu.cpp:998-1024 projects the same way, on the host).
+__global__ void k_basis_project(int nb, const int* __restrict__ cidx, const double* __restrict__ dirs,
+                                const double* __restrict__ u, double* __restrict__ ub) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nb) return;
+    const int c = cidx[a];
+    ub[a] = dirs[3 * a] * u[3 * c] + dirs[3 * a + 1] * u[3 * c + 1] + dirs[3 * a + 2] * u[3 * c + 2];
+}
+
+// F[3c + d] = -sum_a g[a] d_a[d] over the basis rows a of candidate c, in
+// increasing a (clist holds them grouped by candidate, cptr the ranges):
+// -J_basis^T g, the correction of backward_gpu.cpp:924.
+__global__ void k_basis_expand(int K, const int* __restrict__ cptr, const int* __restrict__ clist,
+                               const double* __restrict__ dirs, const double* __restrict__ g, double* __restrict__ F) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int p = cptr[c]; p < cptr[c + 1]; ++p) {
+        const int a = clist[p];
+        f0 -= g[a] * dirs[3 * a];
+        f1 -= g[a] * dirs[3 * a + 1];
+        f2 -= g[a] * dirs[3 * a + 2];
+    }
+    F[3 * c] = f0;
+    F[3 * c + 1] = f1;
+    F[3 * c + 2] = f2;
+}
+
 // out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]: the candidate block
 // of A^-1 applied to impulses supported on `cols` (one wavefront per row).
 __global__ void k_h_apply_idx(int nr, const int* __restrict__ rows, int nc, const int* __restrict__ cols,
@@ -1346,6 +1375,15 @@ template void launch_gemv_sym<double>(hipStream_t, int, int64_t, const double*, 
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G) {
     if (nb == 0) return;
     hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
+}
+void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, double* ub) {
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_basis_project, dim3(blocks_for(nb, 256)), dim3(256), 0, s, nb, cidx, dirs, u, ub);
+}
+void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist, const double* dirs, const double* g,
+                         double* F) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_basis_expand, dim3(blocks_for(K, 256)), dim3(256), 0, s, K, cptr, clist, dirs, g, F);
 }
 void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
                         const double* f, double* out) {

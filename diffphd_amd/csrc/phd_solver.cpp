@@ -1416,7 +1416,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     DevBuf<double> d_G, d_Gx, d_dirs, d_Gs64;
     DevBuf<float> d_Gs32;  // G^-1 as a full symmetric matrix (leading dimension gld)
     int64_t gld = 0;
-    DevBuf<int> d_bcidx;
+    DevBuf<int> d_bcidx, d_bcptr, d_bclist;  // basis row -> candidate; candidate -> its basis rows
     std::vector<int> b_c;            // candidate index of each basis row
     std::vector<double> b_dir;       // 3 per basis row
     // one fixed point  s <- A_c^-1 (rhs + K s)  in residual form with AA
@@ -1425,10 +1425,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
         DevAnderson aa;
         aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
         int it_used = 0;
-        std::vector<double> uC(3 * K_), ub(nb), Fc(3 * K_);
-        if (nb > 0) pin_reserve(((size_t)6 * K_ + 2 * nb) * sizeof(double) + (1 << 20));
         for (int i = 0; i < max_it; ++i) {
-            if (nb > 0) pin_reset();
             hip_check(hipMemcpyAsync(d_sprev.p, d_s.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "sprev");
             kpd_apply(d_s.p, d_kz_.p, precision);
             launch_axpbypcz(stream_, n3, 1.0, d_rhs.p, 1.0, d_kz_.p, 0.0, nullptr, d_tmp_.p);
@@ -1436,26 +1433,15 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             if (nb > 0) {
+                // s <- s - A^-1 J^T G^-1 J A^-1 (...) on the device: project the candidate
+                // vectors on the basis, apply G^-1 (formed once per frame, FP32 copy in mixed
+                // mode), expand back to candidate impulses and scatter through S_C
                 if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
-                const double* hu = down(d_uC_.p, 3 * K_);
-                hip_check(hipStreamSynchronize(stream_), "basis gather");
-                std::memcpy(uC.data(), hu, sizeof(double) * 3 * K_);
-                for (int a = 0; a < nb; ++a) {
-                    const int c = b_c[a];
-                    ub[a] = b_dir[3 * a] * uC[3 * c] + b_dir[3 * a + 1] * uC[3 * c + 1] + b_dir[3 * a + 2] * uC[3 * c + 2];
-                }
                 PhaseTimer tbs(this, kBwdBasisSolve);
-                // G^-1 ub with the inverse formed once per frame (FP32 copy in mixed mode)
-                up(d_rhsK_.p, ub.data(), nb);
+                launch_basis_project(stream_, nb, d_bcidx.p, d_dirs.p, d_uC_.p, d_rhsK_.p);
                 if (precision == 1) launch_gemv_sym<float>(stream_, nb, gld, d_Gs32.p, d_rhsK_.p, d_Gx.p);
                 else launch_gemv_sym<double>(stream_, nb, gld, d_Gs64.p, d_rhsK_.p, d_Gx.p);
-                const double* hg = down(d_Gx.p, nb);
-                hip_check(hipStreamSynchronize(stream_), "basis solve");
-                std::memcpy(ub.data(), hg, sizeof(double) * nb);
-                std::fill(Fc.begin(), Fc.end(), 0.0);
-                for (int a = 0; a < nb; ++a)
-                    for (int d = 0; d < 3; ++d) Fc[3 * b_c[a] + d] -= ub[a] * b_dir[3 * a + d];
-                up(d_dF_.p, Fc.data(), 3 * K_);
+                launch_basis_expand(stream_, K_, d_bcptr.p, d_bclist.p, d_dirs.p, d_Gx.p, d_dF_.p);
                 if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
             }
             if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_sprev.p, d_zero.p, d_s.p);
@@ -1561,6 +1547,15 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 }
                 d_bcidx.upload(b_c);
                 d_dirs.upload(b_dir);
+                {  // basis rows grouped by candidate, increasing row order within a candidate
+                    std::vector<int> cptr(K_ + 1, 0), clist(nb);
+                    for (int a = 0; a < nb; ++a) ++cptr[b_c[a] + 1];
+                    for (int c = 0; c < K_; ++c) cptr[c + 1] += cptr[c];
+                    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+                    for (int a = 0; a < nb; ++a) clist[fill[b_c[a]]++] = a;
+                    d_bcptr.upload(cptr);
+                    d_bclist.upload(clist);
+                }
                 std::unique_ptr<PhaseTimer> tbf(new PhaseTimer(this, kBwdBasisFactor));
                 if (timing_level >= 2) phase[kBasisRows] += nb;
                 d_G.alloc((size_t)nb * nb);
