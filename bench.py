@@ -192,8 +192,8 @@ def profile_spmv(spec, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported by this build "
-              f"(no multi-GPU path; see how the solve would shard in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              f"(the solver has no multi-GPU path; it runs one body per GPU).", file=sys.stderr)
         sys.exit(2)
     from diffphd_amd import scenes
     cells = tuple(int(c) for c in args.cells.split(","))
