@@ -262,6 +262,17 @@ __global__ void __launch_bounds__(256) k_chol_step(int k, int T, int64_t lda, TV
     const int64_t c0 = (int64_t)k * CB, i0 = (int64_t)(k + 1 + w) * CB;
     TV z[CB];
     if (k > 0) {
+        // FP32: the two target blocks are read together with the staging loads
+        const int row = 32 * (wave >> 1) + (r & 31);
+        float aD[16], aR[16];
+        if constexpr (std::is_same<TV, float>::value) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int64_t off = row + (c0 + 32 * (wave & 1) + mfma_col(v, r)) * lda;
+                aD[v] = M[c0 + off];
+                aR[v] = has_row ? M[i0 + off] : 0.f;
+            }
+        }
         stage_block<TV>(M, lda, c0, (int64_t)kp * CB, S0);
         if (has_row) stage_block<TV>(M, lda, i0, (int64_t)kp * CB, S1);
         __syncthreads();
@@ -269,14 +280,6 @@ __global__ void __launch_bounds__(256) k_chol_step(int k, int T, int64_t lda, TV
             const f32x16 accD = tile_product_mfma(S0, S0, wave, r);
             f32x16 accR;
             if (has_row) accR = tile_product_mfma(S1, S0, wave, r);
-            const int row = 32 * (wave >> 1) + (r & 31);
-            float aD[16], aR[16];
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int64_t off = row + (c0 + 32 * (wave & 1) + mfma_col(v, r)) * lda;
-                aD[v] = M[c0 + off];
-                aR[v] = has_row ? M[i0 + off] : 0.f;
-            }
             __syncthreads();
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
