@@ -240,8 +240,10 @@ __device__ __forceinline__ void update_block(TV* __restrict__ M, int64_t lda, in
 //    (column-major 64 x 64) for the identity;
 //  * the remaining workgroups apply the update of step k - 1 to the blocks
 //    (i, j) with k < j <= i <= T.
+// Four workgroups per CU (the FP32 MFMA accumulators stay in VGPRs, so the
+// update workgroups of a step run at four waves per SIMD).
 template <class TV>
-__global__ void __launch_bounds__(256) k_chol_step(int k, int T, int64_t lda, TV* __restrict__ M,
+__global__ void __launch_bounds__(256, 4) k_chol_step(int k, int T, int64_t lda, TV* __restrict__ M,
                                                    TV* __restrict__ Xinv, int* __restrict__ fail) {
     __shared__ __align__(16) TV S0[CB][kTS];
     __shared__ __align__(16) TV S1[CB][kTS];
