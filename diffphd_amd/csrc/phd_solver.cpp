@@ -1115,17 +1115,17 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             const int nJ = (int)Jv.size(), nP = (int)Pv.size(), nF = (int)Fs.size();
             tcset.reset();
             std::unique_ptr<PhaseTimer> tcap(new PhaseTimer(this, kFwdCApply));
+            // H F_prev and H F_g are launched together and share one stream sync: stream order
+            // keeps the reused index and output buffers apart, and each result has its own
+            // pinned copy.
+            const double* h_prev = nullptr;
             if (nF > 0) {
                 up(d_idxP_.p, Pv.data(), nP);
                 up(d_idxF_.p, Fs.data(), nF);
                 launch_h_apply_idx(stream_, nP, d_idxP_.p, nF, d_idxF_.p, d_H_.p, K, d_F_.p, d_tmpK3_.p);
-                const double* h = down(d_tmpK3_.p, 3 * nP);
-                hip_check(hipStreamSynchronize(stream_), "HF_prev");
-                std::memcpy(hv.data(), h, sizeof(double) * 3 * nP);
-                for (int a = 0; a < nP; ++a)
-                    for (int d = 0; d < 3; ++d) uC[3 * Pv[a] + d] -= hv[3 * a + d];
+                h_prev = down(d_tmpK3_.p, 3 * nP);
             }
-            for (int c = 0; c < K; ++c) rhs[c] = hvec[c];
+            const double* h_g = nullptr;
             if (nJ > 0) {
                 // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2. The
                 // friction part can sit on candidates outside J (lambda > 0 with omega rounded to 0).
@@ -1144,12 +1144,17 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                     up(d_dF_.p, Fg.data(), 3 * K);
                     up(d_idxF_.p, Gv.data(), nG);
                     launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nG, d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
-                    const double* h = down(d_tmpK3_.p, 3 * nJ);
-                    hip_check(hipStreamSynchronize(stream_), "HFg");
-                    std::memcpy(hv.data(), h, sizeof(double) * 3 * nJ);
-                } else {
-                    std::fill(hv.begin(), hv.end(), 0.0);
+                    h_g = down(d_tmpK3_.p, 3 * nJ);
                 }
+            }
+            if (h_prev || h_g) hip_check(hipStreamSynchronize(stream_), "HF_prev, HFg");
+            if (h_prev)
+                for (int a = 0; a < nP; ++a)
+                    for (int d = 0; d < 3; ++d) uC[3 * Pv[a] + d] -= h_prev[3 * a + d];
+            for (int c = 0; c < K; ++c) rhs[c] = hvec[c];
+            if (nJ > 0) {
+                if (h_g) std::memcpy(hv.data(), h_g, sizeof(double) * 3 * nJ);
+                else std::fill(hv.begin(), hv.end(), 0.0);
                 std::vector<double> rJ(nJ);
                 for (int a = 0; a < nJ; ++a) {
                     const int c = Jv[a];
