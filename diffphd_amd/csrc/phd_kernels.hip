@@ -143,12 +143,20 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
     const int g = gu_group[u], ch = gu_chunk[u];
     const int c0 = ch * GRP_CW, c1 = c0 + GRP_CW - 1;
     const int ldw = spmv_ldw(n);
+    // columns >= n stage as zeros whatever the padding of w holds: the masked
+    // lanes below still multiply 0 by the staged value, and 0 * NaN is NaN
     for (int e = (int)threadIdx.x * VN; e < GRP_CW; e += 256 * VN)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             V16 t;
-            if (c0 + e + VN <= ldw) t = *reinterpret_cast<const V16*>(w + (int64_t)d * ldw + c0 + e);
-            else memset(&t, 0, sizeof(t));
+            if (c0 + e + VN <= n) {
+                t = *reinterpret_cast<const V16*>(w + (int64_t)d * ldw + c0 + e);
+            } else {
+                TV tt[VN];
+#pragma unroll
+                for (int i = 0; i < VN; ++i) tt[i] = (c0 + e + i < n) ? w[(int64_t)d * ldw + c0 + e + i] : TV(0);
+                memcpy(&t, tt, sizeof(t));
+            }
             *reinterpret_cast<V16*>(&ws[d][e]) = t;
         }
     __syncthreads();

@@ -536,6 +536,9 @@ void Body::setup(double dt, int threads) {
     d_dot_out_.alloc(32);
     w32_.alloc((size_t)3 * spmv_ldw(nv_)); y32_.alloc((size_t)3 * nv_);
     w64_.alloc((size_t)3 * spmv_ldw(nv_)); y64_.alloc((size_t)3 * nv_);
+    // k_perm_in writes columns < n only; the padding stays zero from here on
+    hip_check(hipMemsetAsync(w32_.p, 0, w32_.n * sizeof(float), stream_), "w32 padding");
+    hip_check(hipMemsetAsync(w64_.p, 0, w64_.n * sizeof(double), stream_), "w64 padding");
     check(plan_.chunk_w == 1024, "row-group chunk width must match the kernel");
     part32_.alloc(3 * (size_t)plan_.rbase[nv_] + 3); part64_.alloc(3 * (size_t)plan_.rbase[nv_] + 3);
     xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
