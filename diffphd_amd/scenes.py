@@ -106,13 +106,15 @@ def _smoothed_field(shape, rng, sigma_cells):
     return field - field.mean()
 
 
-def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0, fmu=0.3, frames=100, seed=0):
+def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0, fmu=0.3, frames=100, seed=0,
+          bump=0.05):
     """Contact-rich press: hex block, corotated + volume with per-element E from
     a seeded smoothed Gaussian random field in log space spanning `decades`
     decades, pressed by a rigid plane lowered by 20% of the block height over
     the rollout into a bumpy heightfield ground (sum of 16 seeded sinusoids,
-    total amplitude 0.05 of the block size). Loss = mean squared distance of
-    the final vertex positions to a seeded target."""
+    total amplitude `bump` (0.05) of the block size; 0 gives a flat ground).
+    Loss = mean squared distance of the final vertex positions to a seeded
+    target."""
     rng = np.random.default_rng(seed)
     V, Eidx, _, _ = hex_grid(np.ones(cells), dx, (0.0, 0.0, 0.0))
     ne = Eidx.shape[1]
@@ -120,7 +122,7 @@ def press(cells=(64, 64, 32), dx=0.01, E0=1e5, decades=1.0, nu=0.45, rho=1000.0,
     height = cells[2] * dx
     # ground heightfield
     nw = 16
-    amp = 0.05 * size
+    amp = bump * size
     u = rng.uniform(0.5, 1.0, nw)
     a = amp * u / u.sum()
     kx = 2 * np.pi * rng.integers(1, 5, nw) / (cells[0] * dx) * rng.choice([-1, 1], nw)

@@ -60,7 +60,13 @@ def lib():
                                    ctypes.POINTER(cp), dp, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         L.orc_apply_ainv.argtypes = [P, ctypes.c_double, dp, dp]
         L.orc_local_step.argtypes = [P, dp, dp]
-        L.orc_local_differential.argtypes = [P, dp, ctypes.c_int, dp]
+        L.orc_local_differential.argtypes = [P, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp]
+        L.orc_last_backward_stats.argtypes = [P, dp, ctypes.c_int]
+        L.orc_filter_prox_jacobian.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_filter_prox_jacobian.restype = ctypes.c_double
+        L.orc_jacobi_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.orc_coulomb_step.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_boundary_query.argtypes = [P, dp, dp]
         L.orc_frame_stats.argtypes = [P, ip, ip, ctypes.c_int]
         L.orc_reset_frame_stats.argtypes = [P]
         L.orc_set_refinement.argtypes = [P, ctypes.c_int]
@@ -222,11 +228,12 @@ class OracleDeformable:
         self._ok(lib().orc_local_step(self._h, qp, op))
         return out
 
-    def local_differential(self, q, exact=False):
+    def local_differential(self, q, exact=False, kpd_filter=0, trust_radius=0.9, trust_stage=1):
         q, qp = _d(q)
         nd = 3 * self._edim
         out, op = _d(np.zeros(self._ne * nd * nd))
-        self._ok(lib().orc_local_differential(self._h, qp, int(exact), op))
+        self._ok(lib().orc_local_differential(self._h, qp, int(exact), int(kpd_filter), float(trust_radius),
+                                              int(trust_stage), op))
         return out.reshape(self._ne, nd, nd)
 
     def frame_stats(self):
@@ -253,6 +260,49 @@ class OracleDeformable:
 
     def last_backward_iters(self):
         return lib().orc_last_backward_iters(self._h)
+
+    def boundary_query(self, x):
+        """Frictional boundary at the point x: (distance, normal, t1, t2, press side)."""
+        x, xp = _d(np.asarray(x, dtype=np.float64).copy())
+        o, op = _d(np.zeros(11))
+        self._ok(lib().orc_boundary_query(self._h, xp, op))
+        return o[0], o[1:4].copy(), o[4:7].copy(), o[7:10].copy(), int(o[10])
+
+    def last_backward_stats(self):
+        out, op = _d(np.zeros(9))
+        lib().orc_last_backward_stats(self._h, op, 9)
+        keys = ("backbone_iters", "backbone_converged", "clipped", "woodbury_iters", "basis_rows", "filtered",
+                "samples", "db_norm", "trust_stage")
+        return dict(zip(keys, out.tolist()))
+
+
+def filter_prox_jacobian(M, aw, theta, lo):
+    """Spectral filter of one 9x9 proximal-map block (numpy matrix, row i col j):
+    eigenvalues mu of sym(M)/aw above 1 move to 1 + theta (mu - 1), then up to
+    lo. Returns (filtered copy, largest eigenvalue shift)."""
+    m, mp = _d(np.asarray(M, dtype=np.float64).T.ravel().copy())
+    shift = lib().orc_filter_prox_jacobian(mp, float(aw), float(theta), float(lo))
+    return m.reshape(9, 9).T.copy(), float(shift)
+
+
+def coulomb_step(Wtt, u_t, fmu, lam):
+    """Per-contact Coulomb step: (lt1, lt2) from the 2x2 tangential Delassus
+    block, the tangential displacement (u1, u2), mu and lambda."""
+    w, wp = _d(np.asarray(Wtt, dtype=np.float64).ravel().copy())
+    o, op = _d(np.zeros(2))
+    lib().orc_coulomb_step(wp, float(u_t[0]), float(u_t[1]), float(fmu), float(lam), op)
+    return o.copy()
+
+
+def jacobi_eigen(A):
+    """Eigenvalues and eigenvectors (columns) of a small symmetric matrix."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    a, ap = _d(A.ravel().copy())
+    w, wp = _d(np.zeros(n))
+    z, zp = _d(np.zeros(n * n))
+    lib().orc_jacobi_eigen(n, ap, wp, zp)
+    return w.copy(), z.reshape(n, n).copy()
 
 
 def svd3(F):

@@ -342,6 +342,99 @@ struct FAux {
 
 // ProjectToManifold(F_auxiliary): corotated_pd_element_energy.cpp:21-24,
 // volume_pd_element_energy.cpp:21-59, neohookean_pd_element_energy.cpp:112-119.
+// Symmetric eigen-decomposition of an n x n matrix (n <= 9, row-major A,
+// overwritten) by cyclic Jacobi rotations: eigenvalues in w, eigenvectors in
+// the columns of Z (row-major). Textbook two-sided Jacobi (Golub & Van Loan,
+// Matrix Computations, sec. 8.5), run until the off-diagonal mass is zero to
+// round-off.
+void jacobi_eigen(int n, double* A, double* w, double* Z) {
+    for (int i = 0; i < n * n; ++i) Z[i] = 0;
+    for (int i = 0; i < n; ++i) Z[i * n + i] = 1;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += A[i * n + j] * A[i * n + j];
+                if (i != j) off += A[i * n + j] * A[i * n + j];
+            }
+        if (off <= 1e-32 * tot || off == 0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (apq == 0) continue;
+                const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
+                    const double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - sn * akq;
+                    A[k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
+                    const double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - sn * aqk;
+                    A[q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {  // Z <- Z J
+                    const double zkp = Z[k * n + p], zkq = Z[k * n + q];
+                    Z[k * n + p] = c * zkp - sn * zkq;
+                    Z[k * n + q] = sn * zkp + c * zkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) w[i] = A[i * n + i];
+}
+
+// Spectral filter of one sample's weighted proximal-map Jacobian M (9 x 9,
+// column-major, M = sum_energies k_e dBp/dF) relative to the sample's weight
+// in A, aw = sum_energies w_e vol / s. With mu an eigenvalue of sym(M) / aw
+// (eta = 1 - mu is the eigenvalue of the sample's share of the Hessian A - K):
+//   mu > 1:  mu <- 1 + theta (mu - 1)   (theta = 1 leaves the implicit-function
+//            operator as it is, 0 clamps eta at 0, -1 reflects eta to |eta|:
+//            the clamp (eps = 0) and abs filters of ProjectedNewtonMatrix,
+//            deformable_newton_forward.cpp:408-433, applied per sample block)
+//   then mu <- max(mu, lo)               (lo = -rho bounds the backbone operator
+//            A^-1 K from below; -inf for the plain clamp / abs filters).
+// With theta <= 0 and lo = -rho the block's spectrum lies in [-rho, 1], so
+// |x^T K x| <= x^T (A - M/h^2) x and A^-1 K is a contraction.
+// M is left untouched (bit for bit) when no eigenvalue moves, as the reference
+// re-assembles only when an eigenvalue was modified (:431-433). Returns the
+// largest eigenvalue shift (0: unchanged).
+double filter_prox_jacobian(double* M, double aw, double theta, double lo) {
+    if (!(aw > 0)) return 0;
+    double A[81], w[9], Z[81];
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) A[i * 9 + j] = 0.5 * (M[i + 9 * j] + M[j + 9 * i]) / aw;
+    jacobi_eigen(9, A, w, Z);
+    double shift = 0;
+    bool changed = false;
+    for (int k = 0; k < 9; ++k) {
+        double mu = w[k] > 1 ? 1 + theta * (w[k] - 1) : w[k];
+        mu = std::max(mu, lo);
+        if (mu != w[k]) { changed = true; shift = std::max(shift, std::fabs(mu - w[k])); w[k] = mu; }
+    }
+    if (!changed) return 0;
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 9; ++k) acc += Z[i * 9 + k] * w[k] * Z[j * 9 + k];
+            M[i + 9 * j] = aw * acc;
+        }
+    return shift > 0 ? shift : 1e-300;
+}
+
+// (theta, lo) of a filter mode: 1 clamp, 2 abs, 3 trust region at stage
+// `stage` (0: the unfiltered operator, 1: clamp with lo = -rho, 2: abs with lo = -rho).
+static void filter_params(int mode, int stage, double rho, double& theta, double& lo, bool& active) {
+    active = true;
+    lo = -std::numeric_limits<double>::infinity();
+    if (mode == 1) { theta = 0; return; }
+    if (mode == 2) { theta = -1; return; }
+    if (mode == 3 && stage > 0) { theta = stage == 1 ? 0.0 : -1.0; lo = -rho; return; }
+    active = false;
+    theta = 1;
+}
+
 M3 project(const Energy& en, const FAux& a) {
     if (en.type == kCorotated) return a.R;
     if (en.type == kVolume) {
@@ -536,6 +629,25 @@ struct Boundary {
     }
 };
 
+// Per-contact Coulomb step (forward_gpu.cpp:1262-1302): the stick impulse
+// solves W_tt lt = -u_t (W_tt = the contact's 2x2 tangential Delassus block,
+// row-major, u_t the tangential displacement of the unconstrained step); it is
+// kept inside the cone |lt| <= mu lambda, otherwise the impulse slides,
+// lt = -mu lambda u_t / |u_t|.
+void coulomb_step(const double* Wt, double u1, double u2, double fmu, double lam, double& lt1, double& lt2) {
+    const double det = Wt[0] * Wt[3] - Wt[1] * Wt[2];
+    double s1 = 0, s2 = 0;
+    if (std::fabs(det) > 1e-20) {
+        s1 = (Wt[3] * -u1 - Wt[1] * -u2) / det;
+        s2 = (-Wt[2] * -u1 + Wt[0] * -u2) / det;
+    }
+    const double cone = fmu * lam;
+    if (std::sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1 = s1; lt2 = s2; return; }
+    const double um = std::sqrt(u1 * u1 + u2 * u2);
+    if (um > 1e-14) { lt1 = -cone * u1 / um; lt2 = -cone * u2 / um; }
+    else lt1 = lt2 = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Sparse LDL^T (restates the CSparse-style up-looking factorization of
 // RealSim/src/Scomponent/tools/math/SparseLDLT.cpp:433-518) on a symmetric
@@ -691,6 +803,10 @@ struct Body {
     std::array<std::vector<double>, 3> csr_x;
     std::vector<int> frame_iters, frame_converged;
     int last_bwd_iters = 0;
+    struct BackwardStats {  // of the last backward call
+        double backbone_iters = 0, backbone_converged = 0, clipped = 0, woodbury_iters = 0, basis_rows = 0,
+               filtered = 0, samples = 0, db_norm = 0, trust_stage = 0;
+    } last_bwd;
 
     const double* G(int e, int s) const { return &grad[((size_t)e * samples + s) * edim * 3]; }
 
@@ -969,11 +1085,19 @@ struct Body {
     // per element the 3*edim square matrix sum_s A_s^T (sum_energies w dBp) A_s with
     // w = stiffness() (mean) * vol / s, or the element's own stiffness when
     // `exact` (exact-adjoint mode).
-    std::vector<double> local_differential(const Vec& q, bool exact) const {
+    // filter/stage/rho: spectral filter of each sample's proximal-map block
+    // (filter_params, filter_prox_jacobian); filtered (optional) receives the
+    // number of samples whose block moved by more than 1e-9 in mu.
+    std::vector<double> local_differential(const Vec& q, bool exact, int filter = 0, double rho = 0.9,
+                                           long long* filtered = nullptr, int stage = 1) const {
+        double theta = 1, lo = 0;
+        bool factive = false;
+        filter_params(filter, stage, rho, theta, lo, factive);
         for (auto& kv : dirichlet) check(q[kv.first] == kv.second, "Boundary conditions violated.");
+        long long nfilt = 0;
         const int nd = 3 * edim;
         std::vector<double> K((size_t)ne * nd * nd, 0.0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) reduction(+ : nfilt)
         for (int e = 0; e < ne; ++e) {
             double* Ke = &K[(size_t)e * nd * nd];
             for (int s = 0; s < samples; ++s) {
@@ -994,6 +1118,11 @@ struct Body {
                     }
                 }
                 if (!any) continue;
+                if (factive) {
+                    double aw = 0;
+                    for (auto& en : energies) if (en.member[e]) aw += en.elem_w(e) * element_volume / samples;
+                    if (filter_prox_jacobian(wdBp, aw, theta, lo) > 1e-9) ++nfilt;
+                }
                 // A: 9 x nd with A[i+3j, i+3k] = G(k,j).
                 const double* g = G(e, s);
                 for (int col = 0; col < nd; ++col) {
@@ -1013,6 +1142,7 @@ struct Body {
                 }
             }
         }
+        if (filtered) *filtered = nfilt;
         return K;
     }
 
@@ -1274,20 +1404,7 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
                     for (int d = 0; d < 3; ++d) ut[d] = step[d] - sn * nrm[c][d];
                     const double u1 = ut[0] * t1[c][0] + ut[1] * t1[c][1] + ut[2] * t1[c][2];
                     const double u2 = ut[0] * t2[c][0] + ut[1] * t2[c][1] + ut[2] * t2[c][2];
-                    const auto& Wt = Wtt[c];
-                    const double det = Wt[0] * Wt[3] - Wt[1] * Wt[2];
-                    double s1 = 0, s2 = 0;
-                    if (std::fabs(det) > 1e-20) {
-                        s1 = (Wt[3] * -u1 - Wt[1] * -u2) / det;
-                        s2 = (-Wt[2] * -u1 + Wt[0] * -u2) / det;
-                    }
-                    const double cone = fmu * lambda[c];
-                    if (std::sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1[c] = s1; lt2[c] = s2; }
-                    else {
-                        const double um = std::sqrt(u1 * u1 + u2 * u2);
-                        if (um > 1e-14) { lt1[c] = -cone * u1 / um; lt2[c] = -cone * u2 / um; }
-                        else lt1[c] = lt2[c] = 0.0;
-                    }
+                    coulomb_step(Wtt[c].data(), u1, u2, fmu, lambda[c], lt1[c], lt2[c]);
                 }
             }
             for (int d = 0; d < 3; ++d) {
@@ -1406,7 +1523,19 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
         rhs_qbar[i] = dl_dq_next[i] + dl_dv_next[i] * inv_h;
     }
     const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
-    const std::vector<double> Kpd = local_differential(q_next, exact);
+    const int filter = (int)opt_or(opt, "alg_phd_bwd_kpd_filter", 0.0);
+    const double rho = opt_or(opt, "alg_phd_bwd_trust_radius", 0.9);
+    check(filter >= 0 && filter <= 3, "alg_phd_bwd_kpd_filter must be 0 (none), 1 (clamp), 2 (abs) or 3 (trust region).");
+    check(!(filter == 3) || (rho > 0 && rho < 1), "alg_phd_bwd_trust_radius must lie in (0, 1).");
+    long long nfilt = 0;
+    // trust region (mode 3): stage 0 is the unfiltered operator; the backbone
+    // moves to stage 1 (clamp) and then 2 (abs), both with the lower bound -rho,
+    // when its fixed point diverges
+    int stage = filter == 3 ? 0 : 1;
+    std::vector<double> Kpd = local_differential(q_next, exact, filter, rho, &nfilt, stage);
+    last_bwd = BackwardStats();
+    last_bwd.filtered = (double)nfilt;
+    last_bwd.samples = (double)ne * samples;
 
     AlgPhdReplayCache replay;
     bool from_tape = false;
@@ -1443,10 +1572,16 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
         return x;
     };
     // Fixed point s <- safe_clip(A^-1 (rhs + K_pd s)) with AA (backward_gpu.cpp:818-861).
-    auto fixed_point = [&](const Vec& rhs, int max_it, int* used_iters) {
+    // div (optional): stop early and set it when the iteration diverges: a
+    // non-finite residual, an iterate at the clip norm, or a residual 1e3 x its
+    // smallest so far after 10 iterations
+    auto fixed_point = [&](const Vec& rhs, int max_it, int* used_iters, bool* conv = nullptr, bool* div = nullptr) {
         Vec s(dofs, 0.0);
         Anderson aa(aa_window);
         int it_used = 0;
+        if (conv) *conv = false;
+        if (div) *div = false;
+        double err_min = std::numeric_limits<double>::infinity();
         for (int i = 0; i < max_it; ++i) {
             const Vec kpd = apply_local_differential(Kpd, s);
             Vec t(dofs);
@@ -1461,18 +1596,45 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
             }
             s = s_new;
             it_used = i + 1;
-            if (i > 0 && err < rel_tol * norm(s) + abs_tol) break;
+            if (i > 0 && err < rel_tol * norm(s) + abs_tol) { if (conv) *conv = true; break; }
+            if (div) {
+                err_min = std::min(err_min, err);
+                if (!std::isfinite(err) || (clip > 0 && norm(s) >= 0.999 * clip) || (i >= 10 && err > 1e3 * err_min)) {
+                    *div = true;
+                    break;
+                }
+            }
         }
         if (used_iters) *used_iters = it_used;
         return s;
     };
-    int bb_iters = 0;
-    const Vec db = fixed_point(rhs_qbar, max_pd_iter, &bb_iters);
+    int bb_iters = 0, bb_total = 0;
+    bool bb_conv = false;
+    Vec db;
+    for (;;) {
+        bool div = false;
+        db = fixed_point(rhs_qbar, max_pd_iter, &bb_iters, &bb_conv, (filter == 3 && stage < 2) ? &div : nullptr);
+        bb_total += bb_iters;
+        if (!div) break;
+        ++stage;
+        Kpd = local_differential(q_next, exact, filter, rho, &nfilt, stage);
+        last_bwd.filtered = (double)nfilt;
+    }
+    bb_iters = bb_total;
+    last_bwd.trust_stage = filter == 3 ? stage : 0;
     last_bwd_iters = bb_iters;
     const bool diverged = clip > 0 && norm(db) >= clip * 0.999;
-    if (!diverged) {
+    last_bwd.backbone_iters = bb_iters;
+    last_bwd.backbone_converged = bb_conv ? 1 : 0;
+    last_bwd.clipped = diverged ? 1 : 0;
+    last_bwd.db_norm = norm(db);
+    // Cascade injection (backward_gpu.cpp:878-916) from the backbone adjoint db,
+    // or, with alg_phd_bwd_state_woodbury = 1 (no reference counterpart), from the
+    // Woodbury-corrected adjoint z, which is the exact adjoint of the step with
+    // its active contacts held (the reference routes only dl_dmat_w through z).
+    auto inject = [&](const Vec& src) {
         Vec inj(dofs);
-        for (int i = 0; i < dofs; ++i) inj[i] = inv_h2m * db[i];
+        for (int i = 0; i < dofs; ++i) inj[i] = inv_h2m * src[i];
         const double cap = norm(rhs_qbar) + 1.0, in = norm(inj);
         if (in > cap) for (double& t : inj) t *= cap / in;
         for (int i = 0; i < dofs; ++i) {
@@ -1483,7 +1645,9 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
         // GravitationalStateForce::BackwardForce (gravitational_state_force.cpp:24-33).
         if (gravity.size() == 3)
             for (int i = 0; i < dofs; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
-    }
+    };
+    const bool state_wb = opt_or(opt, "alg_phd_bwd_state_woodbury", 0.0) > 0.5;
+    if (!diverged && !state_wb) inject(db);
     Vec z = db;
     const bool skip_wb = opt_or(opt, "skip_contact_woodbury", 0.0) > 0.5;
     const int nl = (int)replay.contact_candidates.size();
@@ -1549,6 +1713,7 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
                 const int wt_cap = (int)opt_or(opt, "alg_phd_bwd_wt_max_iter", 120);
                 const int wt_max = std::min(max_pd_iter, std::max(1, wt_cap));
                 std::vector<Vec> WT(nb);
+                last_bwd.basis_rows = nb;
                 for (int k = 0; k < nb; ++k) {
                     const BC b = full[sel[k]];
                     Vec j(dofs, 0.0);
@@ -1556,7 +1721,9 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
                     dir(b, dd);
                     const int node = replay.contact_candidates[b.ci];
                     for (int d = 0; d < 3; ++d) j[3 * node + d] = dd[d];
-                    WT[k] = fixed_point(j, wt_max, nullptr);
+                    int used = 0;
+                    WT[k] = fixed_point(j, wt_max, &used);
+                    last_bwd.woodbury_iters = std::max(last_bwd.woodbury_iters, (double)used);
                 }
                 std::vector<double> GT((size_t)nb * nb);
                 Vec rb(nb);
@@ -1572,6 +1739,7 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
             }
         }
     }
+    if (!diverged && state_wb) inject(z);
     // dl/d(mat_w) (:1231-1249) with PdEnergyForceDifferential(require_dw),
     // deformable_pd_energy.cpp:368-403: df/dw = -(vol/s) A^T vec(P / stiffness_).
     const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
@@ -1727,10 +1895,11 @@ int orc_local_step(void* hp, const double* q, double* out) {
     });
 }
 // Per-element K_pd blocks (ne x 3edim x 3edim, row-major).
-int orc_local_differential(void* hp, const double* q, int exact, double* out) {
+int orc_local_differential(void* hp, const double* q, int exact, int filter, double rho, int stage, double* out) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] {
-        const std::vector<double> K = h->body.local_differential(orc::Vec(q, q + h->body.dofs), exact != 0);
+        const std::vector<double> K =
+            h->body.local_differential(orc::Vec(q, q + h->body.dofs), exact != 0, filter, rho, nullptr, stage);
         std::copy(K.begin(), K.end(), out);
     });
 }
@@ -1753,6 +1922,34 @@ int orc_last_contact_multipliers(void* hp, double* l, double* a, double* b, int 
     return n;
 }
 int orc_last_backward_iters(void* hp) { return static_cast<Handle*>(hp)->body.last_bwd_iters; }
+// [backbone iterations (all trust stages), backbone converged, clipped, max Woodbury
+//  basis iterations, basis rows, filtered samples, samples, |db|, trust stage]
+int orc_last_backward_stats(void* hp, double* out, int cap) {
+    const auto& b = static_cast<Handle*>(hp)->body.last_bwd;
+    const double v[9] = {b.backbone_iters, b.backbone_converged, b.clipped, b.woodbury_iters,
+                         b.basis_rows, b.filtered, b.samples, b.db_norm, b.trust_stage};
+    for (int i = 0; i < 9 && i < cap; ++i) out[i] = v[i];
+    return 9;
+}
+void orc_coulomb_step(const double* Wt, double u1, double u2, double fmu, double lam, double* out2) {
+    orc::coulomb_step(Wt, u1, u2, fmu, lam, out2[0], out2[1]);
+}
+// Frictional boundary of the body at q (3): out = [distance, n (3), t1 (3), t2 (3),
+// governing obstacle of the press: 0 ground, 1 plane (-1 for other types)].
+int orc_boundary_query(void* hp, const double* q, double* out) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const auto& b = h->body.boundary;
+        out[0] = b.distance(q);
+        b.frame(q, out + 4, out + 7, out + 1);
+        double dg = 0, dp = 0;
+        out[10] = b.type == orc::kPress ? b.press_side(q, dg, dp) : -1;
+    });
+}
+double orc_filter_prox_jacobian(double* M, double aw, double theta, double lo) {
+    return orc::filter_prox_jacobian(M, aw, theta, lo);
+}
+void orc_jacobi_eigen(int n, double* A, double* w, double* Z) { orc::jacobi_eigen(n, A, w, Z); }
 
 // --- element-level primitives for unit tests -------------------------------
 void orc_svd3(const double* F, double* U, double* sig, double* V) {
