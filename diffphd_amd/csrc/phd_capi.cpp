@@ -94,6 +94,11 @@ int phd_step_forward(phd_ctx ctx, const double* q, const double* v, const double
                      const char* const* keys, const double* vals, int n, double* q_next, double* v_next, int* active,
                      int* n_active) {
     return guard(ctx, [&](phd::Body& b) {
+        // checked before the step: a rejected call leaves the tape and the
+        // lambda warm start as they were
+        if (!n_active || *n_active < b.num_boundary_vertices())
+            throw std::runtime_error("active_contact_idx buffer too small: it needs room for every frictional-boundary vertex (" +
+                                     std::to_string(b.num_boundary_vertices()) + ")");
         std::vector<int> act;
         b.forward(q, v, f_ext, dt, opts(keys, vals, n), q_next, v_next, act);
         if ((int)act.size() > *n_active) throw std::runtime_error("active_contact_idx buffer too small");
@@ -132,6 +137,14 @@ void phd_reset_frame_stats(phd_ctx ctx) {
     if (ctx) static_cast<Ctx*>(ctx)->body->frame_stats.clear();
 }
 int phd_last_backward_iters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->last_backward_iters : -1; }
+int phd_last_backward_stats(phd_ctx ctx, double* out, int cap) {
+    if (!ctx) return -1;
+    const auto& s = static_cast<Ctx*>(ctx)->body->last_bwd;
+    const double v[10] = {s.backbone_iters, s.backbone_converged, s.clipped, s.woodbury_iters, s.woodbury_converged,
+                          s.basis_rows, s.filtered, s.samples, s.db_norm, s.trust_stage};
+    for (int i = 0; i < 10 && i < cap; ++i) out[i] = v[i];
+    return 10;
+}
 int phd_factor_info(phd_ctx ctx, double* out, int cap) {
     return guard(ctx, [&](phd::Body& b) {
         const phd::Factor& f = b.factor();

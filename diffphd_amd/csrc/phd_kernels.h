@@ -134,7 +134,9 @@ void launch_symmetrize_lower(hipStream_t s, int K, double* H);
 void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
 void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
 // ub[a] = dirs_a . u[cidx[a]] (3-vectors per candidate)
-void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, double* ub);
+// ub[a] = d_a . (u + u2)_{c_a} (u2 may be null)
+void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, const double* u2,
+                          double* ub);
 // F[3c + d] = -sum over the basis rows a of candidate c (clist[cptr[c] .. cptr[c+1]), increasing a) of g[a] dirs_a[d]
 void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist, const double* dirs, const double* g,
                          double* F);
@@ -147,6 +149,34 @@ template <class TV>
 void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y);
 
 // out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]
+// ---- NCP iteration on the device (forward_gpu.cpp:1149-1302) ----
+// flag bits of a candidate: in J (omega != 0), carries an impulse (F != 0),
+// carries an impulse of g (omega lambda != 0 or friction), in P (J or F);
+// friction sets: lambda > 0, omega lambda != 0
+constexpr int kNcpJ = 1, kNcpF = 2, kNcpG = 4, kNcpP = 8, kNcpL = 1, kNcpN = 2;
+void launch_ncp_sets(hipStream_t s, int K, const double* delta, const double* lambda, const double* rprec,
+                     const double* nrm, const double* t1, const double* t2, const double* qC, double* uC, const double* F,
+                     const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec, double* Fg,
+                     unsigned char* flags);
+// lists (nl <= 4, K entries each) of the candidates whose flags contain masks[i];
+// counts[i] their sizes; pos[c] = position of c in list 0 or -1
+void launch_ncp_compact(hipStream_t s, int K, const unsigned char* flags, int nl, const int* masks, int* lists, int* counts,
+                        int* pos);
+void launch_ncp_rhs(hipStream_t s, int nP, const int* P, const double* hprev, const int* posJ, const double* hg,
+                    const double* hvec, const double* om, const double* nrm, double* uC, double* rJ);
+template <class TV>
+void launch_ncp_lambda(hipStream_t s, int K, const int* posJ, const TV* x, const double* hvec, const double* E,
+                       const double* om, double* lambda);
+void launch_fric_sets(hipStream_t s, int K, const double* lambda, const double* om, const double* nrm, double* Fn,
+                      unsigned char* flags);
+void launch_fric_coulomb(hipStream_t s, int K, const double* lambda, const int* posL, const double* hn, const double* uC,
+                         const double* qframe, const double* nrm, const double* t1, const double* t2, const double* Hdiag,
+                         double fmu, double* lt1, double* lt2);
+void launch_ncp_impulses(hipStream_t s, int K, const double* om, const double* lambda, const double* nrm, const double* lt1,
+                         const double* lt2, const double* t1, const double* t2, int do_fric, double* F, double* dF);
+// H_rows,cols f with both counts read on the device; grid for max_rows rows
+void launch_h_apply_idx_dev(hipStream_t s, int max_rows, const int* nr_dev, const int* rows, const int* nc_dev,
+                            const int* cols, const double* H, int K, const double* f, double* out);
 void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
                         const double* f, double* out);
 // contact system restricted to the index set J (row-major nJ x nJ)
@@ -160,8 +190,12 @@ void launch_copy(hipStream_t s, int64_t n16, const void* a, void* b);
 // ---- backward ----------------------------------------------------------------
 // per-sample 9x9 proximal-map Jacobians weighted by the K_pd weights
 template <class TW>
+// filter != 0: each sample block's eigenvalues mu (relative to the sample's
+// weight in A) above 1 move to 1 + theta (mu - 1), then up to lo; filter = 0
+// keeps the reference's K_pd. nfilt (may be null) counts the samples whose
+// block moved by more than 1e-9.
 void launch_kpd_build(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* const* kn,
-                      const double* q, TW* W);
+                      const double* q, int filter, double theta, double lo, int* nfilt, TW* W);
 template <class TW>
 void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const double* dq, double* elem_out);
 // g[type][e]: z_e^T (-(vol/s) sum_s A_s^T vec(P_s / stiffness)) per merged energy type

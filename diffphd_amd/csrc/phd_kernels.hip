@@ -838,9 +838,21 @@ __global__ void k_h_apply(int K, const double* __restrict__ H, const double* __r
 // -----------------------------------------------------------------------------
 struct KnPtrs { const double* kn[4]; };
 
+// Spectral filter of a sample's proximal-map block (backward option
+// alg_phd_bwd_kpd_filter) on one eigenvalue mu of the block scaled by the
+// sample's weight in A (eta = 1 - mu is the eigenvalue of the sample's share of
+// the Hessian A - K): mu > 1 moves to 1 + theta (mu - 1) (theta = 0 clamps eta
+// at 0, -1 reflects it to |eta|: the filters of ProjectedNewtonMatrix,
+// deformable_newton_forward.cpp:408-433 with eps = 0), then mu is raised to lo
+// (-rho in the trust region, which keeps A^-1 K's spectrum inside (-rho, 1)).
+__device__ __forceinline__ double kpd_filter_eig(double theta, double lo, double mu) {
+    return fmax(mu > 1.0 ? 1.0 + theta * (mu - 1.0) : mu, lo);
+}
+
 template <int EDIM, class TW>
 __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* __restrict__ kc, const double* __restrict__ kv,
-                                                  KnPtrs kn, const double* __restrict__ q, TW* __restrict__ W) {
+                                                  KnPtrs kn, const double* __restrict__ q, int filter, double theta,
+                                                  double lo, int* __restrict__ nfilt, TW* __restrict__ W) {
     // one thread per (sample, element), elements fastest: the writes of W
     // (element-minor: entry j of sample s at (81 s + j) ne + e) are coalesced
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -870,12 +882,8 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
     if (wv != 0.0) volume_D(sg, D);
     for (int h = 0; h < m.n_nh; ++h)
         if (kn.kn[h] && kn.kn[h][e] != 0.0) nh_prox(m.nh_mu[h], m.nh_la[h], m.nh_k[h], sg, ssn[h]);
-    TW* out = W + (int64_t)s * 81 * m.ne + e;
-    for (int c = 0; c < 9; ++c) {
-        double dF[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) dF[i] = (i == c) ? 1.0 : 0.0;
-        double col[9];
+    // col = sum_energies k_e dBp/dF [dF]
+    auto op = [&](const double* dF, double* col) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) col[i] = 0.0;
         if (wc != 0.0) {
@@ -898,6 +906,132 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
 #pragma unroll
             for (int i = 0; i < 9; ++i) col[i] += w * dP[i];
         }
+    };
+    TW* out = W + (int64_t)s * 81 * m.ne + e;
+    if (filter != 0) {
+        // The proximal maps are isotropic, so in the basis E_kl = u_k v_l^T of
+        // the SVD of F the block splits into a 3x3 stretch block (E_00, E_11,
+        // E_22) and three 2x2 blocks (E_kl, E_lk), k < l: nine applications of
+        // the differential give all of them, and their eigenvalues are those of
+        // the 9x9 block.
+        double aw = 0.0;
+        if (m.wc) aw += m.wc[e];
+        if (m.wv) aw += m.wv[e];
+        for (int h = 0; h < m.n_nh; ++h) if (m.wn[h]) aw += m.wn[h][e];
+        if (aw > 0.0) {
+            double Bs[9], P[3][3];  // stretch block (column-major); pair blocks a, b, c of [[a, b], [b, c]]
+            double dF[9], col[9], t[9];
+            for (int i = 0; i < 3; ++i) {  // stretch: input E_ii
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) dF[a + 3 * b] = U[a + 3 * i] * V[b + 3 * i];
+                op(dF, col);
+                double ct[9];
+                mat_tmul(U, col, ct);  // U^T col
+                mat_mul(ct, V, t);     // U^T col V
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Bs[k + 3 * i] = t[4 * k];
+            }
+            const int pk[3] = {0, 0, 1}, pl[3] = {1, 2, 2};
+            for (int p = 0; p < 3; ++p) {
+                const int k = pk[p], l = pl[p];
+                double x[2][2];  // x[out][in]: out/in 0 = E_kl, 1 = E_lk
+                for (int in = 0; in < 2; ++in) {
+                    const int ki = in == 0 ? k : l, li = in == 0 ? l : k;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) dF[a + 3 * b] = U[a + 3 * ki] * V[b + 3 * li];
+                    op(dF, col);
+                    double ct[9];
+                    mat_tmul(U, col, ct);
+                    mat_mul(ct, V, t);
+                    x[0][in] = t[k + 3 * l];
+                    x[1][in] = t[l + 3 * k];
+                }
+                P[p][0] = x[0][0] / aw;
+                P[p][1] = 0.5 * (x[0][1] + x[1][0]) / aw;
+                P[p][2] = x[1][1] / aw;
+            }
+            double S3[9], Z[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) S3[i + 3 * j] = 0.5 * (Bs[i + 3 * j] + Bs[j + 3 * i]) / aw;
+            sym3_eigen(S3, Z);
+            double shift = 0.0;
+            bool changed = false;
+            double ls[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double mu = S3[4 * k];
+                ls[k] = kpd_filter_eig(theta, lo, mu);
+                changed |= ls[k] != mu;
+                shift = fmax(shift, fabs(ls[k] - mu));
+            }
+            double pl0[3], pl1[3], pcs[3], psn[3];
+            for (int p = 0; p < 3; ++p) {
+                double l0, l1;
+                sym2_eigen(P[p][0], P[p][1], P[p][2], l0, l1, pcs[p], psn[p]);
+                pl0[p] = kpd_filter_eig(theta, lo, l0);
+                pl1[p] = kpd_filter_eig(theta, lo, l1);
+                changed |= (pl0[p] != l0) | (pl1[p] != l1);
+                shift = fmax(shift, fmax(fabs(pl0[p] - l0), fabs(pl1[p] - l1)));
+            }
+            if (changed) {
+                if (shift > 1e-9 && nfilt) atomicAdd(nfilt, 1);
+                // filtered blocks B' (times aw), then column c of the 9x9 block is
+                // U B'(U^T E_c V) V^T with E_c the unit matrix of entry c
+                double Bf[9];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        Bf[i + 3 * j] = aw * (Z[i] * ls[0] * Z[j] + Z[i + 3] * ls[1] * Z[j + 3] + Z[i + 6] * ls[2] * Z[j + 6]);
+                double Pf[3][3];
+                for (int p = 0; p < 3; ++p) {
+                    const double c = pcs[p], sn = psn[p];
+                    // eigenvectors (c, -sn) for l0 and (sn, c) for l1
+                    Pf[p][0] = aw * (c * c * pl0[p] + sn * sn * pl1[p]);
+                    Pf[p][1] = aw * (-c * sn * pl0[p] + sn * c * pl1[p]);
+                    Pf[p][2] = aw * (sn * sn * pl0[p] + c * c * pl1[p]);
+                }
+                for (int c = 0; c < 9; ++c) {
+                    const int ac = c % 3, bc = c / 3;
+                    double T[9], S[9];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) T[k + 3 * l] = U[ac + 3 * k] * V[bc + 3 * l];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) S[4 * k] = Bf[k] * T[0] + Bf[k + 3] * T[4] + Bf[k + 6] * T[8];
+                    for (int p = 0; p < 3; ++p) {
+                        const int k = pk[p], l = pl[p];
+                        const double tkl = T[k + 3 * l], tlk = T[l + 3 * k];
+                        S[k + 3 * l] = Pf[p][0] * tkl + Pf[p][1] * tlk;
+                        S[l + 3 * k] = Pf[p][1] * tkl + Pf[p][2] * tlk;
+                    }
+                    double US[9], cl[9];
+                    mat_mul(U, S, US);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                            cl[i + 3 * j] = US[i] * V[j] + US[i + 3] * V[j + 3] + US[i + 6] * V[j + 6];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) out[(int64_t)(i + 9 * c) * m.ne] = (TW)cl[i];
+                }
+                return;
+            }
+        }
+    }
+    for (int c = 0; c < 9; ++c) {
+        double dF[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dF[i] = (i == c) ? 1.0 : 0.0;
+        double col[9];
+        op(dF, col);
 #pragma unroll
         for (int i = 0; i < 9; ++i) out[(int64_t)(i + 9 * c) * m.ne] = (TW)col[i];
     }
@@ -1076,11 +1210,13 @@ __global__ void k_assemble_g(int nb, const int* __restrict__ cidx, const double*
 // ub[a] = d_a . u[c_a]: the candidate vectors projected on the basis rows,
 // J_basis u (backward_gpu.cpp:998-1024 projects the same way, on the host).
 __global__ void k_basis_project(int nb, const int* __restrict__ cidx, const double* __restrict__ dirs,
-                                const double* __restrict__ u, double* __restrict__ ub) {
+                                const double* __restrict__ u, const double* __restrict__ u2, double* __restrict__ ub) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nb) return;
     const int c = cidx[a];
-    ub[a] = dirs[3 * a] * u[3 * c] + dirs[3 * a + 1] * u[3 * c + 1] + dirs[3 * a + 2] * u[3 * c + 2];
+    double x = u[3 * c], y = u[3 * c + 1], z = u[3 * c + 2];
+    if (u2) { x += u2[3 * c]; y += u2[3 * c + 1]; z += u2[3 * c + 2]; }
+    ub[a] = dirs[3 * a] * x + dirs[3 * a + 1] * y + dirs[3 * a + 2] * z;
 }
 
 // F[3c + d] = -sum_a g[a] d_a[d] over the basis rows a of candidate c, in
@@ -1384,9 +1520,10 @@ void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dir
     if (nb == 0) return;
     hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
 }
-void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, double* ub) {
+void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, const double* u2,
+                          double* ub) {
     if (nb == 0) return;
-    hipLaunchKernelGGL(k_basis_project, dim3(blocks_for(nb, 256)), dim3(256), 0, s, nb, cidx, dirs, u, ub);
+    hipLaunchKernelGGL(k_basis_project, dim3(blocks_for(nb, 256)), dim3(256), 0, s, nb, cidx, dirs, u, u2, ub);
 }
 void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist, const double* dirs, const double* g,
                          double* F) {
@@ -1430,14 +1567,16 @@ void launch_h_apply(hipStream_t s, int K, const double* H, const double* f, doub
 
 template <class TW>
 void launch_kpd_build(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* const* kn,
-                      const double* q, TW* W) {
+                      const double* q, int filter, double theta, double lo, int* nfilt, TW* W) {
     KnPtrs p;
     for (int h = 0; h < 4; ++h) p.kn[h] = (h < m.n_nh) ? kn[h] : nullptr;
     const int64_t total = (int64_t)m.ne * m.samples;
     if (m.edim == 8)
-        hipLaunchKernelGGL((k_kpd_build<8, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, W);
+        hipLaunchKernelGGL((k_kpd_build<8, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, filter, theta,
+                           lo, nfilt, W);
     else
-        hipLaunchKernelGGL((k_kpd_build<4, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, W);
+        hipLaunchKernelGGL((k_kpd_build<4, TW>), dim3(blocks_for(total, 64)), dim3(64), 0, s, m, kc, kv, p, q, filter, theta,
+                           lo, nfilt, W);
 }
 template <class TW>
 void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const double* dq, double* elem_out) {
@@ -1447,9 +1586,9 @@ void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const d
         hipLaunchKernelGGL((k_kpd_apply<4, TW>), dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, W, dq, elem_out);
 }
 template void launch_kpd_build<float>(hipStream_t, const DevMaterials&, const double*, const double*, const double* const*,
-                                      const double*, float*);
+                                       const double*, int, double, double, int*, float*);
 template void launch_kpd_build<double>(hipStream_t, const DevMaterials&, const double*, const double*, const double* const*,
-                                       const double*, double*);
+                                       const double*, int, double, double, int*, double*);
 template void launch_kpd_apply<float>(hipStream_t, const DevMaterials&, const float*, const double*, double*);
 template void launch_kpd_apply<double>(hipStream_t, const DevMaterials&, const double*, const double*, double*);
 
@@ -1459,6 +1598,287 @@ void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const d
         hipLaunchKernelGGL(k_dmatw<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_cor, g_vol, g_nh);
     else
         hipLaunchKernelGGL(k_dmatw<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_cor, g_vol, g_nh);
+}
+
+// -----------------------------------------------------------------------------
+// The per-candidate part of the NCP iteration on the device
+// (ForwardProjectiveDynamicsAlgPhd, forward_gpu.cpp:1149-1302): the smoothed
+// Signorini weights, the index sets of the compressed contact system, the
+// multiplier update and the Coulomb step. Only the set sizes and the dense
+// solve's pivot flag reach the host.
+// -----------------------------------------------------------------------------
+namespace {
+
+// omega, E, h (:1160-1197) per candidate, the current impulses' support flags
+// and F_g = omega lambda n + lt1 t1 + lt2 t2 (the impulses g carries); uC, the
+// candidates' rows of S^T S r, becomes (A^-1 b)_C + H F_prev by adding q_C.
+__global__ void k_ncp_sets(int K, const double* __restrict__ delta, const double* __restrict__ lambda,
+                           const double* __restrict__ rprec, const double* __restrict__ nrm, const double* __restrict__ t1,
+                           const double* __restrict__ t2, const double* __restrict__ qC, double* __restrict__ uC,
+                           const double* __restrict__ F, const double* __restrict__ lt1, const double* __restrict__ lt2,
+                           int do_fric, double* __restrict__ om, double* __restrict__ E, double* __restrict__ hvec,
+                           double* __restrict__ Fg, unsigned char* __restrict__ flags) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const double dj = delta[c], lj = lambda[c], rj = rprec[c];
+    double den = sqrt(dj * dj + rj * rj * lj * lj);
+    if (den < 1e-14) den = 1e-14;
+    const double o = 1.0 - dj / den;
+    om[c] = o;
+    E[c] = (1.0 - rj * lj / den) * rj;
+    const double phi = dj + rj * lj - den;
+    double hq = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double q = qC[3 * c + d];
+        uC[3 * c + d] += q;
+        hq += nrm[3 * c + d] * q;
+    }
+    hvec[c] = -phi + o * hq;
+    const double l1 = lt1[c], l2 = lt2[c];
+    const bool fr = do_fric && (l1 != 0.0 || l2 != 0.0);
+    const bool inG = o * lj != 0.0 || fr;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        Fg[3 * c + d] = inG ? o * lj * nrm[3 * c + d] + (fr ? l1 * t1[3 * c + d] + l2 * t2[3 * c + d] : 0.0) : 0.0;
+    const bool inF = F[3 * c] != 0.0 || F[3 * c + 1] != 0.0 || F[3 * c + 2] != 0.0;
+    flags[c] = (unsigned char)((o != 0.0 ? kNcpJ : 0) | (inF ? kNcpF : 0) | (inG ? kNcpG : 0) |
+                               (o != 0.0 || inF ? kNcpP : 0));
+}
+
+// Exclusive prefix sum over the workgroup (1024 threads, 16 waves).
+__device__ int block_exclusive_scan(int v, int* sh, int& total) {
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) sh[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int t = lane < nw ? sh[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(t, o, WAVE);
+            if (lane >= o) t += y;
+        }
+        if (lane < nw) sh[lane] = t;
+    }
+    __syncthreads();
+    const int base = w > 0 ? sh[w - 1] : 0;
+    total = sh[nw - 1];
+    __syncthreads();
+    return base + x - v;
+}
+
+// Lists (in increasing candidate order) of the candidates whose flags contain
+// mask m.m[i] (all bits), i < nl; counts[i] = list sizes; pos[c] = position of
+// c in list 0, -1 when absent. One workgroup; each thread owns a contiguous run.
+struct Masks { int m[4]; };
+__global__ void __launch_bounds__(1024) k_ncp_compact(int K, const unsigned char* __restrict__ flags, int nl, Masks mk,
+                                                      int* __restrict__ lists, int* __restrict__ counts,
+                                                      int* __restrict__ pos) {
+    __shared__ int sh[16];
+    const int per = (K + blockDim.x - 1) / blockDim.x;
+    const int c0 = min(K, (int)threadIdx.x * per), c1 = min(K, c0 + per);
+    for (int i = 0; i < nl; ++i) {
+        const int m = mk.m[i];
+        int cnt = 0;
+        for (int c = c0; c < c1; ++c) cnt += (flags[c] & m) == m;
+        int total = 0;
+        int at = block_exclusive_scan(cnt, sh, total);
+        int* L = lists + (int64_t)i * K;
+        for (int c = c0; c < c1; ++c) {
+            const bool in = (flags[c] & m) == m;
+            if (in) L[at] = c;
+            if (i == 0) pos[c] = in ? at : -1;
+            at += in;
+        }
+        if (threadIdx.x == 0) counts[i] = total;
+    }
+}
+
+// For a < nP (c = P[a]): uC_c -= (H F_prev)_a; for c in J (a' = pos_J[c]) the
+// right-hand side of the compressed system, h_c - omega_c n_c . (uC_c + (H F_g)_a').
+__global__ void k_ncp_rhs(int nP, const int* __restrict__ P, const double* __restrict__ hprev, const int* __restrict__ posJ,
+                          const double* __restrict__ hg, const double* __restrict__ hvec, const double* __restrict__ om,
+                          const double* __restrict__ nrm, double* __restrict__ uC, double* __restrict__ rJ) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nP) return;
+    const int c = P[a];
+    double u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        u[d] = uC[3 * c + d] - hprev[3 * a + d];
+        uC[3 * c + d] = u[d];
+    }
+    const int j = posJ[c];
+    if (j < 0) return;
+    double val = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (u[d] + hg[3 * j + d]);
+    rJ[j] = hvec[c] - om[c] * val;
+}
+
+// lambda_c <- clamp(lambda_c + dl_c, 0, 1e8): dl from the compressed solve on J,
+// h_c / E_c for the decoupled rest (:1241-1253).
+template <class TV>
+__global__ void k_ncp_lambda(int K, const int* __restrict__ posJ, const TV* __restrict__ x, const double* __restrict__ hvec,
+                             const double* __restrict__ E, const double* __restrict__ om, double* __restrict__ lambda) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const double dl = om[c] != 0.0 ? (double)x[posJ[c]] : hvec[c] / E[c];
+    lambda[c] = fmin(fmax(lambda[c] + dl, 0.0), 1e8);
+}
+
+// Friction sets: lambda > 0 (kNcpL) and omega lambda != 0 (kNcpN, carrying the
+// normal impulse F_n = omega lambda n).
+__global__ void k_fric_sets(int K, const double* __restrict__ lambda, const double* __restrict__ om,
+                            const double* __restrict__ nrm, double* __restrict__ Fn, unsigned char* __restrict__ flags) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const double l = lambda[c], ol = om[c] * l;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Fn[3 * c + d] = ol != 0.0 ? ol * nrm[3 * c + d] : 0.0;
+    flags[c] = (unsigned char)((l > 0.0 ? kNcpL : 0) | (ol != 0.0 ? kNcpN : 0));
+}
+
+// The NCP-integrated Coulomb step (:1255-1302) for lambda_c > 0: the unconstrained
+// step of the contact vertex is uC_c + (H F_n)_c - q_c(frame start); its
+// tangential part u_t gives the stick impulse W_tt lt = -u_t (W_tt = H_cc times
+// the tangents' Gram matrix), kept if inside the cone mu lambda, else the slip
+// impulse -mu lambda u_t / |u_t|. Zero for lambda_c <= 0.
+__global__ void k_fric_coulomb(int K, const double* __restrict__ lambda, const int* __restrict__ posL,
+                               const double* __restrict__ hn, const double* __restrict__ uC,
+                               const double* __restrict__ qframe, const double* __restrict__ nrm,
+                               const double* __restrict__ t1, const double* __restrict__ t2, const double* __restrict__ Hdiag,
+                               double fmu, double* __restrict__ lt1, double* __restrict__ lt2) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const double lam = lambda[c];
+    if (lam <= 0.0) { lt1[c] = 0.0; lt2[c] = 0.0; return; }
+    const int a = posL[c];
+    double step[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) step[d] = uC[3 * c + d] + hn[3 * a + d] - qframe[3 * c + d];
+    const double* n = nrm + 3 * c;
+    const double* e1 = t1 + 3 * c;
+    const double* e2 = t2 + 3 * c;
+    const double sn = step[0] * n[0] + step[1] * n[1] + step[2] * n[2];
+    double ut[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ut[d] = step[d] - sn * n[d];
+    const double u1 = ut[0] * e1[0] + ut[1] * e1[1] + ut[2] * e1[2];
+    const double u2 = ut[0] * e2[0] + ut[1] * e2[1] + ut[2] * e2[2];
+    const double hc = Hdiag[c];
+    const double w11 = hc * (e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    const double w12 = hc * (e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2]);
+    const double w22 = hc * (e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    const double det = w11 * w22 - w12 * w12;
+    double s1 = 0.0, s2 = 0.0;
+    if (fabs(det) > 1e-20) { s1 = (w22 * -u1 - w12 * -u2) / det; s2 = (-w12 * -u1 + w11 * -u2) / det; }
+    const double cone = fmu * lam;
+    if (sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1[c] = s1; lt2[c] = s2; return; }
+    const double um = sqrt(u1 * u1 + u2 * u2);
+    if (um > 1e-14) { lt1[c] = -cone * u1 / um; lt2[c] = -cone * u2 / um; }
+    else { lt1[c] = 0.0; lt2[c] = 0.0; }
+}
+
+// New impulses F = omega lambda n + lt1 t1 + lt2 t2 and their change dF (the S
+// pass carries only the change).
+__global__ void k_ncp_impulses(int K, const double* __restrict__ om, const double* __restrict__ lambda,
+                               const double* __restrict__ nrm, const double* __restrict__ lt1, const double* __restrict__ lt2,
+                               const double* __restrict__ t1, const double* __restrict__ t2, int do_fric,
+                               double* __restrict__ F, double* __restrict__ dF) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const double ol = om[c] * lambda[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double f = ol * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
+        dF[3 * c + d] = f - F[3 * c + d];
+        F[3 * c + d] = f;
+    }
+}
+
+// k_h_apply_idx with the row and column counts read on the device (grid sized
+// for the largest possible row list).
+__global__ void k_h_apply_idx_dev(const int* __restrict__ nr_dev, const int* __restrict__ rows, const int* __restrict__ nc_dev,
+                                  const int* __restrict__ cols, const double* __restrict__ H, int K,
+                                  const double* __restrict__ f, double* __restrict__ out) {
+    const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (a >= *nr_dev) return;
+    const int nc = *nc_dev;
+    const double* Hr = H + (int64_t)rows[a] * K;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int b = lane; b < nc; b += WAVE) {
+        const int c = cols[b];
+        const double h = Hr[c];
+        a0 += h * f[3 * c];
+        a1 += h * f[3 * c + 1];
+        a2 += h * f[3 * c + 2];
+    }
+    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+    if (lane == 0) { out[3 * a] = a0; out[3 * a + 1] = a1; out[3 * a + 2] = a2; }
+}
+
+}  // namespace
+
+void launch_ncp_sets(hipStream_t s, int K, const double* delta, const double* lambda, const double* rprec,
+                     const double* nrm, const double* t1, const double* t2, const double* qC, double* uC, const double* F,
+                     const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec, double* Fg,
+                     unsigned char* flags) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_ncp_sets, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, delta, lambda, rprec, nrm, t1, t2, qC, uC,
+                       F, lt1, lt2, do_fric, om, E, hvec, Fg, flags);
+}
+void launch_ncp_compact(hipStream_t s, int K, const unsigned char* flags, int nl, const int* masks, int* lists, int* counts,
+                        int* pos) {
+    Masks mk = {{0, 0, 0, 0}};
+    for (int i = 0; i < nl && i < 4; ++i) mk.m[i] = masks[i];
+    hipLaunchKernelGGL(k_ncp_compact, dim3(1), dim3(1024), 0, s, K, flags, nl, mk, lists, counts, pos);
+}
+void launch_ncp_rhs(hipStream_t s, int nP, const int* P, const double* hprev, const int* posJ, const double* hg,
+                    const double* hvec, const double* om, const double* nrm, double* uC, double* rJ) {
+    if (nP == 0) return;
+    hipLaunchKernelGGL(k_ncp_rhs, dim3(blocks_for(nP, 128)), dim3(128), 0, s, nP, P, hprev, posJ, hg, hvec, om, nrm, uC, rJ);
+}
+template <class TV>
+void launch_ncp_lambda(hipStream_t s, int K, const int* posJ, const TV* x, const double* hvec, const double* E,
+                       const double* om, double* lambda) {
+    if (K == 0) return;
+    hipLaunchKernelGGL((k_ncp_lambda<TV>), dim3(blocks_for(K, 128)), dim3(128), 0, s, K, posJ, x, hvec, E, om, lambda);
+}
+template void launch_ncp_lambda<float>(hipStream_t, int, const int*, const float*, const double*, const double*,
+                                       const double*, double*);
+template void launch_ncp_lambda<double>(hipStream_t, int, const int*, const double*, const double*, const double*,
+                                        const double*, double*);
+void launch_fric_sets(hipStream_t s, int K, const double* lambda, const double* om, const double* nrm, double* Fn,
+                      unsigned char* flags) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_fric_sets, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, lambda, om, nrm, Fn, flags);
+}
+void launch_fric_coulomb(hipStream_t s, int K, const double* lambda, const int* posL, const double* hn, const double* uC,
+                         const double* qframe, const double* nrm, const double* t1, const double* t2, const double* Hdiag,
+                         double fmu, double* lt1, double* lt2) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_fric_coulomb, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, lambda, posL, hn, uC, qframe, nrm, t1,
+                       t2, Hdiag, fmu, lt1, lt2);
+}
+void launch_ncp_impulses(hipStream_t s, int K, const double* om, const double* lambda, const double* nrm, const double* lt1,
+                         const double* lt2, const double* t1, const double* t2, int do_fric, double* F, double* dF) {
+    if (K == 0) return;
+    hipLaunchKernelGGL(k_ncp_impulses, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, om, lambda, nrm, lt1, lt2, t1, t2,
+                       do_fric, F, dF);
+}
+void launch_h_apply_idx_dev(hipStream_t s, int max_rows, const int* nr_dev, const int* rows, const int* nc_dev,
+                            const int* cols, const double* H, int K, const double* f, double* out) {
+    if (max_rows == 0) return;
+    hipLaunchKernelGGL(k_h_apply_idx_dev, dim3(blocks_for(max_rows, 4)), dim3(256), 0, s, nr_dev, rows, nc_dev, cols, H, K,
+                       f, out);
 }
 
 }  // namespace phd

@@ -70,6 +70,36 @@ __device__ __forceinline__ void jacobi_rotate(T* S, T* V, int p, int q) {
     }
 }
 
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations: S (column-
+// major, symmetric) is overwritten by its diagonal form (eigenvalues on the
+// diagonal), the eigenvectors accumulate in the columns of Z.
+template <class T>
+__device__ void sym3_eigen(T* S, T* Z) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Z[i] = (i % 4 == 0) ? T(1) : T(0);
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        const T off = fabs(S[3]) + fabs(S[6]) + fabs(S[7]);
+        const T dia = fabs(S[0]) + fabs(S[4]) + fabs(S[8]);
+        if (!(off > T(1e-30) * dia)) break;
+        if (S[3] != T(0)) jacobi_rotate(S, Z, 0, 1);
+        if (S[6] != T(0)) jacobi_rotate(S, Z, 0, 2);
+        if (S[7] != T(0)) jacobi_rotate(S, Z, 1, 2);
+    }
+}
+
+// Symmetric 2x2 [[a, b], [b, c]]: eigenvalues l0, l1 with eigenvectors
+// (cs, -sn) and (sn, cs) (one Jacobi rotation).
+template <class T>
+__device__ __forceinline__ void sym2_eigen(T a, T b, T c, T& l0, T& l1, T& cs, T& sn) {
+    if (b == T(0)) { l0 = a; l1 = c; cs = T(1); sn = T(0); return; }
+    const T tau = (c - a) / (T(2) * b);
+    const T t = (tau >= T(0) ? T(1) : T(-1)) / (fabs(tau) + sqrt(T(1) + tau * tau));
+    cs = T(1) / sqrt(T(1) + t * t);
+    sn = cs * t;
+    l0 = a - t * b;
+    l1 = c + t * b;
+}
+
 // SVD F = U diag(sig) V^T: cyclic Jacobi eigen-analysis of F^T F with exact
 // Givens rotations (the structure of McAdams et al. 2011, "Computing the SVD
 // of 3x3 matrices with minimal branching and elementary floating point

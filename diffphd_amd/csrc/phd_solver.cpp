@@ -30,6 +30,7 @@
 
 namespace phd {
 
+
 Body::PhaseTimer::PhaseTimer(Body* b_, int id_) : b(b_), id(id_), t0(0) {
     if (b->timing_level >= 2) {
         hip_check(hipStreamSynchronize(b->stream_), "phase sync");
@@ -89,6 +90,7 @@ static double det3(const double* A) {  // column-major
 Body::Body(int device) : device_(device) {
     hip_check(hipSetDevice(device_), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipHostMalloc((void**)&pin_small_, kPinSmallBytes, hipHostMallocDefault), "hipHostMalloc");
     rocblas_handle h;
     check(rocblas_create_handle(&h) == rocblas_status_success, "rocblas_create_handle failed");
     rocblas_set_stream(h, stream_);
@@ -96,44 +98,10 @@ Body::Body(int device) : device_(device) {
 }
 Body::~Body() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (pin_) (void)hipHostFree(pin_);
+    if (pin_small_) (void)hipHostFree(pin_small_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (blas_) rocblas_destroy_handle((rocblas_handle)blas_);
     if (stream_) (void)hipStreamDestroy(stream_);
-}
-
-void Body::pin_reserve(size_t bytes) {
-    if (bytes <= pin_cap_) return;
-    hip_check(hipStreamSynchronize(stream_), "pin reserve");
-    if (pin_) hip_check(hipHostFree(pin_), "hipHostFree");
-    pin_ = nullptr;
-    pin_cap_ = pin_off_ = 0;
-    hip_check(hipHostMalloc((void**)&pin_, bytes, hipHostMallocDefault), "hipHostMalloc");
-    pin_cap_ = bytes;
-}
-void Body::pin_reset() {
-    hip_check(hipStreamSynchronize(stream_), "pin reset");
-    pin_off_ = 0;
-}
-template <class T>
-T* Body::pin(size_t n) {
-    const size_t off = (pin_off_ + 255) / 256 * 256;
-    check(off + n * sizeof(T) <= pin_cap_, "pinned staging buffer exhausted");
-    pin_off_ = off + n * sizeof(T);
-    return reinterpret_cast<T*>(pin_ + off);
-}
-template <class T>
-void Body::up(T* dev, const T* host, size_t n) {
-    if (n == 0) return;
-    T* s = pin<T>(n);
-    std::memcpy(s, host, n * sizeof(T));
-    hip_check(hipMemcpyAsync(dev, s, n * sizeof(T), hipMemcpyHostToDevice, stream_), "staged upload");
-}
-template <class T>
-const T* Body::down(const T* dev, size_t n) {
-    T* s = pin<T>(n);
-    if (n) hip_check(hipMemcpyAsync(s, dev, n * sizeof(T), hipMemcpyDeviceToHost, stream_), "staged download");
-    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -651,9 +619,14 @@ void Body::setup_contact(int threads) {
     for (int c = 0; c < K_; ++c)
         hip_check(hipMemcpy(&Hdiag[c], d_H_.p + (size_t)c * K_ + c, sizeof(double), hipMemcpyDeviceToHost), "H diag");
     Hdiag_ = Hdiag;
-    for (auto* b : {&d_nrm_, &d_t1_, &d_t2_, &d_F_, &d_dF_, &d_uC_, &d_tmpK3_}) b->alloc((size_t)3 * K_);
-    for (auto* b : {&d_delta_, &d_om_, &d_E_, &d_rhsK_}) b->alloc(K_);
-    for (auto* b : {&d_idxJ_, &d_idxP_, &d_idxF_}) b->alloc(K_);
+    d_Hdiag_.upload(Hdiag);
+    for (auto* b : {&d_nrm_, &d_t1_, &d_t2_, &d_F_, &d_dF_, &d_uC_, &d_tmpK3_, &d_qframeC_, &d_Fg_, &d_Fn_, &d_hprev_})
+        b->alloc((size_t)3 * K_);
+    for (auto* b : {&d_delta_, &d_om_, &d_E_, &d_rhsK_, &d_rprec_, &d_lambda_, &d_lt1_, &d_lt2_, &d_hvec_}) b->alloc(K_);
+    for (auto* b : {&d_idxJ_, &d_idxP_, &d_idxF_, &d_posJ_, &d_posL_}) b->alloc(K_);
+    d_ncp_lists_.alloc((size_t)6 * K_);
+    d_ncp_counts_.alloc(8);
+    d_ncp_flags_.alloc(K_);
     d_rhsK32_.alloc(K_);
     d_info_.alloc(1);
 }
@@ -723,10 +696,10 @@ std::vector<double> Body::dots(const std::vector<std::pair<const double*, const 
     check(a.m <= 32, "too many dot products");
     for (int i = 0; i < a.m; ++i) { a.a[i] = pairs[i].first; a.b[i] = pairs[i].second; }
     launch_dots(stream_, dofs_, a, d_dot_partial_.p, d_dot_out_.p);
-    std::vector<double> out(a.m);
-    hip_check(hipMemcpyAsync(out.data(), d_dot_out_.p, sizeof(double) * a.m, hipMemcpyDeviceToHost, stream_), "dots D2H");
+    double* pd = reinterpret_cast<double*>(pin_small_);
+    hip_check(hipMemcpyAsync(pd, d_dot_out_.p, sizeof(double) * a.m, hipMemcpyDeviceToHost, stream_), "dots D2H");
     hip_check(hipStreamSynchronize(stream_), "dots sync");
-    return out;
+    return std::vector<double>(pd, pd + a.m);
 }
 
 void Body::kpd_apply(const double* x, double* out, int precision) {
@@ -855,31 +828,27 @@ void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) 
 // d_om_ / d_E_ / d_nrm_ current): assembled and solved by the blocked
 // Cholesky of phd_dense.hip. rhs (host, n) is overwritten with the solution.
 // Returns false when a pivot was not positive.
-bool Body::solve_contact_system(int n, int precision, double* rhs) {
+// The compressed contact system on the candidates J (n rows): its right-hand
+// side is in d_rhsK_, the solution stays on the device (d_cx32_ / d_cx64_);
+// only the pivot flag comes back. False when a pivot was not positive.
+bool Body::solve_contact_system(int n, int precision, const int* J) {
     const int np = chol_padded(n);
-    up(d_rhsK_.p, rhs, n);
     launch_clear_flag(stream_, d_info_.p);
-    const int* hf = nullptr;
     if (precision == 1) {
         if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
         if (d_cy32_.n < (size_t)chol_work(n)) { d_cy32_.alloc((size_t)chol_work(n)); d_cx32_.alloc(np); }
-        launch_chol_assemble<float>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
+        launch_chol_assemble<float>(stream_, n, J, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
         launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
-        const float* hx = down(d_cx32_.p, n);
-        hf = down(d_info_.p, 1);
-        hip_check(hipStreamSynchronize(stream_), "contact solve");
-        for (int a = 0; a < n; ++a) rhs[a] = hx[a];
     } else {
         if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
         if (d_cy64_.n < (size_t)chol_work(n)) { d_cy64_.alloc((size_t)chol_work(n)); d_cx64_.alloc(np); }
-        launch_chol_assemble<double>(stream_, n, d_idxJ_.p, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
+        launch_chol_assemble<double>(stream_, n, J, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
         launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
-        const double* hx = down(d_cx64_.p, n);
-        hf = down(d_info_.p, 1);
-        hip_check(hipStreamSynchronize(stream_), "contact solve");
-        std::memcpy(rhs, hx, sizeof(double) * n);
     }
-    return hf[0] == 0;
+    int* flag = reinterpret_cast<int*>(pin_small_);
+    hip_check(hipMemcpyAsync(flag, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "pivot flag");
+    hip_check(hipStreamSynchronize(stream_), "contact solve");
+    return flag[0] == 0;
 }
 
 // x = A^-1 b with optional FP64 refinement (test entry point; Dirichlet rows are identity).
@@ -1025,15 +994,12 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     d_q_.upload(q, n3);
     const int K = K_;
     const DevBoundary bnd = dev_boundary();
-    std::vector<double> nrm(3 * K), t1(3 * K), t2(3 * K), rprec(K), lambda(K, 0.0), lt1(K, 0.0), lt2(K, 0.0), omega_last(K, 0.0);
-    std::vector<double> F(3 * K, 0.0);
+    // per-candidate contact state lives on the device for the whole frame
+    std::vector<double> lambda(K, 0.0);
     const bool do_fric = K > 0 && fmu > 0;
     if (K > 0) {
         launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
-        hip_check(hipStreamSynchronize(stream_), "frames");
-        d_nrm_.download(nrm.data(), 3 * K);
-        d_t1_.download(t1.data(), 3 * K);
-        d_t2_.download(t2.data(), 3 * K);
+        std::vector<double> rprec(K);
         for (int c = 0; c < K; ++c) rprec[c] = std::max(h * h * Hdiag_[c], 1e-10);
         if (!prev_candidates_.empty() && prev_lambda_.size() == prev_candidates_.size()) {
             std::unordered_map<int, int> pidx;
@@ -1043,7 +1009,13 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 if (it != pidx.end()) lambda[c] = prev_lambda_[it->second];
             }
         }
+        d_rprec_.upload(rprec);
+        d_lambda_.upload(lambda);
+        d_lt1_.alloc(K); d_lt2_.alloc(K);
+        hip_check(hipMemsetAsync(d_lt1_.p, 0, sizeof(double) * K, stream_), "lt1=0");
+        hip_check(hipMemsetAsync(d_lt2_.p, 0, sizeof(double) * K, stream_), "lt2=0");
         hip_check(hipMemsetAsync(d_F_.p, 0, sizeof(double) * 3 * K, stream_), "F=0");
+        launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_qframeC_.p);  // vertex positions at the frame start
         if ((int)d_rhsK_.n < K) d_rhsK_.alloc(K);
     }
     const int warm = (int)opt_or(opt, "phd_warm_start", 0.0);
@@ -1055,13 +1027,14 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     bool dense64 = precision == 0;
     bool converged = false;
     int iters = 0;
-    std::vector<double> delta(K), uC(3 * K), hv(3 * K), om(K), E(K), hvec(K), rhs(K), Fg(3 * K), Fn(3 * K), qC(3 * K);
-    std::vector<double> q_frame_C(3 * K);
-    for (int c = 0; c < K; ++c)
-        for (int d = 0; d < 3; ++d) q_frame_C[3 * c + d] = q[3 * cand_[c] + d];
-    pin_reserve((size_t)K * 64 * sizeof(double) + (1 << 20));
+    const int set_masks[4] = {kNcpJ, kNcpP, kNcpF, kNcpG};
+    const int fric_masks[2] = {kNcpL, kNcpN};
+    int* const lists = d_ncp_lists_.p;  // J, P, F, G lists (K each), then the friction lists
+    const int* const listJ = lists;
+    const int* const listP = lists + K;
+    const int* const listF = lists + 2 * (size_t)K;
+    const int* const listG = lists + 3 * (size_t)K;
     for (int iter = 0; iter < max_pd_iter; ++iter) {
-        pin_reset();
         {
             PhaseTimer tl(this, kFwdLocal);
             local_rhs(d_q_.p, inv_h2m);
@@ -1078,190 +1051,71 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         if (K > 0) {
             PhaseTimer tc(this, kFwdContact);
             std::unique_ptr<PhaseTimer> tcs(new PhaseTimer(this, kFwdCStage));
-            // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev
+            // (A^-1 b)_C = q_C + (S^T S r)_C - H F_prev: the candidate rows of S^T y,
+            // the candidate positions and their obstacle distances
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
             launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
-            const double* huC = down(d_uC_.p, 3 * K);
-            const double* hdl = down(d_delta_.p, K);
-            const double* hqC = down(d_dF_.p, 3 * K);
-            hip_check(hipStreamSynchronize(stream_), "contact stage");
-            std::memcpy(uC.data(), huC, sizeof(double) * 3 * K);
-            std::memcpy(delta.data(), hdl, sizeof(double) * K);
-            std::memcpy(qC.data(), hqC, sizeof(double) * 3 * K);
-            for (int i = 0; i < 3 * K; ++i) uC[i] += qC[i];
             tcs.reset();
             std::unique_ptr<PhaseTimer> tcset(new PhaseTimer(this, kFwdCSets));
-            // omega, E, h (forward_gpu.cpp:1160-1197)
-            for (int c = 0; c < K; ++c) {
-                const double dj = delta[c], lj = lambda[c], rj = rprec[c];
-                double den = std::sqrt(dj * dj + rj * rj * lj * lj);
-                if (den < 1e-14) den = 1e-14;
-                om[c] = 1.0 - dj / den;
-                E[c] = (1.0 - rj * lj / den) * rj;
-                const double phi = dj + rj * lj - den;
-                const double Hq = nrm[3 * c] * qC[3 * c] + nrm[3 * c + 1] * qC[3 * c + 1] + nrm[3 * c + 2] * qC[3 * c + 2];
-                hvec[c] = -phi + om[c] * Hq;
-            }
-            omega_last = om;
-            // Candidates with omega == 0 (separated, lambda == 0) decouple exactly: their row of
-            // Omega W Omega + E is E_c e_c. The dense work runs on J = {omega != 0}; P adds the
-            // support of the previous impulses.
-            std::vector<int> Jv, Pv, Fs;
-            for (int c = 0; c < K; ++c) {
-                const bool inJ = om[c] != 0.0;
-                const bool inF = F[3 * c] != 0.0 || F[3 * c + 1] != 0.0 || F[3 * c + 2] != 0.0;
-                if (inJ) Jv.push_back(c);
-                if (inF) Fs.push_back(c);
-                if (inJ || inF) Pv.push_back(c);
-            }
-            const int nJ = (int)Jv.size(), nP = (int)Pv.size(), nF = (int)Fs.size();
+            // omega, E, h (forward_gpu.cpp:1160-1197) and the index sets. Candidates with
+            // omega == 0 (separated, lambda == 0) decouple exactly: their row of
+            // Omega W Omega + E is E_c e_c, so the dense work runs on J = {omega != 0};
+            // P adds the support of the previous impulses, G that of g's impulses.
+            launch_ncp_sets(stream_, K, d_delta_.p, d_lambda_.p, d_rprec_.p, d_nrm_.p, d_t1_.p, d_t2_.p, d_dF_.p, d_uC_.p,
+                            d_F_.p, d_lt1_.p, d_lt2_.p, do_fric ? 1 : 0, d_om_.p, d_E_.p, d_hvec_.p, d_Fg_.p,
+                            d_ncp_flags_.p);
+            launch_ncp_compact(stream_, K, d_ncp_flags_.p, 4, set_masks, lists, d_ncp_counts_.p, d_posJ_.p);
+            hip_check(hipMemcpyAsync(reinterpret_cast<int*>(pin_small_), d_ncp_counts_.p, 4 * sizeof(int), hipMemcpyDeviceToHost, stream_),
+                      "set sizes");
+            hip_check(hipStreamSynchronize(stream_), "set sizes");
+            const int* cnt = reinterpret_cast<const int*>(pin_small_);
+            const int nJ = cnt[0], nP = cnt[1], nF = cnt[2], nG = cnt[3];
             tcset.reset();
             std::unique_ptr<PhaseTimer> tcap(new PhaseTimer(this, kFwdCApply));
-            // H F_prev and H F_g are launched together and share one stream sync: stream order
-            // keeps the reused index and output buffers apart, and each result has its own
-            // pinned copy.
-            const double* h_prev = nullptr;
-            if (nF > 0) {
-                up(d_idxP_.p, Pv.data(), nP);
-                up(d_idxF_.p, Fs.data(), nF);
-                launch_h_apply_idx(stream_, nP, d_idxP_.p, nF, d_idxF_.p, d_H_.p, K, d_F_.p, d_tmpK3_.p);
-                h_prev = down(d_tmpK3_.p, 3 * nP);
-            }
-            const double* h_g = nullptr;
+            // uC -= H F_prev on P; the compressed system's rhs on J with (A^-1 g)_C =
+            // (A^-1 b)_C + H F_g, F_g = omega lambda n + lt1 t1 + lt2 t2 (zero sums
+            // when a column set is empty)
+            launch_h_apply_idx(stream_, nP, listP, nF, listF, d_H_.p, K, d_F_.p, d_hprev_.p);
+            launch_h_apply_idx(stream_, nJ, listJ, nG, listG, d_H_.p, K, d_Fg_.p, d_tmpK3_.p);
+            launch_ncp_rhs(stream_, nP, listP, d_hprev_.p, d_posJ_.p, d_tmpK3_.p, d_hvec_.p, d_om_.p, d_nrm_.p, d_uC_.p,
+                           d_rhsK_.p);
+            tcap.reset();
+            int solved_prec = -1;  // precision of the solution in d_cx32_ / d_cx64_
             if (nJ > 0) {
-                // (A^-1 g)_C = (A^-1 b)_C + H F_g with F_g = omega lambda n + lt1 t1 + lt2 t2. The
-                // friction part can sit on candidates outside J (lambda > 0 with omega rounded to 0).
-                std::fill(Fg.begin(), Fg.end(), 0.0);
-                std::vector<int> Gv;
-                for (int c = 0; c < K; ++c) {
-                    const bool fr = do_fric && (lt1[c] != 0.0 || lt2[c] != 0.0);
-                    if (om[c] * lambda[c] == 0.0 && !fr) continue;
-                    Gv.push_back(c);
-                    for (int d = 0; d < 3; ++d)
-                        Fg[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (fr ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
-                }
-                const int nG = (int)Gv.size();
-                up(d_idxJ_.p, Jv.data(), nJ);
-                if (nG > 0) {
-                    up(d_dF_.p, Fg.data(), 3 * K);
-                    up(d_idxF_.p, Gv.data(), nG);
-                    launch_h_apply_idx(stream_, nJ, d_idxJ_.p, nG, d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
-                    h_g = down(d_tmpK3_.p, 3 * nJ);
-                }
-            }
-            if (h_prev || h_g) hip_check(hipStreamSynchronize(stream_), "HF_prev, HFg");
-            if (h_prev)
-                for (int a = 0; a < nP; ++a)
-                    for (int d = 0; d < 3; ++d) uC[3 * Pv[a] + d] -= h_prev[3 * a + d];
-            for (int c = 0; c < K; ++c) rhs[c] = hvec[c];
-            if (nJ > 0) {
-                if (h_g) std::memcpy(hv.data(), h_g, sizeof(double) * 3 * nJ);
-                else std::fill(hv.begin(), hv.end(), 0.0);
-                std::vector<double> rJ(nJ);
-                for (int a = 0; a < nJ; ++a) {
-                    const int c = Jv[a];
-                    double val = 0;
-                    for (int d = 0; d < 3; ++d) val += nrm[3 * c + d] * (uC[3 * c + d] + hv[3 * a + d]);
-                    rJ[a] = hvec[c] - om[c] * val;
-                }
-                d_om_.alloc(K);
-                d_E_.alloc(K);
-                up(d_om_.p, om.data(), K);
-                up(d_E_.p, E.data(), K);
-                std::vector<double> rJ_keep = rJ;
-                tcap.reset();
                 PhaseTimer td(this, kFwdDense);
                 if (timing_level >= 2) { phase[kDenseCalls] += 1; phase[kDenseRows] += nJ; }
-                bool solved = false;
                 if (precision == 1 && !dense64) {
-                    solved = solve_contact_system(nJ, 1, rJ.data());
-                    if (!solved) {
+                    if (solve_contact_system(nJ, 1, listJ)) solved_prec = 1;
+                    else {
                         dense64 = true;  // the FP32 Cholesky broke down: FP64 for the rest of the frame
                         if (timing_level >= 2) phase[kDenseFp32Fail] += 1;
                     }
                 }
-                if (!solved) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
-                    rJ = rJ_keep;
-                    check(solve_contact_system(nJ, 0, rJ.data()),
+                if (solved_prec < 0) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
+                    check(solve_contact_system(nJ, 0, listJ),
                           "Dense Cholesky of the contact system failed (matrix not positive definite).");
+                    solved_prec = 0;
                 }
-                for (int a = 0; a < nJ; ++a) rhs[Jv[a]] = rJ[a];
-                if (verbose > 1)
-                    for (int a = 0; a < nJ; ++a) {
-                        const int c = Jv[a];
-                        fprintf(stderr, "contact %d c=%d d=%.17g om=%.17g E=%.17g h=%.17g nu=%.17g nhv=%.17g Hcc=%.17g dl=%.17g\n", iter, c,
-                                delta[c], om[c], E[c], hvec[c],
-                                nrm[3 * c] * uC[3 * c] + nrm[3 * c + 1] * uC[3 * c + 1] + nrm[3 * c + 2] * uC[3 * c + 2],
-                                nrm[3 * c] * hv[3 * a] + nrm[3 * c + 1] * hv[3 * a + 1] + nrm[3 * c + 2] * hv[3 * a + 2], Hdiag_[c], rJ[a]);
-                    }
             }
-            for (int c = 0; c < K; ++c) {
-                const double dl = (om[c] != 0.0) ? rhs[c] : rhs[c] / E[c];
-                lambda[c] = std::min(std::max(lambda[c] + dl, 0.0), 1e8);
-            }
-            tcap.reset();
+            std::unique_ptr<PhaseTimer> tlam(new PhaseTimer(this, kFwdCApply));
+            if (solved_prec == 1) launch_ncp_lambda<float>(stream_, K, d_posJ_.p, d_cx32_.p, d_hvec_.p, d_E_.p, d_om_.p, d_lambda_.p);
+            else launch_ncp_lambda<double>(stream_, K, d_posJ_.p, d_cx64_.p, d_hvec_.p, d_E_.p, d_om_.p, d_lambda_.p);
+            tlam.reset();
             // NCP-integrated Coulomb friction (:1255-1302)
             PhaseTimer tfr(this, kFwdFric);
             if (do_fric) {
-                std::vector<int> Lp_, Jn;
-                for (int c = 0; c < K; ++c) {
-                    if (lambda[c] > 0.0) Lp_.push_back(c);
-                    if (om[c] * lambda[c] != 0.0) Jn.push_back(c);
-                }
-                std::vector<int> slot(K, -1);
-                std::fill(Fn.begin(), Fn.end(), 0.0);
-                for (int c : Jn)
-                    for (int d = 0; d < 3; ++d) Fn[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d];
-                if (!Lp_.empty() && !Jn.empty()) {
-                    up(d_dF_.p, Fn.data(), 3 * K);
-                    up(d_idxP_.p, Lp_.data(), Lp_.size());
-                    up(d_idxF_.p, Jn.data(), Jn.size());
-                    launch_h_apply_idx(stream_, (int)Lp_.size(), d_idxP_.p, (int)Jn.size(), d_idxF_.p, d_H_.p, K, d_dF_.p, d_tmpK3_.p);
-                    const double* h = down(d_tmpK3_.p, 3 * Lp_.size());
-                    hip_check(hipStreamSynchronize(stream_), "HFn");
-                    std::memcpy(hv.data(), h, sizeof(double) * 3 * Lp_.size());
-                } else {
-                    std::fill(hv.begin(), hv.end(), 0.0);
-                }
-                for (size_t a = 0; a < Lp_.size(); ++a) slot[Lp_[a]] = (int)a;
-                for (int c = 0; c < K; ++c) {
-                    if (lambda[c] <= 0.0) { lt1[c] = lt2[c] = 0.0; continue; }
-                    double step[3];
-                    for (int d = 0; d < 3; ++d) step[d] = uC[3 * c + d] + hv[3 * slot[c] + d] - q_frame_C[3 * c + d];
-                    const double* n = &nrm[3 * c];
-                    const double sn = step[0] * n[0] + step[1] * n[1] + step[2] * n[2];
-                    double ut[3];
-                    for (int d = 0; d < 3; ++d) ut[d] = step[d] - sn * n[d];
-                    const double u1 = ut[0] * t1[3 * c] + ut[1] * t1[3 * c + 1] + ut[2] * t1[3 * c + 2];
-                    const double u2 = ut[0] * t2[3 * c] + ut[1] * t2[3 * c + 1] + ut[2] * t2[3 * c + 2];
-                    // W_tt = (t_a . t_b) H_cc: diagonal for orthonormal tangents
-                    const double w11 = Hdiag_[c] * (t1[3 * c] * t1[3 * c] + t1[3 * c + 1] * t1[3 * c + 1] + t1[3 * c + 2] * t1[3 * c + 2]);
-                    const double w12 = Hdiag_[c] * (t1[3 * c] * t2[3 * c] + t1[3 * c + 1] * t2[3 * c + 1] + t1[3 * c + 2] * t2[3 * c + 2]);
-                    const double w22 = Hdiag_[c] * (t2[3 * c] * t2[3 * c] + t2[3 * c + 1] * t2[3 * c + 1] + t2[3 * c + 2] * t2[3 * c + 2]);
-                    const double det = w11 * w22 - w12 * w12;
-                    double s1 = 0, s2 = 0;
-                    if (std::fabs(det) > 1e-20) { s1 = (w22 * -u1 - w12 * -u2) / det; s2 = (-w12 * -u1 + w11 * -u2) / det; }
-                    const double cone = fmu * lambda[c];
-                    if (std::sqrt(s1 * s1 + s2 * s2) <= cone + 1e-14) { lt1[c] = s1; lt2[c] = s2; }
-                    else {
-                        const double um = std::sqrt(u1 * u1 + u2 * u2);
-                        if (um > 1e-14) { lt1[c] = -cone * u1 / um; lt2[c] = -cone * u2 / um; }
-                        else lt1[c] = lt2[c] = 0.0;
-                    }
-                }
+                int* flist = lists + 4 * (size_t)K;
+                launch_fric_sets(stream_, K, d_lambda_.p, d_om_.p, d_nrm_.p, d_Fn_.p, d_ncp_flags_.p);
+                launch_ncp_compact(stream_, K, d_ncp_flags_.p, 2, fric_masks, flist, d_ncp_counts_.p + 4, d_posL_.p);
+                launch_h_apply_idx_dev(stream_, K, d_ncp_counts_.p + 4, flist, d_ncp_counts_.p + 5, flist + K, d_H_.p, K,
+                                       d_Fn_.p, d_tmpK3_.p);
+                launch_fric_coulomb(stream_, K, d_lambda_.p, d_posL_.p, d_tmpK3_.p, d_uC_.p, d_qframeC_.p, d_nrm_.p,
+                                    d_t1_.p, d_t2_.p, d_Hdiag_.p, fmu, d_lt1_.p, d_lt2_.p);
             }
             // new impulses; the S pass carries only the change
-            std::vector<double> Fnew(3 * K), dF(3 * K);
-            for (int c = 0; c < K; ++c)
-                for (int d = 0; d < 3; ++d) {
-                    Fnew[3 * c + d] = om[c] * lambda[c] * nrm[3 * c + d] + (do_fric ? lt1[c] * t1[3 * c + d] + lt2[c] * t2[3 * c + d] : 0.0);
-                    dF[3 * c + d] = Fnew[3 * c + d] - F[3 * c + d];
-                }
-            F = Fnew;
-            d_F_.alloc(3 * K);
-            up(d_dF_.p, dF.data(), 3 * K);
-            up(d_F_.p, F.data(), 3 * K);
+            launch_ncp_impulses(stream_, K, d_om_.p, d_lambda_.p, d_nrm_.p, d_lt1_.p, d_lt2_.p, d_t1_.p, d_t2_.p,
+                                do_fric ? 1 : 0, d_F_.p, d_dF_.p);
             if (precision == 1) contact_scatter<float>(d_dF_.p, y32_); else contact_scatter<double>(d_dF_.p, y64_);
         }
         {
@@ -1291,6 +1145,14 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     hip_check(hipStreamSynchronize(stream_), "forward");
     std::vector<double> qs(n3);
     d_q_.download(qs.data(), n3);
+    std::vector<double> nrm(3 * K), lt1(K, 0.0), lt2(K, 0.0), omega_last(K, 0.0);
+    if (K > 0) {
+        d_lambda_.download(lambda.data(), K);
+        d_lt1_.download(lt1.data(), K);
+        d_lt2_.download(lt2.data(), K);
+        d_om_.download(omega_last.data(), K);
+        d_nrm_.download(nrm.data(), 3 * K);
+    }
     if (!converged) fprintf(stderr, "[Warning] alg_phd: PD loop did not converge.\n");
     // restitution post-pass (:1483-1531)
     if (restitution > 0 && K > 0) {
@@ -1360,6 +1222,13 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     setup(dt, threads);
     const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
     if (exact != kpd_exact_) { kpd_exact_ = exact; build_materials(); }
+    const int kfilter = (int)opt_or(opt, "alg_phd_bwd_kpd_filter", 0.0);
+    const double trust_rho = opt_or(opt, "alg_phd_bwd_trust_radius", 0.9);
+    check(kfilter >= 0 && kfilter <= 3, "alg_phd_bwd_kpd_filter must be 0 (none), 1 (clamp), 2 (abs) or 3 (trust region).");
+    check(kfilter != 3 || (trust_rho > 0 && trust_rho < 1), "alg_phd_bwd_trust_radius must lie in (0, 1).");
+    const bool state_wb = opt_or(opt, "alg_phd_bwd_state_woodbury", 0.0) > 0.5;
+    last_bwd = BackwardStats();
+    last_bwd.samples = (double)ne_ * samples_;
     PhaseTimer t_total(this, kBwdTotal);
     const int n3 = dofs_;
     const double h = dt, inv_h = 1 / h, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
@@ -1380,10 +1249,27 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     if (precision == 1) { d_W32_.alloc((size_t)ne_ * samples_ * 81); d_W64_.release(); }
     else { d_W64_.alloc((size_t)ne_ * samples_ * 81); d_W32_.release(); }
     const double* kn[4] = {d_kn_[0].p, d_kn_[1].p, d_kn_[2].p, d_kn_[3].p};
-    std::unique_ptr<PhaseTimer> t_kpd(new PhaseTimer(this, kBwdKpd));
-    if (precision == 1) launch_kpd_build<float>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W32_.p);
-    else launch_kpd_build<double>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, d_W64_.p);
-    t_kpd.reset();
+    // Proximal-map Jacobians, filtered per sample. Trust region (mode 3): stage 0
+    // is the unfiltered operator; the backbone moves to stage 1 (clamp) and 2
+    // (abs), both bounded below by -rho, when its fixed point diverges.
+    int stage = kfilter == 3 ? 0 : 1;
+    d_nfilt_.alloc(1);
+    auto build_kpd = [&]() {
+        PhaseTimer t_kpd(this, kBwdKpd);
+        double theta = 1.0, lo = -INFINITY;
+        int active = kfilter != 0 && !(kfilter == 3 && stage == 0);
+        if (kfilter == 1) theta = 0.0;
+        if (kfilter == 2) theta = -1.0;
+        if (kfilter == 3) { theta = stage == 1 ? 0.0 : -1.0; lo = -trust_rho; }
+        hip_check(hipMemsetAsync(d_nfilt_.p, 0, sizeof(int), stream_), "filter count");
+        if (precision == 1)
+            launch_kpd_build<float>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p,
+                                    active, theta, lo, d_nfilt_.p, d_W32_.p);
+        else
+            launch_kpd_build<double>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p,
+                                     active, theta, lo, d_nfilt_.p, d_W64_.p);
+    };
+    build_kpd();
     // tape
     ReplayFrame replay;
     bool from_tape = false;
@@ -1428,7 +1314,13 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     std::vector<int> b_c;            // candidate index of each basis row
     std::vector<double> b_dir;       // 3 per basis row
     // one fixed point  s <- A_c^-1 (rhs + K s)  in residual form with AA
-    auto fixed_point = [&](int max_it, int* used) {
+    // div (may be null): stop early and set it when the iteration diverges: a
+    // non-finite residual, an iterate at the clip norm, or a residual 1e3 x its
+    // smallest so far after 10 iterations
+    auto fixed_point = [&](int max_it, int* used, bool* conv, bool* div) {
+        *conv = false;
+        if (div) *div = false;
+        double err_min = INFINITY;
         hip_check(hipMemsetAsync(d_s.p, 0, sizeof(double) * n3, stream_), "s=0");
         DevAnderson aa;
         aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
@@ -1441,12 +1333,15 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             if (nb > 0) {
-                // s <- s - A^-1 J^T G^-1 J A^-1 (...) on the device: project the candidate
-                // vectors on the basis, apply G^-1 (formed once per frame, FP32 copy in mixed
-                // mode), expand back to candidate impulses and scatter through S_C
+                // s <- s + A^-1 r - A^-1 J^T G^-1 J (A^-1 r + s) on the device: project the
+                // candidate vectors on the basis, apply G^-1 (formed once per frame, FP32
+                // copy in mixed mode), expand back to candidate impulses and scatter through
+                // S_C. The constraint residual J s rides along, so with an FP32 factor and
+                // G^-1 the iterates converge onto J s = 0 instead of drifting off it.
                 if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
                 PhaseTimer tbs(this, kBwdBasisSolve);
-                launch_basis_project(stream_, nb, d_bcidx.p, d_dirs.p, d_uC_.p, d_rhsK_.p);
+                launch_gather_nodes(stream_, K_, d_cand_.p, d_sprev.p, d_tmpK3_.p);
+                launch_basis_project(stream_, nb, d_bcidx.p, d_dirs.p, d_uC_.p, d_tmpK3_.p, d_rhsK_.p);
                 if (precision == 1) launch_gemv_sym<float>(stream_, nb, gld, d_Gs32.p, d_rhsK_.p, d_Gx.p);
                 else launch_gemv_sym<double>(stream_, nb, gld, d_Gs64.p, d_rhsK_.p, d_Gx.p);
                 launch_basis_expand(stream_, K_, d_bcptr.p, d_bclist.p, d_dirs.p, d_Gx.p, d_dF_.p);
@@ -1466,28 +1361,60 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             }
             it_used = i + 1;
             const double sn = std::sqrt(dots({{d_s.p, d_s.p}})[0]);
-            if (i > 0 && err < rel_tol * sn + abs_tol) break;
+            if (i > 0 && err < rel_tol * sn + abs_tol) { *conv = true; break; }
+            if (div) {
+                err_min = std::min(err_min, err);
+                if (!std::isfinite(err) || (clip > 0 && sn >= 0.999 * clip) || (i >= 10 && err > 1e3 * err_min)) {
+                    *div = true;
+                    break;
+                }
+            }
         }
         if (used) *used = it_used;
     };
     int bb = 0;
+    bool bb_conv = false;
     {
         PhaseTimer tb(this, kBwdBackbone);
-        fixed_point(max_pd_iter, &bb);
+        int total = 0;
+        for (;;) {
+            bool div = false;
+            fixed_point(max_pd_iter, &bb, &bb_conv, (kfilter == 3 && stage < 2) ? &div : nullptr);
+            total += bb;
+            if (!div) break;
+            ++stage;
+            build_kpd();
+        }
+        bb = total;
+        last_bwd.trust_stage = kfilter == 3 ? stage : 0;
     }
     last_backward_iters = bb;
     std::vector<double> db(n3);
     hip_check(hipStreamSynchronize(stream_), "backbone");
     d_s.download(db.data(), n3);
+    {
+        int nf = 0;
+        d_nfilt_.download(&nf, 1);
+        last_bwd.filtered = nf;
+    }
     const bool diverged = clip > 0 && nrm2(db) >= clip * 0.999;
-    if (!diverged) {
+    last_bwd.backbone_iters = bb;
+    last_bwd.backbone_converged = bb_conv ? 1 : 0;
+    last_bwd.clipped = diverged ? 1 : 0;
+    last_bwd.db_norm = nrm2(db);
+    // Cascade injection (backward_gpu.cpp:878-916) from the backbone adjoint db,
+    // or, with alg_phd_bwd_state_woodbury = 1 (no reference counterpart), from
+    // the Woodbury-corrected adjoint z: the exact adjoint of the step with its
+    // active contacts held (the reference routes only dl_dmat_w through z).
+    auto inject = [&](const std::vector<double>& src) {
         std::vector<double> inj(n3);
-        for (int i = 0; i < n3; ++i) inj[i] = inv_h2m * db[i];
+        for (int i = 0; i < n3; ++i) inj[i] = inv_h2m * src[i];
         const double cap = nrm2(rhs_qbar) + 1.0, in = nrm2(inj);
         if (in > cap) for (double& t : inj) t *= cap / in;
         for (int i = 0; i < n3; ++i) { dl_dq[i] += inj[i]; dl_df_ext[i] += h2m * inj[i]; dl_dv[i] += h * inj[i]; }
         if (gravity_.size() == 3) for (int i = 0; i < n3; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
-    }
+    };
+    if (!diverged && !state_wb) inject(db);
     std::vector<double> z = db;
     const bool skip_wb = opt_or(opt, "skip_contact_woodbury", 0.0) > 0.5;
     const int nl = (int)replay.active.size();
@@ -1592,14 +1519,21 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 tbf.reset();
                 PhaseTimer twb(this, kBwdWoodbury);
                 int used = 0;
+                bool wconv = false;
+                // The reference caps each of its per-basis fixed points at
+                // min(max_pd_iter, alg_phd_bwd_wt_max_iter) (:1073-1075); the one
+                // constrained fixed point here takes the same cap.
                 const int wt_cap = (int)opt_or(opt, "alg_phd_bwd_wt_max_iter", 120);
-                (void)wt_cap;
-                fixed_point(max_pd_iter, &used);
+                fixed_point(std::min(max_pd_iter, std::max(1, wt_cap)), &used, &wconv, nullptr);
+                last_bwd.woodbury_iters = used;
+                last_bwd.woodbury_converged = wconv ? 1 : 0;
+                last_bwd.basis_rows = nb;
                 hip_check(hipStreamSynchronize(stream_), "woodbury");
                 d_s.download(z.data(), n3);
             }
         }
     }
+    if (!diverged && state_wb) inject(z);
     // dl/d(mat_w)
     const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
     if (!skip_mat && !diverged) {

@@ -140,6 +140,7 @@ public:
     int dofs() const { return dofs_; }
     int num_elements() const { return ne_; }
     int num_energies() const { return (int)energies_.size(); }
+    int num_boundary_vertices() const { return (int)fb_vertices_.size(); }
 
     // ---- hot path ----
     void forward(const double* q, const double* v, const double* f_ext, double dt,
@@ -156,6 +157,10 @@ public:
     std::vector<StepStats> frame_stats;
     std::vector<double> last_lambda_, last_lt1_, last_lt2_;
     int last_backward_iters = 0;
+    struct BackwardStats {  // of the last backward call
+        double backbone_iters = 0, backbone_converged = 0, clipped = 0, woodbury_iters = 0, woodbury_converged = 0,
+               basis_rows = 0, filtered = 0, samples = 0, db_norm = 0, trust_stage = 0;
+    } last_bwd;
     long long factor_bytes() const;
     long long factor_nnz() const { return factor_.nnz; }
     int cached_n() const { return ready_ ? nv_ : -1; }
@@ -196,16 +201,10 @@ public:
 private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
-    // Pinned staging of the per-iteration host <-> device vectors: bump-allocated
-    // from one hipHostMalloc buffer, reset (after a stream synchronization) at the
-    // top of every fixed-point iteration.
-    char* pin_ = nullptr;
-    size_t pin_cap_ = 0, pin_off_ = 0;
-    void pin_reserve(size_t bytes);
-    void pin_reset();
-    template <class T> T* pin(size_t n);
-    template <class T> void up(T* dev, const T* host, size_t n);
-    template <class T> const T* down(const T* dev, size_t n);
+    // Pinned host memory for the few scalars read back per iteration (the
+    // contact set sizes, the dense solve's pivot flag, the dot products).
+    static constexpr size_t kPinSmallBytes = 4096;
+    void* pin_small_ = nullptr;
     // scene
     int edim_ = 8, nv_ = 0, ne_ = 0, dofs_ = 0, samples_ = 8;
     std::vector<double> X_;
@@ -278,6 +277,10 @@ private:
     DevBuf<double> d_piece_part_;
     DevBuf<double> d_waves_;
     DevBuf<double> d_nrm_, d_t1_, d_t2_, d_delta_, d_F_, d_dF_, d_uC_, d_tmpK3_, d_om_, d_E_;
+    // per-candidate NCP state of the frame (device-resident)
+    DevBuf<double> d_rprec_, d_lambda_, d_lt1_, d_lt2_, d_qframeC_, d_hvec_, d_Fg_, d_Fn_, d_hprev_, d_Hdiag_;
+    DevBuf<unsigned char> d_ncp_flags_;
+    DevBuf<int> d_ncp_lists_, d_ncp_counts_, d_posJ_, d_posL_;
     DevBuf<double> d_C64_, d_cy64_, d_cx64_;  // dense contact system (blocked Cholesky layout)
     DevBuf<float> d_C32_, d_cy32_, d_cx32_;
     DevBuf<double> d_rhsK_;
@@ -292,6 +295,7 @@ private:
     // backward
     DevBuf<double> d_W64_;
     DevBuf<float> d_W32_;
+    DevBuf<int> d_nfilt_;  // samples whose proximal-map block the filter changed
     DevBuf<double> d_gcor_, d_gvol_, d_gnh_;
     // state kept between calls (mutable members of the reference Deformable)
     std::vector<ReplayFrame> tape_;
@@ -315,7 +319,7 @@ private:
     std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
     bool anderson_step(DevAnderson& aa, const double* xprev, const double* g, double* out,
                        const std::vector<std::pair<const double*, const double*>>& extra, std::vector<double>* extra_out);
-    bool solve_contact_system(int n, int precision, double* rhs_inout);
+    bool solve_contact_system(int n, int precision, const int* J);
 };
 
 }  // namespace phd

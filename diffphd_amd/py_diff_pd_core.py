@@ -267,6 +267,13 @@ class _Deformable:
     def last_backward_iters(self):
         return _lib.lib().phd_last_backward_iters(self._ctx)
 
+    def last_backward_stats(self):
+        out, op = _lib.dptr(np.zeros(10))
+        _lib.lib().phd_last_backward_stats(self._ctx, op, 10)
+        keys = ("backbone_iters", "backbone_converged", "clipped", "woodbury_iters", "woodbury_converged",
+                "basis_rows", "filtered", "samples", "db_norm", "trust_stage")
+        return dict(zip(keys, out.tolist()))
+
     def last_contact_multipliers(self):
         cap = self._dofs // 3 + 1
         a, ap = _lib.dptr(np.zeros(cap))

@@ -63,14 +63,25 @@ int phd_apply_ainv(phd_ctx ctx, double dt, const double* b, double* x, int preci
 /* One PD step, ForwardProjectiveDynamicsAlgPhd (forward_gpu.cpp:387-1594),
  * options as the reference's std::map<string, real> (keys/vals, n entries;
  * required: max_pd_iter, abs_tol, rel_tol, verbose, thread_ct). active: caller
- * buffer of capacity *n_active on entry; on exit *n_active = number of contact
- * nodes with lambda > 0 written to active. */
+ * buffer of capacity *n_active on entry, at least the number of frictional-
+ * boundary vertices (checked before the step); on exit *n_active = number of
+ * contact nodes with lambda > 0 written to active. */
 int phd_step_forward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
                      const char* const* keys, const double* vals, int n, double* q_next, double* v_next, int* active,
                      int* n_active);
 
 /* Adjoint of one PD step, BackwardProjectiveDynamicsAlgPhd
- * (deformable_projective_dynamics_backward_gpu.cpp:648-1250). Outputs (3N each):
+ * (deformable_projective_dynamics_backward_gpu.cpp:648-1250). Options beyond the
+ * reference's: alg_phd_bwd_kpd_filter (0 none = the reference's K_pd, 1 clamp,
+ * 2 abs: the eigenvalue filters of deformable_newton_forward.cpp:393-433 applied
+ * to each sample's proximal-map block; 3 trust region: the unfiltered
+ * operator first, then, if the backbone fixed point diverges, the clamp and
+ * then the abs filter with the block's spectrum relative to the sample's weight
+ * in A bounded below by -rho, which makes the backbone a contraction),
+ * alg_phd_bwd_trust_radius (rho, default 0.9), alg_phd_bwd_state_woodbury
+ * (1: dl_dq, dl_dv, dl_df_ext from the
+ * Woodbury-corrected adjoint, the exact adjoint with the active contacts held;
+ * default 0 keeps the reference's routing). Outputs (3N each):
  * dl_dq, dl_dv, dl_df_ext; dl_dmat_w (one per PD element energy); dl_dstate_p
  * (3 when gravity is set); dl_dw_elem (energies x elements, may be NULL): the
  * per-element stiffness gradient whose sum over an energy's elements is dl_dmat_w. */
@@ -93,6 +104,15 @@ int phd_cached_n(phd_ctx ctx);
 int phd_frame_stats(phd_ctx ctx, int* iters, int* converged, double* seconds, int cap);
 void phd_reset_frame_stats(phd_ctx ctx);
 int phd_last_backward_iters(phd_ctx ctx);
+/* Last backward step (no reference counterpart; the reference prints these):
+ * [0] backbone fixed-point iterations, [1] backbone converged (0/1), [2] backbone
+ * clipped, i.e. |db| reached 0.999 x alg_phd_bwd_clip_norm so the cascade, the
+ * Woodbury correction and dl_dmat_w were skipped (backward_gpu.cpp:809-866),
+ * [3] Woodbury fixed-point iterations, [4] Woodbury converged, [5] basis rows,
+ * [6] samples whose proximal-map block the K_pd filter changed, [7] samples,
+ * [8] |db|, [9] trust-region stage that produced db (0 unfiltered, 1 clamp,
+ * 2 abs). Returns the number of entries (10). [0] counts every stage. */
+int phd_last_backward_stats(phd_ctx ctx, double* out, int cap);
 /* Factor description: out[0] nnz(S), [1] padded value count, [2] max etree depth,
  * [3] ordering seconds, [4] factor seconds, [5] setup seconds, [6] row units,
  * [7] column units, [8] nnz(L), [9] contact candidates K, [10] entries of the

@@ -57,3 +57,26 @@ def test_batch_matches_sequential_hip_and_oracle():
         relg = np.linalg.norm(x["grad_v0"] - y["grad_v0"]) / np.linalg.norm(y["grad_v0"])
         print("beam q_T vs oracle", rel, "grad", relg)
         assert rel < 1e-8 and relg < 1e-6
+
+
+@pytest.mark.gpu
+def test_frictional_batch_matches_oracle_at_fixed_iterations():
+    """The batched workload's beams with their ground friction 0.5: four beams
+    rolled out concurrently on the GPU (one stream each) against the oracle,
+    one frame and its adjoint at a fixed 16 PD / 16 adjoint iterations, so both
+    sides take the same NCP + Coulomb + Anderson steps (every active contact's
+    lambda is >= 3e-2 of the largest, away from the complementarity boundary)."""
+    import oracle
+    from diffphd_amd.py_diff_pd_core import HexDeformable
+    oracle.build()
+    specs = scenes.beam_batch(4, seed=1, dims=((3, 5), (3, 5), (2, 3)))
+    fixed = {"rel_tol": 0.0, "abs_tol": 0.0, "max_pd_iter": 16, "alg_phd_bwd_wt_max_iter": 16}
+    hip = rollout_batch(specs, lambda: HexDeformable(0), frames=1, options=fixed, workers=4)
+    orc = rollout_batch(specs, lambda: oracle.OracleDeformable(8), frames=1, options=fixed, workers=1)
+    for s, x, y in zip(specs, hip, orc):
+        assert s["options"]["friction_mu"] == 0.5
+        rel = np.linalg.norm(x["q_T"] - y["q_T"]) / np.linalg.norm(y["q_T"])
+        relg = np.linalg.norm(x["grad_v0"] - y["grad_v0"]) / np.linalg.norm(y["grad_v0"])
+        relw = np.linalg.norm(x["dl_dmat_w"] - y["dl_dmat_w"]) / np.linalg.norm(y["dl_dmat_w"])
+        print("frictional beam q_T", rel, "dl/dv0", relg, "dl/dmat_w", relw)
+        assert rel < 1e-10 and relg < 1e-8 and relw < 1e-8
