@@ -49,8 +49,16 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_steps(body, spec, opt, start_q, start_v, frame0, K):
-    """K forward frames from (start_q, start_v) then the K backward frames."""
+# Backward options of the measured path: per-element stiffness in K_pd and the
+# trust region on the proximal-map spectrum (the reference's mean-stiffness K_pd
+# makes the backbone adjoint diverge to the clip norm on this scene).
+BWD_OPTS = {"alg_phd_bwd_exact_kpd": 1, "alg_phd_bwd_kpd_filter": 3}
+
+
+def run_steps(body, spec, opt, start_q, start_v, frame0, K, bwd_opt=None):
+    """K forward frames from (start_q, start_v) then the K backward frames.
+    Returns states, active sets, the gradients at the window start and the
+    per-frame backward statistics (newest frame first)."""
     from diffphd_amd.py_diff_pd_core import StdRealVector
     dofs = body.dofs()
     f = np.zeros(dofs)
@@ -66,14 +74,21 @@ def run_steps(body, spec, opt, start_q, start_v, frame0, K):
     n = q[-1].size
     dq = 2.0 * (q[-1] - spec["target"]) / n  # loss = mean squared distance to the target
     dv = np.zeros(n)
+    bopt = dict(opt, **(bwd_opt or {}))
+    bstats = []
     for i in reversed(range(K)):
         body.SetPressPlaneHeight(spec["plane_schedule"][frame0 + i])
         o = [StdRealVector(0) for _ in range(8)]
         body.PyBackward("pd_eigen_alg_phd", q[i], v[i], [], f, spec["dt"], q[i + 1], v[i + 1], acts[i + 1], dq, dv,
-                        opt, *o)
+                        bopt, *o)
         dq, dv = np.asarray(o[0]), np.asarray(o[1])
-        log(f"backward frame {frame0 + i}: |dl_dq| {np.linalg.norm(dq):.3e}")
-    return q, v, acts
+        st = body.last_backward_stats() if hasattr(body, "last_backward_stats") else {}
+        st["frame"] = frame0 + i
+        bstats.append(st)
+        log(f"backward frame {frame0 + i}: |dl_dq| {np.linalg.norm(dq):.3e} "
+            + " ".join(f"{k} {st[k]:g}" for k in ("backbone_iters", "woodbury_iters", "trust_stage", "clipped",
+                                                   "filtered", "basis_rows") if k in st))
+    return q, v, acts, dq, dv, bstats
 
 
 def prepare_state(spec, args):
@@ -108,39 +123,41 @@ def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False):
     log(f"{precision}: setup {t_setup:.1f}s {body.factor_info()}")
     q, v = [q0], [v0]
     if W > 0:
-        q, v, _ = run_steps(body, spec, opt, q0, v0, frame0, W)
+        q, v, _, _, _, _ = run_steps(body, spec, opt, q0, v0, frame0, W, BWD_OPTS)
     body.reset_frame_stats()
     body.timing_enable(2 if phases else 1)
     body.timing_read(reset=True)
     body.phase_times(reset=True)
     t0 = time.perf_counter()
-    qq, vv, acts = run_steps(body, spec, opt, q[-1], v[-1], frame0 + W, K)
+    qq, vv, acts, dq0, dv0, bstats = run_steps(body, spec, opt, q[-1], v[-1], frame0 + W, K, BWD_OPTS)
     elapsed = time.perf_counter() - t0
     tim = body.timing_read(reset=True)
     if phases:
         log(f"{precision} phases: " + json.dumps({k: round(v, 4) for k, v in body.phase_times(reset=True).items()}))
-    iters = body.frame_stats()[0]
+    iters, conv, _ = body.frame_stats()
     info = body.factor_info()
     return {
-        "body": body, "elapsed": elapsed, "setup_s": t_setup, "timing": tim, "fwd_iters": iters,
+        "body": body, "elapsed": elapsed, "setup_s": t_setup, "timing": tim, "fwd_iters": iters, "fwd_conv": conv,
         "nnz": info["nnz_S"], "max_depth": info["max_depth"], "factor_s": info["factor_s"],
-        "active": [len(a) for a in acts[1:]], "q": qq, "bwd_iters": body.last_backward_iters(),
+        "active": [len(a) for a in acts[1:]], "q": qq, "dl_dq": dq0, "dl_dv": dv0, "bwd": bstats,
     }
 
 
-def cpu_baseline(args, cells=(16, 16, 8)):
-    """FP64 oracle (CPU restatement of the reference solver: per-axis sparse
-    LDL^T, Delassus columns by triangular solves, host dense NCP and Anderson)
-    timed on the host cores for one forward+backward step of the same press
-    generator at a reduced grid. The full 64x64x32 oracle needs minutes for its
-    LDL^T alone and ~3K triangular solves per frame for the Delassus columns
-    (K ~ 8.4k candidates), far beyond a bounded sample; the reduced grid runs
-    the identical algorithm and options, so its steps/s bounds the CPU rate at
-    full size from above."""
+def bwd_summary(bstats):
+    """Per-frame backward record (oldest frame first) and totals."""
+    rows = [{"frame": int(s["frame"]), "backbone_iters": int(s["backbone_iters"]),
+             "woodbury_iters": int(s["woodbury_iters"]), "backbone_converged": int(s["backbone_converged"]),
+             "trust_stage": int(s["trust_stage"]), "clipped": int(s["clipped"]), "basis_rows": int(s["basis_rows"]),
+             "filtered_samples": int(s["filtered"])} for s in reversed(bstats)]
+    return rows, sum(r["clipped"] for r in rows), sum(r["backbone_iters"] + r["woodbury_iters"] for r in rows)
+
+
+def _cpu_step(args, cells):
+    """One fwd+bwd step (frame 0) of the press generator at `cells` on the FP64
+    oracle, after its factorization; returns (seconds, vertices, PD iterations,
+    factorization seconds, candidates)."""
     import oracle
     from diffphd_amd import scenes
-    os.environ["OMP_NUM_THREADS"] = str(args.threads)
-    oracle.build()
     spec = scenes.press(cells=cells, frames=max(100, args.start_frame + args.warmup + args.steps))
     orc = scenes.apply(spec, oracle.OracleDeformable(8))
     opt = dict(spec["options"], thread_ct=args.threads)
@@ -148,15 +165,43 @@ def cpu_baseline(args, cells=(16, 16, 8)):
     orc.apply_ainv(spec["dt"], np.zeros(orc.dofs()))  # LDL^T factorization
     t_factor = time.perf_counter() - t0
     t0 = time.perf_counter()
-    run_steps(orc, spec, opt, spec["q0"], spec["v0"], 0, 1)
+    run_steps(orc, spec, opt, spec["q0"], spec["v0"], 0, 1, BWD_OPTS)
     t_step = time.perf_counter() - t0
     iters = orc.frame_stats()[0]
-    nv = spec["q0"].size // 3
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": args.threads, "kind": "port",
-            "sample": (f"oracle fwd+bwd step 0 of the press generator at {cells[0]}x{cells[1]}x{cells[2]} cells "
-                       f"({nv} vertices, {len(spec['boundary'][2])} contact candidates, same options; "
-                       f"{int(np.sum(iters))} forward PD iterations) in {t_step:.1f} s, factorization "
-                       f"{t_factor:.2f} s excluded; an upper bound on the CPU steps/s at 64x64x32")}
+    return t_step, spec["q0"].size // 3, int(np.sum(iters)), t_factor, len(spec["boundary"][2])
+
+
+def cpu_baseline(args, cells_list=((8, 8, 4), (16, 16, 8)), target_cells=(64, 64, 32)):
+    """FP64 oracle (CPU restatement of the reference solver: per-axis sparse
+    LDL^T, Delassus columns by triangular solves, host dense NCP and Anderson)
+    timed on the host cores for one forward+backward step of the same press
+    generator at two reduced grids. The 64x64x32 oracle needs minutes for its
+    LDL^T alone and ~3K triangular solves per frame for the Delassus columns
+    (K ~ 8.4k candidates), beyond a bounded sample. `value` is the larger
+    grid's measured steps/s; the power law through the two measured sizes
+    gives the extrapolated 64x64x32 figure, reported beside it."""
+    import oracle
+    os.environ["OMP_NUM_THREADS"] = str(args.threads)
+    oracle.build()
+    pts = []
+    for cells in cells_list:
+        t, nv, it, tf, K = _cpu_step(args, cells)
+        pts.append({"cells": list(cells), "vertices": nv, "candidates": K, "seconds_per_step": t,
+                    "fwd_pd_iters": it, "factor_s": tf})
+        log(f"cpu baseline {cells}: {t:.2f} s/step ({it} PD iterations, factor {tf:.2f} s)")
+    (a, b) = pts[-2], pts[-1]
+    alpha = np.log(b["seconds_per_step"] / a["seconds_per_step"]) / np.log(b["vertices"] / a["vertices"])
+    nv_t = (target_cells[0] + 1) * (target_cells[1] + 1) * (target_cells[2] + 1)
+    t_extra = b["seconds_per_step"] * (nv_t / b["vertices"]) ** alpha
+    return {"value": 1.0 / b["seconds_per_step"], "unit": "steps/s", "cores": args.threads, "kind": "port",
+            "sample": (f"oracle fwd+bwd step 0 of the press generator (same options, factorization excluded) at "
+                       + " and ".join(f"{p['cells'][0]}x{p['cells'][1]}x{p['cells'][2]} ({p['vertices']} vertices, "
+                                      f"{p['seconds_per_step']:.2f} s)" for p in pts)
+                       + f"; value = the larger grid. Power law t ~ N^{alpha:.2f} through the two sizes extrapolates "
+                       f"to {t_extra:.3g} s/step at {target_cells[0]}x{target_cells[1]}x{target_cells[2]} "
+                       f"({nv_t} vertices)"),
+            "points": pts, "power_law_exponent": float(alpha),
+            "extrapolated_steps_per_s_at_bench_grid": float(1.0 / t_extra)}
 
 
 def profile_spmv(spec, args):
@@ -216,6 +261,7 @@ def main():
     achieved = bytes_per_launch / (dom_avg * 1e-3) / 1e9 if dom_avg > 0 else 0.0
     n_dof = spec["q0"].size
     fwd_iters = int(np.sum(res["fwd_iters"]))
+    bwd_rows, clipped, bwd_iters = bwd_summary(res["bwd"])
     # HBM traffic per launch of the dominant kernel from the committed FETCH_SIZE /
     # WRITE_SIZE passes (calibrated on the copy kernel), for the workload they cover
     traffic = None
@@ -239,8 +285,12 @@ def main():
         "data": "synthetic (seeded press scene: GRF stiffness field, sinusoid heightfield)",
         "config": {"workload": f"press {cells[0]}x{cells[1]}x{cells[2]} hex, fwd+bwd PD step, friction 0.3",
                    "elements": int(np.prod(cells)), "dofs": int(n_dof), "precision": args.precision,
+                   "warm_start": "q_t (the reference's initial iterate; the learned start is not in this line)",
+                   "backward": "per-element K_pd, trust region on the proximal-map spectrum (stage 0 = unfiltered)",
                    "timed_frames": [args.start_frame + args.warmup, args.start_frame + args.warmup + K - 1],
                    "active_contacts": res["active"], "fwd_pd_iters_timed": fwd_iters,
+                   "fwd_frames_converged": int(np.sum(res["fwd_conv"])), "max_pd_iter": spec["options"]["max_pd_iter"],
+                   "bwd_iters_timed": bwd_iters, "clipped_bwd_frames": clipped, "bwd_frames": bwd_rows,
                    "setup_s": res["setup_s"], "factor_nnz": nnz, "factor_max_depth": res["max_depth"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -249,21 +299,35 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": dom_avg,
                      "spmv_rows_avg_ms": rows_avg, "spmv_cols_avg_ms": cols_avg,
                      "launches": {"rows": tim["rows_launches"], "cols": tim["cols_launches"]}},
-        "dof_iters_per_s": n_dof * fwd_iters / res["elapsed"],
+        # 3N x (forward PD iterations + backbone and Woodbury adjoint iterations) per second
+        "dof_iters_per_s": n_dof * (fwd_iters + bwd_iters) / res["elapsed"],
     }
+    any_clipped = clipped > 0
     if not args.no_fp64 and args.precision == "mixed":
         del res["body"]
-        K64 = max(1, K // 2)
-        r64 = time_path(spec, "fp64", K64, args.warmup, args.threads, q0, v0, args.start_frame, args.phases)
-        sps64 = K64 / r64["elapsed"]
+        r64 = time_path(spec, "fp64", K, args.warmup, args.threads, q0, v0, args.start_frame, args.phases)
+        sps64 = K / r64["elapsed"]
+        rows64, clipped64, bwd64 = bwd_summary(r64["bwd"])
+        any_clipped |= clipped64 > 0
+
+        def dev(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
         out["fp64_steps_per_s"] = sps64
         out["speedup_vs_fp64"] = steps_per_s / sps64
-        dq = res["q"][K64] - r64["q"][K64]
-        out["rel_dev_q_vs_fp64"] = float(np.linalg.norm(dq) / np.linalg.norm(r64["q"][K64]))
-        out["fp64_fwd_pd_iters_timed"] = int(np.sum(r64["fwd_iters"]))
+        out["fp64"] = {"frames": out["config"]["timed_frames"], "fwd_pd_iters_timed": int(np.sum(r64["fwd_iters"])),
+                       "fwd_frames_converged": int(np.sum(r64["fwd_conv"])), "bwd_iters_timed": bwd64,
+                       "clipped_bwd_frames": clipped64, "bwd_frames": rows64}
+        # same start state, same frames: mixed against FP64 at the end of the
+        # forward window (q) and at its start (the gradients of the window loss)
+        out["rel_dev_vs_fp64"] = {"q_end": dev(res["q"][-1], r64["q"][-1]),
+                                  "dl_dq_start": dev(res["dl_dq"], r64["dl_dq"]),
+                                  "dl_dv_start": dev(res["dl_dv"], r64["dl_dv"])}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args)
+        out["cpu_baseline"] = cpu_baseline(args, target_cells=cells)
     print(json.dumps(out))
+    if any_clipped:
+        log("a timed backward frame reached the clip norm (its gradients were cut): see config.bwd_frames")
+        sys.exit(3)
 
 
 if __name__ == "__main__":
