@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(256) k_chol_assemble(int nJ, int np, int64_t l
         v = (r == c) ? 1.0 : 0.0;
     } else {
         const int i = J[r], j = J[c];
-        const double nd = nrm[3 * i] * nrm[3 * j] + nrm[3 * i + 1] * nrm[3 * j + 1] + nrm[3 * i + 2] * nrm[3 * j + 2];
+        const double nd = nrm ? nrm[3 * i] * nrm[3 * j] + nrm[3 * i + 1] * nrm[3 * j + 1] + nrm[3 * i + 2] * nrm[3 * j + 2]
+                              : 1.0;
         // H is symmetric: read along its row j so that consecutive r stay close
         v = om[i] * nd * H[(int64_t)j * K + i] * om[j];
         if (r == c) v += E[i];
@@ -60,6 +61,7 @@ template <> struct V16<float> { using type = float4; static constexpr int N = 4;
 template <> struct V16<double> { using type = double2; static constexpr int N = 2; };
 
 constexpr int kColStride = 2 * CB + 4;  // one published pivot column, see k_chol_step
+constexpr int kTSB = CB + 4;             // row stride of the published L columns (16-byte aligned rows)
 
 // Pivot of step j from the window's column j (wave 0): publishes the column
 // into buffer j & 1, L(r, j) at CB + r - j for r > j so that L(j + m, j) sits
@@ -135,6 +137,77 @@ __device__ __forceinline__ void panel_quarter(int& j, TV (&z)[CB], TV* colS, int
             for (int e = 0; e < 16; ++e) z[16 * q + e - 1] = z[16 * q + e] - coef * c[q & 1][e];
         }
         if (diag_wave) coef = coef_next;
+    }
+}
+
+// ---- The panel in 16-column sub-blocks ------------------------------------
+// The diagonal wave (lane r holds row r of the diagonal block in z) factors
+// sub-block SB's 16 columns with readlane pivots and rank-1 updates restricted
+// to those columns, so a step waits on register broadcasts only; L(r, j)
+// replaces z[j] (0 above the diagonal) and inv[jj] = 1 / L(j, j).
+template <class TV, int SB>
+__device__ __forceinline__ void diag_chain(TV (&z)[CB], int r, bool& bad_any, TV& inv_mine) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * SB + jj;
+        const TV p = lane_bcast(z[j], j);
+        const bool bad = !(p > TV(0));
+        bad_any |= bad;
+        const TV iv = bad ? TV(1) : pivot_rsqrt(p);
+        if (r == j) inv_mine = iv;
+        const TV l = (r > j) ? z[j] * iv : (r == j ? p * iv : TV(0));
+        z[j] = l;
+#pragma unroll
+        for (int m = j + 1; m < 16 * SB + 16; ++m) z[m] -= l * lane_bcast(l, m);
+    }
+}
+
+// Sub-block SB of L through LDS: Lt[j][m] = L(m, j), invp[j] = 1 / L(j, j)
+// (lane j holds its own pivot's inverse).
+template <class TV, int SB>
+__device__ __forceinline__ void diag_publish(const TV (&z)[CB], int r, TV inv_mine, TV (*Lt)[kTSB], TV* invp) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) Lt[16 * SB + jj][r] = z[16 * SB + jj];
+    if (r >= 16 * SB && r < 16 * SB + 16) invp[r] = inv_mine;
+}
+
+__device__ __forceinline__ float vget(const float4& t, int e) { return e == 0 ? t.x : e == 1 ? t.y : e == 2 ? t.z : t.w; }
+__device__ __forceinline__ double vget(const double2& t, int e) { return e == 0 ? t.x : t.y; }
+
+// Rank-16 update of the columns right of sub-block SB: z[m] -= sum_j z[j] L(m, j)
+// over the sub-block's j (z[j] = L(r, j) in the diagonal wave, X(r, j) in the
+// block-row wave); L(m, j) read as broadcast 16-byte vectors.
+template <class TV, int SB>
+__device__ __forceinline__ void panel_trail(TV (&z)[CB], const TV (*Lt)[kTSB]) {
+    using VT = typename V16<TV>::type;
+    constexpr int VN = V16<TV>::N;
+    constexpr int M0 = 16 * (SB + 1);
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+        const TV x = z[16 * SB + jj];
+#pragma unroll
+        for (int m = M0; m < CB; m += VN) {
+            const VT t = *reinterpret_cast<const VT*>(&Lt[16 * SB + jj][m]);
+#pragma unroll
+            for (int e = 0; e < VN; ++e) z[m + e] -= x * vget(t, e);
+        }
+    }
+}
+
+// The block-row wave's forward substitution on sub-block SB: X(r, j) =
+// (B(r, j) - sum_{i<j} X(r, i) L(j, i)) / L(j, j), right-looking within the
+// sub-block; X is written to out (column-major, out_ld).
+template <class TV, int SB>
+__device__ __forceinline__ void row_solve(TV (&z)[CB], const TV (*Lt)[kTSB], const TV* invp, int r, TV* out,
+                                          int64_t out_ld) {
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * SB + jj;
+        const TV x = z[j] * invp[j];
+        z[j] = x;
+        out[r + (int64_t)j * out_ld] = x;
+#pragma unroll
+        for (int m = j + 1; m < 16 * SB + 16; ++m) z[m] -= x * Lt[j][m];
     }
 }
 
@@ -328,16 +401,57 @@ __global__ void __launch_bounds__(256, 4) k_chol_step(int k, int T, int64_t lda,
         out = has_row ? M + i0 + c0 * lda : Xinv + (int64_t)k * CB * CB;
         out_ld = has_row ? lda : CB;
     }
-    for (int e = t; e < 2 * kColStride; e += 256) colS[e] = TV(0);
-    __syncthreads();
     bool bad_any = false;
-    TV coef = TV(0);
-    if (role == 0) coef = panel_pivot<TV>(0, z[0], colS, r, bad_any);
-    int j = 0;
-    panel_quarter<TV, 63>(j, z, colS, role, r, bad_any, coef, out, out_ld);
-    panel_quarter<TV, 47>(j, z, colS, role, r, bad_any, coef, out, out_ld);
-    panel_quarter<TV, 31>(j, z, colS, role, r, bad_any, coef, out, out_ld);
-    panel_quarter<TV, 15>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+    if constexpr (std::is_same<TV, double>::value) {
+        // FP64: one pivot per barrier (the sub-block form below needs twice the
+        // FP32 registers for its broadcast operands and spills)
+        for (int e = t; e < 2 * kColStride; e += 256) colS[e] = TV(0);
+        __syncthreads();
+        TV coef = TV(0);
+        if (role == 0) coef = panel_pivot<TV>(0, z[0], colS, r, bad_any);
+        int j = 0;
+        panel_quarter<TV, 63>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+        panel_quarter<TV, 47>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+        panel_quarter<TV, 31>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+        panel_quarter<TV, 15>(j, z, colS, role, r, bad_any, coef, out, out_ld);
+        if (role == 0 && bad_any && w == 0) fail[0] = 1;
+        return;
+    }
+    __syncthreads();  // every wave holds its rows: S0 now carries the published L columns
+    TV (*Lt)[kTSB] = reinterpret_cast<TV (*)[kTSB]>(&S0[0][0]);
+    TV* invp = colS;
+    TV inv = TV(1);
+    // the diagonal wave runs one sub-block ahead of the block-row wave
+    if (role == 0) { diag_chain<TV, 0>(z, r, bad_any, inv); diag_publish<TV, 0>(z, r, inv, Lt, invp); }
+    __syncthreads();
+    if (role == 0) {
+        panel_trail<TV, 0>(z, Lt);
+        diag_chain<TV, 1>(z, r, bad_any, inv);
+        diag_publish<TV, 1>(z, r, inv, Lt, invp);
+    } else if (role == 1) {
+        row_solve<TV, 0>(z, Lt, invp, r, out, out_ld);
+        panel_trail<TV, 0>(z, Lt);
+    }
+    __syncthreads();
+    if (role == 0) {
+        panel_trail<TV, 1>(z, Lt);
+        diag_chain<TV, 2>(z, r, bad_any, inv);
+        diag_publish<TV, 2>(z, r, inv, Lt, invp);
+    } else if (role == 1) {
+        row_solve<TV, 1>(z, Lt, invp, r, out, out_ld);
+        panel_trail<TV, 1>(z, Lt);
+    }
+    __syncthreads();
+    if (role == 0) {
+        panel_trail<TV, 2>(z, Lt);
+        diag_chain<TV, 3>(z, r, bad_any, inv);
+        diag_publish<TV, 3>(z, r, inv, Lt, invp);
+    } else if (role == 1) {
+        row_solve<TV, 2>(z, Lt, invp, r, out, out_ld);
+        panel_trail<TV, 2>(z, Lt);
+    }
+    __syncthreads();
+    if (role == 1) row_solve<TV, 3>(z, Lt, invp, r, out, out_ld);
     if (role == 0 && bad_any && w == 0) fail[0] = 1;
 }
 

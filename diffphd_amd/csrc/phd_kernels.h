@@ -154,10 +154,11 @@ void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const doubl
 // carries an impulse of g (omega lambda != 0 or friction), in P (J or F);
 // friction sets: lambda > 0, omega lambda != 0
 constexpr int kNcpJ = 1, kNcpF = 2, kNcpG = 4, kNcpP = 8, kNcpL = 1, kNcpN = 2;
-void launch_ncp_sets(hipStream_t s, int K, const double* delta, const double* lambda, const double* rprec,
-                     const double* nrm, const double* t1, const double* t2, const double* qC, double* uC, const double* F,
-                     const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec, double* Fg,
-                     unsigned char* flags);
+// enabled (may be null): candidates outside the set take no part (omega 0, no impulse)
+void launch_ncp_sets(hipStream_t s, int K, const unsigned char* enabled, const double* delta, const double* lambda,
+                     const double* rprec, const double* nrm, const double* t1, const double* t2, const double* qC, double* uC,
+                     const double* F, const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec,
+                     double* Fg, unsigned char* flags);
 // lists (nl <= 4, K entries each) of the candidates whose flags contain masks[i];
 // counts[i] their sizes; pos[c] = position of c in list 0 or -1
 void launch_ncp_compact(hipStream_t s, int K, const unsigned char* flags, int nl, const int* masks, int* lists, int* counts,
@@ -166,7 +167,7 @@ void launch_ncp_rhs(hipStream_t s, int nP, const int* P, const double* hprev, co
                     const double* hvec, const double* om, const double* nrm, double* uC, double* rJ);
 template <class TV>
 void launch_ncp_lambda(hipStream_t s, int K, const int* posJ, const TV* x, const double* hvec, const double* E,
-                       const double* om, double* lambda);
+                       const double* om, double lam_max, double* lambda);
 void launch_fric_sets(hipStream_t s, int K, const double* lambda, const double* om, const double* nrm, double* Fn,
                       unsigned char* flags);
 void launch_fric_coulomb(hipStream_t s, int K, const double* lambda, const int* posL, const double* hn, const double* uC,

@@ -1612,7 +1612,8 @@ namespace {
 // omega, E, h (:1160-1197) per candidate, the current impulses' support flags
 // and F_g = omega lambda n + lt1 t1 + lt2 t2 (the impulses g carries); uC, the
 // candidates' rows of S^T S r, becomes (A^-1 b)_C + H F_prev by adding q_C.
-__global__ void k_ncp_sets(int K, const double* __restrict__ delta, const double* __restrict__ lambda,
+__global__ void k_ncp_sets(int K, const unsigned char* __restrict__ enabled, const double* __restrict__ delta,
+                           const double* __restrict__ lambda,
                            const double* __restrict__ rprec, const double* __restrict__ nrm, const double* __restrict__ t1,
                            const double* __restrict__ t2, const double* __restrict__ qC, double* __restrict__ uC,
                            const double* __restrict__ F, const double* __restrict__ lt1, const double* __restrict__ lt2,
@@ -1620,6 +1621,15 @@ __global__ void k_ncp_sets(int K, const double* __restrict__ delta, const double
                            double* __restrict__ Fg, unsigned char* __restrict__ flags) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= K) return;
+    if (enabled && !enabled[c]) {  // outside the replayed active set: no impulse, lambda stays 0
+        om[c] = 0.0;
+        E[c] = 1.0;
+        hvec[c] = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { uC[3 * c + d] += qC[3 * c + d]; Fg[3 * c + d] = 0.0; }
+        flags[c] = 0;
+        return;
+    }
     const double dj = delta[c], lj = lambda[c], rj = rprec[c];
     double den = sqrt(dj * dj + rj * rj * lj * lj);
     if (den < 1e-14) den = 1e-14;
@@ -1726,11 +1736,12 @@ __global__ void k_ncp_rhs(int nP, const int* __restrict__ P, const double* __res
 // h_c / E_c for the decoupled rest (:1241-1253).
 template <class TV>
 __global__ void k_ncp_lambda(int K, const int* __restrict__ posJ, const TV* __restrict__ x, const double* __restrict__ hvec,
-                             const double* __restrict__ E, const double* __restrict__ om, double* __restrict__ lambda) {
+                             const double* __restrict__ E, const double* __restrict__ om, double lam_max,
+                             double* __restrict__ lambda) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= K) return;
     const double dl = om[c] != 0.0 ? (double)x[posJ[c]] : hvec[c] / E[c];
-    lambda[c] = fmin(fmax(lambda[c] + dl, 0.0), 1e8);
+    lambda[c] = fmin(fmax(lambda[c] + dl, 0.0), lam_max);
 }
 
 // Friction sets: lambda > 0 (kNcpL) and omega lambda != 0 (kNcpN, carrying the
@@ -1827,13 +1838,13 @@ __global__ void k_h_apply_idx_dev(const int* __restrict__ nr_dev, const int* __r
 
 }  // namespace
 
-void launch_ncp_sets(hipStream_t s, int K, const double* delta, const double* lambda, const double* rprec,
-                     const double* nrm, const double* t1, const double* t2, const double* qC, double* uC, const double* F,
-                     const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec, double* Fg,
-                     unsigned char* flags) {
+void launch_ncp_sets(hipStream_t s, int K, const unsigned char* enabled, const double* delta, const double* lambda,
+                     const double* rprec, const double* nrm, const double* t1, const double* t2, const double* qC, double* uC,
+                     const double* F, const double* lt1, const double* lt2, int do_fric, double* om, double* E, double* hvec,
+                     double* Fg, unsigned char* flags) {
     if (K == 0) return;
-    hipLaunchKernelGGL(k_ncp_sets, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, delta, lambda, rprec, nrm, t1, t2, qC, uC,
-                       F, lt1, lt2, do_fric, om, E, hvec, Fg, flags);
+    hipLaunchKernelGGL(k_ncp_sets, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, enabled, delta, lambda, rprec, nrm, t1, t2,
+                       qC, uC, F, lt1, lt2, do_fric, om, E, hvec, Fg, flags);
 }
 void launch_ncp_compact(hipStream_t s, int K, const unsigned char* flags, int nl, const int* masks, int* lists, int* counts,
                         int* pos) {
@@ -1848,14 +1859,15 @@ void launch_ncp_rhs(hipStream_t s, int nP, const int* P, const double* hprev, co
 }
 template <class TV>
 void launch_ncp_lambda(hipStream_t s, int K, const int* posJ, const TV* x, const double* hvec, const double* E,
-                       const double* om, double* lambda) {
+                       const double* om, double lam_max, double* lambda) {
     if (K == 0) return;
-    hipLaunchKernelGGL((k_ncp_lambda<TV>), dim3(blocks_for(K, 128)), dim3(128), 0, s, K, posJ, x, hvec, E, om, lambda);
+    hipLaunchKernelGGL((k_ncp_lambda<TV>), dim3(blocks_for(K, 128)), dim3(128), 0, s, K, posJ, x, hvec, E, om, lam_max,
+                       lambda);
 }
 template void launch_ncp_lambda<float>(hipStream_t, int, const int*, const float*, const double*, const double*,
-                                       const double*, double*);
+                                       const double*, double, double*);
 template void launch_ncp_lambda<double>(hipStream_t, int, const int*, const double*, const double*, const double*,
-                                        const double*, double*);
+                                        const double*, double, double*);
 void launch_fric_sets(hipStream_t s, int K, const double* lambda, const double* om, const double* nrm, double* Fn,
                       unsigned char* flags) {
     if (K == 0) return;

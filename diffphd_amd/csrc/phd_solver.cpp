@@ -831,18 +831,19 @@ void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) 
 // The compressed contact system on the candidates J (n rows): its right-hand
 // side is in d_rhsK_, the solution stays on the device (d_cx32_ / d_cx64_);
 // only the pivot flag comes back. False when a pivot was not positive.
-bool Body::solve_contact_system(int n, int precision, const int* J) {
+bool Body::solve_contact_system(int n, int precision, const int* J, bool plain_w) {
+    const double* nrm = plain_w ? nullptr : d_nrm_.p;
     const int np = chol_padded(n);
     launch_clear_flag(stream_, d_info_.p);
     if (precision == 1) {
         if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
         if (d_cy32_.n < (size_t)chol_work(n)) { d_cy32_.alloc((size_t)chol_work(n)); d_cx32_.alloc(np); }
-        launch_chol_assemble<float>(stream_, n, J, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
+        launch_chol_assemble<float>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
         launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
     } else {
         if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
         if (d_cy64_.n < (size_t)chol_work(n)) { d_cy64_.alloc((size_t)chol_work(n)); d_cx64_.alloc(np); }
-        launch_chol_assemble<double>(stream_, n, J, d_H_.p, K_, d_nrm_.p, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
+        launch_chol_assemble<double>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
         launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
     }
     int* flag = reinterpret_cast<int*>(pin_small_);
@@ -882,6 +883,7 @@ struct DevAnderson {
     DevBuf<double>* Q = nullptr;  // 8 buffers: 0..6 differences of x, 7 the last x
     DevBuf<double>* G = nullptr;  // 8 buffers: 0..6 differences of g, 7 the last g
     bool have_last = false;
+    bool replay = false;          // the replay's variant: reg 1e-10 max(tr / m, 1), no restart
     std::deque<int> diffs;        // pool indices of the differences, oldest first
     std::vector<int> free_slots;
     double gram[7][7] = {};       // dG_a . dG_b by pool index, filled as differences arrive
@@ -944,10 +946,10 @@ bool Body::anderson_step(DevAnderson& aa, const double* xprev, const double* g, 
         rr[a] = dd[m + a];
         sq += GtG[a * m + a];
     }
-    const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+    const double reg = aa.replay ? 1e-10 * std::max(sq / std::max(m, 1), 1.0) : std::max(1e-6 * sq / std::max(m, 1), 1e-12);
     for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
     const std::vector<double> gamma = small_ldlt_solve(GtG, m, rr);
-    if (nrm2(gamma) >= 10.0) {
+    if (!aa.replay && nrm2(gamma) >= 10.0) {
         aa.clear();
         return false;
     }
@@ -966,7 +968,25 @@ bool Body::anderson_step(DevAnderson& aa, const double* xprev, const double* g, 
 // ---------------------------------------------------------------------------
 void Body::forward(const double* q, const double* v, const double* f_ext, double dt,
                    const std::map<std::string, double>& opt, double* q_next, double* v_next, std::vector<int>& active) {
+    forward_impl(q, v, f_ext, dt, opt, q_next, v_next, active, nullptr, nullptr);
+}
+
+// The forward PD loop; with replay_active set it is the backward's
+// fixed-active-set replay (ReplayAlgPhdFixedActiveSet,
+// deformable_projective_dynamics_backward_gpu.cpp:113-418): only the listed
+// contacts take part, without friction, from lambda = 0, with the obstacle
+// normals re-evaluated at every iterate, r_j = max(W_jj, 1e-10) (no h^2),
+// W_ij = H_ij (the reference scales the column solves of row i by n_i on both
+// sides, :203-220, so for unit normals and one factor per axis the normal
+// products drop out), lambda clamped below only, and Anderson acceleration with
+// reg = 1e-10 max(tr / m, 1) and no restart (:381-405). The result goes to
+// *replay_out; the tape, the lambda warm start and the frame statistics are
+// left as they were.
+void Body::forward_impl(const double* q, const double* v, const double* f_ext, double dt,
+                        const std::map<std::string, double>& opt, double* q_next, double* v_next,
+                        std::vector<int>& active, const std::vector<int>* replay_active, ReplayFrame* replay_out) {
     auto tstart = std::chrono::steady_clock::now();
+    const bool rp = replay_active != nullptr;
     const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
     const double abs_tol = opt_req(opt, "abs_tol");
     const double rel_tol = opt_req(opt, "rel_tol");
@@ -984,7 +1004,7 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
     (void)h2m;
     // new-rollout detection for the tape (:437-449)
-    if (!tape_.empty()) {
+    if (!rp && !tape_.empty()) {
         const auto& last = tape_.back();
         double dn = 0, a = 0, b = 0;
         for (int i = 0; i < n3; ++i) { dn += (last.q_sol[i] - q[i]) * (last.q_sol[i] - q[i]); a += last.q_sol[i] * last.q_sol[i]; b += q[i] * q[i]; }
@@ -996,12 +1016,25 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     const DevBoundary bnd = dev_boundary();
     // per-candidate contact state lives on the device for the whole frame
     std::vector<double> lambda(K, 0.0);
-    const bool do_fric = K > 0 && fmu > 0;
+    const bool do_fric = !rp && K > 0 && fmu > 0;
+    const unsigned char* enabled = nullptr;
+    if (rp) {  // the replay's contacts: a subset of the candidates
+        std::unordered_map<int, int> cidx;
+        for (int c = 0; c < K; ++c) cidx[cand_[c]] = c;
+        std::vector<unsigned char> en(std::max(K, 1), 0);
+        for (int node : *replay_active) {
+            auto it = cidx.find(node);
+            check(it != cidx.end(), "Active contact " + std::to_string(node) + " is not a frictional-boundary vertex.");
+            en[it->second] = 1;
+        }
+        d_enabled_.upload(en);
+        enabled = d_enabled_.p;
+    }
     if (K > 0) {
         launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
         std::vector<double> rprec(K);
-        for (int c = 0; c < K; ++c) rprec[c] = std::max(h * h * Hdiag_[c], 1e-10);
-        if (!prev_candidates_.empty() && prev_lambda_.size() == prev_candidates_.size()) {
+        for (int c = 0; c < K; ++c) rprec[c] = std::max((rp ? 1.0 : h * h) * Hdiag_[c], 1e-10);
+        if (!rp && !prev_candidates_.empty() && prev_lambda_.size() == prev_candidates_.size()) {
             std::unordered_map<int, int> pidx;
             for (size_t i = 0; i < prev_candidates_.size(); ++i) pidx[prev_candidates_[i]] = (int)i;
             for (int c = 0; c < K; ++c) {
@@ -1020,9 +1053,10 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
     }
     const int warm = (int)opt_or(opt, "phd_warm_start", 0.0);
     check(warm >= 0 && warm <= 2, "phd_warm_start must be 0 (q_t), 1 (learned, falls back to q + h v) or 2 (q + h v).");
-    if (warm > 0) warm_start(v, h, inv_h2m, warm == 1);
+    if (warm > 0 && !rp) warm_start(v, h, inv_h2m, warm == 1);
     DevAnderson aa;
     aa.init(aa_window, n3, d_hist_q_, d_hist_g_);
+    aa.replay = rp;
     hip_check(hipMemsetAsync(d_pdrhs_prev_.p, 0, sizeof(double) * n3, stream_), "prev pd_rhs");
     bool dense64 = precision == 0;
     bool converged = false;
@@ -1056,14 +1090,15 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
             launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
+            if (rp) launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
             tcs.reset();
             std::unique_ptr<PhaseTimer> tcset(new PhaseTimer(this, kFwdCSets));
             // omega, E, h (forward_gpu.cpp:1160-1197) and the index sets. Candidates with
             // omega == 0 (separated, lambda == 0) decouple exactly: their row of
             // Omega W Omega + E is E_c e_c, so the dense work runs on J = {omega != 0};
             // P adds the support of the previous impulses, G that of g's impulses.
-            launch_ncp_sets(stream_, K, d_delta_.p, d_lambda_.p, d_rprec_.p, d_nrm_.p, d_t1_.p, d_t2_.p, d_dF_.p, d_uC_.p,
-                            d_F_.p, d_lt1_.p, d_lt2_.p, do_fric ? 1 : 0, d_om_.p, d_E_.p, d_hvec_.p, d_Fg_.p,
+            launch_ncp_sets(stream_, K, enabled, d_delta_.p, d_lambda_.p, d_rprec_.p, d_nrm_.p, d_t1_.p, d_t2_.p, d_dF_.p,
+                            d_uC_.p, d_F_.p, d_lt1_.p, d_lt2_.p, do_fric ? 1 : 0, d_om_.p, d_E_.p, d_hvec_.p, d_Fg_.p,
                             d_ncp_flags_.p);
             launch_ncp_compact(stream_, K, d_ncp_flags_.p, 4, set_masks, lists, d_ncp_counts_.p, d_posJ_.p);
             hip_check(hipMemcpyAsync(reinterpret_cast<int*>(pin_small_), d_ncp_counts_.p, 4 * sizeof(int), hipMemcpyDeviceToHost, stream_),
@@ -1086,21 +1121,24 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
                 PhaseTimer td(this, kFwdDense);
                 if (timing_level >= 2) { phase[kDenseCalls] += 1; phase[kDenseRows] += nJ; }
                 if (precision == 1 && !dense64) {
-                    if (solve_contact_system(nJ, 1, listJ)) solved_prec = 1;
+                    if (solve_contact_system(nJ, 1, listJ, rp)) solved_prec = 1;
                     else {
                         dense64 = true;  // the FP32 Cholesky broke down: FP64 for the rest of the frame
                         if (timing_level >= 2) phase[kDenseFp32Fail] += 1;
                     }
                 }
                 if (solved_prec < 0) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
-                    check(solve_contact_system(nJ, 0, listJ),
+                    check(solve_contact_system(nJ, 0, listJ, rp),
                           "Dense Cholesky of the contact system failed (matrix not positive definite).");
                     solved_prec = 0;
                 }
             }
             std::unique_ptr<PhaseTimer> tlam(new PhaseTimer(this, kFwdCApply));
-            if (solved_prec == 1) launch_ncp_lambda<float>(stream_, K, d_posJ_.p, d_cx32_.p, d_hvec_.p, d_E_.p, d_om_.p, d_lambda_.p);
-            else launch_ncp_lambda<double>(stream_, K, d_posJ_.p, d_cx64_.p, d_hvec_.p, d_E_.p, d_om_.p, d_lambda_.p);
+            const double lam_max = rp ? INFINITY : 1e8;
+            if (solved_prec == 1)
+                launch_ncp_lambda<float>(stream_, K, d_posJ_.p, d_cx32_.p, d_hvec_.p, d_E_.p, d_om_.p, lam_max, d_lambda_.p);
+            else
+                launch_ncp_lambda<double>(stream_, K, d_posJ_.p, d_cx64_.p, d_hvec_.p, d_E_.p, d_om_.p, lam_max, d_lambda_.p);
             tlam.reset();
             // NCP-integrated Coulomb friction (:1255-1302)
             PhaseTimer tfr(this, kFwdFric);
@@ -1152,6 +1190,21 @@ void Body::forward(const double* q, const double* v, const double* f_ext, double
         d_lt2_.download(lt2.data(), K);
         d_om_.download(omega_last.data(), K);
         d_nrm_.download(nrm.data(), 3 * K);
+    }
+    if (rp) {
+        ReplayFrame& fr = *replay_out;
+        fr = ReplayFrame();
+        fr.active = *replay_active;
+        fr.q_sol = qs;
+        std::unordered_map<int, int> cidx;
+        for (int c = 0; c < K; ++c) cidx[cand_[c]] = c;
+        for (int node : fr.active) {
+            const int c = cidx[node];
+            fr.lambda.push_back(lambda[c]);
+            fr.omega.push_back(omega_last[c]);
+            fr.normals.push_back({nrm[3 * c], nrm[3 * c + 1], nrm[3 * c + 2]});
+        }
+        return;
     }
     if (!converged) fprintf(stderr, "[Warning] alg_phd: PD loop did not converge.\n");
     // restitution post-pass (:1483-1531)
@@ -1211,7 +1264,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                     const double* v_next, const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
                     const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
                     double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem) {
-    (void)q; (void)v; (void)f_ext; (void)v_next;
+    (void)v_next;
     const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
     const double abs_tol = opt_req(opt, "abs_tol");
     const double rel_tol = opt_req(opt, "rel_tol");
@@ -1282,8 +1335,24 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             match = std::sqrt(dn) <= 1e-8 * std::max(1.0, std::max(std::sqrt(a), std::sqrt(b)));
         }
         if (match) { replay = tail; tape_.pop_back(); from_tape = true; }
+        else if ((int)tail.q_sol.size() == n3) {  // stale tape (:730-737)
+            double dn = 0, a = 0, b = 0;
+            for (int i = 0; i < n3; ++i) { dn += (tail.q_sol[i] - q_next[i]) * (tail.q_sol[i] - q_next[i]); a += tail.q_sol[i] * tail.q_sol[i]; b += q_next[i] * q_next[i]; }
+            if (std::sqrt(dn) / std::max(1.0, std::max(std::sqrt(a), std::sqrt(b))) > 1e-6) tape_.clear();
+        }
     }
-    check(from_tape || active.empty(), "Backward needs the forward tape of this frame; the fixed-active-set replay is not supported.");
+    if (!from_tape && !active.empty()) {
+        // ReplayAlgPhdFixedActiveSet (:113-418) on the device. The replay reloads
+        // d_q_ with a blocking copy, so the K_pd build queued above (which reads
+        // q_next from d_q_) must finish first; d_q_ is restored after.
+        hip_check(hipStreamSynchronize(stream_), "K_pd before replay");
+        std::vector<double> qn(n3), vn(n3);
+        std::vector<int> unused;
+        const std::vector<double> zeros(n3, 0.0);
+        forward_impl(q, v, f_ext ? f_ext : zeros.data(), dt, opt, qn.data(), vn.data(), unused, &active, &replay);
+        d_q_.upload(q_next, n3);
+    }
+    last_bwd.from_tape = from_tape ? 1 : 0;
     const int aa_window = (int)opt_or(opt, "aa_window", 5);
     check(aa_window >= 0 && aa_window <= 6, "aa_window must be in [0, 6].");
     const double clip = opt_or(opt, "alg_phd_bwd_clip_norm", 1e6);

@@ -159,7 +159,7 @@ public:
     int last_backward_iters = 0;
     struct BackwardStats {  // of the last backward call
         double backbone_iters = 0, backbone_converged = 0, clipped = 0, woodbury_iters = 0, woodbury_converged = 0,
-               basis_rows = 0, filtered = 0, samples = 0, db_norm = 0, trust_stage = 0;
+               basis_rows = 0, filtered = 0, samples = 0, db_norm = 0, trust_stage = 0, from_tape = 1;
     } last_bwd;
     long long factor_bytes() const;
     long long factor_nnz() const { return factor_.nnz; }
@@ -319,7 +319,11 @@ private:
     std::vector<double> dots(const std::vector<std::pair<const double*, const double*>>& pairs);
     bool anderson_step(DevAnderson& aa, const double* xprev, const double* g, double* out,
                        const std::vector<std::pair<const double*, const double*>>& extra, std::vector<double>* extra_out);
-    bool solve_contact_system(int n, int precision, const int* J);
+    bool solve_contact_system(int n, int precision, const int* J, bool plain_w = false);
+    void forward_impl(const double* q, const double* v, const double* f_ext, double dt,
+                      const std::map<std::string, double>& opt, double* q_next, double* v_next, std::vector<int>& active,
+                      const std::vector<int>* replay_active, ReplayFrame* replay_out);
+    DevBuf<unsigned char> d_enabled_;  // the replay's contacts among the candidates
 };
 
 }  // namespace phd

@@ -269,10 +269,10 @@ class OracleDeformable:
         return o[0], o[1:4].copy(), o[4:7].copy(), o[7:10].copy(), int(o[10])
 
     def last_backward_stats(self):
-        out, op = _d(np.zeros(9))
-        lib().orc_last_backward_stats(self._h, op, 9)
+        out, op = _d(np.zeros(10))
+        lib().orc_last_backward_stats(self._h, op, 10)
         keys = ("backbone_iters", "backbone_converged", "clipped", "woodbury_iters", "basis_rows", "filtered",
-                "samples", "db_norm", "trust_stage")
+                "samples", "db_norm", "trust_stage", "from_tape")
         return dict(zip(keys, out.tolist()))
 
 

@@ -803,9 +803,10 @@ struct Body {
     std::array<std::vector<double>, 3> csr_x;
     std::vector<int> frame_iters, frame_converged;
     int last_bwd_iters = 0;
+    bool last_bwd_from_tape = true;
     struct BackwardStats {  // of the last backward call
         double backbone_iters = 0, backbone_converged = 0, clipped = 0, woodbury_iters = 0, basis_rows = 0,
-               filtered = 0, samples = 0, db_norm = 0, trust_stage = 0;
+               filtered = 0, samples = 0, db_norm = 0, trust_stage = 0, from_tape = 1;
     } last_bwd;
 
     const double* G(int e, int s) const { return &grad[((size_t)e * samples + s) * edim * 3]; }
@@ -1170,6 +1171,8 @@ struct Body {
 
     void forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const std::map<std::string, double>& opt,
                  Vec& q_next, Vec& v_next, std::vector<int>& active);
+    AlgPhdReplayCache replay_fixed_active_set(const Vec& q, const Vec& v, const Vec& f_ext, double dt,
+                                              const std::vector<int>& active, const std::map<std::string, double>& opt);
     void backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
                   const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
                   const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext, Vec& dl_dmat_w,
@@ -1192,7 +1195,10 @@ static double opt_req(const std::map<std::string, double>& o, const char* k) {
 struct Anderson {
     int window;
     std::deque<Vec> xh, gh;
-    explicit Anderson(int w) : window(w) {}
+    // replay: the fixed-active-set replay's variant (backward_gpu.cpp:381-405):
+    // reg = 1e-10 max(tr(dG^T dG) / m, 1) and no restart
+    bool replay = false;
+    explicit Anderson(int w, bool replay_ = false) : window(w), replay(replay_) {}
     // x_prev: previous iterate, g_k = x_new - x_prev. On acceptance writes candidate.
     bool step(const Vec& x_prev, const Vec& g_k, Vec& candidate) {
         xh.push_back(x_prev);
@@ -1215,10 +1221,10 @@ struct Anderson {
                 for (int b = 0; b < m; ++b) GtG[a * m + b] = dot(dG[a], dG[b]);
                 rhs[a] = dot(dG[a], g_k);
             }
-            const double reg = std::max(1e-6 * sq / std::max(m, 1), 1e-12);
+            const double reg = replay ? 1e-10 * std::max(sq / std::max(m, 1), 1.0) : std::max(1e-6 * sq / std::max(m, 1), 1e-12);
             for (int a = 0; a < m; ++a) GtG[a * m + a] += reg;
             const Vec gamma = dense_ldlt_solve(GtG, m, rhs);
-            if (norm(gamma) >= 10.0) {
+            if (!replay && norm(gamma) >= 10.0) {
                 xh.clear();
                 gh.clear();
             } else {
@@ -1498,6 +1504,118 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
 
 // BackwardProjectiveDynamicsAlgPhd, deformable_projective_dynamics_backward_gpu.cpp:648-1250.
 // The tape must match (the fixed-active-set replay at :113-418 is not restated).
+// ReplayAlgPhdFixedActiveSet (deformable_projective_dynamics_backward_gpu.cpp:113-418):
+// the backward's own PD loop over a given active set, used when no forward tape
+// entry matches the frame. It differs from the forward: only the active
+// contacts, no friction, lambda from 0, normals re-evaluated at every iterate,
+// W(ci, cj) = sum_d n_ci[d]^2 (A_d^-1)(node_cj, node_ci) for ci <= cj (mirrored;
+// the reference multiplies by n_ci twice, :203-220), r_j = max(W_jj, 1e-10)
+// without the h^2 of the forward, lambda clamped below only, q = A^-1 (b_full +
+// J^T Omega lambda) directly, and Anderson acceleration with reg =
+// 1e-10 max(tr / m, 1) and no restart (:381-405).
+AlgPhdReplayCache Body::replay_fixed_active_set(const Vec& q, const Vec& v, const Vec& f_ext, double dt,
+                                                const std::vector<int>& active,
+                                                const std::map<std::string, double>& opt) {
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    const int aa_window = (int)opt_or(opt, "aa_window", 5);
+    const double h = dt, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    AlgPhdReplayCache cache;
+    cache.contact_candidates = active;
+    const int na = (int)active.size();
+    const Vec fs = state_force();
+    cache.q_tilde.assign(dofs, 0.0);
+    for (int i = 0; i < dofs; ++i) cache.q_tilde[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
+    cache.q_sol = q;
+    if (na == 0) return cache;
+    cache.lambda.assign(na, 0.0);
+    cache.omega.assign(na, 0.0);
+    cache.normals.assign(na, {0, 0, 0});
+    for (int c = 0; c < na; ++c) boundary.normal(&q[3 * active[c]], cache.normals[c].data());
+    std::vector<double> W((size_t)na * na, 0.0);
+    for (int d = 0; d < 3; ++d)
+        for (int ci = 0; ci < na; ++ci) {
+            Vec e(nv, 0.0);
+            e[active[ci]] = cache.normals[ci][d];
+            const Vec col = solve_axis(d, e);
+            for (int cj = ci; cj < na; ++cj) W[(size_t)ci * na + cj] += cache.normals[ci][d] * col[active[cj]];
+        }
+    for (int ci = 0; ci < na; ++ci)
+        for (int cj = ci + 1; cj < na; ++cj) W[(size_t)cj * na + ci] = W[(size_t)ci * na + cj];
+    Anderson aa(aa_window, true);
+    Vec prev_pd_rhs(dofs, 0.0);
+    auto ainv = [&](const Vec& b) {
+        Vec out(dofs);
+        for (int d = 0; d < 3; ++d) {
+            Vec bd(nv);
+            for (int i = 0; i < nv; ++i) bd[i] = b[3 * i + d];
+            const Vec x = solve_axis(d, bd);
+            for (int i = 0; i < nv; ++i) out[3 * i + d] = x[i];
+        }
+        return out;
+    };
+    for (int iter = 0; iter < max_pd_iter; ++iter) {
+        const Vec pd_rhs = local_step(cache.q_sol);
+        Vec b_full(dofs);
+        for (int i = 0; i < dofs; ++i) b_full[i] = inv_h2m * cache.q_tilde[i] + pd_rhs[i];
+        const Vec q_prev = cache.q_sol;
+        Vec delta(na), E(na), hvec(na);
+        for (int c = 0; c < na; ++c) {
+            const double* x = &cache.q_sol[3 * active[c]];
+            delta[c] = boundary.distance(x);
+            boundary.normal(x, cache.normals[c].data());
+        }
+        for (int c = 0; c < na; ++c) {
+            const double dj = delta[c], lj = cache.lambda[c], rj = std::max(W[(size_t)c * na + c], 1e-10);
+            double den = std::sqrt(dj * dj + rj * rj * lj * lj);
+            if (den < 1e-14) den = 1e-14;
+            cache.omega[c] = 1.0 - dj / den;
+            E[c] = (1.0 - rj * lj / den) * rj;
+            const double phi = dj + rj * lj - den;
+            double Hq = 0;
+            for (int d = 0; d < 3; ++d) Hq += cache.normals[c][d] * cache.q_sol[3 * active[c] + d];
+            hvec[c] = -phi + cache.omega[c] * Hq;
+        }
+        std::vector<double> M((size_t)na * na);
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < na; ++b) M[(size_t)a * na + b] = cache.omega[a] * W[(size_t)a * na + b] * cache.omega[b];
+        for (int a = 0; a < na; ++a) M[(size_t)a * na + a] += E[a];
+        Vec g = b_full;
+        for (int c = 0; c < na; ++c)
+            for (int d = 0; d < 3; ++d) g[3 * active[c] + d] += cache.normals[c][d] * cache.omega[c] * cache.lambda[c];
+        const Vec Ag = ainv(g);
+        Vec rhs(na);
+        for (int c = 0; c < na; ++c) {
+            double val = 0;
+            for (int d = 0; d < 3; ++d) val += cache.normals[c][d] * Ag[3 * active[c] + d];
+            rhs[c] = hvec[c] - cache.omega[c] * val;
+        }
+        const Vec dl = dense_ldlt_solve(M, na, rhs);
+        for (int c = 0; c < na; ++c) cache.lambda[c] = std::max(cache.lambda[c] + dl[c], 0.0);
+        Vec bc = b_full;
+        for (int c = 0; c < na; ++c)
+            for (int d = 0; d < 3; ++d) bc[3 * active[c] + d] += cache.normals[c][d] * cache.omega[c] * cache.lambda[c];
+        cache.q_sol = ainv(bc);
+        for (auto& kv : dirichlet) cache.q_sol[kv.first] = kv.second;
+        if (aa_window > 0) {
+            Vec gk(dofs);
+            for (int i = 0; i < dofs; ++i) gk[i] = cache.q_sol[i] - q_prev[i];
+            Vec cand;
+            if (aa.step(q_prev, gk, cand)) {
+                cache.q_sol = cand;
+                for (auto& kv : dirichlet) cache.q_sol[kv.first] = kv.second;
+            }
+        }
+        Vec dq(dofs), dr(dofs);
+        for (int i = 0; i < dofs; ++i) { dq[i] = cache.q_sol[i] - q_prev[i]; dr[i] = pd_rhs[i] - prev_pd_rhs[i]; }
+        const double abs_err = norm(dq), ref_err = norm(q_prev), rc = norm(dr), rr = norm(prev_pd_rhs);
+        prev_pd_rhs = pd_rhs;
+        if (iter > 0 && abs_err <= rel_tol * ref_err + abs_tol && rc <= rel_tol * rr + abs_tol) break;
+    }
+    return cache;
+}
+
 void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
                     const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
                     const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext, Vec& dl_dmat_w,
@@ -1548,9 +1666,15 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
             match = norm(d) <= 1e-8 * std::max(1.0, std::max(norm(tail.q_sol), norm(q_next)));
         }
         if (match) { replay = tail; tape.pop_back(); from_tape = true; }
+        else if (tail.q_sol.size() == q_next.size()) {  // stale tape (:730-737)
+            Vec d(dofs);
+            for (int i = 0; i < dofs; ++i) d[i] = tail.q_sol[i] - q_next[i];
+            if (norm(d) / std::max(1.0, std::max(norm(tail.q_sol), norm(q_next))) > 1e-6) tape.clear();
+        }
     }
-    check(from_tape || active.empty(), "Backward needs the forward tape of this frame (replay is not restated).");
-    if (!from_tape) { replay.contact_candidates = active; }
+    if (!from_tape) replay = replay_fixed_active_set(q, v, f_ext, dt, active, opt);
+    last_bwd_from_tape = from_tape;
+    last_bwd.from_tape = from_tape ? 1 : 0;
 
     auto apply_ainv = [&](const Vec& rhs) {
         Vec out(dofs);
@@ -1923,13 +2047,13 @@ int orc_last_contact_multipliers(void* hp, double* l, double* a, double* b, int 
 }
 int orc_last_backward_iters(void* hp) { return static_cast<Handle*>(hp)->body.last_bwd_iters; }
 // [backbone iterations (all trust stages), backbone converged, clipped, max Woodbury
-//  basis iterations, basis rows, filtered samples, samples, |db|, trust stage]
+//  basis iterations, basis rows, filtered samples, samples, |db|, trust stage, from tape]
 int orc_last_backward_stats(void* hp, double* out, int cap) {
     const auto& b = static_cast<Handle*>(hp)->body.last_bwd;
-    const double v[9] = {b.backbone_iters, b.backbone_converged, b.clipped, b.woodbury_iters,
-                         b.basis_rows, b.filtered, b.samples, b.db_norm, b.trust_stage};
-    for (int i = 0; i < 9 && i < cap; ++i) out[i] = v[i];
-    return 9;
+    const double v[10] = {b.backbone_iters, b.backbone_converged, b.clipped, b.woodbury_iters,
+                          b.basis_rows, b.filtered, b.samples, b.db_norm, b.trust_stage, b.from_tape};
+    for (int i = 0; i < 10 && i < cap; ++i) out[i] = v[i];
+    return 10;
 }
 void orc_coulomb_step(const double* Wt, double u1, double u2, double fmu, double lam, double* out2) {
     orc::coulomb_step(Wt, u1, u2, fmu, lam, out2[0], out2[1]);

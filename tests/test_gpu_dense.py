@@ -58,5 +58,6 @@ def test_dense_spd_solve_reports_indefinite():
     n = 70
     A = np.eye(n)
     A[40, 40] = -1.0
-    rc, _ = _solve(A, np.ones(n), 0)
-    assert rc == 1
+    for precision in (0, 1):
+        rc, _ = _solve(A, np.ones(n), precision)
+        assert rc == 1, precision
