@@ -65,12 +65,15 @@ def test_frictional_batch_matches_oracle_at_fixed_iterations():
     rolled out concurrently on the GPU (one stream each) against the oracle,
     one frame and its adjoint at a fixed 16 PD / 16 adjoint iterations, so both
     sides take the same NCP + Coulomb + Anderson steps (every active contact's
-    lambda is >= 3e-2 of the largest, away from the complementarity boundary)."""
+    lambda is >= 3e-2 of the largest, away from the complementarity boundary).
+    The Woodbury correction is skipped: truncated at a fixed count, the oracle's
+    per-basis fixed points and the HIP body's constrained one are different
+    iterations (they agree at convergence, test_gpu_gradients.py)."""
     import oracle
     from diffphd_amd.py_diff_pd_core import HexDeformable
     oracle.build()
     specs = scenes.beam_batch(4, seed=1, dims=((3, 5), (3, 5), (2, 3)))
-    fixed = {"rel_tol": 0.0, "abs_tol": 0.0, "max_pd_iter": 16, "alg_phd_bwd_wt_max_iter": 16}
+    fixed = {"rel_tol": 0.0, "abs_tol": 0.0, "max_pd_iter": 16, "skip_contact_woodbury": 1}
     hip = rollout_batch(specs, lambda: HexDeformable(0), frames=1, options=fixed, workers=4)
     orc = rollout_batch(specs, lambda: oracle.OracleDeformable(8), frames=1, options=fixed, workers=1)
     for s, x, y in zip(specs, hip, orc):
