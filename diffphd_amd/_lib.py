@@ -30,6 +30,7 @@ SIGNATURES = [
     ("phd_set_press_plane", ctypes.c_int, [_P, ctypes.c_double]),
     ("phd_factor_build", ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int]),
     ("phd_apply_ainv", ctypes.c_int, [_P, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+    ("phd_apply_ainv_noalloc", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("phd_step_forward", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_double, ctypes.POINTER(_cp), _dp, ctypes.c_int,
                                         _dp, _dp, _ip, _ip]),
     ("phd_step_backward", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _ip, ctypes.c_int, _dp, _dp,

@@ -90,6 +90,9 @@ int phd_factor_build(phd_ctx ctx, double dt, int threads) {
 int phd_apply_ainv(phd_ctx ctx, double dt, const double* bvec, double* x, int precision, int refine) {
     return guard(ctx, [&](phd::Body& b) { b.apply_ainv(dt, bvec, x, precision, refine); });
 }
+int phd_apply_ainv_noalloc(phd_ctx ctx, const double* b_dev, double* x_dev, int precision, int refine) {
+    return guard(ctx, [&](phd::Body& b) { b.apply_ainv_noalloc(b_dev, x_dev, precision, refine); });
+}
 int phd_step_forward(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
                      const char* const* keys, const double* vals, int n, double* q_next, double* v_next, int* active,
                      int* n_active) {

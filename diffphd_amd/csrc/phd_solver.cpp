@@ -25,11 +25,38 @@
 #include <deque>
 #include <numeric>
 #include <set>
+#include <mutex>
 #include <stdexcept>
 #include <unordered_map>
 
+// Scratch of the allocation-free A^-1 entry point (Body::apply_ainv_noalloc):
+// one file-scope device array, which every GPU the library runs on holds at
+// its own address until the process ends. Host code uses only the address
+// that hipGetSymbolAddress returns for the current device (scratch_address).
+static constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace phd {
 
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, which must be the current device;
+// resolved once per device.
+static void* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
 
 Body::PhaseTimer::PhaseTimer(Body* b_, int id_) : b(b_), id(id_), t0(0) {
     if (b->timing_level >= 2) {
@@ -89,6 +116,7 @@ static double det3(const double* A) {  // column-major
 
 Body::Body(int device) : device_(device) {
     hip_check(hipSetDevice(device_), "hipSetDevice");
+    scratch_ = scratch_address(device_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipHostMalloc((void**)&pin_small_, kPinSmallBytes, hipHostMallocDefault), "hipHostMalloc");
     rocblas_handle h;
@@ -870,6 +898,70 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
     }
     hip_check(hipStreamSynchronize(stream_), "apply_ainv");
     d_q_.download(x, dofs_);
+}
+
+// Scratch bytes apply_ainv_noalloc needs: w (3 ldw), y (3n), row-chunk partials,
+// column-tile partials (factor precision) and the FP64 residual (3n), each
+// segment 256-byte aligned.
+size_t Body::noalloc_scratch_bytes(int precision) const {
+    const size_t tv = precision == 1 ? sizeof(float) : sizeof(double);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t n = (size_t)nv_;
+    return al(3 * (size_t)spmv_ldw(nv_) * tv) + al(3 * n * tv) + al((3 * (size_t)plan_.rbase[nv_] + 3) * tv) +
+           al((192 * plan_.cu_tile.size() + 192) * tv) + al(3 * n * sizeof(double));
+}
+
+template <class TV>
+void Body::ainv_passes(const double* b_dev, double* x_dev, int refine, char* scr) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t n = (size_t)nv_;
+    TV* w = reinterpret_cast<TV*>(scr);
+    scr += al(3 * (size_t)spmv_ldw(nv_) * sizeof(TV));
+    TV* y = reinterpret_cast<TV*>(scr);
+    scr += al(3 * n * sizeof(TV));
+    TV* part = reinterpret_cast<TV*>(scr);
+    scr += al((3 * (size_t)plan_.rbase[nv_] + 3) * sizeof(TV));
+    TV* xpart = reinterpret_cast<TV*>(scr);
+    scr += al((192 * plan_.cu_tile.size() + 192) * sizeof(TV));
+    double* r = reinterpret_cast<double*>(scr);
+    const TV* V = std::is_same<TV, float>::value ? (const TV*)V32_.p : (const TV*)V64_.p;
+    for (int it = 0; it <= refine; ++it) {
+        const double* rhs = b_dev;
+        if (it > 0) {
+            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, b_dev, x_dev, nullptr, r);
+            rhs = r;
+        }
+        launch_perm_in<TV>(stream_, nv_, d_perm_.p, rhs, nullptr, w);
+        launch_spmv_rows_grp<TV>(stream_, (int)plan_.gu_group.size(), d_gu_group_.p, d_gu_chunk_.p, d_fd_.p, d_base_.p,
+                                 d_rbase_.p, V, w, nv_, part);
+        launch_spmv_rows_slots<TV>(stream_, nv_, d_rbase_.p, part, y);
+        launch_spmv_cols<TV>(stream_, (int)plan_.cu_tile.size(), d_cu_tile_.p, d_cu_i0_.p, d_cu_i1_.p, d_items_.p,
+                             d_fd_.p, d_base_.p, V, y, nv_, xpart);
+        // the first pass writes x = S^T S b; refinement passes add S^T S r to x in place
+        launch_spmv_cols_fin<TV>(stream_, nv_, d_tile_unit_ptr_.p, xpart, d_perm_.p, it == 0 ? nullptr : x_dev, nullptr,
+                                 nullptr, x_dev);
+    }
+}
+
+// apply_ainv on caller-owned device vectors with every intermediate in the
+// file-scope scratch array: no device allocation. Needs the factor built;
+// calls that overlap on one GPU share the scratch and race.
+void Body::apply_ainv_noalloc(const double* b_dev, double* x_dev, int precision, int refine) {
+    if (!ready_) throw std::runtime_error("apply_ainv_noalloc: the factor is not built (call phd_factor_build first)");
+    if (!b_dev || !x_dev) throw std::runtime_error("apply_ainv_noalloc: null vector");
+    if (precision != 0 && precision != 1) throw std::runtime_error("apply_ainv_noalloc: precision must be 0 or 1");
+    if (refine < 0) throw std::runtime_error("apply_ainv_noalloc: refine must be >= 0");
+    if (b_dev == x_dev) throw std::runtime_error("apply_ainv_noalloc: b and x must not alias");
+    const size_t need = noalloc_scratch_bytes(precision);
+    if (need > kNoAllocScratchBytes)
+        throw std::runtime_error("apply_ainv_noalloc: this factor needs " + std::to_string(need) +
+                                 " bytes of scratch; the maximum is " + std::to_string(kNoAllocScratchBytes));
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    char* scr = static_cast<char*>(scratch_);
+    if (precision == 1) ainv_passes<float>(b_dev, x_dev, refine, scr);
+    else ainv_passes<double>(b_dev, x_dev, refine, scr);
+    hip_check(hipGetLastError(), "apply_ainv_noalloc launch");
+    hip_check(hipStreamSynchronize(stream_), "apply_ainv_noalloc");
 }
 
 // ---------------------------------------------------------------------------

@@ -151,6 +151,10 @@ public:
                   double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem);
     // x = A^-1 b (AoS 3N) with the FP32 (precision=1) or FP64 (0) factor and `refine` FP64 refinement steps.
     void apply_ainv(double dt, const double* b, double* x, int precision, int refine);
+    // The same passes on caller-owned device vectors, every intermediate in the
+    // library's fixed device scratch array: allocates nothing (factor must be built).
+    void apply_ainv_noalloc(const double* b_dev, double* x_dev, int precision, int refine);
+    size_t noalloc_scratch_bytes(int precision) const;
     void setup(double dt, int threads);
 
     // ---- statistics ----
@@ -201,6 +205,9 @@ public:
 private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
+    void* scratch_ = nullptr;  // device address of the library's scratch array on device_
+    template <class TV>
+    void ainv_passes(const double* b_dev, double* x_dev, int refine, char* scr);
     // Pinned host memory for the few scalars read back per iteration (the
     // contact set sizes, the dense solve's pivot flag, the dot products).
     static constexpr size_t kPinSmallBytes = 4096;

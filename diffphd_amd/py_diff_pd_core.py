@@ -245,6 +245,12 @@ class _Deformable:
         self._ok(_lib.lib().phd_apply_ainv(self._ctx, float(dt), bp, xp, int(precision), int(refine)))
         return x
 
+    def apply_ainv_noalloc(self, b_dev_ptr, x_dev_ptr, precision=0, refine=0):
+        """x = A^-1 b on device vectors (raw device addresses of 3N doubles);
+        allocates nothing on the device. The factor must already be built."""
+        self._ok(_lib.lib().phd_apply_ainv_noalloc(self._ctx, ctypes.c_void_p(int(b_dev_ptr)),
+                                                   ctypes.c_void_p(int(x_dev_ptr)), int(precision), int(refine)))
+
     def factor_info(self):
         out, op = _lib.dptr(np.zeros(12))
         self._ok(_lib.lib().phd_factor_info(self._ctx, op, 12))

@@ -60,6 +60,19 @@ int phd_factor_build(phd_ctx ctx, double dt, int threads);
  * refine: number of FP64 iterative-refinement steps. */
 int phd_apply_ainv(phd_ctx ctx, double dt, const double* b, double* x, int precision, int refine);
 
+/* phd_apply_ainv on DEVICE vectors (b_dev, x_dev: 3N doubles each, AoS, on the
+ * context's device, not aliased) that allocates nothing: every intermediate
+ * (permuted input, S b, the row-chunk and column-tile partials, the FP64
+ * residual) lives in one fixed device array of the library, at most 64 MiB,
+ * which each GPU holds until the process ends. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops). The factor must already be built
+ * (phd_factor_build); a factor whose intermediates exceed 64 MiB is rejected
+ * with an error. Work is enqueued on the context's stream and the call returns
+ * after that stream has finished. Calls that overlap on one GPU (two contexts
+ * or two threads on the same device) share the array and race. */
+int phd_apply_ainv_noalloc(phd_ctx ctx, const double* b_dev, double* x_dev, int precision, int refine);
+
 /* One PD step, ForwardProjectiveDynamicsAlgPhd (forward_gpu.cpp:387-1594),
  * options as the reference's std::map<string, real> (keys/vals, n entries;
  * required: max_pd_iter, abs_tol, rel_tol, verbose, thread_ct). active: caller
