@@ -176,7 +176,9 @@ __device__ __forceinline__ double vget(const double2& t, int e) { return e == 0 
 
 // Rank-16 update of the columns right of sub-block SB: z[m] -= sum_j z[j] L(m, j)
 // over the sub-block's j (z[j] = L(r, j) in the diagonal wave, X(r, j) in the
-// block-row wave); L(m, j) read as broadcast 16-byte vectors.
+// block-row wave); L(m, j) read as broadcast 16-byte vectors. FP64 only: the
+// FP32 panel runs these updates on the matrix cores (diag_trail_mfma,
+// row_trail_mfma), which keeps LDS latency off the panel's critical path.
 template <class TV, int SB>
 __device__ __forceinline__ void panel_trail(TV (&z)[CB], const TV (*Lt)[kTSB]) {
     using VT = typename V16<TV>::type;
@@ -200,14 +202,30 @@ __device__ __forceinline__ void panel_trail(TV (&z)[CB], const TV (*Lt)[kTSB]) {
 template <class TV, int SB>
 __device__ __forceinline__ void row_solve(TV (&z)[CB], const TV (*Lt)[kTSB], const TV* invp, int r, TV* out,
                                           int64_t out_ld) {
+    using VT = typename V16<TV>::type;
+    constexpr int VN = V16<TV>::N, NV = 16 / VN, j0 = 16 * SB;
+    // the sub-block's pivot inverses and L row j + 1 are read while row j is
+    // applied, so each step waits on registers, not on LDS
+    VT ip[NV], cur[NV], nxt[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+        ip[e] = reinterpret_cast<const VT*>(invp + j0)[e];
+        cur[e] = reinterpret_cast<const VT*>(&Lt[j0][j0])[e];
+    }
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * SB + jj;
-        const TV x = z[j] * invp[j];
+        const int j = j0 + jj;
+        if (jj + 1 < 16) {
+#pragma unroll
+            for (int e = 0; e < NV; ++e) nxt[e] = reinterpret_cast<const VT*>(&Lt[j + 1][j0])[e];
+        }
+        const TV x = z[j] * vget(ip[jj / VN], jj % VN);
         z[j] = x;
         out[r + (int64_t)j * out_ld] = x;
 #pragma unroll
-        for (int m = j + 1; m < 16 * SB + 16; ++m) z[m] -= x * Lt[j][m];
+        for (int m = jj + 1; m < 16; ++m) z[j0 + m] -= x * vget(cur[m / VN], m % VN);
+#pragma unroll
+        for (int e = 0; e < NV; ++e) cur[e] = nxt[e];
     }
 }
 
@@ -274,6 +292,112 @@ __device__ __forceinline__ f32x16 tile_product_mfma(const float (*A)[kTS], const
     return acc;
 }
 
+// Exchange of the upper half-wave of x with the lower half-wave of y
+// (v_permlane32_swap_b32): x <- [x.lo, y.lo], y <- [x.hi, y.hi].
+__device__ __forceinline__ void swap_halves(float& x, float& y) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(p[0]);
+    y = __uint_as_float(p[1]);
+}
+
+// z[m] -= C(r, m) for the columns m >= M0 of column tile TJ (m = 32 TJ ..
+// 32 TJ + 31), given the tile's two row halves in the MFMA result layout (lo:
+// rows 0..31, hi: rows 32..63; register v of lane L holds row 32 t + L % 32,
+// column 32 TJ + mfma_col(v, L)). One half-wave swap per register pair puts
+// row r in lane r: afterwards x holds column 32 TJ + c and y column 32 TJ + c + 4.
+template <int TJ, int M0>
+__device__ __forceinline__ void trail_apply(float (&z)[CB], const f32x16& lo, const f32x16& hi) {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int c = 32 * TJ + (v & 3) + 8 * (v >> 2);
+        if (c + 4 < M0) continue;
+        float x = lo[v], y = hi[v];
+        swap_halves(x, y);
+        if (c >= M0) z[c] -= x;
+        z[c + 4] -= y;
+    }
+}
+
+// The panel's rank-16 updates on the matrix cores, C(r, m) = sum_j P(r, j)
+// L(m, j) over sub-block SB's columns j for the columns m right of it, as
+// 32 x 32 tiles of v_mfma_f32_32x32x2f32 (step s covers j = 16 SB + 2 s + h,
+// h the half-wave). Operand lane i + 32 h of a row tile t is P(32 t + i, j):
+// the half-wave swap of the register pair (z[j0 + 2 s], z[j0 + 2 s + 1]) gives
+// both row tiles of step s at once.
+// Diagonal wave: P = L (its own rows), only the lower part m <= r matters, so
+// the column tile 32..63 needs only the row tile 32..63.
+template <int SB>
+__device__ __forceinline__ void diag_trail_mfma(float (&z)[CB]) {
+    constexpr int j0 = 16 * SB, M0 = 16 * (SB + 1);
+    float p0[8], p1[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        p0[s] = z[j0 + 2 * s];
+        p1[s] = z[j0 + 2 * s + 1];
+        swap_halves(p0[s], p1[s]);
+    }
+    f32x16 zero;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) zero[v] = 0.f;
+    if constexpr (M0 < 32) {
+        f32x16 a0 = zero, a1 = zero;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            a0 = __builtin_amdgcn_mfma_f32_32x32x2f32(p0[s], p0[s], a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(p0[s], p1[s], a1, 0, 0, 0);
+        }
+        trail_apply<0, M0>(z, a0, a1);
+    }
+    f32x16 a1 = zero;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(p1[s], p1[s], a1, 0, 0, 0);
+    trail_apply<1, M0>(z, zero, a1);
+}
+
+// Block-row wave: P = X (its rows of the solved block row), L(m, j) read from
+// the published columns, Lt[j][m] = L(m, j), one 4-byte read per lane and step.
+template <int SB>
+__device__ __forceinline__ void row_trail_mfma(float (&z)[CB], const float (*Lt)[kTSB], int lane) {
+    constexpr int j0 = 16 * SB, M0 = 16 * (SB + 1);
+    const int i = lane & 31, h = lane >> 5;
+    float p0[8], p1[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        p0[s] = z[j0 + 2 * s];
+        p1[s] = z[j0 + 2 * s + 1];
+        swap_halves(p0[s], p1[s]);
+    }
+    f32x16 zero;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) zero[v] = 0.f;
+#pragma unroll
+    for (int tj = M0 / 32; tj < 2; ++tj) {
+        float l[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) l[s] = Lt[j0 + 2 * s + h][32 * tj + i];
+        f32x16 a0 = zero, a1 = zero;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            a0 = __builtin_amdgcn_mfma_f32_32x32x2f32(l[s], p0[s], a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_32x32x2f32(l[s], p1[s], a1, 0, 0, 0);
+        }
+        if (tj == 0) trail_apply<0, M0>(z, a0, a1);
+        else trail_apply<1, M0>(z, a0, a1);
+    }
+}
+
+// The FP32 panel's trailing updates on the matrix cores, FP64's on the VALU.
+template <class TV, int SB>
+__device__ __forceinline__ void diag_trail(TV (&z)[CB], const TV (*Lt)[kTSB]) {
+    if constexpr (std::is_same<TV, float>::value) diag_trail_mfma<SB>(z);
+    else panel_trail<TV, SB>(z, Lt);
+}
+template <class TV, int SB>
+__device__ __forceinline__ void row_trail(TV (&z)[CB], const TV (*Lt)[kTSB], int lane) {
+    if constexpr (std::is_same<TV, float>::value) row_trail_mfma<SB>(z, Lt, lane);
+    else panel_trail<TV, SB>(z, Lt);
+}
+
 // Rank-64 update of step kp on block (ib, jb): A_ij -= L_{i,kp} L_{j,kp}^T
 // (256 threads; the staging tiles are S0 and S1).
 template <class TV>
@@ -305,12 +429,14 @@ __device__ __forceinline__ void update_block(TV* __restrict__ M, int64_t lda, in
 //    step k - 1 is applied here, in LDS, to the two blocks the workgroup
 //    reads: the diagonal block (k, k) (every workgroup repeats it) and block
 //    (k + 1 + w, k); w = T - k - 1 is the right-hand-side block row and
-//    w = T - k the identity. Then wave 0 factors the diagonal block (lane r
-//    holds row r as a window that shifts one column per step, pivots by
-//    readlane, the pivot column published through two alternating LDS
-//    buffers, one barrier per step) and wave 1 solves X L_kk^T = B in
-//    lockstep, writing L(k + 1 + w, k) into M, or Xinv_k = L_kk^-T
-//    (column-major 64 x 64) for the identity;
+//    w = T - k the identity. Then wave 0 factors the diagonal block and
+//    wave 1 solves X L_kk^T = B, writing L(k + 1 + w, k) into M, or
+//    Xinv_k = L_kk^-T (column-major 64 x 64) for the identity. FP32 works in
+//    16-column sub-blocks: wave 0 runs each sub-block's pivot chain on
+//    readlane broadcasts and publishes its columns through LDS, wave 1
+//    solves against them one sub-block behind, and both apply the rank-16
+//    update of the columns to the right as MFMA tiles. FP64 publishes one
+//    pivot column per barrier (two alternating LDS buffers);
 //  * the remaining workgroups apply the update of step k - 1 to the blocks
 //    (i, j) with k < j <= i <= T.
 // Four workgroups per CU (the FP32 MFMA accumulators stay in VGPRs, so the
@@ -336,6 +462,8 @@ __global__ void __launch_bounds__(256, 4) k_chol_step(int k, int T, int64_t lda,
     const bool has_row = w < T - k;          // block row k + 1 + w of M (else the identity)
     const int64_t c0 = (int64_t)k * CB, i0 = (int64_t)(k + 1 + w) * CB;
     TV z[CB];
+    using VT = typename V16<TV>::type;
+    constexpr int VN = V16<TV>::N;
     if (k > 0) {
         // FP32: the two target blocks are read together with the staging loads
         const int row = 32 * (wave >> 1) + (r & 31);
@@ -383,10 +511,18 @@ __global__ void __launch_bounds__(256, 4) k_chol_step(int k, int T, int64_t lda,
         __syncthreads();
         if (role == 0) {
 #pragma unroll
-            for (int c = 0; c < CB; ++c) z[c] = S0[r][c];
+            for (int q = 0; q < CB / VN; ++q) {
+                const VT t4 = reinterpret_cast<const VT*>(&S0[r][0])[q];
+#pragma unroll
+                for (int e = 0; e < VN; ++e) z[VN * q + e] = vget(t4, e);
+            }
         } else if (role == 1) {
 #pragma unroll
-            for (int c = 0; c < CB; ++c) z[c] = has_row ? S1[r][c] : (c == r ? TV(1) : TV(0));
+            for (int q = 0; q < CB / VN; ++q) {
+                const VT t4 = has_row ? reinterpret_cast<const VT*>(&S1[r][0])[q] : VT{};
+#pragma unroll
+                for (int e = 0; e < VN; ++e) z[VN * q + e] = has_row ? vget(t4, e) : (VN * q + e == r ? TV(1) : TV(0));
+            }
         }
     } else if (role == 0) {
 #pragma unroll
@@ -425,30 +561,30 @@ __global__ void __launch_bounds__(256, 4) k_chol_step(int k, int T, int64_t lda,
     if (role == 0) { diag_chain<TV, 0>(z, r, bad_any, inv); diag_publish<TV, 0>(z, r, inv, Lt, invp); }
     __syncthreads();
     if (role == 0) {
-        panel_trail<TV, 0>(z, Lt);
+        diag_trail<TV, 0>(z, Lt);
         diag_chain<TV, 1>(z, r, bad_any, inv);
         diag_publish<TV, 1>(z, r, inv, Lt, invp);
     } else if (role == 1) {
         row_solve<TV, 0>(z, Lt, invp, r, out, out_ld);
-        panel_trail<TV, 0>(z, Lt);
+        row_trail<TV, 0>(z, Lt, r);
     }
     __syncthreads();
     if (role == 0) {
-        panel_trail<TV, 1>(z, Lt);
+        diag_trail<TV, 1>(z, Lt);
         diag_chain<TV, 2>(z, r, bad_any, inv);
         diag_publish<TV, 2>(z, r, inv, Lt, invp);
     } else if (role == 1) {
         row_solve<TV, 1>(z, Lt, invp, r, out, out_ld);
-        panel_trail<TV, 1>(z, Lt);
+        row_trail<TV, 1>(z, Lt, r);
     }
     __syncthreads();
     if (role == 0) {
-        panel_trail<TV, 2>(z, Lt);
+        diag_trail<TV, 2>(z, Lt);
         diag_chain<TV, 3>(z, r, bad_any, inv);
         diag_publish<TV, 3>(z, r, inv, Lt, invp);
     } else if (role == 1) {
         row_solve<TV, 2>(z, Lt, invp, r, out, out_ld);
-        panel_trail<TV, 2>(z, Lt);
+        row_trail<TV, 2>(z, Lt, r);
     }
     __syncthreads();
     if (role == 1) row_solve<TV, 3>(z, Lt, invp, r, out, out_ld);
