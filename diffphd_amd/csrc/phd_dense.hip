@@ -281,15 +281,18 @@ __device__ __forceinline__ int mfma_col(int v, int lane) { return (v & 3) + 4 * 
 
 __device__ __forceinline__ f32x16 tile_product_mfma(const float (*A)[kTS], const float (*B)[kTS], int wave, int lane) {
     const int wi = wave >> 1, wj = wave & 1, i = lane & 31, h = lane >> 5;
-    f32x16 acc;
+    // two accumulation chains (steps st and st + CB / 4) so that consecutive
+    // MFMAs do not wait on each other's results
+    f32x16 acc0, acc1;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-#pragma unroll 8
-    for (int st = 0; st < CB / 2; ++st) {
-        const int l = 2 * st + h;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(B[l][32 * wj + i], A[l][32 * wi + i], acc, 0, 0, 0);
+    for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = 0.f;
+#pragma unroll 4
+    for (int st = 0; st < CB / 4; ++st) {
+        const int l0 = 2 * st + h, l1 = CB / 2 + 2 * st + h;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(B[l0][32 * wj + i], A[l0][32 * wi + i], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(B[l1][32 * wj + i], A[l1][32 * wi + i], acc1, 0, 0, 0);
     }
-    return acc;
+    return acc0 + acc1;
 }
 
 // Exchange of the upper half-wave of x with the lower half-wave of y
@@ -404,16 +407,24 @@ template <class TV>
 __device__ __forceinline__ void update_block(TV* __restrict__ M, int64_t lda, int kp, int ib, int jb, TV (*S0)[kTS],
                                              TV (*S1)[kTS]) {
     const int t = threadIdx.x, tx = t % 64, ty = t / 64;
-    stage_block<TV>(M, lda, (int64_t)ib * CB, (int64_t)kp * CB, S0);
-    stage_block<TV>(M, lda, (int64_t)jb * CB, (int64_t)kp * CB, S1);
-    __syncthreads();
     if constexpr (std::is_same<TV, float>::value) {
+        // the target block is read together with the staging loads, so its
+        // latency overlaps theirs instead of following the product
         const int lane = t % 64, wave = t / 64;
-        const f32x16 acc = tile_product_mfma(S0, S1, wave, lane);
         float* dst = M + (int64_t)ib * CB + 32 * (wave >> 1) + (lane & 31) + ((int64_t)jb * CB + 32 * (wave & 1)) * lda;
+        float old[16];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) dst[mfma_col(v, lane) * lda] -= acc[v];
+        for (int v = 0; v < 16; ++v) old[v] = dst[mfma_col(v, lane) * lda];
+        stage_block<TV>(M, lda, (int64_t)ib * CB, (int64_t)kp * CB, S0);
+        stage_block<TV>(M, lda, (int64_t)jb * CB, (int64_t)kp * CB, S1);
+        __syncthreads();
+        const f32x16 acc = tile_product_mfma(S0, S1, wave, lane);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) dst[mfma_col(v, lane) * lda] = old[v] - acc[v];
     } else {
+        stage_block<TV>(M, lda, (int64_t)ib * CB, (int64_t)kp * CB, S0);
+        stage_block<TV>(M, lda, (int64_t)jb * CB, (int64_t)kp * CB, S1);
+        __syncthreads();
         TV acc[16];
         tile_product<TV>(S0, S1, tx, ty, acc);
         TV* dst = M + (int64_t)ib * CB + tx + ((int64_t)jb * CB + 16 * ty) * lda;
