@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 64,64,32]
                     [--precision mixed|fp64] [--no-fp64] [--no-cpu-baseline]
+                    [--converging F] [--no-warm-start]
 
 One step = one forward PD time step plus its adjoint step on the 64x64x32
 hex press scene (131,072 elements, 139,425 vertices, ~418k DOFs; corotated +
@@ -9,7 +10,9 @@ volume with a 10x log-space random stiffness field; heightfield ground and a
 descending press plane, friction 0.3). Inputs are resident in HBM; the timed
 region covers exactly K forward frames and the K matching backward frames.
 The FP32/FP64 "mixed" solver is the measured path; the plain FP64 path is
-timed beside it for the speedup figure. Prints one JSON line.
+timed beside it for the speedup figure. The "converging" object adds a
+time-to-tolerance leg (the frictionless press from rest, where the forward
+loop reaches the 1e-9 dual gate) and the learned initial iterate on it. Prints one JSON line.
 """
 import argparse
 import json
@@ -40,6 +43,13 @@ def parse():
     p.add_argument("--prep-iters", type=int, default=100, help="PD iteration cap of the untimed preparation frames")
     p.add_argument("--phases", action="store_true",
                    help="synchronize at phase boundaries and report the wall-clock split (perturbs the timing)")
+    p.add_argument("--converging", type=int, default=3,
+                   help="frames of the time-to-tolerance leg (frictionless press from rest at the 1e-9 dual gate, "
+                        "FP64 and mixed); 0 skips it")
+    p.add_argument("--converging-cap", type=int, default=3000, help="PD iteration cap of the time-to-tolerance leg")
+    p.add_argument("--no-warm-start", action="store_true",
+                   help="skip training the learned warm start on plain-solver press rollouts and comparing it with "
+                        "q_t and q + h v on the time-to-tolerance leg")
     p.add_argument("--profile-spmv", type=int, default=0,
                    help="only time N global solves (FP32 and FP64 factor) and a 1 GiB copy, for profiling")
     return p.parse_args()
@@ -141,6 +151,84 @@ def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False):
         "nnz": info["nnz_S"], "max_depth": info["max_depth"], "factor_s": info["factor_s"],
         "active": [len(a) for a in acts[1:]], "q": qq, "dl_dq": dq0, "dl_dv": dv0, "bwd": bstats,
     }
+
+
+def converging_leg(args, cells):
+    """Time to tolerance: the press without friction from rest (frames 0 ..
+    F-1: the block settles on the ground, the plane is still above it), where
+    the forward PD loop reaches the 1e-9 dual gate, forward + backward in FP64
+    and in mixed precision from the same state. Setup (factorization) is
+    outside the timed region."""
+    from diffphd_amd import scenes
+    from diffphd_amd.py_diff_pd_core import HexDeformable
+    F = args.converging
+    spec = scenes.press(cells=cells, fmu=0.0, frames=max(100, F))
+    out = {"workload": f"press {cells[0]}x{cells[1]}x{cells[2]} hex, friction 0, frames 0-{F - 1} from rest, "
+                       f"fwd+bwd, dual gate 1e-9 (cap {args.converging_cap})"}
+    states = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        body = scenes.apply(spec, HexDeformable(0))
+        body.factor_build(spec["dt"], args.threads)
+        opt = dict(spec["options"], rel_tol=1e-9, max_pd_iter=args.converging_cap, phd_precision=prec,
+                   thread_ct=args.threads)
+        body.reset_frame_stats()
+        t0 = time.perf_counter()
+        q, v, acts, dq, dv, bstats = run_steps(body, spec, opt, spec["q0"], spec["v0"], 0, F, BWD_OPTS)
+        secs = time.perf_counter() - t0
+        iters, conv, _ = body.frame_stats()
+        rows, clipped, bwd_iters = bwd_summary(bstats)
+        out[name] = {"steps_per_s": F / secs, "seconds": secs, "fwd_pd_iters": [int(x) for x in iters],
+                     "fwd_frames_converged": int(np.sum(conv)), "bwd_iters": bwd_iters,
+                     "bwd_converged": int(sum(r["backbone_converged"] for r in rows)), "clipped_bwd_frames": clipped}
+        states[name] = (q[-1], dq, dv)
+        log(f"converging leg {name}: {F / secs:.3f} steps/s, forward iterations {list(iters)}")
+        del body
+
+    def dev(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["rel_dev_mixed_vs_fp64"] = {k: dev(a, b) for k, a, b in zip(("q_end", "dl_dq_start", "dl_dv_start"),
+                                                                     states["mixed"], states["fp64"])}
+    return out
+
+
+def warm_start_leg(args, cells):
+    """The learned warm start (per-vertex MLP 13 -> 3 x 128 -> 3, BF16 weights
+    on MFMA) on the time-to-tolerance press in mixed precision:
+    trained on the GPU from plain-solver rollouts of two other presses (their
+    own stiffness fields and grounds, seeded random initial velocities), then
+    the same frames from rest with q_t, q + h v and the learned start."""
+    from diffphd_amd import scenes, warm_start as W
+    from diffphd_amd.py_diff_pd_core import HexDeformable
+    F = args.converging
+    scene = lambda s: scenes.press(cells=cells, fmu=0.0, frames=max(100, F), seed=s)
+    make = lambda spec: scenes.apply(spec, HexDeformable(0))
+    spec = scene(0)
+    opt = dict(spec["options"], rel_tol=1e-9, max_pd_iter=args.converging_cap, phd_precision=1,
+               thread_ct=args.threads)
+    t0 = time.perf_counter()
+    feats, targets, plain_its = W.press_training_set(make, scene, seeds=[1, 2], rollouts=2, frames=F, opt=opt)
+    t_data = time.perf_counter() - t0
+    body = make(spec)
+    body.mlp_init(0)
+    t0 = time.perf_counter()
+    losses = W.train(body, feats, targets, steps=2000, batch=4096, lr=1e-3, seed=0)
+    t_train = time.perf_counter() - t0
+    norm = W.normalization(feats, targets)
+    fh, th, _, sh = W.collect(body, spec, F, opt, [spec["v0"]], spec["plane_schedule"], with_step=True)
+    err = W.start_errors(body, norm, fh, th, sh)
+    res = W.compare_starts(body, spec, F, opt)
+    out = {"training": {"samples": int(feats.shape[0]), "frames": len(plain_its), "data_s": t_data,
+                        "train_s": t_train, "adam_steps": 2000, "loss_first": float(np.mean(losses[:20])),
+                        "loss_last": float(np.mean(losses[-20:]))},
+           "start_distance_to_converged": err}
+    for mode, name in ((0, "q_t"), (2, "q_plus_hv"), (1, "learned")):
+        r = res[mode]
+        out[name] = {"steps_per_s_fwd": F / r["seconds"], "fwd_pd_iters": r["iters"],
+                     "fwd_frames_converged": int(sum(r["converged"])), "used": r["used"], "fallback": r["fallback"]}
+        log(f"warm start {name}: {F / r['seconds']:.3f} fwd steps/s, iterations {r['iters']}")
+    out["learned"]["fallback_rate"] = res[1]["fallback"] / max(F, 1)
+    return out
 
 
 def bwd_summary(bstats):
@@ -285,7 +373,8 @@ def main():
         "data": "synthetic (seeded press scene: GRF stiffness field, sinusoid heightfield)",
         "config": {"workload": f"press {cells[0]}x{cells[1]}x{cells[2]} hex, fwd+bwd PD step, friction 0.3",
                    "elements": int(np.prod(cells)), "dofs": int(n_dof), "precision": args.precision,
-                   "warm_start": "q_t (the reference's initial iterate; the learned start is not in this line)",
+                   "warm_start": "q_t (the reference's initial iterate; the learned start is measured on the "
+                                 "time-to-tolerance leg, converging.warm_start)",
                    "backward": "per-element K_pd, trust region on the proximal-map spectrum (stage 0 = unfiltered)",
                    "timed_frames": [args.start_frame + args.warmup, args.start_frame + args.warmup + K - 1],
                    "active_contacts": res["active"], "fwd_pd_iters_timed": fwd_iters,
@@ -304,9 +393,10 @@ def main():
     }
     any_clipped = clipped > 0
     if not args.no_fp64 and args.precision == "mixed":
-        del res["body"]
+        res.pop("body", None)
         r64 = time_path(spec, "fp64", K, args.warmup, args.threads, q0, v0, args.start_frame, args.phases)
         sps64 = K / r64["elapsed"]
+        r64.pop("body", None)
         rows64, clipped64, bwd64 = bwd_summary(r64["bwd"])
         any_clipped |= clipped64 > 0
 
@@ -322,6 +412,16 @@ def main():
         out["rel_dev_vs_fp64"] = {"q_end": dev(res["q"][-1], r64["q"][-1]),
                                   "dl_dq_start": dev(res["dl_dq"], r64["dl_dq"]),
                                   "dl_dv_start": dev(res["dl_dv"], r64["dl_dv"])}
+    if args.converging > 0:
+        res.pop("body", None)
+        out["converging"] = converging_leg(args, cells)
+        any_clipped |= out["converging"]["fp64"]["clipped_bwd_frames"] + out["converging"]["mixed"]["clipped_bwd_frames"] > 0
+        if not args.no_warm_start:
+            try:
+                out["converging"]["warm_start"] = warm_start_leg(args, cells)
+            except Exception as e:  # reported in the line; the timed legs above stand
+                log(f"warm start leg failed: {e!r}")
+                out["converging"]["warm_start"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, target_cells=cells)
     print(json.dumps(out))

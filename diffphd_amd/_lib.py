@@ -43,6 +43,7 @@ SIGNATURES = [
     ("phd_cached_n", ctypes.c_int, [_P]),
     ("phd_frame_stats", ctypes.c_int, [_P, _ip, _ip, _dp, ctypes.c_int]),
     ("phd_reset_frame_stats", None, [_P]),
+    ("phd_reset_contact_history", ctypes.c_int, [_P]),
     ("phd_last_backward_iters", ctypes.c_int, [_P]),
     ("phd_last_backward_stats", ctypes.c_int, [_P, _dp, ctypes.c_int]),
     ("phd_factor_info", ctypes.c_int, [_P, _dp, ctypes.c_int]),

@@ -139,6 +139,9 @@ int phd_frame_stats(phd_ctx ctx, int* iters, int* converged, double* seconds, in
 void phd_reset_frame_stats(phd_ctx ctx) {
     if (ctx) static_cast<Ctx*>(ctx)->body->frame_stats.clear();
 }
+int phd_reset_contact_history(phd_ctx ctx) {
+    return guard(ctx, [&](phd::Body& b) { b.reset_contact_history(); });
+}
 int phd_last_backward_iters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->last_backward_iters : -1; }
 int phd_last_backward_stats(phd_ctx ctx, double* out, int cap) {
     if (!ctx) return -1;

@@ -200,6 +200,9 @@ public:
     void mlp_features(const double* q_dev, const double* v_dev, double dt, bool standardize);
     void set_inertial_target(const double* q, const double* v, const double* f_ext, double dt);
     const float* feat_dev() const { return d_feat_.p; }
+    // forget the previous frame's contact multipliers (the NCP warm start and
+    // the network's contact flags), as in a newly constructed body
+    void reset_contact_history() { prev_candidates_.clear(); prev_lambda_.clear(); }
     int warm_used = 0, warm_fallback = 0;
 
 private:

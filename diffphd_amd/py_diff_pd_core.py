@@ -270,6 +270,9 @@ class _Deformable:
     def reset_frame_stats(self):
         _lib.lib().phd_reset_frame_stats(self._ctx)
 
+    def reset_contact_history(self):
+        self._ok(_lib.lib().phd_reset_contact_history(self._ctx))
+
     def last_backward_iters(self):
         return _lib.lib().phd_last_backward_iters(self._ctx)
 

@@ -116,6 +116,11 @@ long long phd_s_nnz(phd_ctx ctx);
 int phd_cached_n(phd_ctx ctx);
 int phd_frame_stats(phd_ctx ctx, int* iters, int* converged, double* seconds, int cap);
 void phd_reset_frame_stats(phd_ctx ctx);
+/* Drops the previous frame's contact multipliers, which warm-start the next
+ * frame's NCP iteration (the reference keeps them in its GPU state,
+ * alg_phd_gpu_memory.h, so a new Deformable starts without them): rollouts
+ * from different initial states on one body then start alike. */
+int phd_reset_contact_history(phd_ctx ctx);
 int phd_last_backward_iters(phd_ctx ctx);
 /* Last backward step (no reference counterpart; the reference prints these):
  * [0] backbone fixed-point iterations, [1] backbone converged (0/1), [2] backbone
