@@ -47,6 +47,8 @@ def parse():
                    help="frames of the time-to-tolerance leg (frictionless press from rest at the 1e-9 dual gate, "
                         "FP64 and mixed); 0 skips it")
     p.add_argument("--converging-cap", type=int, default=3000, help="PD iteration cap of the time-to-tolerance leg")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default launch shapes instead of timing the candidates after the factorization")
     p.add_argument("--no-warm-start", action="store_true",
                    help="skip training the learned warm start on plain-solver press rollouts and comparing it with "
                         "q_t and q + h v on the time-to-tolerance leg")
@@ -121,7 +123,7 @@ def prepare_state(spec, args):
     return q, v
 
 
-def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False):
+def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False, autotune=True):
     from diffphd_amd import scenes
     from diffphd_amd.py_diff_pd_core import HexDeformable
     body = scenes.apply(spec, HexDeformable(0))
@@ -130,6 +132,12 @@ def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False):
     t0 = time.perf_counter()
     body.factor_build(spec["dt"], threads)
     t_setup = time.perf_counter() - t0
+    tune = None
+    if autotune:
+        t0 = time.perf_counter()
+        tune = body.autotune(spec["dt"], threads, reps=5)
+        tune["seconds"] = time.perf_counter() - t0
+        log(f"{precision}: autotune {tune}")
     log(f"{precision}: setup {t_setup:.1f}s {body.factor_info()}")
     q, v = [q0], [v0]
     if W > 0:
@@ -142,14 +150,17 @@ def time_path(spec, precision, K, W, threads, q0, v0, frame0, phases=False):
     qq, vv, acts, dq0, dv0, bstats = run_steps(body, spec, opt, q[-1], v[-1], frame0 + W, K, BWD_OPTS)
     elapsed = time.perf_counter() - t0
     tim = body.timing_read(reset=True)
+    ph = body.phase_times(reset=True)
     if phases:
-        log(f"{precision} phases: " + json.dumps({k: round(v, 4) for k, v in body.phase_times(reset=True).items()}))
+        log(f"{precision} phases: " + json.dumps({k: round(v, 4) for k, v in ph.items()}))
     iters, conv, _ = body.frame_stats()
     info = body.factor_info()
     return {
         "body": body, "elapsed": elapsed, "setup_s": t_setup, "timing": tim, "fwd_iters": iters, "fwd_conv": conv,
-        "nnz": info["nnz_S"], "max_depth": info["max_depth"], "factor_s": info["factor_s"],
+        "nnz": info["nnz_S"], "max_depth": info["max_depth"], "factor_s": info["factor_s"], "autotune": tune,
         "active": [len(a) for a in acts[1:]], "q": qq, "dl_dq": dq0, "dl_dv": dv0, "bwd": bstats,
+        "dense": {"solves": int(ph.get("dense_calls", 0)), "mean_rows": ph.get("dense_rows", 0) / max(ph.get("dense_calls", 0), 1),
+                  "fp32_breakdowns_to_fp64": int(ph.get("dense_fp32_fail", 0))},
     }
 
 
@@ -335,7 +346,8 @@ def main():
         profile_spmv(spec, args)
         return
     q0, v0 = prepare_state(spec, args) if args.start_frame > 0 else (spec["q0"], spec["v0"])
-    res = time_path(spec, args.precision, args.steps, args.warmup, args.threads, q0, v0, args.start_frame, args.phases)
+    res = time_path(spec, args.precision, args.steps, args.warmup, args.threads, q0, v0, args.start_frame, args.phases,
+                    not args.no_autotune)
     K = args.steps
     steps_per_s = K / res["elapsed"]
     nnz = res["nnz"]
@@ -380,7 +392,9 @@ def main():
                    "active_contacts": res["active"], "fwd_pd_iters_timed": fwd_iters,
                    "fwd_frames_converged": int(np.sum(res["fwd_conv"])), "max_pd_iter": spec["options"]["max_pd_iter"],
                    "bwd_iters_timed": bwd_iters, "clipped_bwd_frames": clipped, "bwd_frames": bwd_rows,
-                   "setup_s": res["setup_s"], "factor_nnz": nnz, "factor_max_depth": res["max_depth"]},
+                   "dense_contact_solves": res["dense"],
+                   "setup_s": res["setup_s"], "factor_nnz": nnz, "factor_max_depth": res["max_depth"],
+                   "autotune": res["autotune"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": "profiles/spmv_press_*_pmc_traffic.md" if traffic else None,
@@ -394,7 +408,8 @@ def main():
     any_clipped = clipped > 0
     if not args.no_fp64 and args.precision == "mixed":
         res.pop("body", None)
-        r64 = time_path(spec, "fp64", K, args.warmup, args.threads, q0, v0, args.start_frame, args.phases)
+        r64 = time_path(spec, "fp64", K, args.warmup, args.threads, q0, v0, args.start_frame, args.phases,
+                        not args.no_autotune)
         sps64 = K / r64["elapsed"]
         r64.pop("body", None)
         rows64, clipped64, bwd64 = bwd_summary(r64["bwd"])
@@ -405,6 +420,7 @@ def main():
         out["fp64_steps_per_s"] = sps64
         out["speedup_vs_fp64"] = steps_per_s / sps64
         out["fp64"] = {"frames": out["config"]["timed_frames"], "fwd_pd_iters_timed": int(np.sum(r64["fwd_iters"])),
+                       "dense_contact_solves": r64["dense"],
                        "fwd_frames_converged": int(np.sum(r64["fwd_conv"])), "bwd_iters_timed": bwd64,
                        "clipped_bwd_frames": clipped64, "bwd_frames": rows64}
         # same start state, same frames: mixed against FP64 at the end of the

@@ -47,6 +47,8 @@ SIGNATURES = [
     ("phd_last_backward_iters", ctypes.c_int, [_P]),
     ("phd_last_backward_stats", ctypes.c_int, [_P, _dp, ctypes.c_int]),
     ("phd_factor_info", ctypes.c_int, [_P, _dp, ctypes.c_int]),
+    ("phd_autotune", ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]),
+    ("phd_set_launch_shapes", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("phd_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("phd_timing_read", ctypes.c_int, [_P, _dp, ctypes.c_int]),
     ("phd_phase_times", ctypes.c_int, [_P, _dp, ctypes.c_int, ctypes.c_int]),

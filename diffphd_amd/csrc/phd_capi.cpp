@@ -154,11 +154,24 @@ int phd_last_backward_stats(phd_ctx ctx, double* out, int cap) {
 int phd_factor_info(phd_ctx ctx, double* out, int cap) {
     return guard(ctx, [&](phd::Body& b) {
         const phd::Factor& f = b.factor();
-        const double v[12] = {(double)f.nnz, (double)f.v_len, (double)f.max_depth, f.order_seconds, f.factor_seconds,
+        const double v[15] = {(double)f.nnz, (double)f.v_len, (double)f.max_depth, f.order_seconds, f.factor_seconds,
                               b.setup_seconds, (double)b.plan().gu_group.size(), (double)b.plan().cu_tile.size(),
                               (double)f.Lx.size(), (double)b.contact_candidates(), (double)b.contact_entries(),
-                              (double)b.contact_rows()};
-        for (int i = 0; i < cap && i < 12; ++i) out[i] = v[i];
+                              (double)b.contact_rows(), (double)b.chunk_w(), (double)b.unit_items(),
+                              (double)b.local_block()};
+        for (int i = 0; i < cap && i < 15; ++i) out[i] = v[i];
+    });
+}
+int phd_set_launch_shapes(phd_ctx ctx, int chunk_w, int unit_items, int local_block) {
+    return guard(ctx, [&](phd::Body& b) { b.set_launch_shapes(chunk_w, unit_items, local_block); });
+}
+int phd_autotune(phd_ctx ctx, double dt, int threads, int reps, double* out, int cap) {
+    return guard(ctx, [&](phd::Body& b) {
+        const phd::Body::TuneResult r = b.autotune(dt, threads, reps);
+        const double v[13] = {(double)r.chunk_w, (double)r.unit_items, (double)r.local_block,
+                              r.rows_ms[0], r.rows_ms[1], r.rows_ms[2], r.cols_ms[0], r.cols_ms[1], r.cols_ms[2],
+                              r.cols_ms[3], r.local_ms[0], r.local_ms[1], r.local_ms[2]};
+        for (int i = 0; out && i < cap && i < 13; ++i) out[i] = v[i];
     });
 }
 int phd_timing_enable(phd_ctx ctx, int enable) {

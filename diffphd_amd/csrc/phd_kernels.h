@@ -47,11 +47,13 @@ void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t
                      const int* parent, const int* depth, const int* fd, const int64_t* base, const unsigned char* chain,
                      double* ws, int64_t ws_stride, double* V64, float* V32);
 
-// S w by (64-row group, 1024-column chunk) units with w staged in LDS; the
-// per-(row, chunk) partials are then summed per row (plan: SpmvPlan::gu_*, rbase).
+// S w by (64-row group, chunk_w-column chunk) units with w staged in LDS; the
+// per-(row, chunk) partials are then summed per row (plan: SpmvPlan::gu_*, rbase,
+// built for the same chunk_w, one of kSpmvChunks).
+constexpr int kSpmvChunks[3] = {512, 1024, 2048};
 template <class TV>
 void launch_spmv_rows_grp(hipStream_t s, int nunits, const int* gu_group, const int* gu_chunk, const int* fd,
-                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot);
+                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot, int chunk_w);
 template <class TV>
 void launch_spmv_rows_slots(hipStream_t s, int n, const int* rbase, const TV* pslot, TV* y);
 template <class TV>
@@ -66,7 +68,7 @@ template <class TV>
 void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, const unsigned char* dmask, TV* w);
 
 // ---- local step, gather, residual --------------------------------------------
-void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out);
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block = 128);
 // pd_rhs = sum of element contributions - c_dir; b = inv_h2m * q_tilde + pd_rhs
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b);

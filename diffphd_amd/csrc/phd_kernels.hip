@@ -12,6 +12,7 @@
 #include "phd_kernels.h"
 
 #include <cstring>
+#include <stdexcept>
 #include "phd_math.h"
 
 namespace phd {
@@ -120,15 +121,14 @@ template <> struct Vec16<float> { using type = float4; static constexpr int N = 
 template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };
 
 // -----------------------------------------------------------------------------
-// y = S w by (group of 64 consecutive rows, chunk of 1024 columns) units: the
+// y = S w by (group of 64 consecutive rows, chunk of CW columns) units: the
 // workgroup stages w on the chunk in LDS once, then its four waves stream the
 // rows of the group that cross the chunk (16-byte vectors, the whole chunk in
 // flight per row) against it. Row k's dot product on the chunk goes to its
-// slot rbase[k] + chunk - fd(k) / 1024; k_spmv_rows_slots sums the slots.
+// slot rbase[k] + chunk - fd(k) / CW; k_spmv_rows_slots sums the slots. CW is
+// one of kSpmvChunks (picked by the autotuner; the plan is built for it).
 // -----------------------------------------------------------------------------
-constexpr int GRP_CW = 1024;
-
-template <class TV>
+template <class TV, int CW>
 __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __restrict__ gu_group,
                                                        const int* __restrict__ gu_chunk, const int* __restrict__ fd,
                                                        const int64_t* __restrict__ base, const int* __restrict__ rbase,
@@ -136,16 +136,16 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
                                                        TV* __restrict__ pslot) {
     using V16 = typename Vec16<TV>::type;
     constexpr int VN = Vec16<TV>::N;
-    constexpr int U = GRP_CW / (VN * WAVE);
-    __shared__ __align__(16) TV ws[3][GRP_CW];
+    constexpr int U = CW / (VN * WAVE);
+    __shared__ __align__(16) TV ws[3][CW];
     const int u = blockIdx.x;
     if (u >= nunits) return;
     const int g = gu_group[u], ch = gu_chunk[u];
-    const int c0 = ch * GRP_CW, c1 = c0 + GRP_CW - 1;
+    const int c0 = ch * CW, c1 = c0 + CW - 1;
     const int ldw = spmv_ldw(n);
     // columns >= n stage as zeros whatever the padding of w holds: the masked
     // lanes below still multiply 0 by the staged value, and 0 * NaN is NaN
-    for (int e = (int)threadIdx.x * VN; e < GRP_CW; e += 256 * VN)
+    for (int e = (int)threadIdx.x * VN; e < CW; e += 256 * VN)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             V16 t;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) k_spmv_rows_grp(int nunits, const int* __
         a1 = wave_sum(a1);
         a2 = wave_sum(a2);
         if (lane == 0) {
-            TV* p = pslot + 3 * (int64_t)(__builtin_amdgcn_readlane(mrb, l) + ch - f / GRP_CW);
+            TV* p = pslot + 3 * (int64_t)(__builtin_amdgcn_readlane(mrb, l) + ch - f / CW);
             p[0] = a0;
             p[1] = a1;
             p[2] = a2;
@@ -405,7 +405,7 @@ __global__ void k_perm_in(int n, const int* __restrict__ perm, const double* __r
 // projection, contribution sum_energies w A_s^T vec(p(F)).
 // -----------------------------------------------------------------------------
 template <int EDIM>
-__global__ void __launch_bounds__(128) k_local_step(DevMaterials m, const double* __restrict__ q,
+__global__ void __launch_bounds__(256) k_local_step(DevMaterials m, const double* __restrict__ q,
                                                     double* __restrict__ elem_out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m.ne) return;
@@ -1304,10 +1304,24 @@ void launch_expand_s(hipStream_t s, int n, int t_begin, int t_end, const int64_t
 
 template <class TV>
 void launch_spmv_rows_grp(hipStream_t s, int nunits, const int* gu_group, const int* gu_chunk, const int* fd,
-                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot) {
+                          const int64_t* base, const int* rbase, const TV* V, const TV* w, int n, TV* pslot, int chunk_w) {
     if (nunits == 0) return;
-    hipLaunchKernelGGL(k_spmv_rows_grp<TV>, dim3(nunits), dim3(256), 0, s, nunits, gu_group, gu_chunk, fd, base, rbase, V, w,
-                       n, pslot);
+    switch (chunk_w) {
+    case 512:
+        hipLaunchKernelGGL((k_spmv_rows_grp<TV, 512>), dim3(nunits), dim3(256), 0, s, nunits, gu_group, gu_chunk, fd, base,
+                           rbase, V, w, n, pslot);
+        break;
+    case 1024:
+        hipLaunchKernelGGL((k_spmv_rows_grp<TV, 1024>), dim3(nunits), dim3(256), 0, s, nunits, gu_group, gu_chunk, fd, base,
+                           rbase, V, w, n, pslot);
+        break;
+    case 2048:
+        hipLaunchKernelGGL((k_spmv_rows_grp<TV, 2048>), dim3(nunits), dim3(256), 0, s, nunits, gu_group, gu_chunk, fd, base,
+                           rbase, V, w, n, pslot);
+        break;
+    default:
+        throw std::invalid_argument("row-group chunk width must be 512, 1024 or 2048");
+    }
 }
 template <class TV>
 void launch_spmv_rows_slots(hipStream_t s, int n, const int* rbase, const TV* pslot, TV* y) {
@@ -1333,7 +1347,7 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
 
 #define PHD_INST_TV(TV)                                                                                                  \
     template void launch_spmv_rows_grp<TV>(hipStream_t, int, const int*, const int*, const int*, const int64_t*,       \
-                                           const int*, const TV*, const TV*, int, TV*);                                \
+                                           const int*, const TV*, const TV*, int, TV*, int);                           \
     template void launch_spmv_rows_slots<TV>(hipStream_t, int, const int*, const TV*, TV*);                          \
     template void launch_spmv_cols<TV>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,    \
                                        const int64_t*, const TV*, const TV*, int, TV*);                                  \
@@ -1345,11 +1359,12 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
     template void launch_contact_scatter<TV>(hipStream_t, const ContactPieces&, const int*, const int*, const TV*,          \
                                              const double*, double*, int, TV*);
 
-void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out) {
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block) {
+    if (block != 64 && block != 128 && block != 256) throw std::invalid_argument("local-step block must be 64, 128 or 256");
     if (m.edim == 8)
-        hipLaunchKernelGGL(k_local_step<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, elem_out);
+        hipLaunchKernelGGL(k_local_step<8>, dim3(blocks_for(m.ne, block)), dim3(block), 0, s, m, q, elem_out);
     else
-        hipLaunchKernelGGL(k_local_step<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, elem_out);
+        hipLaunchKernelGGL(k_local_step<4>, dim3(blocks_for(m.ne, block)), dim3(block), 0, s, m, q, elem_out);
 }
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b) {

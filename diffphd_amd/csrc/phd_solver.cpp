@@ -420,7 +420,7 @@ void Body::setup(double dt, int threads) {
     }
     // factor (host FP64) and expansion of S = L^-1 (device FP64, FP32 copy)
     factor_.build(nv_, Ap_, Ai_, Ax_, X_, threads);
-    plan_.build(factor_, 512);
+    plan_.build(factor_, unit_items_, chunk_w_);
     {
         DevBuf<int64_t> dLp;
         DevBuf<int> dLi, dpar, ddep;
@@ -458,14 +458,6 @@ void Body::setup(double dt, int threads) {
         hip_check(hipGetLastError(), "expand S launch");
     }
     d_perm_.upload(factor_.perm);
-    d_gu_group_.upload(plan_.gu_group);
-    d_gu_chunk_.upload(plan_.gu_chunk);
-    d_rbase_.upload(plan_.rbase);
-    d_items_.upload(plan_.items);
-    d_cu_tile_.upload(plan_.cu_tile);
-    d_cu_i0_.upload(plan_.cu_i0);
-    d_cu_i1_.upload(plan_.cu_i1);
-    d_tile_unit_ptr_.upload(plan_.tile_unit_ptr);
     {  // A in ELL form for the residual kernel (entries in CSR row order, zero padding)
         ell_w_ = 0;
         for (int v = 0; v < nv_; ++v) ell_w_ = std::max(ell_w_, Ap_[v + 1] - Ap_[v]);
@@ -535,9 +527,7 @@ void Body::setup(double dt, int threads) {
     // k_perm_in writes columns < n only; the padding stays zero from here on
     hip_check(hipMemsetAsync(w32_.p, 0, w32_.n * sizeof(float), stream_), "w32 padding");
     hip_check(hipMemsetAsync(w64_.p, 0, w64_.n * sizeof(double), stream_), "w64 padding");
-    check(plan_.chunk_w == 1024, "row-group chunk width must match the kernel");
-    part32_.alloc(3 * (size_t)plan_.rbase[nv_] + 3); part64_.alloc(3 * (size_t)plan_.rbase[nv_] + 3);
-    xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
+    upload_plan();
     setup_contact(threads);
     ready_ = true;
     ready_dt_ = dt;
@@ -681,13 +671,103 @@ void Body::collect_timing() {
     ev_used_ = 0;
 }
 
+void Body::upload_plan() {
+    // kernels queued on stream_ may still read the previous plan's buffers
+    hip_check(hipStreamSynchronize(stream_), "plan upload");
+    d_gu_group_.upload(plan_.gu_group);
+    d_gu_chunk_.upload(plan_.gu_chunk);
+    d_rbase_.upload(plan_.rbase);
+    d_items_.upload(plan_.items);
+    d_cu_tile_.upload(plan_.cu_tile);
+    d_cu_i0_.upload(plan_.cu_i0);
+    d_cu_i1_.upload(plan_.cu_i1);
+    d_tile_unit_ptr_.upload(plan_.tile_unit_ptr);
+    part32_.alloc(3 * (size_t)plan_.rbase[nv_] + 3); part64_.alloc(3 * (size_t)plan_.rbase[nv_] + 3);
+    xpart32_.alloc(192 * plan_.cu_tile.size() + 192); xpart64_.alloc(192 * plan_.cu_tile.size() + 192);
+}
+
+void Body::set_launch_shapes(int chunk_w, int unit_items, int local_block) {
+    check(ready_, "set_launch_shapes needs the factor built (phd_factor_build).");
+    auto in = [](int v, const int* c, int n) { return std::find(c, c + n, v) != c + n; };
+    check(in(chunk_w, kTuneChunks, 3), "chunk width must be 512, 1024 or 2048.");
+    check(in(unit_items, kTuneUnits, 4), "unit height must be 256, 512, 1024 or 2048.");
+    check(in(local_block, kTuneBlocks, 3), "local-step block must be 64, 128 or 256.");
+    chunk_w_ = chunk_w;
+    unit_items_ = unit_items;
+    local_block_ = local_block;
+    plan_.build(factor_, unit_items_, chunk_w_);
+    upload_plan();
+}
+
+Body::TuneResult Body::autotune(double dt, int threads, int reps) {
+    check(reps >= 1 && reps <= 1000, "autotune: reps must be in [1, 1000].");
+    setup(dt, threads);
+    TuneResult res;
+    hipEvent_t ea, eb;
+    hip_check(hipEventCreate(&ea), "event");
+    hip_check(hipEventCreate(&eb), "event");
+    auto time_ms = [&](auto&& launch) {
+        launch();  // warm-up
+        hip_check(hipEventRecord(ea, stream_), "event");
+        for (int i = 0; i < reps; ++i) launch();
+        hip_check(hipEventRecord(eb, stream_), "event");
+        hip_check(hipEventSynchronize(eb), "event");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ea, eb), "event");
+        return (double)ms / reps;
+    };
+    // inputs: the rest positions as the right-hand side of S w, and as q
+    d_tmp_.upload(X_);
+    d_q_.upload(X_);
+    launch_perm_in<float>(stream_, nv_, d_perm_.p, d_tmp_.p, nullptr, w32_.p);
+    int best = 0;
+    for (int i = 0; i < 3; ++i) {
+        plan_.build(factor_, unit_items_, kTuneChunks[i]);
+        upload_plan();
+        res.rows_ms[i] = time_ms([&] {
+            launch_spmv_rows_grp<float>(stream_, (int)plan_.gu_group.size(), d_gu_group_.p, d_gu_chunk_.p, d_fd_.p,
+                                        d_base_.p, d_rbase_.p, V32_.p, w32_.p, nv_, part32_.p, kTuneChunks[i]);
+            launch_spmv_rows_slots<float>(stream_, nv_, d_rbase_.p, part32_.p, y32_.p);
+        });
+        if (res.rows_ms[i] < res.rows_ms[best]) best = i;
+    }
+    chunk_w_ = kTuneChunks[best];
+    best = 0;
+    for (int i = 0; i < 4; ++i) {
+        plan_.build(factor_, kTuneUnits[i], chunk_w_);
+        upload_plan();
+        res.cols_ms[i] = time_ms([&] {
+            launch_spmv_cols<float>(stream_, (int)plan_.cu_tile.size(), d_cu_tile_.p, d_cu_i0_.p, d_cu_i1_.p, d_items_.p,
+                                    d_fd_.p, d_base_.p, V32_.p, y32_.p, nv_, xpart32_.p);
+            launch_spmv_cols_fin<float>(stream_, nv_, d_tile_unit_ptr_.p, xpart32_.p, d_perm_.p, nullptr, nullptr,
+                                        nullptr, d_g_.p);
+        });
+        if (res.cols_ms[i] < res.cols_ms[best]) best = i;
+    }
+    unit_items_ = kTuneUnits[best];
+    best = 0;
+    for (int i = 0; i < 3; ++i) {
+        res.local_ms[i] = time_ms([&] { launch_local_step(stream_, mat_, d_q_.p, d_elem_.p, kTuneBlocks[i]); });
+        if (res.local_ms[i] < res.local_ms[best]) best = i;
+    }
+    local_block_ = kTuneBlocks[best];
+    plan_.build(factor_, unit_items_, chunk_w_);
+    upload_plan();
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    res.chunk_w = chunk_w_;
+    res.unit_items = unit_items_;
+    res.local_block = local_block_;
+    return res;
+}
+
 template <class TV>
 void Body::spmv_S(const double* r, DevBuf<TV>& w, DevBuf<TV>& y, DevBuf<TV>& part, const TV* V) {
     launch_perm_in<TV>(stream_, nv_, d_perm_.p, r, nullptr, w.p);
     hipEvent_t a = nullptr, b = nullptr;
     if (timing) { a = next_event(); b = next_event(); (void)hipEventRecord(a, stream_); }
     launch_spmv_rows_grp<TV>(stream_, (int)plan_.gu_group.size(), d_gu_group_.p, d_gu_chunk_.p, d_fd_.p, d_base_.p,
-                             d_rbase_.p, V, w.p, nv_, part.p);
+                             d_rbase_.p, V, w.p, nv_, part.p, chunk_w_);
     if (timing) { (void)hipEventRecord(b, stream_); ev_rows_.push_back({a, b}); }
     launch_spmv_rows_slots<TV>(stream_, nv_, d_rbase_.p, part.p, y.p);
 }
@@ -714,7 +794,7 @@ void Body::contact_scatter(const double* f, DevBuf<TV>& y) {
 }
 
 void Body::local_rhs(const double* q, double inv_h2m) {
-    launch_local_step(stream_, mat_, q, d_elem_.p);
+    launch_local_step(stream_, mat_, q, d_elem_.p, local_block_);
     launch_gather_rhs(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, d_c_dir_.p, d_qtilde_.p, inv_h2m, d_pdrhs_.p, d_b_.p);
 }
 
@@ -933,7 +1013,7 @@ void Body::ainv_passes(const double* b_dev, double* x_dev, int refine, char* scr
         }
         launch_perm_in<TV>(stream_, nv_, d_perm_.p, rhs, nullptr, w);
         launch_spmv_rows_grp<TV>(stream_, (int)plan_.gu_group.size(), d_gu_group_.p, d_gu_chunk_.p, d_fd_.p, d_base_.p,
-                                 d_rbase_.p, V, w, nv_, part);
+                                 d_rbase_.p, V, w, nv_, part, chunk_w_);
         launch_spmv_rows_slots<TV>(stream_, nv_, d_rbase_.p, part, y);
         launch_spmv_cols<TV>(stream_, (int)plan_.cu_tile.size(), d_cu_tile_.p, d_cu_i0_.p, d_cu_i1_.p, d_items_.p,
                              d_fd_.p, d_base_.p, V, y, nv_, xpart);
@@ -1211,12 +1291,13 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
             int solved_prec = -1;  // precision of the solution in d_cx32_ / d_cx64_
             if (nJ > 0) {
                 PhaseTimer td(this, kFwdDense);
-                if (timing_level >= 2) { phase[kDenseCalls] += 1; phase[kDenseRows] += nJ; }
+                phase[kDenseCalls] += 1;  // counters are kept at every timing level
+                phase[kDenseRows] += nJ;
                 if (precision == 1 && !dense64) {
                     if (solve_contact_system(nJ, 1, listJ, rp)) solved_prec = 1;
                     else {
                         dense64 = true;  // the FP32 Cholesky broke down: FP64 for the rest of the frame
-                        if (timing_level >= 2) phase[kDenseFp32Fail] += 1;
+                        phase[kDenseFp32Fail] += 1;
                     }
                 }
                 if (solved_prec < 0) {  // FP64 system (also the fallback when the FP32 Cholesky breaks down)
@@ -1653,7 +1734,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                     d_bclist.upload(clist);
                 }
                 std::unique_ptr<PhaseTimer> tbf(new PhaseTimer(this, kBwdBasisFactor));
-                if (timing_level >= 2) phase[kBasisRows] += nb;
+                phase[kBasisRows] += nb;
                 d_G.alloc((size_t)nb * nb);
                 launch_assemble_g(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, d_G.p);
                 if ((int)d_rhsK_.n < nb) d_rhsK_.alloc(nb);

@@ -157,6 +157,25 @@ public:
     size_t noalloc_scratch_bytes(int precision) const;
     void setup(double dt, int threads);
 
+    // ---- autotuner of the launch shapes (factor built for dt first) ----
+    // Times `reps` launches of each candidate on this body's factor and mesh
+    // and keeps the fastest: the column-chunk width of S w (FP32, the mixed
+    // path's product), the unit height of S^T y, the local step's block size.
+    struct TuneResult {
+        int chunk_w = 0, unit_items = 0, local_block = 0;
+        double rows_ms[3] = {0}, cols_ms[4] = {0}, local_ms[3] = {0};
+    };
+    static constexpr int kTuneChunks[3] = {512, 1024, 2048};
+    static constexpr int kTuneUnits[4] = {256, 512, 1024, 2048};
+    static constexpr int kTuneBlocks[3] = {64, 128, 256};
+    TuneResult autotune(double dt, int threads, int reps);
+    // sets the three shapes directly (each from its candidate list); the
+    // factor must be built
+    void set_launch_shapes(int chunk_w, int unit_items, int local_block);
+    int chunk_w() const { return chunk_w_; }
+    int unit_items() const { return unit_items_; }
+    int local_block() const { return local_block_; }
+
     // ---- statistics ----
     std::vector<StepStats> frame_stats;
     std::vector<double> last_lambda_, last_lt1_, last_lt2_;
@@ -206,6 +225,9 @@ public:
     int warm_used = 0, warm_fallback = 0;
 
 private:
+    // launch shapes (defaults; autotune() replaces them)
+    int chunk_w_ = 1024, unit_items_ = 512, local_block_ = 128;
+    void upload_plan();  // plan_ buffers and the partial-sum arrays sized by it
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     void* scratch_ = nullptr;  // device address of the library's scratch array on device_

@@ -252,11 +252,24 @@ class _Deformable:
                                                    ctypes.c_void_p(int(x_dev_ptr)), int(precision), int(refine)))
 
     def factor_info(self):
-        out, op = _lib.dptr(np.zeros(12))
-        self._ok(_lib.lib().phd_factor_info(self._ctx, op, 12))
+        out, op = _lib.dptr(np.zeros(15))
+        self._ok(_lib.lib().phd_factor_info(self._ctx, op, 15))
         keys = ["nnz_S", "v_len", "max_depth", "order_s", "factor_s", "setup_s", "row_units", "col_units", "nnz_L", "K",
-                "contact_entries", "contact_rows"]
+                "contact_entries", "contact_rows", "chunk_w", "unit_items", "local_block"]
         return dict(zip(keys, out.tolist()))
+
+    def autotune(self, dt, threads=8, reps=10):
+        """Times the launch-shape candidates on this body and keeps the fastest
+        (phd_autotune); returns the choice and the per-candidate milliseconds."""
+        out, op = _lib.dptr(np.zeros(13))
+        self._ok(_lib.lib().phd_autotune(self._ctx, float(dt), int(threads), int(reps), op, 13))
+        return {"chunk_w": int(out[0]), "unit_items": int(out[1]), "local_block": int(out[2]),
+                "rows_ms": dict(zip((512, 1024, 2048), out[3:6].tolist())),
+                "cols_ms": dict(zip((256, 512, 1024, 2048), out[6:10].tolist())),
+                "local_ms": dict(zip((64, 128, 256), out[10:13].tolist()))}
+
+    def set_launch_shapes(self, chunk_w, unit_items, local_block):
+        self._ok(_lib.lib().phd_set_launch_shapes(self._ctx, int(chunk_w), int(unit_items), int(local_block)))
 
     def frame_stats(self):
         L = _lib.lib()
@@ -359,7 +372,9 @@ class _Deformable:
               "fwd_c_sets", "fwd_c_apply", "fwd_friction")
 
     def phase_times(self, reset=True):
-        """Wall-clock split (s) recorded with timing_enable(2)."""
+        """Wall-clock split (s) recorded with timing_enable(2); the counters
+        (dense_calls, dense_rows, dense_fp32_fail, basis_rows) are kept at
+        every timing level."""
         out, op = _lib.dptr(np.zeros(len(self.PHASES)))
         n = _lib.lib().phd_phase_times(self._ctx, op, len(self.PHASES), int(reset))
         return dict(zip(self.PHASES[:n], out[:n].tolist()))

@@ -136,8 +136,23 @@ int phd_last_backward_stats(phd_ctx ctx, double* out, int cap);
 /* Factor description: out[0] nnz(S), [1] padded value count, [2] max etree depth,
  * [3] ordering seconds, [4] factor seconds, [5] setup seconds, [6] row units,
  * [7] column units, [8] nnz(L), [9] contact candidates K, [10] entries of the
- * candidate paths of S, [11] rows crossed by those paths (up to `cap` values). */
+ * candidate paths of S, [11] rows crossed by those paths, [12] column-chunk
+ * width of S w, [13] unit height of S^T y, [14] local-step block size (up to
+ * `cap` values). */
 int phd_factor_info(phd_ctx ctx, double* out, int cap);
+/* Autotuner of the launch shapes (no counterpart in the reference, whose
+ * cuSPARSE products have none to tune): builds the factor for dt if needed,
+ * times `reps` launches of each candidate on this body and keeps the fastest
+ * of the column-chunk width of S w (512, 1024, 2048; FP32 product), the unit
+ * height of S^T y (256, 512, 1024, 2048 rows) and the local step's block size
+ * (64, 128, 256). Results are independent of the choice up to the summation
+ * order of the partial sums. out: [0] chunk width, [1] unit height, [2] block,
+ * [3..5] ms per S w, [6..9] ms per S^T y, [10..12] ms per local step, one per
+ * candidate (up to `cap` values; out may be NULL). */
+int phd_autotune(phd_ctx ctx, double dt, int threads, int reps, double* out, int cap);
+/* Sets the three launch shapes directly (values from the candidate lists
+ * above), e.g. to reuse a tuned choice; the factor must be built. */
+int phd_set_launch_shapes(phd_ctx ctx, int chunk_w, int unit_items, int local_block);
 /* Device-event timing of the S (rows) and S^T (columns) kernels: enable = 1 starts
  * recording; phd_timing_read writes [rows_ms, rows_launches, cols_ms, cols_launches]
  * accumulated since the last reset (reset = 1 clears after reading). */
