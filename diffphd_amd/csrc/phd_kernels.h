@@ -72,10 +72,26 @@ void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, do
 // pd_rhs = sum of element contributions - c_dir; b = inv_h2m * q_tilde + pd_rhs
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b);
+// Per-DOF Dirichlet (the bordered correction of a global solve on the shared
+// factor; PdirAxes: axis d's fixed vertices are fv[off[d] .. off[d + 1]) and
+// its G_d^-1 starts at ginv + goff[d]).
+struct PdirAxes {
+    int off[4];
+    long long goff[3];
+};
+void launch_pdir_coef(hipStream_t s, int np, const PdirAxes& ax, const int* map, const int* fv, const double* ginv,
+                      const double* y, const double* t, double* coef);
+void launch_pdir_apply(hipStream_t s, int nv, int np, const double* Z, const double* coef, const unsigned char* pmask,
+                       const double* o, double* y);
+void launch_pdir_columns(hipStream_t s, int nv, int p0, int na, const double* x, double* Z);
+void launch_pdir_gather(hipStream_t s, int np, int nv, const int* pv, const double* Z, double* out);
 // r = b - A q for the three axes (AoS); A in ELL form (entry j of row v at
-// j * nv + v, width entries per row), Dirichlet rows (dmask) set to zero.
+// j * nv + v, width entries per row); dmask: whole-vertex Dirichlet rows
+// (r = 0); pmask (per DOF, may be null): DOFs fixed on their own (r = 0, and
+// their couplings are left out on that axis)
 void launch_residual(hipStream_t s, int nv, int width, const int* Ai, const double* Ax, const double* b, const double* q,
-                     const unsigned char* dmask, double* r);
+                     const unsigned char* dmask, double* r,
+                     const unsigned char* pmask = nullptr);
 void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out);
 
 // ---- reductions and vector updates -------------------------------------------

@@ -514,11 +514,27 @@ __global__ void k_gather_vertex(int nv, const int* __restrict__ inc_ptr, const i
 // consecutive addresses; entries are added in row order.
 __global__ void __launch_bounds__(256) k_residual(int nv, int width, const int* __restrict__ Ai, const double* __restrict__ Ax,
                                                   const double* __restrict__ b, const double* __restrict__ q,
-                                                  const unsigned char* __restrict__ dmask, double* __restrict__ r) {
+                                                  const unsigned char* __restrict__ dmask,
+                                                  const unsigned char* __restrict__ pmask, double* __restrict__ r) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     if (dmask && dmask[v]) { r[3 * v] = r[3 * v + 1] = r[3 * v + 2] = 0.0; return; }
     double s0 = b[3 * v], s1 = b[3 * v + 1], s2 = b[3 * v + 2];
+    if (pmask) {
+        // per-DOF Dirichlet: axis d of the matrix drops the couplings to the
+        // DOFs fixed on axis d, and their rows are identities (residual 0)
+        for (int j = 0; j < width; ++j) {
+            const int c = Ai[(int64_t)j * nv + v];
+            const double a = Ax[(int64_t)j * nv + v];
+            if (!pmask[3 * c]) s0 -= a * q[3 * c];
+            if (!pmask[3 * c + 1]) s1 -= a * q[3 * c + 1];
+            if (!pmask[3 * c + 2]) s2 -= a * q[3 * c + 2];
+        }
+        r[3 * v] = pmask[3 * v] ? 0.0 : s0;
+        r[3 * v + 1] = pmask[3 * v + 1] ? 0.0 : s1;
+        r[3 * v + 2] = pmask[3 * v + 2] ? 0.0 : s2;
+        return;
+    }
     for (int j = 0; j < width; ++j) {
         const int c = Ai[(int64_t)j * nv + v];
         const double a = Ax[(int64_t)j * nv + v];
@@ -527,6 +543,69 @@ __global__ void __launch_bounds__(256) k_residual(int nv, int width, const int* 
         s2 -= a * q[3 * c + 2];
     }
     r[3 * v] = s0; r[3 * v + 1] = s1; r[3 * v + 2] = s2;
+}
+
+// -----------------------------------------------------------------------------
+// Per-DOF Dirichlet conditions on the shared factor. A DOF fixed on its own
+// (its vertex keeps a free DOF) stays in the shared matrix A; axis d's system
+// A_ff x_f = r_f (F_d: the DOFs fixed on axis d) is the bordered system
+// [A E; E^T 0][y; mu] = [r; 0]: y = A^-1 r - Z G_d^-1 (A^-1 r)_F with
+// Z = A^-1 E (columns of A^-1 at the partially fixed vertices, one per vertex
+// for every axis: A is the same scalar matrix on each) and G_d = Z_F.
+// k_pdir_coef: c_d[p] = (G_d^-1 (y_F - t_F))[i] for the vertex p that is entry i
+// of F_d (0 elsewhere); k_pdir_apply: y -= Z c per axis, then y = o on F_d.
+// -----------------------------------------------------------------------------
+__global__ void k_pdir_coef(int np, PdirAxes ax, const int* __restrict__ map, const int* __restrict__ fv,
+                            const double* __restrict__ ginv, const double* __restrict__ y, const double* __restrict__ t,
+                            double* __restrict__ coef) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int d = blockIdx.y;
+    if (p >= np) return;
+    const int i = map[d * np + p];
+    double s = 0.0;
+    if (i >= 0) {
+        const int nf = ax.off[d + 1] - ax.off[d];
+        const double* g = ginv + ax.goff[d] + (int64_t)i * nf;
+        for (int j = 0; j < nf; ++j) {
+            const int v = fv[ax.off[d] + j];
+            s += g[j] * (y[3 * v + d] - (t ? t[3 * v + d] : 0.0));
+        }
+    }
+    coef[d * np + p] = s;
+}
+
+__global__ void __launch_bounds__(256) k_pdir_apply(int nv, int np, const double* __restrict__ Z,
+                                                    const double* __restrict__ coef, const unsigned char* __restrict__ pmask,
+                                                    const double* __restrict__ o, double* __restrict__ y) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int p = 0; p < np; ++p) {
+        const double z = Z[(int64_t)p * nv + v];
+        s0 += z * coef[p];
+        s1 += z * coef[np + p];
+        s2 += z * coef[2 * np + p];
+    }
+    const double s[3] = {s0, s1, s2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) y[3 * v + d] = pmask[3 * v + d] ? (o ? o[3 * v + d] : 0.0) : y[3 * v + d] - s[d];
+}
+
+// out[q np + p] = Z[q nv + pv[p]]: the rows of the columns at the partial vertices
+__global__ void k_pdir_gather(int np, int nv, const int* __restrict__ pv, const double* __restrict__ Z,
+                              double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)np * np) return;
+    const int q = (int)(i / np), p = (int)(i % np);
+    out[i] = Z[(int64_t)q * nv + pv[p]];
+}
+
+// Z[(p0 + a) nv + v] = x[3 v + a] for a < na: axis slot a of one A^-1 product
+// carries the column of vertex p0 + a
+__global__ void k_pdir_columns(int nv, int p0, int na, const double* __restrict__ x, double* __restrict__ Z) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    for (int a = 0; a < na; ++a) Z[(int64_t)(p0 + a) * nv + v] = x[3 * v + a];
 }
 
 // -----------------------------------------------------------------------------
@@ -1374,9 +1453,26 @@ void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc
 void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out, double* out) {
     hipLaunchKernelGGL(k_gather_vertex, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, inc_ptr, inc_idx, elem_out, out);
 }
+void launch_pdir_coef(hipStream_t s, int np, const PdirAxes& ax, const int* map, const int* fv, const double* ginv,
+                      const double* y, const double* t, double* coef) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_pdir_coef, dim3(blocks_for(np, 256), 3), dim3(256), 0, s, np, ax, map, fv, ginv, y, t, coef);
+}
+void launch_pdir_apply(hipStream_t s, int nv, int np, const double* Z, const double* coef, const unsigned char* pmask,
+                       const double* o, double* y) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_pdir_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, np, Z, coef, pmask, o, y);
+}
+void launch_pdir_gather(hipStream_t s, int np, int nv, const int* pv, const double* Z, double* out) {
+    const int64_t m = (int64_t)np * np;
+    hipLaunchKernelGGL(k_pdir_gather, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, np, nv, pv, Z, out);
+}
+void launch_pdir_columns(hipStream_t s, int nv, int p0, int na, const double* x, double* Z) {
+    hipLaunchKernelGGL(k_pdir_columns, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, p0, na, x, Z);
+}
 void launch_residual(hipStream_t s, int nv, int width, const int* Ai, const double* Ax, const double* b, const double* q,
-                     const unsigned char* dmask, double* r) {
-    hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, width, Ai, Ax, b, q, dmask, r);
+                     const unsigned char* dmask, double* r, const unsigned char* pmask) {
+    hipLaunchKernelGGL(k_residual, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, width, Ai, Ax, b, q, dmask, pmask, r);
 }
 int dots_partial_size() { return DOT_BLOCKS * 32; }
 void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out) {

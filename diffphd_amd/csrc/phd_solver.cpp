@@ -359,22 +359,34 @@ void Body::setup(double dt, int threads) {
     check(nv_ > 0, "Mesh is not initialized.");
     check(!energies_.empty(), "No PD energy was added.");
     threads = std::max(1, threads);
-    // Dirichlet conditions must fix whole vertices (one factor shared by the three axes).
+    // Dirichlet conditions. A vertex with its three DOFs fixed leaves the
+    // shared matrix (identity row, couplings dropped) as in the reference's
+    // per-axis matrices; a DOF fixed on its own stays in the shared matrix and
+    // every global solve applies the bordered correction (partial_dirichlet).
     dmask_.assign(nv_, 0);
+    pmask_.assign(dofs_, 0);
+    pverts_.clear();
     dval_full_.assign(dofs_, 0.0);
     ddof_.clear();
     dval_.clear();
     for (auto& kv : dirichlet_) {
-        dmask_[kv.first / 3] = 1;
         dval_full_[kv.first] = kv.second;
         ddof_.push_back(kv.first);
         dval_.push_back(kv.second);
     }
-    for (int v = 0; v < nv_; ++v)
-        if (dmask_[v])
-            for (int d = 0; d < 3; ++d)
-                check(dirichlet_.count(3 * v + d), "Dirichlet conditions must fix all three DOFs of a vertex "
-                                                   "(per-axis Dirichlet sets are not supported by this build).");
+    for (int v = 0; v < nv_; ++v) {
+        const int nfix = (int)dirichlet_.count(3 * v) + (int)dirichlet_.count(3 * v + 1) + (int)dirichlet_.count(3 * v + 2);
+        if (nfix == 3) dmask_[v] = 1;
+        else if (nfix > 0) {
+            for (int d = 0; d < 3; ++d) pmask_[3 * v + d] = dirichlet_.count(3 * v + d) ? 1 : 0;
+            pverts_.push_back(v);
+        }
+    }
+    check((int)pverts_.size() <= kMaxPartialVertices,
+          "At most " + std::to_string(kMaxPartialVertices) + " vertices may have only some of their DOFs fixed.");
+    check(pverts_.empty() || fb_vertices_.empty(),
+          "Per-DOF Dirichlet conditions (a vertex with only some DOFs fixed) cannot be combined with a frictional "
+          "boundary in this build.");
     const double mass = density_ * element_volume_;
     const double inv_h2m = mass / (dt * dt);
     const double vs = element_volume_ / samples_;
@@ -483,7 +495,8 @@ void Body::setup(double dt, int threads) {
         for (int e = 0; e < ne_; ++e) {
             const int* v = &elems_[(size_t)e * edim_];
             bool any = false;
-            for (int k = 0; k < edim_; ++k) any |= dmask_[v[k]] != 0;
+            for (int k = 0; k < edim_; ++k)
+                any |= dmask_[v[k]] != 0 || pmask_[3 * v[k]] || pmask_[3 * v[k] + 1] || pmask_[3 * v[k] + 2];
             if (!any || w_e[e] == 0) continue;
             for (int s = 0; s < samples_; ++s) {
                 const double* g = &G_[(size_t)e * g_stride_ + (size_t)s * edim_ * 3];
@@ -528,6 +541,7 @@ void Body::setup(double dt, int threads) {
     hip_check(hipMemsetAsync(w32_.p, 0, w32_.n * sizeof(float), stream_), "w32 padding");
     hip_check(hipMemsetAsync(w64_.p, 0, w64_.n * sizeof(double), stream_), "w64 padding");
     upload_plan();
+    build_partial_dirichlet();
     setup_contact(threads);
     ready_ = true;
     ready_dt_ = dt;
@@ -669,6 +683,92 @@ void Body::collect_timing() {
     ev_rows_.clear();
     ev_cols_.clear();
     ev_used_ = 0;
+}
+
+// Columns of A^-1 at the partially fixed vertices (three per FP64 product, one
+// in each axis slot), then per axis G_d = (A^-1)_{F_d F_d} inverted on the host
+// by Cholesky (G_d is a principal submatrix of the SPD A^-1).
+void Body::build_partial_dirichlet() {
+    const int np = (int)pverts_.size();
+    d_pmask_.release();
+    if (np == 0) return;
+    d_pmask_.upload(pmask_);
+    d_pZ_.alloc((size_t)np * nv_);
+    std::vector<double> e(dofs_, 0.0);
+    for (int p0 = 0; p0 < np; p0 += 3) {
+        const int na = std::min(3, np - p0);
+        std::fill(e.begin(), e.end(), 0.0);
+        for (int a = 0; a < na; ++a) e[3 * pverts_[p0 + a] + a] = 1.0;
+        hip_check(hipStreamSynchronize(stream_), "pdir");
+        d_r_.upload(e);
+        spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
+        spmv_ST<double>(y64_, xpart64_, V64_.p, nullptr, nullptr, d_tmp_.p);
+        launch_pdir_columns(stream_, nv_, p0, na, d_tmp_.p, d_pZ_.p);
+    }
+    hip_check(hipStreamSynchronize(stream_), "pdir columns");
+    // Z rows at the partial vertices: Zpp[q][p] = (A^-1)_{v_p v_q}
+    std::vector<double> Zpp;
+    {
+        DevBuf<int> dpv;
+        DevBuf<double> dzz;
+        dpv.upload(pverts_);
+        dzz.alloc((size_t)np * np);
+        launch_pdir_gather(stream_, np, nv_, dpv.p, d_pZ_.p, dzz.p);
+        hip_check(hipStreamSynchronize(stream_), "pdir G");
+        Zpp = dzz.to_host();
+    }
+    std::vector<int> map((size_t)3 * np, -1), fv;
+    std::vector<double> ginv;
+    for (int d = 0; d < 3; ++d) {
+        pax_.off[d] = (int)fv.size();
+        pax_.goff[d] = (long long)ginv.size();
+        std::vector<int> F;
+        for (int p = 0; p < np; ++p)
+            if (pmask_[3 * pverts_[p] + d]) { map[(size_t)d * np + p] = (int)F.size(); F.push_back(p); }
+        const int nf = (int)F.size();
+        // Cholesky of G_d, then G_d^-1 column by column
+        std::vector<double> L((size_t)nf * nf, 0.0);
+        for (int j = 0; j < nf; ++j) {
+            double s = Zpp[(size_t)F[j] * np + F[j]];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)j * nf + k] * L[(size_t)j * nf + k];
+            check(s > 0, "per-DOF Dirichlet: the constrained block of A^-1 is not positive definite.");
+            L[(size_t)j * nf + j] = std::sqrt(s);
+            for (int i = j + 1; i < nf; ++i) {
+                double t = Zpp[(size_t)F[j] * np + F[i]];
+                for (int k = 0; k < j; ++k) t -= L[(size_t)i * nf + k] * L[(size_t)j * nf + k];
+                L[(size_t)i * nf + j] = t / L[(size_t)j * nf + j];
+            }
+        }
+        std::vector<double> G((size_t)nf * nf);
+        std::vector<double> col(nf);
+        for (int c = 0; c < nf; ++c) {
+            for (int i = 0; i < nf; ++i) {  // L z = e_c
+                double t = i == c ? 1.0 : 0.0;
+                for (int k = 0; k < i; ++k) t -= L[(size_t)i * nf + k] * col[k];
+                col[i] = t / L[(size_t)i * nf + i];
+            }
+            for (int i = nf - 1; i >= 0; --i) {  // L^T x = z
+                double t = col[i];
+                for (int k = i + 1; k < nf; ++k) t -= L[(size_t)k * nf + i] * col[k];
+                col[i] = t / L[(size_t)i * nf + i];
+            }
+            for (int i = 0; i < nf; ++i) G[(size_t)i * nf + c] = col[i];
+        }
+        for (int p : F) fv.push_back(pverts_[p]);
+        ginv.insert(ginv.end(), G.begin(), G.end());
+    }
+    pax_.off[3] = (int)fv.size();
+    d_pmap_.upload(map);
+    d_pfv_.upload(fv);
+    d_pginv_.upload(ginv);
+    d_pcoef_.alloc((size_t)3 * np);
+}
+
+void Body::partial_dirichlet(double* y, const double* t, const double* o) {
+    const int np = (int)pverts_.size();
+    if (np == 0) return;
+    launch_pdir_coef(stream_, np, pax_, d_pmap_.p, d_pfv_.p, d_pginv_.p, y, t, d_pcoef_.p);
+    launch_pdir_apply(stream_, nv_, np, d_pZ_.p, d_pcoef_.p, d_pmask_.p, o, y);
 }
 
 void Body::upload_plan() {
@@ -886,7 +986,7 @@ void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, boo
     launch_warm_guess(stream_, nv_, q_dev, v_dev, dt, nullptr, 0.f, d_dmask_.p, d_dval_full_.p, d_guess_.p,
                       d_guess_.p + dofs_);
     local_rhs(d_guess_.p, inv_h2m);
-    launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_guess_.p, d_dmask_.p, d_r_.p);
+    launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_guess_.p, d_dmask_.p, d_r_.p, d_pmask_.p);
     launch_mlp_features(stream_, nv_, q_dev, v_dev, d_Xrest_.p, d_r_.p, d_adiag_.p, d_logk_.p, d_cflag_.p, d_cdist_.p, dt,
                         L, g, d_fmean_.p, d_fstd_.p, standardize ? 1 : 0, d_feat_.p);
 }
@@ -921,7 +1021,7 @@ void Body::warm_start(const double* v, double dt, double inv_h2m, bool learned) 
     for (int g = 0; g < 2; ++g) {
         const double* x = d_guess_.p + (size_t)g * n3;
         local_rhs(x, inv_h2m);
-        launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, x, d_dmask_.p, d_r_.p);
+        launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, x, d_dmask_.p, d_r_.p, d_pmask_.p);
         spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
         spmv_ST<double>(y64_, xpart64_, V64_.p, nullptr, nullptr, d_tmp_.p);
         rn[g] = std::sqrt(dots({{d_tmp_.p, d_tmp_.p}})[0]);
@@ -967,7 +1067,7 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
     hip_check(hipMemsetAsync(d_q_.p, 0, sizeof(double) * dofs_, stream_), "zero x");
     hip_check(hipMemcpyAsync(d_r_.p, d_b_.p, sizeof(double) * dofs_, hipMemcpyDeviceToDevice, stream_), "r=b");
     for (int it = 0; it <= refine; ++it) {
-        if (it > 0) launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, nullptr, d_r_.p);
+        if (it > 0) launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, nullptr, d_r_.p, d_pmask_.p);
         if (precision == 1) {
             spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             spmv_ST<float>(y32_, xpart32_, V32_.p, d_q_.p, nullptr, d_q_.p);
@@ -975,6 +1075,8 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
             spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             spmv_ST<double>(y64_, xpart64_, V64_.p, d_q_.p, nullptr, d_q_.p);
         }
+        // per-DOF Dirichlet rows are identities: x_F = b_F, the increment is 0 there
+        partial_dirichlet(d_q_.p, it == 0 ? nullptr : d_b_.p, d_b_.p);
     }
     hip_check(hipStreamSynchronize(stream_), "apply_ainv");
     d_q_.download(x, dofs_);
@@ -1032,6 +1134,8 @@ void Body::apply_ainv_noalloc(const double* b_dev, double* x_dev, int precision,
     if (precision != 0 && precision != 1) throw std::runtime_error("apply_ainv_noalloc: precision must be 0 or 1");
     if (refine < 0) throw std::runtime_error("apply_ainv_noalloc: refine must be >= 0");
     if (b_dev == x_dev) throw std::runtime_error("apply_ainv_noalloc: b and x must not alias");
+    if (!pverts_.empty())
+        throw std::runtime_error("apply_ainv_noalloc: per-DOF Dirichlet conditions are not supported (whole vertices only)");
     const size_t need = noalloc_scratch_bytes(precision);
     if (need > kNoAllocScratchBytes)
         throw std::runtime_error("apply_ainv_noalloc: this factor needs " + std::to_string(need) +
@@ -1249,7 +1353,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
         // residual of the global system with the previous contact impulses
         {
             PhaseTimer ts(this, kFwdSolve);
-            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p);
+            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_b_.p, d_q_.p, d_dmask_.p, d_r_.p, d_pmask_.p);
             if (K > 0) launch_add_node_force(stream_, K, d_cand_.p, d_F_.p, d_r_.p);
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
@@ -1333,6 +1437,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
             PhaseTimer ts(this, kFwdSolve);
             if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
             else spmv_ST<double>(y64_, xpart64_, V64_.p, d_qprev_.p, d_dval_full_.p, d_q_.p);
+            partial_dirichlet(d_q_.p, d_qprev_.p, d_dval_full_.p);
         }
         ++iters;
         // Anderson acceleration
@@ -1571,7 +1676,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             hip_check(hipMemcpyAsync(d_sprev.p, d_s.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "sprev");
             kpd_apply(d_s.p, d_kz_.p, precision);
             launch_axpbypcz(stream_, n3, 1.0, d_rhs.p, 1.0, d_kz_.p, 0.0, nullptr, d_tmp_.p);
-            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_tmp_.p, d_s.p, d_dmask_.p, d_r_.p);
+            launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, d_tmp_.p, d_s.p, d_dmask_.p, d_r_.p, d_pmask_.p);
             if (precision == 1) spmv_S<float>(d_r_.p, w32_, y32_, part32_, V32_.p);
             else spmv_S<double>(d_r_.p, w64_, y64_, part64_, V64_.p);
             if (nb > 0) {
@@ -1591,6 +1696,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
             }
             if (precision == 1) spmv_ST<float>(y32_, xpart32_, V32_.p, d_sprev.p, d_zero.p, d_s.p);
             else spmv_ST<double>(y64_, xpart64_, V64_.p, d_sprev.p, d_zero.p, d_s.p);
+            partial_dirichlet(d_s.p, d_sprev.p, nullptr);
             safe_clip(d_s.p);
             launch_axpbypcz(stream_, n3, 1.0, d_s.p, -1.0, d_sprev.p, 0.0, nullptr, d_g_.p);
             double err;

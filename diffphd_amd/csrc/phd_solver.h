@@ -258,6 +258,20 @@ private:
     std::vector<int> Ap_, Ai_;
     std::vector<double> Ax_;
     std::vector<unsigned char> dmask_;
+    // per-DOF Dirichlet: DOFs fixed on their own (their vertex keeps a free
+    // DOF), the vertices carrying them, and the bordered correction's data
+    static constexpr int kMaxPartialVertices = 4096;
+    std::vector<unsigned char> pmask_;
+    std::vector<int> pverts_;
+    DevBuf<unsigned char> d_pmask_;
+    DevBuf<double> d_pZ_, d_pginv_, d_pcoef_;
+    DevBuf<int> d_pmap_, d_pfv_;
+    PdirAxes pax_{};
+    void build_partial_dirichlet();
+    // y (AoS 3N, the result of a global solve y = t + A^-1 r): constrains the
+    // solve to leave y = t on the partially fixed DOFs, then writes o there
+    // (t, o may be null: zero)
+    void partial_dirichlet(double* y, const double* t, const double* o);
     std::vector<int> ddof_;
     std::vector<double> dval_, dval_full_;
     std::vector<double> c_dir_;
