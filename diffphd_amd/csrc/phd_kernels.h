@@ -68,7 +68,9 @@ template <class TV>
 void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, const unsigned char* dmask, TV* w);
 
 // ---- local step, gather, residual --------------------------------------------
-void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block = 128);
+// fp32: the SVD and the projections in FP32 (F and the sums in FP64)
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block = 128,
+                       bool fp32 = false);
 // pd_rhs = sum of element contributions - c_dir; b = inv_h2m * q_tilde + pd_rhs
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b);

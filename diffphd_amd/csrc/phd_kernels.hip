@@ -404,7 +404,7 @@ __global__ void k_perm_in(int n, const int* __restrict__ perm, const double* __r
 // one thread per element; per sample F = x_e G_s, one SVD shared by every
 // projection, contribution sum_energies w A_s^T vec(p(F)).
 // -----------------------------------------------------------------------------
-template <int EDIM>
+template <int EDIM, class TC>
 __global__ void __launch_bounds__(256) k_local_step(DevMaterials m, const double* __restrict__ q,
                                                     double* __restrict__ elem_out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -435,34 +435,39 @@ __global__ void __launch_bounds__(256) k_local_step(DevMaterials m, const double
                 F[1 + 3 * jj] += x[k][1] * g;
                 F[2 + 3 * jj] += x[k][2] * g;
             }
-        double U[9], sg[3], V[9];
-        svd3(F, U, sg, V);
+        // F in FP64 (positions minus the sum of their gradients cancel to
+        // O(1)); the SVD and the projections in TC
+        TC Fc[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Fc[i] = (TC)F[i];
+        TC U[9], sg[3], V[9];
+        svd3(Fc, U, sg, V);
         double P[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[i] = 0.0;
         if (wc != 0.0) {
-            const double one[3] = {1.0, 1.0, 1.0};
-            double R[9];
+            const TC one[3] = {TC(1), TC(1), TC(1)};
+            TC R[9];
             udv(U, one, V, R);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) P[i] += wc * R[i];
+            for (int i = 0; i < 9; ++i) P[i] += wc * (double)R[i];
         }
         if (wv != 0.0) {
-            double D[3], sd[3], Pv[9];
+            TC D[3], sd[3], Pv[9];
             volume_D(sg, D);
             sd[0] = sg[0] + D[0]; sd[1] = sg[1] + D[1]; sd[2] = sg[2] + D[2];
             udv(U, sd, V, Pv);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) P[i] += wv * Pv[i];
+            for (int i = 0; i < 9; ++i) P[i] += wv * (double)Pv[i];
         }
         for (int h = 0; h < m.n_nh; ++h) {
             const double wn = m.wn[h][e];
             if (wn == 0.0) continue;
-            double ss[3], Pn[9];
-            nh_prox(m.nh_mu[h], m.nh_la[h], m.nh_k[h], sg, ss);
+            TC ss[3], Pn[9];
+            nh_prox((TC)m.nh_mu[h], (TC)m.nh_la[h], (TC)m.nh_k[h], sg, ss);
             udv(U, ss, V, Pn);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) P[i] += wn * Pn[i];
+            for (int i = 0; i < 9; ++i) P[i] += wn * (double)Pn[i];
         }
 #pragma unroll
         for (int k = 0; k < EDIM; ++k)
@@ -1438,12 +1443,16 @@ void launch_perm_in(hipStream_t s, int n, const int* perm, const double* r, cons
     template void launch_contact_scatter<TV>(hipStream_t, const ContactPieces&, const int*, const int*, const TV*,          \
                                              const double*, double*, int, TV*);
 
-void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block) {
+void launch_local_step(hipStream_t s, const DevMaterials& m, const double* q, double* elem_out, int block, bool fp32) {
     if (block != 64 && block != 128 && block != 256) throw std::invalid_argument("local-step block must be 64, 128 or 256");
-    if (m.edim == 8)
-        hipLaunchKernelGGL(k_local_step<8>, dim3(blocks_for(m.ne, block)), dim3(block), 0, s, m, q, elem_out);
-    else
-        hipLaunchKernelGGL(k_local_step<4>, dim3(blocks_for(m.ne, block)), dim3(block), 0, s, m, q, elem_out);
+    const dim3 g(blocks_for(m.ne, block)), b(block);
+    if (m.edim == 8) {
+        if (fp32) hipLaunchKernelGGL((k_local_step<8, float>), g, b, 0, s, m, q, elem_out);
+        else hipLaunchKernelGGL((k_local_step<8, double>), g, b, 0, s, m, q, elem_out);
+    } else {
+        if (fp32) hipLaunchKernelGGL((k_local_step<4, float>), g, b, 0, s, m, q, elem_out);
+        else hipLaunchKernelGGL((k_local_step<4, double>), g, b, 0, s, m, q, elem_out);
+    }
 }
 void launch_gather_rhs(hipStream_t s, int nv, const int* inc_ptr, const int* inc_idx, const double* elem_out,
                        const double* c_dir, const double* q_tilde, double inv_h2m, double* pd_rhs, double* b) {

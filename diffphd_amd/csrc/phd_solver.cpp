@@ -894,7 +894,7 @@ void Body::contact_scatter(const double* f, DevBuf<TV>& y) {
 }
 
 void Body::local_rhs(const double* q, double inv_h2m) {
-    launch_local_step(stream_, mat_, q, d_elem_.p, local_block_);
+    launch_local_step(stream_, mat_, q, d_elem_.p, local_block_, local_fp32_);
     launch_gather_rhs(stream_, nv_, d_inc_ptr_.p, d_inc_idx_.p, d_elem_.p, d_c_dir_.p, d_qtilde_.p, inv_h2m, d_pdrhs_.p, d_b_.p);
 }
 
@@ -1274,6 +1274,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
     const double restitution = opt_or(opt, "restitution", 0.0);
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
     check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
+    local_fp32_ = opt_or(opt, "phd_local_fp32", 0.0) != 0.0;
     setup(dt, threads);
     PhaseTimer t_total(this, kFwdTotal);
     const int n3 = dofs_;

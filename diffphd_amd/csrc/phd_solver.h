@@ -227,6 +227,7 @@ public:
 private:
     // launch shapes (defaults; autotune() replaces them)
     int chunk_w_ = 1024, unit_items_ = 512, local_block_ = 128;
+    bool local_fp32_ = false;  // forward option phd_local_fp32
     void upload_plan();  // plan_ buffers and the partial-sum arrays sized by it
     int device_ = 0;
     hipStream_t stream_ = nullptr;
