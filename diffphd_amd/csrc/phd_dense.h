@@ -29,8 +29,8 @@ inline int64_t chol_work(int n) { return (int64_t)chol_padded(n) * (1 + kCholBlo
 // M_ab = om_i (n_i . n_j) H_ij om_j + [a == b] E_i with i = J[a], j = J[b]
 // (n_i . n_j = 1 when nrm is null); H is the symmetric K x K block of A^-1 on
 // the contact vertices.
-template <class TV>
-void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
+template <class TV, class TH>
+void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const TH* H, int K, const double* nrm,
                           const double* om, const double* E, const double* rhs, TV* M);
 
 // Factorization + both substitutions. work: device scratch of chol_work(n)

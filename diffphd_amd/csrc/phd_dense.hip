@@ -10,9 +10,9 @@ namespace {
 
 constexpr int CB = kCholBlock;
 
-template <class TV>
+template <class TV, class TH>
 __global__ void __launch_bounds__(256) k_chol_assemble(int nJ, int np, int64_t lda, const int* __restrict__ J,
-                                                       const double* __restrict__ H, int K, const double* __restrict__ nrm,
+                                                       const TH* __restrict__ H, int K, const double* __restrict__ nrm,
                                                        const double* __restrict__ om, const double* __restrict__ E,
                                                        const double* __restrict__ rhs, TV* __restrict__ M) {
     const int64_t rows = (int64_t)np + CB;
@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) k_chol_assemble(int nJ, int np, int64_t l
         const double nd = nrm ? nrm[3 * i] * nrm[3 * j] + nrm[3 * i + 1] * nrm[3 * j + 1] + nrm[3 * i + 2] * nrm[3 * j + 2]
                               : 1.0;
         // H is symmetric: read along its row j so that consecutive r stay close
-        v = om[i] * nd * H[(int64_t)j * K + i] * om[j];
+        v = om[i] * nd * (double)H[(int64_t)j * K + i] * om[j];
         if (r == c) v += E[i];
     }
     M[r + (int64_t)c * lda] = (TV)v;
@@ -682,14 +682,14 @@ __global__ void k_clear_flag(int* f) {
 
 }  // namespace
 
-template <class TV>
-void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
+template <class TV, class TH>
+void launch_chol_assemble(hipStream_t s, int nJ, const int* J, const TH* H, int K, const double* nrm,
                           const double* om, const double* E, const double* rhs, TV* M) {
     const int np = chol_padded(nJ);
     const int64_t lda = chol_lda(nJ);
     const int64_t total = lda * np;
-    hipLaunchKernelGGL(k_chol_assemble<TV>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nJ, np, lda, J, H, K,
-                       nrm, om, E, rhs, M);
+    hipLaunchKernelGGL((k_chol_assemble<TV, TH>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nJ, np, lda, J, H,
+                       K, nrm, om, E, rhs, M);
 }
 
 template <class TV>
@@ -712,10 +712,12 @@ void launch_chol_solve(hipStream_t s, int n, TV* M, TV* work, TV* x, int* fail) 
 
 void launch_clear_flag(hipStream_t s, int* flag) { hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, flag); }
 
-template void launch_chol_assemble<float>(hipStream_t, int, const int*, const double*, int, const double*, const double*,
-                                          const double*, const double*, float*);
-template void launch_chol_assemble<double>(hipStream_t, int, const int*, const double*, int, const double*, const double*,
-                                           const double*, const double*, double*);
+template void launch_chol_assemble<float, double>(hipStream_t, int, const int*, const double*, int, const double*,
+                                                  const double*, const double*, const double*, float*);
+template void launch_chol_assemble<float, float>(hipStream_t, int, const int*, const float*, int, const double*,
+                                                 const double*, const double*, const double*, float*);
+template void launch_chol_assemble<double, double>(hipStream_t, int, const int*, const double*, int, const double*,
+                                                   const double*, const double*, const double*, double*);
 template void launch_chol_solve<float>(hipStream_t, int, float*, float*, float*, int*);
 template void launch_chol_solve<double>(hipStream_t, int, double*, double*, double*, int*);
 

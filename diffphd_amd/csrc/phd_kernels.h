@@ -195,11 +195,15 @@ void launch_fric_coulomb(hipStream_t s, int K, const double* lambda, const int* 
                          double fmu, double* lt1, double* lt2);
 void launch_ncp_impulses(hipStream_t s, int K, const double* om, const double* lambda, const double* nrm, const double* lt1,
                          const double* lt2, const double* t1, const double* t2, int do_fric, double* F, double* dF);
-// H_rows,cols f with both counts read on the device; grid for max_rows rows
+// H_rows,cols f with both counts read on the device; grid for max_rows rows.
+// TH: the stored precision of H (FP64, or its FP32 copy), sums in FP64.
+template <class TH>
 void launch_h_apply_idx_dev(hipStream_t s, int max_rows, const int* nr_dev, const int* rows, const int* nc_dev,
-                            const int* cols, const double* H, int K, const double* f, double* out);
-void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
+                            const int* cols, const TH* H, int K, const double* f, double* out);
+template <class TH>
+void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const TH* H, int K,
                         const double* f, double* out);
+void launch_to_float(hipStream_t s, int64_t n, const double* a, float* b);
 // contact system restricted to the index set J (row-major nJ x nJ)
 template <class TM>
 void launch_assemble_msys_idx(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,

@@ -1324,16 +1324,17 @@ __global__ void k_basis_expand(int K, const int* __restrict__ cptr, const int* _
 
 // out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]: the candidate block
 // of A^-1 applied to impulses supported on `cols` (one wavefront per row).
+template <class TH>
 __global__ void k_h_apply_idx(int nr, const int* __restrict__ rows, int nc, const int* __restrict__ cols,
-                              const double* __restrict__ H, int K, const double* __restrict__ f, double* __restrict__ out) {
+                              const TH* __restrict__ H, int K, const double* __restrict__ f, double* __restrict__ out) {
     const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (a >= nr) return;
-    const double* Hr = H + (int64_t)rows[a] * K;
+    const TH* Hr = H + (int64_t)rows[a] * K;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int b = lane; b < nc; b += WAVE) {
         const int c = cols[b];
-        const double h = Hr[c];
+        const double h = (double)Hr[c];
         a0 += h * f[3 * c];
         a1 += h * f[3 * c + 1];
         a2 += h * f[3 * c + 2];
@@ -1650,10 +1651,23 @@ void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist
     if (K == 0) return;
     hipLaunchKernelGGL(k_basis_expand, dim3(blocks_for(K, 256)), dim3(256), 0, s, K, cptr, clist, dirs, g, F);
 }
-void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const double* H, int K,
+template <class TH>
+void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const TH* H, int K,
                         const double* f, double* out) {
     if (nr == 0) return;
-    hipLaunchKernelGGL(k_h_apply_idx, dim3(blocks_for(nr, 4)), dim3(256), 0, s, nr, rows, nc, cols, H, K, f, out);
+    hipLaunchKernelGGL(k_h_apply_idx<TH>, dim3(blocks_for(nr, 4)), dim3(256), 0, s, nr, rows, nc, cols, H, K, f, out);
+}
+template void launch_h_apply_idx<float>(hipStream_t, int, const int*, int, const int*, const float*, int, const double*,
+                                        double*);
+template void launch_h_apply_idx<double>(hipStream_t, int, const int*, int, const int*, const double*, int, const double*,
+                                         double*);
+__global__ void k_to_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+void launch_to_float(hipStream_t s, int64_t n, const double* a, float* b) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, b);
 }
 template <class TM>
 void launch_assemble_msys_idx(hipStream_t s, int nJ, const int* J, const double* H, int K, const double* nrm,
@@ -1936,18 +1950,19 @@ __global__ void k_ncp_impulses(int K, const double* __restrict__ om, const doubl
 
 // k_h_apply_idx with the row and column counts read on the device (grid sized
 // for the largest possible row list).
+template <class TH>
 __global__ void k_h_apply_idx_dev(const int* __restrict__ nr_dev, const int* __restrict__ rows, const int* __restrict__ nc_dev,
-                                  const int* __restrict__ cols, const double* __restrict__ H, int K,
+                                  const int* __restrict__ cols, const TH* __restrict__ H, int K,
                                   const double* __restrict__ f, double* __restrict__ out) {
     const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int lane = threadIdx.x % WAVE;
     if (a >= *nr_dev) return;
     const int nc = *nc_dev;
-    const double* Hr = H + (int64_t)rows[a] * K;
+    const TH* Hr = H + (int64_t)rows[a] * K;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int b = lane; b < nc; b += WAVE) {
         const int c = cols[b];
-        const double h = Hr[c];
+        const double h = (double)Hr[c];
         a0 += h * f[3 * c];
         a1 += h * f[3 * c + 1];
         a2 += h * f[3 * c + 2];
@@ -2006,11 +2021,16 @@ void launch_ncp_impulses(hipStream_t s, int K, const double* om, const double* l
     hipLaunchKernelGGL(k_ncp_impulses, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, om, lambda, nrm, lt1, lt2, t1, t2,
                        do_fric, F, dF);
 }
+template <class TH>
 void launch_h_apply_idx_dev(hipStream_t s, int max_rows, const int* nr_dev, const int* rows, const int* nc_dev,
-                            const int* cols, const double* H, int K, const double* f, double* out) {
+                            const int* cols, const TH* H, int K, const double* f, double* out) {
     if (max_rows == 0) return;
-    hipLaunchKernelGGL(k_h_apply_idx_dev, dim3(blocks_for(max_rows, 4)), dim3(256), 0, s, nr_dev, rows, nc_dev, cols, H, K,
-                       f, out);
+    hipLaunchKernelGGL(k_h_apply_idx_dev<TH>, dim3(blocks_for(max_rows, 4)), dim3(256), 0, s, nr_dev, rows, nc_dev, cols, H,
+                       K, f, out);
 }
+template void launch_h_apply_idx_dev<float>(hipStream_t, int, const int*, const int*, const int*, const int*,
+                                            const float*, int, const double*, double*);
+template void launch_h_apply_idx_dev<double>(hipStream_t, int, const int*, const int*, const int*, const int*,
+                                             const double*, int, const double*, double*);
 
 }  // namespace phd

@@ -645,6 +645,8 @@ void Body::setup_contact(int threads) {
         check(rocblas_dsyrk((rocblas_handle)blas_, rocblas_fill_lower, rocblas_operation_none, K_, R, &one, D.p, K_, &zero,
                             d_H_.p, K_) == rocblas_status_success, "rocblas_dsyrk failed");
         launch_symmetrize_lower(stream_, K_, d_H_.p);
+        d_H32_.alloc((size_t)K_ * K_);
+        launch_to_float(stream_, (int64_t)K_ * K_, d_H_.p, d_H32_.p);
         hip_check(hipStreamSynchronize(stream_), "contact H");
     }
     std::vector<double> Hdiag(K_);
@@ -1046,12 +1048,13 @@ bool Body::solve_contact_system(int n, int precision, const int* J, bool plain_w
     if (precision == 1) {
         if (d_C32_.n < (size_t)chol_elems(n)) d_C32_.alloc((size_t)chol_elems(n));
         if (d_cy32_.n < (size_t)chol_work(n)) { d_cy32_.alloc((size_t)chol_work(n)); d_cx32_.alloc(np); }
-        launch_chol_assemble<float>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
+        if (h32_) launch_chol_assemble<float, float>(stream_, n, J, d_H32_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
+        else launch_chol_assemble<float, double>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C32_.p);
         launch_chol_solve<float>(stream_, n, d_C32_.p, d_cy32_.p, d_cx32_.p, d_info_.p);
     } else {
         if (d_C64_.n < (size_t)chol_elems(n)) d_C64_.alloc((size_t)chol_elems(n));
         if (d_cy64_.n < (size_t)chol_work(n)) { d_cy64_.alloc((size_t)chol_work(n)); d_cx64_.alloc(np); }
-        launch_chol_assemble<double>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
+        launch_chol_assemble<double, double>(stream_, n, J, d_H_.p, K_, nrm, d_om_.p, d_E_.p, d_rhsK_.p, d_C64_.p);
         launch_chol_solve<double>(stream_, n, d_C64_.p, d_cy64_.p, d_cx64_.p, d_info_.p);
     }
     int* flag = reinterpret_cast<int*>(pin_small_);
@@ -1275,6 +1278,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
     check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
     local_fp32_ = opt_or(opt, "phd_local_fp32", 0.0) != 0.0;
+    h32_ = precision == 1 && opt_or(opt, "phd_contact_h32", 1.0) != 0.0;
     setup(dt, threads);
     PhaseTimer t_total(this, kFwdTotal);
     const int n3 = dofs_;
@@ -1388,8 +1392,13 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
             // uC -= H F_prev on P; the compressed system's rhs on J with (A^-1 g)_C =
             // (A^-1 b)_C + H F_g, F_g = omega lambda n + lt1 t1 + lt2 t2 (zero sums
             // when a column set is empty)
-            launch_h_apply_idx(stream_, nP, listP, nF, listF, d_H_.p, K, d_F_.p, d_hprev_.p);
-            launch_h_apply_idx(stream_, nJ, listJ, nG, listG, d_H_.p, K, d_Fg_.p, d_tmpK3_.p);
+            if (h32_) {
+                launch_h_apply_idx<float>(stream_, nP, listP, nF, listF, d_H32_.p, K, d_F_.p, d_hprev_.p);
+                launch_h_apply_idx<float>(stream_, nJ, listJ, nG, listG, d_H32_.p, K, d_Fg_.p, d_tmpK3_.p);
+            } else {
+                launch_h_apply_idx<double>(stream_, nP, listP, nF, listF, d_H_.p, K, d_F_.p, d_hprev_.p);
+                launch_h_apply_idx<double>(stream_, nJ, listJ, nG, listG, d_H_.p, K, d_Fg_.p, d_tmpK3_.p);
+            }
             launch_ncp_rhs(stream_, nP, listP, d_hprev_.p, d_posJ_.p, d_tmpK3_.p, d_hvec_.p, d_om_.p, d_nrm_.p, d_uC_.p,
                            d_rhsK_.p);
             tcap.reset();
@@ -1424,8 +1433,12 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
                 int* flist = lists + 4 * (size_t)K;
                 launch_fric_sets(stream_, K, d_lambda_.p, d_om_.p, d_nrm_.p, d_Fn_.p, d_ncp_flags_.p);
                 launch_ncp_compact(stream_, K, d_ncp_flags_.p, 2, fric_masks, flist, d_ncp_counts_.p + 4, d_posL_.p);
-                launch_h_apply_idx_dev(stream_, K, d_ncp_counts_.p + 4, flist, d_ncp_counts_.p + 5, flist + K, d_H_.p, K,
-                                       d_Fn_.p, d_tmpK3_.p);
+                if (h32_)
+                    launch_h_apply_idx_dev<float>(stream_, K, d_ncp_counts_.p + 4, flist, d_ncp_counts_.p + 5, flist + K,
+                                                  d_H32_.p, K, d_Fn_.p, d_tmpK3_.p);
+                else
+                    launch_h_apply_idx_dev<double>(stream_, K, d_ncp_counts_.p + 4, flist, d_ncp_counts_.p + 5, flist + K,
+                                                   d_H_.p, K, d_Fn_.p, d_tmpK3_.p);
                 launch_fric_coulomb(stream_, K, d_lambda_.p, d_posL_.p, d_tmpK3_.p, d_uC_.p, d_qframeC_.p, d_nrm_.p,
                                     d_t1_.p, d_t2_.p, d_Hdiag_.p, fmu, d_lt1_.p, d_lt2_.p);
             }

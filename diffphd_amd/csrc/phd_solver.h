@@ -228,6 +228,7 @@ private:
     // launch shapes (defaults; autotune() replaces them)
     int chunk_w_ = 1024, unit_items_ = 512, local_block_ = 128;
     bool local_fp32_ = false;  // forward option phd_local_fp32
+    bool h32_ = false;         // this frame reads the FP32 copy of H (mixed precision, phd_contact_h32)
     void upload_plan();  // plan_ buffers and the partial-sum arrays sized by it
     int device_ = 0;
     hipStream_t stream_ = nullptr;
@@ -312,7 +313,8 @@ private:
     int K_ = 0;
     std::vector<int> cand_;
     DevBuf<int> d_cand_;
-    DevBuf<double> d_H_;        // A^-1 restricted to the candidates (K x K, FP64)
+    DevBuf<double> d_H_;
+    DevBuf<float> d_H32_;  // FP32 copy of H for the mixed path's applies and assembly        // A^-1 restricted to the candidates (K x K, FP64)
     std::vector<double> Hdiag_;
     DevBuf<int> d_cp_ptr_, d_cp_row_, d_rc_ptr_, d_rc_row_, d_rc_c_;
     DevBuf<double> d_cp_val64_, d_rc_val64_;
