@@ -27,12 +27,18 @@ struct DotArgs {
 
 // Boundary (frictional obstacle) description for the contact kernels.
 struct DevBoundary {
-    int type = 0;                 // 1 planar, 2 spherical, 3 heightfield, 4 press
+    int type = 0;                 // 1 planar, 2 spherical, 3 heightfield, 4 press, 5 mesh
     double n[3] = {0, 0, 1}, off = 0;
     double c[3] = {0, 0, 0}, r = 0;
     double z0 = 0, plane_z = 0;
     int nwaves = 0;
     const double* waves = nullptr;  // device (amp, kx, ky, phase) quadruples
+    // mesh: two triangle sets of the body itself (left, then right: 3 vertex
+    // indices each); a vertex below `split` queries the right set, any other
+    // the left set, at the body's current positions
+    int split = 0, nfl = 0, nfr = 0;
+    const int* faces = nullptr;
+    double radius = 0;
 };
 
 // Axis stride of the S-product input w (3 axes): 64-aligned with room for the
@@ -115,9 +121,15 @@ void launch_clip_scale(hipStream_t s, int n, double clip, const double* norm2, d
 
 // ---- contact -----------------------------------------------------------------
 // normals / tangents at the frame-start positions (GetLocalFrame)
+// Mesh boundary (type 5): the frames are anchored at the closest point of the
+// opposite triangle set at q (normal (p - closest) / |p - closest|, tangents by
+// Gram-Schmidt from the axis least parallel to it), and enabled_out[c] marks
+// the candidates within 5 contact radii of that set; the distance is the
+// projection of p - closest on the frame-start normal refn minus the radius.
 void launch_contact_frames(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* nrm,
-                           double* t1, double* t2);
-void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta);
+                           double* t1, double* t2, unsigned char* enabled_out = nullptr);
+void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta,
+                             const double* refn = nullptr);
 // Lists (candidate paths, or rows crossed by paths) cut into pieces of at most
 // kContactPiece entries: piece q covers entries [b[q], e[q]) and the pieces of
 // list l are [ptr[l], ptr[l + 1]) (device arrays).

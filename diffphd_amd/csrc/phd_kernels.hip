@@ -773,6 +773,91 @@ __global__ void k_contact_frames(int K, DevBoundary b, const int* __restrict__ n
     t2[3 * c] = y0 / yn; t2[3 * c + 1] = y1 / yn; t2[3 * c + 2] = y2 / yn;
 }
 
+// Mesh boundary (MeshFrictionalBoundary, mesh_frictional_boundary.cpp): one
+// wave per candidate scans the opposite triangle set (lanes stride the faces)
+// for the closest point at the current positions q; ties go to the lowest face
+// index, as the reference's sequential scan keeps the first minimum.
+// FRAMES: frame-start normal (p - closest) / |p - closest| (the face normal
+// when p is on the surface), Gram-Schmidt tangents (forward_gpu.cpp:496-509),
+// and the candidate filter |p - closest| - radius <= 5 radius (:460-478).
+// DISTANCE: (p - closest) . refn - radius (GetSignedDistanceWithReferenceNormal).
+template <bool FRAMES>
+__global__ void __launch_bounds__(256) k_mesh_query(int K, DevBoundary b, const int* __restrict__ nodes,
+                                                    const double* __restrict__ q, const double* __restrict__ refn,
+                                                    double* __restrict__ out0, double* __restrict__ t1,
+                                                    double* __restrict__ t2, unsigned char* __restrict__ enabled) {
+    const int c = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const int lane = threadIdx.x % WAVE;
+    if (c >= K) return;
+    const int v = nodes[c];
+    const bool right = v < b.split;
+    const int f0 = right ? b.nfl : 0, nf = right ? b.nfr : b.nfl;
+    const double p[3] = {q[3 * v], q[3 * v + 1], q[3 * v + 2]};
+    double best = INFINITY;
+    int bf = -1;
+    for (int f = lane; f < nf; f += WAVE) {
+        const int* fv = b.faces + 3 * (int64_t)(f0 + f);
+        double cl[3];
+        closest_on_triangle(p, q + 3 * fv[0], q + 3 * fv[1], q + 3 * fv[2], cl);
+        const double d2 = (p[0] - cl[0]) * (p[0] - cl[0]) + (p[1] - cl[1]) * (p[1] - cl[1]) + (p[2] - cl[2]) * (p[2] - cl[2]);
+        if (d2 < best) { best = d2; bf = f; }
+    }
+    // lexicographic minimum of (distance^2, face) over the wave
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, WAVE);
+        const int of = __shfl_xor(bf, off, WAVE);
+        if (of >= 0 && (bf < 0 || ob < best || (ob == best && of < bf))) { best = ob; bf = of; }
+    }
+    if (lane != 0) return;
+    if (bf < 0) {  // no opposite faces: infinitely far, never a candidate
+        if (FRAMES) {
+            out0[3 * c] = 0; out0[3 * c + 1] = 0; out0[3 * c + 2] = 1;
+            t1[3 * c] = 1; t1[3 * c + 1] = 0; t1[3 * c + 2] = 0;
+            t2[3 * c] = 0; t2[3 * c + 1] = 1; t2[3 * c + 2] = 0;
+            enabled[c] = 0;
+        } else {
+            out0[c] = INFINITY;
+        }
+        return;
+    }
+    const int* fv = b.faces + 3 * (int64_t)(f0 + bf);
+    const double *A = q + 3 * fv[0], *B = q + 3 * fv[1], *C = q + 3 * fv[2];
+    double cl[3];
+    closest_on_triangle(p, A, B, C, cl);
+    const double dp[3] = {p[0] - cl[0], p[1] - cl[1], p[2] - cl[2]};
+    if (!FRAMES) {
+        out0[c] = dp[0] * refn[3 * c] + dp[1] * refn[3 * c + 1] + dp[2] * refn[3 * c + 2] - b.radius;
+        return;
+    }
+    const double dn = sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+    double n[3];
+    if (dn > 1e-12) {
+        n[0] = dp[0] / dn; n[1] = dp[1] / dn; n[2] = dp[2] / dn;
+    } else {
+        const double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        double fn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double fl = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        if (fl > DBL_EPSILON) { n[0] = fn[0] / fl; n[1] = fn[1] / fl; n[2] = fn[2] / fl; }
+        else { n[0] = 0; n[1] = 0; n[2] = 1; }
+    }
+    // t1 from the axis least parallel to n (first minimum), t2 = n x t1
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (fabs(n[d]) < fabs(n[ax])) ax = d;
+    double u[3] = {-n[ax] * n[0], -n[ax] * n[1], -n[ax] * n[2]};
+    u[ax] += 1.0;
+    const double ul = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int d = 0; d < 3; ++d) u[d] /= ul;
+    double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+    const double wl = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int d = 0; d < 3; ++d) {
+        out0[3 * c + d] = n[d];
+        t1[3 * c + d] = u[d];
+        t2[3 * c + d] = w[d] / wl;
+    }
+    enabled[c] = (dn - b.radius > 5.0 * b.radius) ? 0 : 1;
+}
+
 __global__ void k_contact_distance(int K, DevBoundary b, const int* __restrict__ nodes, const double* __restrict__ q,
                                    double* __restrict__ delta) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1517,12 +1602,25 @@ void launch_scale(hipStream_t s, int n, double a, double* x) {
     hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x);
 }
 void launch_contact_frames(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* nrm,
-                           double* t1, double* t2) {
+                           double* t1, double* t2, unsigned char* enabled_out) {
     if (K == 0) return;
+    if (b.type == 5) {
+        if (!enabled_out) throw std::invalid_argument("mesh boundary frames need the candidate filter output");
+        hipLaunchKernelGGL(k_mesh_query<true>, dim3(blocks_for(K, 4)), dim3(256), 0, s, K, b, nodes, q, nullptr, nrm, t1,
+                           t2, enabled_out);
+        return;
+    }
     hipLaunchKernelGGL(k_contact_frames, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, b, nodes, q, nrm, t1, t2);
 }
-void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta) {
+void launch_contact_distance(hipStream_t s, int K, const DevBoundary& b, const int* nodes, const double* q, double* delta,
+                             const double* refn) {
     if (K == 0) return;
+    if (b.type == 5) {
+        if (!refn) throw std::invalid_argument("mesh boundary distances need the frame-start normals");
+        hipLaunchKernelGGL(k_mesh_query<false>, dim3(blocks_for(K, 4)), dim3(256), 0, s, K, b, nodes, q, refn, delta,
+                           nullptr, nullptr, nullptr);
+        return;
+    }
     hipLaunchKernelGGL(k_contact_distance, dim3(blocks_for(K, 128)), dim3(128), 0, s, K, b, nodes, q, delta);
 }
 template <class TV>

@@ -267,6 +267,21 @@ void Body::set_boundary(const std::string& type, const std::vector<double>& p, c
         b.z0 = p[0];
         b.waves.assign(p.begin() + 2, p.begin() + 2 + 4 * nw);
         if (type == "press") b.plane_z = p[2 + 4 * nw];
+    } else if (type == "mesh") {
+        // deformable_friction.cpp:53-73: [radius, split, n_left, n_right, faces_left, faces_right]
+        check(p.size() >= 4, "Incompatible parameter number.");
+        b.type = 5;
+        b.radius = p[0];
+        b.split = (int)p[1];
+        b.nfl = (int)p[2];
+        b.nfr = (int)p[3];
+        check(b.nfl >= 0 && b.nfr >= 0 && p.size() == 4 + 3 * (size_t)(b.nfl + b.nfr),
+              "Inconsistent params for MeshFrictionalBoundary.");
+        b.faces.resize(3 * (size_t)(b.nfl + b.nfr));
+        for (size_t i = 0; i < b.faces.size(); ++i) {
+            b.faces[i] = (int)p[4 + i];
+            check(0 <= b.faces[i] && b.faces[i] < nv_, "Mesh boundary face vertex out of bound.");
+        }
     } else {
         throw std::runtime_error("Unsupported frictional boundary type: " + type);
     }
@@ -289,6 +304,11 @@ DevBoundary Body::dev_boundary() {
     b.plane_z = boundary_.plane_z;
     b.nwaves = (int)boundary_.waves.size() / 4;
     b.waves = d_waves_.p;
+    b.split = boundary_.split;
+    b.nfl = boundary_.nfl;
+    b.nfr = boundary_.nfr;
+    b.faces = d_mesh_faces_.p;
+    b.radius = boundary_.radius;
     return b;
 }
 
@@ -557,6 +577,7 @@ void Body::setup_contact(int threads) {
     for (auto& kv : fb_vertices_) cand_.push_back(kv.first);
     K_ = (int)cand_.size();
     if (!boundary_.waves.empty()) d_waves_.upload(boundary_.waves);
+    if (!boundary_.faces.empty()) d_mesh_faces_.upload(boundary_.faces);
     if (K_ == 0) return;
     d_cand_.upload(cand_);
     // paths: contact c in column iperm[v] walks its ancestors
@@ -969,7 +990,16 @@ void Body::mlp_features(const double* q_dev, const double* v_dev, double dt, boo
         hip_check(hipStreamSynchronize(stream_), "features");
         for (auto& kv : fb_vertices_) {
             double d, nn[3];
-            boundary_.eval(&qh[3 * kv.first], d, nn);
+            if (boundary_.type == 5) {  // unsigned distance to the opposite surface
+                double cl[3], fn[3];
+                const double* x = &qh[3 * kv.first];
+                d = boundary_.mesh_closest(x, qh.data(), kv.first, cl, fn)
+                        ? std::sqrt((x[0] - cl[0]) * (x[0] - cl[0]) + (x[1] - cl[1]) * (x[1] - cl[1]) +
+                                    (x[2] - cl[2]) * (x[2] - cl[2])) - boundary_.radius
+                        : 4.0 * L;
+            } else {
+                boundary_.eval(&qh[3 * kv.first], d, nn);
+            }
             cd[kv.first] = (float)std::max(-4.0, std::min(4.0, d / L));
         }
         for (size_t i = 0; i < prev_candidates_.size() && i < prev_lambda_.size(); ++i)
@@ -1311,8 +1341,18 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
         d_enabled_.upload(en);
         enabled = d_enabled_.p;
     }
+    const bool mesh = boundary_.type == 5;
+    check(!(rp && mesh), "The fixed-active-set replay does not support the mesh boundary in this build.");
     if (K > 0) {
-        launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
+        if (mesh) {
+            // frame-start anchors and the candidate filter (:460-509); the
+            // filtered-out candidates stay out of every contact set this frame
+            d_enabled_.alloc(K);
+            launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p, d_enabled_.p);
+            enabled = d_enabled_.p;
+        } else {
+            launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
+        }
         std::vector<double> rprec(K);
         for (int c = 0; c < K; ++c) rprec[c] = std::max((rp ? 1.0 : h * h) * Hdiag_[c], 1e-10);
         if (!rp && !prev_candidates_.empty() && prev_lambda_.size() == prev_candidates_.size()) {
@@ -1370,7 +1410,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
             // the candidate positions and their obstacle distances
             if (precision == 1) contact_gather<float>(y32_, d_uC_.p); else contact_gather<double>(y64_, d_uC_.p);
             launch_gather_nodes(stream_, K, d_cand_.p, d_q_.p, d_dF_.p);
-            launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p);
+            launch_contact_distance(stream_, K, bnd, d_cand_.p, d_q_.p, d_delta_.p, d_nrm_.p);
             if (rp) launch_contact_frames(stream_, K, bnd, d_cand_.p, d_q_.p, d_nrm_.p, d_t1_.p, d_t2_.p);
             tcs.reset();
             std::unique_ptr<PhaseTimer> tcset(new PhaseTimer(this, kFwdCSets));
@@ -1501,13 +1541,28 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
     if (!converged) fprintf(stderr, "[Warning] alg_phd: PD loop did not converge.\n");
     // restitution post-pass (:1483-1531)
     if (restitution > 0 && K > 0) {
-        const double slop = opt_or(opt, "friction_contact_slop", 1e-8);
+        // mesh: the opposite surface at the last iterate's start (the positions
+        // of the last UpdateVertices call), slop 0.1 radius (:1485-1505)
+        std::vector<double> qf;
+        if (mesh) { qf.resize(n3); d_qprev_.download(qf.data(), n3); }
+        const double slop = opt_or(opt, "friction_contact_slop", mesh ? 0.1 * boundary_.radius : 1e-8);
         for (int c = 0; c < K; ++c) {
             if (lambda[c] <= 0) continue;
             const int node = cand_[c];
             double* x = &qs[3 * node];
             double sd, nh[3];
-            boundary_.eval(x, sd, nh);  // obstacle at the converged position
+            if (mesh) {
+                double cl[3], fn[3];
+                if (!boundary_.mesh_closest(x, qf.data(), node, cl, fn)) continue;
+                const double* rn = &nrm[3 * c];
+                double dp[3] = {x[0] - cl[0], x[1] - cl[1], x[2] - cl[2]};
+                sd = dp[0] * rn[0] + dp[1] * rn[1] + dp[2] * rn[2] - boundary_.radius;
+                const double dl = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+                for (int d = 0; d < 3; ++d) nh[d] = dl > 1e-12 ? dp[d] / dl : fn[d];
+                if (nh[0] * rn[0] + nh[1] * rn[1] + nh[2] * rn[2] < 0) for (int d = 0; d < 3; ++d) nh[d] = -nh[d];
+            } else {
+                boundary_.eval(x, sd, nh);  // obstacle at the converged position
+            }
             const double nn = std::sqrt(nh[0] * nh[0] + nh[1] * nh[1] + nh[2] * nh[2]);
             if (nn <= 1e-12) continue;
             for (int d = 0; d < 3; ++d) nh[d] /= nn;

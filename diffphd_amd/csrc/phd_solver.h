@@ -9,11 +9,13 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <cfloat>
 #include <cmath>
 #include <vector>
 
 #include "phd_factor.h"
 #include "phd_kernels.h"
+#include "phd_math.h"
 
 namespace phd {
 
@@ -69,9 +71,40 @@ struct Energy {
 };
 
 struct HostBoundary {
-    int type = 0;  // 0 none, 1 planar, 2 spherical, 3 heightfield, 4 press
+    int type = 0;  // 0 none, 1 planar, 2 spherical, 3 heightfield, 4 press, 5 mesh
     double n[3] = {0, 0, 1}, off = 0, c[3] = {0, 0, 0}, r = 0, z0 = 0, plane_z = 0;
     std::vector<double> waves;
+    // mesh: contact radius, split index, left then right faces (3 indices each)
+    double radius = 0;
+    int split = 0, nfl = 0, nfr = 0;
+    std::vector<int> faces;
+
+    // Mesh: closest point to p of the triangle set opposite to vertex `node`,
+    // the faces at the positions X (3N); false when that set is empty. fn: the
+    // face's unit normal ((0, 0, 1) for a degenerate face).
+    bool mesh_closest(const double* p, const double* X, int node, double closest[3], double fn[3]) const {
+        const bool right = node < split;
+        const int f0 = right ? nfl : 0, nf = right ? nfr : nfl;
+        double best = INFINITY;
+        int bf = -1;
+        for (int f = 0; f < nf; ++f) {
+            const int* fv = &faces[3 * (size_t)(f0 + f)];
+            double cl[3];
+            closest_on_triangle(p, &X[3 * (size_t)fv[0]], &X[3 * (size_t)fv[1]], &X[3 * (size_t)fv[2]], cl);
+            double d2 = 0;
+            for (int d = 0; d < 3; ++d) d2 += (p[d] - cl[d]) * (p[d] - cl[d]);
+            if (d2 < best) { best = d2; bf = f; for (int d = 0; d < 3; ++d) closest[d] = cl[d]; }
+        }
+        if (bf < 0) return false;
+        const int* fv = &faces[3 * (size_t)(f0 + bf)];
+        const double *A = &X[3 * (size_t)fv[0]], *B = &X[3 * (size_t)fv[1]], *C = &X[3 * (size_t)fv[2]];
+        const double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        fn[0] = e1[1] * e2[2] - e1[2] * e2[1]; fn[1] = e1[2] * e2[0] - e1[0] * e2[2]; fn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        const double fl = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        if (fl > DBL_EPSILON) for (int d = 0; d < 3; ++d) fn[d] /= fl;
+        else { fn[0] = 0; fn[1] = 0; fn[2] = 1; }
+        return true;
+    }
 
     // Signed distance and outward unit normal of the governing obstacle at q
     // (the host twin of the contact kernels' boundary evaluation).
@@ -325,6 +358,7 @@ private:
     DevBuf<int> d_gp_b_, d_gp_e_, d_gp_ptr_, d_sp_b_, d_sp_e_, d_sp_ptr_;
     DevBuf<double> d_piece_part_;
     DevBuf<double> d_waves_;
+    DevBuf<int> d_mesh_faces_;
     DevBuf<double> d_nrm_, d_t1_, d_t2_, d_delta_, d_F_, d_dF_, d_uC_, d_tmpK3_, d_om_, d_E_;
     // per-candidate NCP state of the frame (device-resident)
     DevBuf<double> d_rprec_, d_lambda_, d_lt1_, d_lt2_, d_qframeC_, d_hvec_, d_Fg_, d_Fn_, d_hprev_, d_Hdiag_;

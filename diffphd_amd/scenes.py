@@ -223,3 +223,55 @@ def tet_block(n=40, dx=None, seed=0, nu=0.4, inclusions=8, fmu=0.5):
         "boundary": ("planar", [0.0, 0.0, 1.0, 0.0], list(bottom)), "q0": q0, "v0": np.zeros_like(q0),
         "dt": 5e-3, "frames": 200, "options": default_options(friction_mu=fmu, max_pd_iter=1000),
     }
+
+
+def stack(top=(4, 4, 2), bottom=(8, 8, 2), dx=0.01, E=1e5, nu=0.45, rho=1000.0, fmu=0.5, gap=5e-4, radius=2e-4,
+          fixed_bottom="all", v_down=0.3, both_sides=True):
+    """Two hex blocks in one body in contact through the mesh frictional
+    boundary (MeshFrictionalBoundary, mesh_frictional_boundary.cpp): the upper
+    block (vertices 0 .. split - 1, its bottom face the left triangle set)
+    falls with speed v_down onto the lower block (its top face the right
+    set), starting `gap` above it; contact at distance `radius`. Upper
+    vertices query the right set, lower vertices the left set.
+    fixed_bottom: "all" (every lower vertex fixed: a rigid slab) or "base"
+    (only the lower block's bottom face). both_sides: the lower block's top
+    vertices are candidates too (else only the upper block's bottom ones)."""
+    Vb, Eb, _, fb_flag = hex_grid(np.ones(bottom), dx, (0.0, 0.0, -bottom[2] * dx))
+    ox, oy = (bottom[0] - top[0]) * dx / 2, (bottom[1] - top[1]) * dx / 2
+    Vt, Et, _, ft_flag = hex_grid(np.ones(top), dx, (ox, oy, gap))
+    nt = Vt.shape[1]
+    V = np.concatenate([Vt, Vb], axis=1)
+    Eidx = np.concatenate([Et, Eb + nt], axis=1)
+
+    def face_tris(flag, k, up, off):
+        tris = []
+        for i in range(flag.shape[0] - 1):
+            for j in range(flag.shape[1] - 1):
+                a, b, c, d = flag[i, j, k], flag[i + 1, j, k], flag[i + 1, j + 1, k], flag[i, j + 1, k]
+                if up:  # counter-clockwise seen from +z: outward normal +z
+                    tris += [(a, b, c), (a, c, d)]
+                else:
+                    tris += [(a, c, b), (a, d, c)]
+        return (np.asarray(tris, dtype=np.int64) + off).ravel()
+
+    left = face_tris(ft_flag, 0, False, 0)                       # upper block's bottom face
+    right = face_tris(fb_flag, bottom[2], True, nt)              # lower block's top face
+    cand_up = sorted(int(v) for v in ft_flag[:, :, 0].ravel())
+    cand_low = sorted(int(v) + nt for v in fb_flag[:, :, bottom[2]].ravel())
+    fbv = cand_up + (cand_low if both_sides else [])
+    params = [radius, nt, left.size // 3, right.size // 3] + [float(x) for x in left] + [float(x) for x in right]
+    mu, la = lame(E, nu)
+    if fixed_bottom == "all":
+        fixed = range(nt, V.shape[1])
+    else:
+        fixed = [int(v) + nt for v in fb_flag[:, :, 0].ravel()]
+    dirichlet = [(3 * i + k, float(V[k, i])) for i in fixed for k in range(3)]
+    q0 = V.T.ravel().copy()
+    v0 = np.zeros_like(q0)
+    v0[2:3 * nt:3] = -v_down
+    return {
+        "name": "stack", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "energies": [("corotated", [2 * mu], []), ("volume", [la], [])], "dirichlet": dirichlet,
+        "boundary": ("mesh", params, fbv), "q0": q0, "v0": v0, "dt": 1e-2, "frames": 20, "split": nt,
+        "mu": mu, "la": la, "options": default_options(friction_mu=fmu, max_pd_iter=500, rel_tol=1e-9),
+    }

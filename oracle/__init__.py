@@ -66,6 +66,7 @@ def lib():
         L.orc_filter_prox_jacobian.restype = ctypes.c_double
         L.orc_jacobi_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_coulomb_step.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_tri_closest.argtypes = [dp, dp, dp, dp, dp]
         L.orc_boundary_query.argtypes = [P, dp, dp]
         L.orc_frame_stats.argtypes = [P, ip, ip, ctypes.c_int]
         L.orc_reset_frame_stats.argtypes = [P]
@@ -291,6 +292,14 @@ def coulomb_step(Wtt, u_t, fmu, lam):
     w, wp = _d(np.asarray(Wtt, dtype=np.float64).ravel().copy())
     o, op = _d(np.zeros(2))
     lib().orc_coulomb_step(wp, float(u_t[0]), float(u_t[1]), float(fmu), float(lam), op)
+    return o.copy()
+
+
+def tri_closest(p, a, b, c):
+    """Closest point of triangle (a, b, c) to p (the mesh boundary's query)."""
+    arr = [_d(np.asarray(x, dtype=np.float64).copy()) for x in (p, a, b, c)]
+    o, op = _d(np.zeros(3))
+    lib().orc_tri_closest(arr[0][1], arr[1][1], arr[2][1], arr[3][1], op)
     return o.copy()
 
 

@@ -551,7 +551,38 @@ M3 stress(const Energy& en, const FAux& a, const M3& p) {
 // press scene (no reference counterpart: the reference has one planar or
 // spherical boundary per body).
 // ---------------------------------------------------------------------------
-enum BoundaryType { kNone = 0, kPlanar = 1, kSpherical = 2, kHeightfield = 3, kPress = 4 };
+enum BoundaryType { kNone = 0, kPlanar = 1, kSpherical = 2, kHeightfield = 3, kPress = 4, kMesh = 5 };
+
+// Closest point of triangle (a, b, c) to p (mesh_frictional_boundary.cpp:19-63,
+// the Voronoi-region method of Ericson, Real-Time Collision Detection 5.1.5):
+// vertex regions a, b, the edge ab, vertex c, the edge ac, the edge bc, then
+// the face interior by barycentric weights.
+void tri_closest(const double* p, const double* a, const double* b, const double* c, double* out) {
+    double ab[3], ac[3], ap[3], bp[3], cp[3];
+    for (int i = 0; i < 3; ++i) {
+        ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i];
+        ap[i] = p[i] - a[i]; bp[i] = p[i] - b[i]; cp[i] = p[i] - c[i];
+    }
+    auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { for (int i = 0; i < 3; ++i) out[i] = a[i]; return; }
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { for (int i = 0; i < 3; ++i) out[i] = b[i]; return; }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) { const double t = d1 / (d1 - d3); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ab[i]; return; }
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { for (int i = 0; i < 3; ++i) out[i] = c[i]; return; }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) { const double t = d2 / (d2 - d6); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ac[i]; return; }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int i = 0; i < 3; ++i) out[i] = b[i] + t * (c[i] - b[i]);
+        return;
+    }
+    const double den = 1 / (va + vb + vc);
+    for (int i = 0; i < 3; ++i) out[i] = a[i] + ab[i] * (vb * den) + ac[i] * (vc * den);
+}
 
 struct Boundary {
     int type = kNone;
@@ -560,6 +591,67 @@ struct Boundary {
     double z0 = 0;                         // heightfield base
     std::vector<double> waves;             // (amp, kx, ky, phase) quadruples
     double plane_z = 0;                    // press plane height (normal -z)
+    // mesh (MeshFrictionalBoundary): the body's own left / right triangle sets
+    double radius = 0;
+    int split = 0, nfl = 0, nfr = 0;
+    std::vector<int> faces;
+
+    // Closest point to p on the set opposite to `node` (right set for node <
+    // split, else left) at positions X, and that face's unit normal; false for
+    // an empty set (mesh_frictional_boundary.cpp:126-160, 176-212).
+    bool mesh_closest(const double* p, const Vec& X, int node, double cl[3], double fn[3]) const {
+        const bool right = node < split;
+        const int f0 = right ? nfl : 0, nf = right ? nfr : nfl;
+        double best = std::numeric_limits<double>::infinity();
+        int bf = -1;
+        for (int f = 0; f < nf; ++f) {
+            const int* fv = &faces[3 * (size_t)(f0 + f)];
+            double c[3];
+            tri_closest(p, &X[3 * (size_t)fv[0]], &X[3 * (size_t)fv[1]], &X[3 * (size_t)fv[2]], c);
+            const double d2 = (p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) + (p[2] - c[2]) * (p[2] - c[2]);
+            if (d2 < best) { best = d2; bf = f; for (int i = 0; i < 3; ++i) cl[i] = c[i]; }
+        }
+        if (bf < 0) return false;
+        const int* fv = &faces[3 * (size_t)(f0 + bf)];
+        const double *A = &X[3 * (size_t)fv[0]], *B = &X[3 * (size_t)fv[1]], *C = &X[3 * (size_t)fv[2]];
+        const double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        fn[0] = e1[1] * e2[2] - e1[2] * e2[1]; fn[1] = e1[2] * e2[0] - e1[0] * e2[2]; fn[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        const double fl = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        if (fl > std::numeric_limits<double>::epsilon()) for (int i = 0; i < 3; ++i) fn[i] /= fl;
+        else { fn[0] = 0; fn[1] = 0; fn[2] = 1; }
+        return true;
+    }
+    // GetClosestPointAndNormal (:176-212) and the Gram-Schmidt tangents of the
+    // alg_phd forward (forward_gpu.cpp:496-509); returns the unsigned distance
+    // minus the radius (GetDistance), +inf for an empty opposite set.
+    double mesh_frame(const double* p, const Vec& X, int node, double t1[3], double t2[3], double nn[3]) const {
+        double cl[3], fn[3];
+        if (!mesh_closest(p, X, node, cl, fn)) {
+            nn[0] = 0; nn[1] = 0; nn[2] = 1; t1[0] = 1; t1[1] = 0; t1[2] = 0; t2[0] = 0; t2[1] = 1; t2[2] = 0;
+            return std::numeric_limits<double>::infinity();
+        }
+        const double dp[3] = {p[0] - cl[0], p[1] - cl[1], p[2] - cl[2]};
+        const double dn = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+        for (int i = 0; i < 3; ++i) nn[i] = dn > 1e-12 ? dp[i] / dn : fn[i];
+        int ax = 0;
+        for (int d = 1; d < 3; ++d) if (std::fabs(nn[d]) < std::fabs(nn[ax])) ax = d;
+        double e[3] = {0, 0, 0};
+        e[ax] = 1;
+        const double ne = nn[ax];
+        double u[3] = {e[0] - ne * nn[0], e[1] - ne * nn[1], e[2] - ne * nn[2]};
+        const double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int i = 0; i < 3; ++i) t1[i] = u[i] / ul;
+        double w[3] = {nn[1] * t1[2] - nn[2] * t1[1], nn[2] * t1[0] - nn[0] * t1[2], nn[0] * t1[1] - nn[1] * t1[0]};
+        const double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        for (int i = 0; i < 3; ++i) t2[i] = w[i] / wl;
+        return dn - radius;
+    }
+    // GetSignedDistanceWithReferenceNormal (:216-255)
+    double mesh_signed(const double* p, const Vec& X, int node, const double* ref) const {
+        double cl[3], fn[3];
+        if (!mesh_closest(p, X, node, cl, fn)) return std::numeric_limits<double>::infinity();
+        return (p[0] - cl[0]) * ref[0] + (p[1] - cl[1]) * ref[1] + (p[2] - cl[2]) * ref[2] - radius;
+    }
 
     void hf(double x, double y, double& h, double& hx, double& hy) const {
         h = z0; hx = 0; hy = 0;
@@ -961,6 +1053,18 @@ struct Body {
             b.z0 = p[0];
             b.waves.assign(p.begin() + 2, p.begin() + 2 + 4 * nw);
             if (type == "press") b.plane_z = p[2 + 4 * nw];
+        } else if (type == "mesh") {
+            // deformable_friction.cpp:53-73: [radius, split, n_left, n_right, faces_left, faces_right]
+            check(p.size() >= 4, "Incompatible parameter number.");
+            b.type = kMesh;
+            b.radius = p[0];
+            b.split = (int)p[1];
+            b.nfl = (int)p[2];
+            b.nfr = (int)p[3];
+            check(b.nfl >= 0 && b.nfr >= 0 && p.size() == 4 + 3 * (size_t)(b.nfl + b.nfr),
+                  "Inconsistent params for MeshFrictionalBoundary.");
+            b.faces.resize(3 * (size_t)(b.nfl + b.nfr));
+            for (size_t i = 0; i < b.faces.size(); ++i) b.faces[i] = (int)p[4 + i];
         } else {
             check(false, "Unsupported frictional boundary type: " + type);
         }
@@ -1271,11 +1375,25 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
     }
 
     // Contact candidates and frames (:456-528).
+    // The mesh boundary keeps the candidates within 5 radii of the opposite
+    // surface at the frame start, anchored at their closest points (:460-509).
+    const bool mesh = boundary.type == kMesh;
     std::vector<int> cand;
-    for (auto& kv : fb_vertices) cand.push_back(kv.first);
+    std::vector<std::array<double, 3>> nrm, t1, t2;
+    for (auto& kv : fb_vertices) {
+        std::array<double, 3> a, b, c;
+        if (mesh) {
+            const double du = boundary.mesh_frame(&q[3 * kv.first], q, kv.first, a.data(), b.data(), c.data());
+            if (du > boundary.radius * 5.0) continue;
+        } else {
+            boundary.frame(&q[3 * kv.first], a.data(), b.data(), c.data());
+        }
+        cand.push_back(kv.first);
+        t1.push_back(a);
+        t2.push_back(b);
+        nrm.push_back(c);
+    }
     const int K = (int)cand.size();
-    std::vector<std::array<double, 3>> nrm(K), t1(K), t2(K);
-    for (int c = 0; c < K; ++c) boundary.frame(&q[3 * cand[c]], t1[c].data(), t2[c].data(), nrm[c].data());
 
     // Delassus W via per-column A^-1 (:840-1013).
     std::vector<std::array<Vec, 3>> Acol(K), At1(K), At2(K);
@@ -1343,7 +1461,8 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
         if (K > 0) {
             Vec omega(K), E(K), hvec(K);
             for (int c = 0; c < K; ++c) {
-                const double dj = boundary.distance(&q_sol[3 * cand[c]]);
+                const double dj = mesh ? boundary.mesh_signed(&q_sol[3 * cand[c]], q_sol, cand[c], nrm[c].data())
+                                       : boundary.distance(&q_sol[3 * cand[c]]);
                 const double lj = lambda[c], rj = rprec[c];
                 double den = std::sqrt(dj * dj + rj * rj * lj * lj);
                 if (den < 1e-14) den = 1e-14;
@@ -1452,13 +1571,26 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
 
     // Restitution post-pass (:1483-1531).
     if (restitution > 0) {
-        const double slop = opt_or(opt, "friction_contact_slop", 1e-8);
+        // mesh: the opposite surface where the last UpdateVertices left it (the
+        // last iterate's start), slop 0.1 radius (:1485-1505)
+        const double slop = opt_or(opt, "friction_contact_slop", mesh ? 0.1 * boundary.radius : 1e-8);
         for (int c = 0; c < K; ++c) {
             if (lambda[c] <= 0) continue;
             const int node = cand[c];
-            const double sd = boundary.distance(&q_sol[3 * node]);
-            double nh[3];
-            boundary.normal(&q_sol[3 * node], nh);
+            double sd, nh[3];
+            if (mesh) {
+                double cl[3], fn[3];
+                const double* x = &q_sol[3 * node];
+                if (!boundary.mesh_closest(x, q_prev, node, cl, fn)) continue;
+                const double dp[3] = {x[0] - cl[0], x[1] - cl[1], x[2] - cl[2]};
+                sd = dp[0] * nrm[c][0] + dp[1] * nrm[c][1] + dp[2] * nrm[c][2] - boundary.radius;
+                const double dl = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+                for (int d = 0; d < 3; ++d) nh[d] = dl > 1e-12 ? dp[d] / dl : fn[d];
+                if (nh[0] * nrm[c][0] + nh[1] * nrm[c][1] + nh[2] * nrm[c][2] < 0) for (int d = 0; d < 3; ++d) nh[d] = -nh[d];
+            } else {
+                sd = boundary.distance(&q_sol[3 * node]);
+                boundary.normal(&q_sol[3 * node], nh);
+            }
             const double nn = std::sqrt(nh[0] * nh[0] + nh[1] * nh[1] + nh[2] * nh[2]);
             if (nn <= 1e-12) continue;
             for (int d = 0; d < 3; ++d) nh[d] /= nn;
@@ -1516,6 +1648,7 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
 AlgPhdReplayCache Body::replay_fixed_active_set(const Vec& q, const Vec& v, const Vec& f_ext, double dt,
                                                 const std::vector<int>& active,
                                                 const std::map<std::string, double>& opt) {
+    check(boundary.type != kMesh, "The fixed-active-set replay does not support the mesh boundary in this build.");
     const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
     const double abs_tol = opt_req(opt, "abs_tol");
     const double rel_tol = opt_req(opt, "rel_tol");
@@ -2057,6 +2190,10 @@ int orc_last_backward_stats(void* hp, double* out, int cap) {
 }
 void orc_coulomb_step(const double* Wt, double u1, double u2, double fmu, double lam, double* out2) {
     orc::coulomb_step(Wt, u1, u2, fmu, lam, out2[0], out2[1]);
+}
+// Closest point of triangle (a, b, c) to p (the mesh boundary's query).
+void orc_tri_closest(const double* p, const double* a, const double* b, const double* c, double* out3) {
+    orc::tri_closest(p, a, b, c, out3);
 }
 // Frictional boundary of the body at q (3): out = [distance, n (3), t1 (3), t2 (3),
 // governing obstacle of the press: 0 ground, 1 plane (-1 for other types)].
