@@ -38,6 +38,11 @@ SIGNATURES = [
     ("phd_dofs", ctypes.c_int, [_P]),
     ("phd_num_elements", ctypes.c_int, [_P]),
     ("phd_num_pd_element_energies", ctypes.c_int, [_P]),
+    ("phd_add_actuation", ctypes.c_int, [_P, ctypes.c_double, _dp, _ip, ctypes.c_int]),
+    ("phd_set_actuation", ctypes.c_int, [_P, _dp, ctypes.c_int]),
+    ("phd_act_dofs", ctypes.c_int, [_P]),
+    ("phd_num_pd_muscle_energies", ctypes.c_int, [_P]),
+    ("phd_actuation_gradients", ctypes.c_int, [_P, _dp, ctypes.c_int, _dp, ctypes.c_int]),
     ("phd_s_memory_bytes", ctypes.c_longlong, [_P]),
     ("phd_s_nnz", ctypes.c_longlong, [_P]),
     ("phd_cached_n", ctypes.c_int, [_P]),

@@ -122,6 +122,25 @@ int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const doubl
 int phd_dofs(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->dofs() : -1; }
 int phd_num_elements(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_elements() : -1; }
 int phd_num_pd_element_energies(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_energies() : -1; }
+int phd_add_actuation(phd_ctx ctx, double stiffness, const double* fiber, const int* idx, int ni) {
+    return guard(ctx, [&](phd::Body& b) { b.add_actuation(stiffness, fiber, std::vector<int>(idx, idx + ni)); });
+}
+int phd_set_actuation(phd_ctx ctx, const double* a, int na) {
+    return guard(ctx, [&](phd::Body& b) { b.set_actuation(a, na); });
+}
+int phd_act_dofs(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->act_dofs() : -1; }
+int phd_num_pd_muscle_energies(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_muscles() : -1; }
+int phd_actuation_gradients(phd_ctx ctx, double* dl_da, int na, double* dl_dact_w, int nm) {
+    return guard(ctx, [&](phd::Body& b) {
+        const auto& ga = b.last_dl_da();
+        const auto& gw = b.last_dl_dact_w();
+        if (na != (int)ga.size() || nm != (int)gw.size())
+            throw std::runtime_error("phd_actuation_gradients: sizes differ from the last backward call (" +
+                                     std::to_string(ga.size()) + " levels, " + std::to_string(gw.size()) + " groups)");
+        std::copy(ga.begin(), ga.end(), dl_da);
+        std::copy(gw.begin(), gw.end(), dl_dact_w);
+    });
+}
 long long phd_s_memory_bytes(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->factor_bytes() : 0; }
 long long phd_s_nnz(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->factor_nnz() : 0; }
 int phd_cached_n(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->cached_n() : -1; }

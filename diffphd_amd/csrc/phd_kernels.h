@@ -17,6 +17,12 @@ struct DevMaterials {
     int n_nh = 0;
     const double* wn[4] = {nullptr, nullptr, nullptr, nullptr};
     double nh_mu[4] = {0, 0, 0, 0}, nh_la[4] = {0, 0, 0, 0}, nh_k[4] = {0, 0, 0, 0};
+    // muscle fibers (PdMuscleEnergy), at most one per element: weight
+    // stiffness * vol / samples (0: none), unit fiber direction (SoA, 3 ne) and
+    // actuation level; mw == nullptr when the body has no muscles
+    const double* mw = nullptr;
+    const double* mfib = nullptr;
+    const double* mact = nullptr;
 };
 
 struct DotArgs {
@@ -238,5 +244,8 @@ void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const d
 // g[type][e]: z_e^T (-(vol/s) sum_s A_s^T vec(P_s / stiffness)) per merged energy type
 void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
                   double* g_vol, double* g_nh);
+// per muscle element (m.mw != nullptr): g_da[e] = z^T df/da, g_dw[e] = z^T df/d(act_w) (0 elsewhere)
+void launch_dact(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_da,
+                 double* g_dw);
 
 }  // namespace phd

@@ -420,6 +420,9 @@ __global__ void __launch_bounds__(256) k_local_step(DevMaterials m, const double
     for (int k = 0; k < EDIM; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
     const double wc = m.wc ? m.wc[e] : 0.0;
     const double wv = m.wv ? m.wv[e] : 0.0;
+    const double wm = m.mw ? m.mw[e] : 0.0;
+    double mf[3] = {0.0, 0.0, 0.0}, ma = 0.0;
+    if (wm != 0.0) { mf[0] = m.mfib[e]; mf[1] = m.mfib[m.ne + e]; mf[2] = m.mfib[2 * m.ne + e]; ma = m.mact[e]; }
     const double* Ge = m.G + (int64_t)e * m.g_stride;
     for (int s = 0; s < m.samples; ++s) {
         const double* G = Ge + s * EDIM * 3;
@@ -468,6 +471,19 @@ __global__ void __launch_bounds__(256) k_local_step(DevMaterials m, const double
             udv(U, ss, V, Pn);
 #pragma unroll
             for (int i = 0; i < 9; ++i) P[i] += wn * (double)Pn[i];
+        }
+        if (wm != 0.0) {
+            // muscle (ProjectToManifold, pd_muscle_energy.cpp:118-125): Bp = a F m / |F m|,
+            // contribution w M^T Bp = w Bp m^T (FP64 in both precisions)
+            double Fm[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Fm[i] = F[i] * mf[0] + F[i + 3] * mf[1] + F[i + 6] * mf[2];
+            const double l = sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+            const double r = l <= DBL_EPSILON ? 0.0 : wm * ma / l;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) P[i + 3 * j] += r * Fm[i] * mf[j];
         }
 #pragma unroll
         for (int k = 0; k < EDIM; ++k)
@@ -1044,6 +1060,29 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
     double U[9], sg[3], V[9];
     svd3(F, U, sg, V);
     const double wc = kc ? kc[e] : 0.0, wv = kv ? kv[e] : 0.0;
+    // muscle block w (a / l)(I - n n^T) dF m  m^T with n = F m / l (ProjectToManifoldDifferential,
+    // pd_muscle_energy.cpp:127-165; backward.cpp:54-80), added outside the spectral filter
+    double mc = 0.0, mn[3] = {0.0, 0.0, 0.0}, mf[3] = {0.0, 0.0, 0.0};
+    if (m.mw && m.mw[e] != 0.0) {
+        mf[0] = m.mfib[e]; mf[1] = m.mfib[m.ne + e]; mf[2] = m.mfib[2 * m.ne + e];
+        double Fm[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Fm[i] = F[i] * mf[0] + F[i + 3] * mf[1] + F[i + 6] * mf[2];
+        const double l = sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+        if (l > DBL_EPSILON) {
+            mc = m.mw[e] * m.mact[e] / l;
+            mn[0] = Fm[0] / l; mn[1] = Fm[1] / l; mn[2] = Fm[2] / l;
+        }
+    }
+    // column c = p + 3 q: entry i + 3 j is mc (d_ip - n_i n_p) m_q m_j
+    auto muscle_col = [&](int c, double* col) {
+        if (mc == 0.0) return;
+        const int p = c % 3, qq = c / 3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) col[i + 3 * j] += mc * ((i == p ? 1.0 : 0.0) - mn[i] * mn[p]) * mf[qq] * mf[j];
+    };
     double R[9], Sp[9], D[3], ssn[4][3];
     const double one[3] = {1.0, 1.0, 1.0};
     udv(U, one, V, R);
@@ -1188,6 +1227,7 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
 #pragma unroll
                         for (int j = 0; j < 3; ++j)
                             cl[i + 3 * j] = US[i] * V[j] + US[i + 3] * V[j + 3] + US[i + 6] * V[j + 6];
+                    muscle_col(c, cl);
 #pragma unroll
                     for (int i = 0; i < 9; ++i) out[(int64_t)(i + 9 * c) * m.ne] = (TW)cl[i];
                 }
@@ -1201,6 +1241,7 @@ __global__ void __launch_bounds__(64) k_kpd_build(DevMaterials m, const double* 
         for (int i = 0; i < 9; ++i) dF[i] = (i == c) ? 1.0 : 0.0;
         double col[9];
         op(dF, col);
+        muscle_col(c, col);
 #pragma unroll
         for (int i = 0; i < 9; ++i) out[(int64_t)(i + 9 * c) * m.ne] = (TW)col[i];
     }
@@ -1823,6 +1864,57 @@ template void launch_kpd_build<double>(hipStream_t, const DevMaterials&, const d
                                        const double*, int, double, double, int*, double*);
 template void launch_kpd_apply<float>(hipStream_t, const DevMaterials&, const float*, const double*, double*);
 template void launch_kpd_apply<double>(hipStream_t, const DevMaterials&, const double*, const double*, double*);
+
+// dl/da and dl/d(act_w) per muscle element (ActuationForceDifferential,
+// deformable_actuation.cpp:120-177, contracted with z): with Fm = F m,
+// Fzm = (sum_k z_k G_k^T) m and l = |Fm|, per sample
+//   z^T df/da = (w / l) Fm . Fzm,  z^T df/dw_act = -(vol/s)(1 - a / l) Fm . Fzm
+// (df_k/da = -(vol/s) dP/da G_k with P = k (1 - a/l) F m m^T).
+template <int EDIM>
+__global__ void __launch_bounds__(128) k_dact(DevMaterials m, const double* __restrict__ q, const double* __restrict__ z,
+                                              double vs, double* __restrict__ g_da, double* __restrict__ g_dw) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const double wm = m.mw[e];
+    double sa = 0.0, sw = 0.0;
+    if (wm != 0.0) {
+        const double mf[3] = {m.mfib[e], m.mfib[m.ne + e], m.mfib[2 * m.ne + e]};
+        const double a = m.mact[e];
+        double x[EDIM][3], zz[EDIM][3];
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k) {
+            const int v = m.elems[k * m.ne + e];
+            x[k][0] = q[3 * v]; x[k][1] = q[3 * v + 1]; x[k][2] = q[3 * v + 2];
+            zz[k][0] = z[3 * v]; zz[k][1] = z[3 * v + 1]; zz[k][2] = z[3 * v + 2];
+        }
+        const double* Ge = m.G + (int64_t)e * m.g_stride;
+        for (int s = 0; s < m.samples; ++s) {
+            const double* G = Ge + s * EDIM * 3;
+            double Fm[3] = {0.0, 0.0, 0.0}, Zm[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < EDIM; ++k) {
+                const double gm = G[k * 3] * mf[0] + G[k * 3 + 1] * mf[1] + G[k * 3 + 2] * mf[2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { Fm[i] += x[k][i] * gm; Zm[i] += zz[k][i] * gm; }
+            }
+            const double l = sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+            if (l <= DBL_EPSILON) continue;
+            const double t = Fm[0] * Zm[0] + Fm[1] * Zm[1] + Fm[2] * Zm[2];
+            sa += wm / l * t;
+            sw -= vs * (1.0 - a / l) * t;
+        }
+    }
+    g_da[e] = sa;
+    g_dw[e] = sw;
+}
+
+void launch_dact(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_da,
+                 double* g_dw) {
+    if (m.edim == 8)
+        hipLaunchKernelGGL(k_dact<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_da, g_dw);
+    else
+        hipLaunchKernelGGL(k_dact<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_da, g_dw);
+}
 
 void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
                   double* g_vol, double* g_nh) {

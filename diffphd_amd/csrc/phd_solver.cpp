@@ -192,6 +192,12 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
         }
     }
     energies_.clear();
+    muscles_.clear();
+    act_dofs_ = 0;
+    mus_act_.assign(ne, -1);
+    act_.clear();
+    last_dl_da_.clear();
+    last_dl_dact_w_.clear();
     dirichlet_.clear();
     gravity_.clear();
     boundary_ = HostBoundary();
@@ -232,6 +238,43 @@ void Body::add_pd_energy(const std::string& type, const std::vector<double>& par
     if (idx.empty()) std::fill(en.member.begin(), en.member.end(), 1);
     for (int i : idx) { check(0 <= i && i < ne_, "Element index out of bound."); en.member[i] = 1; }
     energies_.push_back(en);
+}
+
+// AddActuation (deformable_actuation.cpp:5-26) with PdMuscleEnergy::Initialize
+// (pd_muscle_energy.cpp:5-24). Each element carries at most one fiber here
+// (the reference lets groups overlap).
+void Body::add_actuation(double stiffness, const double* fiber, const std::vector<int>& idx) {
+    const double nrm = std::sqrt(fiber[0] * fiber[0] + fiber[1] * fiber[1] + fiber[2] * fiber[2]);
+    check(nrm > DBL_EPSILON, "Singular fiber_direction.");
+    std::vector<char> seen(ne_, 0);
+    for (int i : idx) {
+        check(0 <= i && i < ne_, "Element index out of bound.");
+        check(!seen[i], "Duplicated elements.");
+        seen[i] = 1;
+        check(mus_act_[i] < 0, "Element " + std::to_string(i) + " already belongs to a muscle: each element carries at "
+                               "most one fiber in this build.");
+    }
+    Muscle mu;
+    mu.stiffness = stiffness;
+    for (int d = 0; d < 3; ++d) mu.m[d] = fiber[d] / nrm;
+    mu.elems = idx;
+    for (size_t k = 0; k < idx.size(); ++k) mus_act_[idx[k]] = act_dofs_ + (int)k;
+    act_dofs_ += (int)idx.size();
+    muscles_.push_back(mu);
+    act_.clear();
+    ready_ = false;
+}
+
+// The actuation levels a of the following forward and backward calls
+// (Deformable::Forward's `a`, one entry per muscle element in group order).
+void Body::set_actuation(const double* a, int na) {
+    check(na == act_dofs_, "Inconsistent actuation size.");
+    act_.assign(a, a + na);
+    if (act_dofs_ == 0) return;
+    std::vector<double> pe(ne_, 0.0);
+    for (int e = 0; e < ne_; ++e) if (mus_act_[e] >= 0) pe[e] = act_[mus_act_[e]];
+    d_mact_.upload(pe);
+    mat_.mact = d_mact_.p;
 }
 
 void Body::set_dirichlet(int dof, double val) {
@@ -360,6 +403,24 @@ void Body::build_materials() {
     mat_.wc = has_cor_ ? d_wc_.p : nullptr;
     mat_.wv = has_vol_ ? d_wv_.p : nullptr;
     mat_.n_nh = n_nh_;
+    if (act_dofs_ > 0) {
+        std::vector<double> mw(ne_, 0.0), mf((size_t)3 * ne_, 0.0), pe(ne_, 0.0);
+        for (const Muscle& mu : muscles_)
+            for (int e : mu.elems) {
+                mw[e] = mu.stiffness * vs;
+                for (int d = 0; d < 3; ++d) mf[(size_t)d * ne_ + e] = mu.m[d];
+            }
+        if ((int)act_.size() == act_dofs_)
+            for (int e = 0; e < ne_; ++e) if (mus_act_[e] >= 0) pe[e] = act_[mus_act_[e]];
+        d_mw_.upload(mw);
+        d_mfib_.upload(mf);
+        d_mact_.upload(pe);
+        mat_.mw = d_mw_.p;
+        mat_.mfib = d_mfib_.p;
+        mat_.mact = d_mact_.p;
+    } else {
+        d_mw_.release(); d_mfib_.release(); d_mact_.release();
+    }
     for (int h = 0; h < n_nh_; ++h) {
         const Energy& en = energies_[nh_energy_[h]];
         mat_.wn[h] = d_wn_[h].p;
@@ -434,6 +495,30 @@ void Body::setup(double dt, int threads) {
             for (int l = 0; l < edim_; ++l) {
                 if (dmask_[v[l]]) continue;
                 rows[v[k]].push_back({v[l], w_e[e] * Ke[k * edim_ + l]});
+            }
+        }
+    }
+    // muscle terms w A^T M^T M A (forward.cpp:112-150): block (k, l) is
+    // w (m.g_k)(m.g_l) I_3, the same on the three axes
+    for (const Muscle& mu : muscles_) {
+        const double w = mu.stiffness * vs;
+        for (int e : mu.elems) {
+            std::fill(Ke.begin(), Ke.end(), 0.0);
+            for (int s = 0; s < samples_; ++s) {
+                const double* g = &G_[(size_t)e * g_stride_ + (size_t)s * edim_ * 3];
+                for (int k = 0; k < edim_; ++k) {
+                    const double mk = g[k * 3] * mu.m[0] + g[k * 3 + 1] * mu.m[1] + g[k * 3 + 2] * mu.m[2];
+                    for (int l = 0; l < edim_; ++l)
+                        Ke[k * edim_ + l] += w * mk * (g[l * 3] * mu.m[0] + g[l * 3 + 1] * mu.m[1] + g[l * 3 + 2] * mu.m[2]);
+                }
+            }
+            const int* v = &elems_[(size_t)e * edim_];
+            for (int k = 0; k < edim_; ++k) {
+                if (dmask_[v[k]]) continue;
+                for (int l = 0; l < edim_; ++l) {
+                    if (dmask_[v[l]]) continue;
+                    rows[v[k]].push_back({v[l], Ke[k * edim_ + l]});
+                }
             }
         }
     }
@@ -532,6 +617,27 @@ void Body::setup(double dt, int threads) {
                     }
             }
         }
+        // muscles: w A^T M^T M F_bound = w (F_bound m)(m.g_k) at vertex k
+        for (const Muscle& mu : muscles_)
+            for (int e : mu.elems) {
+                const int* v = &elems_[(size_t)e * edim_];
+                bool any = false;
+                for (int k = 0; k < edim_; ++k)
+                    any |= dmask_[v[k]] != 0 || pmask_[3 * v[k]] || pmask_[3 * v[k] + 1] || pmask_[3 * v[k] + 2];
+                if (!any) continue;
+                for (int s = 0; s < samples_; ++s) {
+                    const double* g = &G_[(size_t)e * g_stride_ + (size_t)s * edim_ * 3];
+                    double Fbm[3] = {0, 0, 0};
+                    for (int k = 0; k < edim_; ++k) {
+                        const double gm = g[k * 3] * mu.m[0] + g[k * 3 + 1] * mu.m[1] + g[k * 3 + 2] * mu.m[2];
+                        for (int i = 0; i < 3; ++i) Fbm[i] += dval_full_[3 * v[k] + i] * gm;
+                    }
+                    for (int k = 0; k < edim_; ++k) {
+                        const double gm = g[k * 3] * mu.m[0] + g[k * 3 + 1] * mu.m[1] + g[k * 3 + 2] * mu.m[2];
+                        for (int i = 0; i < 3; ++i) c_dir_[3 * v[k] + i] += mu.stiffness * vs * Fbm[i] * gm;
+                    }
+                }
+            }
     }
     d_c_dir_.upload(c_dir_);
     // mesh data on device
@@ -1303,6 +1409,7 @@ void Body::forward_impl(const double* q, const double* v, const double* f_ext, d
     const int threads = (int)opt_req(opt, "thread_ct");
     const int aa_window = (int)opt_or(opt, "aa_window", 5);
     check(aa_window >= 0 && aa_window <= 6, "aa_window must be in [0, 6].");
+    check((int)act_.size() == act_dofs_, "Inconsistent actuation size.");
     const double fmu = opt_or(opt, "friction_mu", 0.0);
     const double restitution = opt_or(opt, "restitution", 0.0);
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
@@ -1619,6 +1726,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     const int threads = (int)opt_req(opt, "thread_ct");
     const int precision = (int)opt_or(opt, "phd_precision", 0.0);
     check(precision == 0 || precision == 1, "phd_precision must be 0 (FP64) or 1 (mixed FP32/FP64).");
+    check((int)act_.size() == act_dofs_, "Inconsistent actuation size.");
     setup(dt, threads);
     const bool exact = opt_or(opt, "alg_phd_bwd_exact_kpd", 0.0) > 0.5;
     if (exact != kpd_exact_) { kpd_exact_ = exact; build_materials(); }
@@ -1951,6 +2059,27 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
         }
     }
     if (!diverged && state_wb) inject(z);
+    // dl/da and dl/d(act_w) (backward_gpu.cpp:1209-1228): z^T of the actuation
+    // force's derivatives, kept only when every dl/da entry is finite
+    last_dl_da_.assign(act_dofs_, 0.0);
+    last_dl_dact_w_.assign(muscles_.size(), 0.0);
+    if (!diverged && act_dofs_ > 0) {
+        d_s.upload(z);
+        d_gda_.alloc(ne_); d_gdw_.alloc(ne_);
+        launch_dact(stream_, mat_, d_q_.p, d_s.p, element_volume_ / samples_, d_gda_.p, d_gdw_.p);
+        hip_check(hipStreamSynchronize(stream_), "dact");
+        std::vector<double> ga(ne_), gw(ne_), da(act_dofs_, 0.0), dw(muscles_.size(), 0.0);
+        d_gda_.download(ga.data(), ne_);
+        d_gdw_.download(gw.data(), ne_);
+        bool ok = true;
+        for (size_t gi = 0; gi < muscles_.size(); ++gi)
+            for (int e : muscles_[gi].elems) {
+                da[mus_act_[e]] = ga[e];
+                dw[gi] += gw[e];
+                ok = ok && std::isfinite(ga[e]);
+            }
+        if (ok) { last_dl_da_ = da; last_dl_dact_w_ = dw; }
+    }
     // dl/d(mat_w)
     const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
     if (!skip_mat && !diverged) {

@@ -169,10 +169,19 @@ public:
     void add_gravity(const double* g);
     void set_boundary(const std::string& type, const std::vector<double>& p, const std::vector<int>& idx);
     void set_press_plane(double z) { boundary_.plane_z = z; }
+    // PdMuscleEnergy groups (AddActuation) and the actuation levels a of the
+    // following forward/backward calls (one per group element, group order)
+    void add_actuation(double stiffness, const double* fiber, const std::vector<int>& idx);
+    void set_actuation(const double* a, int na);
+    // dl/da and dl/d(act_w) of the last backward call
+    const std::vector<double>& last_dl_da() const { return last_dl_da_; }
+    const std::vector<double>& last_dl_dact_w() const { return last_dl_dact_w_; }
 
     int dofs() const { return dofs_; }
     int num_elements() const { return ne_; }
     int num_energies() const { return (int)energies_.size(); }
+    int act_dofs() const { return act_dofs_; }
+    int num_muscles() const { return (int)muscles_.size(); }
     int num_boundary_vertices() const { return (int)fb_vertices_.size(); }
 
     // ---- hot path ----
@@ -280,6 +289,16 @@ private:
     std::vector<double> G_;
     int g_stride_ = 0;
     std::vector<Energy> energies_;
+    struct Muscle {
+        double stiffness = 0;
+        double m[3] = {0, 0, 0};  // unit fiber direction
+        std::vector<int> elems;
+    };
+    std::vector<Muscle> muscles_;
+    int act_dofs_ = 0;
+    std::vector<int> mus_act_;     // per element: index into a, or -1 (no muscle)
+    std::vector<double> act_;      // actuation levels of the current call
+    std::vector<double> last_dl_da_, last_dl_dact_w_;
     std::map<int, double> dirichlet_;
     std::vector<double> gravity_;
     HostBoundary boundary_;
@@ -326,6 +345,7 @@ private:
     DevBuf<int> d_elems_, d_inc_ptr_, d_inc_idx_;
     DevBuf<double> d_G_, d_wc_, d_wv_, d_wn_[4];
     DevBuf<double> d_kc_, d_kv_, d_kn_[4];
+    DevBuf<double> d_mw_, d_mfib_, d_mact_, d_gda_, d_gdw_;
     DevMaterials mat_;
     int n_nh_ = 0;
     std::vector<int> nh_energy_;  // energy index of each NH slot

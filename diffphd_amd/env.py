@@ -24,12 +24,22 @@ def _clip_adj(g, clip):
 
 
 def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, require_grad=True,
-             plane_schedule=None, grad_clip_norm=1e3, bwd_opt=None):
+             plane_schedule=None, grad_clip_norm=1e3, bwd_opt=None, a=None):
     """Returns (loss, grad, info); grad = (dl_dq0, dl_dv0, dl_df_ext per frame).
 
     loss_and_grad(q_T, v_T) -> (loss, dl_dq_T, dl_dv_T). plane_schedule: press
-    plane height per frame (prescribed obstacle motion), or None."""
+    plane height per frame (prescribed obstacle motion), or None. a: actuation
+    levels, one vector for every frame or one row per frame (act_dofs entries);
+    info then holds dl_da per frame and dl_dact_w summed over frames."""
     dofs = body.dofs()
+    na = body.act_dofs()
+    if a is None:
+        acts_in = np.zeros((frames, na))
+    else:
+        a = np.asarray(a, dtype=np.float64)
+        acts_in = np.tile(a, (frames, 1)) if a.ndim == 1 else a
+        if acts_in.shape != (frames, na):
+            raise RuntimeError("Inconsistent actuation size.")
     f_ext = np.zeros(dofs) if f_ext is None else np.asarray(f_ext, dtype=np.float64)
     bwd_opt = opt if bwd_opt is None else bwd_opt
     q = [np.asarray(q0, dtype=np.float64).copy()]
@@ -41,7 +51,7 @@ def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, r
         if plane_schedule is not None:
             body.SetPressPlaneHeight(plane_schedule[i])
         qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
-        body.PyForward(method, q[-1], v[-1], [], f_ext, dt, opt, qn, vn, act)
+        body.PyForward(method, q[-1], v[-1], list(acts_in[i]), f_ext, dt, opt, qn, vn, act)
         q.append(np.asarray(qn))
         v.append(np.asarray(vn))
         active.append(list(act))
@@ -56,20 +66,28 @@ def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, r
     dl_dmat_w = np.zeros(nE)
     dl_dw_elem = None
     dl_df = np.zeros((frames, dofs))
+    dl_da = np.zeros((frames, na))
+    dl_dact_w = np.zeros(body.NumOfPdMuscleEnergies())
     dq_next, dv_next = np.asarray(dl_dq, dtype=np.float64).copy(), np.asarray(dl_dv, dtype=np.float64).copy()
     for i in reversed(range(frames)):
         if plane_schedule is not None:
             body.SetPressPlaneHeight(plane_schedule[i])
         o = [StdRealVector(0) for _ in range(8)]
-        body.PyBackward(method, q[i], v[i], [], f_ext, dt, q[i + 1], v[i + 1], active[i + 1], dq_next, dv_next,
-                        bwd_opt, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7])
+        body.PyBackward(method, q[i], v[i], list(acts_in[i]), f_ext, dt, q[i + 1], v[i + 1], active[i + 1], dq_next,
+                        dv_next, bwd_opt, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7])
         dq_next = _clip_adj(np.asarray(o[0]), grad_clip_norm)
         dv_next = _clip_adj(np.asarray(o[1]), grad_clip_norm)
         dl_df[i] = _clip_adj(np.asarray(o[3]), grad_clip_norm)
         dl_dmat_w += np.asarray(o[4])
+        if na:
+            dl_da[i] = np.asarray(o[2])
+        if len(o[5]):
+            dl_dact_w += np.asarray(o[5])
         w = np.asarray(o[7])
         dl_dw_elem = w.copy() if dl_dw_elem is None else dl_dw_elem + w
     info["backward_time"] = time.time() - t0
     info["dl_dmat_w"] = dl_dmat_w
+    info["dl_da"] = dl_da
+    info["dl_dact_w"] = dl_dact_w
     info["dl_dw_elem"] = None if dl_dw_elem is None else dl_dw_elem.reshape(nE, -1)
     return loss, (dq_next, dv_next, dl_df), info

@@ -170,14 +170,24 @@ class _Deformable:
     def dofs(self):
         return self._dofs
 
+    def AddActuation(self, stiffness, fiber_direction, indices):
+        f, fp = _lib.dptr(list(fiber_direction))
+        ix, ip = _lib.iptr(list(indices) or [0])
+        self._ok(_lib.lib().phd_add_actuation(self._ctx, float(stiffness), fp, ip, len(indices)))
+
+    def _set_actuation(self, a):
+        aa = np.asarray(list(a), dtype=np.float64)
+        buf, bp = _lib.dptr(aa if aa.size else np.zeros(1))
+        self._ok(_lib.lib().phd_set_actuation(self._ctx, bp, int(aa.size)))
+
     def act_dofs(self):
-        return 0
+        return _lib.lib().phd_act_dofs(self._ctx)
 
     def NumOfPdElementEnergies(self):
         return _lib.lib().phd_num_pd_element_energies(self._ctx)
 
     def NumOfPdMuscleEnergies(self):
-        return 0
+        return _lib.lib().phd_num_pd_muscle_energies(self._ctx)
 
     def NumOfStateForceParameters(self):
         return 3 if self._gravity else 0
@@ -204,6 +214,7 @@ class _Deformable:
         import ctypes
         na = ctypes.c_int(cap)
         karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a)
         self._ok(_lib.lib().phd_step_forward(self._ctx, qp, vp, fp, float(dt), karr, valp, nk, qnp, vnp, ap,
                                              ctypes.byref(na)))
         q_next[:] = qn.tolist()
@@ -221,15 +232,20 @@ class _Deformable:
         nE = self.NumOfPdElementEnergies()
         outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), 3, max(nE * self._ne, 1))]
         karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a)
         self._ok(_lib.lib().phd_step_backward(self._ctx, ins[0][1], ins[1][1], ins[2][1], float(dt), ins[3][1],
                                               ins[4][1], ap, len(active_contact_idx), ins[5][1], ins[6][1], karr,
                                               valp, nk, *[o[1] for o in outs]))
+        na, nm = self.act_dofs(), self.NumOfPdMuscleEnergies()
+        ga, gap = _lib.dptr(np.zeros(max(na, 1)))
+        gw, gwp = _lib.dptr(np.zeros(max(nm, 1)))
+        self._ok(_lib.lib().phd_actuation_gradients(self._ctx, gap, na, gwp, nm))
         dl_dq[:] = outs[0][0].tolist()
         dl_dv[:] = outs[1][0].tolist()
-        dl_da[:] = []
+        dl_da[:] = ga[:na].tolist()
         dl_df_ext[:] = outs[2][0].tolist()
         dl_dmat_w[:] = outs[3][0][:nE].tolist()
-        dl_dact_w[:] = []
+        dl_dact_w[:] = gw[:nm].tolist()
         dl_dstate_p[:] = outs[4][0][: self.NumOfStateForceParameters()].tolist()
         if dl_dw_elem is not None:
             dl_dw_elem[:] = outs[5][0][: nE * self._ne].tolist()

@@ -26,6 +26,8 @@ def apply(spec, body):
         body.AddStateForce("gravity", list(spec["gravity"]))
     for kind, params, idx in spec["energies"]:
         body.AddPdEnergy(kind, list(params), list(idx))
+    for stiffness, fiber, idx in spec.get("muscles", []):
+        body.AddActuation(float(stiffness), list(fiber), list(idx))
     for dof, val in spec.get("dirichlet", []):
         body.SetDirichletBoundaryCondition(int(dof), float(val))
     b = spec.get("boundary")
@@ -59,6 +61,23 @@ def cantilever(cells=(16, 4, 4), dx=0.01, E=1e5, nu=0.45, rho=1000.0, per_elemen
         "dt": 1e-2, "frames": 100, "loss_g": rng.standard_normal(q0.size), "mu": mu, "la": la,
         "options": default_options(max_pd_iter=5000),
     }
+
+
+def actuated_cantilever(cells=(8, 2, 2), dx=0.01, E=1e5, nu=0.45, rho=1000.0, stiffness=(2e5, 1e5), seed=0):
+    """Cantilever with muscle fibers (CantileverEnv3d,
+    python/py_diff_pd/env/cantilever_env_3d.py:86-96): the bottom row of
+    elements (j = 0, k = 0) actuated along x, plus a second group, the top row
+    (j = ny - 1, k = nz - 1), along the diagonal (1, 0, 1) to exercise a fiber
+    off the grid axes. `act` holds one seeded level in [0.7, 1.0] per muscle
+    element (1 = the rest length, below 1 contracts)."""
+    spec = cantilever(cells, dx, E, nu, rho, seed=seed)
+    nx, ny, nz = cells
+    bottom = [i * ny * nz for i in range(nx)]
+    top = [i * ny * nz + (ny - 1) * nz + (nz - 1) for i in range(nx)]
+    spec["name"] = "actuated_cantilever"
+    spec["muscles"] = [(stiffness[0], [1.0, 0.0, 0.0], bottom), (stiffness[1], [1.0, 0.0, 1.0], top)]
+    spec["act"] = np.random.default_rng(seed + 100).uniform(0.7, 1.0, len(bottom) + len(top))
+    return spec
 
 
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):

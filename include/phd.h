@@ -48,6 +48,22 @@ int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int
 int phd_set_frictional_boundary(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni);
 /* Moves the press plane (prescribed motion between frames). No reference counterpart. */
 int phd_set_press_plane(phd_ctx ctx, double z);
+/* Deformable::AddActuation (deformable_actuation.cpp:5-26): one PdMuscleEnergy
+ * group (pd_muscle_energy.cpp:5-24) of stiffness, fiber direction fiber[3]
+ * (normalized here) and element indices idx[ni]. Each element may belong to one
+ * group only (the reference lets groups overlap). */
+int phd_add_actuation(phd_ctx ctx, double stiffness, const double* fiber, const int* idx, int ni);
+/* The actuation levels `a` of Deformable::Forward / Backward
+ * (deformable.h:116-125) for the following phd_step_forward / phd_step_backward
+ * calls: na == phd_act_dofs(ctx), one level per group element in group order. */
+int phd_set_actuation(phd_ctx ctx, const double* a, int na);
+/* Deformable::act_dofs() and NumOfPdMuscleEnergies() (deformable.h:47, 63-65). */
+int phd_act_dofs(phd_ctx ctx);
+int phd_num_pd_muscle_energies(phd_ctx ctx);
+/* dl_da (na = act_dofs) and dl_dact_w (nm = muscle groups) of the last
+ * phd_step_backward call: the outputs of Deformable::Backward that
+ * deformable_projective_dynamics_backward_gpu.cpp:1209-1228 fills. */
+int phd_actuation_gradients(phd_ctx ctx, double* dl_da, int na, double* dl_dact_w, int nm);
 
 /* SetupProjectiveDynamicsSolver (deformable_projective_dynamics_forward.cpp:8-183)
  * plus the sparse-inverse factor build of forward_gpu.cpp:584-764: assembles A,

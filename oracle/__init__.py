@@ -52,6 +52,11 @@ def lib():
         L.orc_add_gravity.argtypes = [P, dp]
         L.orc_set_frictional_boundary.argtypes = [P, cp, dp, ctypes.c_int, ip, ctypes.c_int]
         L.orc_set_press_plane.argtypes = [P, ctypes.c_double]
+        L.orc_add_actuation.argtypes = [P, ctypes.c_double, dp, ip, ctypes.c_int]
+        L.orc_set_actuation.argtypes = [P, dp, ctypes.c_int]
+        L.orc_act_dofs.argtypes = [P]
+        L.orc_num_muscles.argtypes = [P]
+        L.orc_actuation_grads.argtypes = [P, dp, ctypes.c_int, dp, ctypes.c_int]
         for f in ("orc_dofs", "orc_num_energies", "orc_num_elements", "orc_last_backward_iters"):
             getattr(L, f).argtypes = [P]
         L.orc_forward.argtypes = [P, dp, dp, dp, ctypes.c_double, ctypes.POINTER(cp), dp, ctypes.c_int,
@@ -163,17 +168,27 @@ class OracleDeformable:
     def SetPressPlaneHeight(self, z):
         self._ok(lib().orc_set_press_plane(self._h, float(z)))
 
+    def AddActuation(self, stiffness, fiber_direction, indices):
+        f, fp = _d(list(fiber_direction))
+        ix, ip = _i(list(indices) or [0])
+        self._ok(lib().orc_add_actuation(self._h, float(stiffness), fp, ip, len(indices)))
+
+    def _set_actuation(self, a):
+        a = np.asarray(list(a), dtype=np.float64)
+        buf, bp = _d(a if a.size else np.zeros(1))
+        self._ok(lib().orc_set_actuation(self._h, bp, int(a.size)))
+
     def dofs(self):
         return self._dofs
 
     def act_dofs(self):
-        return 0
+        return lib().orc_act_dofs(self._h)
 
     def NumOfPdElementEnergies(self):
         return lib().orc_num_energies(self._h)
 
     def NumOfPdMuscleEnergies(self):
-        return 0
+        return lib().orc_num_muscles(self._h)
 
     def NumOfStateForceParameters(self):
         return 3
@@ -188,6 +203,7 @@ class OracleDeformable:
         act, ap = _i(np.zeros(n // 3 + 1))
         na = ctypes.c_int(0)
         karr, vals, valp, nk = _opts(options)
+        self._set_actuation(a)
         self._ok(lib().orc_forward(self._h, qp, vp, fp, float(dt), karr, valp, nk, qnp, vnp, ap, ctypes.byref(na),
                                    n // 3 + 1))
         q_next[:] = list(qn)
@@ -203,15 +219,20 @@ class OracleDeformable:
         act, ap = _i(list(active_contact_idx) or [0])
         outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), 3, max(ne * nel, 1))]
         karr, vals, valp, nk = _opts(options)
+        self._set_actuation(a)
         self._ok(lib().orc_backward(self._h, arrs[0][1], arrs[1][1], arrs[2][1], float(dt), arrs[3][1], arrs[4][1],
                                     ap, len(active_contact_idx), arrs[5][1], arrs[6][1], karr, valp, nk,
                                     *[o[1] for o in outs]))
+        na, nm = self.act_dofs(), self.NumOfPdMuscleEnergies()
+        ga, gap = _d(np.zeros(max(na, 1)))
+        gw, gwp = _d(np.zeros(max(nm, 1)))
+        self._ok(lib().orc_actuation_grads(self._h, gap, na, gwp, nm))
         dl_dq[:] = list(outs[0][0])
         dl_dv[:] = list(outs[1][0])
-        dl_da[:] = []
+        dl_da[:] = list(ga[:na])
         dl_df_ext[:] = list(outs[2][0])
         dl_dmat_w[:] = list(outs[3][0][:ne])
-        dl_dact_w[:] = []
+        dl_dact_w[:] = list(gw[:nm])
         dl_dstate_p[:] = list(outs[4][0])
         if dl_dw_elem is not None:
             dl_dw_elem[:] = list(outs[5][0][: ne * nel])
@@ -223,13 +244,17 @@ class OracleDeformable:
         self._ok(lib().orc_apply_ainv(self._h, float(dt), bp, xp))
         return x
 
-    def local_step(self, q):
+    def local_step(self, q, a=None):
+        if a is not None:
+            self._set_actuation(a)
         q, qp = _d(q)
         out, op = _d(np.zeros_like(q))
         self._ok(lib().orc_local_step(self._h, qp, op))
         return out
 
-    def local_differential(self, q, exact=False, kpd_filter=0, trust_radius=0.9, trust_stage=1):
+    def local_differential(self, q, exact=False, kpd_filter=0, trust_radius=0.9, trust_stage=1, a=None):
+        if a is not None:
+            self._set_actuation(a)
         q, qp = _d(q)
         nd = 3 * self._edim
         out, op = _d(np.zeros(self._ne * nd * nd))

@@ -878,6 +878,18 @@ struct Body {
     int samples = 8;
     std::vector<double> grad;     // per (element, sample): edim x 3 row-major grad_undeformed_sample_weight
     std::vector<Energy> energies;
+    // PdMuscleEnergy groups (pd_muscle_energies_, deformable_actuation.cpp:5-26):
+    // stiffness, unit fiber direction m, elements; the actuation vector `act`
+    // holds one level per (group, element) in group order (act_dofs_)
+    struct Muscle {
+        double stiffness = 0;
+        double m[3] = {0, 0, 0};
+        std::vector<int> elems;
+    };
+    std::vector<Muscle> muscles;
+    int act_dofs = 0;
+    Vec act;
+    Vec last_dl_da, last_dl_dact_w;
     std::map<int, double> dirichlet;
     std::vector<double> gravity;  // empty or 3
     Boundary boundary;
@@ -1026,6 +1038,36 @@ struct Body {
         energies.push_back(en);
     }
 
+    // AddActuation (deformable_actuation.cpp:5-26) with PdMuscleEnergy::Initialize
+    // (pd_muscle_energy.cpp:5-24): the fiber direction is normalized.
+    void add_actuation(double stiffness, const double* fiber, const std::vector<int>& idx) {
+        pd_ready = false;
+        Muscle mu;
+        mu.stiffness = stiffness;
+        const double nrm = std::sqrt(fiber[0] * fiber[0] + fiber[1] * fiber[1] + fiber[2] * fiber[2]);
+        check(nrm > std::numeric_limits<double>::epsilon(), "Singular fiber_direction.");
+        for (int i = 0; i < 3; ++i) mu.m[i] = fiber[i] / nrm;
+        std::vector<char> seen(ne, 0);
+        for (int i : idx) {
+            check(0 <= i && i < ne, "Element index out of bound.");
+            check(!seen[i], "Duplicated elements.");
+            seen[i] = 1;
+            mu.elems.push_back(i);
+        }
+        muscles.push_back(mu);
+        act_dofs += (int)idx.size();
+    }
+    void set_actuation(const Vec& a) {
+        check((int)a.size() == act_dofs, "Inconsistent actuation size.");
+        act = a;
+    }
+    // Muscle projection Bp = a / |F m| F m (ProjectToManifold,
+    // pd_muscle_energy.cpp:118-125), zero when |F m| <= eps.
+    static void muscle_fm(const M3& F, const double* m, double Fm[3], double& l) {
+        for (int i = 0; i < 3; ++i) Fm[i] = F(i, 0) * m[0] + F(i, 1) * m[1] + F(i, 2) * m[2];
+        l = std::sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+    }
+
     // SetFrictionalBoundary, deformable_friction.cpp:7-89 (planar/spherical) +
     // heightfield / press.
     void set_boundary(const std::string& type, const Vec& p, const std::vector<int>& idx) {
@@ -1115,6 +1157,26 @@ struct Body {
                     }
             }
         }
+        // PdMuscleEnergy terms w A^T M^T M A (forward.cpp:112-150): with
+        // M vec(F) = F m the (k, l) block is w (m.g_k)(m.g_l) I_3
+        for (const Muscle& mu : muscles) {
+            const double w = mu.stiffness * element_volume / samples;
+            for (int e : mu.elems) {
+                const int* v = &elems[(size_t)e * edim];
+                for (int s = 0; s < samples; ++s) {
+                    const double* g = G(e, s);
+                    for (int k = 0; k < edim; ++k)
+                        for (int l = 0; l < edim; ++l) {
+                            const double mk = g[k * 3] * mu.m[0] + g[k * 3 + 1] * mu.m[1] + g[k * 3 + 2] * mu.m[2];
+                            const double ml = g[l * 3] * mu.m[0] + g[l * 3 + 1] * mu.m[1] + g[l * 3 + 2] * mu.m[2];
+                            for (int d = 0; d < 3; ++d) {
+                                if (dirichlet.count(3 * v[k] + d) || dirichlet.count(3 * v[l] + d)) continue;
+                                rows[d][v[k]][v[l]] += w * mk * ml;
+                            }
+                        }
+                }
+            }
+        }
         // Ordering from the pattern of axis 0 (identical Dirichlet sets give identical patterns;
         // a per-axis ordering is computed otherwise).
         for (int d = 0; d < 3; ++d) {
@@ -1179,6 +1241,32 @@ struct Body {
                 }
             }
         }
+        // PdMuscleEnergy (forward.cpp:240-270): w A^T (M^T Bp - M^T M F_bound),
+        // i.e. A^T of the matrix (Bp - F_bound m) m^T
+        check((int)act.size() == act_dofs, "Inconsistent actuation size.");
+        int base = 0;
+        for (const Muscle& mu : muscles) {
+            const double w = mu.stiffness * element_volume / samples;
+            const int cnt = (int)mu.elems.size();
+#pragma omp parallel for schedule(static)
+            for (int ei = 0; ei < cnt; ++ei) {
+                const int e = mu.elems[ei];
+                const double a = act[base + ei];
+                for (int s = 0; s < samples; ++s) {
+                    const M3 F = deformation_gradient(q.data(), e, s);
+                    const M3 Fb = deformation_gradient(qb.data(), e, s);
+                    double Fm[3], l, Fbm[3], lb;
+                    muscle_fm(F, mu.m, Fm, l);
+                    muscle_fm(Fb, mu.m, Fbm, lb);
+                    const double r = l <= std::numeric_limits<double>::epsilon() ? 0.0 : a / l;
+                    M3 P;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) P(i, j) = (r * Fm[i] - Fbm[i]) * mu.m[j];
+                    add_At(P, e, s, w, &acc[(size_t)e * edim * 3]);
+                }
+            }
+            base += cnt;
+        }
         Vec rhs(dofs, 0.0);
         for (int e = 0; e < ne; ++e)
             for (int k = 0; k < edim; ++k)
@@ -1202,6 +1290,16 @@ struct Body {
         long long nfilt = 0;
         const int nd = 3 * edim;
         std::vector<double> K((size_t)ne * nd * nd, 0.0);
+        // (muscle group, actuation index) of every element
+        std::vector<std::vector<std::pair<int, int>>> mus_of(ne);
+        {
+            int base = 0;
+            for (int gi = 0; gi < (int)muscles.size(); ++gi) {
+                for (int ei = 0; ei < (int)muscles[gi].elems.size(); ++ei) mus_of[muscles[gi].elems[ei]].push_back({gi, base + ei});
+                base += (int)muscles[gi].elems.size();
+            }
+            check((int)act.size() == act_dofs, "Inconsistent actuation size.");
+        }
 #pragma omp parallel for schedule(static) reduction(+ : nfilt)
         for (int e = 0; e < ne; ++e) {
             double* Ke = &K[(size_t)e * nd * nd];
@@ -1222,12 +1320,31 @@ struct Body {
                         for (int r = 0; r < 9; ++r) wdBp[r + 9 * c] += w * dp.a[r];
                     }
                 }
-                if (!any) continue;
-                if (factive) {
+                if (any && factive) {
                     double aw = 0;
                     for (auto& en : energies) if (en.member[e]) aw += en.elem_w(e) * element_volume / samples;
                     if (filter_prox_jacobian(wdBp, aw, theta, lo) > 1e-9) ++nfilt;
                 }
+                // PdMuscleEnergy blocks w A^T M^T dBp/dF A (backward.cpp:54-80,
+                // ProjectToManifoldDifferential, pd_muscle_energy.cpp:127-165),
+                // outside the filter: d(a Fm / l) = (a / l)(I - n n^T) dF m with
+                // n = Fm / l, so entry (i + 3j, p + 3q) is w (a/l)(d_ip - n_i n_p) m_q m_j
+                for (const auto& gm : mus_of[e]) {
+                    const Muscle& mu = muscles[gm.first];
+                    any = true;
+                    double Fm[3], l;
+                    muscle_fm(F, mu.m, Fm, l);
+                    if (l <= std::numeric_limits<double>::epsilon()) continue;
+                    const double c = mu.stiffness * element_volume / samples * act[gm.second] / l;
+                    const double n[3] = {Fm[0] / l, Fm[1] / l, Fm[2] / l};
+                    for (int p = 0; p < 3; ++p)
+                        for (int qq = 0; qq < 3; ++qq)
+                            for (int i = 0; i < 3; ++i)
+                                for (int j = 0; j < 3; ++j)
+                                    wdBp[(i + 3 * j) + 9 * (p + 3 * qq)] +=
+                                        c * ((i == p ? 1.0 : 0.0) - n[i] * n[p]) * mu.m[qq] * mu.m[j];
+                }
+                if (!any) continue;
                 // A: 9 x nd with A[i+3j, i+3k] = G(k,j).
                 const double* g = G(e, s);
                 for (int col = 0; col < nd; ++col) {
@@ -1997,6 +2114,52 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
         }
     }
     if (!diverged && state_wb) inject(z);
+    // dl/da and dl/d(act_w) (:1209-1228) with ActuationForceDifferential
+    // (deformable_actuation.cpp:120-177): df_k/da = -(vol/s) A^T vec(dP/da),
+    // df_k/dw = -(vol/s) A^T vec(P / stiffness), P = k (1 - a/l) F m m^T
+    // (StressTensor / StressTensorDifferential, pd_muscle_energy.cpp:33-116)
+    last_dl_da.assign(act_dofs, 0.0);
+    last_dl_dact_w.assign(muscles.size(), 0.0);
+    if (!diverged && act_dofs > 0) {
+        check((int)act.size() == act_dofs, "Inconsistent actuation size.");
+        Vec da(act_dofs, 0.0), dw(muscles.size(), 0.0);
+        int base = 0;
+        for (int gi = 0; gi < (int)muscles.size(); ++gi) {
+            const Muscle& mu = muscles[gi];
+            for (int ei = 0; ei < (int)mu.elems.size(); ++ei) {
+                const int e = mu.elems[ei];
+                const double a = act[base + ei];
+                const int* vv = &elems[(size_t)e * edim];
+                double fa[24] = {0}, fw[24] = {0};
+                for (int s = 0; s < samples; ++s) {
+                    const M3 F = deformation_gradient(q_next.data(), e, s);
+                    double Fm[3], l;
+                    muscle_fm(F, mu.m, Fm, l);
+                    if (l <= std::numeric_limits<double>::epsilon()) continue;
+                    M3 dPda, Pw;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            dPda(i, j) = -mu.stiffness / l * Fm[i] * mu.m[j];
+                            Pw(i, j) = (1 - a / l) * Fm[i] * mu.m[j];
+                        }
+                    add_At(dPda, e, s, -element_volume / samples, fa);
+                    add_At(Pw, e, s, -element_volume / samples, fw);
+                }
+                double sa = 0, sw = 0;
+                for (int k = 0; k < edim; ++k)
+                    for (int i = 0; i < 3; ++i) {
+                        sa += z[3 * vv[k] + i] * fa[3 * k + i];
+                        sw += z[3 * vv[k] + i] * fw[3 * k + i];
+                    }
+                da[base + ei] += sa;
+                dw[gi] += sw;
+            }
+            base += (int)mu.elems.size();
+        }
+        bool ok = true;
+        for (double x : da) if (!std::isfinite(x)) ok = false;
+        if (ok) { last_dl_da = da; last_dl_dact_w = dw; }
+    }
     // dl/d(mat_w) (:1231-1249) with PdEnergyForceDifferential(require_dw),
     // deformable_pd_energy.cpp:368-403: df/dw = -(vol/s) A^T vec(P / stiffness_).
     const bool skip_mat = opt_or(opt, "skip_material_grad", 0.0) > 0.5;
@@ -2089,6 +2252,26 @@ int orc_set_frictional_boundary(void* hp, const char* type, const double* p, int
 int orc_set_press_plane(void* hp, double z) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.boundary.plane_z = z; });
+}
+int orc_add_actuation(void* hp, double stiffness, const double* fiber, const int* idx, int ni) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_actuation(stiffness, fiber, std::vector<int>(idx, idx + ni)); });
+}
+int orc_set_actuation(void* hp, const double* a, int na) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.set_actuation(orc::Vec(a, a + na)); });
+}
+int orc_act_dofs(void* hp) { return static_cast<Handle*>(hp)->body.act_dofs; }
+int orc_num_muscles(void* hp) { return (int)static_cast<Handle*>(hp)->body.muscles.size(); }
+// dl/da and dl/d(act_w) of the last backward call
+int orc_actuation_grads(void* hp, double* dl_da, int na, double* dl_dact_w, int nm) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const auto& b = h->body;
+        orc::check(na == (int)b.last_dl_da.size() && nm == (int)b.last_dl_dact_w.size(), "actuation gradient sizes");
+        std::copy(b.last_dl_da.begin(), b.last_dl_da.end(), dl_da);
+        std::copy(b.last_dl_dact_w.begin(), b.last_dl_dact_w.end(), dl_dact_w);
+    });
 }
 int orc_dofs(void* hp) { return static_cast<Handle*>(hp)->body.dofs; }
 int orc_num_energies(void* hp) { return (int)static_cast<Handle*>(hp)->body.energies.size(); }

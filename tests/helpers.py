@@ -31,7 +31,7 @@ def rollout(body, spec, frames, opt=None):
         if spec.get("plane_schedule") is not None:
             body.SetPressPlaneHeight(spec["plane_schedule"][i])
         qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
-        body.PyForward("pd_eigen_alg_phd", q[-1], v[-1], [], f, spec["dt"], opt, qn, vn, act)
+        body.PyForward("pd_eigen_alg_phd", q[-1], v[-1], list(spec.get("act", [])), f, spec["dt"], opt, qn, vn, act)
         q.append(np.asarray(qn))
         v.append(np.asarray(vn))
         acts.append(list(act))
@@ -41,6 +41,6 @@ def rollout(body, spec, frames, opt=None):
 def backward_frame(body, spec, q, v, acts, i, dq, dv, opt=None):
     opt = dict(spec["options"] if opt is None else opt)
     o = [StdRealVector(0) for _ in range(8)]
-    body.PyBackward("pd_eigen_alg_phd", q[i], v[i], [], np.zeros(body.dofs()), spec["dt"], q[i + 1], v[i + 1],
-                    acts[i + 1], dq, dv, opt, *o)
+    body.PyBackward("pd_eigen_alg_phd", q[i], v[i], list(spec.get("act", [])), np.zeros(body.dofs()), spec["dt"],
+                    q[i + 1], v[i + 1], acts[i + 1], dq, dv, opt, *o)
     return [np.asarray(x) for x in o]
