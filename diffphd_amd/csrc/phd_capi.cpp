@@ -33,6 +33,26 @@ int guard(phd_ctx c, F&& f) {
         return 1;
     }
 }
+// Copies of caller vectors in device memory of the body's GPU (the
+// device-pointer entry points): the step's host-side logic (tape, candidate
+// sets, restitution) reads its inputs on the host.
+std::vector<double> from_device(const double* p, size_t n) {
+    std::vector<double> h(n);
+    if (n) phd::hip_check(hipMemcpy(h.data(), p, n * sizeof(double), hipMemcpyDeviceToHost), "device input");
+    return h;
+}
+void to_device(double* p, const std::vector<double>& h) {
+    if (!h.empty()) phd::hip_check(hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "device output");
+}
+void check_device_ptr(const phd::Body& b, const void* p, const char* what) {
+    hipPointerAttribute_t attr;
+    const hipError_t e = hipPointerGetAttributes(&attr, p);
+    const bool dev = e == hipSuccess && attr.type == hipMemoryTypeDevice && attr.device == b.device();
+    if (e != hipSuccess) (void)hipGetLastError();
+    if (!dev)
+        throw std::runtime_error(std::string(what) + " is not device memory of the body's GPU (" + std::to_string(b.device()) +
+                                 ")");
+}
 std::map<std::string, double> opts(const char* const* keys, const double* vals, int n) {
     std::map<std::string, double> o;
     for (int i = 0; i < n; ++i) o[keys[i]] = vals[i];
@@ -117,6 +137,50 @@ int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const doubl
     return guard(ctx, [&](phd::Body& b) {
         b.backward(q, v, f_ext, dt, q_next, v_next, std::vector<int>(active, active + n_active), dl_dq_next, dl_dv_next,
                    opts(keys, vals, n), dl_dq, dl_dv, dl_df_ext, dl_dmat_w, dl_dstate_p, dl_dw_elem);
+    });
+}
+int phd_step_forward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                            const char* const* keys, const double* vals, int n, double* q_next, double* v_next,
+                            int* active, int* n_active) {
+    return guard(ctx, [&](phd::Body& b) {
+        const size_t n3 = (size_t)b.dofs();
+        for (auto pv : {std::make_pair((const void*)q, "q"), std::make_pair((const void*)v, "v"),
+                        std::make_pair((const void*)f_ext, "f_ext"), std::make_pair((const void*)q_next, "q_next"),
+                        std::make_pair((const void*)v_next, "v_next")})
+            check_device_ptr(b, pv.first, pv.second);
+        if (!n_active || *n_active < b.num_boundary_vertices())
+            throw std::runtime_error("active_contact_idx buffer too small: it needs room for every frictional-boundary vertex (" +
+                                     std::to_string(b.num_boundary_vertices()) + ")");
+        const auto hq = from_device(q, n3), hv = from_device(v, n3), hf = from_device(f_ext, n3);
+        std::vector<double> qn(n3), vn(n3);
+        std::vector<int> act;
+        b.forward(hq.data(), hv.data(), hf.data(), dt, opts(keys, vals, n), qn.data(), vn.data(), act);
+        to_device(q_next, qn);
+        to_device(v_next, vn);
+        for (size_t i = 0; i < act.size(); ++i) active[i] = act[i];
+        *n_active = (int)act.size();
+    });
+}
+int phd_step_backward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                             const double* q_next, const double* v_next, const int* active, int n_active,
+                             const double* dl_dq_next, const double* dl_dv_next, const char* const* keys,
+                             const double* vals, int n, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                             double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem) {
+    return guard(ctx, [&](phd::Body& b) {
+        const size_t n3 = (size_t)b.dofs();
+        const void* dev[] = {q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next, dl_dq, dl_dv, dl_df_ext};
+        const char* names[] = {"q", "v", "f_ext", "q_next", "v_next", "dl_dq_next", "dl_dv_next", "dl_dq", "dl_dv", "dl_df_ext"};
+        for (int i = 0; i < 10; ++i) check_device_ptr(b, dev[i], names[i]);
+        const auto hq = from_device(q, n3), hv = from_device(v, n3), hf = from_device(f_ext, n3);
+        const auto hqn = from_device(q_next, n3), hvn = from_device(v_next, n3);
+        const auto hdq = from_device(dl_dq_next, n3), hdv = from_device(dl_dv_next, n3);
+        std::vector<double> oq(n3), ov(n3), of(n3);
+        b.backward(hq.data(), hv.data(), hf.data(), dt, hqn.data(), hvn.data(), std::vector<int>(active, active + n_active),
+                   hdq.data(), hdv.data(), opts(keys, vals, n), oq.data(), ov.data(), of.data(), dl_dmat_w, dl_dstate_p,
+                   dl_dw_elem);
+        to_device(dl_dq, oq);
+        to_device(dl_dv, ov);
+        to_device(dl_df_ext, of);
     });
 }
 int phd_dofs(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->dofs() : -1; }

@@ -178,6 +178,7 @@ public:
     const std::vector<double>& last_dl_dact_w() const { return last_dl_dact_w_; }
 
     int dofs() const { return dofs_; }
+    int device() const { return device_; }
     int num_elements() const { return ne_; }
     int num_energies() const { return (int)energies_.size(); }
     int act_dofs() const { return act_dofs_; }

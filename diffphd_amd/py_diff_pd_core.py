@@ -251,6 +251,66 @@ class _Deformable:
             dl_dw_elem[:] = outs[5][0][: nE * self._ne].tolist()
         _last_ctx[0] = self
 
+    # --- device tensors (DLPack handoff) ----------------------------------
+    def _dev(self, t, name):
+        import torch
+        t = torch.from_dlpack(t) if not isinstance(t, torch.Tensor) else t
+        if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous() or t.numel() != self._dofs:
+            raise RuntimeError(f"{name}: expected a contiguous float64 device tensor of {self._dofs} entries")
+        return t
+
+    def forward_device(self, method, q, v, a, f_ext, dt, options):
+        """PyForward on device tensors (torch, or anything torch.from_dlpack
+        takes): the tensors' memory goes to the library by pointer, q_next and
+        v_next come back as new tensors on the same device. Returns (q_next,
+        v_next, active_contact_idx)."""
+        import torch
+        if method not in METHODS:
+            raise RuntimeError("Unsupported forward method: " + str(method))
+        q, v, f_ext = self._dev(q, "q"), self._dev(v, "v"), self._dev(f_ext, "f_ext")
+        qn, vn = torch.empty_like(q), torch.empty_like(q)
+        cap = max(1, self._dofs // 3 + 1)
+        act, ap = _lib.iptr(np.zeros(cap))
+        na = ctypes.c_int(cap)
+        karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a.detach().cpu().numpy() if hasattr(a, "detach") else a)
+        torch.cuda.synchronize(q.device)
+        self._ok(_lib.lib().phd_step_forward_device(self._ctx, q.data_ptr(), v.data_ptr(), f_ext.data_ptr(), float(dt),
+                                                    karr, valp, nk, qn.data_ptr(), vn.data_ptr(), ap, ctypes.byref(na)))
+        _last_ctx[0] = self
+        return qn, vn, [int(x) for x in act[: na.value]]
+
+    def backward_device(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
+                        options):
+        """PyBackward on device tensors. Returns a dict of dl_dq, dl_dv,
+        dl_df_ext (tensors on the inputs' device) and dl_da, dl_dmat_w,
+        dl_dact_w, dl_dstate_p (numpy arrays)."""
+        import torch
+        if method not in METHODS:
+            raise RuntimeError("Unsupported backward method: " + str(method))
+        ins = [self._dev(x, name) for x, name in ((q, "q"), (v, "v"), (f_ext, "f_ext"), (q_next, "q_next"),
+                                                  (v_next, "v_next"), (dl_dq_next, "dl_dq_next"),
+                                                  (dl_dv_next, "dl_dv_next"))]
+        outs = [torch.empty_like(ins[0]) for _ in range(3)]
+        nE = self.NumOfPdElementEnergies()
+        host = [_lib.dptr(np.zeros(k)) for k in (max(nE, 1), 3, max(nE * self._ne, 1))]
+        act, ap = _lib.iptr(list(active_contact_idx) or [0])
+        karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a.detach().cpu().numpy() if hasattr(a, "detach") else a)
+        torch.cuda.synchronize(ins[0].device)
+        self._ok(_lib.lib().phd_step_backward_device(
+            self._ctx, ins[0].data_ptr(), ins[1].data_ptr(), ins[2].data_ptr(), float(dt), ins[3].data_ptr(),
+            ins[4].data_ptr(), ap, len(active_contact_idx), ins[5].data_ptr(), ins[6].data_ptr(), karr, valp, nk,
+            outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(), *[h[1] for h in host]))
+        na, nm = self.act_dofs(), self.NumOfPdMuscleEnergies()
+        ga, gap = _lib.dptr(np.zeros(max(na, 1)))
+        gw, gwp = _lib.dptr(np.zeros(max(nm, 1)))
+        self._ok(_lib.lib().phd_actuation_gradients(self._ctx, gap, na, gwp, nm))
+        _last_ctx[0] = self
+        return {"dl_dq": outs[0], "dl_dv": outs[1], "dl_df_ext": outs[2], "dl_da": ga[:na].copy(),
+                "dl_dmat_w": host[0][0][:nE].copy(), "dl_dact_w": gw[:nm].copy(),
+                "dl_dstate_p": host[1][0][: self.NumOfStateForceParameters()].copy()}
+
     # --- extensions used by tests and the benchmark --------------------------
     def factor_build(self, dt, threads=8):
         self._ok(_lib.lib().phd_factor_build(self._ctx, float(dt), int(threads)))

@@ -6,9 +6,12 @@ defects of the reference wrapper are fixed:
     outputs of `PyBackward` (`dl_dact_w`, `dl_dstate_p`); here the active set
     returned by the forward is kept and handed back, and the full output list
     is passed;
-  * tensors on a GPU device are moved to host memory for the call (the solver
-    takes host arrays) and the results are returned on the caller's device and
-    dtype.
+  * GPU tensors reach the HIP body without a host round-trip through numpy or
+    lists: float64 device tensors go to the library by pointer (DLPack
+    handoff, `forward_device` / `backward_device`) and the results are new
+    tensors on the same device. Other tensors (CPU, another dtype, or a body
+    without the device entry points) are copied to host arrays for the call
+    and come back on the caller's device and dtype.
 The per-energy material gradient `dl_dmat_w` of the last backward is kept in
 `Sim.last_dl_dmat_w` (the reference wrapper drops it).
 """
@@ -28,6 +31,11 @@ def _like(arr, ref):
     return torch.as_tensor(np.asarray(arr, dtype=np.float64)).to(device=ref.device, dtype=ref.dtype)
 
 
+def _device_path(deformable, *tensors):
+    return hasattr(deformable, "forward_device") and all(
+        t.is_cuda and t.dtype == torch.float64 and t.device == tensors[0].device for t in tensors)
+
+
 class SimFunction(autograd.Function):
 
     @staticmethod
@@ -38,13 +46,18 @@ class SimFunction(autograd.Function):
         ctx.method = method
         ctx.dt = dt
         ctx.option = option
-        q_next = StdRealVector(dofs)
-        v_next = StdRealVector(dofs)
-        active = StdIntVector(0)
-        deformable.PyForward(method, _host(q), _host(v), _host(a), _host(f_ext), dt, option, q_next, v_next, active)
-        ctx.active = list(active)
-        qn = _like(q_next, q)
-        vn = _like(v_next, v)
+        if _device_path(deformable, q, v, f_ext):
+            qn, vn, active = deformable.forward_device(method, q.detach().contiguous(), v.detach().contiguous(), a,
+                                                       f_ext.detach().contiguous(), dt, option)
+            ctx.active = list(active)
+        else:
+            q_next = StdRealVector(dofs)
+            v_next = StdRealVector(dofs)
+            active = StdIntVector(0)
+            deformable.PyForward(method, _host(q), _host(v), _host(a), _host(f_ext), dt, option, q_next, v_next, active)
+            ctx.active = list(active)
+            qn = _like(q_next, q)
+            vn = _like(v_next, v)
         ctx.save_for_backward(q, v, a, f_ext, qn, vn)
         return qn, vn
 
@@ -56,6 +69,13 @@ class SimFunction(autograd.Function):
             dl_dq_next = torch.zeros_like(q_next)
         if dl_dv_next is None:
             dl_dv_next = torch.zeros_like(v_next)
+        if _device_path(ctx.deformable, q, v, f_ext, dl_dq_next, dl_dv_next):
+            out = ctx.deformable.backward_device(ctx.method, q.contiguous(), v.contiguous(), a, f_ext.contiguous(),
+                                                 ctx.dt, q_next, v_next, ctx.active, dl_dq_next.contiguous(),
+                                                 dl_dv_next.contiguous(), ctx.option)
+            Sim.last_dl_dmat_w = out["dl_dmat_w"]
+            da = _like(out["dl_da"], a) if out["dl_da"].size == a.numel() else torch.zeros_like(a)
+            return (None, None, None, None, out["dl_dq"], out["dl_dv"], da, out["dl_df_ext"], None, None)
         dl_dq, dl_dv, dl_da, dl_df_ext = (StdRealVector(dofs), StdRealVector(dofs), StdRealVector(ctx.act_dofs),
                                           StdRealVector(dofs))
         dl_dmat_w, dl_dact_w, dl_dstate_p = StdRealVector(0), StdRealVector(0), StdRealVector(0)

@@ -121,6 +121,22 @@ int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const doubl
                       double* dl_dw_elem);
 
 /* Sizes. */
+/* phd_step_forward / phd_step_backward with the state vectors (q, v, f_ext,
+ * q_next, v_next, dl_dq_next, dl_dv_next, dl_dq, dl_dv, dl_df_ext) in device
+ * memory of the body's GPU, e.g. the data pointers of torch tensors handed over
+ * by DLPack; active, dl_dmat_w, dl_dstate_p and dl_dw_elem stay host arrays.
+ * They serve the zero-copy torch wrapper that the reference's sim.py:9-85
+ * lacked. Each call copies the inputs to the host once (the step's host logic
+ * reads them) and writes the outputs straight into the caller's buffers; a
+ * pointer that is not device memory of this GPU is an error. */
+int phd_step_forward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                            const char* const* keys, const double* vals, int n, double* q_next, double* v_next,
+                            int* active, int* n_active);
+int phd_step_backward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                             const double* q_next, const double* v_next, const int* active, int n_active,
+                             const double* dl_dq_next, const double* dl_dv_next, const char* const* keys,
+                             const double* vals, int n, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                             double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem);
 int phd_dofs(phd_ctx ctx);
 int phd_num_elements(phd_ctx ctx);
 int phd_num_pd_element_energies(phd_ctx ctx);
