@@ -50,7 +50,9 @@ def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, r
         t0 = time.time()
         if plane_schedule is not None:
             body.SetPressPlaneHeight(plane_schedule[i])
-        qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
+        # the previous frame's active set goes in (the baseline forward starts its
+        # active-set loop from it, env_base.py:263-265); this frame's comes out
+        qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), list(active[-1])
         body.PyForward(method, q[-1], v[-1], list(acts_in[i]), f_ext, dt, opt, qn, vn, act)
         q.append(np.asarray(qn))
         v.append(np.asarray(vn))

@@ -57,6 +57,8 @@ def lib():
         L.orc_act_dofs.argtypes = [P]
         L.orc_num_muscles.argtypes = [P]
         L.orc_actuation_grads.argtypes = [P, dp, ctypes.c_int, dp, ctypes.c_int]
+        L.orc_forward_baseline.argtypes = [P, cp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(cp), dp, ctypes.c_int,
+                                           dp, dp, ip, ctypes.c_int, ip, ip, ctypes.c_int, ip]
         for f in ("orc_dofs", "orc_num_energies", "orc_num_elements", "orc_last_backward_iters"):
             getattr(L, f).argtypes = [P]
         L.orc_forward.argtypes = [P, dp, dp, dp, ctypes.c_double, ctypes.POINTER(cp), dp, ctypes.c_int,
@@ -194,6 +196,10 @@ class OracleDeformable:
         return 3
 
     def PyForward(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
+        if method not in ("pd_eigen_alg_phd", "pd_eigen_pcg_proj_gpu"):
+            if not method.startswith("pd_eigen"):
+                raise RuntimeError("Unsupported forward method: " + str(method))
+            return self._forward_baseline(method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx)
         n = self._dofs
         qa, qp = _d(q)
         va, vp = _d(v)
@@ -209,6 +215,29 @@ class OracleDeformable:
         q_next[:] = list(qn)
         v_next[:] = list(vn)
         active_contact_idx[:] = [int(x) for x in act[: na.value]]
+
+    def _forward_baseline(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
+        """ForwardProjectiveDynamics (forward.cpp:529-664): active_contact_idx
+        holds the previous frame's active set on entry, this frame's on exit."""
+        n = self._dofs
+        qa, qp = _d(q)
+        va, vp = _d(v)
+        fa, fp = _d(f_ext)
+        qn, qnp = _d(np.zeros(n))
+        vn, vnp = _d(np.zeros(n))
+        ain, aip = _i(list(active_contact_idx) or [0])
+        cap = n // 3 + 1
+        aout, aop = _i(np.zeros(cap))
+        na = ctypes.c_int(0)
+        st, stp = _i(np.zeros(2))
+        karr, vals, valp, nk = _opts(options)
+        self._set_actuation(a)
+        self._ok(lib().orc_forward_baseline(self._h, method.encode(), qp, vp, fp, float(dt), karr, valp, nk, qnp, vnp,
+                                            aip, len(active_contact_idx), aop, ctypes.byref(na), cap, stp))
+        q_next[:] = list(qn)
+        v_next[:] = list(vn)
+        active_contact_idx[:] = [int(x) for x in aout[: na.value]]
+        self.last_baseline_stats = {"contact_iters": int(st[0]), "pd_iters": int(st[1])}
 
     def PyBackward(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
                    options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p, dl_dw_elem=None):

@@ -1392,6 +1392,20 @@ struct Body {
 
     void forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const std::map<std::string, double>& opt,
                  Vec& q_next, Vec& v_next, std::vector<int>& active);
+
+    // ---- the DiffPD baseline forward (ForwardProjectiveDynamics with a
+    // pd_eigen method other than pd_eigen_alg_phd, forward.cpp:294-664) ----
+    std::vector<int> lhs_frozen;  // frozen nodes of the cached factors below
+    SparseLDLT lhs_ldlt[3];
+    bool lhs_valid = false;
+    double baseline_energy(const Vec& q) const;
+    Vec baseline_force(const Vec& q) const;
+    Vec pd_lhs_solve(const Vec& rhs, const std::vector<int>& frozen);
+    Vec pd_nonlinear_solve(const Vec& q_init, double inv_h2m, const Vec& rhs, const std::map<int, double>& extra,
+                           const std::map<std::string, double>& opt, int* iters);
+    void forward_baseline(const std::string& method, const Vec& q, const Vec& v, const Vec& f_ext, double dt,
+                          const std::map<std::string, double>& opt, Vec& q_next, Vec& v_next, std::vector<int>& active);
+    int last_baseline_contact_iters = 0, last_baseline_pd_iters = 0;
     AlgPhdReplayCache replay_fixed_active_set(const Vec& q, const Vec& v, const Vec& f_ext, double dt,
                                               const std::vector<int>& active, const std::map<std::string, double>& opt);
     void backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
@@ -2197,6 +2211,366 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The DiffPD baseline forward.
+// ---------------------------------------------------------------------------
+// ComputePdEnergy (deformable_pd_energy.cpp:156-189) and ActuationEnergy
+// (deformable_actuation.cpp:28-46): element energies w/2 |F - p(F)|^2 with the
+// energy's stiffness() (the mean for per-element parameters, as
+// PdElementEnergy::EnergyDensity reads stiffness_), summed per element over the
+// energies, then over the elements; muscles k/2 (|F m| - a)^2.
+double Body::baseline_energy(const Vec& q) const {
+    std::vector<double> ee(ne, 0.0);
+    for (const auto& en : energies) {
+        check(en.type != kNeoHookean, "The baseline forward supports the corotated and volume energies.");
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            if (!en.member[e]) continue;
+            for (int s = 0; s < samples; ++s) {
+                const M3 F = deformation_gradient(q.data(), e, s);
+                FAux a;
+                a.init(F);
+                const M3 d = add(F, scale(project(en, a), -1.0));
+                double n2 = 0;
+                for (int i = 0; i < 9; ++i) n2 += d.a[i] * d.a[i];
+                ee[e] += en.stiffness * 0.5 * n2 * element_volume / samples;
+            }
+        }
+    }
+    double tot = 0;
+    for (double x : ee) tot += x;
+    int base = 0;
+    for (const Muscle& mu : muscles) {
+        for (int ei = 0; ei < (int)mu.elems.size(); ++ei)
+            for (int s = 0; s < samples; ++s) {
+                double Fm[3], l;
+                muscle_fm(deformation_gradient(q.data(), mu.elems[ei], s), mu.m, Fm, l);
+                const double a = act[base + ei];
+                tot += mu.stiffness * 0.5 * (l - a) * (l - a) * element_volume / samples;
+            }
+        base += (int)mu.elems.size();
+    }
+    return tot;
+}
+
+// PdEnergyForce (deformable_pd_energy.cpp:192-240) + ActuationForce
+// (deformable_actuation.cpp:48-77): f = -(vol/s) sum A^T vec(P), P = k (F - p(F))
+// for the element energies and k (1 - a/l) F m m^T for a muscle.
+Vec Body::baseline_force(const Vec& q) const {
+    std::vector<double> acc((size_t)ne * edim * 3, 0.0);
+    for (const auto& en : energies) {
+        check(en.type != kNeoHookean, "The baseline forward supports the corotated and volume energies.");
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            if (!en.member[e]) continue;
+            for (int s = 0; s < samples; ++s) {
+                const M3 F = deformation_gradient(q.data(), e, s);
+                FAux a;
+                a.init(F);
+                const M3 P = scale(add(F, scale(project(en, a), -1.0)), en.stiffness);
+                add_At(P, e, s, -element_volume / samples, &acc[(size_t)e * edim * 3]);
+            }
+        }
+    }
+    int base = 0;
+    for (const Muscle& mu : muscles) {
+        for (int ei = 0; ei < (int)mu.elems.size(); ++ei) {
+            const int e = mu.elems[ei];
+            for (int s = 0; s < samples; ++s) {
+                double Fm[3], l;
+                muscle_fm(deformation_gradient(q.data(), e, s), mu.m, Fm, l);
+                if (l <= std::numeric_limits<double>::epsilon()) continue;
+                M3 P;
+                const double r = mu.stiffness * (1 - act[base + ei] / l);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) P(i, j) = r * Fm[i] * mu.m[j];
+                add_At(P, e, s, -element_volume / samples, &acc[(size_t)e * edim * 3]);
+            }
+        }
+        base += (int)mu.elems.size();
+    }
+    Vec f(dofs, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < edim; ++k)
+            for (int i = 0; i < 3; ++i) f[3 * elems[(size_t)e * edim + k] + i] += acc[((size_t)e * edim + k) * 3 + i];
+    return f;
+}
+
+// PdLhsSolve (forward.cpp:690-936): the operator PdLhsMatrixOp (:666-688) keeps
+// the rows and columns of the frozen nodes out of A (x_c = rhs_c on them), so the
+// solve is x_f = A_ff^-1 rhs_f per axis. The reference reaches it by a Woodbury
+// update of A's factor; here A_ff is factored directly (cached per frozen set).
+Vec Body::pd_lhs_solve(const Vec& rhs, const std::vector<int>& frozen) {
+    Vec x(dofs, 0.0);
+    if (frozen.empty()) {
+        for (int d = 0; d < 3; ++d) {
+            Vec b(nv);
+            for (int i = 0; i < nv; ++i) b[i] = rhs[3 * i + d];
+            const Vec y = solve_axis(d, b);
+            for (int i = 0; i < nv; ++i) x[3 * i + d] = y[i];
+        }
+        return x;
+    }
+    if (!lhs_valid || lhs_frozen != frozen) {
+        std::vector<char> fz(nv, 0);
+        for (int i : frozen) fz[i] = 1;
+        for (int d = 0; d < 3; ++d) {
+            std::vector<int> Ap(nv + 1, 0), Ai;
+            std::vector<double> Ax;
+            std::vector<std::vector<int>> adj(nv);
+            for (int i = 0; i < nv; ++i) {
+                if (fz[i]) { Ai.push_back(i); Ax.push_back(1.0); }
+                else
+                    for (int p = csr_p[d][i]; p < csr_p[d][i + 1]; ++p) {
+                        const int j = csr_i[d][p];
+                        if (fz[j]) continue;
+                        Ai.push_back(j);
+                        Ax.push_back(csr_x[d][p]);
+                        if (j != i) adj[i].push_back(j);
+                    }
+                Ap[i + 1] = (int)Ai.size();
+            }
+            std::vector<int> all(nv), perm, side(nv, 0);
+            std::iota(all.begin(), all.end(), 0);
+            rcb_order(all, X, adj, perm, side);
+            lhs_ldlt[d].factor(nv, Ap, Ai, Ax, perm);
+        }
+        lhs_frozen = frozen;
+        lhs_valid = true;
+    }
+    std::vector<char> fz(nv, 0);
+    for (int i : frozen) fz[i] = 1;
+    for (int d = 0; d < 3; ++d) {
+        Vec b(nv), y(nv);
+        for (int i = 0; i < nv; ++i) b[i] = fz[i] ? 0.0 : rhs[3 * i + d];
+        lhs_ldlt[d].solve(b.data(), y.data());
+        for (int i = 0; i < nv; ++i) x[3 * i + d] = fz[i] ? rhs[3 * i + d] : y[i];
+    }
+    return x;
+}
+
+// PdNonlinearSolve (forward.cpp:294-526): minimizes
+// 1/2 (q - rhs)^T (M/h^2)(q - rhs) + E_pd(q) + E_act(q) over the free DOFs with
+// L-BFGS (initial inverse Hessian PdLhsSolve) and a backtracking Armijo line
+// search, or with plain local/global PD steps when use_bfgs is 0.
+Vec Body::pd_nonlinear_solve(const Vec& q_init, double inv_h2m, const Vec& rhs, const std::map<int, double>& extra,
+                             const std::map<std::string, double>& opt, int* iters) {
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol");
+    const double rel_tol = opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    const bool use_bfgs = opt_req(opt, "use_bfgs") != 0.0;
+    check(max_pd_iter > 0, "Invalid max_pd_iter: " + std::to_string(max_pd_iter));
+    int hist = 0, max_ls = 0;
+    if (use_bfgs) {
+        hist = (int)opt_req(opt, "bfgs_history_size");
+        check(hist >= 1, "Invalid bfgs_history_size.");
+        max_ls = (int)opt_req(opt, "max_ls_iter");
+        check(max_ls > 0, "Invalid max_ls_iter: " + std::to_string(max_ls));
+    }
+    std::map<int, double> aug = dirichlet;
+    for (auto& kv : extra) aug[kv.first] = kv.second;
+    // frozen nodes of PdLhsSolve: the extra conditions come in whole vertices
+    std::vector<int> frozen;
+    {
+        std::set<int> fn;
+        for (auto& kv : extra) fn.insert(kv.first / 3);
+        frozen.assign(fn.begin(), fn.end());
+    }
+    Vec q_sol = q_init, sel(dofs, 1.0);
+    for (auto& kv : aug) { q_sol[kv.first] = kv.second; sel[kv.first] = 0.0; }
+    auto eval_obj = [&](const Vec& qc, double ec) {
+        double t = 0;
+        for (int i = 0; i < dofs; ++i) t += (qc[i] - rhs[i]) * inv_h2m * (qc[i] - rhs[i]);
+        return 0.5 * t + ec;
+    };
+    Vec force = baseline_force(q_sol);
+    double energy = baseline_energy(q_sol);
+    double obj = eval_obj(q_sol, energy);
+    Vec grad(dofs);
+    for (int i = 0; i < dofs; ++i) grad[i] = (inv_h2m * (q_sol[i] - rhs[i]) - force[i]) * sel[i];
+    double rhs_norm = 0;
+    for (int i = 0; i < dofs; ++i) rhs_norm += (sel[i] * inv_h2m * rhs[i]) * (sel[i] * inv_h2m * rhs[i]);
+    rhs_norm = std::sqrt(rhs_norm);
+    std::deque<Vec> xh, gh, sh, yh;
+    for (int it = 0; it < max_pd_iter; ++it) {
+        if (use_bfgs) {
+            Vec dir;
+            const int bsize = (int)xh.size();
+            if (bsize == 0) {
+                xh.push_back(q_sol);
+                gh.push_back(grad);
+                dir = pd_lhs_solve(grad, frozen);
+            } else {
+                const Vec ql = xh.back(), gl = gh.back();
+                xh.push_back(q_sol);
+                gh.push_back(grad);
+                Vec si(dofs), yi(dofs);
+                for (int i = 0; i < dofs; ++i) { si[i] = q_sol[i] - ql[i]; yi[i] = grad[i] - gl[i]; }
+                sh.push_back(si);
+                yh.push_back(yi);
+                if (bsize == hist + 1) { xh.pop_front(); gh.pop_front(); sh.pop_front(); yh.pop_front(); }
+                Vec qv = grad;
+                std::deque<double> rho, alpha;
+                for (int k = (int)sh.size() - 1; k >= 0; --k) {
+                    const double r = 1 / dot(yh[k], sh[k]);
+                    const double a = r * dot(sh[k], qv);
+                    rho.push_front(r);
+                    alpha.push_front(a);
+                    for (int i = 0; i < dofs; ++i) qv[i] -= a * yh[k][i];
+                }
+                Vec z = pd_lhs_solve(qv, frozen);
+                for (int k = 0; k < (int)sh.size(); ++k) {
+                    const double beta = rho[k] * dot(yh[k], z);
+                    for (int i = 0; i < dofs; ++i) z[i] += sh[k][i] * (alpha[k] - beta);
+                }
+                dir = z;
+            }
+            if (dot(dir, grad) <= 0)
+                for (int i = 0; i < dofs; ++i) dir[i] = grad[i] * sel[i];
+            double step = 1;
+            Vec qn(dofs);
+            for (int i = 0; i < dofs; ++i) qn[i] = q_sol[i] - step * dir[i];
+            double en = baseline_energy(qn), on = eval_obj(qn, en);
+            const double gamma = 1e-4, gd = dot(grad, dir);
+            for (int j = 0; j < max_ls; ++j) {
+                const double cond = obj - gamma * step * gd;
+                if (!std::isnan(on) && on < cond + std::numeric_limits<double>::epsilon()) break;
+                step /= 2;
+                for (int i = 0; i < dofs; ++i) qn[i] = q_sol[i] - step * dir[i];
+                en = baseline_energy(qn);
+                on = eval_obj(qn, en);
+            }
+            q_sol = qn;
+            energy = en;
+            obj = on;
+        } else {
+            // local step with the augmented conditions, then the global step
+            std::map<int, double> saved = dirichlet;
+            dirichlet = aug;
+            Vec pd = local_step(q_sol);
+            dirichlet = saved;
+            Vec b(dofs);
+            for (int i = 0; i < dofs; ++i) b[i] = (inv_h2m * rhs[i] + pd[i]) * sel[i];
+            q_sol = pd_lhs_solve(b, frozen);
+            for (auto& kv : aug) q_sol[kv.first] = kv.second;
+        }
+        force = baseline_force(q_sol);
+        for (int i = 0; i < dofs; ++i) grad[i] = (inv_h2m * (q_sol[i] - rhs[i]) - force[i]) * sel[i];
+        if (norm(grad) <= rel_tol * rhs_norm + abs_tol) {
+            if (iters) *iters = it + 1;
+            return q_sol;
+        }
+    }
+    check(false, "PD method fails to converge.");
+    return Vec(dofs, 0.0);
+}
+
+// SameSet / ProposeNewSet (common.cpp:173-216): keep a /\ b plus the first half
+// (in order) of each one's own part; false when both own parts have at most one
+// element.
+static bool propose_new_set(const std::set<int>& a, const std::set<int>& b, std::set<int>& out) {
+    std::set<int> inter, ac, bc;
+    for (int e : a) (b.count(e) ? inter : ac).insert(e);
+    for (int e : b) if (!inter.count(e)) bc.insert(e);
+    out = inter;
+    if ((int)ac.size() <= 1 && (int)bc.size() <= 1) return false;
+    int k = 0;
+    for (int e : ac) { if (k == (int)ac.size() / 2) break; out.insert(e); ++k; }
+    k = 0;
+    for (int e : bc) { if (k == (int)bc.size() / 2) break; out.insert(e); ++k; }
+    return true;
+}
+
+// ForwardProjectiveDynamics (forward.cpp:529-664): the active set of contact
+// nodes, each frozen where it is at the frame start, is corrected until no
+// frozen node pulls on its obstacle and no free node penetrates (10 rounds at
+// most, ProposeNewSet against cycles). `active` carries the previous frame's
+// set in and this frame's out.
+void Body::forward_baseline(const std::string& method, const Vec& q, const Vec& v, const Vec& f_ext, double dt,
+                            const std::map<std::string, double>& opt, Vec& q_next, Vec& v_next,
+                            std::vector<int>& active) {
+    opt_req(opt, "max_pd_iter");
+    opt_req(opt, "abs_tol");
+    opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    opt_req(opt, "thread_ct");
+    const bool use_bfgs = opt_req(opt, "use_bfgs") != 0.0;
+    if (use_bfgs) {
+        check(opt_req(opt, "bfgs_history_size") > 1, "Invalid bfgs_history_size.");
+        check(opt_req(opt, "max_ls_iter") > 0, "Invalid max_ls_iter.");
+    }
+    check(boundary.type == kPlanar || boundary.type == kSpherical || boundary.type == 0,
+          "The baseline forward supports the planar and spherical boundaries.");
+    check((int)q.size() == dofs && (int)v.size() == dofs && (int)f_ext.size() == dofs, "Inconsistent state size.");
+    setup(dt);
+    const double h = dt, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    const Vec fs = state_force();
+    Vec rhs(dofs);
+    for (int i = 0; i < dofs; ++i) rhs[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
+    auto finish = [&](const Vec& qs) {
+        q_next = qs;
+        v_next.assign(dofs, 0.0);
+        for (int i = 0; i < dofs; ++i) v_next[i] = (q_next[i] - q[i]) / h;
+    };
+    last_baseline_pd_iters = 0;
+    if (method.size() >= 13 && method.compare(method.size() - 13, 13, "fixed_contact") == 0) {
+        std::map<int, double> extra;
+        for (int idx : active) for (int i = 0; i < 3; ++i) extra[3 * idx + i] = q[3 * idx + i];
+        int it = 0;
+        finish(pd_nonlinear_solve(q, inv_h2m, rhs, extra, opt, &it));
+        last_baseline_pd_iters = it;
+        last_baseline_contact_iters = 1;
+        return;
+    }
+    const int max_contact_iter = 10;
+    std::vector<std::set<int>> history;
+    for (int ci = 0; ci < max_contact_iter; ++ci) {
+        std::map<int, double> extra;
+        for (int idx : active) for (int i = 0; i < 3; ++i) extra[3 * idx + i] = q[3 * idx + i];
+        int it = 0;
+        const Vec q_sol = pd_nonlinear_solve(q, inv_h2m, rhs, extra, opt, &it);
+        last_baseline_pd_iters += it;
+        const Vec f_sol = baseline_force(q_sol);
+        const std::set<int> past(active.begin(), active.end());
+        history.push_back(past);
+        active.clear();
+        bool good = true;
+        for (auto& kv : fb_vertices) {
+            const int node = kv.first;
+            const double* x = &q_sol[3 * node];
+            const double dist = boundary.distance(x);
+            double nn[3];
+            boundary.normal(x, nn);
+            double cf = 0;
+            for (int d = 0; d < 3; ++d) cf += nn[d] * (inv_h2m * (q_sol[3 * node + d] - rhs[3 * node + d]) - f_sol[3 * node + d]);
+            if (past.count(node)) {
+                if (cf >= 0) active.push_back(node);
+                else good = false;
+            } else if (dist < 0) {
+                active.push_back(node);
+                good = false;
+            }
+        }
+        bool early = false;
+        if (ci >= 1) {
+            const std::set<int> cur(active.begin(), active.end());
+            if (history[ci - 1] == cur) {
+                std::set<int> ns;
+                if (propose_new_set(history[ci], cur, ns)) active.assign(ns.begin(), ns.end());
+                else early = true;
+            }
+        }
+        if (good || ci == max_contact_iter - 1 || early) {
+            finish(q_sol);
+            active.assign(past.begin(), past.end());
+            last_baseline_contact_iters = ci + 1;
+            return;
+        }
+    }
+}
+
 }  // namespace orc
 
 // =============================================================================
@@ -2271,6 +2645,27 @@ int orc_actuation_grads(void* hp, double* dl_da, int na, double* dl_dact_w, int 
         orc::check(na == (int)b.last_dl_da.size() && nm == (int)b.last_dl_dact_w.size(), "actuation gradient sizes");
         std::copy(b.last_dl_da.begin(), b.last_dl_da.end(), dl_da);
         std::copy(b.last_dl_dact_w.begin(), b.last_dl_dact_w.end(), dl_dact_w);
+    });
+}
+// The DiffPD baseline forward (a pd_eigen method other than pd_eigen_alg_phd):
+// active_in is the previous frame's active set, active_out this frame's.
+int orc_forward_baseline(void* hp, const char* method, const double* q, const double* v, const double* f_ext, double dt,
+                         const char* const* keys, const double* vals, int nopt, double* q_next, double* v_next,
+                         const int* active_in, int n_in, int* active_out, int* n_out, int cap, int* stats) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        orc::Vec qn, vn;
+        std::vector<int> act(active_in, active_in + n_in);
+        h->body.forward_baseline(method, orc::Vec(q, q + n), orc::Vec(v, v + n), orc::Vec(f_ext, f_ext + n), dt,
+                                 make_opts(keys, vals, nopt), qn, vn, act);
+        std::copy(qn.begin(), qn.end(), q_next);
+        std::copy(vn.begin(), vn.end(), v_next);
+        orc::check((int)act.size() <= cap, "active buffer too small");
+        std::copy(act.begin(), act.end(), active_out);
+        *n_out = (int)act.size();
+        stats[0] = h->body.last_baseline_contact_iters;
+        stats[1] = h->body.last_baseline_pd_iters;
     });
 }
 int orc_dofs(void* hp) { return static_cast<Handle*>(hp)->body.dofs; }

@@ -22,7 +22,7 @@ def oracle_body(spec):
     return scenes.apply(spec, oracle.OracleDeformable(spec["edim"]))
 
 
-def rollout(body, spec, frames, opt=None):
+def rollout(body, spec, frames, opt=None, method="pd_eigen_alg_phd"):
     opt = dict(spec["options"] if opt is None else opt)
     dofs = body.dofs()
     q, v, acts = [np.asarray(spec["q0"], float)], [np.asarray(spec["v0"], float)], [[]]
@@ -30,8 +30,8 @@ def rollout(body, spec, frames, opt=None):
     for i in range(frames):
         if spec.get("plane_schedule") is not None:
             body.SetPressPlaneHeight(spec["plane_schedule"][i])
-        qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), []
-        body.PyForward("pd_eigen_alg_phd", q[-1], v[-1], list(spec.get("act", [])), f, spec["dt"], opt, qn, vn, act)
+        qn, vn, act = StdRealVector(dofs), StdRealVector(dofs), list(acts[-1])
+        body.PyForward(method, q[-1], v[-1], list(spec.get("act", [])), f, spec["dt"], opt, qn, vn, act)
         q.append(np.asarray(qn))
         v.append(np.asarray(vn))
         acts.append(list(act))
