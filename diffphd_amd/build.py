@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphd.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["phd_kernels.hip", "phd_dense.hip", "phd_mlp.hip", "phd_factor.cpp", "phd_solver.cpp", "phd_capi.cpp"]
+SOURCES = ["phd_kernels.hip", "phd_dense.hip", "phd_mlp.hip", "phd_factor.cpp", "phd_solver.cpp", "phd_baseline.cpp", "phd_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PHD_OFFLOAD_ARCH", "gfx950")

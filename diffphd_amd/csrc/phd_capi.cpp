@@ -139,6 +139,25 @@ int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const doubl
                    opts(keys, vals, n), dl_dq, dl_dv, dl_df_ext, dl_dmat_w, dl_dstate_p, dl_dw_elem);
     });
 }
+int phd_step_forward_baseline(phd_ctx ctx, const char* method, const double* q, const double* v, const double* f_ext,
+                              double dt, const char* const* keys, const double* vals, int n, double* q_next,
+                              double* v_next, const int* active_in, int n_in, int* active_out, int* n_active,
+                              int* stats) {
+    return guard(ctx, [&](phd::Body& b) {
+        if (!n_active || *n_active < b.num_boundary_vertices())
+            throw std::runtime_error("active_contact_idx buffer too small: it needs room for every frictional-boundary vertex (" +
+                                     std::to_string(b.num_boundary_vertices()) + ")");
+        std::vector<int> act(active_in, active_in + n_in);
+        b.forward_baseline(method, q, v, f_ext, dt, opts(keys, vals, n), q_next, v_next, act);
+        for (size_t i = 0; i < act.size(); ++i) active_out[i] = act[i];
+        *n_active = (int)act.size();
+        if (stats) {
+            stats[0] = b.last_baseline.contact_iters;
+            stats[1] = b.last_baseline.pd_iters;
+            stats[2] = b.last_baseline.ls_steps;
+        }
+    });
+}
 int phd_step_forward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
                             const char* const* keys, const double* vals, int n, double* q_next, double* v_next,
                             int* active, int* n_active) {

@@ -244,6 +244,14 @@ void launch_kpd_apply(hipStream_t s, const DevMaterials& m, const TW* W, const d
 // g[type][e]: z_e^T (-(vol/s) sum_s A_s^T vec(P_s / stiffness)) per merged energy type
 void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
                   double* g_vol, double* g_nh);
+// the baseline forward's per-element PD + muscle energy and element forces
+void launch_pd_energy_force(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* q,
+                            double* energy, double* elem_out);
+// x += Z[zcol[c]] mu[c] per axis over the nc frozen nodes; dst = src on the nodes
+void launch_frozen_apply(hipStream_t s, int nv, int nc, const double* Z, const int* zcol, const double* mu, double* x);
+void launch_copy_nodes(hipStream_t s, int nc, const int* nodes, const double* src, double* dst);
+// G[i nc + j] = Z[zcol[j]][nodes[i]]
+void launch_gather_z(hipStream_t s, int nc, int nv, const int* nodes, const int* zcol, const double* Z, double* G);
 // per muscle element (m.mw != nullptr): g_da[e] = z^T df/da, g_dw[e] = z^T df/d(act_w) (0 elsewhere)
 void launch_dact(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_da,
                  double* g_dw);

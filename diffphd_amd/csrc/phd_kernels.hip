@@ -1916,6 +1916,152 @@ void launch_dact(hipStream_t s, const DevMaterials& m, const double* q, const do
         hipLaunchKernelGGL(k_dact<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, q, z, vol_over_s, g_da, g_dw);
 }
 
+// -----------------------------------------------------------------------------
+// The DiffPD baseline forward's energy and force (ComputePdEnergy / PdEnergyForce,
+// deformable_pd_energy.cpp:156-240; ActuationEnergy / ActuationForce,
+// deformable_actuation.cpp:28-77): per element
+//   E_e = sum_s kc/2 |F - R|^2 + kv/2 |F - P_vol|^2 + mw/2 (|F m| - a)^2
+//   f_e = -sum_s A_s^T vec(kc (F - R) + kv (F - P_vol) + mw (1 - a/l) F m m^T)
+// with kc, kv the energies' stiffness() times vol/s (the mean for per-element
+// parameters, as the reference's EnergyDensity / StressTensor read it).
+// -----------------------------------------------------------------------------
+template <int EDIM>
+__global__ void __launch_bounds__(128) k_pd_energy_force(DevMaterials m, const double* __restrict__ kc,
+                                                         const double* __restrict__ kv, const double* __restrict__ q,
+                                                         double* __restrict__ energy, double* __restrict__ elem_out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    double x[EDIM][3], acc[EDIM][3];
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) {
+        const int v = m.elems[k * m.ne + e];
+        x[k][0] = q[3 * v]; x[k][1] = q[3 * v + 1]; x[k][2] = q[3 * v + 2];
+        acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+    }
+    const double wc = kc ? kc[e] : 0.0, wv = kv ? kv[e] : 0.0;
+    const double wm = m.mw ? m.mw[e] : 0.0;
+    double mf[3] = {0.0, 0.0, 0.0}, ma = 0.0;
+    if (wm != 0.0) { mf[0] = m.mfib[e]; mf[1] = m.mfib[m.ne + e]; mf[2] = m.mfib[2 * m.ne + e]; ma = m.mact[e]; }
+    double en = 0.0;
+    const double* Ge = m.G + (int64_t)e * m.g_stride;
+    for (int s = 0; s < m.samples; ++s) {
+        const double* G = Ge + s * EDIM * 3;
+        double F[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const double g = G[k * 3 + jj];
+                F[0 + 3 * jj] += x[k][0] * g;
+                F[1 + 3 * jj] += x[k][1] * g;
+                F[2 + 3 * jj] += x[k][2] * g;
+            }
+        double P[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = 0.0;
+        if (wc != 0.0 || wv != 0.0) {
+            double U[9], sg[3], V[9];
+            svd3(F, U, sg, V);
+            if (wc != 0.0) {
+                const double one[3] = {1.0, 1.0, 1.0};
+                double R[9], n2 = 0.0;
+                udv(U, one, V, R);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) { const double d = F[i] - R[i]; n2 += d * d; P[i] += wc * d; }
+                en += 0.5 * wc * n2;
+            }
+            if (wv != 0.0) {
+                double D[3], sd[3], Pv[9], n2 = 0.0;
+                volume_D(sg, D);
+                sd[0] = sg[0] + D[0]; sd[1] = sg[1] + D[1]; sd[2] = sg[2] + D[2];
+                udv(U, sd, V, Pv);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) { const double d = F[i] - Pv[i]; n2 += d * d; P[i] += wv * d; }
+                en += 0.5 * wv * n2;
+            }
+        }
+        if (wm != 0.0) {
+            double Fm[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Fm[i] = F[i] * mf[0] + F[i + 3] * mf[1] + F[i + 6] * mf[2];
+            const double l = sqrt(Fm[0] * Fm[0] + Fm[1] * Fm[1] + Fm[2] * Fm[2]);
+            en += 0.5 * wm * (l - ma) * (l - ma);
+            if (l > DBL_EPSILON) {
+                const double r = wm * (1.0 - ma / l);
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) P[i + 3 * j] += r * Fm[i] * mf[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < EDIM; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                acc[k][i] -= P[i] * G[k * 3] + P[i + 3] * G[k * 3 + 1] + P[i + 6] * G[k * 3 + 2];
+    }
+    energy[e] = en;
+    double* o = elem_out + (int64_t)e * EDIM * 3;
+#pragma unroll
+    for (int k = 0; k < EDIM; ++k) { o[3 * k] = acc[k][0]; o[3 * k + 1] = acc[k][1]; o[3 * k + 2] = acc[k][2]; }
+}
+
+void launch_pd_energy_force(hipStream_t s, const DevMaterials& m, const double* kc, const double* kv, const double* q,
+                            double* energy, double* elem_out) {
+    if (m.edim == 8)
+        hipLaunchKernelGGL(k_pd_energy_force<8>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, kc, kv, q, energy, elem_out);
+    else
+        hipLaunchKernelGGL(k_pd_energy_force<4>, dim3(blocks_for(m.ne, 128)), dim3(128), 0, s, m, kc, kv, q, energy, elem_out);
+}
+
+// PdLhsSolve's frozen nodes (forward.cpp:690-936) on the shared factor: with
+// Z[c] = A^-1 e_c (one column per frozen node, the same on the three axes),
+// x += Z mu per axis (mu: 3 values per frozen node), then x = src on the frozen
+// nodes.
+__global__ void __launch_bounds__(256) k_frozen_apply(int nv, int nc, const double* __restrict__ Z,
+                                                      const int* __restrict__ zcol, const double* __restrict__ mu,
+                                                      double* __restrict__ x) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int c = 0; c < nc; ++c) {
+        const double z = Z[(int64_t)zcol[c] * nv + v];
+        s0 += z * mu[3 * c];
+        s1 += z * mu[3 * c + 1];
+        s2 += z * mu[3 * c + 2];
+    }
+    x[3 * v] += s0;
+    x[3 * v + 1] += s1;
+    x[3 * v + 2] += s2;
+}
+__global__ void k_copy_nodes(int nc, const int* __restrict__ nodes, const double* __restrict__ src, double* __restrict__ dst) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int v = nodes[c];
+    dst[3 * v] = src[3 * v];
+    dst[3 * v + 1] = src[3 * v + 1];
+    dst[3 * v + 2] = src[3 * v + 2];
+}
+// G[i * nc + j] = Z[zcol[j]][nodes[i]]: the frozen block of A^-1
+__global__ void k_gather_z(int nc, int nv, const int* __restrict__ nodes, const int* __restrict__ zcol,
+                           const double* __restrict__ Z, double* __restrict__ G) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc * nc) return;
+    const int i = t / nc, j = t % nc;
+    G[t] = Z[(int64_t)zcol[j] * nv + nodes[i]];
+}
+void launch_gather_z(hipStream_t s, int nc, int nv, const int* nodes, const int* zcol, const double* Z, double* G) {
+    if (nc > 0) hipLaunchKernelGGL(k_gather_z, dim3(blocks_for((int64_t)nc * nc, 256)), dim3(256), 0, s, nc, nv, nodes, zcol, Z, G);
+}
+void launch_frozen_apply(hipStream_t s, int nv, int nc, const double* Z, const int* zcol, const double* mu, double* x) {
+    hipLaunchKernelGGL(k_frozen_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, s, nv, nc, Z, zcol, mu, x);
+}
+void launch_copy_nodes(hipStream_t s, int nc, const int* nodes, const double* src, double* dst) {
+    if (nc > 0) hipLaunchKernelGGL(k_copy_nodes, dim3(blocks_for(nc, 128)), dim3(128), 0, s, nc, nodes, src, dst);
+}
+
 void launch_dmatw(hipStream_t s, const DevMaterials& m, const double* q, const double* z, double vol_over_s, double* g_cor,
                   double* g_vol, double* g_nh) {
     if (m.edim == 8)

@@ -538,6 +538,8 @@ void Body::setup(double dt, int threads) {
     // factor (host FP64) and expansion of S = L^-1 (device FP64, FP32 copy)
     factor_.build(nv_, Ap_, Ai_, Ax_, X_, threads);
     plan_.build(factor_, unit_items_, chunk_w_);
+    zcol_.clear();  // the baseline forward's A^-1 columns belong to the old factor
+    g_frozen_.clear();
     {
         DevBuf<int64_t> dLp;
         DevBuf<int> dLi, dpar, ddep;
@@ -1219,6 +1221,12 @@ void Body::apply_ainv(double dt, const double* b, double* x, int precision, int 
     }
     hip_check(hipStreamSynchronize(stream_), "apply_ainv");
     d_q_.download(x, dofs_);
+}
+
+// x = A^-1 r with the FP64 factor on device vectors (the baseline's global solve).
+void Body::ainv64(const double* r, double* x) {
+    spmv_S<double>(r, w64_, y64_, part64_, V64_.p);
+    spmv_ST<double>(y64_, xpart64_, V64_.p, nullptr, nullptr, x);
 }
 
 // Scratch bytes apply_ainv_noalloc needs: w (3 ldw), y (3n), row-chunk partials,

@@ -192,6 +192,15 @@ public:
                   const double* v_next, const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
                   const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
                   double* dl_dmat_w, double* dl_dstate_p, double* dl_dw_elem);
+    // The DiffPD baseline forward (ForwardProjectiveDynamics with a pd_eigen
+    // method other than pd_eigen_alg_phd, forward.cpp:294-664): `active` carries
+    // the previous frame's active contact set in and this frame's out.
+    void forward_baseline(const std::string& method, const double* q, const double* v, const double* f_ext, double dt,
+                          const std::map<std::string, double>& opt, double* q_next, double* v_next,
+                          std::vector<int>& active);
+    struct BaselineStats {
+        int contact_iters = 0, pd_iters = 0, ls_steps = 0;
+    } last_baseline;
     // x = A^-1 b (AoS 3N) with the FP32 (precision=1) or FP64 (0) factor and `refine` FP64 refinement steps.
     void apply_ainv(double dt, const double* b, double* x, int precision, int refine);
     // The same passes on caller-owned device vectors, every intermediate in the
@@ -428,6 +437,21 @@ private:
                       const std::map<std::string, double>& opt, double* q_next, double* v_next, std::vector<int>& active,
                       const std::vector<int>* replay_active, ReplayFrame* replay_out);
     DevBuf<unsigned char> d_enabled_;  // the replay's contacts among the candidates
+    // baseline forward (phd_baseline.cpp)
+    double baseline_energy(const double* q);
+    void baseline_force(const double* q, double* f);
+    void ainv64(const double* r, double* x);
+    void pd_lhs_solve(const double* r, double* x, const std::vector<int>& frozen);
+    int pd_nonlinear_solve(const double* q_init, double inv_h2m, const double* rhs, const std::vector<int>& frozen,
+                           const std::vector<double>& frozen_vals, const std::map<std::string, double>& opt,
+                           double* q_sol);
+    std::map<int, int> zcol_;                  // frozen node -> column of d_Z_ (A^-1 e_node)
+    DevBuf<double> d_Z_, d_bl_energy_, d_bl_mu_;
+    DevBuf<int> d_bl_zcol_, d_bl_nodes_, d_bl_fix_;
+    DevBuf<double> d_bl_fixv_;
+    std::vector<int> g_frozen_;                // frozen set of the factored G below
+    std::vector<double> g_llt_;                // Cholesky factor of G = Z restricted to the frozen nodes
+    std::vector<std::unique_ptr<DevBuf<double>>> d_bl_;  // work vectors (3N)
 };
 
 }  // namespace phd

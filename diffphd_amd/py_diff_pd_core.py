@@ -200,6 +200,8 @@ class _Deformable:
     # --- PyForward / PyBackward (deformable.cpp:157-187) ---------------------
     def PyForward(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
         if method not in METHODS:
+            if str(method).startswith("pd_eigen"):
+                return self._forward_baseline(method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx)
             raise RuntimeError("Unsupported forward method: " + str(method))
         n = self._dofs
         qa, qp = _lib.dptr(q)
@@ -222,10 +224,40 @@ class _Deformable:
         active_contact_idx[:] = [int(x) for x in act[: na.value]]
         _last_ctx[0] = self
 
+    def _forward_baseline(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
+        """The DiffPD baseline forward (ForwardProjectiveDynamics with a pd_eigen
+        method, deformable_projective_dynamics_forward.cpp:529-664):
+        active_contact_idx holds the previous frame's active set on entry, this
+        frame's on exit."""
+        n = self._dofs
+        qa, qp = _lib.dptr(q)
+        va, vp = _lib.dptr(v)
+        fa, fp = _lib.dptr(f_ext)
+        if qa.size != n or va.size != n or fa.size != n:
+            raise RuntimeError("Inconsistent state size.")
+        qn, qnp = _lib.dptr(np.zeros(n))
+        vn, vnp = _lib.dptr(np.zeros(n))
+        ain, aip = _lib.iptr(list(active_contact_idx) or [0])
+        cap = n // 3 + 1
+        aout, aop = _lib.iptr(np.zeros(cap))
+        na = ctypes.c_int(cap)
+        st, stp = _lib.iptr(np.zeros(3))
+        karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a)
+        self._ok(_lib.lib().phd_step_forward_baseline(self._ctx, str(method).encode(), qp, vp, fp, float(dt), karr, valp,
+                                                      nk, qnp, vnp, aip, len(active_contact_idx), aop, ctypes.byref(na),
+                                                      stp))
+        q_next[:] = qn.tolist()
+        v_next[:] = vn.tolist()
+        active_contact_idx[:] = [int(x) for x in aout[: na.value]]
+        self.last_baseline_stats = {"contact_iters": int(st[0]), "pd_iters": int(st[1]), "ls_steps": int(st[2])}
+        _last_ctx[0] = self
+
     def PyBackward(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
                    options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p, dl_dw_elem=None):
         if method not in METHODS:
-            raise RuntimeError("Unsupported backward method: " + str(method))
+            raise RuntimeError("Unsupported backward method: " + str(method) + (
+                " (the baseline's backward is not built)" if str(method).startswith("pd_eigen") else ""))
         n = self._dofs
         ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next)]
         act, ap = _lib.iptr(list(active_contact_idx) or [0])

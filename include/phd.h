@@ -121,6 +121,17 @@ int phd_step_backward(phd_ctx ctx, const double* q, const double* v, const doubl
                       double* dl_dw_elem);
 
 /* Sizes. */
+/* ForwardProjectiveDynamics with a pd_eigen method other than pd_eigen_alg_phd
+ * (deformable_projective_dynamics_forward.cpp:294-664): the DiffPD baseline
+ * (L-BFGS PD, PdNonlinearSolve, inside the active-set contact loop; method
+ * "..._fixed_contact" keeps active_in frozen). active_in holds the previous
+ * frame's active contact set (the reference's in/out argument), active_out this
+ * frame's; stats (3 ints, may be NULL): contact rounds, PD iterations, line-search
+ * halvings. Host arrays; FP64; corotated, volume and muscle energies. */
+int phd_step_forward_baseline(phd_ctx ctx, const char* method, const double* q, const double* v, const double* f_ext,
+                              double dt, const char* const* keys, const double* vals, int n, double* q_next,
+                              double* v_next, const int* active_in, int n_in, int* active_out, int* n_active,
+                              int* stats);
 /* phd_step_forward / phd_step_backward with the state vectors (q, v, f_ext,
  * q_next, v_next, dl_dq_next, dl_dv_next, dl_dq, dl_dv, dl_df_ext) in device
  * memory of the body's GPU, e.g. the data pointers of torch tensors handed over
