@@ -57,6 +57,8 @@ def lib():
         L.orc_act_dofs.argtypes = [P]
         L.orc_num_muscles.argtypes = [P]
         L.orc_actuation_grads.argtypes = [P, dp, ctypes.c_int, dp, ctypes.c_int]
+        L.orc_backward_baseline.argtypes = [P, dp, dp, dp, ctypes.c_double, dp, ip, ctypes.c_int, dp, dp,
+                                            ctypes.POINTER(cp), dp, ctypes.c_int, dp, dp, dp, dp, dp, ip]
         L.orc_forward_baseline.argtypes = [P, cp, dp, dp, dp, ctypes.c_double, ctypes.POINTER(cp), dp, ctypes.c_int,
                                            dp, dp, ip, ctypes.c_int, ip, ip, ctypes.c_int, ip]
         for f in ("orc_dofs", "orc_num_energies", "orc_num_elements", "orc_last_backward_iters"):
@@ -241,6 +243,11 @@ class OracleDeformable:
 
     def PyBackward(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
                    options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p, dl_dw_elem=None):
+        if method not in ("pd_eigen_alg_phd", "pd_eigen_pcg_proj_gpu"):
+            if not method.startswith("pd_eigen"):
+                raise RuntimeError("Unsupported backward method: " + str(method))
+            return self._backward_baseline(q, v, a, f_ext, dt, q_next, active_contact_idx, dl_dq_next, dl_dv_next,
+                                           options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p)
         n = self._dofs
         ne = lib().orc_num_energies(self._h)
         nel = lib().orc_num_elements(self._h)
@@ -265,6 +272,34 @@ class OracleDeformable:
         dl_dstate_p[:] = list(outs[4][0])
         if dl_dw_elem is not None:
             dl_dw_elem[:] = list(outs[5][0][: ne * nel])
+
+    def _backward_baseline(self, q, v, a, f_ext, dt, q_next, active_contact_idx, dl_dq_next, dl_dv_next, options,
+                           dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p):
+        """BackwardProjectiveDynamics for a pd_eigen baseline method
+        (deformable_projective_dynamics_backward.cpp:141-679)."""
+        n = self._dofs
+        ne = lib().orc_num_energies(self._h)
+        arrs = [_d(x) for x in (q, v, f_ext, q_next, dl_dq_next, dl_dv_next)]
+        act, ap = _i(list(active_contact_idx) or [0])
+        outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), 3)]
+        st, stp = _i(np.zeros(2))
+        karr, vals, valp, nk = _opts(options)
+        self._set_actuation(a)
+        self._ok(lib().orc_backward_baseline(self._h, arrs[0][1], arrs[1][1], arrs[2][1], float(dt), arrs[3][1], ap,
+                                             len(active_contact_idx), arrs[4][1], arrs[5][1], karr, valp, nk,
+                                             *[o[1] for o in outs], stp))
+        na, nm = self.act_dofs(), self.NumOfPdMuscleEnergies()
+        ga, gap = _d(np.zeros(max(na, 1)))
+        gw, gwp = _d(np.zeros(max(nm, 1)))
+        self._ok(lib().orc_actuation_grads(self._h, gap, na, gwp, nm))
+        dl_dq[:] = list(outs[0][0])
+        dl_dv[:] = list(outs[1][0])
+        dl_da[:] = list(ga[:na])
+        dl_df_ext[:] = list(outs[2][0])
+        dl_dmat_w[:] = list(outs[3][0][:ne])
+        dl_dact_w[:] = list(gw[:nm])
+        dl_dstate_p[:] = list(outs[4][0])
+        self.last_baseline_bwd_stats = {"iters": int(st[0]), "newton_fallback": int(st[1])}
 
     # --- test-only helpers -------------------------------------------------
     def apply_ainv(self, dt, b):

@@ -1406,6 +1406,13 @@ struct Body {
     void forward_baseline(const std::string& method, const Vec& q, const Vec& v, const Vec& f_ext, double dt,
                           const std::map<std::string, double>& opt, Vec& q_next, Vec& v_next, std::vector<int>& active);
     int last_baseline_contact_iters = 0, last_baseline_pd_iters = 0;
+    Vec pd_lhs_op(const Vec& x, const std::vector<int>& frozen) const;
+    Vec pd_force_differential(const Vec& q, const Vec& dq) const;
+    void backward_baseline(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next,
+                           const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
+                           const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext,
+                           Vec& dl_dmat_w, Vec& dl_dstate_p);
+    int last_baseline_bwd_iters = 0, last_baseline_bwd_newton = 0;
     AlgPhdReplayCache replay_fixed_active_set(const Vec& q, const Vec& v, const Vec& f_ext, double dt,
                                               const std::vector<int>& active, const std::map<std::string, double>& opt);
     void backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next, const Vec& v_next,
@@ -2467,6 +2474,301 @@ Vec Body::pd_nonlinear_solve(const Vec& q_init, double inv_h2m, const Vec& rhs, 
     return Vec(dofs, 0.0);
 }
 
+// PdEnergyForceDifferential(q, dq, 0) (deformable_pd_energy.cpp:243-300) +
+// ActuationForceDifferential(q, a, dq, 0, 0) (deformable_actuation.cpp:79-114):
+// df = -(vol/s) sum A^T vec(dP), dP = k (dF - dp/dF[dF]) with each energy's
+// stiffness(), and for a muscle k (dr F + r dF) m m^T with r = 1 - a/l,
+// dr = a (n . dF m) / l^2 (StressTensorDifferential, pd_muscle_energy.cpp:47-64).
+Vec Body::pd_force_differential(const Vec& q, const Vec& dq) const {
+    std::vector<double> acc((size_t)ne * edim * 3, 0.0);
+    for (const auto& en : energies) {
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            if (!en.member[e]) continue;
+            for (int s = 0; s < samples; ++s) {
+                const M3 F = deformation_gradient(q.data(), e, s);
+                const M3 dF = deformation_gradient(dq.data(), e, s);
+                FAux a;
+                a.init(F);
+                const M3 dP = scale(add(dF, scale(project_diff(en, a, dF), -1.0)), en.stiffness);
+                add_At(dP, e, s, -element_volume / samples, &acc[(size_t)e * edim * 3]);
+            }
+        }
+    }
+    int base = 0;
+    for (const Muscle& mu : muscles) {
+        for (int ei = 0; ei < (int)mu.elems.size(); ++ei) {
+            const int e = mu.elems[ei];
+            const double a = act[base + ei];
+            for (int s = 0; s < samples; ++s) {
+                double Fm[3], l, dFm[3], dl_unused;
+                muscle_fm(deformation_gradient(q.data(), e, s), mu.m, Fm, l);
+                muscle_fm(deformation_gradient(dq.data(), e, s), mu.m, dFm, dl_unused);
+                if (l <= std::numeric_limits<double>::epsilon()) continue;
+                const double dl = (Fm[0] * dFm[0] + Fm[1] * dFm[1] + Fm[2] * dFm[2]) / l;
+                const double r = 1 - a / l, dr = a * dl / (l * l);
+                M3 dP;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) dP(i, j) = mu.stiffness * (dr * Fm[i] + r * dFm[i]) * mu.m[j];
+                add_At(dP, e, s, -element_volume / samples, &acc[(size_t)e * edim * 3]);
+            }
+        }
+        base += (int)mu.elems.size();
+    }
+    Vec f(dofs, 0.0);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < edim; ++k)
+            for (int i = 0; i < 3; ++i) f[3 * elems[(size_t)e * edim + k] + i] += acc[((size_t)e * edim + k) * 3 + i];
+    return f;
+}
+
+// PdLhsMatrixOp (forward.cpp:666-688): A x per axis with the frozen nodes'
+// entries of x zeroed, then x itself on the frozen rows.
+Vec Body::pd_lhs_op(const Vec& x, const std::vector<int>& frozen) const {
+    std::vector<char> fz(nv, 0);
+    for (int i : frozen) fz[i] = 1;
+    Vec out(dofs, 0.0);
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < nv; ++i) {
+            double acc = 0;
+            for (int p = csr_p[d][i]; p < csr_p[d][i + 1]; ++p) {
+                const int j = csr_i[d][p];
+                if (!fz[j]) acc += csr_x[d][p] * x[3 * j + d];
+            }
+            out[3 * i + d] = fz[i] ? x[3 * i + d] : acc;
+        }
+    return out;
+}
+
+// BackwardProjectiveDynamics for a pd_eigen baseline method
+// (deformable_projective_dynamics_backward.cpp:141-679), its dense-solver path
+// (material 'none', no MAS): L-BFGS on 1/2 x^T S x - b^T x with
+// S = PdLhsMatrixOp - K_pd (mean-stiffness K_pd, backward.cpp:10-80) over the
+// free DOFs, b = dl/dq_next + dl/dv_next / h; when it meets an indefinite
+// direction or the iteration cap, the Newton system is solved instead (dense
+// LDL^T of S here, which is the Newton matrix for energies of uniform
+// stiffness). Then the fixed DOFs' share through NewtonMatrixOp, the actuation
+// and material gradients with the adjoint, and the chain back through
+// rhs = q + h v + h^2/m (f_ext + f_state).
+void Body::backward_baseline(const Vec& q, const Vec& v, const Vec& f_ext, double dt, const Vec& q_next,
+                             const std::vector<int>& active, const Vec& dl_dq_next, const Vec& dl_dv_next,
+                             const std::map<std::string, double>& opt, Vec& dl_dq, Vec& dl_dv, Vec& dl_df_ext,
+                             Vec& dl_dmat_w, Vec& dl_dstate_p) {
+    (void)v; (void)f_ext;
+    const int max_pd_iter = (int)opt_req(opt, "max_pd_iter");
+    const double abs_tol = opt_req(opt, "abs_tol"), rel_tol = opt_req(opt, "rel_tol");
+    opt_req(opt, "verbose");
+    opt_req(opt, "thread_ct");
+    const bool use_bfgs = opt_req(opt, "use_bfgs") != 0.0;
+    check(max_pd_iter > 0, "Invalid max_pd_iter: " + std::to_string(max_pd_iter));
+    int hist = 0, max_ls = 0;
+    if (use_bfgs) {
+        hist = (int)opt_req(opt, "bfgs_history_size");
+        check(hist >= 1, "Invalid bfgs_history_size.");
+        max_ls = (int)opt_req(opt, "max_ls_iter");
+        check(max_ls > 0, "Invalid max_ls_iter: " + std::to_string(max_ls));
+    }
+    setup(dt);
+    const int nE = (int)energies.size();
+    dl_dq.assign(dofs, 0.0);
+    dl_dv.assign(dofs, 0.0);
+    dl_df_ext.assign(dofs, 0.0);
+    dl_dmat_w.assign(nE, 0.0);
+    dl_dstate_p.assign(gravity.size() == 3 ? 3 : 0, 0.0);
+    last_dl_da.assign(act_dofs, 0.0);
+    last_dl_dact_w.assign(muscles.size(), 0.0);
+    const double h = dt, inv_h = 1 / h, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    const std::vector<double> Kpd = local_differential(q_next, false);
+    std::map<int, double> aug = dirichlet;
+    std::set<int> fset(active.begin(), active.end());
+    const std::vector<int> frozen(fset.begin(), fset.end());
+    for (int idx : frozen) for (int i = 0; i < 3; ++i) aug[3 * idx + i] = q[3 * idx + i];
+    Vec agg(dofs), sel(dofs, 1.0);
+    for (int i = 0; i < dofs; ++i) {
+        dl_dq[i] += -dl_dv_next[i] * inv_h;
+        agg[i] = dl_dq_next[i] + dl_dv_next[i] * inv_h;
+    }
+    for (auto& kv : aug) sel[kv.first] = 0.0;
+    auto Sop = [&](const Vec& x) {
+        Vec a = pd_lhs_op(x, frozen);
+        const Vec k = apply_local_differential(Kpd, x);
+        for (int i = 0; i < dofs; ++i) a[i] -= k[i];
+        return a;
+    };
+    Vec x(dofs, 0.0);
+    Vec Sx = Sop(x), grad(dofs);
+    for (int i = 0; i < dofs; ++i) grad[i] = (Sx[i] - agg[i]) * sel[i];
+    double obj = 0.5 * dot(x, Sx) - dot(agg, x);
+    const double bn = norm(agg);
+    bool success = false;
+    std::deque<Vec> xh, gh, sh, yh;
+    int it_used = 0;
+    for (int it = 0; it < max_pd_iter && !success; ++it) {
+        it_used = it + 1;
+        if (use_bfgs) {
+            Vec dir;
+            const int bsize = (int)xh.size();
+            if (bsize == 0) {
+                xh.push_back(x);
+                gh.push_back(grad);
+                dir = pd_lhs_solve(grad, frozen);
+            } else {
+                const Vec xl = xh.back(), gl = gh.back();
+                xh.push_back(x);
+                gh.push_back(grad);
+                Vec si(dofs), yi(dofs);
+                for (int i = 0; i < dofs; ++i) { si[i] = x[i] - xl[i]; yi[i] = grad[i] - gl[i]; }
+                sh.push_back(si);
+                yh.push_back(yi);
+                if (bsize == hist + 1) { xh.pop_front(); gh.pop_front(); sh.pop_front(); yh.pop_front(); }
+                Vec qv = grad;
+                std::deque<double> rho, alpha;
+                for (int k = (int)sh.size() - 1; k >= 0; --k) {
+                    const double r = 1 / dot(yh[k], sh[k]);
+                    const double a = r * dot(sh[k], qv);
+                    rho.push_front(r);
+                    alpha.push_front(a);
+                    for (int i = 0; i < dofs; ++i) qv[i] -= a * yh[k][i];
+                }
+                Vec z = pd_lhs_solve(qv, frozen);
+                for (int k = 0; k < (int)sh.size(); ++k) {
+                    const double beta = rho[k] * dot(yh[k], z);
+                    for (int i = 0; i < dofs; ++i) z[i] += sh[k][i] * (alpha[k] - beta);
+                }
+                dir = z;
+            }
+            for (int i = 0; i < dofs; ++i) dir[i] *= sel[i];
+            const double gd = dot(dir, grad);
+            if (gd < -1e-4) break;  // indefinite: the Newton solve below
+            double step = 1;
+            Vec xn(dofs);
+            for (int i = 0; i < dofs; ++i) xn[i] = x[i] - step * dir[i];
+            Vec Sxn = Sop(xn);
+            double on = 0.5 * dot(xn, Sxn) - dot(agg, xn);
+            for (int j = 0; j < max_ls; ++j) {
+                const double cond = obj - 1e-4 * step * gd;
+                if (!std::isnan(on) && on < cond + std::numeric_limits<double>::epsilon()) break;
+                step /= 2;
+                for (int i = 0; i < dofs; ++i) xn[i] = x[i] - step * dir[i];
+                Sxn = Sop(xn);
+                on = 0.5 * dot(xn, Sxn) - dot(agg, xn);
+            }
+            x = xn;
+            Sx = Sxn;
+            obj = on;
+        } else {
+            const Vec k = apply_local_differential(Kpd, x);
+            Vec b(dofs);
+            for (int i = 0; i < dofs; ++i) b[i] = (agg[i] + k[i]) * sel[i];
+            x = pd_lhs_solve(b, frozen);
+            for (int i = 0; i < dofs; ++i) x[i] *= sel[i];
+            Sx = Sop(x);
+            obj = 0.5 * dot(x, Sx) - dot(agg, x);
+        }
+        for (int i = 0; i < dofs; ++i) grad[i] = (Sx[i] - agg[i]) * sel[i];
+        if (norm(grad) <= rel_tol * bn + abs_tol) {
+            success = true;
+            for (auto& kv : aug) x[kv.first] = agg[kv.first];
+        }
+    }
+    last_baseline_bwd_iters = it_used;
+    last_baseline_bwd_newton = success ? 0 : 1;
+    if (!success) {
+        // Newton system with the augmented rows and columns as identities
+        std::vector<double> M((size_t)dofs * dofs, 0.0);
+        for (int c = 0; c < dofs; ++c) {
+            Vec e(dofs, 0.0);
+            if (sel[c] == 0.0) { M[(size_t)c * dofs + c] = 1.0; continue; }
+            e[c] = 1.0;
+            const Vec col = Sop(e);
+            for (int r = 0; r < dofs; ++r) M[(size_t)r * dofs + c] = sel[r] == 0.0 ? 0.0 : col[r];
+        }
+        x = dense_ldlt_solve(M, dofs, agg);
+    }
+    // equation (13) and the fixed DOFs' share (:614-630)
+    Vec dl_drhs(dofs);
+    for (int i = 0; i < dofs; ++i) dl_drhs[i] = x[i] * inv_h2m;
+    for (auto& kv : aug) dl_drhs[kv.first] = agg[kv.first];
+    Vec adjoint = x;
+    for (auto& kv : aug) adjoint[kv.first] = 0.0;
+    {
+        // NewtonMatrixOp(q_next, a, inv_h2m, {}, -adjoint) = inv_h2m (-adjoint) - df(-adjoint), with
+        // df the PD + muscle force differential (each energy's stiffness(), deformable_pd_energy.cpp:243-300)
+        Vec mv(dofs);
+        for (int i = 0; i < dofs; ++i) mv[i] = -adjoint[i];
+        Vec df = pd_force_differential(q_next, mv);
+        for (auto& kv : aug) dl_drhs[kv.first] += inv_h2m * mv[kv.first] - df[kv.first];
+    }
+    // dl/da, dl/d(act_w) (:632-638) and dl/d(mat_w) (:643-647) with the adjoint
+    {
+        Vec da(act_dofs, 0.0), dw(muscles.size(), 0.0);
+        int base = 0;
+        for (int gi = 0; gi < (int)muscles.size(); ++gi) {
+            const Muscle& mu = muscles[gi];
+            for (int ei = 0; ei < (int)mu.elems.size(); ++ei) {
+                const int e = mu.elems[ei];
+                const double a = act[base + ei];
+                const int* vv = &elems[(size_t)e * edim];
+                double fa[24] = {0}, fw[24] = {0};
+                for (int s = 0; s < samples; ++s) {
+                    double Fm[3], l;
+                    muscle_fm(deformation_gradient(q_next.data(), e, s), mu.m, Fm, l);
+                    if (l <= std::numeric_limits<double>::epsilon()) continue;
+                    M3 dPda, Pw;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            dPda(i, j) = -mu.stiffness / l * Fm[i] * mu.m[j];
+                            Pw(i, j) = (1 - a / l) * Fm[i] * mu.m[j];
+                        }
+                    add_At(dPda, e, s, -element_volume / samples, fa);
+                    add_At(Pw, e, s, -element_volume / samples, fw);
+                }
+                for (int k = 0; k < edim; ++k)
+                    for (int i = 0; i < 3; ++i) {
+                        da[base + ei] += adjoint[3 * vv[k] + i] * fa[3 * k + i];
+                        dw[gi] += adjoint[3 * vv[k] + i] * fw[3 * k + i];
+                    }
+            }
+            base += (int)mu.elems.size();
+        }
+        last_dl_da = da;
+        last_dl_dact_w = dw;
+        for (int ei = 0; ei < nE; ++ei) {
+            const Energy& en = energies[ei];
+            const double inv_w = 1 / en.stiffness;
+            for (int e = 0; e < ne; ++e) {
+                if (!en.member[e]) continue;
+                const int* vv = &elems[(size_t)e * edim];
+                double acc[24] = {0};
+                for (int s = 0; s < samples; ++s) {
+                    const M3 F = deformation_gradient(q_next.data(), e, s);
+                    FAux a;
+                    a.init(F);
+                    const M3 p = project(en, a);
+                    add_At(stress(en, a, p), e, s, -inv_w * element_volume / samples, acc);
+                }
+                for (int k = 0; k < edim; ++k)
+                    for (int i = 0; i < 3; ++i) dl_dmat_w[ei] += adjoint[3 * vv[k] + i] * acc[3 * k + i];
+            }
+        }
+    }
+    // rhs = q (frozen nodes) / dirichlet values / q + h v + h2m (f_ext + f_state)
+    Vec basic = dl_drhs;
+    for (int idx : frozen)
+        for (int i = 0; i < 3; ++i) {
+            basic[3 * idx + i] = 0;
+            dl_dq[3 * idx + i] += dl_drhs[3 * idx + i];
+        }
+    for (auto& kv : dirichlet) basic[kv.first] = 0;
+    for (int i = 0; i < dofs; ++i) {
+        dl_dq[i] += basic[i];
+        dl_dv[i] += basic[i] * h;
+        dl_df_ext[i] += basic[i] * h2m;
+    }
+    if (gravity.size() == 3)
+        for (int i = 0; i < dofs; ++i) dl_dstate_p[i % 3] += basic[i] * h2m * mass;
+}
+
 // SameSet / ProposeNewSet (common.cpp:173-216): keep a /\ b plus the first half
 // (in order) of each one's own part; false when both own parts have at most one
 // element.
@@ -2666,6 +2968,27 @@ int orc_forward_baseline(void* hp, const char* method, const double* q, const do
         *n_out = (int)act.size();
         stats[0] = h->body.last_baseline_contact_iters;
         stats[1] = h->body.last_baseline_pd_iters;
+    });
+}
+int orc_backward_baseline(void* hp, const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                          const int* active, int n_active, const double* dl_dq_next, const double* dl_dv_next,
+                          const char* const* keys, const double* vals, int nopt, double* dl_dq, double* dl_dv,
+                          double* dl_df_ext, double* dl_dmat_w, double* dl_dstate_p, int* stats) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        orc::Vec a, b, c, d, e;
+        h->body.backward_baseline(orc::Vec(q, q + n), orc::Vec(v, v + n), orc::Vec(f_ext, f_ext + n), dt,
+                                  orc::Vec(q_next, q_next + n), std::vector<int>(active, active + n_active),
+                                  orc::Vec(dl_dq_next, dl_dq_next + n), orc::Vec(dl_dv_next, dl_dv_next + n),
+                                  make_opts(keys, vals, nopt), a, b, c, d, e);
+        std::copy(a.begin(), a.end(), dl_dq);
+        std::copy(b.begin(), b.end(), dl_dv);
+        std::copy(c.begin(), c.end(), dl_df_ext);
+        std::copy(d.begin(), d.end(), dl_dmat_w);
+        std::copy(e.begin(), e.end(), dl_dstate_p);
+        stats[0] = h->body.last_baseline_bwd_iters;
+        stats[1] = h->body.last_baseline_bwd_newton;
     });
 }
 int orc_dofs(void* hp) { return static_cast<Handle*>(hp)->body.dofs; }
