@@ -37,6 +37,8 @@ SIGNATURES = [
                                          ctypes.POINTER(_cp), _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("phd_step_forward_baseline", ctypes.c_int, [_P, _cp, _dp, _dp, _dp, ctypes.c_double, ctypes.POINTER(_cp), _dp,
                                                  ctypes.c_int, _dp, _dp, _ip, ctypes.c_int, _ip, _ip, _ip]),
+    ("phd_step_backward_baseline", ctypes.c_int, [_P, _dp, _dp, _dp, ctypes.c_double, _dp, _ip, ctypes.c_int, _dp, _dp,
+                                                  ctypes.POINTER(_cp), _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _ip]),
     ("phd_step_forward_device", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, ctypes.POINTER(_cp), _dp, ctypes.c_int,
                                                _P, _P, _ip, _ip]),
     ("phd_step_backward_device", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, _P, _P, _ip, ctypes.c_int, _P, _P,

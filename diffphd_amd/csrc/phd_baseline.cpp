@@ -409,3 +409,253 @@ void Body::forward_baseline(const std::string& method, const double* q, const do
 }
 
 }  // namespace phd
+
+namespace phd {
+
+// BackwardProjectiveDynamics for a pd_eigen baseline method
+// (deformable_projective_dynamics_backward.cpp:141-679), the path the reference
+// takes for bodies without an elastic material: L-BFGS on 1/2 x^T S x - b^T x,
+// S = PdLhsMatrixOp - K_pd (each energy's stiffness() in K_pd, backward.cpp:10-80),
+// over the free DOFs with b = dl/dq_next + dl/dv_next / h; then
+// dl/drhs = x M/h^2 on the free DOFs, b + NewtonMatrixOp(-x) on the fixed ones
+// (for energies of uniform stiffness NewtonMatrixOp = A - K_pd), the actuation and
+// material gradients with the adjoint x, and the chain back through the frozen
+// nodes (rhs = q), the Dirichlet values and rhs = q + h v + h^2/m (f_ext + f_state).
+// The reference falls back to a sparse Cholesky of the Newton matrix when L-BFGS
+// meets an indefinite direction or its cap; that fallback is not built here.
+void Body::backward_baseline(const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                             const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
+                             const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                             double* dl_dmat_w, double* dl_dstate_p) {
+    (void)v; (void)f_ext;
+    const int max_pd_iter = (int)req(opt, "max_pd_iter");
+    const double abs_tol = req(opt, "abs_tol"), rel_tol = req(opt, "rel_tol");
+    req(opt, "verbose");
+    const int threads = (int)req(opt, "thread_ct");
+    const bool use_bfgs = req(opt, "use_bfgs") != 0.0;
+    require(max_pd_iter > 0, "Invalid max_pd_iter: " + std::to_string(max_pd_iter));
+    int hist = 0, max_ls = 0;
+    if (use_bfgs) {
+        hist = (int)req(opt, "bfgs_history_size");
+        require(hist >= 1, "Invalid bfgs_history_size.");
+        max_ls = (int)req(opt, "max_ls_iter");
+        require(max_ls > 0, "Invalid max_ls_iter: " + std::to_string(max_ls));
+    }
+    require(std::none_of(energies_.begin(), energies_.end(), [](const Energy& e) { return e.type == kNeoHookean; }),
+            "The baseline backward supports the corotated and volume energies.");
+    require(std::all_of(energies_.begin(), energies_.end(), [](const Energy& e) { return e.elem_stiff.empty(); }),
+            "The baseline backward needs energies of uniform stiffness (per-element parameters: use one energy per "
+            "element group).");
+    require((int)act_.size() == act_dofs_, "Inconsistent actuation size.");
+    setup(dt, threads);
+    require(pverts_.empty(), "The baseline backward supports whole-vertex Dirichlet conditions only.");
+    if (kpd_exact_) { kpd_exact_ = false; build_materials(); }
+    const int n3 = dofs_, nE = (int)energies_.size();
+    const double h = dt, inv_h = 1 / h, mass = element_volume_ * density_, h2m = h * h / mass, inv_h2m = mass / (h * h);
+    // K_pd at q_next (unfiltered, FP64)
+    d_q_.upload(q_next, n3);
+    d_W64_.alloc((size_t)ne_ * samples_ * 81);
+    d_nfilt_.alloc(1);
+    const double* kn[4] = {d_kn_[0].p, d_kn_[1].p, d_kn_[2].p, d_kn_[3].p};
+    launch_kpd_build<double>(stream_, mat_, has_cor_ ? d_kc_.p : nullptr, has_vol_ ? d_kv_.p : nullptr, kn, d_q_.p, 0, 1.0,
+                             -INFINITY, d_nfilt_.p, d_W64_.p);
+    // augmented conditions: the body's Dirichlet DOFs and the frozen contact nodes (at q)
+    std::set<int> fset(active.begin(), active.end());
+    const std::vector<int> frozen(fset.begin(), fset.end());
+    std::vector<char> fixed(n3, 0);
+    for (auto& kv : dirichlet_) fixed[kv.first] = 1;
+    for (int v0 : frozen) for (int d = 0; d < 3; ++d) fixed[3 * (size_t)v0 + d] = 1;
+    std::vector<int> fix;
+    for (int i = 0; i < n3; ++i) if (fixed[i]) fix.push_back(i);
+    const int nfix = (int)fix.size();
+    d_bl_fix_.upload(fix);
+    d_bl_nodes_.upload(frozen);
+    std::vector<double> agg(n3);
+    for (int i = 0; i < n3; ++i) {
+        dl_dq[i] = -dl_dv_next[i] * inv_h;
+        dl_dv[i] = 0;
+        dl_df_ext[i] = 0;
+        agg[i] = dl_dq_next[i] + dl_dv_next[i] * inv_h;
+    }
+    auto vec = [&](size_t i) -> double* {
+        while (d_bl_.size() <= i) d_bl_.emplace_back(new DevBuf<double>());
+        d_bl_[i]->alloc(n3);
+        return d_bl_[i]->p;
+    };
+    double *b = vec(0), *x = vec(1), *Sx = vec(2), *grad = vec(3), *dir = vec(4), *xn = vec(5), *Sxn = vec(6),
+           *tmp = vec(7), *xl = vec(8), *gl = vec(9), *kx = vec(10), *zero = vec(11);
+    hip_check(hipMemcpy(b, agg.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "b");
+    hip_check(hipMemsetAsync(zero, 0, sizeof(double) * n3, stream_), "zero");
+    hip_check(hipMemsetAsync(x, 0, sizeof(double) * n3, stream_), "x = 0");
+    // S x = PdLhsMatrixOp(x) - K_pd x: A x with the frozen entries of x zeroed,
+    // x itself on the frozen rows
+    auto apply_S = [&](const double* xin, double* out) {
+        hip_check(hipMemcpyAsync(tmp, xin, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "S x");
+        launch_zero_dirichlet(stream_, nfix, d_bl_fix_.p, tmp);  // the body's Dirichlet entries are 0 already
+        launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, zero, tmp, d_dmask_.p, out, nullptr);  // -A x
+        launch_axpbypcz(stream_, n3, -1.0, out, 0.0, nullptr, 0.0, nullptr, out);
+        launch_copy_nodes(stream_, (int)frozen.size(), d_bl_nodes_.p, xin, out);
+        kpd_apply(xin, kx, 0);
+        launch_axpbypcz(stream_, n3, 1.0, out, -1.0, kx, 0.0, nullptr, out);
+    };
+    auto objective = [&](const double* xc, const double* Sxc) {
+        const auto d = dots({{xc, Sxc}, {b, xc}});
+        return 0.5 * d[0] - d[1];
+    };
+    auto gradient = [&](const double* Sxc, double* g) {
+        launch_axpbypcz(stream_, n3, 1.0, Sxc, -1.0, b, 0.0, nullptr, g);
+        launch_zero_dirichlet(stream_, nfix, d_bl_fix_.p, g);
+    };
+    apply_S(x, Sx);
+    gradient(Sx, grad);
+    double obj = objective(x, Sx);
+    const double bn = std::sqrt(dots({{b, b}})[0]);
+    bool success = false;
+    std::deque<int> sh, yh;
+    std::vector<int> free_slots;
+    int next_slot = 12;
+    auto slot = [&]() {
+        if (!free_slots.empty()) { const int k = free_slots.back(); free_slots.pop_back(); return k; }
+        return next_slot++;
+    };
+    bool have_last = false;
+    int it_used = 0;
+    for (int it = 0; it < max_pd_iter && !success; ++it) {
+        it_used = it + 1;
+        if (use_bfgs) {
+            if (!have_last) {
+                pd_lhs_solve(grad, dir, frozen);
+            } else {
+                const int ks = slot(), ky = slot();
+                double* s = vec(ks);
+                double* y = vec(ky);
+                launch_axpbypcz(stream_, n3, 1.0, x, -1.0, xl, 0.0, nullptr, s);
+                launch_axpbypcz(stream_, n3, 1.0, grad, -1.0, gl, 0.0, nullptr, y);
+                sh.push_back(ks);
+                yh.push_back(ky);
+                if ((int)sh.size() > hist) {
+                    free_slots.push_back(sh.front());
+                    free_slots.push_back(yh.front());
+                    sh.pop_front();
+                    yh.pop_front();
+                }
+                double* qv = xn;  // scratch for the two-loop vector
+                launch_axpbypcz(stream_, n3, 1.0, grad, 0.0, nullptr, 0.0, nullptr, qv);
+                const int m = (int)sh.size();
+                std::vector<double> rho(m), alpha(m);
+                for (int k = m - 1; k >= 0; --k) {
+                    const double* sk = d_bl_[sh[k]]->p;
+                    const double* yk = d_bl_[yh[k]]->p;
+                    const auto d = dots({{yk, sk}, {sk, qv}});
+                    rho[k] = 1 / d[0];
+                    alpha[k] = rho[k] * d[1];
+                    launch_axpbypcz(stream_, n3, 1.0, qv, -alpha[k], yk, 0.0, nullptr, qv);
+                }
+                pd_lhs_solve(qv, dir, frozen);
+                for (int k = 0; k < m; ++k) {
+                    const double* sk = d_bl_[sh[k]]->p;
+                    const double* yk = d_bl_[yh[k]]->p;
+                    const double beta = rho[k] * dots({{yk, dir}})[0];
+                    launch_axpbypcz(stream_, n3, 1.0, dir, alpha[k] - beta, sk, 0.0, nullptr, dir);
+                }
+            }
+            hip_check(hipMemcpyAsync(xl, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "x_last");
+            hip_check(hipMemcpyAsync(gl, grad, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "g_last");
+            have_last = true;
+            launch_zero_dirichlet(stream_, nfix, d_bl_fix_.p, dir);
+            const double gd = dots({{dir, grad}})[0];
+            require(gd >= -1e-4, "The baseline backward's L-BFGS met an indefinite operator; the reference's Newton "
+                                 "fallback is not built.");
+            double step = 1;
+            launch_axpbypcz(stream_, n3, 1.0, x, -step, dir, 0.0, nullptr, xn);
+            apply_S(xn, Sxn);
+            double on = objective(xn, Sxn);
+            for (int j = 0; j < max_ls; ++j) {
+                const double cond = obj - 1e-4 * step * gd;
+                if (!std::isnan(on) && on < cond + DBL_EPSILON) break;
+                step /= 2;
+                launch_axpbypcz(stream_, n3, 1.0, x, -step, dir, 0.0, nullptr, xn);
+                apply_S(xn, Sxn);
+                on = objective(xn, Sxn);
+            }
+            std::swap(x, xn);
+            std::swap(Sx, Sxn);
+            obj = on;
+        } else {
+            kpd_apply(x, kx, 0);
+            launch_axpbypcz(stream_, n3, 1.0, b, 1.0, kx, 0.0, nullptr, tmp);
+            launch_zero_dirichlet(stream_, nfix, d_bl_fix_.p, tmp);
+            pd_lhs_solve(tmp, x, frozen);
+            launch_zero_dirichlet(stream_, nfix, d_bl_fix_.p, x);
+            apply_S(x, Sx);
+            obj = objective(x, Sx);
+        }
+        gradient(Sx, grad);
+        if (std::sqrt(dots({{grad, grad}})[0]) <= rel_tol * bn + abs_tol) success = true;
+    }
+    require(success, "The baseline backward's L-BFGS did not converge in max_pd_iter; the reference's Newton "
+                     "fallback is not built.");
+    last_baseline.bwd_iters = it_used;
+    // adjoint = x with the fixed entries 0 (x is 0 there); NewtonMatrixOp(-adjoint) = -(A - K_pd) x
+    hip_check(hipMemcpyAsync(tmp, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "adj");
+    launch_residual(stream_, nv_, ell_w_, d_Aie_.p, d_Axe_.p, zero, tmp, d_dmask_.p, Sxn, nullptr);  // -A x
+    kpd_apply(x, kx, 0);
+    launch_axpbypcz(stream_, n3, 1.0, Sxn, 1.0, kx, 0.0, nullptr, Sxn);  // -(A - K_pd) x = NewtonMatrixOp(-x)
+    std::vector<double> hx(n3), hn(n3);
+    hip_check(hipStreamSynchronize(stream_), "baseline backward");
+    hip_check(hipMemcpy(hx.data(), x, sizeof(double) * n3, hipMemcpyDeviceToHost), "x");
+    hip_check(hipMemcpy(hn.data(), Sxn, sizeof(double) * n3, hipMemcpyDeviceToHost), "newton");
+    std::vector<double> dl_drhs(n3);
+    for (int i = 0; i < n3; ++i) dl_drhs[i] = fixed[i] ? agg[i] + hn[i] : hx[i] * inv_h2m;
+    // actuation and material gradients with the adjoint x
+    last_dl_da_.assign(act_dofs_, 0.0);
+    last_dl_dact_w_.assign(muscles_.size(), 0.0);
+    if (act_dofs_ > 0) {
+        d_gda_.alloc(ne_); d_gdw_.alloc(ne_);
+        launch_dact(stream_, mat_, d_q_.p, x, element_volume_ / samples_, d_gda_.p, d_gdw_.p);
+        hip_check(hipStreamSynchronize(stream_), "dact");
+        std::vector<double> ga(ne_), gw(ne_);
+        d_gda_.download(ga.data(), ne_);
+        d_gdw_.download(gw.data(), ne_);
+        for (size_t gi = 0; gi < muscles_.size(); ++gi)
+            for (int e : muscles_[gi].elems) {
+                last_dl_da_[mus_act_[e]] = ga[e];
+                last_dl_dact_w_[gi] += gw[e];
+            }
+    }
+    for (int i = 0; i < nE; ++i) dl_dmat_w[i] = 0;
+    if (nE > 0) {
+        d_gcor_.alloc(ne_); d_gvol_.alloc(ne_); d_gnh_.alloc(1);
+        launch_dmatw(stream_, mat_, d_q_.p, x, element_volume_ / samples_, has_cor_ ? d_gcor_.p : nullptr,
+                     has_vol_ ? d_gvol_.p : nullptr, nullptr);
+        hip_check(hipStreamSynchronize(stream_), "dmatw");
+        std::vector<double> gc(ne_, 0.0), gv(ne_, 0.0);
+        if (has_cor_) d_gcor_.download(gc.data(), ne_);
+        if (has_vol_) d_gvol_.download(gv.data(), ne_);
+        for (int ei = 0; ei < nE; ++ei) {
+            const Energy& en = energies_[ei];
+            const std::vector<double>& src = en.type == kCorotated ? gc : gv;
+            double s = 0;
+            for (int e = 0; e < ne_; ++e) if (en.member[e]) s += src[e];
+            dl_dmat_w[ei] = s;
+        }
+    }
+    // rhs = q on the frozen nodes, the Dirichlet values, q + h v + h2m (f_ext + f_state)
+    std::vector<double> basic = dl_drhs;
+    for (int v0 : frozen)
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = 3 * (size_t)v0 + d;
+            basic[i] = 0;
+            dl_dq[i] += dl_drhs[i];
+        }
+    for (auto& kv : dirichlet_) basic[kv.first] = 0;
+    if (gravity_.size() == 3) for (int d = 0; d < 3; ++d) dl_dstate_p[d] = 0;
+    for (int i = 0; i < n3; ++i) {
+        dl_dq[i] += basic[i];
+        dl_dv[i] += basic[i] * h;
+        dl_df_ext[i] += basic[i] * h2m;
+        if (gravity_.size() == 3) dl_dstate_p[i % 3] += basic[i] * h2m * mass;
+    }
+}
+
+}  // namespace phd

@@ -158,6 +158,17 @@ int phd_step_forward_baseline(phd_ctx ctx, const char* method, const double* q, 
         }
     });
 }
+int phd_step_backward_baseline(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                               const double* q_next, const int* active, int n_active, const double* dl_dq_next,
+                               const double* dl_dv_next, const char* const* keys, const double* vals, int n,
+                               double* dl_dq, double* dl_dv, double* dl_df_ext, double* dl_dmat_w, double* dl_dstate_p,
+                               int* stats) {
+    return guard(ctx, [&](phd::Body& b) {
+        b.backward_baseline(q, v, f_ext, dt, q_next, std::vector<int>(active, active + n_active), dl_dq_next, dl_dv_next,
+                            opts(keys, vals, n), dl_dq, dl_dv, dl_df_ext, dl_dmat_w, dl_dstate_p);
+        if (stats) stats[0] = b.last_baseline.bwd_iters;
+    });
+}
 int phd_step_forward_device(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
                             const char* const* keys, const double* vals, int n, double* q_next, double* v_next,
                             int* active, int* n_active) {

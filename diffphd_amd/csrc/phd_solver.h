@@ -199,8 +199,14 @@ public:
                           const std::map<std::string, double>& opt, double* q_next, double* v_next,
                           std::vector<int>& active);
     struct BaselineStats {
-        int contact_iters = 0, pd_iters = 0, ls_steps = 0;
+        int contact_iters = 0, pd_iters = 0, ls_steps = 0, bwd_iters = 0;
     } last_baseline;
+    // BackwardProjectiveDynamics for the baseline (backward.cpp:141-679, its
+    // dense-solver path); dl_da / dl_dact_w go to last_dl_da() / last_dl_dact_w().
+    void backward_baseline(const double* q, const double* v, const double* f_ext, double dt, const double* q_next,
+                           const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
+                           const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
+                           double* dl_dmat_w, double* dl_dstate_p);
     // x = A^-1 b (AoS 3N) with the FP32 (precision=1) or FP64 (0) factor and `refine` FP64 refinement steps.
     void apply_ainv(double dt, const double* b, double* x, int precision, int refine);
     // The same passes on caller-owned device vectors, every intermediate in the

@@ -256,8 +256,11 @@ class _Deformable:
     def PyBackward(self, method, q, v, a, f_ext, dt, q_next, v_next, active_contact_idx, dl_dq_next, dl_dv_next,
                    options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p, dl_dw_elem=None):
         if method not in METHODS:
-            raise RuntimeError("Unsupported backward method: " + str(method) + (
-                " (the baseline's backward is not built)" if str(method).startswith("pd_eigen") else ""))
+            if str(method).startswith("pd_eigen"):
+                return self._backward_baseline(q, v, a, f_ext, dt, q_next, active_contact_idx, dl_dq_next, dl_dv_next,
+                                               options, dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w,
+                                               dl_dstate_p)
+            raise RuntimeError("Unsupported backward method: " + str(method))
         n = self._dofs
         ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next)]
         act, ap = _lib.iptr(list(active_contact_idx) or [0])
@@ -281,6 +284,35 @@ class _Deformable:
         dl_dstate_p[:] = outs[4][0][: self.NumOfStateForceParameters()].tolist()
         if dl_dw_elem is not None:
             dl_dw_elem[:] = outs[5][0][: nE * self._ne].tolist()
+        _last_ctx[0] = self
+
+    def _backward_baseline(self, q, v, a, f_ext, dt, q_next, active_contact_idx, dl_dq_next, dl_dv_next, options,
+                           dl_dq, dl_dv, dl_da, dl_df_ext, dl_dmat_w, dl_dact_w, dl_dstate_p):
+        """BackwardProjectiveDynamics for a pd_eigen baseline method
+        (deformable_projective_dynamics_backward.cpp:141-679)."""
+        n = self._dofs
+        ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, dl_dq_next, dl_dv_next)]
+        act, ap = _lib.iptr(list(active_contact_idx) or [0])
+        nE = self.NumOfPdElementEnergies()
+        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), 3)]
+        st, stp = _lib.iptr(np.zeros(1))
+        karr, vals, valp, nk = _lib.options(options)
+        self._set_actuation(a)
+        self._ok(_lib.lib().phd_step_backward_baseline(self._ctx, ins[0][1], ins[1][1], ins[2][1], float(dt), ins[3][1],
+                                                       ap, len(active_contact_idx), ins[4][1], ins[5][1], karr, valp,
+                                                       nk, *[o[1] for o in outs], stp))
+        na, nm = self.act_dofs(), self.NumOfPdMuscleEnergies()
+        ga, gap = _lib.dptr(np.zeros(max(na, 1)))
+        gw, gwp = _lib.dptr(np.zeros(max(nm, 1)))
+        self._ok(_lib.lib().phd_actuation_gradients(self._ctx, gap, na, gwp, nm))
+        dl_dq[:] = outs[0][0].tolist()
+        dl_dv[:] = outs[1][0].tolist()
+        dl_da[:] = ga[:na].tolist()
+        dl_df_ext[:] = outs[2][0].tolist()
+        dl_dmat_w[:] = outs[3][0][:nE].tolist()
+        dl_dact_w[:] = gw[:nm].tolist()
+        dl_dstate_p[:] = outs[4][0][: self.NumOfStateForceParameters()].tolist()
+        self.last_baseline_bwd_stats = {"iters": int(st[0])}
         _last_ctx[0] = self
 
     # --- device tensors (DLPack handoff) ----------------------------------

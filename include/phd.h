@@ -132,6 +132,19 @@ int phd_step_forward_baseline(phd_ctx ctx, const char* method, const double* q, 
                               double dt, const char* const* keys, const double* vals, int n, double* q_next,
                               double* v_next, const int* active_in, int n_in, int* active_out, int* n_active,
                               int* stats);
+/* BackwardProjectiveDynamics for a pd_eigen baseline method
+ * (deformable_projective_dynamics_backward.cpp:141-679, the path of bodies
+ * without an elastic material): L-BFGS on the adjoint system over the free DOFs,
+ * then the frozen nodes' and Dirichlet DOFs' shares; dl_da / dl_dact_w through
+ * phd_actuation_gradients. active: the frame's active set (the forward's
+ * output). Energies of uniform stiffness; the reference's Newton-Cholesky
+ * fallback is not built (an indefinite operator or the iteration cap is an
+ * error). stats (1 int, may be NULL): L-BFGS iterations. */
+int phd_step_backward_baseline(phd_ctx ctx, const double* q, const double* v, const double* f_ext, double dt,
+                               const double* q_next, const int* active, int n_active, const double* dl_dq_next,
+                               const double* dl_dv_next, const char* const* keys, const double* vals, int n,
+                               double* dl_dq, double* dl_dv, double* dl_df_ext, double* dl_dmat_w, double* dl_dstate_p,
+                               int* stats);
 /* phd_step_forward / phd_step_backward with the state vectors (q, v, f_ext,
  * q_next, v_next, dl_dq_next, dl_dv_next, dl_dq, dl_dv, dl_df_ext) in device
  * memory of the body's GPU, e.g. the data pointers of torch tensors handed over

@@ -38,9 +38,9 @@ def rollout(body, spec, frames, opt=None, method="pd_eigen_alg_phd"):
     return q, v, acts
 
 
-def backward_frame(body, spec, q, v, acts, i, dq, dv, opt=None):
+def backward_frame(body, spec, q, v, acts, i, dq, dv, opt=None, method="pd_eigen_alg_phd"):
     opt = dict(spec["options"] if opt is None else opt)
     o = [StdRealVector(0) for _ in range(8)]
-    body.PyBackward("pd_eigen_alg_phd", q[i], v[i], list(spec.get("act", [])), np.zeros(body.dofs()), spec["dt"],
+    body.PyBackward(method, q[i], v[i], list(spec.get("act", [])), np.zeros(body.dofs()), spec["dt"],
                     q[i + 1], v[i + 1], acts[i + 1], dq, dv, opt, *o)
     return [np.asarray(x) for x in o]

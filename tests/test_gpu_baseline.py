@@ -70,3 +70,35 @@ def test_baseline_contact_frames_match_oracle():
     qn = np.asarray(qn)
     for k in nodes:
         assert np.array_equal(qn[3 * k:3 * k + 3], qo[30][3 * k:3 * k + 3])
+
+
+def test_baseline_backward_matches_oracle():
+    """The baseline's adjoint on the HIP body against the oracle: the actuated
+    cantilever (no contact) and napkin frames resting on the sphere (frozen
+    contact nodes), every gradient output to 1e-8."""
+    from helpers import backward_frame
+    spec = scenes.actuated_cantilever(cells=(6, 2, 2))
+    opt = dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000)
+    prod, orc = product_body(spec), oracle_body(spec)
+    qp, vp, ap = rollout(prod, spec, 3, opt, method="pd_eigen_pcg")
+    qo, vo, ao = rollout(orc, spec, 3, opt, method="pd_eigen_pcg")
+    rng = np.random.default_rng(11)
+    dq, dv = rng.standard_normal(prod.dofs()), rng.standard_normal(prod.dofs())
+    outp = backward_frame(prod, spec, qp, vp, ap, 2, dq, dv, opt, method="pd_eigen_pcg")
+    outo = backward_frame(orc, spec, qo, vo, ao, 2, dq, dv, opt, method="pd_eigen_pcg")
+    for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_da", 2), ("dl_df_ext", 3), ("dl_dmat_w", 4), ("dl_dact_w", 5),
+                    ("dl_dstate_p", 6)):
+        print("cantilever", name, rel_err(outp[k], outo[k]))
+        assert np.abs(outo[k]).max() > 0 and rel_err(outp[k], outo[k]) < 1e-8, name
+    spec = scenes.napkin()
+    opt = dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13)
+    prod, orc = product_body(spec), oracle_body(spec)
+    qp, vp, ap = rollout(prod, spec, 42, opt, method="pd_eigen_pcg")
+    qo, vo, ao = rollout(orc, spec, 42, opt, method="pd_eigen_pcg")
+    assert ap == ao and len(ao[-1]) > 0
+    dq, dv = rng.standard_normal(prod.dofs()), rng.standard_normal(prod.dofs())
+    outp = backward_frame(prod, spec, qp, vp, ap, 41, dq, dv, opt, method="pd_eigen_pcg")
+    outo = backward_frame(orc, spec, qo, vo, ao, 41, dq, dv, opt, method="pd_eigen_pcg")
+    for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_dmat_w", 4), ("dl_dstate_p", 6)):
+        print("napkin", len(ao[-1]), "frozen", name, rel_err(outp[k], outo[k]))
+        assert rel_err(outp[k], outo[k]) < 1e-8, name
