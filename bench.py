@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--converging-cap", type=int, default=3000, help="PD iteration cap of the time-to-tolerance leg")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default launch shapes instead of timing the candidates after the factorization")
+    p.add_argument("--no-napkin", action="store_true",
+                   help="skip the napkin leg (alg_phd and the DiffPD baseline against the published answers)")
     p.add_argument("--no-warm-start", action="store_true",
                    help="skip training the learned warm start on plain-solver press rollouts and comparing it with "
                         "q_t and q + h v on the time-to-tolerance leg")
@@ -239,6 +241,32 @@ def warm_start_leg(args, cells):
                      "fwd_frames_converged": int(sum(r["converged"])), "used": r["used"], "fallback": r["fallback"]}
         log(f"warm start {name}: {F / r['seconds']:.3f} fwd steps/s, iterations {r['iters']}")
     out["learned"]["fallback_rate"] = res[1]["fallback"] / max(F, 1)
+    return out
+
+
+def napkin_leg():
+    """The reference's installation check (python/example/napkin_3d_test.py):
+    the heterogeneous napkin on a sphere, 125 frames, forward with
+    pd_eigen_alg_phd and with the DiffPD baseline pd_eigen_pcg on the HIP body,
+    against the published summary (README.md:214-217)."""
+    from diffphd_amd import scenes
+    from diffphd_amd.env import simulate
+    from diffphd_amd.py_diff_pd_core import HexDeformable
+    spec = scenes.napkin()
+    baseline_opt = {"max_pd_iter": 5000, "max_ls_iter": 10, "abs_tol": 1e-9, "rel_tol": 1e-4, "verbose": 0,
+                    "thread_ct": 8, "use_bfgs": 1, "use_acc": 1, "bfgs_history_size": 10, "project_newton": 0}
+    out, zs = {}, {}
+    for method, opt in (("pd_eigen_alg_phd", spec["options"]), ("pd_eigen_pcg", baseline_opt)):
+        body = scenes.apply(spec, HexDeformable(0))
+        t0 = time.time()
+        _, _, info = simulate(body, spec["dt"], spec["frames"], method, opt, spec["q0"], spec["v0"],
+                              lambda q, v: (0.0, 0 * q, 0 * v), require_grad=False)
+        sec = time.time() - t0
+        z = np.array(info["q"]).reshape(len(info["q"]), -1, 3)[:, :, 2]
+        zs[method] = z
+        out[method] = {"forward_s": sec, "max_z_std_after_landing": float(z[spec["frames"] - 35:].std(axis=1).max())}
+    out["max_abs_z_drift"] = float(np.abs(zs["pd_eigen_alg_phd"] - zs["pd_eigen_pcg"]).max())
+    out["published"] = {"pd_eigen_alg_phd": 0.06671, "pd_eigen_pcg": 0.06569, "max_abs_z_drift": 1.7370e-02}
     return out
 
 
@@ -438,6 +466,9 @@ def main():
             except Exception as e:  # reported in the line; the timed legs above stand
                 log(f"warm start leg failed: {e!r}")
                 out["converging"]["warm_start"] = {"error": repr(e)}
+    if not args.no_napkin:
+        res.pop("body", None)
+        out["napkin"] = napkin_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, target_cells=cells)
     print(json.dumps(out))
