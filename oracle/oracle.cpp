@@ -1201,6 +1201,7 @@ struct Body {
         }
         pd_ready = true;
         pd_dt = dt;
+        lhs_valid = false;  // the baseline's frozen-node factors belong to the old matrix
     }
 
     Vec solve_axis(int d, const Vec& b) const {
