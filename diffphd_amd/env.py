@@ -70,6 +70,7 @@ def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, r
     dl_df = np.zeros((frames, dofs))
     dl_da = np.zeros((frames, na))
     dl_dact_w = np.zeros(body.NumOfPdMuscleEnergies())
+    dl_dstate_p = np.zeros(body.NumOfStateForceParameters())
     dq_next, dv_next = np.asarray(dl_dq, dtype=np.float64).copy(), np.asarray(dl_dv, dtype=np.float64).copy()
     for i in reversed(range(frames)):
         if plane_schedule is not None:
@@ -85,11 +86,14 @@ def simulate(body, dt, frames, method, opt, q0, v0, loss_and_grad, f_ext=None, r
             dl_da[i] = np.asarray(o[2])
         if len(o[5]):
             dl_dact_w += np.asarray(o[5])
+        if len(o[6]):
+            dl_dstate_p += np.asarray(o[6])
         w = np.asarray(o[7])
         dl_dw_elem = w.copy() if dl_dw_elem is None else dl_dw_elem + w
     info["backward_time"] = time.time() - t0
     info["dl_dmat_w"] = dl_dmat_w
     info["dl_da"] = dl_da
     info["dl_dact_w"] = dl_dact_w
+    info["dl_dstate_p"] = dl_dstate_p
     info["dl_dw_elem"] = None if dl_dw_elem is None else dl_dw_elem.reshape(nE, -1)
     return loss, (dq_next, dv_next, dl_df), info

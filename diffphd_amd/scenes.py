@@ -24,6 +24,8 @@ def apply(spec, body):
     body.InitializeFromArrays(spec["V"], spec["E"], spec["density"])
     if spec.get("gravity") is not None:
         body.AddStateForce("gravity", list(spec["gravity"]))
+    for kind, params in spec.get("state_forces", []):
+        body.AddStateForce(kind, list(params))
     for kind, params, idx in spec["energies"]:
         body.AddPdEnergy(kind, list(params), list(idx))
     for stiffness, fiber, idx in spec.get("muscles", []):
@@ -78,6 +80,28 @@ def actuated_cantilever(cells=(8, 2, 2), dx=0.01, E=1e5, nu=0.45, rho=1000.0, st
     spec["muscles"] = [(stiffness[0], [1.0, 0.0, 0.0], bottom), (stiffness[1], [1.0, 0.0, 1.0], top)]
     spec["act"] = np.random.default_rng(seed + 100).uniform(0.7, 1.0, len(bottom) + len(top))
     return spec
+
+
+def penalty_drop(cells=(4, 2, 2), dx=0.01, E=1e5, nu=0.45, rho=1000.0, kn=5.0, kf=0.005, mu=0.4, seed=0):
+    """A free block on a penalty floor (PlanarContactStateForce, the state force
+    of AddStateForce("planar_contact", [n, offset, p, kn, kf, mu]),
+    planar_contact_state_force.cpp): the bottom starts 0.2 dx below z = 0 with a
+    downward and a sideways velocity. The force is explicit (evaluated at the
+    frame start), so kn and kf stay below m / h^2 and m / h (1e-3 kg vertices,
+    dt = 1e-2); with the defaults the friction is clamped at kf (viscous), with
+    kf large it is Coulomb, mu f_n."""
+    V, Eidx, _, _ = hex_grid(np.ones(cells), dx, (0.0, 0.0, -0.2 * dx))
+    mu_l, la = lame(E, nu)
+    q0 = V.T.ravel().copy()
+    rng = np.random.default_rng(seed)
+    v0 = np.tile([0.3, 0.1, -0.2], V.shape[1]) + 0.02 * rng.standard_normal(q0.size)
+    return {
+        "name": "penalty_drop", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "state_forces": [("planar_contact", [0.0, 0.0, 1.0, 0.0, 2, kn, kf, mu])],
+        "energies": [("corotated", [2 * mu_l], []), ("volume", [la], [])], "dirichlet": [], "boundary": None,
+        "q0": q0, "v0": v0, "dt": 1e-2, "frames": 10, "loss_g": rng.standard_normal(q0.size),
+        "options": default_options(max_pd_iter=5000),
+    }
 
 
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):

@@ -52,6 +52,10 @@ def lib():
         L.orc_add_gravity.argtypes = [P, dp]
         L.orc_set_frictional_boundary.argtypes = [P, cp, dp, ctypes.c_int, ip, ctypes.c_int]
         L.orc_set_press_plane.argtypes = [P, ctypes.c_double]
+        L.orc_add_planar_contact.argtypes = [P, dp, ctypes.c_int]
+        L.orc_num_state_params.argtypes = [P]
+        L.orc_state_force.argtypes = [P, dp, dp, dp]
+        L.orc_state_force_backward.argtypes = [P, dp, dp, dp, dp, dp, dp]
         L.orc_add_actuation.argtypes = [P, ctypes.c_double, dp, ip, ctypes.c_int]
         L.orc_set_actuation.argtypes = [P, dp, ctypes.c_int]
         L.orc_act_dofs.argtypes = [P]
@@ -160,9 +164,13 @@ class OracleDeformable:
         self._ok(lib().orc_set_dirichlet(self._h, int(dof), float(val)))
 
     def AddStateForce(self, force_type, params):
-        assert force_type == "gravity"
-        g, gp = _d(params)
-        self._ok(lib().orc_add_gravity(self._h, gp))
+        p, pp = _d(params)
+        if force_type == "gravity":
+            self._ok(lib().orc_add_gravity(self._h, pp))
+        elif force_type == "planar_contact":
+            self._ok(lib().orc_add_planar_contact(self._h, pp, len(params)))
+        else:
+            raise RuntimeError("Unsupported state force: " + str(force_type))
 
     def SetFrictionalBoundary(self, boundary_type, params, indices):
         p, pp = _d(list(params))
@@ -195,7 +203,24 @@ class OracleDeformable:
         return lib().orc_num_muscles(self._h)
 
     def NumOfStateForceParameters(self):
-        return 3
+        return lib().orc_num_state_params(self._h)
+
+    def state_force(self, q, v):
+        q, qp = _d(q)
+        v, vp = _d(v)
+        f, fp = _d(np.zeros_like(q))
+        self._ok(lib().orc_state_force(self._h, qp, vp, fp))
+        return f
+
+    def state_force_backward(self, q, v, dl_df):
+        q, qp = _d(q)
+        v, vp = _d(v)
+        g, gp = _d(dl_df)
+        a, ap = _d(np.zeros_like(q))
+        b, bp = _d(np.zeros_like(q))
+        c, cp = _d(np.zeros(max(1, self.NumOfStateForceParameters())))
+        self._ok(lib().orc_state_force_backward(self._h, qp, vp, gp, ap, bp, cp))
+        return a, b, c[: self.NumOfStateForceParameters()]
 
     def PyForward(self, method, q, v, a, f_ext, dt, options, q_next, v_next, active_contact_idx):
         if method not in ("pd_eigen_alg_phd", "pd_eigen_pcg_proj_gpu"):
@@ -253,7 +278,8 @@ class OracleDeformable:
         nel = lib().orc_num_elements(self._h)
         arrs = [_d(x) for x in (q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next)]
         act, ap = _i(list(active_contact_idx) or [0])
-        outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), 3, max(ne * nel, 1))]
+        outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), max(1, self.NumOfStateForceParameters()),
+                                          max(ne * nel, 1))]
         karr, vals, valp, nk = _opts(options)
         self._set_actuation(a)
         self._ok(lib().orc_backward(self._h, arrs[0][1], arrs[1][1], arrs[2][1], float(dt), arrs[3][1], arrs[4][1],
@@ -269,7 +295,7 @@ class OracleDeformable:
         dl_df_ext[:] = list(outs[2][0])
         dl_dmat_w[:] = list(outs[3][0][:ne])
         dl_dact_w[:] = list(gw[:nm])
-        dl_dstate_p[:] = list(outs[4][0])
+        dl_dstate_p[:] = list(outs[4][0][: self.NumOfStateForceParameters()])
         if dl_dw_elem is not None:
             dl_dw_elem[:] = list(outs[5][0][: ne * nel])
 
@@ -281,7 +307,7 @@ class OracleDeformable:
         ne = lib().orc_num_energies(self._h)
         arrs = [_d(x) for x in (q, v, f_ext, q_next, dl_dq_next, dl_dv_next)]
         act, ap = _i(list(active_contact_idx) or [0])
-        outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), 3)]
+        outs = [_d(np.zeros(k)) for k in (n, n, n, max(ne, 1), max(1, self.NumOfStateForceParameters()))]
         st, stp = _i(np.zeros(2))
         karr, vals, valp, nk = _opts(options)
         self._set_actuation(a)
@@ -298,7 +324,7 @@ class OracleDeformable:
         dl_df_ext[:] = list(outs[2][0])
         dl_dmat_w[:] = list(outs[3][0][:ne])
         dl_dact_w[:] = list(gw[:nm])
-        dl_dstate_p[:] = list(outs[4][0])
+        dl_dstate_p[:] = list(outs[4][0][: self.NumOfStateForceParameters()])
         self.last_baseline_bwd_stats = {"iters": int(st[0]), "newton_fallback": int(st[1])}
 
     # --- test-only helpers -------------------------------------------------

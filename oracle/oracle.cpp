@@ -892,6 +892,17 @@ struct Body {
     Vec last_dl_da, last_dl_dact_w;
     std::map<int, double> dirichlet;
     std::vector<double> gravity;  // empty or 3
+    // PlanarContactStateForce (planar_contact_state_force.cpp:3-14): unit normal,
+    // offset / |normal|, exponent p, stiffness kn, friction clamp kf, mu; each
+    // state force's parameters sit at its offset of dl_dstate_p (AddStateForce order)
+    struct PlanarContact {
+        double n[3] = {0, 0, 1}, off = 0;
+        int p = 2;
+        double kn = 0, kf = 0, mu = 0;
+        int poff = 0;
+    };
+    std::vector<PlanarContact> pcontacts;
+    int gravity_off = 0, n_state_params = 0;
     Boundary boundary;
     std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
 
@@ -1118,15 +1129,96 @@ struct Body {
         for (int i : uniq) fb_vertices[i] = cnt++;
     }
 
-    // ForwardStateForce with GravitationalStateForce (gravitational_state_force.cpp:14-22):
-    // f_i = mass * g, mass = density * element_volume.
-    Vec state_force() const {
+    // AddStateForce("planar_contact", [n (3), offset, p, kn, kf, mu])
+    // (deformable_state_force.cpp:20-31, planar_contact_state_force.cpp:3-14).
+    void add_planar_contact(const Vec& prm) {
+        check(prm.size() == 8, "Inconsistent params for PlanarContactStateForce.");
+        PlanarContact c;
+        const double nn = std::sqrt(prm[0] * prm[0] + prm[1] * prm[1] + prm[2] * prm[2]);
+        check(nn > 1e-5, "Singular normal.");
+        for (int i = 0; i < 3; ++i) c.n[i] = prm[i] / nn;
+        c.off = prm[3] / nn;
+        c.p = (int)prm[4];
+        c.kn = prm[5]; c.kf = prm[6]; c.mu = prm[7];
+        c.poff = n_state_params;
+        n_state_params += 3;
+        pcontacts.push_back(c);
+    }
+    // ForwardStateForce (deformable_state_force.cpp:114-125): gravity f_i = mass g
+    // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
+    // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s,
+    // u_s the tangential velocity (planar_contact_state_force.cpp:16-37).
+    Vec state_force(const Vec& q, const Vec& v) const {
         Vec f(dofs, 0.0);
         if (gravity.size() == 3) {
             const double m = density * element_volume;
             for (int i = 0; i < dofs; ++i) f[i] = m * gravity[i % 3];
         }
+        const double eps = std::numeric_limits<double>::epsilon();
+        for (const auto& c : pcontacts)
+            for (int i = 0; i < nv; ++i) {
+                const double* x = &q[3 * i];
+                const double* vi = &v[3 * i];
+                const double d = c.n[0] * x[0] + c.n[1] * x[1] + c.n[2] * x[2] + c.off;
+                const double fn = c.kn * std::pow(-std::min(0.0, d), c.p - 1);
+                const double vn = c.n[0] * vi[0] + c.n[1] * vi[1] + c.n[2] * vi[2];
+                double us[3];
+                for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * c.n[k];
+                const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+                const double cf = std::min(c.kf, c.mu * fn / (um + eps));
+                for (int k = 0; k < 3; ++k) f[3 * i + k] += fn * c.n[k] - cf * us[k];
+            }
         return f;
+    }
+    // BackwardStateForce (deformable_state_force.cpp:127-145): gravity's
+    // dl/dg = mass sum dl/df per axis (gravitational_state_force.cpp:24-33) and the
+    // planar contacts' derivatives (planar_contact_state_force.cpp:39-84).
+    void state_force_backward(const Vec& q, const Vec& v, const Vec& dl_df, Vec& dl_dq, Vec& dl_dv, Vec& dl_dp) const {
+        dl_dq.assign(dofs, 0.0);
+        dl_dv.assign(dofs, 0.0);
+        dl_dp.assign(n_state_params, 0.0);
+        if (gravity.size() == 3) {
+            const double m = density * element_volume;
+            for (int i = 0; i < dofs; ++i) dl_dp[gravity_off + i % 3] += m * dl_df[i];
+        }
+        const double eps = std::numeric_limits<double>::epsilon();
+        for (const auto& c : pcontacts)
+            for (int i = 0; i < nv; ++i) {
+                const double* x = &q[3 * i];
+                const double* vi = &v[3 * i];
+                const double* g = &dl_df[3 * i];
+                const double d = c.n[0] * x[0] + c.n[1] * x[1] + c.n[2] * x[2] + c.off;
+                const double pen = -std::min(0.0, d);
+                const double fn = c.kn * std::pow(pen, c.p - 1);
+                // d fn / dq = kn (p-1) pen^(p-2) (-n) inside the plane, 0 outside
+                const double dfn = d < 0 ? -c.kn * (c.p - 1) * std::pow(pen, c.p - 2) : 0.0;
+                const double vn = c.n[0] * vi[0] + c.n[1] * vi[1] + c.n[2] * vi[2];
+                double us[3];
+                for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * c.n[k];
+                const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+                const double ue = um + eps;
+                const double gn = g[0] * c.n[0] + g[1] * c.n[1] + g[2] * c.n[2];
+                const double gu = g[0] * us[0] + g[1] * us[1] + g[2] * us[2];
+                // normal part: f_n n
+                for (int k = 0; k < 3; ++k) dl_dq[3 * i + k] += gn * dfn * c.n[k];
+                dl_dp[c.poff] += gn * fn / c.kn;
+                // (I - n n^T) g: the tangential projection of dl/df
+                double gt[3];
+                for (int k = 0; k < 3; ++k) gt[k] = g[k] - gn * c.n[k];
+                if (c.kf <= c.mu * fn / ue) {
+                    // ff = -kf u_s
+                    for (int k = 0; k < 3; ++k) dl_dv[3 * i + k] += -c.kf * gt[k];
+                    dl_dp[c.poff + 1] += -gu;
+                } else {
+                    // ff = -mu fn u_s / (|u_s| + eps)
+                    for (int k = 0; k < 3; ++k) dl_dq[3 * i + k] += -c.mu * gu / ue * dfn * c.n[k];
+                    // d(u_s / ue)/dv^T g = gt / ue - u_s (u_s . g) / ue^3: the reference takes
+                    // d|u_s|/dv as P u_s / (|u_s| + eps)
+                    for (int k = 0; k < 3; ++k) dl_dv[3 * i + k] += -c.mu * fn * (gt[k] / ue - us[k] * gu / (ue * ue * ue));
+                    dl_dp[c.poff] += -c.mu * gu / ue * (fn / c.kn);
+                    dl_dp[c.poff + 2] += -gu / ue * fn;
+                }
+            }
     }
 
     // SetupProjectiveDynamicsSolver, deformable_projective_dynamics_forward.cpp:8-183
@@ -1497,7 +1589,7 @@ void Body::forward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, cons
     check((int)q.size() == dofs && (int)v.size() == dofs && (int)f_ext.size() == dofs, "Inconsistent state size.");
     setup(dt);
     const double h = dt, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
-    const Vec fs = state_force();
+    const Vec fs = state_force(q, v);
     Vec q_tilde(dofs);
     for (int i = 0; i < dofs; ++i) q_tilde[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
 
@@ -1796,7 +1888,7 @@ AlgPhdReplayCache Body::replay_fixed_active_set(const Vec& q, const Vec& v, cons
     AlgPhdReplayCache cache;
     cache.contact_candidates = active;
     const int na = (int)active.size();
-    const Vec fs = state_force();
+    const Vec fs = state_force(q, v);
     cache.q_tilde.assign(dofs, 0.0);
     for (int i = 0; i < dofs; ++i) cache.q_tilde[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
     cache.q_sol = q;
@@ -1905,7 +1997,7 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
     dl_dv.assign(dofs, 0.0);
     dl_df_ext.assign(dofs, 0.0);
     dl_dmat_w.assign(nE, 0.0);
-    dl_dstate_p.assign(gravity.size() == 3 ? 3 : 0, 0.0);
+    dl_dstate_p.assign(n_state_params, 0.0);
     dl_dw_elem.assign((size_t)nE * ne, 0.0);
     Vec rhs_qbar(dofs);
     for (int i = 0; i < dofs; ++i) {
@@ -2038,9 +2130,12 @@ void Body::backward(const Vec& q, const Vec& v, const Vec& f_ext, double dt, con
             dl_df_ext[i] += h2m * inj[i];
             dl_dv[i] += h * inj[i];
         }
-        // GravitationalStateForce::BackwardForce (gravitational_state_force.cpp:24-33).
-        if (gravity.size() == 3)
-            for (int i = 0; i < dofs; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
+        // through h2m f_state(q, v) (backward_gpu.cpp:904-915)
+        Vec dl_df(dofs), sq, sv, sp;
+        for (int i = 0; i < dofs; ++i) dl_df[i] = h2m * inj[i];
+        state_force_backward(q, v, dl_df, sq, sv, sp);
+        for (int i = 0; i < dofs; ++i) { dl_dq[i] += sq[i]; dl_dv[i] += sv[i]; }
+        for (int k = 0; k < n_state_params; ++k) dl_dstate_p[k] += sp[k];
     };
     const bool state_wb = opt_or(opt, "alg_phd_bwd_state_woodbury", 0.0) > 0.5;
     if (!diverged && !state_wb) inject(db);
@@ -2575,7 +2670,7 @@ void Body::backward_baseline(const Vec& q, const Vec& v, const Vec& f_ext, doubl
     dl_dv.assign(dofs, 0.0);
     dl_df_ext.assign(dofs, 0.0);
     dl_dmat_w.assign(nE, 0.0);
-    dl_dstate_p.assign(gravity.size() == 3 ? 3 : 0, 0.0);
+    dl_dstate_p.assign(n_state_params, 0.0);
     last_dl_da.assign(act_dofs, 0.0);
     last_dl_dact_w.assign(muscles.size(), 0.0);
     const double h = dt, inv_h = 1 / h, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
@@ -2766,8 +2861,13 @@ void Body::backward_baseline(const Vec& q, const Vec& v, const Vec& f_ext, doubl
         dl_dv[i] += basic[i] * h;
         dl_df_ext[i] += basic[i] * h2m;
     }
-    if (gravity.size() == 3)
-        for (int i = 0; i < dofs; ++i) dl_dstate_p[i % 3] += basic[i] * h2m * mass;
+    {
+        Vec dl_df(dofs), sq, sv, sp;
+        for (int i = 0; i < dofs; ++i) dl_df[i] = basic[i] * h2m;
+        state_force_backward(q, v, dl_df, sq, sv, sp);
+        for (int i = 0; i < dofs; ++i) { dl_dq[i] += sq[i]; dl_dv[i] += sv[i]; }
+        for (int k = 0; k < n_state_params; ++k) dl_dstate_p[k] += sp[k];
+    }
 }
 
 // SameSet / ProposeNewSet (common.cpp:173-216): keep a /\ b plus the first half
@@ -2809,7 +2909,7 @@ void Body::forward_baseline(const std::string& method, const Vec& q, const Vec& 
     check((int)q.size() == dofs && (int)v.size() == dofs && (int)f_ext.size() == dofs, "Inconsistent state size.");
     setup(dt);
     const double h = dt, mass = element_volume * density, h2m = h * h / mass, inv_h2m = mass / (h * h);
-    const Vec fs = state_force();
+    const Vec fs = state_force(q, v);
     Vec rhs(dofs);
     for (int i = 0; i < dofs; ++i) rhs[i] = q[i] + h * v[i] + h2m * f_ext[i] + h2m * fs[i];
     auto finish = [&](const Vec& qs) {
@@ -2920,7 +3020,11 @@ int orc_set_dirichlet(void* hp, int dof, double val) {
 }
 int orc_add_gravity(void* hp, const double* g) {
     Handle* h = static_cast<Handle*>(hp);
-    return guard(h, [&] { h->body.gravity.assign(g, g + 3); });
+    return guard(h, [&] {
+        h->body.gravity.assign(g, g + 3);
+        h->body.gravity_off = h->body.n_state_params;
+        h->body.n_state_params += 3;
+    });
 }
 int orc_set_frictional_boundary(void* hp, const char* type, const double* p, int np, const int* idx, int ni) {
     Handle* h = static_cast<Handle*>(hp);
@@ -2990,6 +3094,32 @@ int orc_backward_baseline(void* hp, const double* q, const double* v, const doub
         std::copy(e.begin(), e.end(), dl_dstate_p);
         stats[0] = h->body.last_baseline_bwd_iters;
         stats[1] = h->body.last_baseline_bwd_newton;
+    });
+}
+int orc_add_planar_contact(void* hp, const double* p, int np) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_planar_contact(orc::Vec(p, p + np)); });
+}
+int orc_num_state_params(void* hp) { return static_cast<Handle*>(hp)->body.n_state_params; }
+// f_state(q, v) and its backward for a given dl/df (test access)
+int orc_state_force(void* hp, const double* q, const double* v, double* f) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        const orc::Vec r = h->body.state_force(orc::Vec(q, q + n), orc::Vec(v, v + n));
+        std::copy(r.begin(), r.end(), f);
+    });
+}
+int orc_state_force_backward(void* hp, const double* q, const double* v, const double* dl_df, double* dq, double* dv,
+                             double* dp) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] {
+        const int n = h->body.dofs;
+        orc::Vec a, b, c;
+        h->body.state_force_backward(orc::Vec(q, q + n), orc::Vec(v, v + n), orc::Vec(dl_df, dl_df + n), a, b, c);
+        std::copy(a.begin(), a.end(), dq);
+        std::copy(b.begin(), b.end(), dv);
+        std::copy(c.begin(), c.end(), dp);
     });
 }
 int orc_dofs(void* hp) { return static_cast<Handle*>(hp)->body.dofs; }
