@@ -49,6 +49,7 @@ SIGNATURES = [
     ("phd_add_actuation", ctypes.c_int, [_P, ctypes.c_double, _dp, _ip, ctypes.c_int]),
     ("phd_set_actuation", ctypes.c_int, [_P, _dp, ctypes.c_int]),
     ("phd_act_dofs", ctypes.c_int, [_P]),
+    ("phd_num_state_force_parameters", ctypes.c_int, [_P]),
     ("phd_num_pd_muscle_energies", ctypes.c_int, [_P]),
     ("phd_actuation_gradients", ctypes.c_int, [_P, _dp, ctypes.c_int, _dp, ctypes.c_int]),
     ("phd_s_memory_bytes", ctypes.c_longlong, [_P]),

@@ -427,7 +427,7 @@ void Body::backward_baseline(const double* q, const double* v, const double* f_e
                              const std::vector<int>& active, const double* dl_dq_next, const double* dl_dv_next,
                              const std::map<std::string, double>& opt, double* dl_dq, double* dl_dv, double* dl_df_ext,
                              double* dl_dmat_w, double* dl_dstate_p) {
-    (void)v; (void)f_ext;
+    (void)f_ext;
     const int max_pd_iter = (int)req(opt, "max_pd_iter");
     const double abs_tol = req(opt, "abs_tol"), rel_tol = req(opt, "rel_tol");
     req(opt, "verbose");
@@ -649,13 +649,15 @@ void Body::backward_baseline(const double* q, const double* v, const double* f_e
             dl_dq[i] += dl_drhs[i];
         }
     for (auto& kv : dirichlet_) basic[kv.first] = 0;
-    if (gravity_.size() == 3) for (int d = 0; d < 3; ++d) dl_dstate_p[d] = 0;
+    std::vector<double> dl_df(n3), sq(n3), sv(n3);
     for (int i = 0; i < n3; ++i) {
         dl_dq[i] += basic[i];
         dl_dv[i] += basic[i] * h;
         dl_df_ext[i] += basic[i] * h2m;
-        if (gravity_.size() == 3) dl_dstate_p[i % 3] += basic[i] * h2m * mass;
+        dl_df[i] = basic[i] * h2m;
     }
+    state_force_backward(q, v, dl_df.data(), sq.data(), sv.data(), dl_dstate_p);
+    for (int i = 0; i < n3; ++i) { dl_dq[i] += sq[i]; dl_dv[i] += sv[i]; }
 }
 
 }  // namespace phd

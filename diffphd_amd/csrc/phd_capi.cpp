@@ -91,11 +91,18 @@ int phd_set_dirichlet(phd_ctx ctx, int dof, double val) {
 }
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np) {
     return guard(ctx, [&](phd::Body& b) {
-        if (std::string(type) != "gravity") throw std::runtime_error(std::string("Unsupported state force type: ") + type);
-        if (np != 3) throw std::runtime_error("Inconsistent params for GravitionalStateForce.");
-        b.add_gravity(params);
+        const std::string t(type);
+        if (t == "gravity") {
+            if (np != 3) throw std::runtime_error("Inconsistent params for GravitionalStateForce.");
+            b.add_gravity(params);
+        } else if (t == "planar_contact") {
+            b.add_planar_contact(std::vector<double>(params, params + np));
+        } else {
+            throw std::runtime_error(std::string("Unsupported state force type: ") + type);
+        }
     });
 }
+int phd_num_state_force_parameters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->body->num_state_params() : -1; }
 int phd_set_frictional_boundary(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni) {
     return guard(ctx, [&](phd::Body& b) {
         b.set_boundary(type, std::vector<double>(params, params + np), std::vector<int>(idx, idx + ni));

@@ -192,6 +192,8 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
         }
     }
     energies_.clear();
+    pcontacts_.clear();
+    gravity_off_ = n_state_params_ = 0;
     muscles_.clear();
     act_dofs_ = 0;
     mus_act_.assign(ne, -1);
@@ -283,7 +285,90 @@ void Body::set_dirichlet(int dof, double val) {
     ready_ = false;
 }
 
-void Body::add_gravity(const double* g) { gravity_.assign(g, g + 3); }
+void Body::add_gravity(const double* g) {
+    gravity_.assign(g, g + 3);
+    gravity_off_ = n_state_params_;
+    n_state_params_ += 3;
+}
+
+// PlanarContactStateForce::Initialize (planar_contact_state_force.cpp:3-14).
+void Body::add_planar_contact(const std::vector<double>& p) {
+    check(p.size() == 8, "Inconsistent params for PlanarContactStateForce.");
+    PlanarContactForce c;
+    const double nn = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    check(nn > 1e-5, "Singular normal.");
+    for (int i = 0; i < 3; ++i) c.n[i] = p[i] / nn;
+    c.off = p[3] / nn;
+    c.p = (int)p[4];
+    c.kn = p[5]; c.kf = p[6]; c.mu = p[7];
+    c.poff = n_state_params_;
+    n_state_params_ += 3;
+    pcontacts_.push_back(c);
+}
+
+// ForwardStateForce (deformable_state_force.cpp:114-125): gravity m g
+// (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
+// kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s
+// (planar_contact_state_force.cpp:16-37), evaluated at the frame start.
+void Body::state_force(const double* q, const double* v, double* f) const {
+    const double mass = element_volume_ * density_;
+    for (int i = 0; i < dofs_; ++i) f[i] = gravity_.size() == 3 ? mass * gravity_[i % 3] : 0.0;
+    for (const auto& c : pcontacts_)
+        for (int i = 0; i < nv_; ++i) {
+            const double* x = q + 3 * (size_t)i;
+            const double* vi = v + 3 * (size_t)i;
+            const double d = c.n[0] * x[0] + c.n[1] * x[1] + c.n[2] * x[2] + c.off;
+            const double fn = c.kn * std::pow(-std::min(0.0, d), c.p - 1);
+            const double vn = c.n[0] * vi[0] + c.n[1] * vi[1] + c.n[2] * vi[2];
+            double us[3];
+            for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * c.n[k];
+            const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+            const double cf = std::min(c.kf, c.mu * fn / (um + DBL_EPSILON));
+            for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] += fn * c.n[k] - cf * us[k];
+        }
+}
+
+// BackwardStateForce (deformable_state_force.cpp:127-145): dl/dq, dl/dv and
+// dl/dp of a given dl/df (planar_contact_state_force.cpp:39-84, with the
+// reference's d|u_s|/dv = P u_s / (|u_s| + eps); gravity: dl/dg = m sum dl/df).
+void Body::state_force_backward(const double* q, const double* v, const double* dl_df, double* dl_dq, double* dl_dv,
+                                double* dl_dp) const {
+    for (int i = 0; i < dofs_; ++i) { dl_dq[i] = 0; dl_dv[i] = 0; }
+    for (int k = 0; k < n_state_params_; ++k) dl_dp[k] = 0;
+    const double mass = element_volume_ * density_;
+    if (gravity_.size() == 3) for (int i = 0; i < dofs_; ++i) dl_dp[gravity_off_ + i % 3] += mass * dl_df[i];
+    for (const auto& c : pcontacts_)
+        for (int i = 0; i < nv_; ++i) {
+            const double* x = q + 3 * (size_t)i;
+            const double* vi = v + 3 * (size_t)i;
+            const double* g = dl_df + 3 * (size_t)i;
+            const double d = c.n[0] * x[0] + c.n[1] * x[1] + c.n[2] * x[2] + c.off;
+            const double pen = -std::min(0.0, d);
+            const double fn = c.kn * std::pow(pen, c.p - 1);
+            const double dfn = d < 0 ? -c.kn * (c.p - 1) * std::pow(pen, c.p - 2) : 0.0;  // d f_n / d(n . q)
+            const double vn = c.n[0] * vi[0] + c.n[1] * vi[1] + c.n[2] * vi[2];
+            double us[3];
+            for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * c.n[k];
+            const double ue = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]) + DBL_EPSILON;
+            const double gn = g[0] * c.n[0] + g[1] * c.n[1] + g[2] * c.n[2];
+            const double gu = g[0] * us[0] + g[1] * us[1] + g[2] * us[2];
+            double gt[3];
+            for (int k = 0; k < 3; ++k) gt[k] = g[k] - gn * c.n[k];
+            for (int k = 0; k < 3; ++k) dl_dq[3 * (size_t)i + k] += gn * dfn * c.n[k];
+            dl_dp[c.poff] += gn * fn / c.kn;
+            if (c.kf <= c.mu * fn / ue) {
+                for (int k = 0; k < 3; ++k) dl_dv[3 * (size_t)i + k] += -c.kf * gt[k];
+                dl_dp[c.poff + 1] += -gu;
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    dl_dq[3 * (size_t)i + k] += -c.mu * gu / ue * dfn * c.n[k];
+                    dl_dv[3 * (size_t)i + k] += -c.mu * fn * (gt[k] / ue - us[k] * gu / (ue * ue * ue));
+                }
+                dl_dp[c.poff] += -c.mu * gu / ue * (fn / c.kn);
+                dl_dp[c.poff + 2] += -gu / ue * fn;
+            }
+        }
+}
 
 // SetFrictionalBoundary, deformable_friction.cpp:7-89, plus heightfield/press.
 void Body::set_boundary(const std::string& type, const std::vector<double>& p, const std::vector<int>& idx) {
@@ -1051,10 +1136,9 @@ void Body::kpd_apply(const double* x, double* out, int precision) {
 void Body::set_inertial_target(const double* q, const double* v, const double* f_ext, double dt) {
     const int n3 = dofs_;
     const double h = dt, mass = element_volume_ * density_, h2m = h * h / mass;
-    const double mg[3] = {gravity_.size() == 3 ? mass * gravity_[0] : 0.0, gravity_.size() == 3 ? mass * gravity_[1] : 0.0,
-                          gravity_.size() == 3 ? mass * gravity_[2] : 0.0};
-    std::vector<double> qt(n3);
-    for (int i = 0; i < n3; ++i) qt[i] = q[i] + h * v[i] + h2m * (f_ext ? f_ext[i] : 0.0) + h2m * mg[i % 3];
+    std::vector<double> fs(n3), qt(n3);
+    state_force(q, v, fs.data());
+    for (int i = 0; i < n3; ++i) qt[i] = q[i] + h * v[i] + h2m * (f_ext ? f_ext[i] : 0.0) + h2m * fs[i];
     d_qtilde_.upload(qt);
 }
 
@@ -1758,7 +1842,7 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
     }
     const int nE = (int)energies_.size();
     for (int i = 0; i < nE; ++i) dl_dmat_w[i] = 0;
-    if (gravity_.size() == 3) for (int d = 0; d < 3; ++d) dl_dstate_p[d] = 0;
+    for (int k = 0; k < n_state_params_; ++k) dl_dstate_p[k] = 0;
     if (dl_dw_elem) std::fill(dl_dw_elem, dl_dw_elem + (size_t)nE * ne_, 0.0);
     // proximal-map Jacobians at q_next
     d_q_.upload(q_next, n3);
@@ -1945,7 +2029,12 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
         const double cap = nrm2(rhs_qbar) + 1.0, in = nrm2(inj);
         if (in > cap) for (double& t : inj) t *= cap / in;
         for (int i = 0; i < n3; ++i) { dl_dq[i] += inj[i]; dl_df_ext[i] += h2m * inj[i]; dl_dv[i] += h * inj[i]; }
-        if (gravity_.size() == 3) for (int i = 0; i < n3; ++i) dl_dstate_p[i % 3] += h2m * inj[i] * mass;
+        // through h2m f_state(q, v) (backward_gpu.cpp:904-915)
+        std::vector<double> dl_df(n3), sq(n3), sv(n3), sp(n_state_params_);
+        for (int i = 0; i < n3; ++i) dl_df[i] = h2m * inj[i];
+        state_force_backward(q, v, dl_df.data(), sq.data(), sv.data(), sp.data());
+        for (int i = 0; i < n3; ++i) { dl_dq[i] += sq[i]; dl_dv[i] += sv[i]; }
+        for (int k = 0; k < n_state_params_; ++k) dl_dstate_p[k] += sp[k];
     };
     if (!diverged && !state_wb) inject(db);
     std::vector<double> z = db;

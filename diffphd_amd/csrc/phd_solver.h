@@ -167,6 +167,13 @@ public:
     void add_pd_energy(const std::string& type, const std::vector<double>& params, const std::vector<int>& idx);
     void set_dirichlet(int dof, double val);
     void add_gravity(const double* g);
+    // AddStateForce("planar_contact", [n (3), offset, p, kn, kf, mu]) (PlanarContactStateForce)
+    void add_planar_contact(const std::vector<double>& p);
+    int num_state_params() const { return n_state_params_; }
+    // ForwardStateForce / BackwardStateForce at (q, v) on the host (3N vectors)
+    void state_force(const double* q, const double* v, double* f) const;
+    void state_force_backward(const double* q, const double* v, const double* dl_df, double* dl_dq, double* dl_dv,
+                              double* dl_dp) const;
     void set_boundary(const std::string& type, const std::vector<double>& p, const std::vector<int>& idx);
     void set_press_plane(double z) { boundary_.plane_z = z; }
     // PdMuscleEnergy groups (AddActuation) and the actuation levels a of the
@@ -317,6 +324,14 @@ private:
     std::vector<double> last_dl_da_, last_dl_dact_w_;
     std::map<int, double> dirichlet_;
     std::vector<double> gravity_;
+    struct PlanarContactForce {
+        double n[3] = {0, 0, 1}, off = 0;
+        int p = 2;
+        double kn = 0, kf = 0, mu = 0;
+        int poff = 0;  // its parameters' offset in dl_dstate_p
+    };
+    std::vector<PlanarContactForce> pcontacts_;
+    int gravity_off_ = 0, n_state_params_ = 0;
     HostBoundary boundary_;
     std::map<int, int> fb_vertices_;
 

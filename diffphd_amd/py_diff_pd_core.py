@@ -107,7 +107,6 @@ class _Deformable:
             raise RuntimeError("phd_ctx_create failed: " + L.phd_last_error(None).decode())
         self._dofs = 0
         self._ne = 0
-        self._gravity = False
         _last_ctx[0] = self
 
     def __del__(self):
@@ -156,7 +155,6 @@ class _Deformable:
     def AddStateForce(self, force_type, params):
         p, pp = _lib.dptr(list(params))
         self._ok(_lib.lib().phd_add_state_force(self._ctx, force_type.encode(), pp, len(params)))
-        self._gravity = True
 
     def SetFrictionalBoundary(self, boundary_type, params, indices):
         p, pp = _lib.dptr(list(params))
@@ -190,7 +188,7 @@ class _Deformable:
         return _lib.lib().phd_num_pd_muscle_energies(self._ctx)
 
     def NumOfStateForceParameters(self):
-        return 3 if self._gravity else 0
+        return _lib.lib().phd_num_state_force_parameters(self._ctx)
 
     def mesh(self):
         m = HexMesh3d() if self._edim == 8 else TetMesh3d()
@@ -265,7 +263,8 @@ class _Deformable:
         ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, v_next, dl_dq_next, dl_dv_next)]
         act, ap = _lib.iptr(list(active_contact_idx) or [0])
         nE = self.NumOfPdElementEnergies()
-        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), 3, max(nE * self._ne, 1))]
+        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), max(1, self.NumOfStateForceParameters()),
+                                                 max(nE * self._ne, 1))]
         karr, vals, valp, nk = _lib.options(options)
         self._set_actuation(a)
         self._ok(_lib.lib().phd_step_backward(self._ctx, ins[0][1], ins[1][1], ins[2][1], float(dt), ins[3][1],
@@ -294,7 +293,7 @@ class _Deformable:
         ins = [_lib.dptr(x) for x in (q, v, f_ext, q_next, dl_dq_next, dl_dv_next)]
         act, ap = _lib.iptr(list(active_contact_idx) or [0])
         nE = self.NumOfPdElementEnergies()
-        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), 3)]
+        outs = [_lib.dptr(np.zeros(k)) for k in (n, n, n, max(nE, 1), max(1, self.NumOfStateForceParameters()))]
         st, stp = _lib.iptr(np.zeros(1))
         karr, vals, valp, nk = _lib.options(options)
         self._set_actuation(a)
@@ -357,7 +356,8 @@ class _Deformable:
                                                   (dl_dv_next, "dl_dv_next"))]
         outs = [torch.empty_like(ins[0]) for _ in range(3)]
         nE = self.NumOfPdElementEnergies()
-        host = [_lib.dptr(np.zeros(k)) for k in (max(nE, 1), 3, max(nE * self._ne, 1))]
+        host = [_lib.dptr(np.zeros(k)) for k in (max(nE, 1), max(1, self.NumOfStateForceParameters()),
+                                                 max(nE * self._ne, 1))]
         act, ap = _lib.iptr(list(active_contact_idx) or [0])
         karr, vals, valp, nk = _lib.options(options)
         self._set_actuation(a.detach().cpu().numpy() if hasattr(a, "detach") else a)

@@ -38,8 +38,13 @@ int phd_init_mesh(phd_ctx ctx, int edim, int nv, const double* X, int ne, const 
 int phd_add_pd_energy(phd_ctx ctx, const char* type, const double* params, int np, const int* idx, int ni);
 /* Deformable::SetDirichletBoundaryCondition (deformable.h:70-73). */
 int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
-/* Deformable::AddStateForce("gravity", g) (deformable_state_force.cpp:13-20). */
+/* Deformable::AddStateForce (deformable_state_force.cpp:10-31): "gravity" [gx, gy, gz]
+ * or "planar_contact" [nx, ny, nz, offset, p, kn, kf, mu] (a penalty floor with
+ * friction, planar_contact_state_force.cpp). */
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
+/* Deformable::NumOfStateForceParameters(): the length of dl_dstate_p (3 per
+ * state force, in AddStateForce order). */
+int phd_num_state_force_parameters(phd_ctx ctx);
 /* Deformable::SetFrictionalBoundary (deformable_friction.cpp:7-89): "planar"
  * [nx, ny, nz, offset], "spherical" [cx, cy, cz, r], and the two boundaries with
  * no reference counterpart used by the press scene: "heightfield"
