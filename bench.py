@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--converging-cap", type=int, default=3000, help="PD iteration cap of the time-to-tolerance leg")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep the default launch shapes instead of timing the candidates after the factorization")
+    p.add_argument("--no-tet", action="store_true",
+                   help="skip the heterogeneous Neo-Hookean tet block leg (40^3 cubes, 320k tets)")
     p.add_argument("--no-napkin", action="store_true",
                    help="skip the napkin leg (alg_phd and the DiffPD baseline against the published answers)")
     p.add_argument("--no-warm-start", action="store_true",
@@ -241,6 +243,64 @@ def warm_start_leg(args, cells):
                      "fwd_frames_converged": int(sum(r["converged"])), "used": r["used"], "fallback": r["fallback"]}
         log(f"warm start {name}: {F / r['seconds']:.3f} fwd steps/s, iterations {r['iters']}")
     out["learned"]["fallback_rate"] = res[1]["fallback"] / max(F, 1)
+    return out
+
+
+def tet_leg(args, n=40, prep=14, K=2):
+    """The heterogeneous Neo-Hookean tet block (40^3 cubes split in 5 tets:
+    68,921 vertices, 320k tets; 100x inclusions, friction 0.5, dt 5e-3) landing
+    on the ground: `prep` untimed frames until the bottom touches, then K timed
+    fwd+bwd frames in mixed and in FP64 from the same state, with the deviation
+    of the end state and of the window's gradients."""
+    from diffphd_amd import scenes
+    from diffphd_amd.py_diff_pd_core import StdRealVector, TetDeformable
+    spec = scenes.tet_block(n=n)
+    out = {"workload": f"tet block {n}^3 cubes x 5 tets ({spec['E'].shape[1]} tets, {spec['V'].shape[1]} vertices), "
+                       "pd_neohookean, friction 0.5, dt 5e-3"}
+    res = {}
+    for name, prec in (("mixed", 1), ("fp64", 0)):
+        body = scenes.apply(spec, TetDeformable(0))
+        opt = dict(spec["options"], phd_precision=prec, thread_ct=args.threads)
+        body.factor_build(spec["dt"], args.threads)
+        ndof = body.dofs()
+        q, v, act = np.asarray(spec["q0"], float), np.asarray(spec["v0"], float), []
+        for _ in range(prep):
+            qn, vn, a = StdRealVector(ndof), StdRealVector(ndof), list(act)
+            body.PyForward("pd_eigen_alg_phd", q, v, [], np.zeros(ndof), spec["dt"], dict(opt, max_pd_iter=200), qn, vn, a)
+            q, v, act = np.asarray(qn), np.asarray(vn), list(a)
+        qs, vs, acts = [q], [v], [act]
+        body.reset_frame_stats()
+        t0 = time.time()
+        for _ in range(K):
+            qn, vn, a = StdRealVector(ndof), StdRealVector(ndof), list(acts[-1])
+            body.PyForward("pd_eigen_alg_phd", qs[-1], vs[-1], [], np.zeros(ndof), spec["dt"], opt, qn, vn, a)
+            qs.append(np.asarray(qn))
+            vs.append(np.asarray(vn))
+            acts.append(list(a))
+        iters, conv = body.frame_stats()[:2]
+        g = np.random.default_rng(0).standard_normal(ndof)
+        dq, dv = g.copy(), np.zeros(ndof)
+        bst = []
+        for i in reversed(range(K)):
+            o = [StdRealVector(0) for _ in range(8)]
+            body.PyBackward("pd_eigen_alg_phd", qs[i], vs[i], [], np.zeros(ndof), spec["dt"], qs[i + 1], vs[i + 1],
+                            acts[i + 1], dq, dv, dict(opt, **BWD_OPTS), *o)
+            dq, dv = np.asarray(o[0]), np.asarray(o[1])
+            bst.append(body.last_backward_stats())
+        sec = time.time() - t0
+        res[name] = (qs[-1], dq, dv)
+        out[name] = {"steps_per_s": K / sec, "fwd_pd_iters": [int(x) for x in iters[-K:]],
+                     "fwd_frames_converged": int(np.sum(conv[-K:])), "active_contacts": len(acts[-1]),
+                     "bwd_backbone_iters": [int(b["backbone_iters"]) for b in bst],
+                     "clipped_bwd_frames": int(sum(b["clipped"] for b in bst))}
+        del body
+
+    def dev(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out["speedup_vs_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["rel_dev_mixed_vs_fp64"] = {"q_end": dev(res["mixed"][0], res["fp64"][0]),
+                                    "dl_dq_start": dev(res["mixed"][1], res["fp64"][1]),
+                                    "dl_dv_start": dev(res["mixed"][2], res["fp64"][2])}
     return out
 
 
@@ -466,6 +526,9 @@ def main():
             except Exception as e:  # reported in the line; the timed legs above stand
                 log(f"warm start leg failed: {e!r}")
                 out["converging"]["warm_start"] = {"error": repr(e)}
+    if not args.no_tet:
+        res.pop("body", None)
+        out["tet_block"] = tet_leg(args)
     if not args.no_napkin:
         res.pop("body", None)
         out["napkin"] = napkin_leg()
