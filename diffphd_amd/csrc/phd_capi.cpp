@@ -266,10 +266,10 @@ int phd_last_backward_iters(phd_ctx ctx) { return ctx ? static_cast<Ctx*>(ctx)->
 int phd_last_backward_stats(phd_ctx ctx, double* out, int cap) {
     if (!ctx) return -1;
     const auto& s = static_cast<Ctx*>(ctx)->body->last_bwd;
-    const double v[11] = {s.backbone_iters, s.backbone_converged, s.clipped, s.woodbury_iters, s.woodbury_converged,
-                          s.basis_rows, s.filtered, s.samples, s.db_norm, s.trust_stage, s.from_tape};
-    for (int i = 0; i < 11 && i < cap; ++i) out[i] = v[i];
-    return 11;
+    const double v[12] = {s.backbone_iters, s.backbone_converged, s.clipped, s.woodbury_iters, s.woodbury_converged,
+                          s.basis_rows, s.filtered, s.samples, s.db_norm, s.trust_stage, s.from_tape, s.gram_fp64_fallbacks};
+    for (int i = 0; i < 12 && i < cap; ++i) out[i] = v[i];
+    return 12;
 }
 int phd_factor_info(phd_ctx ctx, double* out, int cap) {
     return guard(ctx, [&](phd::Body& b) {

@@ -170,7 +170,8 @@ void launch_scatter_dense(hipStream_t s, int64_t m, const int* ridx, const int* 
 void launch_symmetrize_lower(hipStream_t s, int K, double* H);
 
 void launch_gather_nodes(hipStream_t s, int K, const int* nodes, const double* q, double* out);
-void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G);
+template <class TG>
+void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, TG* G);
 // ub[a] = dirs_a . u[cidx[a]] (3-vectors per candidate)
 // ub[a] = d_a . (u + u2)_{c_a} (u2 may be null)
 void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, const double* u2,
@@ -180,8 +181,8 @@ void launch_basis_expand(hipStream_t s, int K, const int* cptr, const int* clist
                          double* F);
 // Gs (ld x ld, column-major, ld a multiple of 64, zero past n) = full symmetric
 // matrix from the lower triangle of G (n x n, column-major).
-template <class TV>
-void launch_sym_full(hipStream_t s, int n, int64_t ld, const double* G, TV* Gs);
+template <class TG, class TV>
+void launch_sym_full(hipStream_t s, int n, int64_t ld, const TG* G, TV* Gs);
 // y = Gs x for the full symmetric Gs above.
 template <class TV>
 void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y);

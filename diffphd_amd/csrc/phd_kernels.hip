@@ -1407,14 +1407,16 @@ __global__ void k_gather_nodes(int K, const int* __restrict__ nodes, const doubl
     out[3 * c + 1] = q[3 * v + 1];
     out[3 * c + 2] = q[3 * v + 2];
 }
-// G[a][b] = (d_a . d_b) H[c_a][c_b]: Gram of node-local directions under A^-1.
+// G[a][b] = (d_a . d_b) H[c_a][c_b]: Gram of node-local directions under A^-1
+// (computed in FP64, stored as TG).
+template <class TG>
 __global__ void k_assemble_g(int nb, const int* __restrict__ cidx, const double* __restrict__ dirs,
-                             const double* __restrict__ H, int K, double* __restrict__ G) {
+                             const double* __restrict__ H, int K, TG* __restrict__ G) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nb * nb) return;
     const int a = (int)(idx / nb), b = (int)(idx % nb);
     const double dd = dirs[3 * a] * dirs[3 * b] + dirs[3 * a + 1] * dirs[3 * b + 1] + dirs[3 * a + 2] * dirs[3 * b + 2];
-    G[idx] = dd * H[(int64_t)cidx[a] * K + cidx[b]];
+    G[idx] = (TG)(dd * H[(int64_t)cidx[a] * K + cidx[b]]);
 }
 
 // ub[a] = d_a . u[c_a]: the candidate vectors projected on the basis rows,
@@ -1706,20 +1708,20 @@ namespace {
 // Gs (ld x ld, column-major, zero past n) = the full symmetric matrix whose
 // lower triangle G (n x n, column-major) holds; 32 x 32 tiles through LDS so
 // that the mirrored reads stay coalesced.
-template <class TV>
-__global__ void __launch_bounds__(256) k_sym_full(int n, int64_t ld, const double* __restrict__ G, TV* __restrict__ Gs) {
-    __shared__ double t[32][33];  // t[q][i] = G(32 sr + i, 32 sc + q), lower entries only
+template <class TG, class TV>
+__global__ void __launch_bounds__(256) k_sym_full(int n, int64_t ld, const TG* __restrict__ G, TV* __restrict__ Gs) {
+    __shared__ TG t[32][33];  // t[q][i] = G(32 sr + i, 32 sc + q), lower entries only
     const int tr = blockIdx.x, tc = blockIdx.y;
     const int lx = threadIdx.x % 32, ly = threadIdx.x / 32;
     const int sr = max(tr, tc), sc = min(tr, tc);
     for (int q = ly; q < 32; q += 8) {
         const int r = 32 * sr + lx, c = 32 * sc + q;
-        t[q][lx] = (r < n && c < n && r >= c) ? G[r + (int64_t)c * n] : 0.0;
+        t[q][lx] = (r < n && c < n && r >= c) ? G[r + (int64_t)c * n] : TG(0);
     }
     __syncthreads();
     for (int q = ly; q < 32; q += 8) {
         const int r = 32 * tr + lx, c = 32 * tc + q;
-        const double v = (tr > tc || (tr == tc && lx >= q)) ? t[q][lx] : t[lx][q];
+        const TG v = (tr > tc || (tr == tc && lx >= q)) ? t[q][lx] : t[lx][q];
         Gs[r + (int64_t)c * ld] = (TV)v;
     }
 }
@@ -1760,26 +1762,30 @@ __global__ void __launch_bounds__(256) k_gemv_sym(int n, int64_t ld, const TV* _
 }
 }  // namespace
 
-template <class TV>
-void launch_sym_full(hipStream_t s, int n, int64_t ld, const double* G, TV* Gs) {
+template <class TG, class TV>
+void launch_sym_full(hipStream_t s, int n, int64_t ld, const TG* G, TV* Gs) {
     if (n == 0) return;
     const unsigned t = (unsigned)(ld / 32);
-    hipLaunchKernelGGL(k_sym_full<TV>, dim3(t, t), dim3(256), 0, s, n, ld, G, Gs);
+    hipLaunchKernelGGL((k_sym_full<TG, TV>), dim3(t, t), dim3(256), 0, s, n, ld, G, Gs);
 }
 template <class TV>
 void launch_gemv_sym(hipStream_t s, int n, int64_t ld, const TV* Gs, const double* x, double* y) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_gemv_sym<TV>, dim3((n + 15) / 16), dim3(256), 0, s, n, ld, Gs, x, y);
 }
-template void launch_sym_full<float>(hipStream_t, int, int64_t, const double*, float*);
-template void launch_sym_full<double>(hipStream_t, int, int64_t, const double*, double*);
+template void launch_sym_full<double, float>(hipStream_t, int, int64_t, const double*, float*);
+template void launch_sym_full<double, double>(hipStream_t, int, int64_t, const double*, double*);
+template void launch_sym_full<float, float>(hipStream_t, int, int64_t, const float*, float*);
 template void launch_gemv_sym<float>(hipStream_t, int, int64_t, const float*, const double*, double*);
 template void launch_gemv_sym<double>(hipStream_t, int, int64_t, const double*, const double*, double*);
 
-void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, double* G) {
+template <class TG>
+void launch_assemble_g(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* H, int K, TG* G) {
     if (nb == 0) return;
-    hipLaunchKernelGGL(k_assemble_g, dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
+    hipLaunchKernelGGL((k_assemble_g<TG>), dim3(blocks_for((int64_t)nb * nb, 256)), dim3(256), 0, s, nb, cidx, dirs, H, K, G);
 }
+template void launch_assemble_g<double>(hipStream_t, int, const int*, const double*, const double*, int, double*);
+template void launch_assemble_g<float>(hipStream_t, int, const int*, const double*, const double*, int, float*);
 void launch_basis_project(hipStream_t s, int nb, const int* cidx, const double* dirs, const double* u, const double* u2,
                           double* ub) {
     if (nb == 0) return;

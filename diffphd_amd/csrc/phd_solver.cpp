@@ -2115,27 +2115,57 @@ void Body::backward(const double* q, const double* v, const double* f_ext, doubl
                 }
                 std::unique_ptr<PhaseTimer> tbf(new PhaseTimer(this, kBwdBasisFactor));
                 phase[kBasisRows] += nb;
-                d_G.alloc((size_t)nb * nb);
-                launch_assemble_g(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, d_G.p);
                 if ((int)d_rhsK_.n < nb) d_rhsK_.alloc(nb);
-                check(rocsolver_dpotrf((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) == rocblas_status_success,
-                      "dpotrf (basis) failed");
-                int info = 0;
-                hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
-                hip_check(hipStreamSynchronize(stream_), "basis factor");
-                check(info == 0, "Contact basis Gram matrix is not positive definite.");
-                check(rocsolver_dpotri((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) == rocblas_status_success,
-                      "dpotri (basis) failed");
-                hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
-                hip_check(hipStreamSynchronize(stream_), "basis inverse");
-                check(info == 0, "Contact basis Gram matrix is singular.");
                 gld = ((int64_t)nb + 63) / 64 * 64;
-                if (precision == 1) {
-                    d_Gs32.alloc((size_t)(gld * gld));
-                    launch_sym_full<float>(stream_, nb, gld, d_G.p, d_Gs32.p);
-                } else {
-                    d_Gs64.alloc((size_t)(gld * gld));
-                    launch_sym_full<double>(stream_, nb, gld, d_G.p, d_Gs64.p);
+                int info = 0;
+                // Mixed precision factors and inverts G in FP32 (the fixed point applies
+                // an FP32 copy of G^-1 anyway, and its constraint residual rides along,
+                // so the iterates still converge onto J s = 0); FP64 when the FP32
+                // factor breaks down or with phd_gram32 = 0.
+                bool g32 = precision == 1 && opt_or(opt, "phd_gram32", 1.0) != 0.0;
+                if (g32) {
+                    DevBuf<float> dGf;
+                    dGf.alloc((size_t)nb * nb);
+                    launch_assemble_g<float>(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, dGf.p);
+                    check(rocsolver_spotrf((rocblas_handle)blas_, rocblas_fill_lower, nb, dGf.p, nb, d_info_.p) ==
+                              rocblas_status_success, "spotrf (basis) failed");
+                    hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                    hip_check(hipStreamSynchronize(stream_), "basis factor");
+                    if (info == 0) {
+                        check(rocsolver_spotri((rocblas_handle)blas_, rocblas_fill_lower, nb, dGf.p, nb, d_info_.p) ==
+                                  rocblas_status_success, "spotri (basis) failed");
+                        hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                        hip_check(hipStreamSynchronize(stream_), "basis inverse");
+                    }
+                    if (info == 0) {
+                        d_Gs32.alloc((size_t)(gld * gld));
+                        launch_sym_full<float, float>(stream_, nb, gld, dGf.p, d_Gs32.p);
+                        hip_check(hipStreamSynchronize(stream_), "basis sym");
+                    } else {
+                        g32 = false;
+                        ++last_bwd.gram_fp64_fallbacks;
+                    }
+                }
+                if (!g32) {
+                    d_G.alloc((size_t)nb * nb);
+                    launch_assemble_g<double>(stream_, nb, d_bcidx.p, d_dirs.p, d_H_.p, K_, d_G.p);
+                    check(rocsolver_dpotrf((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) ==
+                              rocblas_status_success, "dpotrf (basis) failed");
+                    hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                    hip_check(hipStreamSynchronize(stream_), "basis factor");
+                    check(info == 0, "Contact basis Gram matrix is not positive definite.");
+                    check(rocsolver_dpotri((rocblas_handle)blas_, rocblas_fill_lower, nb, d_G.p, nb, d_info_.p) ==
+                              rocblas_status_success, "dpotri (basis) failed");
+                    hip_check(hipMemcpyAsync(&info, d_info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+                    hip_check(hipStreamSynchronize(stream_), "basis inverse");
+                    check(info == 0, "Contact basis Gram matrix is singular.");
+                    if (precision == 1) {
+                        d_Gs32.alloc((size_t)(gld * gld));
+                        launch_sym_full<double, float>(stream_, nb, gld, d_G.p, d_Gs32.p);
+                    } else {
+                        d_Gs64.alloc((size_t)(gld * gld));
+                        launch_sym_full<double, double>(stream_, nb, gld, d_G.p, d_Gs64.p);
+                    }
                 }
                 d_Gx.alloc(nb);
                 tbf.reset();

@@ -247,7 +247,8 @@ public:
     int last_backward_iters = 0;
     struct BackwardStats {  // of the last backward call
         double backbone_iters = 0, backbone_converged = 0, clipped = 0, woodbury_iters = 0, woodbury_converged = 0,
-               basis_rows = 0, filtered = 0, samples = 0, db_norm = 0, trust_stage = 0, from_tape = 1;
+               basis_rows = 0, filtered = 0, samples = 0, db_norm = 0, trust_stage = 0, from_tape = 1,
+               gram_fp64_fallbacks = 0;
     } last_bwd;
     long long factor_bytes() const;
     long long factor_nnz() const { return factor_.nnz; }

@@ -430,10 +430,10 @@ class _Deformable:
         return _lib.lib().phd_last_backward_iters(self._ctx)
 
     def last_backward_stats(self):
-        out, op = _lib.dptr(np.zeros(11))
-        _lib.lib().phd_last_backward_stats(self._ctx, op, 11)
+        out, op = _lib.dptr(np.zeros(12))
+        _lib.lib().phd_last_backward_stats(self._ctx, op, 12)
         keys = ("backbone_iters", "backbone_converged", "clipped", "woodbury_iters", "woodbury_converged",
-                "basis_rows", "filtered", "samples", "db_norm", "trust_stage", "from_tape")
+                "basis_rows", "filtered", "samples", "db_norm", "trust_stage", "from_tape", "gram_fp64_fallbacks")
         return dict(zip(keys, out.tolist()))
 
     def last_contact_multipliers(self):

@@ -191,8 +191,10 @@ int phd_last_backward_iters(phd_ctx ctx);
  * [6] samples whose proximal-map block the K_pd filter changed, [7] samples,
  * [8] |db|, [9] trust-region stage that produced db (0 unfiltered, 1 clamp,
  * 2 abs), [10] 1 when the contact data came from the forward tape, 0 when the
- * frame was replayed over the given active set (backward_gpu.cpp:113-418).
- * Returns the number of entries (11). [0] counts every stage. */
+ * frame was replayed over the given active set (backward_gpu.cpp:113-418),
+ * [11] 1 when the mixed path's FP32 factor of the basis Gram matrix broke down
+ * and the FP64 one was used. Returns the number of entries (12). [0] counts
+ * every stage. */
 int phd_last_backward_stats(phd_ctx ctx, double* out, int cap);
 /* Factor description: out[0] nnz(S), [1] padded value count, [2] max etree depth,
  * [3] ordering seconds, [4] factor seconds, [5] setup seconds, [6] row units,
