@@ -1451,24 +1451,89 @@ __global__ void k_basis_expand(int K, const int* __restrict__ cptr, const int* _
 }
 
 // out[3a + d] = sum_b H[rows[a]][cols[b]] f[3 cols[b] + d]: the candidate block
-// of A^-1 applied to impulses supported on `cols` (one wavefront per row).
+// of A^-1 applied to impulses supported on `cols`. A workgroup of kHWaves
+// wavefronts takes kHRows rows: wave w walks the columns b = w WAVE + lane,
+// stepping kHWaves WAVE, and loads each column index and f_c once for all
+// kHRows rows; kHCols columns are in flight per step, with the next step's
+// indices loaded one step ahead. The waves' partial sums are added in wave
+// order through LDS, so the result does not depend on scheduling.
+constexpr int kHRows = 4;
+constexpr int kHWaves = 4;
+constexpr int kHCols = 4;
 template <class TH>
-__global__ void k_h_apply_idx(int nr, const int* __restrict__ rows, int nc, const int* __restrict__ cols,
-                              const TH* __restrict__ H, int K, const double* __restrict__ f, double* __restrict__ out) {
-    const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    if (a >= nr) return;
-    const TH* Hr = H + (int64_t)rows[a] * K;
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int b = lane; b < nc; b += WAVE) {
-        const int c = cols[b];
-        const double h = (double)Hr[c];
-        a0 += h * f[3 * c];
-        a1 += h * f[3 * c + 1];
-        a2 += h * f[3 * c + 2];
+__device__ __forceinline__ void h_apply_rows(int a0, int nr, const int* __restrict__ rows, int nc,
+                                             const int* __restrict__ cols, const TH* __restrict__ H, int K,
+                                             const double* __restrict__ f, double* __restrict__ out) {
+    __shared__ double part[kHWaves][kHRows * 3];
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    constexpr int stride = kHWaves * WAVE;
+    const TH* Hr[kHRows];
+#pragma unroll
+    for (int r = 0; r < kHRows; ++r) Hr[r] = H + (int64_t)rows[min(a0 + r, nr - 1)] * K;  // rows past nr: not written
+    double acc[kHRows][3];
+#pragma unroll
+    for (int r = 0; r < kHRows; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    int b = w * WAVE + lane;
+    int cn[kHCols];  // the next step's column indices
+#pragma unroll
+    for (int u = 0; u < kHCols; ++u) cn[u] = b + u * stride < nc ? cols[b + u * stride] : 0;
+    for (; b + (kHCols - 1) * stride < nc; b += kHCols * stride) {
+        int c[kHCols];
+#pragma unroll
+        for (int u = 0; u < kHCols; ++u) {
+            c[u] = cn[u];
+            const int bn = b + (kHCols + u) * stride;
+            cn[u] = bn < nc ? cols[bn] : 0;
+        }
+        double fv[kHCols][3];
+        TH h[kHCols][kHRows];
+#pragma unroll
+        for (int u = 0; u < kHCols; ++u) {
+            fv[u][0] = f[3 * c[u]]; fv[u][1] = f[3 * c[u] + 1]; fv[u][2] = f[3 * c[u] + 2];
+#pragma unroll
+            for (int r = 0; r < kHRows; ++r) h[u][r] = Hr[r][c[u]];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // every index, H and f load is in flight before the first product
+#pragma unroll
+        for (int u = 0; u < kHCols; ++u)
+#pragma unroll
+            for (int r = 0; r < kHRows; ++r) {
+                const double hv = (double)h[u][r];
+                acc[r][0] += hv * fv[u][0]; acc[r][1] += hv * fv[u][1]; acc[r][2] += hv * fv[u][2];
+            }
     }
-    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
-    if (lane == 0) { out[3 * a] = a0; out[3 * a + 1] = a1; out[3 * a + 2] = a2; }
+    for (; b < nc; b += stride) {
+        const int c0 = cols[b];
+        const double f00 = f[3 * c0], f01 = f[3 * c0 + 1], f02 = f[3 * c0 + 2];
+#pragma unroll
+        for (int r = 0; r < kHRows; ++r) {
+            const double hv = (double)Hr[r][c0];
+            acc[r][0] += hv * f00; acc[r][1] += hv * f01; acc[r][2] += hv * f02;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kHRows; ++r)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double v = wave_sum(acc[r][d]);
+            if (lane == 0) part[w][3 * r + d] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < kHRows * 3 && a0 + (int)threadIdx.x / 3 < nr) {
+        double v = part[0][threadIdx.x];
+#pragma unroll
+        for (int q = 1; q < kHWaves; ++q) v += part[q][threadIdx.x];
+        out[3 * a0 + threadIdx.x] = v;
+    }
+}
+template <class TH>
+__global__ __launch_bounds__(kHWaves * WAVE) void k_h_apply_idx(int nr, const int* __restrict__ rows, int nc,
+                                                                const int* __restrict__ cols, const TH* __restrict__ H,
+                                                                int K, const double* __restrict__ f,
+                                                                double* __restrict__ out) {
+    const int a0 = blockIdx.x * kHRows;
+    if (a0 >= nr) return;  // uniform over the workgroup
+    h_apply_rows<TH>(a0, nr, rows, nc, cols, H, K, f, out);
 }
 // Compressed contact system on the index set J: M[a][b] = om_i (n_i . n_j) H[i][j] om_j + (a == b) E_i.
 template <class TM>
@@ -1800,7 +1865,8 @@ template <class TH>
 void launch_h_apply_idx(hipStream_t s, int nr, const int* rows, int nc, const int* cols, const TH* H, int K,
                         const double* f, double* out) {
     if (nr == 0) return;
-    hipLaunchKernelGGL(k_h_apply_idx<TH>, dim3(blocks_for(nr, 4)), dim3(256), 0, s, nr, rows, nc, cols, H, K, f, out);
+    hipLaunchKernelGGL(k_h_apply_idx<TH>, dim3(blocks_for(nr, kHRows)), dim3(kHWaves * WAVE), 0, s, nr, rows, nc, cols, H, K,
+                       f, out);
 }
 template void launch_h_apply_idx<float>(hipStream_t, int, const int*, int, const int*, const float*, int, const double*,
                                         double*);
@@ -2293,24 +2359,17 @@ __global__ void k_ncp_impulses(int K, const double* __restrict__ om, const doubl
 // k_h_apply_idx with the row and column counts read on the device (grid sized
 // for the largest possible row list).
 template <class TH>
-__global__ void k_h_apply_idx_dev(const int* __restrict__ nr_dev, const int* __restrict__ rows, const int* __restrict__ nc_dev,
-                                  const int* __restrict__ cols, const TH* __restrict__ H, int K,
-                                  const double* __restrict__ f, double* __restrict__ out) {
-    const int a = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const int lane = threadIdx.x % WAVE;
-    if (a >= *nr_dev) return;
-    const int nc = *nc_dev;
-    const TH* Hr = H + (int64_t)rows[a] * K;
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int b = lane; b < nc; b += WAVE) {
-        const int c = cols[b];
-        const double h = (double)Hr[c];
-        a0 += h * f[3 * c];
-        a1 += h * f[3 * c + 1];
-        a2 += h * f[3 * c + 2];
-    }
-    a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
-    if (lane == 0) { out[3 * a] = a0; out[3 * a + 1] = a1; out[3 * a + 2] = a2; }
+__global__ __launch_bounds__(kHWaves * WAVE) void k_h_apply_idx_dev(const int* __restrict__ nr_dev,
+                                                                    const int* __restrict__ rows,
+                                                                    const int* __restrict__ nc_dev,
+                                                                    const int* __restrict__ cols,
+                                                                    const TH* __restrict__ H, int K,
+                                                                    const double* __restrict__ f,
+                                                                    double* __restrict__ out) {
+    const int nr = *nr_dev;
+    const int a0 = blockIdx.x * kHRows;
+    if (a0 >= nr) return;  // uniform over the workgroup
+    h_apply_rows<TH>(a0, nr, rows, *nc_dev, cols, H, K, f, out);
 }
 
 }  // namespace
@@ -2367,8 +2426,8 @@ template <class TH>
 void launch_h_apply_idx_dev(hipStream_t s, int max_rows, const int* nr_dev, const int* rows, const int* nc_dev,
                             const int* cols, const TH* H, int K, const double* f, double* out) {
     if (max_rows == 0) return;
-    hipLaunchKernelGGL(k_h_apply_idx_dev<TH>, dim3(blocks_for(max_rows, 4)), dim3(256), 0, s, nr_dev, rows, nc_dev, cols, H,
-                       K, f, out);
+    hipLaunchKernelGGL(k_h_apply_idx_dev<TH>, dim3(blocks_for(max_rows, kHRows)), dim3(kHWaves * WAVE), 0, s, nr_dev, rows,
+                       nc_dev, cols, H, K, f, out);
 }
 template void launch_h_apply_idx_dev<float>(hipStream_t, int, const int*, const int*, const int*, const int*,
                                             const float*, int, const double*, double*);
