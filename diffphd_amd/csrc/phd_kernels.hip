@@ -2198,58 +2198,63 @@ __global__ void k_ncp_sets(int K, const unsigned char* __restrict__ enabled, con
                                (o != 0.0 || inF ? kNcpP : 0));
 }
 
-// Exclusive prefix sum over the workgroup (1024 threads, 16 waves).
-__device__ int block_exclusive_scan(int v, int* sh, int& total) {
+// Lists (in increasing candidate order) of the candidates whose flags contain
+// mask m.m[i] (all bits), i < nl; counts[i] = list sizes; pos[c] = position of
+// c in list 0, -1 when absent. One workgroup; each thread owns a contiguous run
+// of candidates, counts its members of every list in one pass over the flags,
+// and the nl counts are scanned together (one pair of barriers for all lists).
+struct Masks { int m[4]; };
+__global__ void __launch_bounds__(1024) k_ncp_compact(int K, const unsigned char* __restrict__ flags, int nl, Masks mk,
+                                                      int* __restrict__ lists, int* __restrict__ counts,
+                                                      int* __restrict__ pos) {
+    __shared__ int sh[4][16];
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-    int x = v;
+    const int per = (K + blockDim.x - 1) / blockDim.x;
+    const int c0 = min(K, (int)threadIdx.x * per), c1 = min(K, c0 + per);
+    int cnt[4] = {0, 0, 0, 0};
+    for (int c = c0; c < c1; ++c) {
+        const int f = flags[c];
 #pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        const int y = __shfl_up(x, o, WAVE);
-        if (lane >= o) x += y;
+        for (int i = 0; i < 4; ++i) cnt[i] += i < nl && (f & mk.m[i]) == mk.m[i];
     }
-    if (lane == WAVE - 1) sh[w] = x;
+    // inclusive scan of the four counts within the wavefront, then over the wavefront totals
+    int x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        x[i] = cnt[i];
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(x[i], o, WAVE);
+            if (lane >= o) x[i] += y;
+        }
+        if (lane == WAVE - 1) sh[i][w] = x[i];
+    }
     __syncthreads();
-    if (w == 0) {
-        int t = lane < nw ? sh[lane] : 0;
+    if (w < 4) {  // wavefront i scans the totals of list i
+        int t = lane < nw ? sh[w][lane] : 0;
 #pragma unroll
         for (int o = 1; o < WAVE; o <<= 1) {
             const int y = __shfl_up(t, o, WAVE);
             if (lane >= o) t += y;
         }
-        if (lane < nw) sh[lane] = t;
+        if (lane < nw) sh[w][lane] = t;
     }
     __syncthreads();
-    const int base = w > 0 ? sh[w - 1] : 0;
-    total = sh[nw - 1];
-    __syncthreads();
-    return base + x - v;
-}
-
-// Lists (in increasing candidate order) of the candidates whose flags contain
-// mask m.m[i] (all bits), i < nl; counts[i] = list sizes; pos[c] = position of
-// c in list 0, -1 when absent. One workgroup; each thread owns a contiguous run.
-struct Masks { int m[4]; };
-__global__ void __launch_bounds__(1024) k_ncp_compact(int K, const unsigned char* __restrict__ flags, int nl, Masks mk,
-                                                      int* __restrict__ lists, int* __restrict__ counts,
-                                                      int* __restrict__ pos) {
-    __shared__ int sh[16];
-    const int per = (K + blockDim.x - 1) / blockDim.x;
-    const int c0 = min(K, (int)threadIdx.x * per), c1 = min(K, c0 + per);
-    for (int i = 0; i < nl; ++i) {
-        const int m = mk.m[i];
-        int cnt = 0;
-        for (int c = c0; c < c1; ++c) cnt += (flags[c] & m) == m;
-        int total = 0;
-        int at = block_exclusive_scan(cnt, sh, total);
-        int* L = lists + (int64_t)i * K;
-        for (int c = c0; c < c1; ++c) {
-            const bool in = (flags[c] & m) == m;
-            if (in) L[at] = c;
-            if (i == 0) pos[c] = in ? at : -1;
-            at += in;
+    int at[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) at[i] = (w > 0 ? sh[i][w - 1] : 0) + x[i] - cnt[i];
+    for (int c = c0; c < c1; ++c) {
+        const int f = flags[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= nl) break;
+            const bool in = (f & mk.m[i]) == mk.m[i];
+            if (in) lists[(int64_t)i * K + at[i]] = c;
+            if (i == 0) pos[c] = in ? at[0] : -1;
+            at[i] += in;
         }
-        if (threadIdx.x == 0) counts[i] = total;
     }
+    if (threadIdx.x < nl) counts[threadIdx.x] = sh[threadIdx.x][nw - 1];
 }
 
 // For a < nP (c = P[a]): uC_c -= (H F_prev)_a; for c in J (a' = pos_J[c]) the
@@ -2385,7 +2390,8 @@ void launch_ncp_sets(hipStream_t s, int K, const unsigned char* enabled, const d
 void launch_ncp_compact(hipStream_t s, int K, const unsigned char* flags, int nl, const int* masks, int* lists, int* counts,
                         int* pos) {
     Masks mk = {{0, 0, 0, 0}};
-    for (int i = 0; i < nl && i < 4; ++i) mk.m[i] = masks[i];
+    nl = nl < 4 ? nl : 4;
+    for (int i = 0; i < nl; ++i) mk.m[i] = masks[i];
     hipLaunchKernelGGL(k_ncp_compact, dim3(1), dim3(1024), 0, s, K, flags, nl, mk, lists, counts, pos);
 }
 void launch_ncp_rhs(hipStream_t s, int nP, const int* P, const double* hprev, const int* posJ, const double* hg,
