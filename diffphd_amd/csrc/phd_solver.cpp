@@ -193,6 +193,7 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
     }
     energies_.clear();
     pcontacts_.clear();
+    acontacts_.clear();
     gravity_off_ = n_state_params_ = 0;
     muscles_.clear();
     act_dofs_ = 0;
@@ -306,6 +307,42 @@ void Body::add_planar_contact(const std::vector<double>& p) {
     pcontacts_.push_back(c);
 }
 
+// ArcContactStateForce::Initialize (arc_contact_state_force.cpp:18-30): params
+// [center (3), dir (3), start (3), radius, angle, p, kn, kf, mu]
+// (deformable_state_force.cpp:33-49); aux = dir x start.
+void Body::add_arc_contact(const std::vector<double>& p) {
+    check(p.size() == 15, "Inconsistent params for ArcContactStateForce.");
+    ArcContactForce a;
+    const double dn = std::sqrt(p[3] * p[3] + p[4] * p[4] + p[5] * p[5]);
+    const double sn = std::sqrt(p[6] * p[6] + p[7] * p[7] + p[8] * p[8]);
+    for (int i = 0; i < 3; ++i) { a.c[i] = p[i]; a.dir[i] = p[3 + i] / dn; a.start[i] = p[6 + i] / sn; }
+    for (int i = 0; i < 3; ++i) a.aux[i] = a.dir[(i + 1) % 3] * a.start[(i + 2) % 3] - a.dir[(i + 2) % 3] * a.start[(i + 1) % 3];
+    a.R = p[9]; a.angle = p[10]; a.p = (int)p[11];
+    a.kn = p[12]; a.kf = p[13]; a.mu = p[14];
+    a.poff = n_state_params_;
+    n_state_params_ += 3;
+    acontacts_.push_back(a);
+}
+
+namespace {
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// A vertex x against an arc (arc_contact_state_force.cpp:32-62): qp is x - c
+// without its axial part; contact when the angle of qp from `start` toward `aux`
+// lies in (0, angle). Returns |qp| and n = -qp / (|qp| + eps), or false.
+template <class Arc>
+bool arc_hit(const Arc& a, const double* x, double* qp, double& r, double* n) {
+    double ql[3];
+    for (int k = 0; k < 3; ++k) ql[k] = x[k] - a.c[k];
+    const double ax = dot3(ql, a.dir);
+    for (int k = 0; k < 3; ++k) qp[k] = ql[k] - ax * a.dir[k];
+    const double th = std::atan2(dot3(qp, a.aux), dot3(qp, a.start));
+    if (!(th > 0 && th < a.angle)) return false;
+    r = std::sqrt(dot3(qp, qp));
+    for (int k = 0; k < 3; ++k) n[k] = -qp[k] / (r + DBL_EPSILON);
+    return true;
+}
+}  // namespace
+
 // ForwardStateForce (deformable_state_force.cpp:114-125): gravity m g
 // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
 // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s
@@ -325,6 +362,18 @@ void Body::state_force(const double* q, const double* v, double* f) const {
             const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
             const double cf = std::min(c.kf, c.mu * fn / (um + DBL_EPSILON));
             for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] += fn * c.n[k] - cf * us[k];
+        }
+    for (const auto& a : acontacts_)
+        for (int i = 0; i < nv_; ++i) {
+            double qp[3], r, n[3];
+            if (!arc_hit(a, q + 3 * (size_t)i, qp, r, n)) continue;
+            const double fn = a.kn * std::pow(-std::min(0.0, a.R - r), a.p - 1);
+            const double* vi = v + 3 * (size_t)i;
+            const double vn = dot3(vi, n);
+            double us[3];
+            for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * n[k];
+            const double cf = std::min(a.kf, a.mu * fn / (std::sqrt(dot3(us, us)) + DBL_EPSILON));
+            for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] += fn * n[k] - cf * us[k];
         }
 }
 
@@ -366,6 +415,73 @@ void Body::state_force_backward(const double* q, const double* v, const double* 
                 }
                 dl_dp[c.poff] += -c.mu * gu / ue * (fn / c.kn);
                 dl_dp[c.poff + 2] += -gu / ue * fn;
+            }
+        }
+    // arcs (arc_contact_state_force.cpp:65-131) as vector-Jacobian products, with
+    // P = I - dir dir^T, s = 1 / (|qp| + eps) and the reference's dn/dq = -s (I - n n^T) P,
+    // so (dn/dq)^T w = -s P (w - (n . w) n)
+    for (const auto& a : acontacts_)
+        for (int i = 0; i < nv_; ++i) {
+            double qp[3], r, n[3];
+            if (!arc_hit(a, q + 3 * (size_t)i, qp, r, n)) continue;
+            const double* vi = v + 3 * (size_t)i;
+            const double* g = dl_df + 3 * (size_t)i;
+            const double sc = 1.0 / (r + DBL_EPSILON), d = a.R - r, pen = -std::min(0.0, d);
+            const double fn = a.kn * std::pow(pen, a.p - 1);
+            auto proj_axis = [&](const double* w, double* out) {  // P w
+                const double t = dot3(w, a.dir);
+                for (int k = 0; k < 3; ++k) out[k] = w[k] - t * a.dir[k];
+            };
+            auto dn_T = [&](const double* w, double* out) {  // (dn/dq)^T w
+                const double t = dot3(n, w);
+                double u[3];
+                for (int k = 0; k < 3; ++k) u[k] = w[k] - t * n[k];
+                proj_axis(u, out);
+                for (int k = 0; k < 3; ++k) out[k] *= -sc;
+            };
+            // d f_n / dq = kn (p-1) pen^(p-2) P qp / (|qp| + eps) inside the arc's radius
+            double dfn[3];
+            proj_axis(qp, dfn);
+            const double c1 = d < 0 ? a.kn * (a.p - 1) * std::pow(pen, a.p - 2) * sc : 0.0;
+            for (int k = 0; k < 3; ++k) dfn[k] *= c1;
+            const double vn = dot3(vi, n), gn = dot3(g, n);
+            double us[3];
+            for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * n[k];
+            const double ue = std::sqrt(dot3(us, us)) + DBL_EPSILON, gu = dot3(g, us);
+            // normal part f_n n
+            double tq[3];
+            dn_T(g, tq);
+            for (int k = 0; k < 3; ++k) dl_dq[3 * (size_t)i + k] += gn * dfn[k] + fn * tq[k];
+            dl_dp[a.poff] += gn * fn / a.kn;
+            // (du_s/dq)^T g = -(dn/dq)^T (v (n . g) + (v . n) g); du_s/dv = I - n n^T
+            double w[3], dusT_g[3], gt[3];
+            for (int k = 0; k < 3; ++k) w[k] = vi[k] * gn + vn * g[k];
+            dn_T(w, dusT_g);
+            for (int k = 0; k < 3; ++k) { dusT_g[k] = -dusT_g[k]; gt[k] = g[k] - gn * n[k]; }
+            if (a.kf <= a.mu * fn / ue) {  // ff = -kf u_s
+                for (int k = 0; k < 3; ++k) {
+                    dl_dq[3 * (size_t)i + k] += -a.kf * dusT_g[k];
+                    dl_dv[3 * (size_t)i + k] += -a.kf * gt[k];
+                }
+                dl_dp[a.poff + 1] += -gu;
+            } else {  // ff = -mu f_n u_s / (|u_s| + eps)
+                // the reference's d|u_s|/dq is (du_s/dq) u_s / (|u_s| + eps) (not transposed):
+                // du_s/dq x = -(n (v . y) + (v . n) y) with y = (dn/dq) x = -s (I - n n^T) P x
+                double pu[3], y[3], dum_q[3], dum_v[3];
+                proj_axis(us, pu);
+                const double npu = dot3(n, pu);
+                for (int k = 0; k < 3; ++k) y[k] = -sc * (pu[k] - npu * n[k]);
+                const double vy = dot3(vi, y), nus = dot3(n, us);
+                for (int k = 0; k < 3; ++k) {
+                    dum_q[k] = -(n[k] * vy + vn * y[k]) / ue;
+                    dum_v[k] = (us[k] - nus * n[k]) / ue;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    dl_dq[3 * (size_t)i + k] += -a.mu * (dfn[k] * gu / ue + fn * (dusT_g[k] / ue - dum_q[k] * gu / (ue * ue)));
+                    dl_dv[3 * (size_t)i + k] += -a.mu * fn * (gt[k] / ue - dum_v[k] * gu / (ue * ue));
+                }
+                dl_dp[a.poff] += -gu / ue * a.mu * fn / a.kn;
+                dl_dp[a.poff + 2] += -gu / ue * fn;
             }
         }
 }

@@ -169,6 +169,7 @@ public:
     void add_gravity(const double* g);
     // AddStateForce("planar_contact", [n (3), offset, p, kn, kf, mu]) (PlanarContactStateForce)
     void add_planar_contact(const std::vector<double>& p);
+    void add_arc_contact(const std::vector<double>& p);
     int num_state_params() const { return n_state_params_; }
     // ForwardStateForce / BackwardStateForce at (q, v) on the host (3N vectors)
     void state_force(const double* q, const double* v, double* f) const;
@@ -332,6 +333,13 @@ private:
         int poff = 0;  // its parameters' offset in dl_dstate_p
     };
     std::vector<PlanarContactForce> pcontacts_;
+    struct ArcContactForce {  // inside of a cylindrical arc about the axis (c, dir)
+        double c[3], dir[3], start[3], aux[3], R = 0, angle = 0;
+        int p = 2;
+        double kn = 0, kf = 0, mu = 0;
+        int poff = 0;
+    };
+    std::vector<ArcContactForce> acontacts_;
     int gravity_off_ = 0, n_state_params_ = 0;
     HostBoundary boundary_;
     std::map<int, int> fb_vertices_;

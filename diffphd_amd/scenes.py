@@ -104,6 +104,21 @@ def penalty_drop(cells=(4, 2, 2), dx=0.01, E=1e5, nu=0.45, rho=1000.0, kn=5.0, k
     }
 
 
+def arc_drop(kn=5.0, kf=0.005, mu=0.4, seed=0):
+    """The penalty_drop block in a cylindrical trough (ArcContactStateForce,
+    AddStateForce("arc_contact", [center, dir, start, radius, angle, p, kn, kf,
+    mu]), arc_contact_state_force.cpp): axis along x through (0, 0.01, 0.05),
+    the arc spanning 90 degrees centred on straight down (start (0, -1, -1),
+    toward aux = dir x start = (0, 1, -1) / sqrt 2), radius 0.0515, so the bottom
+    vertices (0.052-0.053 from the axis) start just outside it."""
+    spec = penalty_drop(kn=kn, kf=kf, mu=mu, seed=seed)
+    s = 1.0 / np.sqrt(2.0)
+    spec["name"] = "arc_drop"
+    spec["state_forces"] = [("arc_contact", [0.0, 0.01, 0.05, 1.0, 0.0, 0.0, 0.0, -s, -s, 0.0515, np.pi / 2, 2,
+                                             kn, kf, mu])]
+    return spec
+
+
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):
     """Heterogeneous napkin dropped on a sphere (python/example/napkin_3d_test.py:32-45,
     python/py_diff_pd/env/napkin_env_3d.py:19-79): stiff half / 0.1x soft half

@@ -40,7 +40,9 @@ int phd_add_pd_energy(phd_ctx ctx, const char* type, const double* params, int n
 int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
 /* Deformable::AddStateForce (deformable_state_force.cpp:10-31): "gravity" [gx, gy, gz]
  * or "planar_contact" [nx, ny, nz, offset, p, kn, kf, mu] (a penalty floor with
- * friction, planar_contact_state_force.cpp). */
+ * friction, planar_contact_state_force.cpp), or "arc_contact" [center (3), dir (3),
+ * start (3), radius, angle, p, kn, kf, mu] (the inside of a cylindrical arc,
+ * deformable_state_force.cpp:33-49, arc_contact_state_force.cpp). */
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
 /* Deformable::NumOfStateForceParameters(): the length of dl_dstate_p (3 per
  * state force, in AddStateForce order). */

@@ -53,6 +53,7 @@ def lib():
         L.orc_set_frictional_boundary.argtypes = [P, cp, dp, ctypes.c_int, ip, ctypes.c_int]
         L.orc_set_press_plane.argtypes = [P, ctypes.c_double]
         L.orc_add_planar_contact.argtypes = [P, dp, ctypes.c_int]
+        L.orc_add_arc_contact.argtypes = [P, dp, ctypes.c_int]
         L.orc_num_state_params.argtypes = [P]
         L.orc_state_force.argtypes = [P, dp, dp, dp]
         L.orc_state_force_backward.argtypes = [P, dp, dp, dp, dp, dp, dp]
@@ -169,6 +170,8 @@ class OracleDeformable:
             self._ok(lib().orc_add_gravity(self._h, pp))
         elif force_type == "planar_contact":
             self._ok(lib().orc_add_planar_contact(self._h, pp, len(params)))
+        elif force_type == "arc_contact":
+            self._ok(lib().orc_add_arc_contact(self._h, pp, len(params)))
         else:
             raise RuntimeError("Unsupported state force: " + str(force_type))
 

@@ -902,6 +902,16 @@ struct Body {
         int poff = 0;
     };
     std::vector<PlanarContact> pcontacts;
+    // ArcContactStateForce (arc_contact_state_force.cpp:18-30): the inside of a
+    // cylindrical arc of radius R about the axis through c along unit `dir`,
+    // covering the angles (0, angle) from unit `start` toward aux = dir x start
+    struct ArcContact {
+        double c[3], dir[3], start[3], aux[3], R = 0, angle = 0;
+        int p = 2;
+        double kn = 0, kf = 0, mu = 0;
+        int poff = 0;
+    };
+    std::vector<ArcContact> acontacts;
     int gravity_off = 0, n_state_params = 0;
     Boundary boundary;
     std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
@@ -1144,6 +1154,39 @@ struct Body {
         n_state_params += 3;
         pcontacts.push_back(c);
     }
+    // AddStateForce("arc_contact", [center (3), dir (3), start (3), radius, angle,
+    // p, kn, kf, mu]) (deformable_state_force.cpp:33-49, arc_contact_state_force.cpp:18-30).
+    void add_arc_contact(const Vec& prm) {
+        check(prm.size() == 15, "Inconsistent params for ArcContactStateForce.");
+        ArcContact a;
+        const double dn = std::sqrt(prm[3] * prm[3] + prm[4] * prm[4] + prm[5] * prm[5]);
+        const double sn = std::sqrt(prm[6] * prm[6] + prm[7] * prm[7] + prm[8] * prm[8]);
+        for (int i = 0; i < 3; ++i) { a.c[i] = prm[i]; a.dir[i] = prm[3 + i] / dn; a.start[i] = prm[6 + i] / sn; }
+        a.aux[0] = a.dir[1] * a.start[2] - a.dir[2] * a.start[1];
+        a.aux[1] = a.dir[2] * a.start[0] - a.dir[0] * a.start[2];
+        a.aux[2] = a.dir[0] * a.start[1] - a.dir[1] * a.start[0];
+        a.R = prm[9]; a.angle = prm[10]; a.p = (int)prm[11];
+        a.kn = prm[12]; a.kf = prm[13]; a.mu = prm[14];
+        a.poff = n_state_params;
+        n_state_params += 3;
+        acontacts.push_back(a);
+    }
+    // One vertex against an arc (arc_contact_state_force.cpp:32-62): qp = the part of
+    // x - c normal to the axis; in contact when its angle from `start` lies in
+    // (0, angle); d = R - |qp| (negative: penetration), n = -qp / (|qp| + eps).
+    static bool arc_geometry(const ArcContact& a, const double* x, double qp[3], double& r, double n[3]) {
+        double ql[3];
+        for (int k = 0; k < 3; ++k) ql[k] = x[k] - a.c[k];
+        const double ax = ql[0] * a.dir[0] + ql[1] * a.dir[1] + ql[2] * a.dir[2];
+        for (int k = 0; k < 3; ++k) qp[k] = ql[k] - ax * a.dir[k];
+        const double th = std::atan2(qp[0] * a.aux[0] + qp[1] * a.aux[1] + qp[2] * a.aux[2],
+                                     qp[0] * a.start[0] + qp[1] * a.start[1] + qp[2] * a.start[2]);
+        if (!(0 < th && th < a.angle)) return false;
+        r = std::sqrt(qp[0] * qp[0] + qp[1] * qp[1] + qp[2] * qp[2]);
+        const double eps = std::numeric_limits<double>::epsilon();
+        for (int k = 0; k < 3; ++k) n[k] = -qp[k] / (r + eps);
+        return true;
+    }
     // ForwardStateForce (deformable_state_force.cpp:114-125): gravity f_i = mass g
     // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
     // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s,
@@ -1167,6 +1210,19 @@ struct Body {
                 const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
                 const double cf = std::min(c.kf, c.mu * fn / (um + eps));
                 for (int k = 0; k < 3; ++k) f[3 * i + k] += fn * c.n[k] - cf * us[k];
+            }
+        for (const auto& a : acontacts)
+            for (int i = 0; i < nv; ++i) {
+                double qp[3], r, n[3];
+                if (!arc_geometry(a, &q[3 * i], qp, r, n)) continue;
+                const double fn = a.kn * std::pow(-std::min(0.0, a.R - r), a.p - 1);
+                const double* vi = &v[3 * i];
+                const double vn = vi[0] * n[0] + vi[1] * n[1] + vi[2] * n[2];
+                double us[3];
+                for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * n[k];
+                const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+                const double cf = std::min(a.kf, a.mu * fn / (um + eps));
+                for (int k = 0; k < 3; ++k) f[3 * i + k] += fn * n[k] - cf * us[k];
             }
         return f;
     }
@@ -1217,6 +1273,74 @@ struct Body {
                     for (int k = 0; k < 3; ++k) dl_dv[3 * i + k] += -c.mu * fn * (gt[k] / ue - us[k] * gu / (ue * ue * ue));
                     dl_dp[c.poff] += -c.mu * gu / ue * (fn / c.kn);
                     dl_dp[c.poff + 2] += -gu / ue * fn;
+                }
+            }
+        // arcs (arc_contact_state_force.cpp:65-131), as 3x3 Jacobians J[k][j] = d out_k / d in_j
+        for (const auto& a : acontacts)
+            for (int i = 0; i < nv; ++i) {
+                double qp[3], r, n[3];
+                if (!arc_geometry(a, &q[3 * i], qp, r, n)) continue;
+                const double* vi = &v[3 * i];
+                const double* g = &dl_df[3 * i];
+                const double re = r + eps, d = a.R - r, pen = -std::min(0.0, d);
+                const double fn = a.kn * std::pow(pen, a.p - 1);
+                double P[3][3], dn[3][3], dfn_q[3], dus_q[3][3], dus_v[3][3];
+                for (int k = 0; k < 3; ++k)
+                    for (int j = 0; j < 3; ++j) P[k][j] = (k == j) - a.dir[k] * a.dir[j];
+                // d|qp|/dq = P^T qp / (|qp| + eps); d f_n / dq = kn (p-1) pen^(p-2) d|qp|/dq inside
+                for (int j = 0; j < 3; ++j) {
+                    const double dr = (P[0][j] * qp[0] + P[1][j] * qp[1] + P[2][j] * qp[2]) / re;
+                    dfn_q[j] = d < 0 ? a.kn * (a.p - 1) * std::pow(pen, a.p - 2) * dr : 0.0;
+                }
+                // dn/dq = -(I - n n^T) P / (|qp| + eps)
+                for (int k = 0; k < 3; ++k)
+                    for (int j = 0; j < 3; ++j) {
+                        double t = 0;
+                        for (int m = 0; m < 3; ++m) t += ((k == m) - n[k] * n[m]) * P[m][j];
+                        dn[k][j] = -t / re;
+                    }
+                const double vn = vi[0] * n[0] + vi[1] * n[1] + vi[2] * n[2];
+                double us[3];
+                for (int k = 0; k < 3; ++k) us[k] = vi[k] - vn * n[k];
+                // du_s/dq = -(n v^T + (v . n) I) dn/dq, du_s/dv = I - n n^T
+                for (int k = 0; k < 3; ++k)
+                    for (int j = 0; j < 3; ++j) {
+                        double t = 0;
+                        for (int m = 0; m < 3; ++m) t += (n[k] * vi[m] + (k == m) * vn) * dn[m][j];
+                        dus_q[k][j] = -t;
+                        dus_v[k][j] = (k == j) - n[k] * n[j];
+                    }
+                const double um = std::sqrt(us[0] * us[0] + us[1] * us[1] + us[2] * us[2]), ue = um + eps;
+                double Jq[3][3], Jv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Jp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int k = 0; k < 3; ++k) {
+                    for (int j = 0; j < 3; ++j) Jq[k][j] = n[k] * dfn_q[j] + dn[k][j] * fn;  // d(f_n n)/dq
+                    Jp[k][0] = n[k] * fn / a.kn;
+                }
+                if (a.kf <= a.mu * fn / ue) {  // ff = -kf u_s
+                    for (int k = 0; k < 3; ++k) {
+                        for (int j = 0; j < 3; ++j) { Jq[k][j] += -a.kf * dus_q[k][j]; Jv[k][j] = -a.kf * dus_v[k][j]; }
+                        Jp[k][1] = -us[k];
+                    }
+                } else {  // ff = -mu f_n u_s / (|u_s| + eps)
+                    // the reference forms d|u_s|/dq as (du_s/dq) u_s / (|u_s| + eps), not its transpose
+                    double dum_q[3], dum_v[3];
+                    for (int k = 0; k < 3; ++k) {
+                        dum_q[k] = (dus_q[k][0] * us[0] + dus_q[k][1] * us[1] + dus_q[k][2] * us[2]) / ue;
+                        dum_v[k] = (dus_v[k][0] * us[0] + dus_v[k][1] * us[1] + dus_v[k][2] * us[2]) / ue;
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        for (int j = 0; j < 3; ++j) {
+                            Jq[k][j] += -a.mu * (us[k] * dfn_q[j] / ue + fn * (dus_q[k][j] / ue - us[k] * dum_q[j] / (ue * ue)));
+                            Jv[k][j] = -a.mu * fn * (dus_v[k][j] / ue - us[k] * dum_v[j] / (ue * ue));
+                        }
+                        Jp[k][0] += -us[k] / ue * a.mu * fn / a.kn;
+                        Jp[k][2] = -us[k] / ue * fn;
+                    }
+                }
+                for (int j = 0; j < 3; ++j) {
+                    dl_dq[3 * i + j] += g[0] * Jq[0][j] + g[1] * Jq[1][j] + g[2] * Jq[2][j];
+                    dl_dv[3 * i + j] += g[0] * Jv[0][j] + g[1] * Jv[1][j] + g[2] * Jv[2][j];
+                    dl_dp[a.poff + j] += g[0] * Jp[0][j] + g[1] * Jp[1][j] + g[2] * Jp[2][j];
                 }
             }
     }
@@ -3099,6 +3223,10 @@ int orc_backward_baseline(void* hp, const double* q, const double* v, const doub
 int orc_add_planar_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.add_planar_contact(orc::Vec(p, p + np)); });
+}
+int orc_add_arc_contact(void* hp, const double* p, int np) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_arc_contact(orc::Vec(p, p + np)); });
 }
 int orc_num_state_params(void* hp) { return static_cast<Handle*>(hp)->body.n_state_params; }
 // f_state(q, v) and its backward for a given dl/df (test access)

@@ -1,6 +1,7 @@
-"""The planar-contact state force (AddStateForce("planar_contact"), a penalty
-floor with friction) in the HIP body against the oracle: rollouts and adjoints
-in both friction branches, through the alg_phd and the baseline methods."""
+"""The contact state forces (AddStateForce("planar_contact"), a penalty floor
+with friction, and AddStateForce("arc_contact"), a cylindrical trough) in the
+HIP body against the oracle: rollouts and adjoints in both friction branches,
+through the alg_phd and the baseline methods."""
 import numpy as np
 import pytest
 
@@ -11,21 +12,22 @@ from test_oracle_baseline import BASELINE_OPT
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("scene", ["penalty_drop", "arc_drop"])
 @pytest.mark.parametrize("kf", [0.005, 1e3])
-def test_penalty_floor_matches_oracle(kf):
-    spec = scenes.penalty_drop(kf=kf)
+def test_penalty_floor_matches_oracle(scene, kf):
+    spec = getattr(scenes, scene)(kf=kf)
     for method, opt in (("pd_eigen_alg_phd", dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000)),
                         ("pd_eigen_pcg", dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000))):
         prod, orc = product_body(spec), oracle_body(spec)
         assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == 6
         qp, vp, ap = rollout(prod, spec, 3, opt, method=method)
         qo, vo, ao = rollout(orc, spec, 3, opt, method=method)
-        print(method, kf, "q", rel_err(qp[-1], qo[-1]))
+        print(scene, method, kf, "q", rel_err(qp[-1], qo[-1]))
         assert rel_err(qp[-1], qo[-1]) < 1e-8
         rng = np.random.default_rng(5)
         dq, dv = rng.standard_normal(prod.dofs()), rng.standard_normal(prod.dofs())
         outp = backward_frame(prod, spec, qp, vp, ap, 2, dq, dv, opt, method=method)
         outo = backward_frame(orc, spec, qo, vo, ao, 2, dq, dv, opt, method=method)
         for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_dstate_p", 6)):
-            print(method, kf, name, rel_err(outp[k], outo[k]))
+            print(scene, method, kf, name, rel_err(outp[k], outo[k]))
             assert np.abs(outo[k]).max() > 0 and rel_err(outp[k], outo[k]) < 1e-8, name
