@@ -376,40 +376,45 @@ __device__ void nh_diff(T mu, T la, T k, const T* U, const T* sig, const T* V, c
 // with p's offsets (the region order and tests of the reference's
 // MeshFrictionalBoundary, so that ties resolve alike).
 __host__ __device__ inline void closest_on_triangle(const double* p, const double* a, const double* b, const double* c,
-                                                    double* out) {
+                                                    double* out, double* wt = nullptr) {
     double ab[3], ac[3], ap[3], bp[3], cp[3];
     for (int i = 0; i < 3; ++i) {
         ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i];
         ap[i] = p[i] - a[i]; bp[i] = p[i] - b[i]; cp[i] = p[i] - c[i];
     }
     auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    auto weights = [&](double wa, double wb, double wc) { if (wt) { wt[0] = wa; wt[1] = wb; wt[2] = wc; } };
     const double d1 = dot(ab, ap), d2 = dot(ac, ap);
-    if (d1 <= 0 && d2 <= 0) { for (int i = 0; i < 3; ++i) out[i] = a[i]; return; }
+    if (d1 <= 0 && d2 <= 0) { for (int i = 0; i < 3; ++i) out[i] = a[i]; weights(1, 0, 0); return; }
     const double d3 = dot(ab, bp), d4 = dot(ac, bp);
-    if (d3 >= 0 && d4 <= d3) { for (int i = 0; i < 3; ++i) out[i] = b[i]; return; }
+    if (d3 >= 0 && d4 <= d3) { for (int i = 0; i < 3; ++i) out[i] = b[i]; weights(0, 1, 0); return; }
     const double vc = d1 * d4 - d3 * d2;
     if (vc <= 0 && d1 >= 0 && d3 <= 0) {
         const double t = d1 / (d1 - d3);
         for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ab[i];
+        weights(1 - t, t, 0);
         return;
     }
     const double d5 = dot(ab, cp), d6 = dot(ac, cp);
-    if (d6 >= 0 && d5 <= d6) { for (int i = 0; i < 3; ++i) out[i] = c[i]; return; }
+    if (d6 >= 0 && d5 <= d6) { for (int i = 0; i < 3; ++i) out[i] = c[i]; weights(0, 0, 1); return; }
     const double vb = d5 * d2 - d1 * d6;
     if (vb <= 0 && d2 >= 0 && d6 <= 0) {
         const double t = d2 / (d2 - d6);
         for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ac[i];
+        weights(1 - t, 0, t);
         return;
     }
     const double va = d3 * d6 - d5 * d4;
     if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
         const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         for (int i = 0; i < 3; ++i) out[i] = b[i] + t * (c[i] - b[i]);
+        weights(0, 1 - t, t);
         return;
     }
     const double den = 1 / (va + vb + vc);
     const double v = vb * den, w = vc * den;
     for (int i = 0; i < 3; ++i) out[i] = a[i] + ab[i] * v + ac[i] * w;
+    weights(1 - v - w, v, w);
 }
 
 }  // namespace phd

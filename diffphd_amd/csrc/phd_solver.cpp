@@ -194,6 +194,7 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
     energies_.clear();
     pcontacts_.clear();
     acontacts_.clear();
+    mcontacts_.clear();
     gravity_off_ = n_state_params_ = 0;
     muscles_.clear();
     act_dofs_ = 0;
@@ -324,8 +325,102 @@ void Body::add_arc_contact(const std::vector<double>& p) {
     acontacts_.push_back(a);
 }
 
+// MeshContactStateForce::Initialize (mesh_contact_state_force.cpp:101-142): params
+// [radius, kn_left, kn_right, kf, mu, #faces A, #faces B, faces A, faces B]
+// (deformable_state_force.cpp:86-112); 4 parameters (kn_left, kn_right, kf, mu).
+void Body::add_mesh_contact(const std::vector<double>& p) {
+    check(p.size() >= 7, "Inconsistent params for MeshContactStateForce.");
+    const int na = (int)p[5], nb = (int)p[6];
+    check(na >= 0 && nb >= 0 && p.size() == 7 + 3 * (size_t)(na + nb), "Inconsistent params for MeshContactStateForce.");
+    MeshContactForce m;
+    m.radius = p[0]; m.kn_l = p[1]; m.kn_r = p[2]; m.kf = p[3]; m.mu = p[4];
+    std::set<int> sa, sb;
+    for (int f = 0; f < na + nb; ++f) {
+        std::array<int, 3> t = {(int)p[7 + 3 * f], (int)p[8 + 3 * f], (int)p[9 + 3 * f]};
+        for (int k : t) check(k >= 0 && k < nv_, "Mesh contact face index out of range.");
+        (f < na ? m.fa : m.fb).push_back(t);
+        (f < na ? sa : sb).insert(t.begin(), t.end());
+    }
+    m.va.assign(sa.begin(), sa.end());
+    m.vb.assign(sb.begin(), sb.end());
+    m.poff = n_state_params_;
+    n_state_params_ += 4;
+    mcontacts_.push_back(m);
+}
+
 namespace {
 inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// MeshContactStateForce::ForwardForce (mesh_contact_state_force.cpp:144-215), added
+// into f: nothing when the two surfaces' bounding boxes are more than the radius
+// apart; else, for each vertex of one surface and triangle of the other closer than
+// the radius, kn (radius - dist) along the separating normal plus the friction
+// -min(kf, mu f_n / (|v_t| + eps)) v_t of the velocity relative to the triangle's
+// point (when |v_t| > eps), on the vertex and, barycentrically, against the corners.
+template <class M>
+void mesh_contact_add(const M& m, int dofs, const double* q, const double* v, double* f) {
+    if (m.fa.empty() || m.fb.empty()) return;
+    const double r2 = m.radius * m.radius;
+    double lo[2][3], hi[2][3];
+    for (int s = 0; s < 2; ++s) {
+        for (int k = 0; k < 3; ++k) { lo[s][k] = INFINITY; hi[s][k] = -INFINITY; }
+        for (int i : (s == 0 ? m.va : m.vb))
+            for (int k = 0; k < 3; ++k) {
+                lo[s][k] = std::min(lo[s][k], q[3 * (size_t)i + k]);
+                hi[s][k] = std::max(hi[s][k], q[3 * (size_t)i + k]);
+            }
+    }
+    double gap2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double g = hi[0][k] < lo[1][k] ? lo[1][k] - hi[0][k] : (hi[1][k] < lo[0][k] ? lo[0][k] - hi[1][k] : 0.0);
+        gap2 += g * g;
+    }
+    if (gap2 > r2) return;
+    std::vector<double> fm(dofs, 0.0);  // this force's own vector, added to f at the end
+    auto sweep = [&](const std::vector<int>& verts, const std::vector<std::array<int, 3>>& tris, double kn) {
+        for (int i : verts) {
+            const double* p = q + 3 * (size_t)i;
+            for (const auto& t : tris) {
+                const double *a = q + 3 * (size_t)t[0], *b = q + 3 * (size_t)t[1], *c = q + 3 * (size_t)t[2];
+                double x[3], w[3];
+                closest_on_triangle(p, a, b, c, x, w);  // the method of mesh_contact_state_force.cpp:44-97
+                const double d2 = (p[0] - x[0]) * (p[0] - x[0]) + (p[1] - x[1]) * (p[1] - x[1]) + (p[2] - x[2]) * (p[2] - x[2]);
+                if (d2 >= r2) continue;
+                const double dist = std::sqrt(d2);
+                double n[3], px[3];
+                for (int k = 0; k < 3; ++k) px[k] = p[k] - x[k];
+                if (dist > DBL_EPSILON) {
+                    for (int k = 0; k < 3; ++k) n[k] = px[k] / dist;
+                } else {  // on the triangle: its normal
+                    const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, z[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                    for (int k = 0; k < 3; ++k) n[k] = u[(k + 1) % 3] * z[(k + 2) % 3] - u[(k + 2) % 3] * z[(k + 1) % 3];
+                    const double nn = std::sqrt(dot3(n, n));
+                    if (nn <= DBL_EPSILON) continue;
+                    for (int k = 0; k < 3; ++k) n[k] /= nn;
+                }
+                if (dot3(n, px) < 0) for (int k = 0; k < 3; ++k) n[k] = -n[k];
+                const double fn = kn * (m.radius - dist);
+                double vr[3];
+                for (int k = 0; k < 3; ++k)
+                    vr[k] = v[3 * (size_t)i + k] - (w[0] * v[3 * (size_t)t[0] + k] + w[1] * v[3 * (size_t)t[1] + k] +
+                                                    w[2] * v[3 * (size_t)t[2] + k]);
+                const double vn = dot3(vr, n);
+                double vt[3];
+                for (int k = 0; k < 3; ++k) vt[k] = vr[k] - n[k] * vn;
+                const double vtm = std::sqrt(dot3(vt, vt));
+                const double cf = vtm > DBL_EPSILON ? std::min(m.kf, m.mu * fn / (vtm + DBL_EPSILON)) : 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double tot = fn * n[k] - cf * vt[k];
+                    fm[3 * (size_t)i + k] += tot;
+                    for (int j = 0; j < 3; ++j) fm[3 * (size_t)t[j] + k] -= tot * w[j];
+                }
+            }
+        }
+    };
+    sweep(m.va, m.fb, m.kn_l);
+    sweep(m.vb, m.fa, m.kn_r);
+    for (int i = 0; i < dofs; ++i) f[i] += fm[i];
+}
 // A vertex x against an arc (arc_contact_state_force.cpp:32-62): qp is x - c
 // without its axial part; contact when the angle of qp from `start` toward `aux`
 // lies in (0, angle). Returns |qp| and n = -qp / (|qp| + eps), or false.
@@ -375,6 +470,9 @@ void Body::state_force(const double* q, const double* v, double* f) const {
             const double cf = std::min(a.kf, a.mu * fn / (std::sqrt(dot3(us, us)) + DBL_EPSILON));
             for (int k = 0; k < 3; ++k) f[3 * (size_t)i + k] += fn * n[k] - cf * us[k];
         }
+    // mesh contacts: forward only; their BackwardForce returns zeros
+    // (mesh_contact_state_force.cpp:217-223), so they add nothing to the adjoint
+    for (const auto& m : mcontacts_) mesh_contact_add(m, dofs_, q, v, f);
 }
 
 // BackwardStateForce (deformable_state_force.cpp:127-145): dl/dq, dl/dv and

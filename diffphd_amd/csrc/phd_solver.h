@@ -170,6 +170,7 @@ public:
     // AddStateForce("planar_contact", [n (3), offset, p, kn, kf, mu]) (PlanarContactStateForce)
     void add_planar_contact(const std::vector<double>& p);
     void add_arc_contact(const std::vector<double>& p);
+    void add_mesh_contact(const std::vector<double>& p);
     int num_state_params() const { return n_state_params_; }
     // ForwardStateForce / BackwardStateForce at (q, v) on the host (3N vectors)
     void state_force(const double* q, const double* v, double* f) const;
@@ -340,6 +341,13 @@ private:
         int poff = 0;
     };
     std::vector<ArcContactForce> acontacts_;
+    struct MeshContactForce {  // vertices of surface A against triangles of B and the reverse
+        double radius = 0, kn_l = 0, kn_r = 0, kf = 0, mu = 0;
+        std::vector<std::array<int, 3>> fa, fb;
+        std::vector<int> va, vb;  // sorted vertex sets of fa and fb
+        int poff = 0;
+    };
+    std::vector<MeshContactForce> mcontacts_;
     int gravity_off_ = 0, n_state_params_ = 0;
     HostBoundary boundary_;
     std::map<int, int> fb_vertices_;

@@ -119,6 +119,46 @@ def arc_drop(kn=5.0, kf=0.005, mu=0.4, seed=0):
     return spec
 
 
+def mesh_stack(kn=5.0, kf=0.005, mu=0.4, radius=0.003, dx=0.01, E=1e5, nu=0.45, rho=1000.0, seed=0):
+    """Two 3x3x2-cell blocks in one body, the upper one 0.2 dx above the lower,
+    meeting through MeshContactStateForce (AddStateForce("mesh_contact",
+    [radius, kn_left, kn_right, kf, mu, #faces A, #faces B, faces A, faces B]),
+    mesh_contact_state_force.cpp): A = the upper block's bottom face, B = the
+    lower block's top face (two triangles per quad). The lower block's bottom
+    layer is fixed; the upper block moves sideways and down."""
+    vox = np.ones((3, 3, 5), dtype=bool)
+    vox[:, :, 2] = False
+    V, Eidx, _, vf = hex_grid(vox, dx, (0.0, 0.0, 0.0))
+    upper = np.unique(vf[:, :, 3:][vf[:, :, 3:] >= 0])
+    V[2, upper] -= 0.8 * dx
+
+    def face(k, flip):
+        tris = []
+        for i in range(3):
+            for j in range(3):
+                a, b, c, d = vf[i, j, k], vf[i + 1, j, k], vf[i + 1, j + 1, k], vf[i, j + 1, k]
+                tris += [(a, c, b), (a, d, c)] if flip else [(a, b, c), (a, c, d)]
+        return tris
+
+    fa, fb = face(3, True), face(2, False)
+    prm = [radius, kn, kn, kf, mu, len(fa), len(fb)] + [float(x) for t in fa + fb for x in t]
+    mu_l, la = lame(E, nu)
+    q0 = V.T.ravel().copy()
+    rng = np.random.default_rng(seed)
+    v0 = np.zeros_like(q0)
+    v0.reshape(-1, 3)[upper] = [0.2, 0.1, -0.1]
+    v0 += 0.01 * rng.standard_normal(q0.size) * np.repeat(np.isin(np.arange(V.shape[1]), upper), 3)
+    bottom = vf[:, :, 0].ravel()
+    return {
+        "name": "mesh_stack", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": (0.0, 0.0, -9.81),
+        "state_forces": [("mesh_contact", prm)], "faces": (fa, fb), "upper": upper,
+        "energies": [("corotated", [2 * mu_l], []), ("volume", [la], [])],
+        "dirichlet": [(3 * int(i) + d, float(q0[3 * int(i) + d])) for i in bottom for d in range(3)], "boundary": None,
+        "q0": q0, "v0": v0, "dt": 1e-2, "frames": 10, "loss_g": rng.standard_normal(q0.size),
+        "options": default_options(max_pd_iter=5000),
+    }
+
+
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):
     """Heterogeneous napkin dropped on a sphere (python/example/napkin_3d_test.py:32-45,
     python/py_diff_pd/env/napkin_env_3d.py:19-79): stiff half / 0.1x soft half

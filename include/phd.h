@@ -42,7 +42,10 @@ int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
  * or "planar_contact" [nx, ny, nz, offset, p, kn, kf, mu] (a penalty floor with
  * friction, planar_contact_state_force.cpp), or "arc_contact" [center (3), dir (3),
  * start (3), radius, angle, p, kn, kf, mu] (the inside of a cylindrical arc,
- * deformable_state_force.cpp:33-49, arc_contact_state_force.cpp). */
+ * deformable_state_force.cpp:33-49, arc_contact_state_force.cpp), or
+ * "mesh_contact" [radius, kn_left, kn_right, kf, mu, #faces A, #faces B, faces A,
+ * faces B] (two surfaces of the body, deformable_state_force.cpp:86-112,
+ * mesh_contact_state_force.cpp; forward only, its backward is zero). */
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
 /* Deformable::NumOfStateForceParameters(): the length of dl_dstate_p (3 per
  * state force, in AddStateForce order). */

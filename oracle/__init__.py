@@ -54,6 +54,7 @@ def lib():
         L.orc_set_press_plane.argtypes = [P, ctypes.c_double]
         L.orc_add_planar_contact.argtypes = [P, dp, ctypes.c_int]
         L.orc_add_arc_contact.argtypes = [P, dp, ctypes.c_int]
+        L.orc_add_mesh_contact.argtypes = [P, dp, ctypes.c_int]
         L.orc_num_state_params.argtypes = [P]
         L.orc_state_force.argtypes = [P, dp, dp, dp]
         L.orc_state_force_backward.argtypes = [P, dp, dp, dp, dp, dp, dp]
@@ -80,7 +81,7 @@ def lib():
         L.orc_filter_prox_jacobian.restype = ctypes.c_double
         L.orc_jacobi_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_coulomb_step.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
-        L.orc_tri_closest.argtypes = [dp, dp, dp, dp, dp]
+        L.orc_tri_closest.argtypes = [dp, dp, dp, dp, dp, dp]
         L.orc_boundary_query.argtypes = [P, dp, dp]
         L.orc_frame_stats.argtypes = [P, ip, ip, ctypes.c_int]
         L.orc_reset_frame_stats.argtypes = [P]
@@ -172,6 +173,8 @@ class OracleDeformable:
             self._ok(lib().orc_add_planar_contact(self._h, pp, len(params)))
         elif force_type == "arc_contact":
             self._ok(lib().orc_add_arc_contact(self._h, pp, len(params)))
+        elif force_type == "mesh_contact":
+            self._ok(lib().orc_add_mesh_contact(self._h, pp, len(params)))
         else:
             raise RuntimeError("Unsupported state force: " + str(force_type))
 
@@ -413,12 +416,14 @@ def coulomb_step(Wtt, u_t, fmu, lam):
     return o.copy()
 
 
-def tri_closest(p, a, b, c):
-    """Closest point of triangle (a, b, c) to p (the mesh boundary's query)."""
+def tri_closest(p, a, b, c, weights=False):
+    """Closest point of triangle (a, b, c) to p (the mesh boundary's and the mesh
+    contact force's query), and with weights=True also its barycentric weights."""
     arr = [_d(np.asarray(x, dtype=np.float64).copy()) for x in (p, a, b, c)]
     o, op = _d(np.zeros(3))
-    lib().orc_tri_closest(arr[0][1], arr[1][1], arr[2][1], arr[3][1], op)
-    return o.copy()
+    w, wp = _d(np.zeros(3))
+    lib().orc_tri_closest(arr[0][1], arr[1][1], arr[2][1], arr[3][1], op, wp)
+    return (o.copy(), w.copy()) if weights else o.copy()
 
 
 def jacobi_eigen(A):

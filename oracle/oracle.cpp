@@ -557,31 +557,34 @@ enum BoundaryType { kNone = 0, kPlanar = 1, kSpherical = 2, kHeightfield = 3, kP
 // the Voronoi-region method of Ericson, Real-Time Collision Detection 5.1.5):
 // vertex regions a, b, the edge ab, vertex c, the edge ac, the edge bc, then
 // the face interior by barycentric weights.
-void tri_closest(const double* p, const double* a, const double* b, const double* c, double* out) {
+void tri_closest(const double* p, const double* a, const double* b, const double* c, double* out, double* wt = nullptr) {
     double ab[3], ac[3], ap[3], bp[3], cp[3];
     for (int i = 0; i < 3; ++i) {
         ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i];
         ap[i] = p[i] - a[i]; bp[i] = p[i] - b[i]; cp[i] = p[i] - c[i];
     }
     auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    auto wts = [&](double wa, double wb, double wc) { if (wt) { wt[0] = wa; wt[1] = wb; wt[2] = wc; } };
     const double d1 = dot(ab, ap), d2 = dot(ac, ap);
-    if (d1 <= 0 && d2 <= 0) { for (int i = 0; i < 3; ++i) out[i] = a[i]; return; }
+    if (d1 <= 0 && d2 <= 0) { for (int i = 0; i < 3; ++i) out[i] = a[i]; wts(1, 0, 0); return; }
     const double d3 = dot(ab, bp), d4 = dot(ac, bp);
-    if (d3 >= 0 && d4 <= d3) { for (int i = 0; i < 3; ++i) out[i] = b[i]; return; }
+    if (d3 >= 0 && d4 <= d3) { for (int i = 0; i < 3; ++i) out[i] = b[i]; wts(0, 1, 0); return; }
     const double vc = d1 * d4 - d3 * d2;
-    if (vc <= 0 && d1 >= 0 && d3 <= 0) { const double t = d1 / (d1 - d3); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ab[i]; return; }
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) { const double t = d1 / (d1 - d3); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ab[i]; wts(1 - t, t, 0); return; }
     const double d5 = dot(ab, cp), d6 = dot(ac, cp);
-    if (d6 >= 0 && d5 <= d6) { for (int i = 0; i < 3; ++i) out[i] = c[i]; return; }
+    if (d6 >= 0 && d5 <= d6) { for (int i = 0; i < 3; ++i) out[i] = c[i]; wts(0, 0, 1); return; }
     const double vb = d5 * d2 - d1 * d6;
-    if (vb <= 0 && d2 >= 0 && d6 <= 0) { const double t = d2 / (d2 - d6); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ac[i]; return; }
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) { const double t = d2 / (d2 - d6); for (int i = 0; i < 3; ++i) out[i] = a[i] + t * ac[i]; wts(1 - t, 0, t); return; }
     const double va = d3 * d6 - d5 * d4;
     if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
         const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         for (int i = 0; i < 3; ++i) out[i] = b[i] + t * (c[i] - b[i]);
+        wts(0, 1 - t, t);
         return;
     }
     const double den = 1 / (va + vb + vc);
     for (int i = 0; i < 3; ++i) out[i] = a[i] + ab[i] * (vb * den) + ac[i] * (vc * den);
+    wts(1 - vb * den - vc * den, vb * den, vc * den);
 }
 
 struct Boundary {
@@ -912,6 +915,16 @@ struct Body {
         int poff = 0;
     };
     std::vector<ArcContact> acontacts;
+    // MeshContactStateForce (mesh_contact_state_force.cpp:101-142): penalty contact
+    // between the vertices of surface A and the triangles of surface B (kn_left) and
+    // the reverse (kn_right), within contact_radius; parameters (kn_left, kn_right, kf, mu)
+    struct MeshContact {
+        double radius = 0, kn_l = 0, kn_r = 0, kf = 0, mu = 0;
+        std::vector<std::array<int, 3>> fa, fb;
+        std::vector<int> va, vb;  // sorted vertex sets of the faces
+        int poff = 0;
+    };
+    std::vector<MeshContact> mcontacts;
     int gravity_off = 0, n_state_params = 0;
     Boundary boundary;
     std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
@@ -1187,6 +1200,99 @@ struct Body {
         for (int k = 0; k < 3; ++k) n[k] = -qp[k] / (r + eps);
         return true;
     }
+    // AddStateForce("mesh_contact", [radius, kn_left, kn_right, kf, mu, #faces A,
+    // #faces B, faces A (3 each), faces B]) (deformable_state_force.cpp:86-112).
+    void add_mesh_contact(const Vec& prm) {
+        check(prm.size() >= 7, "Inconsistent params for MeshContactStateForce.");
+        const int na = (int)prm[5], nb = (int)prm[6];
+        check(na >= 0 && nb >= 0 && prm.size() == 7 + 3 * (size_t)(na + nb), "Inconsistent params for MeshContactStateForce.");
+        MeshContact m;
+        m.radius = prm[0]; m.kn_l = prm[1]; m.kn_r = prm[2]; m.kf = prm[3]; m.mu = prm[4];
+        std::set<int> sa, sb;
+        for (int f = 0; f < na + nb; ++f) {
+            std::array<int, 3> t;
+            for (int k = 0; k < 3; ++k) {
+                t[k] = (int)prm[7 + 3 * f + k];
+                check(t[k] >= 0 && t[k] < nv, "Mesh contact face index out of range.");
+            }
+            if (f < na) { m.fa.push_back(t); sa.insert(t.begin(), t.end()); }
+            else { m.fb.push_back(t); sb.insert(t.begin(), t.end()); }
+        }
+        m.va.assign(sa.begin(), sa.end());
+        m.vb.assign(sb.begin(), sb.end());
+        m.poff = n_state_params;
+        n_state_params += 4;
+        mcontacts.push_back(m);
+    }
+    // MeshContactStateForce::ForwardForce (mesh_contact_state_force.cpp:144-215): for
+    // each vertex of one surface and each triangle of the other within the radius,
+    // f = kn (radius - dist) n + friction -min(kf, mu f_n / (|v_t| + eps)) v_t on the
+    // velocity relative to the triangle's point (only when |v_t| > eps), applied to
+    // the vertex and, with the barycentric weights, opposite to the triangle's corners.
+    // Nothing when the surfaces' bounding boxes lie more than the radius apart.
+    void mesh_contact_force(const MeshContact& m, const Vec& q, const Vec& v, Vec& f) const {
+        if (m.fa.empty() || m.fb.empty()) return;
+        const double eps = std::numeric_limits<double>::epsilon(), r2 = m.radius * m.radius;
+        auto box = [&](const std::vector<int>& vs, double* lo, double* hi) {
+            for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+            for (int i : vs)
+                for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], q[3 * i + k]); hi[k] = std::max(hi[k], q[3 * i + k]); }
+        };
+        double la[3], ha[3], lb[3], hb[3];
+        box(m.va, la, ha);
+        box(m.vb, lb, hb);
+        double gap2 = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double g = ha[k] < lb[k] ? lb[k] - ha[k] : (hb[k] < la[k] ? la[k] - hb[k] : 0.0);
+            gap2 += g * g;
+        }
+        if (gap2 > r2) return;
+        auto side = [&](const std::vector<int>& verts, const std::vector<std::array<int, 3>>& faces, double kn) {
+            for (int i : verts) {
+                const double* p = &q[3 * i];
+                for (const auto& t : faces) {
+                    const double *a = &q[3 * t[0]], *b = &q[3 * t[1]], *c = &q[3 * t[2]];
+                    double pt[3], w[3];
+                    tri_closest(p, a, b, c, pt, w);  // the method of mesh_contact_state_force.cpp:44-97
+                    const double d2 = (p[0] - pt[0]) * (p[0] - pt[0]) + (p[1] - pt[1]) * (p[1] - pt[1]) +
+                                      (p[2] - pt[2]) * (p[2] - pt[2]);
+                    if (d2 >= r2) continue;
+                    const double dist = std::sqrt(d2);
+                    double n[3];
+                    if (dist > eps) {
+                        for (int k = 0; k < 3; ++k) n[k] = (p[k] - pt[k]) / dist;
+                    } else {
+                        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                        n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+                        n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+                        n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+                        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                        if (nn <= eps) continue;
+                        for (int k = 0; k < 3; ++k) n[k] /= nn;
+                    }
+                    if (n[0] * (p[0] - pt[0]) + n[1] * (p[1] - pt[1]) + n[2] * (p[2] - pt[2]) < 0)
+                        for (int k = 0; k < 3; ++k) n[k] = -n[k];
+                    const double fn = kn * (m.radius - dist);
+                    double vr[3];
+                    for (int k = 0; k < 3; ++k)
+                        vr[k] = v[3 * i + k] - (w[0] * v[3 * t[0] + k] + w[1] * v[3 * t[1] + k] + w[2] * v[3 * t[2] + k]);
+                    const double vn = vr[0] * n[0] + vr[1] * n[1] + vr[2] * n[2];
+                    double vt[3];
+                    for (int k = 0; k < 3; ++k) vt[k] = vr[k] - n[k] * vn;
+                    const double vtm = std::sqrt(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+                    const double cf = vtm > eps ? std::min(m.kf, m.mu * fn / (vtm + eps)) : 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double tot = fn * n[k] - cf * vt[k];
+                        f[3 * i + k] += tot;
+                        for (int j = 0; j < 3; ++j) f[3 * t[j] + k] -= tot * w[j];
+                    }
+                }
+            }
+        };
+        side(m.va, m.fb, m.kn_l);
+        side(m.vb, m.fa, m.kn_r);
+    }
     // ForwardStateForce (deformable_state_force.cpp:114-125): gravity f_i = mass g
     // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
     // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s,
@@ -1224,6 +1330,13 @@ struct Body {
                 const double cf = std::min(a.kf, a.mu * fn / (um + eps));
                 for (int k = 0; k < 3; ++k) f[3 * i + k] += fn * n[k] - cf * us[k];
             }
+        // each force is evaluated into its own vector and then added, as the
+        // reference sums ForwardForce over its state forces
+        for (const auto& m : mcontacts) {
+            Vec fm(dofs, 0.0);
+            mesh_contact_force(m, q, v, fm);
+            for (int i = 0; i < dofs; ++i) f[i] += fm[i];
+        }
         return f;
     }
     // BackwardStateForce (deformable_state_force.cpp:127-145): gravity's
@@ -3224,6 +3337,10 @@ int orc_add_planar_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.add_planar_contact(orc::Vec(p, p + np)); });
 }
+int orc_add_mesh_contact(void* hp, const double* p, int np) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_mesh_contact(orc::Vec(p, p + np)); });
+}
 int orc_add_arc_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.add_arc_contact(orc::Vec(p, p + np)); });
@@ -3352,8 +3469,8 @@ void orc_coulomb_step(const double* Wt, double u1, double u2, double fmu, double
     orc::coulomb_step(Wt, u1, u2, fmu, lam, out2[0], out2[1]);
 }
 // Closest point of triangle (a, b, c) to p (the mesh boundary's query).
-void orc_tri_closest(const double* p, const double* a, const double* b, const double* c, double* out3) {
-    orc::tri_closest(p, a, b, c, out3);
+void orc_tri_closest(const double* p, const double* a, const double* b, const double* c, double* out3, double* w3) {
+    orc::tri_closest(p, a, b, c, out3, w3);
 }
 // Frictional boundary of the body at q (3): out = [distance, n (3), t1 (3), t2 (3),
 // governing obstacle of the press: 0 ground, 1 plane (-1 for other types)].

@@ -1,7 +1,9 @@
 """The contact state forces (AddStateForce("planar_contact"), a penalty floor
-with friction, and AddStateForce("arc_contact"), a cylindrical trough) in the
-HIP body against the oracle: rollouts and adjoints in both friction branches,
-through the alg_phd and the baseline methods."""
+with friction, AddStateForce("arc_contact"), a cylindrical trough, and
+AddStateForce("mesh_contact"), two surfaces of the body meeting) in the HIP body
+against the oracle: rollouts and adjoints in both friction branches, through
+the alg_phd and the baseline methods (the mesh contact's backward is zero, as
+in the reference, so its adjoint runs through the other terms)."""
 import numpy as np
 import pytest
 
@@ -12,14 +14,14 @@ from test_oracle_baseline import BASELINE_OPT
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("scene", ["penalty_drop", "arc_drop"])
+@pytest.mark.parametrize("scene", ["penalty_drop", "arc_drop", "mesh_stack"])
 @pytest.mark.parametrize("kf", [0.005, 1e3])
 def test_penalty_floor_matches_oracle(scene, kf):
     spec = getattr(scenes, scene)(kf=kf)
     for method, opt in (("pd_eigen_alg_phd", dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000)),
                         ("pd_eigen_pcg", dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000))):
         prod, orc = product_body(spec), oracle_body(spec)
-        assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == 6
+        assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == (7 if scene == "mesh_stack" else 6)
         qp, vp, ap = rollout(prod, spec, 3, opt, method=method)
         qo, vo, ao = rollout(orc, spec, 3, opt, method=method)
         print(scene, method, kf, "q", rel_err(qp[-1], qo[-1]))
