@@ -195,6 +195,7 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
     pcontacts_.clear();
     acontacts_.clear();
     mcontacts_.clear();
+    bballs_.clear();
     gravity_off_ = n_state_params_ = 0;
     muscles_.clear();
     act_dofs_ = 0;
@@ -348,8 +349,54 @@ void Body::add_mesh_contact(const std::vector<double>& p) {
     mcontacts_.push_back(m);
 }
 
+// BilliardBallStateForce::Initialize (billiard_ball_state_force.cpp:6-18): params
+// [radius, vertices per ball, stiffness per ball, friction coefficient per ball]
+// (deformable_state_force.cpp:72-85); the body's vertices are the balls in order.
+void Body::add_billiard_ball(const std::vector<double>& p) {
+    check(p.size() >= 2 && p.size() % 2 == 0, "Inconsistent params for BilliardBallStateForce.");
+    BilliardBallForce b;
+    b.R = p[0];
+    b.svn = (int)p[1];
+    b.nb = (int)(p.size() - 2) / 2;
+    check(b.svn > 0 && nv_ % b.svn == 0 && nv_ / b.svn == b.nb, "Inconsistent ball_num.");
+    b.k.assign(p.begin() + 2, p.begin() + 2 + b.nb);
+    b.mu.assign(p.begin() + 2 + b.nb, p.end());
+    b.poff = n_state_params_;
+    n_state_params_ += 2 * b.nb;
+    bballs_.push_back(b);
+}
+
 namespace {
 inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Per-ball means of x (centres of q, centre velocities of v).
+template <class B>
+std::vector<std::array<double, 3>> ball_means(const B& b, const double* x) {
+    std::vector<std::array<double, 3>> c(b.nb, std::array<double, 3>{0, 0, 0});
+    for (int i = 0; i < b.nb; ++i) {
+        for (int p = 0; p < b.svn; ++p)
+            for (int k = 0; k < 3; ++k) c[i][k] += x[3 * ((size_t)i * b.svn + p) + k];
+        for (int k = 0; k < 3; ++k) c[i][k] /= b.svn;
+    }
+    return c;
+}
+
+// One pair i < j of balls (billiard_ball_state_force.cpp:51-84): unit u from
+// centre i to centre j, overlap 2R - dist (clamped at 0), and the friction
+// direction for ball i, u turned by +-90 degrees in x-y by the sign of the z part of
+// (v_i - v_j) x u. Returns dist.
+template <class B>
+double ball_pair(const B& b, const std::array<double, 3>& ci, const std::array<double, 3>& cj,
+                 const std::array<double, 3>& vi, const std::array<double, 3>& vj, double* u, double& pen, double* rot,
+                 double& sg) {
+    for (int k = 0; k < 3; ++k) u[k] = cj[k] - ci[k];
+    const double dist = std::sqrt(dot3(u, u));
+    for (int k = 0; k < 3; ++k) u[k] /= dist;
+    pen = std::max(2 * b.R - dist, 0.0);
+    sg = (vi[0] - vj[0]) * u[1] - (vi[1] - vj[1]) * u[0] > 0 ? 1.0 : -1.0;
+    rot[0] = -sg * u[1]; rot[1] = sg * u[0]; rot[2] = 0.0;
+    return dist;
+}
 
 // MeshContactStateForce::ForwardForce (mesh_contact_state_force.cpp:144-215), added
 // into f: nothing when the two surfaces' bounding boxes are more than the radius
@@ -473,6 +520,27 @@ void Body::state_force(const double* q, const double* v, double* f) const {
     // mesh contacts: forward only; their BackwardForce returns zeros
     // (mesh_contact_state_force.cpp:217-223), so they add nothing to the adjoint
     for (const auto& m : mcontacts_) mesh_contact_add(m, dofs_, q, v, f);
+    // billiard balls (billiard_ball_state_force.cpp:20-86): ball i gets
+    // k_i pen (-u + mu_i rot), ball j gets k_j pen (u - mu_j rot), spread over their vertices
+    for (const auto& b : bballs_) {
+        const auto c = ball_means(b, q), vc = ball_means(b, v);
+        std::vector<double> fb(dofs_, 0.0);
+        for (int i = 0; i < b.nb; ++i)
+            for (int j = i + 1; j < b.nb; ++j) {
+                double u[3], rot[3], pen, sg;
+                ball_pair(b, c[i], c[j], vc[i], vc[j], u, pen, rot, sg);
+                const double fi = pen * b.k[i], fj = pen * b.k[j];
+                for (int k = 0; k < 3; ++k) {
+                    const double Fi = (-u[k] * fi + rot[k] * fi * b.mu[i]) / b.svn;
+                    const double Fj = (u[k] * fj - rot[k] * fj * b.mu[j]) / b.svn;
+                    for (int p = 0; p < b.svn; ++p) {
+                        fb[3 * ((size_t)i * b.svn + p) + k] += Fi;
+                        fb[3 * ((size_t)j * b.svn + p) + k] += Fj;
+                    }
+                }
+            }
+        for (int i = 0; i < dofs_; ++i) f[i] += fb[i];
+    }
 }
 
 // BackwardStateForce (deformable_state_force.cpp:127-145): dl/dq, dl/dv and
@@ -582,6 +650,54 @@ void Body::state_force_backward(const double* q, const double* v, const double* 
                 dl_dp[a.poff + 2] += -gu / ue * fn;
             }
         }
+    // billiard balls (billiard_ball_state_force.cpp:88-229): through the centres only
+    // (dl/dv = 0: the friction direction is piecewise constant in v). The reference
+    // sizes the friction magnitude's parameter gradient by 2 svn instead of 2 ball_num
+    // (:165-166), so its backward is defined only when they agree; other layouts raise.
+    for (const auto& b : bballs_) {
+        check(b.svn == b.nb, "BilliardBallStateForce backward needs as many vertices per ball as balls.");
+        const auto c = ball_means(b, q), vc = ball_means(b, v);
+        for (int i = 0; i < b.nb; ++i)
+            for (int j = i + 1; j < b.nb; ++j) {
+                double u[3], rot[3], pen, sg;
+                const double dist = ball_pair(b, c[i], c[j], vc[i], vc[j], u, pen, rot, sg);
+                check(dist > 0.1 * b.R, "Balls are too close to each other");
+                const double fi = pen * b.k[i], fj = pen * b.k[j];
+                double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};  // each ball's mean dl/df
+                for (int p = 0; p < b.svn; ++p)
+                    for (int k = 0; k < 3; ++k) {
+                        gi[k] += dl_df[3 * ((size_t)i * b.svn + p) + k] / b.svn;
+                        gj[k] += dl_df[3 * ((size_t)j * b.svn + p) + k] / b.svn;
+                    }
+                // w = weights of u and rot in g_i . F_i + g_j . F_j:
+                // F_i = fi (-u + mu_i rot), F_j = fj (u - mu_j rot)
+                const double ai = fi, aj = fj;
+                double wu[3], wr[3];
+                for (int k = 0; k < 3; ++k) {
+                    wu[k] = -ai * gi[k] + aj * gj[k];
+                    wr[k] = ai * b.mu[i] * gi[k] - aj * b.mu[j] * gj[k];
+                }
+                // d u / d c_j = (I - u u^T) / dist; d rot / d c_j = sg (-row 1, row 0, 0) of it;
+                // d f / d dist = -k when the balls overlap
+                const double wrr[3] = {sg * wr[1], -sg * wr[0], 0.0};  // (d rot/du)^T wr
+                double t[3];
+                for (int k = 0; k < 3; ++k) t[k] = wu[k] + wrr[k];
+                const double tu = dot3(t, u);
+                const double dfi = fi > 0 ? -b.k[i] : 0.0, dfj = fj > 0 ? -b.k[j] : 0.0;
+                const double mag = dfi * (-dot3(gi, u) + b.mu[i] * dot3(gi, rot)) + dfj * (dot3(gj, u) - b.mu[j] * dot3(gj, rot));
+                double dcj[3];
+                for (int k = 0; k < 3; ++k) dcj[k] = (t[k] - tu * u[k]) / dist + mag * u[k];
+                for (int p = 0; p < b.svn; ++p)
+                    for (int k = 0; k < 3; ++k) {
+                        dl_dq[3 * ((size_t)i * b.svn + p) + k] -= dcj[k] / b.svn;
+                        dl_dq[3 * ((size_t)j * b.svn + p) + k] += dcj[k] / b.svn;
+                    }
+                dl_dp[b.poff + i] += pen * (-dot3(gi, u) + b.mu[i] * dot3(gi, rot));
+                dl_dp[b.poff + j] += pen * (dot3(gj, u) - b.mu[j] * dot3(gj, rot));
+                dl_dp[b.poff + b.nb + i] += fi * dot3(gi, rot);
+                dl_dp[b.poff + b.nb + j] += -fj * dot3(gj, rot);
+            }
+    }
 }
 
 // SetFrictionalBoundary, deformable_friction.cpp:7-89, plus heightfield/press.

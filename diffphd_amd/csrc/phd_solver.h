@@ -171,6 +171,7 @@ public:
     void add_planar_contact(const std::vector<double>& p);
     void add_arc_contact(const std::vector<double>& p);
     void add_mesh_contact(const std::vector<double>& p);
+    void add_billiard_ball(const std::vector<double>& p);
     int num_state_params() const { return n_state_params_; }
     // ForwardStateForce / BackwardStateForce at (q, v) on the host (3N vectors)
     void state_force(const double* q, const double* v, double* f) const;
@@ -348,6 +349,13 @@ private:
         int poff = 0;
     };
     std::vector<MeshContactForce> mcontacts_;
+    struct BilliardBallForce {  // consecutive blocks of svn vertices as balls
+        double R = 0;
+        int svn = 0, nb = 0;
+        std::vector<double> k, mu;
+        int poff = 0;
+    };
+    std::vector<BilliardBallForce> bballs_;
     int gravity_off_ = 0, n_state_params_ = 0;
     HostBoundary boundary_;
     std::map<int, int> fb_vertices_;

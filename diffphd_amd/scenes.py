@@ -159,6 +159,32 @@ def mesh_stack(kn=5.0, kf=0.005, mu=0.4, radius=0.003, dx=0.01, E=1e5, nu=0.45, 
     }
 
 
+def billiard(balls=8, dx=0.01, radius=1.05, E=1e5, nu=0.45, rho=1000.0, seed=0, k=None, mu=None):
+    """A row of one-cell cubes 2 dx apart along x as balls of
+    BilliardBallStateForce (AddStateForce("billiard_ball", [radius, vertices per
+    ball, stiffness per ball, friction per ball]), billiard_ball_state_force.cpp):
+    each cube's 8 vertices are one ball (8 balls, so the reference's backward is
+    defined), ball radius `radius` dx, so neighbours overlap by 2 (radius - 1) dx;
+    random sideways velocities set the friction directions; no gravity."""
+    vox = np.zeros((2 * balls - 1, 1, 1), dtype=bool)
+    vox[::2] = True
+    V, Eidx, _, _ = hex_grid(vox, dx, (0.0, 0.0, 0.0))
+    rng = np.random.default_rng(seed)
+    k = list(rng.uniform(3.0, 6.0, balls)) if k is None else list(k)
+    mu = list(rng.uniform(0.2, 0.5, balls)) if mu is None else list(mu)
+    mu_l, la = lame(E, nu)
+    q0 = V.T.ravel().copy()
+    vb = rng.standard_normal((balls, 3)) * [0.05, 0.2, 0.05]
+    v0 = np.repeat(vb, 8, axis=0).ravel() + 0.005 * rng.standard_normal(q0.size)
+    return {
+        "name": "billiard", "edim": 8, "V": V, "E": Eidx, "density": rho, "gravity": None,
+        "state_forces": [("billiard_ball", [radius * dx, 8] + k + mu)],
+        "energies": [("corotated", [2 * mu_l], []), ("volume", [la], [])], "dirichlet": [], "boundary": None,
+        "q0": q0, "v0": v0, "dt": 1e-2, "frames": 10, "loss_g": rng.standard_normal(q0.size),
+        "options": default_options(max_pd_iter=5000),
+    }
+
+
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):
     """Heterogeneous napkin dropped on a sphere (python/example/napkin_3d_test.py:32-45,
     python/py_diff_pd/env/napkin_env_3d.py:19-79): stiff half / 0.1x soft half

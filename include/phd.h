@@ -45,7 +45,9 @@ int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
  * deformable_state_force.cpp:33-49, arc_contact_state_force.cpp), or
  * "mesh_contact" [radius, kn_left, kn_right, kf, mu, #faces A, #faces B, faces A,
  * faces B] (two surfaces of the body, deformable_state_force.cpp:86-112,
- * mesh_contact_state_force.cpp; forward only, its backward is zero). */
+ * mesh_contact_state_force.cpp; forward only, its backward is zero), or
+ * "billiard_ball" [radius, vertices per ball, stiffness per ball, friction per
+ * ball] (deformable_state_force.cpp:72-85, billiard_ball_state_force.cpp). */
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
 /* Deformable::NumOfStateForceParameters(): the length of dl_dstate_p (3 per
  * state force, in AddStateForce order). */

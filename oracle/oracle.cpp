@@ -925,6 +925,17 @@ struct Body {
         int poff = 0;
     };
     std::vector<MeshContact> mcontacts;
+    // BilliardBallStateForce (billiard_ball_state_force.cpp:6-18): the body is
+    // ball_num consecutive blocks of svn vertices; the balls (centres = vertex
+    // means) push apart within 2 radius, with a sideways friction in the x-y plane;
+    // parameters (stiffness per ball, friction coefficient per ball)
+    struct BilliardBall {
+        double R = 0;
+        int svn = 0, nb = 0;
+        std::vector<double> k, mu;
+        int poff = 0;
+    };
+    std::vector<BilliardBall> bballs;
     int gravity_off = 0, n_state_params = 0;
     Boundary boundary;
     std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
@@ -1293,6 +1304,122 @@ struct Body {
         side(m.va, m.fb, m.kn_l);
         side(m.vb, m.fa, m.kn_r);
     }
+    // AddStateForce("billiard_ball", [radius, svn, stiffness (ball_num), friction
+    // coefficients (ball_num)]) (deformable_state_force.cpp:72-85).
+    void add_billiard_ball(const Vec& prm) {
+        check(prm.size() >= 2 && prm.size() % 2 == 0, "Inconsistent params for BilliardBallStateForce.");
+        BilliardBall b;
+        b.R = prm[0];
+        b.svn = (int)prm[1];
+        b.nb = (int)(prm.size() - 2) / 2;
+        check(b.svn > 0 && nv % b.svn == 0 && nv / b.svn == b.nb, "Inconsistent ball_num.");
+        for (int i = 0; i < b.nb; ++i) { b.k.push_back(prm[2 + i]); b.mu.push_back(prm[2 + b.nb + i]); }
+        b.poff = n_state_params;
+        n_state_params += 2 * b.nb;
+        bballs.push_back(b);
+    }
+    // Ball centres and centre velocities (vertex means).
+    void ball_centres(const BilliardBall& b, const Vec& x, std::vector<std::array<double, 3>>& c) const {
+        c.assign(b.nb, {0, 0, 0});
+        for (int i = 0; i < b.nb; ++i) {
+            for (int j = 0; j < b.svn; ++j)
+                for (int k = 0; k < 3; ++k) c[i][k] += x[3 * ((size_t)i * b.svn + j) + k];
+            for (int k = 0; k < 3; ++k) c[i][k] /= b.svn;
+        }
+    }
+    // ForwardForce (billiard_ball_state_force.cpp:20-86): for balls i < j with
+    // centres closer than 2 radius, ball j gets k_j (2R - |cj - ci|) along i->j and ball
+    // i gets k_i (...) against it; friction of magnitude mu times that, along i->j
+    // turned by +90 degrees in x-y for ball i when (v_i - v_j) x i2j (z part) > 0,
+    // else by -90 degrees, and the opposite for ball j; each ball's force is spread
+    // evenly over its vertices.
+    void billiard_force(const BilliardBall& b, const Vec& q, const Vec& v, Vec& f) const {
+        std::vector<std::array<double, 3>> c, vc;
+        ball_centres(b, q, c);
+        ball_centres(b, v, vc);
+        for (int i = 0; i < b.nb; ++i)
+            for (int j = i + 1; j < b.nb; ++j) {
+                double dir[3], u[3], rot[3];
+                for (int k = 0; k < 3; ++k) dir[k] = c[j][k] - c[i][k];
+                const double dist = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                const double pen = std::max(2 * b.R - dist, 0.0);
+                const double fj = pen * b.k[j], fi = pen * b.k[i];
+                for (int k = 0; k < 3; ++k) u[k] = dir[k] / dist;
+                const double rx = vc[i][0] - vc[j][0], ry = vc[i][1] - vc[j][1];
+                if (rx * u[1] - ry * u[0] > 0) { rot[0] = -u[1]; rot[1] = u[0]; }
+                else { rot[0] = u[1]; rot[1] = -u[0]; }
+                rot[2] = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const double Fi = (-u[k] * fi + rot[k] * fi * b.mu[i]) / b.svn;
+                    const double Fj = (u[k] * fj - rot[k] * fj * b.mu[j]) / b.svn;
+                    for (int p = 0; p < b.svn; ++p) {
+                        f[3 * ((size_t)i * b.svn + p) + k] += Fi;
+                        f[3 * ((size_t)j * b.svn + p) + k] += Fj;
+                    }
+                }
+            }
+    }
+    // BackwardForce (billiard_ball_state_force.cpp:88-229): dl/dq through the
+    // centres, dl/dv = 0 (the friction direction is piecewise constant in v), and
+    // dl/d(k, mu). The reference sizes the friction magnitude's parameter gradient by
+    // 2 svn instead of 2 ball_num, so its result is defined only for svn == ball_num;
+    // other layouts raise an error here.
+    void billiard_backward(const BilliardBall& b, const Vec& q, const Vec& v, const Vec& g, Vec& dl_dq, Vec& dl_dp) const {
+        check(b.svn == b.nb, "BilliardBallStateForce backward needs as many vertices per ball as balls.");
+        std::vector<std::array<double, 3>> c, vc;
+        ball_centres(b, q, c);
+        ball_centres(b, v, vc);
+        for (int i = 0; i < b.nb; ++i)
+            for (int j = i + 1; j < b.nb; ++j) {
+                double dir[3], u[3];
+                for (int k = 0; k < 3; ++k) dir[k] = c[j][k] - c[i][k];
+                const double dist = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                check(dist > 0.1 * b.R, "Balls are too close to each other");
+                const double pen = std::max(2 * b.R - dist, 0.0);
+                const double fj = pen * b.k[j], fi = pen * b.k[i];
+                for (int k = 0; k < 3; ++k) u[k] = dir[k] / dist;
+                // d dist / d c_j = u = -d dist / d c_i; d u / d c_j = (I - u u^T) / dist = -d u / d c_i
+                double Du[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) Du[r][s] = ((r == s) - u[r] * u[s]) / dist;
+                const double dfj = fj > 0 ? -b.k[j] : 0.0, dfi = fi > 0 ? -b.k[i] : 0.0;  // d f / d dist
+                const double rx = vc[i][0] - vc[j][0], ry = vc[i][1] - vc[j][1];
+                const double sg = rx * u[1] - ry * u[0] > 0 ? 1.0 : -1.0;
+                // rot = sg * (-u_y, u_x, 0): its Jacobian rows are sg * (-Du[1], Du[0], 0)
+                const double rot[3] = {-sg * u[1], sg * u[0], 0.0};
+                double Drot[3][3];
+                for (int s = 0; s < 3; ++s) { Drot[0][s] = -sg * Du[1][s]; Drot[1][s] = sg * Du[0][s]; Drot[2][s] = 0; }
+                // summed dl/df over each ball's vertices, / svn
+                double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};
+                for (int p = 0; p < b.svn; ++p)
+                    for (int k = 0; k < 3; ++k) {
+                        gi[k] += g[3 * ((size_t)i * b.svn + p) + k] / b.svn;
+                        gj[k] += g[3 * ((size_t)j * b.svn + p) + k] / b.svn;
+                    }
+                // F_i = fi (-u + mu_i rot), F_j = fj (u - mu_j rot); gradient in c_j (c_i is its negative)
+                double dcj[3] = {0, 0, 0};
+                const double giu = gi[0] * u[0] + gi[1] * u[1] + gi[2] * u[2];
+                const double gju = gj[0] * u[0] + gj[1] * u[1] + gj[2] * u[2];
+                const double gir = gi[0] * rot[0] + gi[1] * rot[1];
+                const double gjr = gj[0] * rot[0] + gj[1] * rot[1];
+                for (int s = 0; s < 3; ++s) {
+                    double t = 0;
+                    for (int r = 0; r < 3; ++r)
+                        t += gi[r] * fi * (-Du[r][s] + b.mu[i] * Drot[r][s]) + gj[r] * fj * (Du[r][s] - b.mu[j] * Drot[r][s]);
+                    t += u[s] * (dfi * (-giu + b.mu[i] * gir) + dfj * (gju - b.mu[j] * gjr));
+                    dcj[s] = t;
+                }
+                for (int p = 0; p < b.svn; ++p)
+                    for (int k = 0; k < 3; ++k) {
+                        dl_dq[3 * ((size_t)i * b.svn + p) + k] += -dcj[k] / b.svn;
+                        dl_dq[3 * ((size_t)j * b.svn + p) + k] += dcj[k] / b.svn;
+                    }
+                dl_dp[b.poff + i] += pen * (-giu + b.mu[i] * gir);
+                dl_dp[b.poff + j] += pen * (gju - b.mu[j] * gjr);
+                dl_dp[b.poff + b.nb + i] += fi * gir;
+                dl_dp[b.poff + b.nb + j] += -fj * gjr;
+            }
+    }
     // ForwardStateForce (deformable_state_force.cpp:114-125): gravity f_i = mass g
     // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
     // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s,
@@ -1336,6 +1463,11 @@ struct Body {
             Vec fm(dofs, 0.0);
             mesh_contact_force(m, q, v, fm);
             for (int i = 0; i < dofs; ++i) f[i] += fm[i];
+        }
+        for (const auto& b : bballs) {
+            Vec fb(dofs, 0.0);
+            billiard_force(b, q, v, fb);
+            for (int i = 0; i < dofs; ++i) f[i] += fb[i];
         }
         return f;
     }
@@ -1456,6 +1588,7 @@ struct Body {
                     dl_dp[a.poff + j] += g[0] * Jp[0][j] + g[1] * Jp[1][j] + g[2] * Jp[2][j];
                 }
             }
+        for (const auto& b : bballs) billiard_backward(b, q, v, dl_df, dl_dq, dl_dp);
     }
 
     // SetupProjectiveDynamicsSolver, deformable_projective_dynamics_forward.cpp:8-183
@@ -3336,6 +3469,10 @@ int orc_backward_baseline(void* hp, const double* q, const double* v, const doub
 int orc_add_planar_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.add_planar_contact(orc::Vec(p, p + np)); });
+}
+int orc_add_billiard_ball(void* hp, const double* p, int np) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_billiard_ball(orc::Vec(p, p + np)); });
 }
 int orc_add_mesh_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);

@@ -1,6 +1,7 @@
 """The contact state forces (AddStateForce("planar_contact"), a penalty floor
-with friction, AddStateForce("arc_contact"), a cylindrical trough, and
-AddStateForce("mesh_contact"), two surfaces of the body meeting) in the HIP body
+with friction; AddStateForce("arc_contact"), a cylindrical trough;
+AddStateForce("mesh_contact"), two surfaces of the body meeting; and
+AddStateForce("billiard_ball"), vertex blocks as balls) in the HIP body
 against the oracle: rollouts and adjoints in both friction branches, through
 the alg_phd and the baseline methods (the mesh contact's backward is zero, as
 in the reference, so its adjoint runs through the other terms)."""
@@ -14,22 +15,31 @@ from test_oracle_baseline import BASELINE_OPT
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("scene", ["penalty_drop", "arc_drop", "mesh_stack"])
-@pytest.mark.parametrize("kf", [0.005, 1e3])
-def test_penalty_floor_matches_oracle(scene, kf):
-    spec = getattr(scenes, scene)(kf=kf)
+def _compare(spec, label, nparams):
     for method, opt in (("pd_eigen_alg_phd", dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000)),
                         ("pd_eigen_pcg", dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000))):
         prod, orc = product_body(spec), oracle_body(spec)
-        assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == (7 if scene == "mesh_stack" else 6)
+        assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == nparams
         qp, vp, ap = rollout(prod, spec, 3, opt, method=method)
         qo, vo, ao = rollout(orc, spec, 3, opt, method=method)
-        print(scene, method, kf, "q", rel_err(qp[-1], qo[-1]))
+        print(label, method, "q", rel_err(qp[-1], qo[-1]))
         assert rel_err(qp[-1], qo[-1]) < 1e-8
         rng = np.random.default_rng(5)
         dq, dv = rng.standard_normal(prod.dofs()), rng.standard_normal(prod.dofs())
         outp = backward_frame(prod, spec, qp, vp, ap, 2, dq, dv, opt, method=method)
         outo = backward_frame(orc, spec, qo, vo, ao, 2, dq, dv, opt, method=method)
         for name, k in (("dl_dq", 0), ("dl_dv", 1), ("dl_dstate_p", 6)):
-            print(scene, method, kf, name, rel_err(outp[k], outo[k]))
+            print(label, method, name, rel_err(outp[k], outo[k]))
             assert np.abs(outo[k]).max() > 0 and rel_err(outp[k], outo[k]) < 1e-8, name
+
+
+@pytest.mark.parametrize("scene", ["penalty_drop", "arc_drop", "mesh_stack"])
+@pytest.mark.parametrize("kf", [0.005, 1e3])
+def test_penalty_floor_matches_oracle(scene, kf):
+    _compare(getattr(scenes, scene)(kf=kf), "%s %g" % (scene, kf), 7 if scene == "mesh_stack" else 6)
+
+
+def test_billiard_balls_match_oracle():
+    """Eight one-cell balls in a row (BilliardBallStateForce): 16 parameters
+    (stiffness and friction per ball), no gravity."""
+    _compare(scenes.billiard(), "billiard", 16)
