@@ -103,6 +103,8 @@ int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int
             b.add_mesh_contact(std::vector<double>(params, params + np));
         } else if (t == "billiard_ball") {
             b.add_billiard_ball(std::vector<double>(params, params + np));
+        } else if (t == "hydrodynamics") {
+            b.add_hydrodynamics(std::vector<double>(params, params + np));
         } else {
             throw std::runtime_error(std::string("Unsupported state force type: ") + type);
         }

@@ -196,6 +196,7 @@ void Body::init_mesh(int edim, int nv, const double* X, int ne, const int* elems
     acontacts_.clear();
     mcontacts_.clear();
     bballs_.clear();
+    hydros_.clear();
     gravity_off_ = n_state_params_ = 0;
     muscles_.clear();
     act_dofs_ = 0;
@@ -366,8 +367,111 @@ void Body::add_billiard_ball(const std::vector<double>& p) {
     bballs_.push_back(b);
 }
 
+// HydrodynamicsStateForce::Initialize (hydrodynamics_state_force.cpp:11-42): params
+// [rho, v_water (3), Cd points (8), Ct points (8), max_force, faces]
+// (deformable_state_force.cpp:50-71); faces are quads (hex) or triangles (tet).
+void Body::add_hydrodynamics(const std::vector<double>& p) {
+    const int fd = edim_ == 8 ? 4 : 3;
+    check(p.size() >= 21 + (size_t)fd && (p.size() - 21) % fd == 0, "Inconsistent params for HydrodynamicsStateForce.");
+    HydroForce h;
+    h.rho = p[0];
+    std::copy(p.begin() + 1, p.begin() + 4, h.vw);
+    std::copy(p.begin() + 4, p.begin() + 20, h.P);
+    h.fmax = p[20];
+    h.fd = fd;
+    for (size_t i = 21; i < p.size(); ++i) {
+        h.faces.push_back((int)p[i]);
+        check(h.faces.back() >= 0 && h.faces.back() < nv_, "Hydrodynamics face index out of range.");
+    }
+    h.poff = n_state_params_;
+    n_state_params_ += 16;
+    hydros_.push_back(h);
+}
+
 namespace {
 inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1]; c[1] = a[2] * b[0] - a[0] * b[2]; c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Triangles of a hydrodynamic face, by node slot: a quad is 012, 023, 013, 312
+// (hydrodynamics_state_force.cpp:618-623), a triangle 012.
+constexpr int kHydroTris[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 1, 3}, {3, 1, 2}};
+
+// The state of one face (hydrodynamics_state_force.cpp:457-499, 614-673): triangle
+// vectors c_t = (b - a) x (c - b), area A = mean |c_t| (quad) or |c_0| (triangle),
+// un = sum c_t, n = un / |un|, relative water velocity w = v_water - mean node velocity,
+// and the angle of attack phi = acos(n . w/|w|) - pi/2. False when the face takes no force.
+template <class H>
+struct HydroFaceState {
+    const int* fc;
+    double ct[4][3], A = 0, un[3] = {0, 0, 0}, unn = 0, n[3] = {0, 0, 0}, w[3], wl = 0, d[3], nd = 0, phi = 0;
+    int nt = 1;
+    bool eval(const H& h, const double* q, const double* v, const int* f) {
+        fc = f;
+        nt = h.fd == 4 ? 4 : 1;
+        double sum = 0;
+        for (int t = 0; t < nt; ++t) {
+            const double *a = q + 3 * (size_t)f[kHydroTris[t][0]], *b = q + 3 * (size_t)f[kHydroTris[t][1]],
+                         *c = q + 3 * (size_t)f[kHydroTris[t][2]];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+            cross3(e1, e2, ct[t]);
+            for (int k = 0; k < 3; ++k) un[k] += ct[t][k];
+            sum += std::sqrt(dot3(ct[t], ct[t]));
+        }
+        A = h.fd == 4 ? sum / 4 : sum;
+        if (A <= DBL_EPSILON) return false;
+        unn = std::sqrt(dot3(un, un));
+        if (unn > DBL_EPSILON) for (int k = 0; k < 3; ++k) n[k] = un[k] / unn;
+        for (int k = 0; k < 3; ++k) {
+            double m = 0;
+            for (int j = 0; j < h.fd; ++j) m += v[3 * (size_t)f[j] + k];
+            w[k] = h.vw[k] - (h.fd == 4 ? m * 0.25 : m / 3);
+        }
+        wl = std::sqrt(dot3(w, w));
+        if (wl <= DBL_EPSILON) return false;
+        for (int k = 0; k < 3; ++k) d[k] = w[k] / wl;
+        nd = dot3(n, d);
+        if (nd >= 0.0) return false;
+        phi = std::acos(nd) - M_PI / 2;
+        return true;
+    }
+};
+
+// Cd (t = |phi| / (pi/2)) and Ct (t = phi / pi + 1/2): cubic Bezier curves through the
+// y values of the four points (hydrodynamics_state_force.cpp:101-162); value,
+// derivative in phi and the weights of the y values.
+inline void hydro_curve(const double* P, bool drag, double phi, double& val, double& dphi, double* wts) {
+    const double tn = phi / (M_PI / 2);
+    const double t = drag ? std::fabs(tn) : 0.5 * tn + 0.5;
+    const double dt = drag ? (tn < 0 ? -2 / M_PI : 2 / M_PI) : 1 / M_PI;
+    const double y0 = P[1], y1 = P[3], y2 = P[5], y3 = P[7], s = 1 - t;
+    wts[0] = s * s * s; wts[1] = 3 * s * s * t; wts[2] = 3 * s * t * t; wts[3] = t * t * t;
+    val = wts[0] * y0 + wts[1] * y1 + wts[2] * y2 + wts[3] * y3;
+    dphi = 3 * dt * (-s * s * y0 + (s * s - 2 * s * t) * y1 + (2 * s * t - t * t) * y2 + t * t * y3);
+}
+
+// The face force F = 1/2 rho A (Cd |w| w - Ct |w|^2 n), F / fd on each node, then every
+// entry clamped to [-max_force, max_force] (this force's own vector).
+template <class H>
+std::vector<double> hydro_forward(const H& h, int dofs, const double* q, const double* v) {
+    std::vector<double> f(dofs, 0.0);
+    const int nf = (int)h.faces.size() / h.fd;
+    for (int i = 0; i < nf; ++i) {
+        HydroFaceState<H> S;
+        if (!S.eval(h, q, v, &h.faces[(size_t)i * h.fd])) continue;
+        double cd, dcd, wcd[4], ct, dct, wct[4];
+        hydro_curve(h.P, true, S.phi, cd, dcd, wcd);
+        hydro_curve(h.P + 8, false, S.phi, ct, dct, wct);
+        for (int k = 0; k < 3; ++k) {
+            const double F = 0.5 * h.rho * S.A * cd * S.wl * S.w[k] - 0.5 * h.rho * S.A * ct * S.wl * S.wl * S.n[k];
+            const double fn = h.fd == 4 ? F * 0.25 : F / 3;
+            for (int j = 0; j < h.fd; ++j) f[3 * (size_t)S.fc[j] + k] += fn;
+        }
+    }
+    for (auto& x : f) x = std::min(std::max(x, -h.fmax), h.fmax);
+    return f;
+}
 
 // Per-ball means of x (centres of q, centre velocities of v).
 template <class B>
@@ -520,6 +624,10 @@ void Body::state_force(const double* q, const double* v, double* f) const {
     // mesh contacts: forward only; their BackwardForce returns zeros
     // (mesh_contact_state_force.cpp:217-223), so they add nothing to the adjoint
     for (const auto& m : mcontacts_) mesh_contact_add(m, dofs_, q, v, f);
+    for (const auto& h : hydros_) {
+        const std::vector<double> fh = hydro_forward(h, dofs_, q, v);
+        for (int i = 0; i < dofs_; ++i) f[i] += fh[i];
+    }
     // billiard balls (billiard_ball_state_force.cpp:20-86): ball i gets
     // k_i pen (-u + mu_i rot), ball j gets k_j pen (u - mu_j rot), spread over their vertices
     for (const auto& b : bballs_) {
@@ -697,6 +805,69 @@ void Body::state_force_backward(const double* q, const double* v, const double* 
                 dl_dp[b.poff + b.nb + i] += fi * dot3(gi, rot);
                 dl_dp[b.poff + b.nb + j] += -fj * dot3(gj, rot);
             }
+    }
+    // hydrodynamics (hydrodynamics_state_force.cpp:501-612, 675-829) as a scalar
+    // chain: with G = the clamp-masked dl/df summed over a face's nodes / fd,
+    // L = G . F = r A (Cd |w| (G.w) - Ct |w|^2 (G.n)), r = rho / 2; A, n and phi carry it
+    // to the nodes' positions through the triangle vectors, |w| and w to their velocities.
+    for (const auto& h : hydros_) {
+        const std::vector<double> fh = hydro_forward(h, dofs_, q, v);
+        const int nf = (int)h.faces.size() / h.fd;
+        const double inv_fd = h.fd == 4 ? 0.25 : 1.0 / 3.0, r = 0.5 * h.rho;
+        for (int i = 0; i < nf; ++i) {
+            HydroFaceState<HydroForce> S;
+            if (!S.eval(h, q, v, &h.faces[(size_t)i * h.fd])) continue;
+            double G[3] = {0, 0, 0};
+            for (int j = 0; j < h.fd; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    const size_t e = 3 * (size_t)S.fc[j] + k;
+                    if (fh[e] != h.fmax && fh[e] != -h.fmax) G[k] += dl_df[e];
+                }
+            for (int k = 0; k < 3; ++k) G[k] *= inv_fd;
+            double cd, dcd, wcd[4], ct, dct, wct[4];
+            hydro_curve(h.P, true, S.phi, cd, dcd, wcd);
+            hydro_curve(h.P + 8, false, S.phi, ct, dct, wct);
+            const double Gw = dot3(G, S.w), Gn = dot3(G, S.n), wl = S.wl, w2 = wl * wl;
+            const double L_A = r * (cd * wl * Gw - ct * w2 * Gn);
+            const double L_phi = r * S.A * (dcd * wl * Gw - dct * w2 * Gn);
+            const double dacos = -1 / std::sqrt(1 - S.nd * S.nd);
+            // dL/dn (direct, thrust) + through phi (dphi/dn = dacos d)
+            double Ln[3], Lun[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) Ln[k] = -r * S.A * ct * w2 * G[k] + L_phi * dacos * S.d[k];
+            if (S.unn > DBL_EPSILON) {
+                const double nl = dot3(S.n, Ln);
+                for (int k = 0; k < 3; ++k) Lun[k] = (Ln[k] - nl * S.n[k]) / S.unn;
+            }
+            // each triangle vector c_t = (b - a) x (c - b): (dc_t/da)^T x = x x (c - b),
+            // (dc_t/db)^T x = x x (a - c), (dc_t/dc)^T x = x x (b - a); x = dL/dun + dL/dA dA/dc_t
+            const double cA = h.fd == 4 ? 0.25 : 1.0;
+            for (int t = 0; t < S.nt; ++t) {
+                const int sl[3] = {kHydroTris[t][0], kHydroTris[t][1], kHydroTris[t][2]};
+                const double *a = q + 3 * (size_t)S.fc[sl[0]], *b = q + 3 * (size_t)S.fc[sl[1]], *c = q + 3 * (size_t)S.fc[sl[2]];
+                const double cn = std::sqrt(dot3(S.ct[t], S.ct[t]));
+                double x[3];
+                for (int k = 0; k < 3; ++k) x[k] = Lun[k] + (cn > DBL_EPSILON ? L_A * cA * S.ct[t][k] / cn : 0.0);
+                const double e[3][3] = {{c[0] - b[0], c[1] - b[1], c[2] - b[2]},
+                                        {a[0] - c[0], a[1] - c[1], a[2] - c[2]},
+                                        {b[0] - a[0], b[1] - a[1], b[2] - a[2]}};
+                for (int m = 0; m < 3; ++m) {
+                    double y[3];
+                    cross3(x, e[m], y);
+                    for (int k = 0; k < 3; ++k) dl_dq[3 * (size_t)S.fc[sl[m]] + k] += y[k];
+                }
+            }
+            // velocities: dw/dv_m = -I / fd; d|w| = d . dw; dphi/dw = dacos (n - (n.d) d) / |w|
+            double Lw[3];
+            for (int k = 0; k < 3; ++k)
+                Lw[k] = r * S.A * (cd * (wl * G[k] + Gw * S.d[k]) - ct * 2 * wl * Gn * S.d[k]) +
+                        L_phi * dacos * (S.n[k] - S.nd * S.d[k]) / wl;
+            for (int m = 0; m < h.fd; ++m)
+                for (int k = 0; k < 3; ++k) dl_dv[3 * (size_t)S.fc[m] + k] += -inv_fd * Lw[k];
+            for (int j = 0; j < 4; ++j) {
+                dl_dp[h.poff + 2 * j + 1] += r * S.A * wl * Gw * wcd[j];
+                dl_dp[h.poff + 8 + 2 * j + 1] += -r * S.A * w2 * Gn * wct[j];
+            }
+        }
     }
 }
 

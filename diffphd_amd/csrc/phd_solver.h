@@ -172,6 +172,7 @@ public:
     void add_arc_contact(const std::vector<double>& p);
     void add_mesh_contact(const std::vector<double>& p);
     void add_billiard_ball(const std::vector<double>& p);
+    void add_hydrodynamics(const std::vector<double>& p);
     int num_state_params() const { return n_state_params_; }
     // ForwardStateForce / BackwardStateForce at (q, v) on the host (3N vectors)
     void state_force(const double* q, const double* v, double* f) const;
@@ -356,6 +357,13 @@ private:
         int poff = 0;
     };
     std::vector<BilliardBallForce> bballs_;
+    struct HydroForce {  // water drag and thrust on surface faces (fd nodes each)
+        double rho = 0, vw[3] = {0, 0, 0}, P[16] = {0}, fmax = 0;
+        int fd = 4;
+        std::vector<int> faces;
+        int poff = 0;
+    };
+    std::vector<HydroForce> hydros_;
     int gravity_off_ = 0, n_state_params_ = 0;
     HostBoundary boundary_;
     std::map<int, int> fb_vertices_;

@@ -185,6 +185,76 @@ def billiard(balls=8, dx=0.01, radius=1.05, E=1e5, nu=0.45, rho=1000.0, seed=0, 
     }
 
 
+HYDRO_CD = [0.0, 0.05, 0.4, 0.05, 0.7, 1.85, 1.0, 2.05]
+HYDRO_CT = [-1.0, -0.8, -0.3, -0.5, 0.3, 0.1, 1.0, 2.5]
+
+
+def swim(kind="hex", v_water=(-0.6, 0.1, 0.05), max_force=1e3, faces=None, dx=0.01, E=1e5, nu=0.45, rho=1000.0,
+         rho_water=1000.0, seed=0):
+    """A free block in moving water (HydrodynamicsStateForce,
+    AddStateForce("hydrodynamics", [rho, v_water, Cd points, Ct points,
+    max_force, faces]), hydrodynamics_state_force.cpp): a 4x2x2-cell hex block
+    with its boundary quads, or a 2x2x2-cube tet block (5 tets per cube) with its
+    boundary triangles, every face ordered so that its normal points outward.
+    `faces` restricts the surface to the given face indices. The tet block's
+    vertices weigh a fifth of the hex block's (the reference's per-vertex mass is
+    density times the mean element volume), so a rollout of it wants a lighter
+    fluid (rho_water)."""
+    if kind == "hex":
+        V, Eidx, _, vf = hex_grid(np.ones((4, 2, 2)), dx, (0.0, 0.0, 0.0))
+        n = np.array(vf.shape) - 1
+        quads = []
+        for ax in range(3):
+            for side in (0, n[ax]):
+                o1, o2 = [a for a in range(3) if a != ax]
+                for u in range(n[o1]):
+                    for w in range(n[o2]):
+                        idx = []
+                        for du, dw in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                            c = [0, 0, 0]
+                            c[ax], c[o1], c[o2] = side, u + du, w + dw
+                            idx.append(int(vf[tuple(c)]))
+                        quads.append(idx)
+        surf = quads
+        edim = 8
+    else:
+        V, Eidx = tet_grid(2, dx, (0.0, 0.0, 0.0))
+        count = {}
+        for e in Eidx.T:
+            for drop in range(4):
+                f = tuple(int(x) for x in np.delete(e, drop))
+                key = tuple(sorted(f))
+                count.setdefault(key, []).append((f, int(e[drop])))
+        surf = [lst[0][0] + (lst[0][1],) for lst in count.values() if len(lst) == 1]
+        edim = 4
+    X = V.T
+    center = X.mean(axis=0)
+    oriented = []
+    for f in surf:
+        f = list(f[:4] if edim == 8 else f[:3])
+        P = X[f]
+        if edim == 8:
+            un = np.cross(P[2] - P[0], P[3] - P[1])
+        else:
+            un = np.cross(P[1] - P[0], P[2] - P[1])
+        if un @ (P.mean(axis=0) - center) < 0:
+            f = f[::-1]
+        oriented.append(f)
+    if faces is not None:
+        oriented = [oriented[i] for i in faces]
+    mu_l, la = lame(E, nu)
+    q0 = V.T.ravel().copy()
+    rng = np.random.default_rng(seed)
+    prm = [rho_water] + list(v_water) + HYDRO_CD + HYDRO_CT + [max_force] + [float(i) for f in oriented for i in f]
+    return {
+        "name": "swim_" + kind, "edim": edim, "V": V, "E": Eidx, "density": rho, "gravity": None,
+        "state_forces": [("hydrodynamics", prm)], "surface": oriented,
+        "energies": [("corotated", [2 * mu_l], []), ("volume", [la], [])], "dirichlet": [], "boundary": None,
+        "q0": q0, "v0": 0.02 * rng.standard_normal(q0.size), "dt": 1e-2, "frames": 10,
+        "loss_g": rng.standard_normal(q0.size), "options": default_options(max_pd_iter=5000),
+    }
+
+
 def napkin(cells=(10, 10, 1), contact_ratio=0.4, E=4e5, nu=0.45, seed=42):
     """Heterogeneous napkin dropped on a sphere (python/example/napkin_3d_test.py:32-45,
     python/py_diff_pd/env/napkin_env_3d.py:19-79): stiff half / 0.1x soft half

@@ -47,7 +47,10 @@ int phd_set_dirichlet(phd_ctx ctx, int dof, double val);
  * faces B] (two surfaces of the body, deformable_state_force.cpp:86-112,
  * mesh_contact_state_force.cpp; forward only, its backward is zero), or
  * "billiard_ball" [radius, vertices per ball, stiffness per ball, friction per
- * ball] (deformable_state_force.cpp:72-85, billiard_ball_state_force.cpp). */
+ * ball] (deformable_state_force.cpp:72-85, billiard_ball_state_force.cpp), or
+ * "hydrodynamics" [rho, v_water (3), Cd points (8), Ct points (8), max_force,
+ * faces (4 per quad on hex bodies, 3 per triangle on tet bodies)]
+ * (deformable_state_force.cpp:50-71, hydrodynamics_state_force.cpp). */
 int phd_add_state_force(phd_ctx ctx, const char* type, const double* params, int np);
 /* Deformable::NumOfStateForceParameters(): the length of dl_dstate_p (3 per
  * state force, in AddStateForce order). */

@@ -56,6 +56,7 @@ def lib():
         L.orc_add_arc_contact.argtypes = [P, dp, ctypes.c_int]
         L.orc_add_mesh_contact.argtypes = [P, dp, ctypes.c_int]
         L.orc_add_billiard_ball.argtypes = [P, dp, ctypes.c_int]
+        L.orc_add_hydrodynamics.argtypes = [P, dp, ctypes.c_int]
         L.orc_num_state_params.argtypes = [P]
         L.orc_state_force.argtypes = [P, dp, dp, dp]
         L.orc_state_force_backward.argtypes = [P, dp, dp, dp, dp, dp, dp]
@@ -178,6 +179,8 @@ class OracleDeformable:
             self._ok(lib().orc_add_mesh_contact(self._h, pp, len(params)))
         elif force_type == "billiard_ball":
             self._ok(lib().orc_add_billiard_ball(self._h, pp, len(params)))
+        elif force_type == "hydrodynamics":
+            self._ok(lib().orc_add_hydrodynamics(self._h, pp, len(params)))
         else:
             raise RuntimeError("Unsupported state force: " + str(force_type))
 

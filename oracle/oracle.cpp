@@ -936,6 +936,18 @@ struct Body {
         int poff = 0;
     };
     std::vector<BilliardBall> bballs;
+    // HydrodynamicsStateForce (hydrodynamics_state_force.cpp:11-42): drag and
+    // thrust of water moving at v_water on surface faces (quads of a hex mesh,
+    // triangles of a tet mesh), with cubic Bezier coefficient curves Cd(phi), Ct(phi)
+    // of the angle of attack and a per-entry clamp at max_force; parameters (the
+    // Cd points (4 x 2), the Ct points (4 x 2))
+    struct Hydro {
+        double rho = 0, vw[3] = {0, 0, 0}, P[16] = {0}, fmax = 0;
+        int fd = 4;
+        std::vector<int> faces;  // fd per face
+        int poff = 0;
+    };
+    std::vector<Hydro> hydros;
     int gravity_off = 0, n_state_params = 0;
     Boundary boundary;
     std::map<int, int> fb_vertices;  // frictional_boundary_vertex_indices_ (sorted)
@@ -1420,6 +1432,209 @@ struct Body {
                 dl_dp[b.poff + b.nb + j] += -fj * gjr;
             }
     }
+    // AddStateForce("hydrodynamics", [rho, v_water (3), Cd points (8), Ct points (8),
+    // max_force, faces (fd each)]) (deformable_state_force.cpp:50-71); fd = 4 (hex) or 3 (tet).
+    void add_hydrodynamics(const Vec& prm) {
+        const int fd = edim == 8 ? 4 : 3;
+        check(prm.size() >= 21 + (size_t)fd && (prm.size() - 21) % fd == 0, "Inconsistent params for HydrodynamicsStateForce.");
+        Hydro h;
+        h.rho = prm[0];
+        for (int i = 0; i < 3; ++i) h.vw[i] = prm[1 + i];
+        for (int i = 0; i < 16; ++i) h.P[i] = prm[4 + i];
+        h.fmax = prm[20];
+        h.fd = fd;
+        for (size_t i = 21; i < prm.size(); ++i) {
+            h.faces.push_back((int)prm[i]);
+            check(h.faces.back() >= 0 && h.faces.back() < nv, "Hydrodynamics face index out of range.");
+        }
+        h.poff = n_state_params;
+        n_state_params += 16;
+        hydros.push_back(h);
+    }
+    // Cd(phi) and Ct(phi) (hydrodynamics_state_force.cpp:101-162): cubic Bezier
+    // curves in t = |phi| / (pi / 2) (Cd) and t = phi / pi + 1/2 (Ct) through the y values
+    // of the points; their phi-derivatives and weights on the four y values.
+    static void bezier(double t, const double* y, double& val, double& dval_dt, double w[4]) {
+        const double s = 1 - t;
+        w[0] = s * s * s; w[1] = 3 * s * s * t; w[2] = 3 * s * t * t; w[3] = t * t * t;
+        val = w[0] * y[0] + w[1] * y[1] + w[2] * y[2] + w[3] * y[3];
+        dval_dt = 3 * (-s * s * y[0] + (s * s - 2 * s * t) * y[1] + (2 * s * t - t * t) * y[2] + t * t * y[3]);
+    }
+    void hydro_coeffs(const Hydro& h, double phi, double& cd, double& dcd, double wcd[4], double& ct, double& dct,
+                      double wct[4]) const {
+        const double pi = 3.14159265358979323846;
+        const double ycd[4] = {h.P[1], h.P[3], h.P[5], h.P[7]}, yct[4] = {h.P[9], h.P[11], h.P[13], h.P[15]};
+        const double tn = phi / (pi / 2);
+        bezier(tn < 0 ? -tn : tn, ycd, cd, dcd, wcd);
+        dcd *= tn < 0 ? -2 / pi : 2 / pi;
+        bezier(tn * 0.5 + 0.5, yct, ct, dct, wct);
+        dct *= 1 / pi;
+    }
+    // One face: its area A, normal n (zero when degenerate), the summed triangle
+    // vectors un, and the triangles (vertex slots) that make it up: a quad is the
+    // four triangles 012, 023, 013, 312 with A the mean of their |cross| (:625-640),
+    // a triangle is 012 with A = |cross| (:466-470).
+    struct HydroFace { double A, n[3], un[3], unn; double Aj[4][3]; int nt; };
+    HydroFace hydro_face(const Hydro& h, const Vec& q, const int* f) const {
+        static const int quad[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 1, 3}, {3, 1, 2}};
+        HydroFace F{};
+        F.nt = h.fd == 4 ? 4 : 1;
+        double sumA = 0;
+        for (int t = 0; t < F.nt; ++t) {
+            const double* a = &q[3 * f[quad[t][0]]];
+            const double* b = &q[3 * f[quad[t][1]]];
+            const double* c = &q[3 * f[quad[t][2]]];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+            F.Aj[t][0] = e1[1] * e2[2] - e1[2] * e2[1];
+            F.Aj[t][1] = e1[2] * e2[0] - e1[0] * e2[2];
+            F.Aj[t][2] = e1[0] * e2[1] - e1[1] * e2[0];
+            for (int k = 0; k < 3; ++k) F.un[k] += F.Aj[t][k];
+            sumA += std::sqrt(F.Aj[t][0] * F.Aj[t][0] + F.Aj[t][1] * F.Aj[t][1] + F.Aj[t][2] * F.Aj[t][2]);
+        }
+        F.A = h.fd == 4 ? sumA / 4 : sumA;
+        F.unn = std::sqrt(F.un[0] * F.un[0] + F.un[1] * F.un[1] + F.un[2] * F.un[2]);
+        const double eps = std::numeric_limits<double>::epsilon();
+        for (int k = 0; k < 3; ++k) F.n[k] = F.unn > eps ? F.un[k] / F.unn : 0.0;
+        return F;
+    }
+    // ForwardForce (:457-499 tet, :614-673 hex): per face with A > eps, relative
+    // water velocity w = v_water - mean node velocity (|w| > eps) and n . w/|w| < 0:
+    // phi = acos(n . d) - pi/2, F = 1/2 rho A (Cd |w| w - Ct |w|^2 n), F / fd on each
+    // node; then every entry clamped to [-max_force, max_force].
+    Vec hydro_force(const Hydro& h, const Vec& q, const Vec& v) const {
+        Vec f(dofs, 0.0);
+        const double eps = std::numeric_limits<double>::epsilon(), pi = 3.14159265358979323846;
+        const int nf = (int)h.faces.size() / h.fd;
+        for (int i = 0; i < nf; ++i) {
+            const int* fc = &h.faces[(size_t)i * h.fd];
+            const HydroFace F = hydro_face(h, q, fc);
+            if (F.A <= eps) continue;
+            double w[3];
+            for (int k = 0; k < 3; ++k) {
+                double m = 0;
+                for (int j = 0; j < h.fd; ++j) m += v[3 * fc[j] + k];
+                w[k] = h.vw[k] - (h.fd == 4 ? m * 0.25 : m / 3);
+            }
+            const double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            if (wl <= eps) continue;
+            const double nd = (F.n[0] * w[0] + F.n[1] * w[1] + F.n[2] * w[2]) / wl;
+            if (nd >= 0.0) continue;
+            const double phi = std::acos(nd) - pi / 2;
+            double cd, dcd, wcd[4], ct, dct, wct[4];
+            hydro_coeffs(h, phi, cd, dcd, wcd, ct, dct, wct);
+            for (int k = 0; k < 3; ++k) {
+                const double Fk = 0.5 * h.rho * F.A * cd * wl * w[k] - 0.5 * h.rho * F.A * ct * wl * wl * F.n[k];
+                const double fn = h.fd == 4 ? Fk * 0.25 : Fk / 3;
+                for (int j = 0; j < h.fd; ++j) f[3 * fc[j] + k] += fn;
+            }
+        }
+        for (auto& x : f) x = std::min(std::max(x, -h.fmax), h.fmax);
+        return f;
+    }
+    // BackwardForce (:501-612 tet, :675-829 hex), with the derivatives written out
+    // as the reference forms them: dA/dp from the triangles' unit cross products,
+    // dn/dp = (I - n n^T)/|un| dun/dp, dphi = -1/sqrt(1 - (n.d)^2) d(n.d), the Bezier
+    // weights for the parameters; entries of dl/df where the clamp is active are dropped.
+    void hydro_backward(const Hydro& h, const Vec& q, const Vec& v, const Vec& dl_df, Vec& dl_dq, Vec& dl_dv, Vec& dl_dp) const {
+        static const int quad[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 1, 3}, {3, 1, 2}};
+        const double eps = std::numeric_limits<double>::epsilon(), pi = 3.14159265358979323846;
+        const Vec f = hydro_force(h, q, v);
+        Vec g = dl_df;
+        for (int i = 0; i < dofs; ++i) if (f[i] == h.fmax || f[i] == -h.fmax) g[i] = 0;
+        const int nf = (int)h.faces.size() / h.fd;
+        const double inv_fd = h.fd == 4 ? 0.25 : 1.0 / 3.0;
+        for (int i = 0; i < nf; ++i) {
+            const int* fc = &h.faces[(size_t)i * h.fd];
+            const HydroFace F = hydro_face(h, q, fc);
+            if (F.A <= eps) continue;
+            double w[3];
+            for (int k = 0; k < 3; ++k) {
+                double m = 0;
+                for (int j = 0; j < h.fd; ++j) m += v[3 * fc[j] + k];
+                w[k] = h.vw[k] - (h.fd == 4 ? m * 0.25 : m / 3);
+            }
+            const double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            if (wl <= eps) continue;
+            double d[3];
+            for (int k = 0; k < 3; ++k) d[k] = w[k] / wl;
+            const double nd = F.n[0] * d[0] + F.n[1] * d[1] + F.n[2] * d[2];
+            if (nd >= 0.0) continue;
+            const double phi = std::acos(nd) - pi / 2, dacos = -1 / std::sqrt(1 - nd * nd);
+            double cd, dcd, wcd[4], ct, dct, wct[4];
+            hydro_coeffs(h, phi, cd, dcd, wcd, ct, dct, wct);
+            // dA/dp_m and dun/dp_m (3x3) per face node m
+            double dA[4][3] = {}, dun[4][3][3] = {};
+            const double cA = h.fd == 4 ? 0.25 : 1.0;
+            for (int t = 0; t < F.nt; ++t) {
+                const int s0 = quad[t][0], s1 = quad[t][1], s2 = quad[t][2];
+                const double* a = &q[3 * fc[s0]];
+                const double* b = &q[3 * fc[s1]];
+                const double* c = &q[3 * fc[s2]];
+                // cross = (b - a) x (c - b): d/da = [c - b]x, d/db = [a - c]x, d/dc = [b - a]x
+                double e[3][3];
+                for (int k = 0; k < 3; ++k) { e[0][k] = c[k] - b[k]; e[1][k] = a[k] - c[k]; e[2][k] = b[k] - a[k]; }
+                const double an = std::sqrt(F.Aj[t][0] * F.Aj[t][0] + F.Aj[t][1] * F.Aj[t][1] + F.Aj[t][2] * F.Aj[t][2]);
+                double u[3] = {0, 0, 0};
+                if (an > eps) for (int k = 0; k < 3; ++k) u[k] = F.Aj[t][k] / an;
+                const int slot[3] = {s0, s1, s2};
+                for (int r = 0; r < 3; ++r) {
+                    double S[3][3] = {{0, -e[r][2], e[r][1]}, {e[r][2], 0, -e[r][0]}, {-e[r][1], e[r][0], 0}};
+                    for (int x = 0; x < 3; ++x) {
+                        for (int y = 0; y < 3; ++y) dun[slot[r]][x][y] += S[x][y];
+                        // dA/dp = cA * S^T u
+                        dA[slot[r]][x] += cA * (S[0][x] * u[0] + S[1][x] * u[1] + S[2][x] * u[2]);
+                    }
+                }
+            }
+            double dn[4][3][3] = {};
+            if (F.unn > eps)
+                for (int m = 0; m < h.fd; ++m)
+                    for (int x = 0; x < 3; ++x)
+                        for (int y = 0; y < 3; ++y) {
+                            double t = 0;
+                            for (int z = 0; z < 3; ++z) t += (((x == z) - F.n[x] * F.n[z]) / F.unn) * dun[m][z][y];
+                            dn[m][x][y] = t;
+                        }
+            // G = the masked dl/df summed over the face's nodes (each node gets F / fd)
+            double G[3] = {0, 0, 0};
+            for (int j = 0; j < h.fd; ++j) for (int k = 0; k < 3; ++k) G[k] += g[3 * fc[j] + k];
+            const double r = 0.5 * h.rho;
+            for (int m = 0; m < h.fd; ++m) {
+                double dphi[3];
+                for (int y = 0; y < 3; ++y) dphi[y] = dacos * (d[0] * dn[m][0][y] + d[1] * dn[m][1][y] + d[2] * dn[m][2][y]);
+                for (int y = 0; y < 3; ++y) {
+                    double col = 0;  // G . dF/dp_m[:, y]
+                    for (int x = 0; x < 3; ++x) {
+                        const double ddrag = r * wl * w[x] * (dA[m][y] * cd + F.A * dcd * dphi[y]);
+                        const double dthr = -r * wl * wl * (dn[m][x][y] * F.A * ct + F.n[x] * dA[m][y] * ct + F.n[x] * F.A * dct * dphi[y]);
+                        col += G[x] * (ddrag + dthr);
+                    }
+                    dl_dq[3 * fc[m] + y] += col * inv_fd;
+                }
+            }
+            // velocities: w = v_water - mean, d|w|/dv_m = -d / fd, dd/dv_m = -(I - d d^T)/(fd |w|)
+            double dphi_v[3];
+            for (int y = 0; y < 3; ++y) {
+                double t = 0;
+                for (int x = 0; x < 3; ++x) t += F.n[x] * (-((x == y) - d[x] * d[y]) * inv_fd / wl);
+                dphi_v[y] = dacos * t;
+            }
+            for (int y = 0; y < 3; ++y) {
+                double col = 0;
+                for (int x = 0; x < 3; ++x) {
+                    const double ddrag = r * F.A * (-(x == y) * cd * wl * inv_fd + w[x] * dcd * dphi_v[y] * wl - w[x] * cd * d[y] * inv_fd);
+                    const double dthr = -r * F.A * F.n[x] * (dct * dphi_v[y] * wl * wl - 2 * ct * w[y] * inv_fd);
+                    col += G[x] * (ddrag + dthr);
+                }
+                for (int m = 0; m < h.fd; ++m) dl_dv[3 * fc[m] + y] += col * inv_fd;
+            }
+            const double Gw = G[0] * w[0] + G[1] * w[1] + G[2] * w[2], Gn = G[0] * F.n[0] + G[1] * F.n[1] + G[2] * F.n[2];
+            for (int j = 0; j < 4; ++j) {
+                dl_dp[h.poff + 2 * j + 1] += r * F.A * wl * Gw * wcd[j] * inv_fd;
+                dl_dp[h.poff + 8 + 2 * j + 1] += -r * F.A * wl * wl * Gn * wct[j] * inv_fd;
+            }
+        }
+    }
     // ForwardStateForce (deformable_state_force.cpp:114-125): gravity f_i = mass g
     // (gravitational_state_force.cpp:14-22) plus each planar contact's penalty
     // kn (-min(0, d))^(p-1) n and friction -min(kf, mu f_n / (|u_s| + eps)) u_s,
@@ -1468,6 +1683,10 @@ struct Body {
             Vec fb(dofs, 0.0);
             billiard_force(b, q, v, fb);
             for (int i = 0; i < dofs; ++i) f[i] += fb[i];
+        }
+        for (const auto& h : hydros) {
+            const Vec fh = hydro_force(h, q, v);
+            for (int i = 0; i < dofs; ++i) f[i] += fh[i];
         }
         return f;
     }
@@ -1589,6 +1808,7 @@ struct Body {
                 }
             }
         for (const auto& b : bballs) billiard_backward(b, q, v, dl_df, dl_dq, dl_dp);
+        for (const auto& h : hydros) hydro_backward(h, q, v, dl_df, dl_dq, dl_dv, dl_dp);
     }
 
     // SetupProjectiveDynamicsSolver, deformable_projective_dynamics_forward.cpp:8-183
@@ -3469,6 +3689,10 @@ int orc_backward_baseline(void* hp, const double* q, const double* v, const doub
 int orc_add_planar_contact(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);
     return guard(h, [&] { h->body.add_planar_contact(orc::Vec(p, p + np)); });
+}
+int orc_add_hydrodynamics(void* hp, const double* p, int np) {
+    Handle* h = static_cast<Handle*>(hp);
+    return guard(h, [&] { h->body.add_hydrodynamics(orc::Vec(p, p + np)); });
 }
 int orc_add_billiard_ball(void* hp, const double* p, int np) {
     Handle* h = static_cast<Handle*>(hp);

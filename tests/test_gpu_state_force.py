@@ -1,7 +1,8 @@
 """The contact state forces (AddStateForce("planar_contact"), a penalty floor
 with friction; AddStateForce("arc_contact"), a cylindrical trough;
-AddStateForce("mesh_contact"), two surfaces of the body meeting; and
-AddStateForce("billiard_ball"), vertex blocks as balls) in the HIP body
+AddStateForce("mesh_contact"), two surfaces of the body meeting;
+AddStateForce("billiard_ball"), vertex blocks as balls; and
+AddStateForce("hydrodynamics"), drag and thrust of moving water) in the HIP body
 against the oracle: rollouts and adjoints in both friction branches, through
 the alg_phd and the baseline methods (the mesh contact's backward is zero, as
 in the reference, so its adjoint runs through the other terms)."""
@@ -15,9 +16,11 @@ from test_oracle_baseline import BASELINE_OPT
 pytestmark = pytest.mark.gpu
 
 
-def _compare(spec, label, nparams):
-    for method, opt in (("pd_eigen_alg_phd", dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000)),
-                        ("pd_eigen_pcg", dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000))):
+def _compare(spec, label, nparams, baseline=True):
+    runs = [("pd_eigen_alg_phd", dict(spec["options"], rel_tol=1e-12, abs_tol=1e-14, max_pd_iter=20000))]
+    if baseline:
+        runs.append(("pd_eigen_pcg", dict(BASELINE_OPT, rel_tol=1e-12, abs_tol=1e-13, max_pd_iter=20000)))
+    for method, opt in runs:
         prod, orc = product_body(spec), oracle_body(spec)
         assert prod.NumOfStateForceParameters() == orc.NumOfStateForceParameters() == nparams
         qp, vp, ap = rollout(prod, spec, 3, opt, method=method)
@@ -43,3 +46,21 @@ def test_billiard_balls_match_oracle():
     """Eight one-cell balls in a row (BilliardBallStateForce): 16 parameters
     (stiffness and friction per ball), no gravity."""
     _compare(scenes.billiard(), "billiard", 16)
+
+
+@pytest.mark.parametrize("kind,rho_water,max_force", [("hex", 1000.0, 1e3), ("hex", 1000.0, 4e-3),
+                                                      ("tet", 10.0, 1e3), ("tet", 10.0, 4e-5)])
+def test_hydrodynamics_match_oracle(kind, rho_water, max_force):
+    """A block in moving water (HydrodynamicsStateForce) on hex quads and tet
+    triangles, with and without clamped entries: 16 parameters (Cd, Ct points).
+    The tet block carries 5x lighter vertices and more faces per vertex, so its
+    water is 100x lighter: with rho_water = 1000 its PD iteration does not
+    converge in either body within 20,000 iterations, and the two unconverged
+    states differ. On the tet block the baseline's backward L-BFGS meets an
+    indefinite operator (where the reference falls back to Newton, which is not
+    built), so the tets run the alg_phd method only."""
+    spec = scenes.swim(kind, rho_water=rho_water, max_force=max_force)
+    orc = oracle_body(spec)
+    f = orc.state_force(spec["q0"], spec["v0"])
+    assert (np.abs(np.abs(f) - max_force) == 0).any() == (max_force < 1)  # the clamp acts where intended
+    _compare(spec, "swim %s %g" % (kind, max_force), 16, baseline=kind == "hex")
