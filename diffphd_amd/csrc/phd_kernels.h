@@ -112,6 +112,10 @@ void launch_gather_vertex(hipStream_t s, int nv, const int* inc_ptr, const int* 
 void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, double* out);
 int dots_partial_size();
 // out = x + alpha * (y - z) ... generic axpby: out = a*x + b*y (+ c*z)
+// Anderson history push: dQ = x - x_last, dG = g - g_last (skipped when dQ is null),
+// then x_last = x, g_last = g.
+void launch_aa_push(hipStream_t s, int n, const double* x, const double* g, double* x_last, double* g_last, double* dQ,
+                    double* dG);
 void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out);
 // out = q_prev + g - sum_i gamma_i (dQ_i + dG_i)

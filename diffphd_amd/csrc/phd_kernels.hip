@@ -682,6 +682,21 @@ __global__ void k_axpbypcz(int n, double a, const double* __restrict__ x, double
     out[i] = r;
 }
 
+// Anderson history push: with a previous iterate, dQ = x - x_last and dG = g - g_last;
+// then x_last = x and g_last = g (each thread reads its entries before it writes them).
+__global__ void k_aa_push(int n, const double* __restrict__ x, const double* __restrict__ g, double* __restrict__ x_last,
+                          double* __restrict__ g_last, double* __restrict__ dQ, double* __restrict__ dG) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i], gi = g[i];
+    if (dQ) {
+        dQ[i] = xi - x_last[i];
+        dG[i] = gi - g_last[i];
+    }
+    x_last[i] = xi;
+    g_last[i] = gi;
+}
+
 struct AAPtrs {
     const double* dQ[8];
     const double* dG[8];
@@ -1685,6 +1700,11 @@ void launch_dots(hipStream_t s, int n, const DotArgs& args, double* partial, dou
 void launch_axpbypcz(hipStream_t s, int n, double a, const double* x, double b, const double* y, double c,
                      const double* z, double* out) {
     hipLaunchKernelGGL(k_axpbypcz, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x, b, y, c, z, out);
+}
+void launch_aa_push(hipStream_t s, int n, const double* x, const double* g, double* x_last, double* g_last, double* dQ,
+                    double* dG) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_aa_push, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, g, x_last, g_last, dQ, dG);
 }
 void launch_aa_combine(hipStream_t s, int n, const double* q_prev, const double* g, int m, const double* const* dQ,
                        const double* const* dG, const double* gamma, double* out) {

@@ -1923,15 +1923,16 @@ bool Body::anderson_step(DevAnderson& aa, const double* xprev, const double* g, 
                          const std::vector<std::pair<const double*, const double*>>& extra,
                          std::vector<double>* extra_out) {
     const int n3 = aa.n;
+    // the newest difference pair and the new last iterate in one pass
+    double *dq = nullptr, *dg = nullptr;
     if (aa.have_last) {
         const int pi = aa.free_slots.back();
         aa.free_slots.pop_back();
-        launch_axpbypcz(stream_, n3, 1.0, xprev, -1.0, aa.Q[7].p, 0.0, nullptr, aa.Q[pi].p);
-        launch_axpbypcz(stream_, n3, 1.0, g, -1.0, aa.G[7].p, 0.0, nullptr, aa.G[pi].p);
+        dq = aa.Q[pi].p;
+        dg = aa.G[pi].p;
         aa.diffs.push_back(pi);
     }
-    hip_check(hipMemcpyAsync(aa.Q[7].p, xprev, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa x");
-    hip_check(hipMemcpyAsync(aa.G[7].p, g, sizeof(double) * n3, hipMemcpyDeviceToDevice, stream_), "aa g");
+    launch_aa_push(stream_, n3, xprev, g, aa.Q[7].p, aa.G[7].p, dq, dg);
     aa.have_last = true;
     const int m = (int)aa.diffs.size();
     std::vector<std::pair<const double*, const double*>> pr;
